@@ -1,0 +1,375 @@
+"""Python host side of the MI355X fractional step: ctypes bindings of the
+C-ABI in include/amressif.h (libamressif_hip.so), with torch used only for
+device memory and the HIP stream.
+
+The functions mirror the reference's module functions (Source/src1_rever)
+one for one -- same names, argument meaning and in-place semantics -- on
+torch device tensors laid out like the reference's single-box MultiFabs:
+
+  x-face  (velCont[0], ...)  : float64 (n, n+1)
+  y-face  (velCont[1], ...)  : float64 (n+1, n)
+  userCtx (p, phi), 1 ghost   : float64 (2, n+2, n+2)
+  velCart, 2 ghosts           : float64 (2, n+4, n+4)
+  Poisson rhs/sol, 1 ghost    : float64 (n+2, n+2)
+
+There is no CPU fallback: importing this module raises if the HIP library is
+missing, and every entry point raises on a non-zero status.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libamressif_hip.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"{LIB_PATH} is missing: build it with `python -m amressif_amd.build`")
+
+_lib = ctypes.CDLL(LIB_PATH)
+
+AMRESSIF_FP64 = 0
+AMRESSIF_MIXED = 1
+FLAG_CAP, FLAG_DIVERGED, FLAG_NONFINITE, FLAG_MG_CAP = 1, 2, 4, 8
+ERRORS = {-1: "ERR_ARG", -2: "ERR_BC", -3: "ERR_UNSUPPORTED", -4: "ERR_HIP", -5: "ERR_NONFINITE", -6: "ERR_PARSE"}
+
+
+class Inputs(ctypes.Structure):
+    _fields_ = [
+        ("n_cell", ctypes.c_int),
+        ("max_grid_size", ctypes.c_int * 3),
+        ("lo_phy_dim", ctypes.c_double * 3),
+        ("hi_phy_dim", ctypes.c_double * 3),
+        ("IterNum", ctypes.c_int),
+        ("nsteps", ctypes.c_int),
+        ("cfl", ctypes.c_double),
+        ("fixed_dt", ctypes.c_double),
+        ("ren", ctypes.c_double),
+        ("vis", ctypes.c_double),
+        ("phy_bc_lo", ctypes.c_int * 3),
+        ("phy_bc_hi", ctypes.c_int * 3),
+        ("inflow_waveform", ctypes.c_double * 3),
+        ("target_resolution", ctypes.c_int),
+        ("momentum_tolerance", ctypes.c_double),
+        ("PSEUDO_TIMESTEPPING", ctypes.c_int),
+        ("plot_int", ctypes.c_int),
+        ("chk_int", ctypes.c_int),
+        ("txt_int", ctypes.c_int),
+        ("chk_out", ctypes.c_int),
+        ("poisson_tol_rel", ctypes.c_double),
+        ("poisson_tol_abs", ctypes.c_double),
+        ("mg_max_iter", ctypes.c_int),
+        ("mg_bottom_max", ctypes.c_int),
+        ("precision", ctypes.c_int),
+    ]
+
+
+class Array4(ctypes.Structure):
+    _fields_ = [
+        ("ptr", ctypes.c_void_p),
+        ("lo", ctypes.c_int * 3),
+        ("hi", ctypes.c_int * 3),
+        ("ncomp", ctypes.c_int),
+        ("dtype", ctypes.c_int),
+    ]
+
+
+class StepStats(ctypes.Structure):
+    _fields_ = [
+        ("pseudo_iters", ctypes.c_int),
+        ("vcycles", ctypes.c_int),
+        ("flags", ctypes.c_int),
+        ("pad", ctypes.c_int),
+        ("norm_error", ctypes.c_double),
+        ("div_star", ctypes.c_double),
+        ("rhs_sum", ctypes.c_double),
+        ("poisson_res", ctypes.c_double),
+        ("div_new", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "pad"}
+
+
+_P = ctypes.POINTER
+_A4 = _P(Array4)
+_c = ctypes
+_sig = {
+    "amressif_inputs_default": (None, [_P(Inputs)]),
+    "amressif_parse_inputs": (_c.c_int, [_c.c_char_p, _c.c_int, _P(_c.c_char_p), _P(Inputs), _c.c_char_p, _c.c_int]),
+    "amressif_dt": (_c.c_double, [_P(Inputs)]),
+    "amressif_create": (_c.c_int, [_P(Inputs), _c.c_void_p, _P(_c.c_void_p)]),
+    "amressif_destroy": (_c.c_int, [_c.c_void_p]),
+    "amressif_last_error": (_c.c_char_p, [_c.c_void_p]),
+    "amressif_workspace_bytes": (_c.c_int64, [_c.c_void_p]),
+    "amressif_set_lid_profile": (_c.c_int, [_c.c_void_p, _c.c_void_p]),
+    "amressif_load_state": (_c.c_int, [_c.c_void_p, _A4, _A4, _A4]),
+    "amressif_store_state": (_c.c_int, [_c.c_void_p, _A4, _A4, _A4, _A4]),
+    "amressif_advance": (_c.c_int, [_c.c_void_p, _c.c_int, _P(StepStats)]),
+    "amressif_cont2cart": (_c.c_int, [_c.c_void_p, _A4, _A4]),
+    "amressif_momentum_rhs": (_c.c_int, [_c.c_void_p, _A4, _A4, _A4]),
+    "amressif_rk_stage": (_c.c_int, [_c.c_void_p, _c.c_int, _A4, _A4, _A4, _A4, _A4]),
+    "amressif_pseudo_norm": (_c.c_int, [_c.c_void_p, _A4, _A4, _P(_c.c_double)]),
+    "amressif_poisson_rhs": (_c.c_int, [_c.c_void_p, _A4, _A4, _P(_c.c_double), _P(_c.c_double)]),
+    "amressif_poisson_solve": (_c.c_int, [_c.c_void_p, _A4, _A4, _c.c_double, _c.c_double, _P(_c.c_int),
+                                          _P(_c.c_double)]),
+    "amressif_project": (_c.c_int, [_c.c_void_p, _A4, _A4, _A4, _P(_c.c_double)]),
+}
+for _name, (_res, _args) in _sig.items():
+    _f = getattr(_lib, _name)
+    _f.restype = _res
+    _f.argtypes = _args
+
+
+class AmressifError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def default_inputs(**kw) -> Inputs:
+    inp = Inputs()
+    _lib.amressif_inputs_default(ctypes.byref(inp))
+    for k, v in kw.items():
+        cur = getattr(inp, k)
+        if isinstance(cur, ctypes.Array):
+            for i, x in enumerate(v):
+                cur[i] = x
+        else:
+            setattr(inp, k, v)
+    return inp
+
+
+def cavity_inputs(n: int, ren: float, precision: int = AMRESSIF_FP64, cfl: float = 0.5, **kw) -> Inputs:
+    """The workloads' inputs: unit square, phy_bc_lo = 135 135, phy_bc_hi =
+    135 165, lid inflow 1 0, IterNum 100, momentum_tolerance 1e-10, one box."""
+    args = dict(n_cell=n, max_grid_size=(n, n, n), IterNum=100, cfl=cfl, ren=ren, vis=1.0 / ren,
+                phy_bc_lo=(135, 135, 135), phy_bc_hi=(135, 165, 135), inflow_waveform=(1.0, 0.0, 0.0),
+                momentum_tolerance=1e-10, precision=precision)
+    args.update(kw)
+    return default_inputs(**args)
+
+
+def parse_inputs(path: str | None, overrides=()) -> Inputs:
+    inp = Inputs()
+    arr = (ctypes.c_char_p * max(1, len(overrides)))(*[o.encode() for o in overrides])
+    err = ctypes.create_string_buffer(512)
+    rc = _lib.amressif_parse_inputs(path.encode() if path else None, len(overrides), arr, ctypes.byref(inp), err, 512)
+    if rc:
+        raise AmressifError(rc, err.value.decode())
+    return inp
+
+
+def dt_of(inp: Inputs) -> float:
+    return _lib.amressif_dt(ctypes.byref(inp))
+
+
+# ------------------------------------------------------------ Array4 views
+def _a4(t: torch.Tensor, lo, hi, ncomp) -> Array4:
+    if t.dtype != torch.float64 or not t.is_contiguous() or t.device.type != "cuda":
+        raise ValueError("expected a contiguous float64 cuda tensor")
+    a = Array4()
+    a.ptr = t.data_ptr()
+    a.lo[0], a.lo[1], a.lo[2] = lo[0], lo[1], 0
+    a.hi[0], a.hi[1], a.hi[2] = hi[0], hi[1], 0
+    a.ncomp = ncomp
+    a.dtype = 0
+    return a
+
+
+def xface_view(t, n):
+    assert tuple(t.shape) == (n, n + 1), t.shape
+    return _a4(t, (0, 0), (n, n - 1), 1)
+
+
+def yface_view(t, n):
+    assert tuple(t.shape) == (n + 1, n), t.shape
+    return _a4(t, (0, 0), (n - 1, n), 1)
+
+
+def faces_view(u, v, n):
+    arr = (Array4 * 2)()
+    arr[0] = xface_view(u, n)
+    arr[1] = yface_view(v, n)
+    return arr
+
+
+def cell_view(t, n, ghosts):
+    ncomp = 1 if t.dim() == 2 else t.shape[0]
+    assert tuple(t.shape[-2:]) == (n + 2 * ghosts, n + 2 * ghosts), t.shape
+    return _a4(t, (-ghosts, -ghosts), (n - 1 + ghosts, n - 1 + ghosts), ncomp)
+
+
+# ----------------------------------------------------------------- context
+class Context:
+    """One amressif_ctx on the current torch HIP stream."""
+
+    def __init__(self, inp: Inputs, stream: torch.cuda.Stream | None = None):
+        if not torch.cuda.is_available():
+            raise RuntimeError("amressif_amd needs a GPU: no HIP device is visible")
+        self.inp = inp
+        self.n = inp.n_cell
+        self.stream = stream or torch.cuda.current_stream()
+        h = ctypes.c_void_p()
+        rc = _lib.amressif_create(ctypes.byref(inp), ctypes.c_void_p(self.stream.cuda_stream), ctypes.byref(h))
+        self._h = h
+        if rc:
+            msg = _lib.amressif_last_error(h).decode() if h.value else ""
+            _lib.amressif_destroy(h)
+            self._h = ctypes.c_void_p()
+            raise AmressifError(rc, msg)
+        self._lid = None
+
+    def close(self):
+        if self._h and self._h.value:
+            _lib.amressif_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc:
+            raise AmressifError(rc, _lib.amressif_last_error(self._h).decode())
+
+    @property
+    def dt(self):
+        return dt_of(self.inp)
+
+    def workspace_bytes(self):
+        return _lib.amressif_workspace_bytes(self._h)
+
+    def set_lid_profile(self, lid: torch.Tensor | None):
+        if lid is not None:
+            assert lid.dtype == torch.float64 and lid.numel() == self.n and lid.is_cuda
+            self._lid = lid.contiguous()
+            self._check(_lib.amressif_set_lid_profile(self._h, ctypes.c_void_p(self._lid.data_ptr())))
+        else:
+            self._lid = None
+            self._check(_lib.amressif_set_lid_profile(self._h, None))
+
+    # ---- state
+    def load_state(self, u, v, uprev, vprev, ctx=None):
+        n = self.n
+        vc = faces_view(u, v, n)
+        vp = faces_view(uprev, vprev, n)
+        uc = cell_view(ctx, n, (ctx.shape[-1] - n) // 2) if ctx is not None else None
+        self._check(_lib.amressif_load_state(self._h, vc, vp, ctypes.byref(uc) if uc is not None else None))
+
+    def store_state(self, u=None, v=None, uprev=None, vprev=None, ctx=None, velCart=None):
+        n = self.n
+        vc = faces_view(u, v, n) if u is not None else None
+        vp = faces_view(uprev, vprev, n) if uprev is not None else None
+        uc = cell_view(ctx, n, (ctx.shape[-1] - n) // 2) if ctx is not None else None
+        cc = cell_view(velCart, n, (velCart.shape[-1] - n) // 2) if velCart is not None else None
+        self._check(_lib.amressif_store_state(self._h, vc, vp, ctypes.byref(uc) if uc is not None else None,
+                                              ctypes.byref(cc) if cc is not None else None))
+
+    def advance(self, nsteps: int, with_stats: bool = True):
+        st = (StepStats * max(1, nsteps))() if with_stats else None
+        self._check(_lib.amressif_advance(self._h, nsteps, st))
+        return [st[k].as_dict() for k in range(nsteps)] if with_stats else None
+
+    # ---- module functions (reference names)
+    def cont2cart(self, u, v, velCart=None):
+        """utilities.cpp:16-350"""
+        n = self.n
+        velCart = torch.zeros((2, n + 4, n + 4), dtype=torch.float64, device=u.device) if velCart is None else velCart
+        cc = cell_view(velCart, n, 2)
+        self._check(_lib.amressif_cont2cart(self._h, faces_view(u, v, n), ctypes.byref(cc)))
+        return velCart
+
+    def momentum_rhs(self, ustar, vstar, userCtx):
+        """One stage's momentum RHS: gradient_calc_approach2 + convective_flux_calc_new_quick +
+        viscous_flux_calc + flux wall ghosts + momentum_righthand_side_calc."""
+        n = self.n
+        rx = torch.empty((n, n + 1), dtype=torch.float64, device=ustar.device)
+        ry = torch.empty((n + 1, n), dtype=torch.float64, device=ustar.device)
+        uc = cell_view(userCtx, n, (userCtx.shape[-1] - n) // 2)
+        self._check(_lib.amressif_momentum_rhs(self._h, faces_view(ustar, vstar, n), ctypes.byref(uc),
+                                               faces_view(rx, ry, n)))
+        return rx, ry
+
+    def runge_kutta4_pseudo_time_stepping(self, sub, ustar, vstar, ucont, vcont, uprev, vprev, uprevprev,
+                                          vprevprev, userCtx):
+        """momentum.cpp:9-94 with the stage RHS: velStar updated in place."""
+        n = self.n
+        uc = cell_view(userCtx, n, (userCtx.shape[-1] - n) // 2)
+        self._check(_lib.amressif_rk_stage(self._h, sub, faces_view(ustar, vstar, n), faces_view(ucont, vcont, n),
+                                           faces_view(uprev, vprev, n), faces_view(uprevprev, vprevprev, n),
+                                           ctypes.byref(uc)))
+
+    def Error_Computation(self, ucont, vcont, ustar, vstar) -> float:
+        """utilities.cpp:421-493"""
+        out = ctypes.c_double()
+        self._check(_lib.amressif_pseudo_norm(self._h, faces_view(ucont, vcont, self.n),
+                                              faces_view(ustar, vstar, self.n), ctypes.byref(out)))
+        return out.value
+
+    def poisson_righthand_side_calc(self, u, v, rhs=None):
+        """fn_rhs.cpp:71-125: returns (rhs, ||D||inf, sum rhs)"""
+        n = self.n
+        rhs = torch.zeros((n + 2, n + 2), dtype=torch.float64, device=u.device) if rhs is None else rhs
+        rv = cell_view(rhs, n, 1)
+        linf, s = ctypes.c_double(), ctypes.c_double()
+        self._check(_lib.amressif_poisson_rhs(self._h, faces_view(u, v, n), ctypes.byref(rv), ctypes.byref(linf),
+                                              ctypes.byref(s)))
+        return rhs, linf.value, s.value
+
+    def poisson_advance(self, rhs, tol_rel=1e-10, tol_abs=0.0, sol=None):
+        """poisson.cpp:41-186: returns (sol, V-cycles, ||r||inf/||b||inf); sol mean-zero."""
+        n = self.n
+        sol = torch.zeros((n + 2, n + 2), dtype=torch.float64, device=rhs.device) if sol is None else sol
+        sv, rv = cell_view(sol, n, 1), cell_view(rhs, n, 1)
+        it, res = ctypes.c_int(), ctypes.c_double()
+        self._check(_lib.amressif_poisson_solve(self._h, ctypes.byref(sv), ctypes.byref(rv), tol_rel, tol_abs,
+                                                ctypes.byref(it), ctypes.byref(res)))
+        return sol, it.value, res.value
+
+    def update_solution(self, ustar, vstar, userCtx):
+        """poisson.cpp:192-392: returns (u, v, ||D(u)||inf); userCtx p updated in place."""
+        n = self.n
+        u = torch.empty((n, n + 1), dtype=torch.float64, device=ustar.device)
+        v = torch.empty((n + 1, n), dtype=torch.float64, device=ustar.device)
+        uc = cell_view(userCtx, n, (userCtx.shape[-1] - n) // 2)
+        d = ctypes.c_double()
+        self._check(_lib.amressif_project(self._h, faces_view(ustar, vstar, n), ctypes.byref(uc),
+                                          faces_view(u, v, n), ctypes.byref(d)))
+        return u, v, d.value
+
+
+@dataclass
+class Run:
+    """A whole single-box run: state tensors on the GPU and a context."""
+    ctx: Context
+    u: torch.Tensor
+    v: torch.Tensor
+    uprev: torch.Tensor
+    vprev: torch.Tensor
+    userCtx: torch.Tensor
+
+    @classmethod
+    def from_numpy(cls, inp: Inputs, state: dict, device="cuda"):
+        ctx = Context(inp)
+        t = {k: torch.as_tensor(state[k], dtype=torch.float64, device=device).contiguous()
+             for k in ("u", "v", "uprev", "vprev", "ctx")}
+        r = cls(ctx, t["u"], t["v"], t["uprev"], t["vprev"], t["ctx"])
+        ctx.load_state(r.u, r.v, r.uprev, r.vprev, r.userCtx)
+        return r
+
+    def advance(self, nsteps):
+        return self.ctx.advance(nsteps)
+
+    def fetch(self):
+        """state back to host numpy (same keys as the oracle's state dict)"""
+        n = self.ctx.n
+        velCart = torch.zeros((2, n + 4, n + 4), dtype=torch.float64, device=self.u.device)
+        self.ctx.store_state(self.u, self.v, self.uprev, self.vprev, self.userCtx, velCart)
+        return {"u": self.u.cpu().numpy(), "v": self.v.cpu().numpy(), "uprev": self.uprev.cpu().numpy(),
+                "vprev": self.vprev.cpu().numpy(), "ctx": self.userCtx.cpu().numpy(), "vc": velCart.cpu().numpy()}

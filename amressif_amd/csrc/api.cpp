@@ -1,0 +1,772 @@
+// C-ABI entry points (include/amressif.h) and the host-side step driver.
+//
+// The driver mirrors the reference's time loop (Source/src1_rever/main.cpp:
+// 875-1202) with the per-iteration host syncs replaced by device-side loop
+// control: pseudo-iterations and V-cycles are enqueued in batches of guarded
+// kernels (each returns at once when the control block says the loop is
+// done), and the host polls the control block once per batch.  The batch size
+// is the previous step's count, so a step usually costs one poll per loop.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+
+struct amressif_ctx {
+    amr::Ctx c;
+};
+
+using amr::Ctx;
+
+namespace amr {
+template <class T>
+hipError_t dev_alloc_public(Ctx &c, T **ptr, size_t count)
+{
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, count * sizeof(T) + 256);
+    if (e != hipSuccess) return e;
+    c.allocs.push_back(p);
+    c.bytes += (int64_t)(count * sizeof(T) + 256);
+    *ptr = (T *)p;
+    return hipSuccess;
+}
+
+}  // namespace amr
+
+namespace {
+
+int fail(amressif_ctx *ctx, int code, const std::string &msg)
+{
+    if (ctx) ctx->c.err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            return fail(ctx, AMRESSIF_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));     \
+    } while (0)
+
+int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+// main.cpp:810-835 (BCRec mapping) and the cont2cart branches
+// (utilities.cpp:87-269): returns 0, AMRESSIF_ERR_BC where the reference
+// aborts, AMRESSIF_ERR_UNSUPPORTED for paths not built here.
+int check_bc(const amressif_inputs &in, std::string &msg)
+{
+    for (int d = 0; d < 2; ++d) {
+        const int lo = in.phy_bc_lo[d], hi = in.phy_bc_hi[d];
+        if (!(lo == 111 || std::abs(lo) == 135 || std::abs(lo) == 165 || std::abs(lo) == 195)) {
+            msg = "Invalid bc_lo";
+            return AMRESSIF_ERR_BC;
+        }
+        // the reference's hi-side test reads phy_bc_lo for the 135 case (main.cpp:827)
+        if (!(hi == 111 || std::abs(lo) == 135 || std::abs(hi) == 165 || std::abs(hi) == 195)) {
+            msg = "Invalid bc_hi";
+            return AMRESSIF_ERR_BC;
+        }
+        if (lo == 111 || hi == 111) {
+            msg = "periodic boundaries (111) are not supported by this build";
+            return AMRESSIF_ERR_UNSUPPORTED;
+        }
+    }
+    const int W = in.phy_bc_lo[0], E = in.phy_bc_hi[0], S = in.phy_bc_lo[1], N = in.phy_bc_hi[1];
+    if (W == 195 || W == -195) { msg = "west 195: Not yet implemented"; return AMRESSIF_ERR_BC; }
+    if (E == -165 || E == 195 || E == -195) { msg = "east code: Not yet implemented"; return AMRESSIF_ERR_BC; }
+    if (S == -165 || S == 195 || S == -195) { msg = "south code: Not yet implemented"; return AMRESSIF_ERR_BC; }
+    if (N == -165 || N == 195 || N == -195) { msg = "north code: Not yet implemented"; return AMRESSIF_ERR_BC; }
+    if (!(E == 135 || E == 165)) {
+        msg = "east code leaves the reference's ghost cells unfilled (utilities.cpp:137-184)";
+        return AMRESSIF_ERR_UNSUPPORTED;
+    }
+    return 0;
+}
+
+// dense pseudo-inverse of the Neumann 5-point Laplacian on the coarsest
+// level: pinv(L) = J - (J - L)^{-1}, J = 11^T/m, via Cholesky of J - L.
+bool neumann_pinv(int nx, int ny, double ax, double ay, std::vector<double> &out)
+{
+    const int m = nx * ny;
+    std::vector<double> A((size_t)m * m, 0.0);
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            const int r = i + nx * j;
+            double d = 0.0;
+            if (i > 0) { A[(size_t)r * m + r - 1] -= ax; d += ax; }
+            if (i < nx - 1) { A[(size_t)r * m + r + 1] -= ax; d += ax; }
+            if (j > 0) { A[(size_t)r * m + r - nx] -= ay; d += ay; }
+            if (j < ny - 1) { A[(size_t)r * m + r + nx] -= ay; d += ay; }
+            A[(size_t)r * m + r] += d;
+        }
+    for (auto &a : A) a += 1.0 / m;
+    // in-place Cholesky A = R^T R (upper R)
+    for (int k = 0; k < m; ++k) {
+        double d = A[(size_t)k * m + k];
+        for (int s = 0; s < k; ++s) d -= A[(size_t)s * m + k] * A[(size_t)s * m + k];
+        if (!(d > 0)) return false;
+        d = std::sqrt(d);
+        A[(size_t)k * m + k] = d;
+        for (int j = k + 1; j < m; ++j) {
+            double v = A[(size_t)k * m + j];
+            for (int s = 0; s < k; ++s) v -= A[(size_t)s * m + k] * A[(size_t)s * m + j];
+            A[(size_t)k * m + j] = v / d;
+        }
+    }
+    out.assign((size_t)m * m, 0.0);
+    std::vector<double> col(m);
+    for (int c = 0; c < m; ++c) {
+        std::fill(col.begin(), col.end(), 0.0);
+        col[c] = 1.0;
+        for (int i = 0; i < m; ++i) {
+            double v = col[i];
+            for (int s = 0; s < i; ++s) v -= A[(size_t)s * m + i] * col[s];
+            col[i] = v / A[(size_t)i * m + i];
+        }
+        for (int i = m - 1; i >= 0; --i) {
+            double v = col[i];
+            for (int s = i + 1; s < m; ++s) v -= A[(size_t)i * m + s] * col[s];
+            col[i] = v / A[(size_t)i * m + i];
+        }
+        for (int i = 0; i < m; ++i) out[(size_t)i * m + c] = 1.0 / m - col[i];
+    }
+    return true;
+}
+
+// ---- Array4 validation: the view must cover the box the entry point uses
+int check_view(amressif_ctx *ctx, const amr_array4 *a, int lo0, int lo1, int hi0, int hi1, int ncomp,
+               const char *what)
+{
+    if (!a || !a->ptr) return fail(ctx, AMRESSIF_ERR_ARG, std::string(what) + ": null view");
+    if (a->dtype != 0) return fail(ctx, AMRESSIF_ERR_ARG, std::string(what) + ": double views only");
+    if (a->lo[0] > lo0 || a->lo[1] > lo1 || a->hi[0] < hi0 || a->hi[1] < hi1)
+        return fail(ctx, AMRESSIF_ERR_ARG, std::string(what) + ": view does not cover the box");
+    if (a->hi[2] < a->lo[2] || a->ncomp < ncomp)
+        return fail(ctx, AMRESSIF_ERR_ARG, std::string(what) + ": bad component/z extent");
+    return 0;
+}
+
+int check_face(amressif_ctx *ctx, const amr_array4 *a, int dir, const char *what)
+{
+    const int n = ctx->c.g.n;
+    return dir == 0 ? check_view(ctx, a, 0, 0, n, n - 1, 1, what) : check_view(ctx, a, 0, 0, n - 1, n, 1, what);
+}
+
+int check_cell(amressif_ctx *ctx, const amr_array4 *a, int ncomp, const char *what)
+{
+    const int n = ctx->c.g.n;
+    return check_view(ctx, a, 0, 0, n - 1, n - 1, ncomp, what);
+}
+
+int poll(amressif_ctx *ctx)
+{
+    Ctx &c = ctx->c;
+    HIPCHK(hipMemcpyAsync(c.ctl_host, c.ctl, sizeof(amr::Ctl), hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    return 0;
+}
+
+int ensure_module_scratch(amressif_ctx *ctx)
+{
+    Ctx &c = ctx->c;
+    if (c.Mc[0]) return 0;
+    const size_t fsz = (size_t)(c.g.n + 1) * c.g.P;
+    for (int k = 0; k < 4; ++k)
+        for (int d = 0; d < 2; ++d) HIPCHK(amr::dev_alloc_public(c, &c.M[k][d], fsz));
+    for (int d = 0; d < 2; ++d) HIPCHK(amr::dev_alloc_public(c, &c.Mc[d], fsz));
+    HIPCHK(amr::dev_alloc_public(c, &c.dscalar, 8));
+    return 0;
+}
+
+// ---- the Poisson solve on c.phi / c.b (b already centred), shared by the
+// step and the module entry point
+int poisson_loop(amressif_ctx *ctx)
+{
+    Ctx &c = ctx->c;
+    const int mixed = c.in.precision == AMRESSIF_MIXED;
+    int nb = 0;
+    HIPCHK(amr::launch_mg_start(c));
+    HIPCHK(amr::launch_fill(c, c.phi, (int64_t)(c.g.n + 1) * c.g.P, 0.0));
+    HIPCHK(amr::launch_residual(c, c.phi, c.b, c.lev[0].r, &nb));
+    HIPCHK(amr::launch_fin_resid(c, nb));
+    int launched = 0, target = std::max(1, c.last_V);
+    const int cap = std::max(1, c.in.mg_max_iter);
+    for (;;) {
+        while (launched < target && launched < cap) {
+            HIPCHK(amr::launch_vcycle(c, mixed));
+            HIPCHK(amr::launch_add_correction(c, c.phi, mixed));
+            HIPCHK(amr::launch_residual(c, c.phi, c.b, c.lev[0].r, &nb));
+            HIPCHK(amr::launch_fin_resid(c, nb));
+            ++launched;
+        }
+        if (int rc = poll(ctx)) return rc;
+        if (c.ctl_host->mg_done || launched >= cap) break;
+        target = launched + 2;
+    }
+    c.last_V = std::max(1, c.ctl_host->mg_iters);
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+void amressif_inputs_default(amressif_inputs *in)
+{
+    std::memset(in, 0, sizeof(*in));
+    in->n_cell = -1;
+    in->max_grid_size[0] = in->max_grid_size[1] = in->max_grid_size[2] = -1;
+    in->lo_phy_dim[0] = in->lo_phy_dim[1] = in->lo_phy_dim[2] = 0.0;
+    in->hi_phy_dim[0] = in->hi_phy_dim[1] = in->hi_phy_dim[2] = 1.0;
+    in->IterNum = -1;
+    in->nsteps = 1;
+    in->cfl = 0.9;
+    in->fixed_dt = -1.0;
+    in->ren = -1.0;
+    in->vis = -1.0;
+    in->target_resolution = -1;
+    in->momentum_tolerance = 1.e-10;
+    in->PSEUDO_TIMESTEPPING = 1;
+    in->plot_int = -1;
+    in->chk_int = -1;
+    in->txt_int = 0;
+    in->chk_out = 0;
+    in->poisson_tol_rel = 1.0e-10;
+    in->poisson_tol_abs = 0.0;
+    in->mg_max_iter = 2000;
+    in->mg_bottom_max = 16;
+    in->precision = AMRESSIF_FP64;
+}
+
+double amressif_dt(const amressif_inputs *in)
+{
+    const double dx = (in->hi_phy_dim[0] - in->lo_phy_dim[0]) / in->n_cell;
+    const double dy = (in->hi_phy_dim[1] - in->lo_phy_dim[1]) / in->n_cell;
+    const double coeff = 1. / (dx * dx) + 1. / (dy * dy);
+    double dt = in->cfl / (2.0 * coeff);
+    if (in->fixed_dt != -1.0) dt = in->fixed_dt;
+    return dt;
+}
+
+int amressif_parse_inputs(const char *path, int argc, const char *const *argv, amressif_inputs *out, char *err,
+                          int errlen)
+{
+    auto seterr = [&](const std::string &m) {
+        if (err && errlen > 0) {
+            std::snprintf(err, (size_t)errlen, "%s", m.c_str());
+        }
+        return AMRESSIF_ERR_PARSE;
+    };
+    amressif_inputs_default(out);
+    std::map<std::string, std::vector<std::string>> kv;
+    auto add_line = [&](std::string line) -> bool {
+        const auto h = line.find('#');
+        if (h != std::string::npos) line = line.substr(0, h);
+        const auto eq = line.find('=');
+        if (eq == std::string::npos) {
+            for (char ch : line)
+                if (!std::isspace((unsigned char)ch)) return false;
+            return true;
+        }
+        std::string key = line.substr(0, eq);
+        key.erase(0, key.find_first_not_of(" \t\r"));
+        key.erase(key.find_last_not_of(" \t\r") + 1);
+        std::istringstream vs(line.substr(eq + 1));
+        std::vector<std::string> vals;
+        std::string tok;
+        while (vs >> tok) vals.push_back(tok);
+        if (key.empty()) return false;
+        kv[key] = vals;
+        return true;
+    };
+    if (path && path[0]) {
+        std::ifstream f(path);
+        if (!f) return seterr(std::string("cannot open inputs file ") + path);
+        std::string line;
+        int ln = 0;
+        while (std::getline(f, line)) {
+            ++ln;
+            if (!add_line(line)) return seterr("malformed line " + std::to_string(ln) + ": " + line);
+        }
+    }
+    for (int a = 0; a < argc; ++a)
+        if (!add_line(argv[a])) return seterr(std::string("malformed override: ") + argv[a]);
+
+    auto num = [&](const std::string &key, const std::string &s, double &v) -> bool {
+        char *end = nullptr;
+        v = std::strtod(s.c_str(), &end);
+        return end && *end == '\0' && !s.empty();
+    };
+    std::string bad;
+    auto get_int = [&](const char *key, int *dst, int count, bool required) -> bool {
+        auto it = kv.find(key);
+        if (it == kv.end()) {
+            if (required) bad = std::string("missing required key ") + key;
+            return !required;
+        }
+        if ((int)it->second.size() < count) { bad = std::string("too few values for ") + key; return false; }
+        for (int k = 0; k < count; ++k) {
+            double v;
+            if (!num(key, it->second[k], v) || v != std::floor(v)) { bad = std::string("bad integer for ") + key; return false; }
+            dst[k] = (int)v;
+        }
+        return true;
+    };
+    auto get_real = [&](const char *key, double *dst, int count, bool required) -> bool {
+        auto it = kv.find(key);
+        if (it == kv.end()) {
+            if (required) bad = std::string("missing required key ") + key;
+            return !required;
+        }
+        if ((int)it->second.size() < count) { bad = std::string("too few values for ") + key; return false; }
+        for (int k = 0; k < count; ++k)
+            if (!num(key, it->second[k], dst[k])) { bad = std::string("bad number for ") + key; return false; }
+        return true;
+    };
+    // 2D: the array keys take AMREX_SPACEDIM = 2 values (extra values ignored)
+    bool ok = get_int("n_cell", &out->n_cell, 1, true) && get_int("max_grid_size_x", &out->max_grid_size[0], 1, true) &&
+              get_int("max_grid_size_y", &out->max_grid_size[1], 1, true) &&
+              get_int("max_grid_size_z", &out->max_grid_size[2], 1, true) &&
+              get_real("lo_phy_dim", out->lo_phy_dim, 2, false) && get_real("hi_phy_dim", out->hi_phy_dim, 2, false) &&
+              get_int("IterNum", &out->IterNum, 1, true) && get_int("nsteps", &out->nsteps, 1, false) &&
+              get_real("cfl", &out->cfl, 1, false) && get_real("fixed_dt", &out->fixed_dt, 1, false) &&
+              get_real("ren", &out->ren, 1, true) && get_real("vis", &out->vis, 1, true) &&
+              get_int("phy_bc_lo", out->phy_bc_lo, 2, false) && get_int("phy_bc_hi", out->phy_bc_hi, 2, false) &&
+              get_real("inflow_waveform", out->inflow_waveform, 2, false) &&
+              get_int("target_resolution", &out->target_resolution, 1, false) &&
+              get_real("momentum_tolerance", &out->momentum_tolerance, 1, false) &&
+              get_int("PSEUDO_TIMESTEPPING", &out->PSEUDO_TIMESTEPPING, 1, false) &&
+              get_int("plot_int", &out->plot_int, 1, false) && get_int("chk_int", &out->chk_int, 1, false) &&
+              get_int("txt_int", &out->txt_int, 1, false) && get_int("chk_out", &out->chk_out, 1, false) &&
+              get_real("amressif.poisson_tol_rel", &out->poisson_tol_rel, 1, false) &&
+              get_real("amressif.poisson_tol_abs", &out->poisson_tol_abs, 1, false) &&
+              get_int("amressif.mg_max_iter", &out->mg_max_iter, 1, false) &&
+              get_int("amressif.mg_bottom_max", &out->mg_bottom_max, 1, false) &&
+              get_int("amressif.precision", &out->precision, 1, false);
+    if (!ok) return seterr(bad);
+    return AMRESSIF_OK;
+}
+
+int amressif_create(const amressif_inputs *in, void *hip_stream, amressif_ctx **out)
+{
+    if (!in || !out) return AMRESSIF_ERR_ARG;
+    *out = nullptr;
+    auto *ctx = new amressif_ctx();
+    Ctx &c = ctx->c;
+    c.in = *in;
+    c.stream = (hipStream_t)hip_stream;
+    auto bail = [&](int code, const std::string &m) {
+        // keep the context so the caller can read the message; it owns nothing yet
+        ctx->c.err = m;
+        *out = ctx;
+        return code;
+    };
+    const int n = in->n_cell;
+    if (n < 4 || n > 16384) return bail(AMRESSIF_ERR_ARG, "n_cell must be in [4, 16384]");
+    if (in->ren <= 0) return bail(AMRESSIF_ERR_ARG, "ren must be > 0");
+    if (in->IterNum < 0) return bail(AMRESSIF_ERR_ARG, "IterNum must be >= 0");
+    if (in->PSEUDO_TIMESTEPPING != 1)
+        return bail(AMRESSIF_ERR_UNSUPPORTED, "only PSEUDO_TIMESTEPPING = 1 (the dual-time RK4 path) is built");
+    if (in->precision != AMRESSIF_FP64 && in->precision != AMRESSIF_MIXED)
+        return bail(AMRESSIF_ERR_ARG, "precision must be 0 (FP64) or 1 (mixed)");
+    if (in->mg_bottom_max < 2 || in->mg_bottom_max > 32) return bail(AMRESSIF_ERR_ARG, "mg_bottom_max must be in [2, 32]");
+    {
+        std::string msg;
+        if (int rc = check_bc(*in, msg)) return bail(rc, msg);
+    }
+    amr::Geometry &g = c.g;
+    g.n = n;
+    g.P = round_up(n + 1, 64);
+    g.dx = (in->hi_phy_dim[0] - in->lo_phy_dim[0]) / n;
+    g.dy = (in->hi_phy_dim[1] - in->lo_phy_dim[1]) / n;
+    if (!(g.dx > 0) || !(g.dy > 0)) return bail(AMRESSIF_ERR_ARG, "hi_phy_dim must exceed lo_phy_dim");
+    g.dt = amressif_dt(in);
+    g.bc_lo0 = in->phy_bc_lo[0]; g.bc_hi0 = in->phy_bc_hi[0];
+    g.bc_lo1 = in->phy_bc_lo[1]; g.bc_hi1 = in->phy_bc_hi[1];
+    g.inflow0 = in->inflow_waveform[0]; g.inflow1 = in->inflow_waveform[1];
+    g.ren = in->ren;
+    const double d_tau = 0.4 * g.dt;  // main.cpp:861-869
+    c.rk[0] = d_tau * 0.25;
+    c.rk[1] = d_tau * (1.0 / 3.0);
+    c.rk[2] = d_tau * 0.5;
+    c.rk[3] = d_tau * 1.0;
+
+    const size_t fsz = (size_t)(n + 1) * g.P;
+    auto A = [&](auto **p, size_t cnt) { return amr::dev_alloc_public(c, p, cnt); };
+#define ALLOC(p, cnt)                                                               \
+    do {                                                                            \
+        hipError_t e_ = A(p, cnt);                                                  \
+        if (e_ != hipSuccess) return bail(AMRESSIF_ERR_HIP, hipGetErrorString(e_)); \
+    } while (0)
+    for (int d = 0; d < 2; ++d) {
+        ALLOC(&c.U[d], fsz); ALLOC(&c.Uprev[d], fsz); ALLOC(&c.C[d], fsz);
+        ALLOC(&c.S1[d], fsz); ALLOC(&c.S2[d], fsz);
+    }
+    ALLOC(&c.p, fsz); ALLOC(&c.phi, fsz); ALLOC(&c.b, fsz);
+    // multigrid hierarchy (level 0 shares the fine pitch)
+    {
+        int nx = n, ny = n;
+        double hx = g.dx, hy = g.dy;
+        int l = 0;
+        for (;;) {
+            amr::Level &L = c.lev[l];
+            L.nx = nx; L.ny = ny;
+            L.pitch = l == 0 ? g.P : round_up(nx, nx >= 64 ? 64 : 16);
+            L.ax = 1.0 / (hx * hx); L.ay = 1.0 / (hy * hy);
+            const size_t lsz = (size_t)L.pitch * (ny + 1);
+            ALLOC(&L.e, lsz); ALLOC(&L.r, lsz); ALLOC(&L.e32, lsz); ALLOC(&L.r32, lsz);
+            ++l;
+            if (nx > in->mg_bottom_max && ny > in->mg_bottom_max && nx % 2 == 0 && ny % 2 == 0 && l < amr::kMaxLevels) {
+                nx /= 2; ny /= 2; hx *= 2; hy *= 2;
+            } else break;
+        }
+        c.nlev = l;
+        const amr::Level &B = c.lev[l - 1];
+        c.mb = B.nx * B.ny;
+        if (c.mb > 1024) return bail(AMRESSIF_ERR_UNSUPPORTED, "coarsest multigrid level exceeds 1024 cells");
+        std::vector<double> pinv;
+        if (!neumann_pinv(B.nx, B.ny, B.ax, B.ay, pinv)) return bail(AMRESSIF_ERR_ARG, "bottom factorisation failed");
+        std::vector<float> pinv32(pinv.begin(), pinv.end());
+        ALLOC(&c.pinv, pinv.size()); ALLOC(&c.pinv32, pinv.size());
+        hipError_t e1 = hipMemcpy(c.pinv, pinv.data(), pinv.size() * 8, hipMemcpyHostToDevice);
+        hipError_t e2 = hipMemcpy(c.pinv32, pinv32.data(), pinv32.size() * 4, hipMemcpyHostToDevice);
+        if (e1 != hipSuccess || e2 != hipSuccess) return bail(AMRESSIF_ERR_HIP, "pinv upload failed");
+    }
+    ALLOC(&c.partials, (size_t)amr::kMaxPartials * 4);
+    ALLOC(&c.ctl, 1);
+    {
+        hipError_t e = hipHostMalloc((void **)&c.ctl_host, sizeof(amr::Ctl), hipHostMallocDefault);
+        if (e != hipSuccess) return bail(AMRESSIF_ERR_HIP, hipGetErrorString(e));
+        std::memset(c.ctl_host, 0, sizeof(amr::Ctl));
+        e = hipMemcpy(c.ctl, c.ctl_host, sizeof(amr::Ctl), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return bail(AMRESSIF_ERR_HIP, hipGetErrorString(e));
+    }
+    // zero the state (a cavity at rest) so advance works before load_state
+    for (int d = 0; d < 2; ++d) {
+        hipMemset(c.U[d], 0, fsz * 8);
+        hipMemset(c.Uprev[d], 0, fsz * 8);
+    }
+    hipMemset(c.p, 0, fsz * 8);
+    hipMemset(c.phi, 0, fsz * 8);
+    if (hipDeviceSynchronize() != hipSuccess) return bail(AMRESSIF_ERR_HIP, "device sync failed");
+#undef ALLOC
+    *out = ctx;
+    return AMRESSIF_OK;
+}
+
+int amressif_destroy(amressif_ctx *ctx)
+{
+    if (!ctx) return AMRESSIF_OK;
+    Ctx &c = ctx->c;
+    if (c.stream) hipStreamSynchronize(c.stream);
+    for (void *p : c.allocs) hipFree(p);
+    if (c.ctl_host) hipHostFree(c.ctl_host);
+    if (c.log) hipFree(c.log);
+    delete ctx;
+    return AMRESSIF_OK;
+}
+
+const char *amressif_last_error(const amressif_ctx *ctx) { return ctx ? ctx->c.err.c_str() : "null context"; }
+
+int64_t amressif_workspace_bytes(const amressif_ctx *ctx) { return ctx ? ctx->c.bytes : 0; }
+
+int amressif_set_lid_profile(amressif_ctx *ctx, const double *dev_lid)
+{
+    if (!ctx) return AMRESSIF_ERR_ARG;
+    if (dev_lid && ctx->c.g.bc_hi1 != 165)
+        return fail(ctx, AMRESSIF_ERR_ARG, "a lid profile needs the north inlet code 165");
+    ctx->c.lid = dev_lid;
+    return AMRESSIF_OK;
+}
+
+int amressif_load_state(amressif_ctx *ctx, const amr_array4 velCont[2], const amr_array4 velContPrev[2],
+                        const amr_array4 *userCtx)
+{
+    if (!ctx || !velCont || !velContPrev) return fail(ctx, AMRESSIF_ERR_ARG, "null argument");
+    Ctx &c = ctx->c;
+    for (int d = 0; d < 2; ++d) {
+        if (int rc = check_face(ctx, &velCont[d], d, "velCont")) return rc;
+        if (int rc = check_face(ctx, &velContPrev[d], d, "velContPrev")) return rc;
+        HIPCHK(amr::launch_pack_face(c, velCont[d], c.U[d], d));
+        HIPCHK(amr::launch_pack_face(c, velContPrev[d], c.Uprev[d], d));
+    }
+    if (userCtx) {
+        if (int rc = check_cell(ctx, userCtx, 2, "userCtx")) return rc;
+        HIPCHK(amr::launch_pack_cell(c, *userCtx, 0, c.p));
+        HIPCHK(amr::launch_pack_cell(c, *userCtx, 1, c.phi));
+    } else {
+        HIPCHK(amr::launch_fill(c, c.p, (int64_t)(c.g.n + 1) * c.g.P, 0.0));
+        HIPCHK(amr::launch_fill(c, c.phi, (int64_t)(c.g.n + 1) * c.g.P, 0.0));
+    }
+    HIPCHK(amr::launch_mg_start(c));  // phi_mean = 0 for the loaded phi
+    HIPCHK(hipStreamSynchronize(c.stream));
+    return AMRESSIF_OK;
+}
+
+int amressif_store_state(const amressif_ctx *cctx, amr_array4 velCont[2], amr_array4 velContPrev[2],
+                         amr_array4 *userCtx, amr_array4 *velCart)
+{
+    amressif_ctx *ctx = const_cast<amressif_ctx *>(cctx);
+    if (!ctx) return AMRESSIF_ERR_ARG;
+    Ctx &c = ctx->c;
+    for (int d = 0; d < 2; ++d) {
+        if (velCont) {
+            if (int rc = check_face(ctx, &velCont[d], d, "velCont")) return rc;
+            HIPCHK(amr::launch_unpack_face(c, c.U[d], velCont[d], d));
+        }
+        if (velContPrev) {
+            if (int rc = check_face(ctx, &velContPrev[d], d, "velContPrev")) return rc;
+            HIPCHK(amr::launch_unpack_face(c, c.Uprev[d], velContPrev[d], d));
+        }
+    }
+    if (userCtx) {
+        if (int rc = check_cell(ctx, userCtx, 2, "userCtx")) return rc;
+        HIPCHK(amr::launch_unpack_cell_shift(c, c.p, *userCtx, 0, 1, nullptr));
+        HIPCHK(amr::launch_unpack_cell_shift(c, c.phi, *userCtx, 1, 1, &c.ctl->phi_mean));
+    }
+    if (velCart) {
+        if (int rc = check_cell(ctx, velCart, 2, "velCart")) return rc;
+        HIPCHK(amr::launch_cont2cart_out(c, c.U[0], c.U[1], *velCart));
+    }
+    HIPCHK(hipStreamSynchronize(c.stream));
+    return AMRESSIF_OK;
+}
+
+int amressif_advance(amressif_ctx *ctx, int nsteps, amressif_step_stats *stats)
+{
+    if (!ctx || nsteps < 0) return fail(ctx, AMRESSIF_ERR_ARG, "bad argument");
+    Ctx &c = ctx->c;
+    if (nsteps == 0) return AMRESSIF_OK;
+    if (c.log_cap < nsteps) {
+        if (c.log) HIPCHK(hipFree(c.log));
+        c.log = nullptr;
+        HIPCHK(hipMalloc((void **)&c.log, sizeof(amressif_step_stats) * (size_t)nsteps));
+        c.log_cap = nsteps;
+    }
+    HIPCHK(amr::launch_ctl_set(c, c.in.momentum_tolerance, c.in.poisson_tol_rel, c.in.poisson_tol_abs, c.in.IterNum,
+                               c.in.mg_max_iter));
+    int status = AMRESSIF_OK;
+    for (int s = 0; s < nsteps; ++s) {
+        HIPCHK(amr::launch_ctl_reset_step(c));
+        HIPCHK(amr::launch_bdf2(c));
+        // ---- momentum: RK4 dual-time pseudo-iterations (main.cpp:898-1040)
+        int launched = 0, target = std::max(1, c.last_P), nb = 0;
+        const int cap = c.in.IterNum + 1;
+        for (;;) {
+            while (launched < target && launched < cap) {
+                amr::StageLaunch a{};
+                a.ub = c.U[0]; a.vb = c.U[1]; a.cu = c.C[0]; a.cv = c.C[1]; a.p = c.p;
+                const double *in_u[4] = {c.U[0], c.S1[0], c.S2[0], c.S1[0]};
+                const double *in_v[4] = {c.U[1], c.S1[1], c.S2[1], c.S1[1]};
+                double *out_u[4] = {c.S1[0], c.S2[0], c.S1[0], c.U[0]};
+                double *out_v[4] = {c.S1[1], c.S2[1], c.S1[1], c.U[1]};
+                for (int sub = 0; sub < 4; ++sub) {
+                    a.us = in_u[sub]; a.vs = in_v[sub]; a.uo = out_u[sub]; a.vo = out_v[sub];
+                    a.rk = c.rk[sub];
+                    a.mode = sub == 3 ? 1 : 0;
+                    HIPCHK(amr::launch_stage(c, a, &nb));
+                }
+                HIPCHK(amr::launch_fin_pseudo(c, nb));
+                ++launched;
+            }
+            if (int rc = poll(ctx)) return rc;
+            if (c.ctl_host->mom_done || launched >= cap) break;
+            target = launched + 2;
+        }
+        c.last_P = std::max(1, c.ctl_host->mom_count);
+        const int flags = c.ctl_host->flags;
+        // ---- projection: Poisson RHS, MLMG-equivalent solve, update
+        HIPCHK(amr::launch_poisson_rhs(c, c.U[0], c.U[1], c.b, &nb));
+        HIPCHK(amr::launch_fin_prhs(c, nb));
+        HIPCHK(amr::launch_center_b(c, c.b));
+        if (int rc = poisson_loop(ctx)) return rc;
+        HIPCHK(amr::launch_project(c, c.U[0], c.U[1], c.S1[0], c.S1[1], &nb));
+        HIPCHK(amr::launch_fin_project(c, nb, s));
+        std::swap(c.U[0], c.S1[0]);
+        std::swap(c.U[1], c.S1[1]);
+        if (flags & AMRESSIF_FLAG_NONFINITE) {
+            status = fail(ctx, AMRESSIF_ERR_NONFINITE, "non-finite pseudo-iteration norm at step " + std::to_string(s));
+            nsteps = s + 1;
+            break;
+        }
+    }
+    if (stats) HIPCHK(hipMemcpyAsync(stats, c.log, sizeof(amressif_step_stats) * (size_t)nsteps,
+                                     hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    return status;
+}
+
+// ------------------------------------------------------ module entry points
+int amressif_cont2cart(amressif_ctx *ctx, const amr_array4 velCont[2], amr_array4 *velCart)
+{
+    if (!ctx || !velCont) return fail(ctx, AMRESSIF_ERR_ARG, "null argument");
+    Ctx &c = ctx->c;
+    if (int rc = ensure_module_scratch(ctx)) return rc;
+    for (int d = 0; d < 2; ++d) {
+        if (int rc = check_face(ctx, &velCont[d], d, "velCont")) return rc;
+        HIPCHK(amr::launch_pack_face(c, velCont[d], c.M[0][d], d));
+    }
+    if (int rc = check_cell(ctx, velCart, 2, "velCart")) return rc;
+    HIPCHK(amr::launch_cont2cart_out(c, c.M[0][0], c.M[0][1], *velCart));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    return AMRESSIF_OK;
+}
+
+int amressif_momentum_rhs(amressif_ctx *ctx, const amr_array4 velStar[2], const amr_array4 *userCtx,
+                          amr_array4 rhs[2])
+{
+    if (!ctx || !velStar || !userCtx || !rhs) return fail(ctx, AMRESSIF_ERR_ARG, "null argument");
+    Ctx &c = ctx->c;
+    if (int rc = ensure_module_scratch(ctx)) return rc;
+    for (int d = 0; d < 2; ++d) {
+        if (int rc = check_face(ctx, &velStar[d], d, "velStar")) return rc;
+        if (int rc = check_face(ctx, &rhs[d], d, "rhs")) return rc;
+        HIPCHK(amr::launch_pack_face(c, velStar[d], c.M[0][d], d));
+    }
+    if (int rc = check_cell(ctx, userCtx, 1, "userCtx")) return rc;
+    HIPCHK(amr::launch_pack_cell(c, *userCtx, 0, c.Mc[0]));
+    amr::StageLaunch a{};
+    a.us = c.M[0][0]; a.vs = c.M[0][1]; a.p = c.Mc[0];
+    a.uo = c.M[1][0]; a.vo = c.M[1][1];
+    a.mode = 2;
+    int nb = 0;
+    HIPCHK(amr::launch_stage(c, a, &nb));
+    for (int d = 0; d < 2; ++d) HIPCHK(amr::launch_unpack_face(c, c.M[1][d], rhs[d], d));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    return AMRESSIF_OK;
+}
+
+int amressif_rk_stage(amressif_ctx *ctx, int sub, amr_array4 velStar[2], const amr_array4 velCont[2],
+                      const amr_array4 velContPrev[2], const amr_array4 velContPrevPrev[2], const amr_array4 *userCtx)
+{
+    if (!ctx || !velStar || !velCont || !velContPrev || !velContPrevPrev || !userCtx)
+        return fail(ctx, AMRESSIF_ERR_ARG, "null argument");
+    if (sub < 0 || sub > 3) return fail(ctx, AMRESSIF_ERR_ARG, "sub must be 0..3");
+    Ctx &c = ctx->c;
+    if (int rc = ensure_module_scratch(ctx)) return rc;
+    for (int d = 0; d < 2; ++d) {
+        if (int rc = check_face(ctx, &velStar[d], d, "velStar")) return rc;
+        if (int rc = check_face(ctx, &velCont[d], d, "velCont")) return rc;
+        if (int rc = check_face(ctx, &velContPrev[d], d, "velContPrev")) return rc;
+        if (int rc = check_face(ctx, &velContPrevPrev[d], d, "velContPrevPrev")) return rc;
+        HIPCHK(amr::launch_pack_face(c, velStar[d], c.M[0][d], d));
+        HIPCHK(amr::launch_pack_face(c, velCont[d], c.M[1][d], d));
+        HIPCHK(amr::launch_pack_face(c, velContPrev[d], c.M[2][d], d));
+        HIPCHK(amr::launch_pack_face(c, velContPrevPrev[d], c.M[3][d], d));
+        HIPCHK(amr::launch_bdf2_const(c, c.M[2][d], c.M[3][d], c.M[2][d], d));  // in place: c from (u^n, u^{n-1})
+    }
+    if (int rc = check_cell(ctx, userCtx, 1, "userCtx")) return rc;
+    HIPCHK(amr::launch_pack_cell(c, *userCtx, 0, c.Mc[0]));
+    amr::StageLaunch a{};
+    a.us = c.M[0][0]; a.vs = c.M[0][1];
+    a.ub = c.M[1][0]; a.vb = c.M[1][1];
+    a.cu = c.M[2][0]; a.cv = c.M[2][1];
+    a.p = c.Mc[0];
+    a.uo = c.M[3][0]; a.vo = c.M[3][1];
+    a.rk = c.rk[sub];
+    a.mode = 0;
+    HIPCHK(amr::launch_ctl_reset_step(c));  // clears mom_done so the guarded kernel runs
+    int nb = 0;
+    HIPCHK(amr::launch_stage(c, a, &nb));
+    for (int d = 0; d < 2; ++d) HIPCHK(amr::launch_unpack_face(c, c.M[3][d], velStar[d], d));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    return AMRESSIF_OK;
+}
+
+int amressif_pseudo_norm(amressif_ctx *ctx, const amr_array4 velCont[2], const amr_array4 velStar[2], double *norm)
+{
+    if (!ctx || !velCont || !velStar || !norm) return fail(ctx, AMRESSIF_ERR_ARG, "null argument");
+    Ctx &c = ctx->c;
+    if (int rc = ensure_module_scratch(ctx)) return rc;
+    for (int d = 0; d < 2; ++d) {
+        if (int rc = check_face(ctx, &velCont[d], d, "velCont")) return rc;
+        if (int rc = check_face(ctx, &velStar[d], d, "velStar")) return rc;
+        HIPCHK(amr::launch_pack_face(c, velCont[d], c.M[0][d], d));
+        HIPCHK(amr::launch_pack_face(c, velStar[d], c.M[1][d], d));
+    }
+    HIPCHK(amr::launch_norm(c, c.M[0][0], c.M[0][1], c.M[1][0], c.M[1][1], c.dscalar));
+    HIPCHK(hipMemcpyAsync(norm, c.dscalar, 8, hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    return AMRESSIF_OK;
+}
+
+int amressif_poisson_rhs(amressif_ctx *ctx, const amr_array4 vel[2], amr_array4 *rhs, double *div_linf,
+                         double *rhs_sum)
+{
+    if (!ctx || !vel || !rhs) return fail(ctx, AMRESSIF_ERR_ARG, "null argument");
+    Ctx &c = ctx->c;
+    if (int rc = ensure_module_scratch(ctx)) return rc;
+    for (int d = 0; d < 2; ++d) {
+        if (int rc = check_face(ctx, &vel[d], d, "vel")) return rc;
+        HIPCHK(amr::launch_pack_face(c, vel[d], c.M[0][d], d));
+    }
+    if (int rc = check_cell(ctx, rhs, 1, "rhs")) return rc;
+    int nb = 0;
+    HIPCHK(amr::launch_poisson_rhs(c, c.M[0][0], c.M[0][1], c.Mc[0], &nb));
+    HIPCHK(amr::launch_fin_prhs(c, nb));
+    HIPCHK(amr::launch_unpack_cell(c, c.Mc[0], *rhs, 0, 0));
+    if (int rc = poll(ctx)) return rc;
+    if (div_linf) *div_linf = c.ctl_host->div_star;
+    if (rhs_sum) *rhs_sum = c.ctl_host->rhs_sum;
+    return AMRESSIF_OK;
+}
+
+int amressif_poisson_solve(amressif_ctx *ctx, amr_array4 *sol, const amr_array4 *rhs, double tol_rel, double tol_abs,
+                           int *iters, double *res_rel)
+{
+    if (!ctx || !sol || !rhs) return fail(ctx, AMRESSIF_ERR_ARG, "null argument");
+    Ctx &c = ctx->c;
+    if (int rc = check_cell(ctx, rhs, 1, "rhs")) return rc;
+    if (int rc = check_cell(ctx, sol, 1, "sol")) return rc;
+    if (int rc = ensure_module_scratch(ctx)) return rc;
+    HIPCHK(amr::launch_ctl_set(c, c.in.momentum_tolerance, tol_rel, tol_abs, c.in.IterNum, c.in.mg_max_iter));
+    HIPCHK(amr::launch_pack_cell(c, *rhs, 0, c.b));
+    HIPCHK(amr::launch_mean_b(c, c.b));
+    HIPCHK(amr::launch_center_b(c, c.b));
+    if (int rc = poisson_loop(ctx)) return rc;
+    HIPCHK(amr::launch_unpack_cell_shift(c, c.phi, *sol, 0, 0, &c.ctl->phi_mean));
+    if (int rc = poll(ctx)) return rc;
+    if (iters) *iters = c.ctl_host->mg_iters;
+    if (res_rel) *res_rel = c.ctl_host->b_norm > 0 ? c.ctl_host->res_norm / c.ctl_host->b_norm : c.ctl_host->res_norm;
+    // restore the step tolerances
+    HIPCHK(amr::launch_ctl_set(c, c.in.momentum_tolerance, c.in.poisson_tol_rel, c.in.poisson_tol_abs, c.in.IterNum,
+                               c.in.mg_max_iter));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    return AMRESSIF_OK;
+}
+
+int amressif_project(amressif_ctx *ctx, const amr_array4 velStar[2], amr_array4 *userCtx, amr_array4 velCont[2],
+                     double *div_linf)
+{
+    if (!ctx || !velStar || !userCtx || !velCont) return fail(ctx, AMRESSIF_ERR_ARG, "null argument");
+    Ctx &c = ctx->c;
+    if (int rc = ensure_module_scratch(ctx)) return rc;
+    for (int d = 0; d < 2; ++d) {
+        if (int rc = check_face(ctx, &velStar[d], d, "velStar")) return rc;
+        if (int rc = check_face(ctx, &velCont[d], d, "velCont")) return rc;
+        HIPCHK(amr::launch_pack_face(c, velStar[d], c.M[0][d], d));
+    }
+    if (int rc = check_cell(ctx, userCtx, 2, "userCtx")) return rc;
+    HIPCHK(amr::launch_pack_cell(c, *userCtx, 0, c.Mc[0]));
+    HIPCHK(amr::launch_pack_cell(c, *userCtx, 1, c.Mc[1]));
+    int nb = 0;
+    HIPCHK(amr::launch_project_ex(c, c.M[0][0], c.M[0][1], c.M[1][0], c.M[1][1], c.Mc[0], c.Mc[1], 0, &nb));
+    HIPCHK(amr::launch_fin_project(c, nb, -1));
+    for (int d = 0; d < 2; ++d) HIPCHK(amr::launch_unpack_face(c, c.M[1][d], velCont[d], d));
+    HIPCHK(amr::launch_unpack_cell(c, c.Mc[0], *userCtx, 0, 1));
+    if (int rc = poll(ctx)) return rc;
+    if (div_linf) *div_linf = c.ctl_host->div_new;
+    return AMRESSIF_OK;
+}
+
+}  // extern "C"

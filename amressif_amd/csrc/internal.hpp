@@ -1,0 +1,126 @@
+// Internal types shared by the HIP kernels (kernels_*.hip) and the host
+// driver (api.cpp).  gfx950 only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/amressif.h"
+
+namespace amr {
+
+// Device-side control block.  Written only by single-block finalize kernels;
+// every guarded kernel reads its `done` flag first and returns if it is set,
+// so the host can enqueue work ahead and poll it once per batch.
+struct Ctl {
+    int mom_done;   // pseudo-iteration loop finished for this step
+    int mom_count;  // countIter
+    int flags;      // AMRESSIF_FLAG_*
+    int mg_done;    // Poisson solve converged (or capped)
+    int mg_iters;   // V-cycles done
+    int mg_started; // first residual evaluated
+    int pad0, pad1;
+    double norm_error;
+    double div_star, rhs_sum, b_mean;
+    double b_norm, res_norm, phi_sum, phi_mean;
+    double div_new;
+    double tol_mom, tol_rel, tol_abs;
+    int iter_num, mg_max_iter;
+};
+
+// Pitched 2D device arrays.  Fine level: x-faces rows j in [0,n), cols
+// i in [0,n]; y-faces rows [0,n], cols [0,n); cells rows/cols [0,n).  All
+// share one pitch (elements), a multiple of 64.
+struct Level {
+    int nx, ny, pitch;
+    double ax, ay;   // 1/dx^2, 1/dy^2
+    double *e, *r;   // FP64 correction and right-hand side
+    float *e32, *r32; // FP32 copies for the mixed V-cycle
+};
+
+struct Geometry {
+    int n, P;
+    double dx, dy, dt;
+    int bc_lo0, bc_hi0, bc_lo1, bc_hi1;
+    double inflow0, inflow1;
+    double ren;
+};
+
+constexpr int kMaxLevels = 24;
+constexpr int kMaxPartials = 1 << 16;
+
+struct Ctx {
+    amressif_inputs in{};
+    hipStream_t stream = nullptr;
+    Geometry g{};
+    double rk[4]{};
+    // faces: [0] = x-faces (u), [1] = y-faces (v)
+    double *U[2]{}, *Uprev[2]{}, *C[2]{}, *S1[2]{}, *S2[2]{};
+    double *p = nullptr, *phi = nullptr, *b = nullptr;
+    const double *lid = nullptr;  // caller-owned device lid profile or null
+    Level lev[kMaxLevels]{};
+    int nlev = 0;
+    double *pinv = nullptr;  // bottom pseudo-inverse, mb x mb
+    float *pinv32 = nullptr;
+    int mb = 0;
+    double *partials = nullptr;  // kMaxPartials * 4 doubles
+    Ctl *ctl = nullptr;
+    Ctl *ctl_host = nullptr;  // pinned
+    amressif_step_stats *log = nullptr;  // device step log
+    int log_cap = 0;
+    int last_P = 16, last_V = 8;
+    int64_t bytes = 0;
+    // scratch of the module-level entry points (allocated on first use)
+    double *M[4][2]{};
+    double *Mc[2]{};
+    double *dscalar = nullptr;
+    std::vector<void *> allocs;
+    std::string err;
+};
+
+// ---- kernel launchers (kernels_*.hip)
+struct StageLaunch {
+    const double *us, *vs;  // stencil input faces
+    const double *ub, *vb;  // base u^k
+    const double *cu, *cv;  // BDF2 constant (1.5/dt) u^n + (0.5/dt)(u^n - u^{n-1})
+    const double *p;        // pressure (cells)
+    double *uo, *vo;        // outputs
+    double rk;
+    int mode;               // 0 = stage update, 1 = stage update + norm partials, 2 = RHS only
+};
+hipError_t launch_stage(const Ctx &c, const StageLaunch &a, int *nblocks);
+hipError_t launch_fin_pseudo(const Ctx &c, int nblocks);
+hipError_t launch_ctl_reset_step(const Ctx &c);
+hipError_t launch_bdf2(const Ctx &c);
+hipError_t launch_poisson_rhs(const Ctx &c, const double *u, const double *v, double *b, int *nblocks);
+hipError_t launch_fin_prhs(const Ctx &c, int nblocks);
+hipError_t launch_center_b(const Ctx &c, double *b);
+hipError_t launch_mg_start(const Ctx &c);
+hipError_t launch_residual(const Ctx &c, const double *phi, const double *b, double *r, int *nblocks);
+hipError_t launch_fin_resid(const Ctx &c, int nblocks);
+hipError_t launch_vcycle(const Ctx &c, int mixed);
+hipError_t launch_add_correction(const Ctx &c, double *phi, int mixed);
+hipError_t launch_project(const Ctx &c, const double *us, const double *vs, double *uo, double *vo, int *nblocks);
+hipError_t launch_fin_project(const Ctx &c, int nblocks, int step_index);
+hipError_t launch_cont2cart_out(const Ctx &c, const double *u, const double *v, const amr_array4 &vc);
+hipError_t launch_pack_face(const Ctx &c, const amr_array4 &a, double *dst, int dir);
+hipError_t launch_unpack_face(const Ctx &c, const double *src, const amr_array4 &a, int dir);
+hipError_t launch_pack_cell(const Ctx &c, const amr_array4 &a, int comp, double *dst);
+hipError_t launch_unpack_cell(const Ctx &c, const double *src, const amr_array4 &a, int comp, int fill_ghosts);
+hipError_t launch_copy_faces(const Ctx &c, const double *src, double *dst, int dir);
+hipError_t launch_fill(const Ctx &c, double *dst, int64_t count, double value);
+hipError_t launch_norm(const Ctx &c, const double *ub, const double *vb, const double *uo, const double *vo,
+                       double *out_dev);
+hipError_t launch_ctl_set(const Ctx &c, double tol_mom, double tol_rel, double tol_abs, int iter_num,
+                          int mg_max_iter);
+hipError_t launch_project_ex(const Ctx &c, const double *us, const double *vs, double *uo, double *vo, double *p,
+                             const double *phi, int use_mean, int *nblocks);
+hipError_t launch_unpack_cell_shift(const Ctx &c, const double *src, const amr_array4 &a, int comp,
+                                    int fill_ghosts, const double *shift);
+hipError_t launch_mean_b(const Ctx &c, const double *b);
+hipError_t launch_bdf2_const(const Ctx &c, const double *un, const double *um, double *cc, int dir);
+
+}  // namespace amr

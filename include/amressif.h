@@ -1,0 +1,179 @@
+/*
+ * amressif.h -- C-ABI of the MI355X (gfx950) implementation of AMRESSIF's
+ * fractional-step time step (reference: milk-white-way/AMRESSIF,
+ * Source/src1_rever).  Plain C11, no exceptions cross the ABI, no torch types.
+ *
+ * The reference has no advance function: its step is inline in main_main
+ * (Source/src1_rever/main.cpp:875-1202) and its module functions take AMReX
+ * MultiFabs.  This header replaces those module functions and the step loop
+ * with entry points on AMReX Array4 views of one box (device pointers, ghosts
+ * included, i fastest, then j, then component), one box per GPU.
+ *
+ * Conventions
+ *  - The caller owns every field array (device memory).  A context owns its
+ *    working set (internal pitched copies, multigrid hierarchy, reductions).
+ *  - Entry points are ordered on the context's HIP stream.  Those that return
+ *    host scalars or statistics synchronise that stream.
+ *  - One context per stream/thread; no global state.
+ *  - Status 0 is success.  Negative codes replace the reference's amrex::Abort
+ *    and add explicit failure reports (non-finite norm, unsupported input).
+ */
+#ifndef AMRESSIF_H
+#define AMRESSIF_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AMRESSIF_OK 0
+#define AMRESSIF_ERR_ARG (-1)         /* bad argument / shape mismatch */
+#define AMRESSIF_ERR_BC (-2)          /* BC code the reference aborts on (main.cpp:822,832; utilities.cpp:128) */
+#define AMRESSIF_ERR_UNSUPPORTED (-3) /* valid in the reference but not built here (periodic 111, 3D, multi-box) */
+#define AMRESSIF_ERR_HIP (-4)         /* HIP runtime failure */
+#define AMRESSIF_ERR_NONFINITE (-5)   /* non-finite pseudo-iteration norm (reference exits silently) */
+#define AMRESSIF_ERR_PARSE (-6)       /* inputs file error */
+
+/* precision modes */
+#define AMRESSIF_FP64 0
+#define AMRESSIF_MIXED 1 /* FP32 stencils + FP32 V-cycle inside FP64 iterative refinement */
+
+/* step flags (amressif_step_stats.flags) */
+#define AMRESSIF_FLAG_CAP 1       /* countIter > IterNum (main.cpp:1030-1034) */
+#define AMRESSIF_FLAG_DIVERGED 2  /* normError > 1e2 (main.cpp:1035-1039) */
+#define AMRESSIF_FLAG_NONFINITE 4 /* NaN/Inf norm: the reference's while test exits silently (main.cpp:898) */
+#define AMRESSIF_FLAG_MG_CAP 8    /* Poisson iteration cap reached before tolerance */
+
+/* The reference's ParmParse keys (Source/src1_rever/main.cpp:89-157) plus the
+ * solver settings it hard-codes (poisson.cpp:159-182) and this build's
+ * precision switch.  Defaults are the reference's. */
+typedef struct {
+    int n_cell;                 /* required */
+    int max_grid_size[3];       /* required (x, y, z) */
+    double lo_phy_dim[3];       /* default 0 */
+    double hi_phy_dim[3];       /* default 1 */
+    int IterNum;                /* required */
+    int nsteps;                 /* default 1 */
+    double cfl;                 /* default 0.9 */
+    double fixed_dt;            /* default -1 (use cfl) */
+    double ren;                 /* required */
+    double vis;                 /* required, unused by the reference's arithmetic */
+    int phy_bc_lo[3];           /* default 0 (invalid) */
+    int phy_bc_hi[3];
+    double inflow_waveform[3];  /* default 0 */
+    int target_resolution;      /* default -1 */
+    double momentum_tolerance;  /* default 1e-10 */
+    int PSEUDO_TIMESTEPPING;    /* default 1 */
+    int plot_int;               /* default -1 */
+    int chk_int;                /* default -1 */
+    int txt_int;                /* default 0 */
+    int chk_out;                /* default 0 */
+    /* solver settings the reference hard-codes in poisson_advance */
+    double poisson_tol_rel;     /* 1e-10 (poisson.cpp:181) */
+    double poisson_tol_abs;     /* 0 (poisson.cpp:182) */
+    int mg_max_iter;            /* 2000 (poisson.cpp:159) */
+    int mg_bottom_max;          /* coarsen while the level is larger than this and even; 16 */
+    /* this build */
+    int precision;              /* AMRESSIF_FP64 or AMRESSIF_MIXED */
+} amressif_inputs;
+
+/* AMReX Array4 view of one box: ptr addresses element (lo[0], lo[1], lo[2], comp 0);
+ * strides are those of a contiguous FAB, jstride = hi0-lo0+1, kstride =
+ * jstride*(hi1-lo1+1), nstride = kstride*(hi2-lo2+1). */
+typedef struct {
+    void *ptr;
+    int lo[3];
+    int hi[3];
+    int ncomp;
+    int dtype; /* 0 = double, 1 = float */
+} amr_array4;
+
+typedef struct {
+    int pseudo_iters;   /* RK4 pseudo-iterations (countIter) */
+    int vcycles;        /* Poisson V-cycles */
+    int flags;          /* AMRESSIF_FLAG_* */
+    int pad;
+    double norm_error;  /* last pseudo-iteration norm (Error_Computation) */
+    double div_star;    /* ||D(u*)||inf (fn_rhs.cpp:103-104) */
+    double rhs_sum;     /* sum of the Poisson RHS (fn_rhs.cpp:123-124) */
+    double poisson_res; /* final ||r||inf / ||b||inf */
+    double div_new;     /* ||D(u^{n+1})||inf (main.cpp:1180) */
+} amressif_step_stats;
+
+typedef struct amressif_ctx amressif_ctx;
+
+/* ---------------------------------------------------------------- inputs */
+/* Fill the reference's defaults (main.cpp:89-157; required keys set to -1). */
+void amressif_inputs_default(amressif_inputs *in);
+/* ParmParse-style reader: "key = v1 v2 ..." lines, '#' comments, then
+ * "key=value" overrides from argv (replaces amrex::Initialize + ParmParse,
+ * main.cpp:89-157).  Returns AMRESSIF_ERR_PARSE with a message in err. */
+int amressif_parse_inputs(const char *path, int argc, const char *const *argv, amressif_inputs *out,
+                          char *err, int errlen);
+/* dt = cfl/(2 sum 1/dx^2) or fixed_dt (main.cpp:284-294) */
+double amressif_dt(const amressif_inputs *in);
+
+/* --------------------------------------------------------------- context */
+/* Validates the BC codes (main.cpp:810-835) and allocates the working set.
+ * stream: a hipStream_t (NULL = default stream). */
+int amressif_create(const amressif_inputs *in, void *hip_stream, amressif_ctx **out);
+int amressif_destroy(amressif_ctx *ctx);
+const char *amressif_last_error(const amressif_ctx *ctx);
+/* Device bytes held by the context. */
+int64_t amressif_workspace_bytes(const amressif_ctx *ctx);
+/* Per-column north-lid velocity (n_cell doubles, device pointer; NULL resets
+ * to the uniform inflow_waveform[0] of utilities.cpp:251-261). */
+int amressif_set_lid_profile(amressif_ctx *ctx, const double *dev_lid);
+
+/* ------------------------------------------------- state and time step */
+/* Copy the caller's state into the context: velCont[2] and velContPrev[2]
+ * (faces, 0 ghosts), userCtx (2 comps p, phi; >= 0 ghosts). */
+int amressif_load_state(amressif_ctx *ctx, const amr_array4 velCont[2], const amr_array4 velContPrev[2],
+                        const amr_array4 *userCtx);
+/* Copy the state out.  Any pointer may be NULL.  velCart (2 comps, 2 ghosts)
+ * is cont2cart(velCont) with the wall ghost rules; userCtx ghosts get the
+ * Neumann mirror (main.cpp:1078-1158). */
+int amressif_store_state(const amressif_ctx *ctx, amr_array4 velCont[2], amr_array4 velContPrev[2],
+                         amr_array4 *userCtx, amr_array4 *velCart);
+/* Advance nsteps time steps (main.cpp:875-1202 per step).  stats: host array
+ * of nsteps entries or NULL.  Synchronises the stream. */
+int amressif_advance(amressif_ctx *ctx, int nsteps, amressif_step_stats *stats);
+
+/* ------------------------------------------- module-level entry points
+ * Each runs the same kernels as amressif_advance on one call's inputs (Array4
+ * device views, copied into and out of the context's working layout). */
+
+/* cont2cart (utilities.cpp:16-350): face -> cell average + wall-BC ghost fill */
+int amressif_cont2cart(amressif_ctx *ctx, const amr_array4 velCont[2], amr_array4 *velCart);
+/* One stage's momentum RHS before the dual-time terms: gradient_calc_approach2
+ * (fn_flux_calc.cpp:825), convective_flux_calc_new_quick (:11), viscous_flux_calc
+ * (:695), flux wall ghosts (main.cpp:945-1014), momentum_righthand_side_calc
+ * (fn_rhs.cpp:8) -- one fused kernel.  rhs: 2 face arrays. */
+int amressif_momentum_rhs(amressif_ctx *ctx, const amr_array4 velStar[2], const amr_array4 *userCtx,
+                          amr_array4 rhs[2]);
+/* One RK4 dual-time stage (momentum.cpp:9-94 together with the above):
+ * velStar <- velCont + rk[sub] * (R(velStar) - (1.5/dt)(velStar - velContPrev)
+ * + (0.5/dt) velContDiff), where velContDiff = velContPrev - velContPrevPrev. */
+int amressif_rk_stage(amressif_ctx *ctx, int sub, amr_array4 velStar[2], const amr_array4 velCont[2],
+                      const amr_array4 velContPrev[2], const amr_array4 velContPrevPrev[2],
+                      const amr_array4 *userCtx);
+/* Error_Computation (utilities.cpp:421-493): weighted RMS of velCont - velStar */
+int amressif_pseudo_norm(amressif_ctx *ctx, const amr_array4 velCont[2], const amr_array4 velStar[2],
+                         double *norm);
+/* poisson_righthand_side_calc (fn_rhs.cpp:71-125): rhs = (1.5/dt) D(vel) */
+int amressif_poisson_rhs(amressif_ctx *ctx, const amr_array4 vel[2], amr_array4 *rhs, double *div_linf,
+                         double *rhs_sum);
+/* poisson_advance (poisson.cpp:41-186): L sol = rhs, homogeneous Neumann,
+ * ||r||inf <= max(tol_abs, tol_rel ||rhs - mean||inf); sol returned mean-zero. */
+int amressif_poisson_solve(amressif_ctx *ctx, amr_array4 *sol, const amr_array4 *rhs, double tol_rel,
+                           double tol_abs, int *iters, double *res_rel);
+/* update_solution (poisson.cpp:192-392) with phi from userCtx comp 1:
+ * velCont = velStar - G phi dt/1.5, p += phi.  Returns ||D(velCont)||inf. */
+int amressif_project(amressif_ctx *ctx, const amr_array4 velStar[2], amr_array4 *userCtx,
+                     amr_array4 velCont[2], double *div_linf);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,251 @@
+"""ctypes wrapper of the CPU FP64 oracle (oracle/librefcpu.so).
+
+TEST INFRASTRUCTURE ONLY.  This module is the checker: only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg import it.  The product
+package (amressif_amd) never does.
+
+Arrays follow the reference's single-box AMReX layout (Array4: i fastest, then
+j, then component), held as C-ordered numpy arrays:
+  x-face (n+1) x n          -> shape (n, n+1)
+  y-face n x (n+1)          -> shape (n+1, n)
+  velCart 2 comps 2 ghosts  -> shape (2, n+4, n+4)
+  userCtx 2 comps 1 ghost   -> shape (2, n+2, n+2)
+  fluxTotal 2 comps 1 ghost -> shape (2, n+2, n+2)
+  Poisson rhs/sol 1 ghost   -> shape (n+2, n+2)
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "librefcpu.so")
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int),
+        ("dx", ctypes.c_double),
+        ("dy", ctypes.c_double),
+        ("bc_lo", ctypes.c_int * 2),
+        ("bc_hi", ctypes.c_int * 2),
+        ("inflow", ctypes.c_double * 2),
+        ("lid", ctypes.POINTER(ctypes.c_double)),
+        ("ren", ctypes.c_double),
+        ("dt", ctypes.c_double),
+        ("mom_tol", ctypes.c_double),
+        ("iter_num", ctypes.c_int),
+        ("poisson_kind", ctypes.c_int),
+        ("poisson_tol_rel", ctypes.c_double),
+        ("mg_max_iter", ctypes.c_int),
+        ("mg_bottom_max", ctypes.c_int),
+        ("nthreads", ctypes.c_int),
+    ]
+
+
+class StepStats(ctypes.Structure):
+    _fields_ = [
+        ("pseudo_iters", ctypes.c_int),
+        ("vcycles", ctypes.c_int),
+        ("flags", ctypes.c_int),
+        ("pad", ctypes.c_int),
+        ("norm_error", ctypes.c_double),
+        ("div_star", ctypes.c_double),
+        ("rhs_sum", ctypes.c_double),
+        ("poisson_res", ctypes.c_double),
+        ("div_new", ctypes.c_double),
+    ]
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        _lib = ctypes.CDLL(_LIB)
+        d = ctypes.POINTER(ctypes.c_double)
+        P = ctypes.POINTER(Params)
+        _lib.rc_cont2cart.argtypes = [P, d, d, d]
+        _lib.rc_convective_flux.argtypes = [P, d, d, d, d, d, d, d, d, d]
+        _lib.rc_viscous_flux.argtypes = [P, d, d, d]
+        _lib.rc_gradient.argtypes = [P, d, d, d, d, d]
+        _lib.rc_flux_wall_ghosts.argtypes = [P, d]
+        _lib.rc_momentum_rhs.argtypes = [P, d, d, d, d, d]
+        _lib.rc_rk4_stage.argtypes = [P, ctypes.c_double, d, d, d, d, d, d, d, d, d, d, d]
+        _lib.rc_error_computation.argtypes = [P, d, d, d, d]
+        _lib.rc_error_computation.restype = ctypes.c_double
+        _lib.rc_poisson_rhs.argtypes = [P, d, d, d, d, d]
+        _lib.rc_poisson_solve.argtypes = [P, d, d, ctypes.POINTER(ctypes.c_int), d]
+        _lib.rc_userctx_ghosts.argtypes = [P, d]
+        _lib.rc_update_solution.argtypes = [P, d, d, d, d, d, d]
+        _lib.rc_advance.argtypes = [P, d, d, d, d, d, d, ctypes.c_int, ctypes.POINTER(StepStats)]
+        _lib.rc_apply_laplacian.argtypes = [P, d, d]
+        _lib.rc_pinv_neumann.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, d]
+    return _lib
+
+
+def _p(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Oracle:
+    """FP64 CPU restatement of one single-box AMRESSIF run."""
+
+    def __init__(self, n, ren, dt, dx=None, dy=None, bc_lo=(135, 135), bc_hi=(135, 165),
+                 inflow=(1.0, 0.0), mom_tol=1e-10, iter_num=100, poisson="mg",
+                 poisson_tol_rel=1e-10, mg_max_iter=200, mg_bottom_max=16, lid=None, nthreads=0):
+        self.n = n
+        self._lid = None if lid is None else np.ascontiguousarray(lid, dtype=np.float64)
+        prm = Params()
+        prm.n = n
+        prm.dx = 1.0 / n if dx is None else dx
+        prm.dy = 1.0 / n if dy is None else dy
+        prm.bc_lo[0], prm.bc_lo[1] = bc_lo
+        prm.bc_hi[0], prm.bc_hi[1] = bc_hi
+        prm.inflow[0], prm.inflow[1] = inflow
+        prm.lid = _p(self._lid) if self._lid is not None else ctypes.POINTER(ctypes.c_double)()
+        prm.ren = ren
+        prm.dt = dt
+        prm.mom_tol = mom_tol
+        prm.iter_num = iter_num
+        prm.poisson_kind = {"mg": 0, "dct": 1}[poisson]
+        prm.poisson_tol_rel = poisson_tol_rel
+        prm.mg_max_iter = mg_max_iter
+        prm.mg_bottom_max = mg_bottom_max
+        prm.nthreads = nthreads
+        self.prm = prm
+        self.L = lib()
+
+    # ---- allocation helpers
+    def xface(self):
+        return np.zeros((self.n, self.n + 1))
+
+    def yface(self):
+        return np.zeros((self.n + 1, self.n))
+
+    def cart(self):
+        return np.zeros((2, self.n + 4, self.n + 4))
+
+    def ctx(self):
+        return np.zeros((2, self.n + 2, self.n + 2))
+
+    # ---- module functions
+    def cont2cart(self, u, v, vc=None):
+        vc = self.cart() if vc is None else vc
+        rc = self.L.rc_cont2cart(ctypes.byref(self.prm), _p(vc), _p(u), _p(v))
+        if rc:
+            raise ValueError(f"cont2cart: unsupported boundary code (status {rc})")
+        return vc
+
+    def convective_flux(self, vc, u, v, ftot=None):
+        n = self.n
+        fxu, fxv, fyu, fyv = self.xface(), self.xface(), self.yface(), self.yface()
+        fconv = np.zeros((2, n, n))
+        ftot = self.ctx() if ftot is None else ftot
+        self.L.rc_convective_flux(ctypes.byref(self.prm), _p(vc), _p(u), _p(v), _p(fxu), _p(fxv),
+                                  _p(fyu), _p(fyv), _p(fconv), _p(ftot))
+        return fxu, fxv, fyu, fyv, fconv, ftot
+
+    def viscous_flux(self, vc, ftot=None):
+        n = self.n
+        fvisc = np.zeros((2, n, n))
+        ftot = self.ctx() if ftot is None else ftot
+        self.L.rc_viscous_flux(ctypes.byref(self.prm), _p(vc), _p(fvisc), _p(ftot))
+        return fvisc, ftot
+
+    def gradient(self, ctx):
+        gpx, gpy, gphix, gphiy = self.xface(), self.yface(), self.xface(), self.yface()
+        self.L.rc_gradient(ctypes.byref(self.prm), _p(ctx), _p(gpx), _p(gpy), _p(gphix), _p(gphiy))
+        return gpx, gpy, gphix, gphiy
+
+    def flux_wall_ghosts(self, ftot):
+        self.L.rc_flux_wall_ghosts(ctypes.byref(self.prm), _p(ftot))
+        return ftot
+
+    def momentum_rhs(self, ftot, gpx, gpy):
+        rx, ry = self.xface(), self.yface()
+        self.L.rc_momentum_rhs(ctypes.byref(self.prm), _p(ftot), _p(gpx), _p(gpy), _p(rx), _p(ry))
+        return rx, ry
+
+    def stage_rhs(self, u, v, ctx):
+        """momentum RHS R(u*) of one stage (main.cpp:933-1015), before the dual-time terms"""
+        vc = self.cont2cart(u, v)
+        gpx, gpy, _, _ = self.gradient(ctx)
+        ftot = self.ctx()
+        self.convective_flux(vc, u, v, ftot)
+        self.viscous_flux(vc, ftot)
+        self.flux_wall_ghosts(ftot)
+        return self.momentum_rhs(ftot, gpx, gpy)
+
+    def rk4_stage(self, rk_s, rhsx, rhsy, ustar, vstar, ucont, vcont, udiff, vdiff, uprev, vprev, vc=None):
+        vc = self.cart() if vc is None else vc
+        rc = self.L.rc_rk4_stage(ctypes.byref(self.prm), rk_s, _p(rhsx), _p(rhsy), _p(ustar), _p(vstar),
+                                 _p(ucont), _p(vcont), _p(udiff), _p(vdiff), _p(uprev), _p(vprev), _p(vc))
+        if rc:
+            raise ValueError(f"rk4_stage: status {rc}")
+        return vc
+
+    def error_computation(self, ucont, vcont, ustar, vstar):
+        return self.L.rc_error_computation(ctypes.byref(self.prm), _p(ucont), _p(vcont), _p(ustar), _p(vstar))
+
+    def poisson_rhs(self, u, v):
+        rhs = np.zeros((self.n + 2, self.n + 2))
+        linf, s = ctypes.c_double(), ctypes.c_double()
+        self.L.rc_poisson_rhs(ctypes.byref(self.prm), _p(u), _p(v), _p(rhs), ctypes.byref(linf), ctypes.byref(s))
+        return rhs, linf.value, s.value
+
+    def poisson_solve(self, rhs):
+        sol = np.zeros((self.n + 2, self.n + 2))
+        it, res = ctypes.c_int(), ctypes.c_double()
+        self.L.rc_poisson_solve(ctypes.byref(self.prm), _p(sol), _p(rhs), ctypes.byref(it), ctypes.byref(res))
+        return sol, it.value, res.value
+
+    def userctx_ghosts(self, ctx):
+        self.L.rc_userctx_ghosts(ctypes.byref(self.prm), _p(ctx))
+        return ctx
+
+    def update_solution(self, ctx, ustar, vstar):
+        u, v, vc = self.xface(), self.yface(), self.cart()
+        rc = self.L.rc_update_solution(ctypes.byref(self.prm), _p(ctx), _p(vc), _p(u), _p(v), _p(ustar), _p(vstar))
+        if rc:
+            raise ValueError(f"update_solution: status {rc}")
+        return u, v, vc
+
+    def apply_laplacian(self, phi):
+        out = np.zeros_like(phi)
+        self.L.rc_apply_laplacian(ctypes.byref(self.prm), _p(np.ascontiguousarray(phi)), _p(out))
+        return out
+
+    @staticmethod
+    def pinv_neumann(nx, ny, dx, dy):
+        out = np.zeros((nx * ny, nx * ny))
+        rc = lib().rc_pinv_neumann(nx, ny, dx, dy, _p(out))
+        assert rc == 0
+        return out
+
+    # ---- whole advance
+    def advance(self, state: dict, nsteps: int):
+        """state: dict with u, v, uprev, vprev (faces), ctx (userCtx); updated in place.
+        Returns a list of per-step stats dicts."""
+        st = (StepStats * max(nsteps, 1))()
+        vc = state.setdefault("vc", self.cart())
+        rc = self.L.rc_advance(ctypes.byref(self.prm), _p(state["u"]), _p(state["v"]), _p(state["uprev"]),
+                               _p(state["vprev"]), _p(state["ctx"]), _p(vc), nsteps, st)
+        if rc:
+            raise ValueError(f"advance: status {rc}")
+        out = []
+        for k in range(nsteps):
+            s = st[k]
+            out.append({f: getattr(s, f) for f, _ in StepStats._fields_ if f != "pad"})
+        return out

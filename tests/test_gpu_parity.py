@@ -1,0 +1,266 @@
+"""GPU parity of the HIP path (through the C-ABI) against the CPU FP64 oracle.
+
+Module-level entry points are compared one by one with the oracle's
+restatement of the reference functions; whole steps are compared after N
+steps.  Tolerances are written per test: relative to the reference field's
+max-norm, 1e-12 for single kernels (FMA contraction and operation order are
+the only differences), 1e-10 for velocity and mean-removed pressure after N
+steps (the parity bar).
+"""
+import numpy as np
+import pytest
+
+from amressif_amd import workloads as W
+from oracle.refcpu import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _api():
+    from amressif_amd import api
+    return api
+
+
+def rel(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    s = np.abs(b).max()
+    return float(np.abs(a - b).max() / (s if s > 0 else 1.0))
+
+
+def make_pair(n, ren, bc_lo=(135, 135), bc_hi=(135, 165), inflow=(1.0, 0.0), precision=0, poisson="mg",
+              poisson_tol=1e-10, lid=None, **kw):
+    api = _api()
+    inp = api.cavity_inputs(n, ren, precision=precision, phy_bc_lo=(bc_lo[0], bc_lo[1], 135),
+                            phy_bc_hi=(bc_hi[0], bc_hi[1], 135), inflow_waveform=(inflow[0], inflow[1], 0.0),
+                            poisson_tol_rel=poisson_tol, **kw)
+    dt = api.dt_of(inp)
+    o = Oracle(n, ren, dt, bc_lo=bc_lo, bc_hi=bc_hi, inflow=inflow, poisson=poisson,
+               poisson_tol_rel=1e-13 if poisson == "mg" else 1e-10, lid=lid,
+               iter_num=inp.IterNum, mom_tol=inp.momentum_tolerance)
+    return inp, o
+
+
+def cuda(a):
+    torch = _torch()
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda")
+
+
+def random_faces(n, rng, walls_zero=False):
+    u = rng.uniform(-1, 1, (n, n + 1))
+    v = rng.uniform(-1, 1, (n + 1, n))
+    if walls_zero:
+        u[:, 0] = u[:, -1] = 0.0
+        v[0, :] = v[-1, :] = 0.0
+    return u, v
+
+
+BCS = [((135, 135), (135, 165)), ((-135, -135), (135, -135)), ((165, 135), (165, 165)), ((135, -135), (135, 135))]
+
+
+@pytest.mark.parametrize("n", [64, 100])
+@pytest.mark.parametrize("bc", BCS)
+def test_cont2cart_parity(n, bc):
+    rng = np.random.default_rng(1)
+    inp, o = make_pair(n, 100.0, bc_lo=bc[0], bc_hi=bc[1], inflow=(0.7, -0.2))
+    api = _api()
+    ctx = api.Context(inp)
+    u, v = random_faces(n, rng)
+    vc_ref = o.cont2cart(u, v)
+    vc = ctx.cont2cart(cuda(u), cuda(v)).cpu().numpy()
+    assert rel(vc, vc_ref) == 0.0  # same arithmetic, corners included
+
+
+@pytest.mark.parametrize("n", [64, 100, 120])
+@pytest.mark.parametrize("bc", BCS[:3])
+def test_momentum_rhs_parity(n, bc):
+    rng = np.random.default_rng(2)
+    inp, o = make_pair(n, 100.0, bc_lo=bc[0], bc_hi=bc[1])
+    ctx = _api().Context(inp)
+    u, v = random_faces(n, rng)
+    uc = np.zeros((2, n + 2, n + 2))
+    uc[:, 1:-1, 1:-1] = rng.uniform(-1, 1, (2, n, n))
+    o.userctx_ghosts(uc)
+    rx_ref, ry_ref = o.stage_rhs(u, v, uc)
+    rx, ry = ctx.momentum_rhs(cuda(u), cuda(v), cuda(uc))
+    assert rel(rx.cpu().numpy(), rx_ref) < 1e-12
+    assert rel(ry.cpu().numpy(), ry_ref) < 1e-12
+    # wall-normal faces: R = 0 exactly
+    assert np.all(rx.cpu().numpy()[:, [0, -1]] == 0.0)
+    assert np.all(ry.cpu().numpy()[[0, -1], :] == 0.0)
+
+
+@pytest.mark.parametrize("sub", [0, 1, 2, 3])
+def test_rk_stage_parity(sub):
+    n = 96
+    rng = np.random.default_rng(3 + sub)
+    inp, o = make_pair(n, 400.0)
+    ctx = _api().Context(inp)
+    us, vs = random_faces(n, rng, walls_zero=True)
+    uk, vk = random_faces(n, rng, walls_zero=True)
+    un, vn = random_faces(n, rng, walls_zero=True)
+    um, vm = random_faces(n, rng, walls_zero=True)
+    uc = np.zeros((2, n + 2, n + 2))
+    uc[0, 1:-1, 1:-1] = rng.uniform(-1, 1, (n, n))
+    o.userctx_ghosts(uc)
+    # oracle: stage RHS then the dual-time update (momentum.cpp:68-87)
+    rx, ry = o.stage_rhs(us, vs, uc)
+    ust, vst = us.copy(), vs.copy()
+    rk = W.rk_coefficients(o.prm.dt)
+    o.rk4_stage(rk[sub], rx, ry, ust, vst, uk, vk, un - um, vn - vm, un, vn)
+    tu, tv = cuda(us), cuda(vs)
+    ctx.runge_kutta4_pseudo_time_stepping(sub, tu, tv, cuda(uk), cuda(vk), cuda(un), cuda(vn), cuda(um), cuda(vm),
+                                          cuda(uc))
+    assert rel(tu.cpu().numpy(), ust) < 1e-12
+    assert rel(tv.cpu().numpy(), vst) < 1e-12
+
+
+def test_pseudo_norm_parity():
+    n = 128
+    rng = np.random.default_rng(7)
+    inp, o = make_pair(n, 100.0)
+    ctx = _api().Context(inp)
+    a, b = random_faces(n, rng)
+    c, d = random_faces(n, rng)
+    ref = o.error_computation(a, b, c, d)
+    got = ctx.Error_Computation(cuda(a), cuda(b), cuda(c), cuda(d))
+    assert abs(got - ref) <= 1e-14 * ref
+
+
+def test_poisson_rhs_parity():
+    n = 100
+    rng = np.random.default_rng(8)
+    inp, o = make_pair(n, 100.0)
+    ctx = _api().Context(inp)
+    u, v = random_faces(n, rng)
+    rhs_ref, linf_ref, sum_ref = o.poisson_rhs(u, v)
+    rhs, linf, s = ctx.poisson_righthand_side_calc(cuda(u), cuda(v))
+    assert rel(rhs.cpu().numpy()[1:-1, 1:-1], rhs_ref[1:-1, 1:-1]) == 0.0
+    assert linf == linf_ref
+    assert abs(s - sum_ref) <= 1e-12 * np.abs(rhs_ref).sum()
+
+
+@pytest.mark.parametrize("n", [8, 64, 100, 256])
+def test_poisson_solve_vs_exact(n):
+    rng = np.random.default_rng(9)
+    inp, o = make_pair(n, 100.0, poisson="dct")
+    ctx = _api().Context(inp)
+    b = np.zeros((n + 2, n + 2))
+    b[1:-1, 1:-1] = rng.standard_normal((n, n))
+    sol_ref, _, _ = o.poisson_solve(b)  # exact (DCT), mean-zero
+    sol, iters, res = ctx.poisson_advance(cuda(b), tol_rel=1e-12)
+    assert res <= 1e-12
+    assert iters <= 30
+    assert rel(sol.cpu().numpy()[1:-1, 1:-1], sol_ref[1:-1, 1:-1]) < 1e-9
+
+
+def test_project_parity():
+    n = 96
+    rng = np.random.default_rng(10)
+    inp, o = make_pair(n, 100.0)
+    ctx = _api().Context(inp)
+    us, vs = random_faces(n, rng, walls_zero=True)
+    uc = np.zeros((2, n + 2, n + 2))
+    uc[:, 1:-1, 1:-1] = rng.uniform(-1, 1, (2, n, n))
+    o.userctx_ghosts(uc)
+    uc_ref = uc.copy()
+    u_ref, v_ref, _ = o.update_solution(uc_ref, us, vs)
+    tuc = cuda(uc)
+    u, v, d = ctx.update_solution(cuda(us), cuda(vs), tuc)
+    assert rel(u.cpu().numpy(), u_ref) < 1e-14
+    assert rel(v.cpu().numpy(), v_ref) < 1e-14
+    assert rel(tuc.cpu().numpy()[0], uc_ref[0]) < 1e-15
+
+
+def run_both(n, ren, steps, init="zero", **kw):
+    api = _api()
+    inp, o = make_pair(n, ren, **kw)
+    st = W.initial_state(n, init)
+    ref = {k: v.copy() for k, v in st.items()}
+    ref_stats = o.advance(ref, steps)
+    run = api.Run.from_numpy(inp, st)
+    if kw.get("lid") is not None:
+        run.ctx.set_lid_profile(cuda(kw["lid"]))
+    stats = run.advance(steps)
+    got = run.fetch()
+    return ref, ref_stats, got, stats
+
+
+def check_step_parity(ref, ref_stats, got, stats, tol=1e-10):
+    assert rel(got["u"], ref["u"]) < tol
+    assert rel(got["v"], ref["v"]) < tol
+    p, pr = got["ctx"][0, 1:-1, 1:-1], ref["ctx"][0, 1:-1, 1:-1]
+    assert rel(p - p.mean(), pr - pr.mean()) < tol
+    assert rel(got["uprev"], ref["uprev"]) < tol
+    for s, r in zip(stats, ref_stats):
+        assert s["pseudo_iters"] == r["pseudo_iters"]
+        assert s["flags"] == 0
+        assert s["div_new"] <= 1e-10 * s["div_star"] + 1e-12
+    # wall-normal faces stay exactly zero
+    assert np.all(got["u"][:, [0, -1]] == 0.0) and np.all(got["v"][[0, -1], :] == 0.0)
+
+
+def test_advance_cavity128_re100_zero_start():
+    ref, rs, got, st = run_both(128, 100.0, 20)
+    check_step_parity(ref, rs, got, st)
+    assert rel(got["vc"][:, 2:-2, 2:-2], ref["vc"][:, 2:-2, 2:-2]) < 1e-10
+
+
+def test_advance_seeded_256_re1000():
+    ref, rs, got, st = run_both(256, 1000.0, 6, init="seeded")
+    check_step_parity(ref, rs, got, st)
+
+
+@pytest.mark.parametrize("bc", BCS[1:3])
+def test_advance_other_walls(bc):
+    ref, rs, got, st = run_both(64, 200.0, 5, init="seeded", bc_lo=bc[0], bc_hi=bc[1])
+    assert rel(got["u"], ref["u"]) < 1e-10
+    assert rel(got["v"], ref["v"]) < 1e-10
+    for s, r in zip(st, rs):
+        assert s["pseudo_iters"] == r["pseudo_iters"]
+
+
+def test_advance_lid_profile():
+    n = 64
+    lid = W.lid_profile(n)
+    ref, rs, got, st = run_both(n, 500.0, 5, init="seeded", lid=lid)
+    check_step_parity(ref, rs, got, st)
+
+
+@pytest.mark.parametrize("n", [8, 100])
+def test_advance_odd_sizes(n):
+    ref, rs, got, st = run_both(n, 100.0, 4, init="seeded")
+    check_step_parity(ref, rs, got, st)
+
+
+def test_unsupported_multigrid_size_reported():
+    api = _api()
+    with pytest.raises(api.AmressifError) as e:
+        api.Context(api.cavity_inputs(130, 100.0))  # coarsens to an odd 65 x 65 bottom level
+    assert e.value.code == -3
+
+
+def test_iternum_cap_from_host():
+    api = _api()
+    n = 32
+    inp = api.cavity_inputs(n, 100.0, IterNum=0)
+    run = api.Run.from_numpy(inp, W.initial_state(n, "seeded"))
+    st = run.advance(2)
+    assert all(s["pseudo_iters"] == 1 and (s["flags"] & api.FLAG_CAP) for s in st)
+
+
+def test_nonfinite_reported():
+    api = _api()
+    n = 32
+    inp = api.cavity_inputs(n, 100.0)
+    state = W.initial_state(n, "seeded")
+    state["u"][5, 7] = np.nan
+    run = api.Run.from_numpy(inp, state)
+    with pytest.raises(api.AmressifError) as e:
+        run.advance(3)
+    assert e.value.code == -5

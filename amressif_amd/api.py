@@ -117,6 +117,8 @@ _sig = {
     "amressif_poisson_solve": (_c.c_int, [_c.c_void_p, _A4, _A4, _c.c_double, _c.c_double, _P(_c.c_int),
                                           _P(_c.c_double)]),
     "amressif_project": (_c.c_int, [_c.c_void_p, _A4, _A4, _A4, _P(_c.c_double)]),
+    "amressif_time_kernel": (_c.c_int, [_c.c_void_p, _c.c_int, _c.c_int, _c.c_int64, _P(_c.c_double),
+                                        _P(_c.c_double)]),
 }
 for _name, (_res, _args) in _sig.items():
     _f = getattr(_lib, _name)
@@ -275,6 +277,12 @@ class Context:
         st = (StepStats * max(1, nsteps))() if with_stats else None
         self._check(_lib.amressif_advance(self._h, nsteps, st))
         return [st[k].as_dict() for k in range(nsteps)] if with_stats else None
+
+    def time_kernel(self, which: int, reps: int = 20, copy_bytes: int = 0):
+        """(average ms per launch, algorithmic bytes per launch) of one kernel, HIP events on this stream"""
+        ms, b = ctypes.c_double(), ctypes.c_double()
+        self._check(_lib.amressif_time_kernel(self._h, which, reps, copy_bytes, ctypes.byref(ms), ctypes.byref(b)))
+        return ms.value, b.value
 
     # ---- module functions (reference names)
     def cont2cart(self, u, v, velCart=None):

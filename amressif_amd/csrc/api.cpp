@@ -217,6 +217,48 @@ int poisson_loop(amressif_ctx *ctx)
     return 0;
 }
 
+// One RK4 pseudo-iteration (four guarded stage launches).  FP64: stencil
+// input ping-pongs U -> S1 -> S2 -> S1 and the last stage updates U (the
+// pseudo base u^k) in place.  Mixed: increments ping-pong dA/dB and the FP32
+// u^k ping-pongs by the host-side iteration index (a guarded no-op iteration
+// writes nothing, so executed iterations always follow that order).
+int launch_pseudo_iteration(amressif_ctx *ctx, int it, int *nb)
+{
+    Ctx &c = ctx->c;
+    if (c.in.precision == AMRESSIF_MIXED) {
+        const int par = it & 1;
+        amr::MixedStageLaunch a{};
+        a.uk = c.uk32[par][0]; a.vk = c.uk32[par][1];
+        a.wu = c.w32[0]; a.wv = c.w32[1];
+        a.su = c.s32[0]; a.sv = c.s32[1];
+        a.ukn = c.uk32[par ^ 1][0]; a.vkn = c.uk32[par ^ 1][1];
+        const float *din_u[4] = {nullptr, c.dA32[0], c.dB32[0], c.dA32[0]};
+        const float *din_v[4] = {nullptr, c.dA32[1], c.dB32[1], c.dA32[1]};
+        float *dout_u[4] = {c.dA32[0], c.dB32[0], c.dA32[0], c.dB32[0]};
+        float *dout_v[4] = {c.dA32[1], c.dB32[1], c.dA32[1], c.dB32[1]};
+        for (int sub = 0; sub < 4; ++sub) {
+            a.ud = din_u[sub]; a.vd = din_v[sub]; a.dou = dout_u[sub]; a.dov = dout_v[sub];
+            a.rk = c.rk[sub];
+            a.mode = sub == 3 ? 1 : 0;
+            HIPCHK(amr::launch_stage_mixed(c, a, nb));
+        }
+        return 0;
+    }
+    amr::StageLaunch a{};
+    a.ub = c.U[0]; a.vb = c.U[1]; a.cu = c.C[0]; a.cv = c.C[1]; a.p = c.p;
+    const double *in_u[4] = {c.U[0], c.S1[0], c.S2[0], c.S1[0]};
+    const double *in_v[4] = {c.U[1], c.S1[1], c.S2[1], c.S1[1]};
+    double *out_u[4] = {c.S1[0], c.S2[0], c.S1[0], c.U[0]};
+    double *out_v[4] = {c.S1[1], c.S2[1], c.S1[1], c.U[1]};
+    for (int sub = 0; sub < 4; ++sub) {
+        a.us = in_u[sub]; a.vs = in_v[sub]; a.uo = out_u[sub]; a.vo = out_v[sub];
+        a.rk = c.rk[sub];
+        a.mode = sub == 3 ? 1 : 0;
+        HIPCHK(amr::launch_stage(c, a, nb));
+    }
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -413,6 +455,11 @@ int amressif_create(const amressif_inputs *in, void *hip_stream, amressif_ctx **
         ALLOC(&c.S1[d], fsz); ALLOC(&c.S2[d], fsz);
     }
     ALLOC(&c.p, fsz); ALLOC(&c.phi, fsz); ALLOC(&c.b, fsz);
+    if (in->precision == AMRESSIF_MIXED)
+        for (int d = 0; d < 2; ++d) {
+            ALLOC(&c.uk32[0][d], fsz); ALLOC(&c.uk32[1][d], fsz); ALLOC(&c.w32[d], fsz);
+            ALLOC(&c.dA32[d], fsz); ALLOC(&c.dB32[d], fsz); ALLOC(&c.s32[d], fsz);
+        }
     // multigrid hierarchy (level 0 shares the fine pitch)
     {
         int nx = n, ny = n;
@@ -556,26 +603,17 @@ int amressif_advance(amressif_ctx *ctx, int nsteps, amressif_step_stats *stats)
     HIPCHK(amr::launch_ctl_set(c, c.in.momentum_tolerance, c.in.poisson_tol_rel, c.in.poisson_tol_abs, c.in.IterNum,
                                c.in.mg_max_iter));
     int status = AMRESSIF_OK;
+    const bool mixed = c.in.precision == AMRESSIF_MIXED;
     for (int s = 0; s < nsteps; ++s) {
         HIPCHK(amr::launch_ctl_reset_step(c));
-        HIPCHK(amr::launch_bdf2(c));
+        if (mixed) HIPCHK(amr::launch_bdf2_mixed(c));
+        else HIPCHK(amr::launch_bdf2(c));
         // ---- momentum: RK4 dual-time pseudo-iterations (main.cpp:898-1040)
         int launched = 0, target = std::max(1, c.last_P), nb = 0;
         const int cap = c.in.IterNum + 1;
         for (;;) {
             while (launched < target && launched < cap) {
-                amr::StageLaunch a{};
-                a.ub = c.U[0]; a.vb = c.U[1]; a.cu = c.C[0]; a.cv = c.C[1]; a.p = c.p;
-                const double *in_u[4] = {c.U[0], c.S1[0], c.S2[0], c.S1[0]};
-                const double *in_v[4] = {c.U[1], c.S1[1], c.S2[1], c.S1[1]};
-                double *out_u[4] = {c.S1[0], c.S2[0], c.S1[0], c.U[0]};
-                double *out_v[4] = {c.S1[1], c.S2[1], c.S1[1], c.U[1]};
-                for (int sub = 0; sub < 4; ++sub) {
-                    a.us = in_u[sub]; a.vs = in_v[sub]; a.uo = out_u[sub]; a.vo = out_v[sub];
-                    a.rk = c.rk[sub];
-                    a.mode = sub == 3 ? 1 : 0;
-                    HIPCHK(amr::launch_stage(c, a, &nb));
-                }
+                if (int rc = launch_pseudo_iteration(ctx, launched, &nb)) return rc;
                 HIPCHK(amr::launch_fin_pseudo(c, nb));
                 ++launched;
             }
@@ -586,11 +624,13 @@ int amressif_advance(amressif_ctx *ctx, int nsteps, amressif_step_stats *stats)
         c.last_P = std::max(1, c.ctl_host->mom_count);
         const int flags = c.ctl_host->flags;
         // ---- projection: Poisson RHS, MLMG-equivalent solve, update
-        HIPCHK(amr::launch_poisson_rhs(c, c.U[0], c.U[1], c.b, &nb));
+        // mixed: u* = u^n + S (FP64 + FP32 accumulated change)
+        const float *au = mixed ? c.s32[0] : nullptr, *av = mixed ? c.s32[1] : nullptr;
+        HIPCHK(amr::launch_poisson_rhs(c, c.U[0], c.U[1], c.b, &nb, au, av));
         HIPCHK(amr::launch_fin_prhs(c, nb));
         HIPCHK(amr::launch_center_b(c, c.b));
         if (int rc = poisson_loop(ctx)) return rc;
-        HIPCHK(amr::launch_project(c, c.U[0], c.U[1], c.S1[0], c.S1[1], &nb));
+        HIPCHK(amr::launch_project(c, c.U[0], c.U[1], c.S1[0], c.S1[1], &nb, au, av));
         HIPCHK(amr::launch_fin_project(c, nb, s));
         std::swap(c.U[0], c.S1[0]);
         std::swap(c.U[1], c.S1[1]);
@@ -766,6 +806,72 @@ int amressif_project(amressif_ctx *ctx, const amr_array4 velStar[2], amr_array4 
     HIPCHK(amr::launch_unpack_cell(c, c.Mc[0], *userCtx, 0, 1));
     if (int rc = poll(ctx)) return rc;
     if (div_linf) *div_linf = c.ctl_host->div_new;
+    return AMRESSIF_OK;
+}
+
+int amressif_time_kernel(amressif_ctx *ctx, int which, int reps, int64_t copy_bytes, double *avg_ms,
+                         double *bytes_per_launch)
+{
+    if (!ctx || !avg_ms || reps < 1) return fail(ctx, AMRESSIF_ERR_ARG, "bad argument");
+    Ctx &c = ctx->c;
+    const double cells = (double)c.g.n * (double)c.g.n;
+    void *src = nullptr, *dst = nullptr;
+    if (which == 2) {
+        if (copy_bytes < 16) return fail(ctx, AMRESSIF_ERR_ARG, "copy_bytes must be >= 16");
+        HIPCHK(hipMalloc(&src, (size_t)copy_bytes));
+        HIPCHK(hipMalloc(&dst, (size_t)copy_bytes));
+        HIPCHK(hipMemsetAsync(src, 0, (size_t)copy_bytes, c.stream));
+    } else if (which != 0 && which != 1) {
+        return fail(ctx, AMRESSIF_ERR_ARG, "which must be 0, 1 or 2");
+    }
+    auto launch = [&]() -> hipError_t {
+        if (which == 0 && c.in.precision == AMRESSIF_MIXED) {
+            amr::MixedStageLaunch a{};  // a middle stage: reads u^k, d, w; writes d
+            a.uk = c.uk32[0][0]; a.vk = c.uk32[0][1]; a.ud = c.dA32[0]; a.vd = c.dA32[1];
+            a.dou = c.dB32[0]; a.dov = c.dB32[1]; a.wu = c.w32[0]; a.wv = c.w32[1];
+            a.rk = c.rk[1];
+            a.mode = 0;
+            a.unguarded = 1;
+            int nb = 0;
+            return amr::launch_stage_mixed(c, a, &nb);
+        }
+        if (which == 0) {
+            amr::StageLaunch a{};
+            a.us = c.U[0]; a.vs = c.U[1]; a.ub = c.U[0]; a.vb = c.U[1];
+            a.cu = c.C[0]; a.cv = c.C[1]; a.p = c.p;
+            a.uo = c.S2[0]; a.vo = c.S2[1];
+            a.rk = c.rk[0];
+            a.mode = 0;
+            a.unguarded = 1;
+            int nb = 0;
+            return amr::launch_stage(c, a, &nb);
+        }
+        if (which == 1) return amr::launch_gsrb_fine(c, 0);
+        return amr::launch_copy_bytes(c, src, dst, copy_bytes);
+    };
+    HIPCHK(launch());  // warm-up
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, c.stream));
+    for (int r = 0; r < reps; ++r) HIPCHK(launch());
+    HIPCHK(hipEventRecord(e1, c.stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (src) hipFree(src);
+    if (dst) hipFree(dst);
+    *avg_ms = (double)ms / reps;
+    if (bytes_per_launch) {
+        // stage: reads u*, v*, u^k, v^k, c_u, c_v, p and writes u*, v* (9 arrays x 8 B per cell);
+        // colour sweep: half the cells read e(5 pts, counted once), r and write e -> 12 B per cell;
+        // copy: read + write
+        // mixed middle stage: reads u^k, d, w and writes d in FP32 (4 arrays x 2 dirs x 4 B per cell)
+        const double stage = c.in.precision == AMRESSIF_MIXED ? 32.0 : 72.0;
+        *bytes_per_launch = which == 0 ? stage * cells : which == 1 ? 12.0 * cells : 2.0 * (double)copy_bytes;
+    }
     return AMRESSIF_OK;
 }
 

@@ -1,0 +1,63 @@
+// Device helpers shared by the kernel files (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace amr {
+
+// Deterministic block reduction: fixed wave-shuffle tree, then the wave
+// partials combined in order by thread 0.  Block size a multiple of 64; every
+// thread of the block must call it.  The result is valid in thread 0.
+template <bool IsMax>
+__device__ inline double block_reduce(double v, double *sh)
+{
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int o = 32; o > 0; o >>= 1) {
+        const double w = __shfl_down(v, o, 64);
+        if (IsMax) v = (w > v || w != w) ? w : v;
+        else v += w;
+    }
+    __syncthreads();
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) {
+        r = sh[0];
+        for (int w = 1; w < nw; ++w) {
+            if (IsMax) r = (sh[w] > r || sh[w] != sh[w]) ? sh[w] : r;
+            else r += sh[w];
+        }
+    }
+    return r;
+}
+
+// QUICK half-node flux of one face (fn_flux_calc.cpp:66-73): vel is the face
+// velocity, a2 = U(i-2), a1 = U(i-1), b0 = U(i), b1 = U(i+1) along the face
+// normal.  ud = 0.5 vel + |0.5 vel| = max(vel, 0), up = min(vel, 0).
+template <class R>
+__device__ __forceinline__ R quick_flux(R vel, R a2, R a1, R b0, R b1)
+{
+    const R con = R(0.5) * vel;
+    const R ud = con + fabs(con), up = con - fabs(con);
+    return up * ((R(1) / R(8)) * (-b1 - R(2) * b0 + R(3) * a1) + b0) +
+           ud * ((R(1) / R(8)) * (-a2 - R(2) * a1 + R(3) * b0) + a1);
+}
+
+// Wall ghost rule of cont2cart (utilities.cpp:87-269) for one non-corner ghost
+// cell from its mirror cell's values.  side: 0 W, 1 E, 2 S, 3 N.
+template <class R>
+__device__ __forceinline__ void ghost_rule(int side, int code, R um, R vm, R gx, R gy, R &U, R &V)
+{
+    if (code == 135) { U = -um; V = -vm; return; }
+    if (side == 0) {  // W: -135 slip, +-165 unit inflow
+        if (code == -135) { U = -um; V = vm; } else { U = R(2) - um; V = R(2) - vm; }
+    } else if (side == 1) {  // E: the slip branch tests the code 1; 165 unit inflow
+        if (code == 1) { U = -um; V = vm; } else { U = R(2) - um; V = R(2) - vm; }
+    } else if (side == 2) {  // S
+        if (code == -135) { U = um; V = -vm; } else { U = R(2) - um; V = R(2) - vm; }
+    } else {  // N: 165 is the lid, 2 * inflow - mirror
+        if (code == -135) { U = um; V = -vm; } else { U = R(2) * gx - um; V = R(2) * gy - vm; }
+    }
+}
+
+}  // namespace amr

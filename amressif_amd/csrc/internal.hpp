@@ -59,6 +59,9 @@ struct Ctx {
     double rk[4]{};
     // faces: [0] = x-faces (u), [1] = y-faces (v)
     double *U[2]{}, *Uprev[2]{}, *C[2]{}, *S1[2]{}, *S2[2]{};
+    // mixed-precision momentum state (FP32 faces): u^k ping-pong, w, stage
+    // increments and the step's accumulated change
+    float *uk32[2][2]{}, *w32[2]{}, *dA32[2]{}, *dB32[2]{}, *s32[2]{};
     double *p = nullptr, *phi = nullptr, *b = nullptr;
     const double *lid = nullptr;  // caller-owned device lid profile or null
     Level lev[kMaxLevels]{};
@@ -90,12 +93,28 @@ struct StageLaunch {
     double *uo, *vo;        // outputs
     double rk;
     int mode;               // 0 = stage update, 1 = stage update + norm partials, 2 = RHS only
+    int unguarded;          // 1: ignore the control block (diagnostic timing)
 };
 hipError_t launch_stage(const Ctx &c, const StageLaunch &a, int *nblocks);
+// mixed stage on the increment d = u* - u^k (FP32): see kernels_stage.hip
+struct MixedStageLaunch {
+    const float *uk, *vk;    // u^k in FP32 (stencil base)
+    const float *ud, *vd;    // previous stage's increment (null on stage 1)
+    float *dou, *dov;        // increment out
+    float *wu, *wv;          // w = c - G p - (1.5/dt) u^k (last stage updates it)
+    float *su, *sv;          // accumulated change of the step (last stage adds d)
+    float *ukn, *vkn;        // last stage: u^{k+1} in FP32
+    double rk;
+    int mode;                // 0 stage, 1 last stage (+ norm partials)
+    int unguarded;
+};
+hipError_t launch_stage_mixed(const Ctx &c, const MixedStageLaunch &a, int *nblocks);
+hipError_t launch_bdf2_mixed(const Ctx &c);
 hipError_t launch_fin_pseudo(const Ctx &c, int nblocks);
 hipError_t launch_ctl_reset_step(const Ctx &c);
 hipError_t launch_bdf2(const Ctx &c);
-hipError_t launch_poisson_rhs(const Ctx &c, const double *u, const double *v, double *b, int *nblocks);
+hipError_t launch_poisson_rhs(const Ctx &c, const double *u, const double *v, double *b, int *nblocks,
+                              const float *au = nullptr, const float *av = nullptr);
 hipError_t launch_fin_prhs(const Ctx &c, int nblocks);
 hipError_t launch_center_b(const Ctx &c, double *b);
 hipError_t launch_mg_start(const Ctx &c);
@@ -103,7 +122,8 @@ hipError_t launch_residual(const Ctx &c, const double *phi, const double *b, dou
 hipError_t launch_fin_resid(const Ctx &c, int nblocks);
 hipError_t launch_vcycle(const Ctx &c, int mixed);
 hipError_t launch_add_correction(const Ctx &c, double *phi, int mixed);
-hipError_t launch_project(const Ctx &c, const double *us, const double *vs, double *uo, double *vo, int *nblocks);
+hipError_t launch_project(const Ctx &c, const double *us, const double *vs, double *uo, double *vo, int *nblocks,
+                          const float *au = nullptr, const float *av = nullptr);
 hipError_t launch_fin_project(const Ctx &c, int nblocks, int step_index);
 hipError_t launch_cont2cart_out(const Ctx &c, const double *u, const double *v, const amr_array4 &vc);
 hipError_t launch_pack_face(const Ctx &c, const amr_array4 &a, double *dst, int dir);
@@ -117,9 +137,12 @@ hipError_t launch_norm(const Ctx &c, const double *ub, const double *vb, const d
 hipError_t launch_ctl_set(const Ctx &c, double tol_mom, double tol_rel, double tol_abs, int iter_num,
                           int mg_max_iter);
 hipError_t launch_project_ex(const Ctx &c, const double *us, const double *vs, double *uo, double *vo, double *p,
-                             const double *phi, int use_mean, int *nblocks);
+                             const double *phi, int use_mean, int *nblocks, const float *au = nullptr,
+                             const float *av = nullptr);
 hipError_t launch_unpack_cell_shift(const Ctx &c, const double *src, const amr_array4 &a, int comp,
                                     int fill_ghosts, const double *shift);
+hipError_t launch_copy_bytes(const Ctx &c, const void *src, void *dst, int64_t bytes);
+hipError_t launch_gsrb_fine(const Ctx &c, int color);
 hipError_t launch_mean_b(const Ctx &c, const double *b);
 hipError_t launch_bdf2_const(const Ctx &c, const double *un, const double *um, double *cc, int dir);
 

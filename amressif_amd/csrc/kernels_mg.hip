@@ -166,6 +166,31 @@ static hipError_t vcycle_impl(const Ctx &c, T *const *E, T *const *R, const T *p
     return hipGetLastError();
 }
 
+// diagnostic: one unguarded colour sweep on the finest level (FP64 arrays)
+__global__ __launch_bounds__(256) void k_gsrb_free(double *e, const double *r, int nx, int ny, int pitch, double ax,
+                                                   double ay, int color)
+{
+    const int t = blockIdx.x * 64 + (threadIdx.x & 63), j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int i = 2 * t + ((j + color) & 1);
+    if (i >= nx || j >= ny) return;
+    const size_t k = i + (size_t)pitch * j;
+    double s = 0, d = 0;
+    if (i > 0) { s += ax * e[k - 1]; d += ax; }
+    if (i < nx - 1) { s += ax * e[k + 1]; d += ax; }
+    if (j > 0) { s += ay * e[k - pitch]; d += ay; }
+    if (j < ny - 1) { s += ay * e[k + pitch]; d += ay; }
+    e[k] = (s - r[k]) / d;
+}
+
+hipError_t launch_gsrb_fine(const Ctx &c, int color)
+{
+    const Level &L = c.lev[0];
+    dim3 gh(((L.nx + 1) / 2 + 63) / 64, (L.ny + 3) / 4);
+    hipLaunchKernelGGL(k_gsrb_free, gh, dim3(256), 0, c.stream, L.e, (const double *)L.r, L.nx, L.ny, L.pitch, L.ax,
+                       L.ay, color);
+    return hipGetLastError();
+}
+
 hipError_t launch_vcycle(const Ctx &c, int mixed)
 {
     if (mixed) {

@@ -1,276 +1,19 @@
 // HIP kernels of the fractional-step time step, gfx950.
 //
+// (The fused RK stage itself lives in kernels_stage.hip.)
+//
 // Data layout (fine level, one box): pitched row-major arrays with pitch P
 // (a multiple of 64 doubles, rows 512 B aligned): x-face u(i,j) at
 // u[i + P*j], i in [0,n], j in [0,n); y-face v(i,j), i in [0,n), j in [0,n];
 // cells q(i,j), i,j in [0,n).  Ghost cells are never stored: wall ghost
 // values are produced in registers/LDS from their mirror cells.
+#include "device_common.hpp"
 #include "internal.hpp"
 
+#include <algorithm>
 #include <cmath>
 
 namespace amr {
-
-// ------------------------------------------------------------ reductions
-// Deterministic: fixed wave-shuffle tree, then wave partials summed in
-// order by thread 0.  Block size must be a multiple of 64.
-__device__ __forceinline__ double wave_sum(double v)
-{
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-    return v;
-}
-
-__device__ __forceinline__ double wave_max(double v)
-{
-    for (int o = 32; o > 0; o >>= 1) {
-        const double w = __shfl_down(v, o, 64);
-        v = (w > v || w != w) ? w : v;
-    }
-    return v;
-}
-
-template <bool IsMax>
-__device__ double block_reduce(double v, double *sh)
-{
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    v = IsMax ? wave_max(v) : wave_sum(v);
-    __syncthreads();
-    if (lane == 0) sh[wid] = v;
-    __syncthreads();
-    double r = 0.0;
-    if (threadIdx.x == 0) {
-        r = sh[0];
-        for (int w = 1; w < nw; ++w) {
-            if (IsMax) r = (sh[w] > r || sh[w] != sh[w]) ? sh[w] : r;
-            else r += sh[w];
-        }
-    }
-    return r;  // valid in thread 0
-}
-
-// ================================================================ stage
-// One RK4 dual-time stage, fused: cell velocities from faces with the wall
-// ghost rules (cont2cart, utilities.cpp:16-350), QUICK half-node fluxes
-// (fn_flux_calc.cpp:11-202), viscous flux (:695-766), flux wall ghosts
-// (main.cpp:945-1014), face pressure gradient (fn_flux_calc.cpp:825-911),
-// momentum RHS (fn_rhs.cpp:8-68), dual-time residual and stage update
-// (momentum.cpp:68-87).  One block = TX x TY cells.
-constexpr int TX = 64, TY = 8, NT = 256;
-constexpr int FU_W = TX + 6, FU_H = TY + 5;  // x-faces: cols i0-3..i0+TX+2, rows j0-3..j0+TY+1
-constexpr int FV_W = TX + 5, FV_H = TY + 6;  // y-faces: cols i0-3..i0+TX+1, rows j0-3..j0+TY+2
-constexpr int CE_W = TX + 5, CE_H = TY + 5;  // cells:   cols i0-3..i0+TX+1, rows j0-3..j0+TY+1
-constexpr int FX_W = TX + 2, FX_H = TY + 1;  // x-face fluxes: cols i0-1..i0+TX, rows j0-1..j0+TY-1
-constexpr int FY_W = TX + 1, FY_H = TY + 2;  // y-face fluxes: cols i0-1..i0+TX-1, rows j0-1..j0+TY
-constexpr int T_W = TX + 1, T_H = TY + 1;    // total flux T: cols i0-1..i0+TX-1, rows j0-1..j0+TY-1
-
-struct StageParams {
-    const double *us, *vs, *ub, *vb, *cu, *cv, *p;
-    double *uo, *vo;
-    const double *lid;
-    double *partials;
-    const Ctl *ctl;
-    int n, P;
-    double dx, dy, dt, rk, ren;
-    int bc_lo0, bc_hi0, bc_lo1, bc_hi1;
-    double inflow0, inflow1;
-    int mode;
-};
-
-__device__ __forceinline__ double quick_face(double up, double ud, double a2, double a1, double b0,
-                                             double b1)
-{
-    // a2 = U(i-2), a1 = U(i-1), b0 = U(i), b1 = U(i+1) (fn_flux_calc.cpp:71-73)
-    return up * ((1.0 / 8.0) * (-b1 - 2.0 * b0 + 3.0 * a1) + b0) +
-           ud * ((1.0 / 8.0) * (-a2 - 2.0 * a1 + 3.0 * b0) + a1);
-}
-
-// wall ghost rules of cont2cart for one non-corner ghost cell, given the
-// mirror cell's values (um, vm).  side: 0 W, 1 E, 2 S, 3 N.
-__device__ __forceinline__ void ghost_rule(int side, int code, double um, double vm, double gx,
-                                           double gy, double &U, double &V)
-{
-    if (code == 135) { U = -um; V = -vm; return; }
-    if (side == 0) {  // W: -135 slip, +-165 unit inflow
-        if (code == -135) { U = -um; V = vm; } else { U = 2.0 - um; V = 2.0 - vm; }
-    } else if (side == 1) {  // E: slip branch tests the code 1, 165 unit inflow
-        if (code == 1) { U = -um; V = vm; } else { U = 2.0 - um; V = 2.0 - vm; }
-    } else if (side == 2) {  // S
-        if (code == -135) { U = um; V = -vm; } else { U = 2.0 - um; V = 2.0 - vm; }
-    } else {  // N: 165 is the lid 2*inflow - mirror
-        if (code == -135) { U = um; V = -vm; } else { U = 2.0 * gx - um; V = 2.0 * gy - vm; }
-    }
-}
-
-__global__ __launch_bounds__(NT) void k_stage(StageParams a)
-{
-    if (a.ctl && a.ctl->mom_done) return;
-    __shared__ double sFu[FU_H][FU_W];
-    __shared__ double sFv[FV_H][FV_W];
-    __shared__ double sU[CE_H][CE_W], sV[CE_H][CE_W];
-    __shared__ double sFxU[FX_H][FX_W], sFxV[FX_H][FX_W];
-    __shared__ double sFyU[FY_H][FY_W], sFyV[FY_H][FY_W];
-    __shared__ double sTU[T_H][T_W], sTV[T_H][T_W];
-    __shared__ double sred[16];
-
-    const int n = a.n, P = a.P, tid = threadIdx.x;
-    const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
-
-    // 1. stencil-input faces (zero outside the domain; never read there)
-    for (int e = tid; e < FU_W * FU_H; e += NT) {
-        const int li = e % FU_W, lj = e / FU_W, i = i0 - 3 + li, j = j0 - 3 + lj;
-        sFu[lj][li] = (i >= 0 && i <= n && j >= 0 && j < n) ? a.us[i + (size_t)P * j] : 0.0;
-    }
-    for (int e = tid; e < FV_W * FV_H; e += NT) {
-        const int li = e % FV_W, lj = e / FV_W, i = i0 - 3 + li, j = j0 - 3 + lj;
-        sFv[lj][li] = (i >= 0 && i < n && j >= 0 && j <= n) ? a.vs[i + (size_t)P * j] : 0.0;
-    }
-    __syncthreads();
-
-    // 2. cell velocities (face average, utilities.cpp:47-54) and non-corner
-    //    ghosts from their mirror cell (utilities.cpp:87-269)
-    for (int e = tid; e < CE_W * CE_H; e += NT) {
-        const int li = e % CE_W, lj = e / CE_W, ci = i0 - 3 + li, cj = j0 - 3 + lj;
-        const bool inx = ci >= 0 && ci < n, iny = cj >= 0 && cj < n;
-        double U = 0.0, V = 0.0;
-        if (inx && iny) {
-            U = 0.5 * (sFu[lj][li] + sFu[lj][li + 1]);
-            V = 0.5 * (sFv[lj][li] + sFv[lj + 1][li]);
-        } else if (!inx && iny && ci >= -2 && ci <= n + 1) {
-            const int mi = ci < 0 ? -ci - 1 : 2 * n - 1 - ci;
-            const int lm = mi - (i0 - 3);
-            if (lm >= 0 && lm < CE_W) {
-                const double um = 0.5 * (sFu[lj][lm] + sFu[lj][lm + 1]);
-                const double vm = 0.5 * (sFv[lj][lm] + sFv[lj + 1][lm]);
-                ghost_rule(ci < 0 ? 0 : 1, ci < 0 ? a.bc_lo0 : a.bc_hi0, um, vm, 0.0, 0.0, U, V);
-            }
-        } else if (inx && !iny && cj >= -2 && cj <= n + 1) {
-            const int mj = cj < 0 ? -cj - 1 : 2 * n - 1 - cj;
-            const int lm = mj - (j0 - 3);
-            if (lm >= 0 && lm < CE_H) {
-                const double um = 0.5 * (sFu[lm][li] + sFu[lm][li + 1]);
-                const double vm = 0.5 * (sFv[lm][li] + sFv[lm + 1][li]);
-                const double gx = a.lid ? a.lid[ci] : a.inflow0;
-                ghost_rule(cj < 0 ? 2 : 3, cj < 0 ? a.bc_lo1 : a.bc_hi1, um, vm, gx, a.inflow1, U, V);
-            }
-        }
-        sU[lj][li] = U;
-        sV[lj][li] = V;
-    }
-    __syncthreads();
-
-    // 3. QUICK half-node fluxes on x-faces (upwind split of ucont) and y-faces
-    for (int e = tid; e < FX_W * FX_H; e += NT) {
-        const int li = e % FX_W, lj = e / FX_W, fi = i0 - 1 + li, fj = j0 - 1 + lj;
-        if (fi < 0 || fi > n || fj < 0 || fj >= n) continue;
-        const double ucon = 0.5 * sFu[lj + 2][li + 2];
-        const double ud = ucon + fabs(ucon), up = ucon - fabs(ucon);
-        sFxU[lj][li] = quick_face(up, ud, sU[lj + 2][li], sU[lj + 2][li + 1], sU[lj + 2][li + 2], sU[lj + 2][li + 3]);
-        sFxV[lj][li] = quick_face(up, ud, sV[lj + 2][li], sV[lj + 2][li + 1], sV[lj + 2][li + 2], sV[lj + 2][li + 3]);
-    }
-    for (int e = tid; e < FY_W * FY_H; e += NT) {
-        const int li = e % FY_W, lj = e / FY_W, fi = i0 - 1 + li, fj = j0 - 1 + lj;
-        if (fi < 0 || fi >= n || fj < 0 || fj > n) continue;
-        const double vcon = 0.5 * sFv[lj + 2][li + 2];
-        const double vd = vcon + fabs(vcon), vp = vcon - fabs(vcon);
-        sFyU[lj][li] = quick_face(vp, vd, sU[lj][li + 2], sU[lj + 1][li + 2], sU[lj + 2][li + 2], sU[lj + 3][li + 2]);
-        sFyV[lj][li] = quick_face(vp, vd, sV[lj][li + 2], sV[lj + 1][li + 2], sV[lj + 2][li + 2], sV[lj + 3][li + 2]);
-    }
-    __syncthreads();
-
-    // 4. total cell flux T = -C + V (fluxTotal after the convective and
-    //    viscous updates)
-    const double dx = a.dx, dy = a.dy;
-    for (int e = tid; e < T_W * T_H; e += NT) {
-        const int li = e % T_W, lj = e / T_W, ci = i0 - 1 + li, cj = j0 - 1 + lj;
-        if (ci < 0 || ci >= n || cj < 0 || cj >= n) continue;
-        const double cU = (sFxU[lj][li + 1] - sFxU[lj][li]) / dx + (sFyU[lj + 1][li] - sFyU[lj][li]) / dy;
-        const double cV = (sFxV[lj][li + 1] - sFxV[lj][li]) / dx + (sFyV[lj + 1][li] - sFyV[lj][li]) / dy;
-        const int ui = li + 2, uj = lj + 2;
-        const double vU = ((sU[uj][ui + 1] - 2.0 * sU[uj][ui] + sU[uj][ui - 1]) / (dx * dx) +
-                           (sU[uj + 1][ui] - 2.0 * sU[uj][ui] + sU[uj - 1][ui]) / (dy * dy)) / a.ren;
-        const double vV = ((sV[uj][ui + 1] - 2.0 * sV[uj][ui] + sV[uj][ui - 1]) / (dx * dx) +
-                           (sV[uj + 1][ui] - 2.0 * sV[uj][ui] + sV[uj - 1][ui]) / (dy * dy)) / a.ren;
-        sTU[lj][li] = (0.0 - cU) + vU;
-        sTV[lj][li] = (0.0 - cV) + vV;
-    }
-    __syncthreads();
-
-    // 5. face RHS and stage update.  Wall-normal faces get R = 0 exactly:
-    //    the flux ghost is minus its mirror and the pressure ghost its mirror.
-    const double c15 = 1.5 / a.dt;
-    double sx = 0.0, sy = 0.0;
-    for (int e = tid; e < (TX + 1) * TY; e += NT) {
-        const int lf = e % (TX + 1), lr = e / (TX + 1), i = i0 + lf, j = j0 + lr;
-        if (lf == TX && i != n) continue;
-        if (i > n || j >= n) continue;
-        double R = 0.0;
-        if (i != 0 && i != n) {
-            const double gp = (a.p[i + (size_t)P * j] - a.p[i - 1 + (size_t)P * j]) / dx;
-            R = -gp + 0.5 * (sTU[lr + 1][lf] + sTU[lr + 1][lf + 1]);
-        }
-        const size_t k = i + (size_t)P * j;
-        if (a.mode == 2) { a.uo[k] = R; continue; }
-        const double own = sFu[lr + 3][lf + 3];
-        const double Rf = R - c15 * own + a.cu[k];
-        const double base = a.ub[k];
-        const double unew = base + a.rk * Rf;
-        if (a.mode == 1) {
-            const double d = base - unew;
-            sx += fabs(d * d * ((i > 0 && i < n) ? 1.0 : 0.5));
-        }
-        a.uo[k] = unew;
-    }
-    for (int e = tid; e < TX * (TY + 1); e += NT) {
-        const int lf = e % TX, lr = e / TX, i = i0 + lf, j = j0 + lr;
-        if (lr == TY && j != n) continue;
-        if (i >= n || j > n) continue;
-        double R = 0.0;
-        if (j != 0 && j != n) {
-            const double gp = (a.p[i + (size_t)P * j] - a.p[i + (size_t)P * (j - 1)]) / dy;
-            R = -gp + 0.5 * (sTV[lr][lf + 1] + sTV[lr + 1][lf + 1]);
-        }
-        const size_t k = i + (size_t)P * j;
-        if (a.mode == 2) { a.vo[k] = R; continue; }
-        const double own = sFv[lr + 3][lf + 3];
-        const double Rf = R - c15 * own + a.cv[k];
-        const double base = a.vb[k];
-        const double vnew = base + a.rk * Rf;
-        if (a.mode == 1) {
-            const double d = base - vnew;
-            sy += fabs(d * d * ((j > 0 && j < n) ? 1.0 : 0.5));
-        }
-        a.vo[k] = vnew;
-    }
-    if (a.mode == 1) {
-        const double bx = block_reduce<false>(sx, sred);
-        const double by = block_reduce<false>(sy, sred + 8);
-        if (tid == 0) {
-            const size_t b = blockIdx.x + (size_t)gridDim.x * blockIdx.y;
-            a.partials[2 * b] = bx;
-            a.partials[2 * b + 1] = by;
-        }
-    }
-}
-
-hipError_t launch_stage(const Ctx &c, const StageLaunch &s, int *nblocks)
-{
-    const Geometry &g = c.g;
-    StageParams a;
-    a.us = s.us; a.vs = s.vs; a.ub = s.ub; a.vb = s.vb; a.cu = s.cu; a.cv = s.cv; a.p = s.p;
-    a.uo = s.uo; a.vo = s.vo;
-    a.lid = c.lid;
-    a.partials = c.partials;
-    a.ctl = s.mode == 2 ? nullptr : c.ctl;
-    a.n = g.n; a.P = g.P; a.dx = g.dx; a.dy = g.dy; a.dt = g.dt; a.rk = s.rk; a.ren = g.ren;
-    a.bc_lo0 = g.bc_lo0; a.bc_hi0 = g.bc_hi0; a.bc_lo1 = g.bc_lo1; a.bc_hi1 = g.bc_hi1;
-    a.inflow0 = g.inflow0; a.inflow1 = g.inflow1;
-    a.mode = s.mode;
-    dim3 grid((g.n + TX - 1) / TX, (g.n + TY - 1) / TY);
-    if ((int64_t)grid.x * grid.y > kMaxPartials) return hipErrorInvalidValue;
-    if (nblocks) *nblocks = grid.x * grid.y;
-    hipLaunchKernelGGL(k_stage, grid, dim3(NT), 0, c.stream, a);
-    return hipGetLastError();
-}
 
 // ------------------------------------------------------- norm finalize
 // Error_Computation (utilities.cpp:421-493): e_c = sqrt(s_c)/sqrt(npts),
@@ -410,15 +153,19 @@ hipError_t launch_bdf2(const Ctx &c)
 // ------------------------------------------------------------ Poisson RHS
 // fn_rhs.cpp:71-125: D = (u_{i+1}-u_i)/dx + (v_{j+1}-v_j)/dy, b = (1.5/dt) D,
 // block partials of ||D||inf and sum b.
-__global__ __launch_bounds__(256) void k_poisson_rhs(const double *u, const double *v, double *b, int n, int P,
-                                                     double dx, double dy, double dt, double *partials)
+// With au/av non-null the velocity is u + au (the mixed step's u^n + S).
+__global__ __launch_bounds__(256) void k_poisson_rhs(const double *u, const double *v, const float *au,
+                                                     const float *av, double *b, int n, int P, double dx, double dy,
+                                                     double dt, double *partials)
 {
     __shared__ double sh[16];
     const int i = blockIdx.x * 64 + (threadIdx.x & 63), j = blockIdx.y * 4 + (threadIdx.x >> 6);
     double m = 0.0, s = 0.0;
     if (i < n && j < n) {
         const size_t k = i + (size_t)P * j;
-        const double d = (u[k + 1] - u[k]) / dx + (v[k + P] - v[k]) / dy;
+        double uw = u[k], ue = u[k + 1], vs = v[k], vn = v[k + P];
+        if (au) { uw += (double)au[k]; ue += (double)au[k + 1]; vs += (double)av[k]; vn += (double)av[k + P]; }
+        const double d = (ue - uw) / dx + (vn - vs) / dy;
         const double bb = (1.5 / dt) * d;
         b[k] = bb;
         m = fabs(d);
@@ -433,13 +180,14 @@ __global__ __launch_bounds__(256) void k_poisson_rhs(const double *u, const doub
     }
 }
 
-hipError_t launch_poisson_rhs(const Ctx &c, const double *u, const double *v, double *b, int *nblocks)
+hipError_t launch_poisson_rhs(const Ctx &c, const double *u, const double *v, double *b, int *nblocks,
+                              const float *au, const float *av)
 {
     const int n = c.g.n;
     dim3 grid((n + 63) / 64, (n + 3) / 4);
     if ((int64_t)grid.x * grid.y > kMaxPartials) return hipErrorInvalidValue;
     *nblocks = grid.x * grid.y;
-    hipLaunchKernelGGL(k_poisson_rhs, grid, dim3(256), 0, c.stream, u, v, b, n, c.g.P, c.g.dx, c.g.dy, c.g.dt,
+    hipLaunchKernelGGL(k_poisson_rhs, grid, dim3(256), 0, c.stream, u, v, au, av, b, n, c.g.P, c.g.dx, c.g.dy, c.g.dt,
                        c.partials);
     return hipGetLastError();
 }
@@ -511,9 +259,10 @@ hipError_t launch_center_b(const Ctx &c, double *b)
 // update_solution (poisson.cpp:192-392): u = u* - (G_x phi) dt / 1.5 on
 // interior faces (wall faces keep u*: the Neumann ghost makes G phi = 0),
 // p <- p + (phi - mean phi), and the partial ||D(u^{n+1})||inf.
-__global__ __launch_bounds__(256) void k_project(const double *us, const double *vs, double *uo, double *vo,
-                                                 double *p, const double *phi, int n, int P, double dx, double dy,
-                                                 double dt, const Ctl *ctl, int use_mean, double *partials)
+__global__ __launch_bounds__(256) void k_project(const double *us, const double *vs, const float *au,
+                                                 const float *av, double *uo, double *vo, double *p,
+                                                 const double *phi, int n, int P, double dx, double dy, double dt,
+                                                 const Ctl *ctl, int use_mean, double *partials)
 {
     __shared__ double sh[16];
     const int i = blockIdx.x * 64 + (threadIdx.x & 63), j = blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -523,15 +272,17 @@ __global__ __launch_bounds__(256) void k_project(const double *us, const double 
         const double ph = phi[k];
         auto ux = [&](int fi) -> double {  // new x-face velocity at (fi, j)
             const size_t f = fi + (size_t)P * j;
-            if (fi == 0 || fi == n) return us[f];
+            const double ustar = au ? us[f] + (double)au[f] : us[f];
+            if (fi == 0 || fi == n) return ustar;
             const double g = (phi[f] - phi[f - 1]) / dx;
-            return us[f] - (g * dt / 1.5);
+            return ustar - (g * dt / 1.5);
         };
         auto vy = [&](int fj) -> double {
             const size_t f = i + (size_t)P * fj;
-            if (fj == 0 || fj == n) return vs[f];
+            const double vstar = av ? vs[f] + (double)av[f] : vs[f];
+            if (fj == 0 || fj == n) return vstar;
             const double g = (phi[f] - phi[f - P]) / dy;
-            return vs[f] - (g * dt / 1.5);
+            return vstar - (g * dt / 1.5);
         };
         const double uw = ux(i), ue = ux(i + 1), vsth = vy(j), vn = vy(j + 1);
         uo[k] = uw;
@@ -547,20 +298,21 @@ __global__ __launch_bounds__(256) void k_project(const double *us, const double 
 }
 
 hipError_t launch_project_ex(const Ctx &c, const double *us, const double *vs, double *uo, double *vo, double *p,
-                             const double *phi, int use_mean, int *nblocks)
+                             const double *phi, int use_mean, int *nblocks, const float *au, const float *av)
 {
     const int n = c.g.n;
     dim3 grid((n + 63) / 64, (n + 3) / 4);
     if ((int64_t)grid.x * grid.y > kMaxPartials) return hipErrorInvalidValue;
     *nblocks = grid.x * grid.y;
-    hipLaunchKernelGGL(k_project, grid, dim3(256), 0, c.stream, us, vs, uo, vo, p, phi, n, c.g.P, c.g.dx, c.g.dy,
-                       c.g.dt, c.ctl, use_mean, c.partials);
+    hipLaunchKernelGGL(k_project, grid, dim3(256), 0, c.stream, us, vs, au, av, uo, vo, p, phi, n, c.g.P, c.g.dx,
+                       c.g.dy, c.g.dt, c.ctl, use_mean, c.partials);
     return hipGetLastError();
 }
 
-hipError_t launch_project(const Ctx &c, const double *us, const double *vs, double *uo, double *vo, int *nblocks)
+hipError_t launch_project(const Ctx &c, const double *us, const double *vs, double *uo, double *vo, int *nblocks,
+                          const float *au, const float *av)
 {
-    return launch_project_ex(c, us, vs, uo, vo, c.p, c.phi, 1, nblocks);
+    return launch_project_ex(c, us, vs, uo, vo, c.p, c.phi, 1, nblocks, au, av);
 }
 
 __global__ __launch_bounds__(1024) void k_fin_project(const double *partials, int nb, Ctl *ctl,
@@ -741,6 +493,21 @@ hipError_t launch_copy_faces(const Ctx &c, const double *src, double *dst, int d
     const int n = c.g.n, nx = dir == 0 ? n + 1 : n, ny = dir == 0 ? n : n + 1;
     hipLaunchKernelGGL(k_copy2d, dim3((nx + 63) / 64, (ny + 3) / 4), dim3(256), 0, c.stream, src, dst, nx, ny,
                        c.g.P);
+    return hipGetLastError();
+}
+
+// plain 16-byte-vector copy (bandwidth calibration)
+__global__ __launch_bounds__(256) void k_copy_bytes(const double2 *src, double2 *dst, int64_t n16)
+{
+    for (int64_t k = blockIdx.x * (int64_t)256 + threadIdx.x; k < n16; k += (int64_t)gridDim.x * 256) dst[k] = src[k];
+}
+
+hipError_t launch_copy_bytes(const Ctx &c, const void *src, void *dst, int64_t bytes)
+{
+    const int64_t n16 = bytes / 16;
+    const int64_t blocks = std::min<int64_t>((n16 + 255) / 256, 256 * 64);
+    hipLaunchKernelGGL(k_copy_bytes, dim3((unsigned)blocks), dim3(256), 0, c.stream, (const double2 *)src,
+                       (double2 *)dst, n16);
     return hipGetLastError();
 }
 

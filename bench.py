@@ -1,0 +1,169 @@
+#!/usr/bin/env python
+"""Benchmark of the MI355X fractional step on the 2048^2 lid-driven cavity.
+
+Workload (the one the headline metric is quoted on): unit-square cavity,
+2048 x 2048 cells, Re = 5000, lid u = 1, seeded divergence-free start, cfl 0.5
+(dt = 0.5 / (4 n^2)), momentum tolerance 1e-10, Poisson tolerance 1e-10.
+
+A "step" is one full time step (RK4 dual-time pseudo-iterations to
+convergence, Poisson RHS, multigrid projection, update) over the whole grid.
+The state is resident in HBM before the timed region; W warm-up steps run
+first, then exactly K steps are timed between device synchronisations.
+
+Both precisions are run from the same initial state:
+  * mixed  -- FP32 stencils and FP32 V-cycle inside FP64 iterative refinement
+              (the headline `value`)
+  * fp64   -- the plain FP64 step
+and the mixed result is compared with the FP64 one per field after the same
+number of steps.  One JSON line is printed.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=2048)
+    ap.add_argument("--ren", type=float, default=5000.0)
+    ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps timed for cpu_baseline (0: skip)")
+    ap.add_argument("--only", choices=["both", "mixed", "fp64"], default="both")
+    ap.add_argument("--profile-run", action="store_true", help="short run for rocprofv3: no cpu baseline")
+    return ap.parse_args()
+
+
+def read_traffic(kernel_key: str):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, scaled by
+    the copy-kernel calibration recorded next to it (None if absent)."""
+    path = os.path.join(ROOT, "profiles", "pmc_summary.txt")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        m = re.match(r"\s*([A-Za-z0-9_.]+)\s*:\s*([-+0-9.eE]+)\s*$", line)
+        if m:
+            vals[m.group(1)] = float(m.group(2))
+    key = f"{kernel_key}.traffic_bytes"
+    return vals.get(key)
+
+
+def run_precision(api, W, n, ren, precision, state, warmup, steps, torch):
+    inp = api.cavity_inputs(n, ren, precision=precision)
+    run = api.Run.from_numpy(inp, state)
+    if warmup:
+        run.advance(warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stats = run.advance(steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    return run, stats, t1 - t0
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        sys.stderr.write("bench.py: this build runs on one GPU only (the step is measured on one MI355X); "
+                         f"--gpus {args.gpus} is not supported\n")
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from amressif_amd import api
+    from amressif_amd import workloads as W
+
+    n, ren, K, Wu = args.n, args.ren, args.steps, args.warmup
+    state = W.initial_state(n, "seeded")
+    out = {}
+    runs = {}
+    for name, prec in (("mixed", api.AMRESSIF_MIXED), ("fp64", api.AMRESSIF_FP64)):
+        if args.only not in ("both", name):
+            continue
+        run, stats, sec = run_precision(api, W, n, ren, prec, state, Wu, K, torch)
+        P = float(np.mean([s["pseudo_iters"] for s in stats]))
+        V = float(np.mean([s["vcycles"] for s in stats]))
+        flags = int(np.bitwise_or.reduce([s["flags"] for s in stats]))
+        worst_div = max(s["div_new"] / max(s["div_star"], 1e-300) for s in stats)
+        out[name] = dict(mcell_steps_per_s=n * n * K / sec / 1e6, ms_per_step=1e3 * sec / K, pseudo_iters=P,
+                         vcycles=V, flags=flags, max_div_ratio=worst_div)
+        runs[name] = run
+
+    # per-field error of mixed vs FP64 after the same W+K steps
+    err = None
+    if "mixed" in runs and "fp64" in runs:
+        a, b = runs["mixed"].fetch(), runs["fp64"].fetch()
+
+        def rel(x, y):
+            return float(np.abs(x - y).max() / max(np.abs(y).max(), 1e-300))
+
+        pa, pb = a["ctx"][0, 1:-1, 1:-1], b["ctx"][0, 1:-1, 1:-1]
+        fa, fb = a["ctx"][1, 1:-1, 1:-1], b["ctx"][1, 1:-1, 1:-1]
+        err = {"U": rel(a["u"], b["u"]), "V": rel(a["v"], b["v"]),
+               "p": rel(pa - pa.mean(), pb - pb.mean()), "phi": rel(fa - fa.mean(), fb - fb.mean())}
+
+    # roofline of the dominant kernel (the fused RK stage) of the headline run
+    head = "mixed" if "mixed" in runs else "fp64"
+    ctx = runs[head].ctx
+    ms, nbytes = ctx.time_kernel(0, reps=50)
+    achieved = nbytes / (ms * 1e-3)
+    traffic = read_traffic(f"stage_{head}_{n}")
+    roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK, "traffic": traffic, "kernel": "amr::k_stage",
+                "avg_launch_ms": ms, "algorithmic_bytes_per_launch": nbytes}
+
+    cpu = None
+    if args.cpu_steps > 0 and not args.profile_run:
+        from oracle.refcpu import Oracle
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+        o = Oracle(n, ren, W.dt_from_cfl(n, 0.5), nthreads=threads)
+        st = W.initial_state(n, "seeded")
+        t0 = time.perf_counter()
+        o.advance(st, args.cpu_steps)
+        sec = time.perf_counter() - t0
+        cpu = {"value": n * n * args.cpu_steps / sec / 1e6, "unit": "Mcell-steps/s", "cores": threads,
+               "kind": "port",
+               "sample": f"oracle/refcpu.c FP64 (MG Poisson), first {args.cpu_steps} steps of the same "
+                         f"{n}^2 Re={ren:g} seeded workload, {threads} OpenMP threads, {sec:.1f} s"}
+
+    value = out[head]["mcell_steps_per_s"]
+    line = {
+        "metric": "Mcell-steps/s 2048² mixed vs FP64 (err≤1e-6); adjoint:forward time ratio",
+        "value": value,
+        "unit": "Mcell-steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": Wu,
+        "ms_per_step": out[head]["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if head == "mixed" else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"lid-driven cavity {n}x{n}, Re={ren:g}, seeded divergence-free start, "
+                               f"cfl 0.5, tol 1e-10, {head} precision", "n_cell": n, "ren": ren,
+                   "precision": head},
+        "runs": out,
+        "mixed_vs_fp64_speedup": (out["mixed"]["mcell_steps_per_s"] / out["fp64"]["mcell_steps_per_s"]
+                                  if "mixed" in out and "fp64" in out else None),
+        "mixed_vs_fp64_rel_err": err,
+        "adjoint_forward_ratio": None,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -173,6 +173,18 @@ int amressif_poisson_solve(amressif_ctx *ctx, amr_array4 *sol, const amr_array4 
 int amressif_project(amressif_ctx *ctx, const amr_array4 velStar[2], amr_array4 *userCtx,
                      amr_array4 velCont[2], double *div_linf);
 
+/* ------------------------------------------------------------ diagnostics
+ * No reference counterpart (the reference prints per-step wall time only,
+ * main.cpp:1173-1175).  Average device time of one kernel at the context's
+ * shape and current state, timed with HIP events on the context's stream over
+ * `reps` back-to-back launches:
+ *   which = 0  fused RK stage of the configured precision (the step's dominant kernel)
+ *   which = 1  one red-black Gauss-Seidel colour sweep on the finest level
+ *   which = 2  plain device-to-device copy of copy_bytes bytes (bandwidth calibration)
+ * bytes_per_launch receives the kernel's algorithmic HBM bytes per launch. */
+int amressif_time_kernel(amressif_ctx *ctx, int which, int reps, int64_t copy_bytes, double *avg_ms,
+                         double *bytes_per_launch);
+
 #ifdef __cplusplus
 }
 #endif

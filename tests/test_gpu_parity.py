@@ -264,3 +264,52 @@ def test_nonfinite_reported():
     with pytest.raises(api.AmressifError) as e:
         run.advance(3)
     assert e.value.code == -5
+
+
+# ------------------------------------------------------------- mixed precision
+def _run_gpu(n, ren, steps, precision, init="seeded"):
+    api = _api()
+    inp = api.cavity_inputs(n, ren, precision=precision)
+    run = api.Run.from_numpy(inp, W.initial_state(n, init))
+    stats = run.advance(steps)
+    return run.fetch(), stats
+
+
+@pytest.mark.parametrize("n,ren,steps", [(128, 100.0, 30), (256, 1000.0, 20)])
+def test_mixed_matches_fp64_within_1e_6(n, ren, steps):
+    api = _api()
+    a, sa = _run_gpu(n, ren, steps, api.AMRESSIF_MIXED)
+    b, sb = _run_gpu(n, ren, steps, api.AMRESSIF_FP64)
+    # per-field bar of the mixed form: 1e-6 relative to the FP64 field's max-norm
+    assert rel(a["u"], b["u"]) < 1e-6
+    assert rel(a["v"], b["v"]) < 1e-6
+    pa, pb = a["ctx"][0, 1:-1, 1:-1], b["ctx"][0, 1:-1, 1:-1]
+    assert rel(pa - pa.mean(), pb - pb.mean()) < 1e-6
+    for s, r in zip(sa, sb):
+        assert abs(s["pseudo_iters"] - r["pseudo_iters"]) <= 1
+        assert s["flags"] == 0
+        # the outer Poisson residual and the divergence check stay FP64
+        assert s["div_new"] <= 1e-10 * s["div_star"] + 1e-12
+    assert np.all(a["u"][:, [0, -1]] == 0.0) and np.all(a["v"][[0, -1], :] == 0.0)
+
+
+def test_mixed_matches_oracle_within_1e_6():
+    api = _api()
+    n, ren, steps = 128, 400.0, 10
+    got, _ = _run_gpu(n, ren, steps, api.AMRESSIF_MIXED)
+    inp, o = make_pair(n, ren)
+    ref = W.initial_state(n, "seeded")
+    o.advance(ref, steps)
+    assert rel(got["u"], ref["u"]) < 1e-6 and rel(got["v"], ref["v"]) < 1e-6
+
+
+def test_kernel_timing_diagnostic():
+    api = _api()
+    for prec in (api.AMRESSIF_FP64, api.AMRESSIF_MIXED):
+        run = api.Run.from_numpy(api.cavity_inputs(256, 1000.0, precision=prec), W.initial_state(256, "seeded"))
+        run.advance(1)
+        for which in (0, 1):
+            ms, nbytes = run.ctx.time_kernel(which, reps=5)
+            assert ms > 0 and nbytes > 0
+    ms, nbytes = run.ctx.time_kernel(2, reps=5, copy_bytes=1 << 24)
+    assert ms > 0 and nbytes == 2 * (1 << 24)

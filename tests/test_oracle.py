@@ -1,0 +1,340 @@
+"""Pins of the CPU FP64 oracle (oracle/refcpu.c) -- CPU only.
+
+The reference ships no test vectors for its time step and cannot be built
+here (it needs AMReX, un-vendored and unpinned), so each oracle function is
+pinned by what the mathematics fixes: closed forms, exact identities on
+integer-valued data, an independent dense construction, golden vectors from
+scipy's DCT, and exact rational QUICK weights.  Each check is chosen so that a
+dropped term, a sign or index slip, or a transposed operand fails it.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from amressif_amd import workloads as W
+from oracle.refcpu import Oracle
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def neumann_dense(nx, ny, hx, hy):
+    """Dense 5-point Neumann Laplacian built from 1D second differences by
+    Kronecker products (independent of the oracle's stencil code)."""
+    def d2(m, h):
+        D = np.zeros((m, m))
+        for i in range(m):
+            if i > 0:
+                D[i, i - 1] += 1; D[i, i] -= 1
+            if i < m - 1:
+                D[i, i + 1] += 1; D[i, i] -= 1
+        return D / (h * h)
+    # unknown index = i + nx*j  (i fastest)
+    return np.kron(np.eye(ny), d2(nx, hx)) + np.kron(d2(ny, hy), np.eye(nx))
+
+
+# ------------------------------------------------------------- cont2cart
+@pytest.mark.parametrize("bc_lo,bc_hi", [((135, 135), (135, 165)), ((-135, -135), (135, -135)),
+                                         ((165, 165), (165, 135))])
+def test_ghost_rules_closed_forms(bc_lo, bc_hi):
+    n = 12
+    rng = np.random.default_rng(0)
+    o = Oracle(n, 100.0, 1e-4, bc_lo=bc_lo, bc_hi=bc_hi, inflow=(0.75, -0.25))
+    u = rng.uniform(-1, 1, (n, n + 1))
+    v = rng.uniform(-1, 1, (n + 1, n))
+    vc = o.cont2cart(u, v)
+    g = 2  # ghost offset in the array
+    # valid cells: face averages
+    assert np.allclose(vc[0, g:-g, g:-g], 0.5 * (u[:, :-1] + u[:, 1:]), rtol=0, atol=0)
+    assert np.allclose(vc[1, g:-g, g:-g], 0.5 * (v[:-1, :] + v[1:, :]), rtol=0, atol=0)
+    rows = slice(g, g + n)
+
+    def wall_value(code, side, comp):
+        # (ghost + mirror)/2 == wall value; None means "ghost == mirror" (slip tangential)
+        if code == 135:
+            return 0.0
+        if code == -135:
+            normal = (side in "WE" and comp == 0) or (side in "SN" and comp == 1)
+            return 0.0 if normal else None
+        if code == 165:
+            if side == "N":
+                return 0.75 if comp == 0 else -0.25
+            return 1.0
+        raise AssertionError
+
+    for layer in (1, 2):
+        for comp in (0, 1):
+            for side, code in (("W", bc_lo[0]), ("E", bc_hi[0]), ("S", bc_lo[1]), ("N", bc_hi[1])):
+                if side == "W":
+                    ghost, mirror = vc[comp, rows, g - layer], vc[comp, rows, g + layer - 1]
+                elif side == "E":
+                    ghost, mirror = vc[comp, rows, g + n - 1 + layer], vc[comp, rows, g + n - layer]
+                elif side == "S":
+                    ghost, mirror = vc[comp, g - layer, rows], vc[comp, g + layer - 1, rows]
+                else:
+                    ghost, mirror = vc[comp, g + n - 1 + layer, rows], vc[comp, g + n - layer, rows]
+                wv = wall_value(code, side, comp)
+                if wv is None:
+                    assert np.array_equal(ghost, mirror)
+                else:
+                    assert np.allclose(0.5 * (ghost + mirror), wv, rtol=0, atol=1e-15)
+
+
+def test_ghost_codes_the_reference_aborts_on():
+    o = Oracle(8, 100.0, 1e-4, bc_lo=(195, 135), bc_hi=(135, 165))
+    with pytest.raises(ValueError):
+        o.cont2cart(np.zeros((8, 9)), np.zeros((9, 8)))
+
+
+# ------------------------------------------------------------------ QUICK
+def _quick_setup(n=8):
+    return Oracle(n, 100.0, 1e-4)
+
+
+def test_quick_golden_rational_cases():
+    cases = json.load(open(os.path.join(GOLDEN, "quick_cases.json")))
+    n = 8
+    o = _quick_setup(n)
+    i, j = 4, 3
+    for c in cases:
+        vc = o.cart()
+        for k, val in enumerate(c["U"]):
+            vc[0, j + 2, i - 2 + k + 2] = val  # U(i-2..i+1, j)
+        u = np.zeros((n, n + 1))
+        u[j, i] = c["ucont"]
+        fxu = o.convective_flux(vc, u, np.zeros((n + 1, n)))[0]
+        assert fxu[j, i] == c["flux"], c
+
+
+@pytest.mark.parametrize("sign", [1.0, -1.0])
+def test_quick_exact_on_quadratics(sign):
+    n = 16
+    h = 1.0 / n
+    o = Oracle(n, 100.0, 1e-4)
+    rng = np.random.default_rng(5)
+    a, b, c = 0.75, -1.25, 0.5
+    xc = (np.arange(-2, n + 2) + 0.5) * h
+    vc = o.cart()
+    vc[0] = (a * xc ** 2 + b * xc + c)[None, :]  # U varies in x only
+    vc[1] = (a * xc ** 2 + b * xc + c)[:, None]  # V varies in y only
+    u = sign * rng.uniform(0.1, 1.0, (n, n + 1))
+    v = sign * rng.uniform(0.1, 1.0, (n + 1, n))
+    fxu, fxv, fyu, fyv, _, _ = o.convective_flux(vc, u, v)
+    xf = np.arange(n + 1) * h
+    exact_x = a * xf ** 2 + b * xf + c
+    assert np.allclose(fxu, u * exact_x[None, :], rtol=1e-13, atol=1e-14)
+    assert np.allclose(fyv, v * exact_x[:, None], rtol=1e-13, atol=1e-14)
+
+
+def test_quick_divergence_of_uniform_field_is_zero():
+    n = 16
+    o = Oracle(n, 100.0, 1e-4)
+    vc = o.cart()
+    vc[0] = 0.625
+    vc[1] = -0.375
+    u = np.full((n, n + 1), 0.5)
+    v = np.full((n + 1, n), -0.25)
+    _, _, _, _, fconv, _ = o.convective_flux(vc, u, v)
+    assert np.all(fconv == 0.0)
+
+
+# ---------------------------------------------------------------- viscous
+def test_viscous_exact_on_quadratics():
+    n = 16
+    h = 1.0 / n
+    ren = 7.0
+    o = Oracle(n, ren, 1e-4)
+    x = (np.arange(-2, n + 2) + 0.5) * h
+    X, Y = np.meshgrid(x, x)  # rows j (y), cols i (x)
+    vc = o.cart()
+    vc[0] = X ** 2 + Y ** 2
+    vc[1] = 3 * X ** 2 - Y ** 2
+    fvisc, ftot = o.viscous_flux(vc)
+    assert np.allclose(fvisc[0], 4.0 / ren, rtol=1e-12, atol=0)
+    assert np.allclose(fvisc[1], 4.0 / ren, rtol=1e-12, atol=0)
+    assert np.array_equal(ftot[:, 1:-1, 1:-1], fvisc)
+
+
+# ---------------------------------------------- gradient / divergence / L
+def test_divergence_of_gradient_is_the_neumann_laplacian():
+    n = 8
+    h = 1.0 / n
+    rng = np.random.default_rng(11)
+    # dt = 1.5 makes the Poisson-RHS scale factor (1.5/dt) exactly 1
+    o = Oracle(n, 100.0, 1.5)
+    phi = rng.integers(-20, 20, (n, n)).astype(float)
+    ctx = o.ctx()
+    ctx[1, 1:-1, 1:-1] = phi
+    o.userctx_ghosts(ctx)
+    _, _, gx, gy = o.gradient(ctx)
+    div, _, _ = o.poisson_rhs(gx, gy)
+    L = neumann_dense(n, n, h, h)
+    exact = (L @ phi.reshape(-1)).reshape(n, n)
+    assert np.array_equal(div[1:-1, 1:-1], exact)
+    assert np.array_equal(o.apply_laplacian(phi), exact)
+    # wall faces carry a zero gradient
+    assert np.all(gx[:, [0, -1]] == 0.0) and np.all(gy[[0, -1], :] == 0.0)
+
+
+def test_momentum_rhs_vanishes_on_wall_faces():
+    n = 10
+    rng = np.random.default_rng(12)
+    o = Oracle(n, 50.0, 1e-4)
+    u = rng.uniform(-1, 1, (n, n + 1))
+    v = rng.uniform(-1, 1, (n + 1, n))
+    ctx = o.ctx()
+    ctx[:, 1:-1, 1:-1] = rng.uniform(-1, 1, (2, n, n))
+    o.userctx_ghosts(ctx)
+    rx, ry = o.stage_rhs(u, v, ctx)
+    assert np.all(rx[:, [0, -1]] == 0.0) and np.all(ry[[0, -1], :] == 0.0)
+    assert np.abs(rx[:, 1:-1]).min() > 0
+
+
+# ------------------------------------------------------------------- RK4
+def test_rk4_dual_time_amplification_polynomial():
+    n = 8
+    dt = 3e-4
+    rng = np.random.default_rng(13)
+    o = Oracle(n, 100.0, dt)
+    u0 = rng.uniform(-1, 1, (n, n + 1)); v0 = rng.uniform(-1, 1, (n + 1, n))
+    up = rng.uniform(-1, 1, (n, n + 1)); vp = rng.uniform(-1, 1, (n + 1, n))
+    ud = rng.uniform(-1, 1, (n, n + 1)); vd = rng.uniform(-1, 1, (n + 1, n))
+    us, vs = u0.copy(), v0.copy()
+    for s, rk in enumerate(W.rk_coefficients(dt)):
+        rx, ry = np.zeros_like(u0), np.zeros_like(v0)  # zero flux RHS
+        o.rk4_stage(rk, rx, ry, us, vs, u0, v0, ud, vd, up, vp)
+    z = -1.5 * 0.4  # lambda * dtau with lambda = -1.5/dt, dtau = 0.4 dt
+    R4 = 1 + z + z ** 2 / 2 + z ** 3 / 6 + z ** 4 / 24
+    ueq, veq = up + ud / 3, vp + vd / 3
+    assert np.allclose(us - ueq, R4 * (u0 - ueq), rtol=1e-12, atol=1e-14)
+    assert np.allclose(vs - veq, R4 * (v0 - veq), rtol=1e-12, atol=1e-14)
+
+
+def test_rk_coefficients_pinned_by_the_reference_loop_test():
+    # Tests/loop_test/forloopforvec.cpp:12-22 prints {1/4, 1/3, 1/2, 1}
+    assert [r / (0.4 * 2.0) for r in W.rk_coefficients(2.0)] == [0.25, 1.0 / 3.0, 0.5, 1.0]
+
+
+# ------------------------------------------------------------ error norm
+def test_error_norm_closed_forms():
+    n = 6
+    o = Oracle(n, 100.0, 1e-4)
+    zx, zy = np.zeros((n, n + 1)), np.zeros((n + 1, n))
+    ox, oy = np.ones((n, n + 1)), np.ones((n + 1, n))
+    assert o.error_computation(ox, oy, zx, zy) == 1.0  # weights sum to n*n per direction
+    wx = zx.copy(); wx[:, [0, -1]] = 1.0
+    assert math.isclose(o.error_computation(wx, zy, zx, zy), math.sqrt(n) / n, rel_tol=1e-15)
+    wy = zy.copy(); wy[1, :] = 2.0
+    assert math.isclose(o.error_computation(zx, wy, zx, zy), math.sqrt(4 * n) / n, rel_tol=1e-15)
+
+
+# ---------------------------------------------------------------- Poisson
+def test_poisson_dct_matches_scipy_golden():
+    g = np.load(os.path.join(GOLDEN, "poisson_dct_n16.npz"))
+    n = g["b"].shape[0]
+    o = Oracle(n, 100.0, 1e-4, poisson="dct")
+    rhs = np.zeros((n + 2, n + 2))
+    rhs[1:-1, 1:-1] = g["b"]
+    sol, _, res = o.poisson_solve(rhs)
+    assert np.allclose(sol[1:-1, 1:-1], g["x"], rtol=0, atol=1e-12 * np.abs(g["x"]).max())
+    assert res < 1e-12
+
+
+@pytest.mark.parametrize("nx,ny,hx,hy", [(4, 4, 0.25, 0.25), (6, 4, 0.125, 0.5)])
+def test_pinv_matches_numpy(nx, ny, hx, hy):
+    P = Oracle.pinv_neumann(nx, ny, hx, hy)
+    Pn = np.linalg.pinv(neumann_dense(nx, ny, hx, hy))
+    assert np.allclose(P, Pn, rtol=0, atol=1e-11 * np.abs(Pn).max())
+
+
+@pytest.mark.parametrize("n", [64, 96])
+def test_multigrid_matches_exact_solve(n):
+    rng = np.random.default_rng(n)
+    rhs = np.zeros((n + 2, n + 2))
+    rhs[1:-1, 1:-1] = rng.standard_normal((n, n))
+    xe, _, _ = Oracle(n, 100.0, 1e-4, poisson="dct").poisson_solve(rhs)
+    xm, it, res = Oracle(n, 100.0, 1e-4, poisson="mg", poisson_tol_rel=1e-12).poisson_solve(rhs)
+    assert res <= 1e-12 and 3 <= it <= 25
+    assert np.allclose(xm, xe, rtol=0, atol=1e-9 * np.abs(xe).max())
+    assert abs(xm[1:-1, 1:-1].mean()) < 1e-15
+
+
+# -------------------------------------------------------------- whole step
+def _cavity(n, init, poisson, steps, ren=100.0, **kw):
+    o = Oracle(n, ren, W.dt_from_cfl(n, 0.5), poisson=poisson, **kw)
+    st = W.initial_state(n, init)
+    stats = o.advance(st, steps)
+    return st, stats
+
+
+@pytest.mark.parametrize("init", ["zero", "seeded"])
+def test_step_invariants_and_solver_agreement(init):
+    n = 32
+    a, sa = _cavity(n, init, "mg", 5)
+    b, sb = _cavity(n, init, "dct", 5)
+    for s in sa:
+        assert s["div_new"] <= 1e-10 * s["div_star"] + 1e-13
+        assert s["flags"] == 0 and 1 <= s["pseudo_iters"] <= 100
+    assert [s["pseudo_iters"] for s in sa] == [s["pseudo_iters"] for s in sb]
+    for k in ("u", "v"):
+        assert np.abs(a[k] - b[k]).max() <= 1e-12 * max(1.0, np.abs(b[k]).max())
+    assert np.all(a["u"][:, [0, -1]] == 0.0) and np.all(a["v"][[0, -1], :] == 0.0)
+    p = a["ctx"][0, 1:-1, 1:-1]
+    assert abs(p.mean()) <= 1e-12 * max(1.0, np.abs(p).max())
+
+
+def test_first_step_from_rest_is_lid_driven():
+    st, stats = _cavity(16, "zero", "dct", 1)
+    assert stats[0]["pseudo_iters"] > 1
+    # the flow near the lid moves with the lid, the bottom stays nearly at rest
+    assert st["u"][-1, 1:-1].mean() > 0 and abs(st["u"][0, 1:-1]).max() < st["u"][-1, 1:-1].mean()
+
+
+# ---------------------------------------------------- failure guards
+def test_iternum_cap_guard():
+    _, stats = _cavity(16, "seeded", "dct", 2, iter_num=0)
+    assert all(s["pseudo_iters"] == 1 and s["flags"] & 1 for s in stats)
+
+
+def test_divergence_guard():
+    n = 16
+    o = Oracle(n, 100.0, 1e-2, poisson="dct")  # far too large a dt for the injected velocity
+    st = W.initial_state(n, "seeded")
+    st["u"][5, 5] = 1e3
+    stats = o.advance(st, 1)
+    assert stats[0]["flags"] & 2
+
+
+def test_nonfinite_is_detected():
+    n = 16
+    o = Oracle(n, 100.0, W.dt_from_cfl(n), poisson="dct")
+    st = W.initial_state(n, "seeded")
+    st["v"][3, 4] = np.nan
+    stats = o.advance(st, 1)
+    assert stats[0]["flags"] & 4 and stats[0]["pseudo_iters"] == 1
+
+
+# ------------------------------------------------------------ workloads
+def test_seeded_field_is_discretely_divergence_free():
+    n = 64
+    u, v = W.seeded_velocity(n)
+    o = Oracle(n, 100.0, 1.5)
+    d, linf, _ = o.poisson_rhs(u, v)
+    assert linf <= 1e-12 * np.abs(u).max() * n
+    assert np.all(u[:, [0, -1]] == 0.0) and np.all(v[[0, -1], :] == 0.0)
+    assert np.abs(u).max() > 0.1
+
+
+def test_dt_formula():
+    # main.cpp:284-288: dt = cfl / (2 * sum 1/dx^2)
+    assert W.dt_from_cfl(128, 0.5) == 0.5 / (4 * 128 ** 2)
+    assert W.dt_from_cfl(32, 3.0, fixed_dt=1e-3) == 1e-3
+
+
+def test_lid_profile_is_smooth_and_seeded():
+    g = W.lid_profile(512)
+    assert g.shape == (512,) and abs(g.mean() - 1.0) < 0.02
+    assert np.abs(np.diff(g)).max() < 0.05
+    assert np.array_equal(g, W.lid_profile(512))

@@ -196,16 +196,16 @@ int poisson_loop(amressif_ctx *ctx)
     const int mixed = c.in.precision == AMRESSIF_MIXED;
     int nb = 0;
     HIPCHK(amr::launch_mg_start(c));
-    HIPCHK(amr::launch_fill(c, c.phi, (int64_t)(c.g.n + 1) * c.g.P, 0.0));
-    HIPCHK(amr::launch_residual(c, c.phi, c.b, c.lev[0].r, &nb));
+    HIPCHK(amr::launch_fill(c, c.phiv[0], (int64_t)(c.g.n + 1) * c.g.P, 0.0));
+    HIPCHK(amr::launch_correct_residual(c, c.phiv[0], nullptr, 0, mixed, &nb));  // r = b, ||b||
     HIPCHK(amr::launch_fin_resid(c, nb));
     int launched = 0, target = std::max(1, c.last_V);
     const int cap = std::max(1, c.in.mg_max_iter);
     for (;;) {
         while (launched < target && launched < cap) {
+            const int par = launched & 1;  // executed cycles always follow this order
             HIPCHK(amr::launch_vcycle(c, mixed));
-            HIPCHK(amr::launch_add_correction(c, c.phi, mixed));
-            HIPCHK(amr::launch_residual(c, c.phi, c.b, c.lev[0].r, &nb));
+            HIPCHK(amr::launch_correct_residual(c, c.phiv[par], c.phiv[par ^ 1], 1, mixed, &nb));
             HIPCHK(amr::launch_fin_resid(c, nb));
             ++launched;
         }
@@ -214,34 +214,28 @@ int poisson_loop(amressif_ctx *ctx)
         target = launched + 2;
     }
     c.last_V = std::max(1, c.ctl_host->mg_iters);
+    c.phi = c.phiv[c.ctl_host->mg_iters & 1];
     return 0;
 }
 
-// One RK4 pseudo-iteration (four guarded stage launches).  FP64: stencil
+// One RK4 pseudo-iteration.  FP64: four guarded stage launches; the stencil
 // input ping-pongs U -> S1 -> S2 -> S1 and the last stage updates U (the
-// pseudo base u^k) in place.  Mixed: increments ping-pong dA/dB and the FP32
-// u^k ping-pongs by the host-side iteration index (a guarded no-op iteration
-// writes nothing, so executed iterations always follow that order).
+// pseudo base u^k) in place.  Mixed: the FP64 residual of U, then the FP32
+// tangent-linear update of U in place (first order in the RK4 recursion).
 int launch_pseudo_iteration(amressif_ctx *ctx, int it, int *nb)
 {
     Ctx &c = ctx->c;
+    (void)it;
     if (c.in.precision == AMRESSIF_MIXED) {
-        const int par = it & 1;
         amr::MixedStageLaunch a{};
-        a.uk = c.uk32[par][0]; a.vk = c.uk32[par][1];
-        a.wu = c.w32[0]; a.wv = c.w32[1];
-        a.su = c.s32[0]; a.sv = c.s32[1];
-        a.ukn = c.uk32[par ^ 1][0]; a.vkn = c.uk32[par ^ 1][1];
-        const float *din_u[4] = {nullptr, c.dA32[0], c.dB32[0], c.dA32[0]};
-        const float *din_v[4] = {nullptr, c.dA32[1], c.dB32[1], c.dA32[1]};
-        float *dout_u[4] = {c.dA32[0], c.dB32[0], c.dA32[0], c.dB32[0]};
-        float *dout_v[4] = {c.dA32[1], c.dB32[1], c.dA32[1], c.dB32[1]};
-        for (int sub = 0; sub < 4; ++sub) {
-            a.ud = din_u[sub]; a.vd = din_v[sub]; a.dou = dout_u[sub]; a.dov = dout_v[sub];
-            a.rk = c.rk[sub];
-            a.mode = sub == 3 ? 1 : 0;
-            HIPCHK(amr::launch_stage_mixed(c, a, nb));
-        }
+        a.u64 = c.U[0]; a.v64 = c.U[1]; a.cu = c.C[0]; a.cv = c.C[1]; a.p = c.p;
+        a.uk = c.uk32[0]; a.vk = c.uk32[1]; a.ru = c.r32[0]; a.rv = c.r32[1];
+        a.cj = c.rk[2] * c.rk[3];
+        a.cr = c.rk[3] - a.cj * (1.5 / c.g.dt);
+        a.stage = 0;
+        HIPCHK(amr::launch_stage_mixed(c, a, nb));
+        a.stage = 1;
+        HIPCHK(amr::launch_stage_mixed(c, a, nb));
         return 0;
     }
     amr::StageLaunch a{};
@@ -454,11 +448,11 @@ int amressif_create(const amressif_inputs *in, void *hip_stream, amressif_ctx **
         ALLOC(&c.U[d], fsz); ALLOC(&c.Uprev[d], fsz); ALLOC(&c.C[d], fsz);
         ALLOC(&c.S1[d], fsz); ALLOC(&c.S2[d], fsz);
     }
-    ALLOC(&c.p, fsz); ALLOC(&c.phi, fsz); ALLOC(&c.b, fsz);
+    ALLOC(&c.p, fsz); ALLOC(&c.phiv[0], fsz); ALLOC(&c.phiv[1], fsz); ALLOC(&c.b, fsz);
+    c.phi = c.phiv[0];
     if (in->precision == AMRESSIF_MIXED)
         for (int d = 0; d < 2; ++d) {
-            ALLOC(&c.uk32[0][d], fsz); ALLOC(&c.uk32[1][d], fsz); ALLOC(&c.w32[d], fsz);
-            ALLOC(&c.dA32[d], fsz); ALLOC(&c.dB32[d], fsz); ALLOC(&c.s32[d], fsz);
+            ALLOC(&c.uk32[d], fsz); ALLOC(&c.r32[d], fsz);
         }
     // multigrid hierarchy (level 0 shares the fine pitch)
     {
@@ -471,7 +465,12 @@ int amressif_create(const amressif_inputs *in, void *hip_stream, amressif_ctx **
             L.pitch = l == 0 ? g.P : round_up(nx, nx >= 64 ? 64 : 16);
             L.ax = 1.0 / (hx * hx); L.ay = 1.0 / (hy * hy);
             const size_t lsz = (size_t)L.pitch * (ny + 1);
-            ALLOC(&L.e, lsz); ALLOC(&L.r, lsz); ALLOC(&L.e32, lsz); ALLOC(&L.r32, lsz);
+            if (in->precision == AMRESSIF_MIXED) {
+                ALLOC(&L.e32, lsz); ALLOC(&L.e32b, lsz); ALLOC(&L.r32, lsz);
+                if (l == 0) { ALLOC(&L.e, lsz); ALLOC(&L.r, lsz); }  // diagnostics only
+            } else {
+                ALLOC(&L.e, lsz); ALLOC(&L.e2, lsz); ALLOC(&L.r, lsz);
+            }
             ++l;
             if (nx > in->mg_bottom_max && ny > in->mg_bottom_max && nx % 2 == 0 && ny % 2 == 0 && l < amr::kMaxLevels) {
                 nx /= 2; ny /= 2; hx *= 2; hy *= 2;
@@ -488,6 +487,7 @@ int amressif_create(const amressif_inputs *in, void *hip_stream, amressif_ctx **
         hipError_t e1 = hipMemcpy(c.pinv, pinv.data(), pinv.size() * 8, hipMemcpyHostToDevice);
         hipError_t e2 = hipMemcpy(c.pinv32, pinv32.data(), pinv32.size() * 4, hipMemcpyHostToDevice);
         if (e1 != hipSuccess || e2 != hipSuccess) return bail(AMRESSIF_ERR_HIP, "pinv upload failed");
+        if (amr::configure_tail(c) != hipSuccess) return bail(AMRESSIF_ERR_UNSUPPORTED, "multigrid tail does not fit LDS");
     }
     ALLOC(&c.partials, (size_t)amr::kMaxPartials * 4);
     ALLOC(&c.ctl, 1);
@@ -504,7 +504,8 @@ int amressif_create(const amressif_inputs *in, void *hip_stream, amressif_ctx **
         hipMemset(c.Uprev[d], 0, fsz * 8);
     }
     hipMemset(c.p, 0, fsz * 8);
-    hipMemset(c.phi, 0, fsz * 8);
+    hipMemset(c.phiv[0], 0, fsz * 8);
+    hipMemset(c.phiv[1], 0, fsz * 8);
     if (hipDeviceSynchronize() != hipSuccess) return bail(AMRESSIF_ERR_HIP, "device sync failed");
 #undef ALLOC
     *out = ctx;
@@ -606,8 +607,7 @@ int amressif_advance(amressif_ctx *ctx, int nsteps, amressif_step_stats *stats)
     const bool mixed = c.in.precision == AMRESSIF_MIXED;
     for (int s = 0; s < nsteps; ++s) {
         HIPCHK(amr::launch_ctl_reset_step(c));
-        if (mixed) HIPCHK(amr::launch_bdf2_mixed(c));
-        else HIPCHK(amr::launch_bdf2(c));
+        HIPCHK(amr::launch_bdf2(c));
         // ---- momentum: RK4 dual-time pseudo-iterations (main.cpp:898-1040)
         int launched = 0, target = std::max(1, c.last_P), nb = 0;
         const int cap = c.in.IterNum + 1;
@@ -624,13 +624,11 @@ int amressif_advance(amressif_ctx *ctx, int nsteps, amressif_step_stats *stats)
         c.last_P = std::max(1, c.ctl_host->mom_count);
         const int flags = c.ctl_host->flags;
         // ---- projection: Poisson RHS, MLMG-equivalent solve, update
-        // mixed: u* = u^n + S (FP64 + FP32 accumulated change)
-        const float *au = mixed ? c.s32[0] : nullptr, *av = mixed ? c.s32[1] : nullptr;
-        HIPCHK(amr::launch_poisson_rhs(c, c.U[0], c.U[1], c.b, &nb, au, av));
+        HIPCHK(amr::launch_poisson_rhs(c, c.U[0], c.U[1], c.b, &nb));
         HIPCHK(amr::launch_fin_prhs(c, nb));
         HIPCHK(amr::launch_center_b(c, c.b));
         if (int rc = poisson_loop(ctx)) return rc;
-        HIPCHK(amr::launch_project(c, c.U[0], c.U[1], c.S1[0], c.S1[1], &nb, au, av));
+        HIPCHK(amr::launch_project(c, c.U[0], c.U[1], c.S1[0], c.S1[1], &nb));
         HIPCHK(amr::launch_fin_project(c, nb, s));
         std::swap(c.U[0], c.S1[0]);
         std::swap(c.U[1], c.S1[1]);
@@ -821,16 +819,21 @@ int amressif_time_kernel(amressif_ctx *ctx, int which, int reps, int64_t copy_by
         HIPCHK(hipMalloc(&src, (size_t)copy_bytes));
         HIPCHK(hipMalloc(&dst, (size_t)copy_bytes));
         HIPCHK(hipMemsetAsync(src, 0, (size_t)copy_bytes, c.stream));
-    } else if (which != 0 && which != 1) {
-        return fail(ctx, AMRESSIF_ERR_ARG, "which must be 0, 1 or 2");
+    } else if (which < 0 || which > 4) {
+        return fail(ctx, AMRESSIF_ERR_ARG, "which must be 0..4");
     }
+    if (which >= 3 && c.in.precision != AMRESSIF_MIXED)
+        return fail(ctx, AMRESSIF_ERR_ARG, "which = 3, 4 need a mixed-precision context");
     auto launch = [&]() -> hipError_t {
-        if (which == 0 && c.in.precision == AMRESSIF_MIXED) {
-            amr::MixedStageLaunch a{};  // a middle stage: reads u^k, d, w; writes d
-            a.uk = c.uk32[0][0]; a.vk = c.uk32[0][1]; a.ud = c.dA32[0]; a.vd = c.dA32[1];
-            a.dou = c.dB32[0]; a.dov = c.dB32[1]; a.wu = c.w32[0]; a.wv = c.w32[1];
-            a.rk = c.rk[1];
-            a.mode = 0;
+        if (which == 3 || which == 4) {  // the two kernels of a mixed pseudo-iteration
+            amr::MixedStageLaunch a{};
+            a.stage = which - 3;
+            a.u64 = which == 3 ? c.U[0] : c.S2[0];  // the residual only reads it
+            a.v64 = which == 3 ? c.U[1] : c.S2[1];
+            a.cu = c.C[0]; a.cv = c.C[1]; a.p = c.p;
+            a.uk = c.uk32[0]; a.vk = c.uk32[1]; a.ru = c.r32[0]; a.rv = c.r32[1];
+            a.cj = 0.0;  // keeps the timed in-place update of S2 a no-op in value
+            a.cr = 0.0;
             a.unguarded = 1;
             int nb = 0;
             return amr::launch_stage_mixed(c, a, &nb);
@@ -846,7 +849,7 @@ int amressif_time_kernel(amressif_ctx *ctx, int which, int reps, int64_t copy_by
             int nb = 0;
             return amr::launch_stage(c, a, &nb);
         }
-        if (which == 1) return amr::launch_gsrb_fine(c, 0);
+        if (which == 1) return amr::launch_smooth_fine(c);
         return amr::launch_copy_bytes(c, src, dst, copy_bytes);
     };
     HIPCHK(launch());  // warm-up
@@ -865,12 +868,16 @@ int amressif_time_kernel(amressif_ctx *ctx, int which, int reps, int64_t copy_by
     if (dst) hipFree(dst);
     *avg_ms = (double)ms / reps;
     if (bytes_per_launch) {
-        // stage: reads u*, v*, u^k, v^k, c_u, c_v, p and writes u*, v* (9 arrays x 8 B per cell);
-        // colour sweep: half the cells read e(5 pts, counted once), r and write e -> 12 B per cell;
-        // copy: read + write
-        // mixed middle stage: reads u^k, d, w and writes d in FP32 (4 arrays x 2 dirs x 4 B per cell)
-        const double stage = c.in.precision == AMRESSIF_MIXED ? 32.0 : 72.0;
-        *bytes_per_launch = which == 0 ? stage * cells : which == 1 ? 12.0 * cells : 2.0 * (double)copy_bytes;
+        // algorithmic bytes per cell:
+        // 0 FP64 stage: reads u*, v*, u^k, v^k, c_u, c_v, p, writes u*, v* (9 x 8 B);
+        // 1 fused fine smoother from zero: reads r, writes e (2 x 8 B);
+        // 2 copy: read + write of copy_bytes;
+        // 3 mixed residual: reads u^k, v^k, c_u, c_v, p (5 x 8 B), writes the FP32 base and
+        //   residual (4 x 4 B);
+        // 4 tangent-linear update: reads the FP32 base and residual (4 x 4 B), reads and
+        //   writes u^k, v^k (4 x 8 B)
+        static const double per_cell[5] = {72.0, 16.0, 0.0, 56.0, 48.0};
+        *bytes_per_launch = which == 2 ? 2.0 * (double)copy_bytes : per_cell[which] * cells;
     }
     return AMRESSIF_OK;
 }

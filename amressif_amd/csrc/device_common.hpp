@@ -44,19 +44,22 @@ __device__ __forceinline__ R quick_flux(R vel, R a2, R a1, R b0, R b1)
 }
 
 // Wall ghost rule of cont2cart (utilities.cpp:87-269) for one non-corner ghost
-// cell from its mirror cell's values.  side: 0 W, 1 E, 2 S, 3 N.
-template <class R>
+// cell from its mirror cell's values.  side: 0 W, 1 E, 2 S, 3 N.  Every rule
+// is affine, ghost = +-mirror + constant; Homog = true drops the constant (the
+// rule applied to an increment of the velocity).
+template <class R, bool Homog = false>
 __device__ __forceinline__ void ghost_rule(int side, int code, R um, R vm, R gx, R gy, R &U, R &V)
 {
+    const R two = Homog ? R(0) : R(2);
     if (code == 135) { U = -um; V = -vm; return; }
     if (side == 0) {  // W: -135 slip, +-165 unit inflow
-        if (code == -135) { U = -um; V = vm; } else { U = R(2) - um; V = R(2) - vm; }
+        if (code == -135) { U = -um; V = vm; } else { U = two - um; V = two - vm; }
     } else if (side == 1) {  // E: the slip branch tests the code 1; 165 unit inflow
-        if (code == 1) { U = -um; V = vm; } else { U = R(2) - um; V = R(2) - vm; }
+        if (code == 1) { U = -um; V = vm; } else { U = two - um; V = two - vm; }
     } else if (side == 2) {  // S
-        if (code == -135) { U = um; V = -vm; } else { U = R(2) - um; V = R(2) - vm; }
+        if (code == -135) { U = um; V = -vm; } else { U = two - um; V = two - vm; }
     } else {  // N: 165 is the lid, 2 * inflow - mirror
-        if (code == -135) { U = um; V = -vm; } else { U = R(2) * gx - um; V = R(2) * gy - vm; }
+        if (code == -135) { U = um; V = -vm; } else { U = two * gx - um; V = two * gy - vm; }
     }
 }
 

@@ -37,8 +37,8 @@ struct Ctl {
 struct Level {
     int nx, ny, pitch;
     double ax, ay;   // 1/dx^2, 1/dy^2
-    double *e, *r;   // FP64 correction and right-hand side
-    float *e32, *r32; // FP32 copies for the mixed V-cycle
+    double *e, *e2, *r;   // FP64 correction (pre- and post-smoothing buffers) and right-hand side
+    float *e32, *e32b, *r32; // the same in FP32 for the mixed V-cycle
 };
 
 struct Geometry {
@@ -59,10 +59,12 @@ struct Ctx {
     double rk[4]{};
     // faces: [0] = x-faces (u), [1] = y-faces (v)
     double *U[2]{}, *Uprev[2]{}, *C[2]{}, *S1[2]{}, *S2[2]{};
-    // mixed-precision momentum state (FP32 faces): u^k ping-pong, w, stage
-    // increments and the step's accumulated change
-    float *uk32[2][2]{}, *w32[2]{}, *dA32[2]{}, *dB32[2]{}, *s32[2]{};
+    // mixed-precision momentum state (FP32 faces): copy of the pseudo base and
+    // its residual
+    float *uk32[2]{}, *r32[2]{};
     double *p = nullptr, *phi = nullptr, *b = nullptr;
+    double *phiv[2]{};   // Poisson iterates (ping-pong); phi points at the last solution
+    int tail_level = 0;  // first multigrid level handled by the single-workgroup tail kernel
     const double *lid = nullptr;  // caller-owned device lid profile or null
     Level lev[kMaxLevels]{};
     int nlev = 0;
@@ -96,20 +98,17 @@ struct StageLaunch {
     int unguarded;          // 1: ignore the control block (diagnostic timing)
 };
 hipError_t launch_stage(const Ctx &c, const StageLaunch &a, int *nblocks);
-// mixed stage on the increment d = u* - u^k (FP32): see kernels_stage.hip
+// mixed (defect-correction) pseudo-iteration: see kernels_stage.hip
 struct MixedStageLaunch {
-    const float *uk, *vk;    // u^k in FP32 (stencil base)
-    const float *ud, *vd;    // previous stage's increment (null on stage 1)
-    float *dou, *dov;        // increment out
-    float *wu, *wv;          // w = c - G p - (1.5/dt) u^k (last stage updates it)
-    float *su, *sv;          // accumulated change of the step (last stage adds d)
-    float *ukn, *vkn;        // last stage: u^{k+1} in FP32
-    double rk;
-    int mode;                // 0 stage, 1 last stage (+ norm partials)
+    int stage;                   // 0: FP64 residual of the base; 1: FP32 tangent-linear update
+    double *u64, *v64;           // FP64 pseudo base u^k (stage 1 updates it in place)
+    const double *cu, *cv, *p;   // stage 0: BDF2 constant and pressure
+    float *uk, *vk;              // FP32 copy of the base (written by stage 0)
+    float *ru, *rv;              // FP64 residual of the base, stored FP32 (written by stage 0)
+    double cr, cj;               // stage 1: u^{k+1} = u^k + cr r + cj R'(u^k) r
     int unguarded;
 };
 hipError_t launch_stage_mixed(const Ctx &c, const MixedStageLaunch &a, int *nblocks);
-hipError_t launch_bdf2_mixed(const Ctx &c);
 hipError_t launch_fin_pseudo(const Ctx &c, int nblocks);
 hipError_t launch_ctl_reset_step(const Ctx &c);
 hipError_t launch_bdf2(const Ctx &c);
@@ -118,10 +117,13 @@ hipError_t launch_poisson_rhs(const Ctx &c, const double *u, const double *v, do
 hipError_t launch_fin_prhs(const Ctx &c, int nblocks);
 hipError_t launch_center_b(const Ctx &c, double *b);
 hipError_t launch_mg_start(const Ctx &c);
-hipError_t launch_residual(const Ctx &c, const double *phi, const double *b, double *r, int *nblocks);
 hipError_t launch_fin_resid(const Ctx &c, int nblocks);
 hipError_t launch_vcycle(const Ctx &c, int mixed);
-hipError_t launch_add_correction(const Ctx &c, double *phi, int mixed);
+hipError_t launch_correct_residual(const Ctx &c, const double *phi_in, double *phi_out, int with_correction,
+                                   int mixed, int *nblocks);
+size_t tail_lds_bytes(const Ctx &c, int mixed);
+hipError_t configure_tail(Ctx &c);
+hipError_t launch_smooth_fine(const Ctx &c);
 hipError_t launch_project(const Ctx &c, const double *us, const double *vs, double *uo, double *vo, int *nblocks,
                           const float *au = nullptr, const float *av = nullptr);
 hipError_t launch_fin_project(const Ctx &c, int nblocks, int step_index);
@@ -142,7 +144,6 @@ hipError_t launch_project_ex(const Ctx &c, const double *us, const double *vs, d
 hipError_t launch_unpack_cell_shift(const Ctx &c, const double *src, const amr_array4 &a, int comp,
                                     int fill_ghosts, const double *shift);
 hipError_t launch_copy_bytes(const Ctx &c, const void *src, void *dst, int64_t bytes);
-hipError_t launch_gsrb_fine(const Ctx &c, int color);
 hipError_t launch_mean_b(const Ctx &c, const double *b);
 hipError_t launch_bdf2_const(const Ctx &c, const double *un, const double *um, double *cc, int dir);
 

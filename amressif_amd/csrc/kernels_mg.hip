@@ -6,61 +6,86 @@
 // mirror drops the out-of-domain neighbour from the stencil and from the
 // diagonal, so no ghost cells are stored.
 //
-// Solve (poisson.cpp:181-185): iterate phi += V(b - L phi) until
-// ||b - L phi||inf <= max(tol_abs, tol_rel ||b||inf) with b made solvable by
+// Solve (poisson.cpp:181-185): phi_{k+1} = phi_k + V(b - L phi_k) until
+// ||b - L phi||inf <= max(tol_abs, tol_rel ||b||inf), with b made solvable by
 // removing its mean.  V is one V-cycle on the correction from zero: two
 // red-black Gauss-Seidel smooths before and after, 2x2 average restriction,
-// piecewise-constant prolongation, and a dense pseudo-inverse solve on the
-// coarsest level.  The V-cycle is templated on its working precision: FP64,
-// or FP32 inside FP64 iterative refinement (FP64 residual, correction and
-// reductions).
+// piecewise-constant prolongation, dense pseudo-inverse on the coarsest
+// level.  V runs in T = double (FP64 step) or T = float (mixed step: FP32
+// V-cycle inside FP64 iterative refinement); the outer residual, the
+// correction, the norms and the mean stay FP64.
+//
+// Kernels per V-cycle: on each level larger than 64^2 one fused smoother
+// (four dependent half-sweeps in an LDS tile with a 4-cell halo) before and
+// one after (with the coarse correction added on load), and one fused
+// residual + restriction; the levels from 64^2 down, including the bottom
+// solve, run inside one workgroup (k_tail); then one kernel applies the
+// correction in FP64 and forms the next residual and its norm partials.
+#include "device_common.hpp"
 #include "internal.hpp"
 
 #include <cmath>
 
 namespace amr {
 
-template <bool IsMax>
-__device__ double block_reduce_mg(double v, double *sh)
-{
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    for (int o = 32; o > 0; o >>= 1) {
-        const double w = __shfl_down(v, o, 64);
-        if (IsMax) v = (w > v || w != w) ? w : v;
-        else v += w;
-    }
-    __syncthreads();
-    if (lane == 0) sh[wid] = v;
-    __syncthreads();
-    double r = 0.0;
-    if (threadIdx.x == 0) {
-        r = sh[0];
-        for (int w = 1; w < nw; ++w) {
-            if (IsMax) r = (sh[w] > r || sh[w] != sh[w]) ? sh[w] : r;
-            else r += sh[w];
-        }
-    }
-    return r;
-}
+// --------------------------------------------------------- fused smoother
+constexpr int SX = 64, SY = 16, SHALO = 4;
+constexpr int SW = SX + 2 * SHALO, SHT = SY + 2 * SHALO;  // 72 x 24 LDS region
 
-// ----------------------------------------------------------- smoothing
-// one colour of red-black Gauss-Seidel: colour c updates (i+j)%2 == c; a
-// thread owns one cell of that colour.
+// x_out = GSRB^2(x_in + P(ec)) for L x = r on one level.  x_in null: start
+// from zero; ec null: no coarse correction.  Red (i+j even) then black, twice.
 template <class T>
-__global__ __launch_bounds__(256) void k_gsrb(T *e, const T *r, int nx, int ny, int pitch, T ax, T ay, int color,
-                                              const Ctl *ctl)
+__global__ __launch_bounds__(256) void k_smooth(const T *xin, const T *ec, int pc, T *xout, const T *rhs, int nx,
+                                                int ny, int pitch, T ax, T ay, const Ctl *ctl)
 {
     if (ctl->mg_done) return;
-    const int t = blockIdx.x * 64 + (threadIdx.x & 63), j = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int i = 2 * t + ((j + color) & 1);
-    if (i >= nx || j >= ny) return;
-    const size_t k = i + (size_t)pitch * j;
-    T s = 0, d = 0;
-    if (i > 0) { s += ax * e[k - 1]; d += ax; }
-    if (i < nx - 1) { s += ax * e[k + 1]; d += ax; }
-    if (j > 0) { s += ay * e[k - pitch]; d += ay; }
-    if (j < ny - 1) { s += ay * e[k + pitch]; d += ay; }
-    e[k] = (s - r[k]) / d;
+    __shared__ T sx[SHT][SW], sr[SHT][SW];
+    const int tid = threadIdx.x;
+    const int i0 = blockIdx.x * SX, j0 = blockIdx.y * SY;
+    for (int e = tid; e < SW * SHT; e += 256) {
+        const int li = e % SW, lj = e / SW, i = i0 - SHALO + li, j = j0 - SHALO + lj;
+        T x = 0, r = 0;
+        if (i >= 0 && i < nx && j >= 0 && j < ny) {
+            const size_t k = i + (size_t)pitch * j;
+            if (xin) x = xin[k];
+            if (ec) x += ec[(i >> 1) + (size_t)pc * (j >> 1)];
+            r = rhs[k];
+        }
+        sx[lj][li] = x;
+        sr[lj][li] = r;
+    }
+    __syncthreads();
+    const T inv_d = T(1) / (T(2) * ax + T(2) * ay);
+    for (int h = 0; h < 4; ++h) {
+        const int color = h & 1;
+        const int lo = 1 + h, W = SW - 2 * lo, H = SHT - 2 * lo, half = W / 2;
+        for (int e = tid; e < half * H; e += 256) {
+            const int lj = lo + e / half, kk = e % half;
+            const int j = j0 - SHALO + lj;
+            // first column of this colour in the row
+            const int istart = i0 - SHALO + lo;
+            const int par = ((istart + j) & 1) == color ? 0 : 1;
+            const int li = lo + 2 * kk + par, i = i0 - SHALO + li;
+            if (li >= lo + W || i < 0 || i >= nx || j < 0 || j >= ny) continue;
+            T s = 0;
+            if (i > 0 && i < nx - 1 && j > 0 && j < ny - 1) {
+                s = ax * sx[lj][li - 1] + ax * sx[lj][li + 1] + ay * sx[lj - 1][li] + ay * sx[lj + 1][li];
+                sx[lj][li] = (s - sr[lj][li]) * inv_d;
+            } else {
+                T d = 0;
+                if (i > 0) { s += ax * sx[lj][li - 1]; d += ax; }
+                if (i < nx - 1) { s += ax * sx[lj][li + 1]; d += ax; }
+                if (j > 0) { s += ay * sx[lj - 1][li]; d += ay; }
+                if (j < ny - 1) { s += ay * sx[lj + 1][li]; d += ay; }
+                sx[lj][li] = (s - sr[lj][li]) / d;
+            }
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < SX * SY; e += 256) {
+        const int li = SHALO + (e % SX), lj = SHALO + (e / SX), i = i0 + (e % SX), j = j0 + (e / SX);
+        if (i < nx && j < ny) xout[i + (size_t)pitch * j] = sx[lj][li];
+    }
 }
 
 template <class T>
@@ -75,8 +100,8 @@ __device__ __forceinline__ T lap_at(const T *e, int i, int j, int nx, int ny, in
     return (E + W - T(2) * P) * ax + (N + S - T(2) * P) * ay;
 }
 
-// residual of the correction equation on the fine level, averaged over each
-// 2x2 block into the coarse right-hand side
+// residual of the correction equation, averaged over each 2x2 block into the
+// coarse right-hand side
 template <class T>
 __global__ __launch_bounds__(256) void k_resid_restrict(const T *e, const T *r, int nx, int ny, int pf, T ax, T ay,
                                                         T *rc, int ncx, int ncy, int pc, const Ctl *ctl)
@@ -92,118 +117,140 @@ __global__ __launch_bounds__(256) void k_resid_restrict(const T *e, const T *r, 
     rc[I + (size_t)pc * J] = T(0.25) * (((t00 + t10) + t01) + t11);
 }
 
-template <class T>
-__global__ __launch_bounds__(256) void k_prolong_add(T *ef, int nx, int ny, int pf, const T *ec, int pc,
-                                                     const Ctl *ctl)
-{
-    if (ctl->mg_done) return;
-    const int i = blockIdx.x * 64 + (threadIdx.x & 63), j = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (i >= nx || j >= ny) return;
-    ef[i + (size_t)pf * j] += ec[(i >> 1) + (size_t)pc * (j >> 1)];
-}
+// ------------------------------------------------------ single-block tail
+struct TailLevels {
+    int nlev;          // levels in the tail (the last one is the bottom)
+    int nx[12], ny[12];
+    int off[12];       // offset of each level in the LDS arrays
+    double ax[12], ay[12];
+    int total;         // cells over all tail levels
+    int mb;            // bottom cells
+};
 
 template <class T>
-__global__ __launch_bounds__(256) void k_zero_level(T *e, int nx, int ny, int pitch, const Ctl *ctl)
+__device__ void tail_gsrb(T *E, const T *Rr, int nx, int ny, T ax, T ay, int color)
 {
-    if (ctl->mg_done) return;
-    const int i = blockIdx.x * 64 + (threadIdx.x & 63), j = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (i >= nx || j >= ny) return;
-    e[i + (size_t)pitch * j] = T(0);
-}
-
-// coarsest level: e = pinv(L) r, pinv precomputed on the host (mean-zero
-// least-squares solution of the singular Neumann problem)
-template <class T>
-__global__ __launch_bounds__(256) void k_bottom(const T *pinv, const T *r, int nx, int ny, int pitch, T *e,
-                                                const Ctl *ctl)
-{
-    if (ctl->mg_done) return;
-    __shared__ T sr[1024];
-    const int m = nx * ny;
-    for (int k = threadIdx.x; k < m; k += blockDim.x) sr[k] = r[(k % nx) + (size_t)pitch * (k / nx)];
-    __syncthreads();
-    for (int row = threadIdx.x; row < m; row += blockDim.x) {
+    const T inv_d = T(1) / (T(2) * ax + T(2) * ay);
+    const int half = (nx + 1) / 2;
+    for (int e = threadIdx.x; e < half * ny; e += blockDim.x) {
+        const int j = e / half, i = 2 * (e % half) + ((j + color) & 1);
+        if (i >= nx) continue;
+        const int k = i + nx * j;
         T s = 0;
-        const T *pr = pinv + (size_t)row * m;
-        for (int k = 0; k < m; ++k) s += pr[k] * sr[k];
-        e[(row % nx) + (size_t)pitch * (row / nx)] = s;
+        if (i > 0 && i < nx - 1 && j > 0 && j < ny - 1) {
+            s = ax * E[k - 1] + ax * E[k + 1] + ay * E[k - nx] + ay * E[k + nx];
+            E[k] = (s - Rr[k]) * inv_d;
+        } else {
+            T d = 0;
+            if (i > 0) { s += ax * E[k - 1]; d += ax; }
+            if (i < nx - 1) { s += ax * E[k + 1]; d += ax; }
+            if (j > 0) { s += ay * E[k - nx]; d += ay; }
+            if (j < ny - 1) { s += ay * E[k + nx]; d += ay; }
+            E[k] = (s - Rr[k]) / d;
+        }
     }
+    __syncthreads();
 }
 
+// V-cycle of the coarse levels (all <= 64^2) in one workgroup: r of the top
+// tail level in, its correction out.
 template <class T>
-static hipError_t vcycle_impl(const Ctx &c, T *const *E, T *const *R, const T *pinv)
+__global__ __launch_bounds__(1024) void k_tail(const T *rtop, int ptop, T *etop, TailLevels tl, const T *pinv,
+                                               const Ctl *ctl)
 {
-    const Ctl *ctl = c.ctl;
-    for (int l = 0; l < c.nlev - 1; ++l) {
-        const Level &L = c.lev[l], &Cl = c.lev[l + 1];
-        const T ax = (T)L.ax, ay = (T)L.ay;
-        dim3 gc((L.nx + 63) / 64, (L.ny + 3) / 4), gh(((L.nx + 1) / 2 + 63) / 64, (L.ny + 3) / 4);
-        hipLaunchKernelGGL(k_zero_level<T>, gc, dim3(256), 0, c.stream, E[l], L.nx, L.ny, L.pitch, ctl);
-        for (int s = 0; s < 2; ++s)
-            for (int col = 0; col < 2; ++col)
-                hipLaunchKernelGGL(k_gsrb<T>, gh, dim3(256), 0, c.stream, E[l], (const T *)R[l], L.nx, L.ny, L.pitch,
-                                   ax, ay, col, ctl);
-        hipLaunchKernelGGL(k_resid_restrict<T>, dim3((Cl.nx + 63) / 64, (Cl.ny + 3) / 4), dim3(256), 0, c.stream,
-                           (const T *)E[l], (const T *)R[l], L.nx, L.ny, L.pitch, ax, ay, R[l + 1], Cl.nx, Cl.ny,
-                           Cl.pitch, ctl);
+    if (ctl->mg_done) return;
+    extern __shared__ unsigned char smem_raw[];
+    T *E = reinterpret_cast<T *>(smem_raw);
+    T *Rr = E + tl.total;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    {
+        const int nx = tl.nx[0], ny = tl.ny[0];
+        for (int k = tid; k < nx * ny; k += nt) Rr[k] = rtop[(k % nx) + (size_t)ptop * (k / nx)];
+    }
+    __syncthreads();
+    for (int l = 0; l < tl.nlev - 1; ++l) {
+        const int nx = tl.nx[l], ny = tl.ny[l];
+        T *El = E + tl.off[l];
+        const T *Rl = Rr + tl.off[l];
+        const T ax = (T)tl.ax[l], ay = (T)tl.ay[l];
+        for (int k = tid; k < nx * ny; k += nt) El[k] = T(0);
+        __syncthreads();
+        for (int s = 0; s < 2; ++s) { tail_gsrb(El, Rl, nx, ny, ax, ay, 0); tail_gsrb(El, Rl, nx, ny, ax, ay, 1); }
+        const int cx = tl.nx[l + 1], cy = tl.ny[l + 1];
+        T *Rc = Rr + tl.off[l + 1];
+        for (int k = tid; k < cx * cy; k += nt) {
+            const int I = k % cx, J = k / cx, i = 2 * I, j = 2 * J;
+            const T t00 = Rl[i + nx * j] - lap_at(El, i, j, nx, ny, nx, ax, ay);
+            const T t10 = Rl[i + 1 + nx * j] - lap_at(El, i + 1, j, nx, ny, nx, ax, ay);
+            const T t01 = Rl[i + nx * (j + 1)] - lap_at(El, i, j + 1, nx, ny, nx, ax, ay);
+            const T t11 = Rl[i + 1 + nx * (j + 1)] - lap_at(El, i + 1, j + 1, nx, ny, nx, ax, ay);
+            Rc[k] = T(0.25) * (((t00 + t10) + t01) + t11);
+        }
+        __syncthreads();
+    }
+    {  // bottom: e = pinv r (pinv symmetric: read it column-wise for coalescing)
+        const int b = tl.nlev - 1, m = tl.mb;
+        T *Eb = E + tl.off[b];
+        const T *Rb = Rr + tl.off[b];
+        for (int row = tid; row < m; row += nt) {
+            T s = 0;
+            for (int k = 0; k < m; ++k) s += pinv[(size_t)k * m + row] * Rb[k];
+            Eb[row] = s;
+        }
+        __syncthreads();
+    }
+    for (int l = tl.nlev - 2; l >= 0; --l) {
+        const int nx = tl.nx[l], ny = tl.ny[l], cx = tl.nx[l + 1];
+        T *El = E + tl.off[l];
+        const T *Ec = E + tl.off[l + 1];
+        const T *Rl = Rr + tl.off[l];
+        const T ax = (T)tl.ax[l], ay = (T)tl.ay[l];
+        for (int k = tid; k < nx * ny; k += nt) El[k] += Ec[((k % nx) >> 1) + cx * ((k / nx) >> 1)];
+        __syncthreads();
+        for (int s = 0; s < 2; ++s) { tail_gsrb(El, Rl, nx, ny, ax, ay, 0); tail_gsrb(El, Rl, nx, ny, ax, ay, 1); }
     }
     {
-        const Level &B = c.lev[c.nlev - 1];
-        hipLaunchKernelGGL(k_bottom<T>, dim3(1), dim3(256), 0, c.stream, pinv, (const T *)R[c.nlev - 1], B.nx, B.ny,
-                           B.pitch, E[c.nlev - 1], ctl);
+        const int nx = tl.nx[0], ny = tl.ny[0];
+        for (int k = tid; k < nx * ny; k += nt) etop[(k % nx) + (size_t)ptop * (k / nx)] = E[k];
     }
-    for (int l = c.nlev - 2; l >= 0; --l) {
-        const Level &L = c.lev[l], &Cl = c.lev[l + 1];
-        const T ax = (T)L.ax, ay = (T)L.ay;
-        dim3 gc((L.nx + 63) / 64, (L.ny + 3) / 4), gh(((L.nx + 1) / 2 + 63) / 64, (L.ny + 3) / 4);
-        hipLaunchKernelGGL(k_prolong_add<T>, gc, dim3(256), 0, c.stream, E[l], L.nx, L.ny, L.pitch,
-                           (const T *)E[l + 1], Cl.pitch, ctl);
-        for (int s = 0; s < 2; ++s)
-            for (int col = 0; col < 2; ++col)
-                hipLaunchKernelGGL(k_gsrb<T>, gh, dim3(256), 0, c.stream, E[l], (const T *)R[l], L.nx, L.ny, L.pitch,
-                                   ax, ay, col, ctl);
+}
+
+// ------------------------------------------------ FP64 outer iteration
+// phi_out = phi_in + e (FP64; e null: no correction and no write),
+// r = b - L phi_out in FP64, stored in T for the next V-cycle; block
+// partials of ||r||inf and sum phi_out.
+template <class T>
+__global__ __launch_bounds__(256) void k_correct_residual(const double *phi_in, double *phi_out, const T *e,
+                                                          const double *b, T *r, int n, int P, double ax, double ay,
+                                                          const Ctl *ctl, double *partials)
+{
+    __shared__ double sh[16];
+    if (ctl->mg_done) return;
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63), j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    double m = 0.0, s = 0.0;
+    if (i < n && j < n) {
+        const size_t k = i + (size_t)P * j;
+        auto x = [&](size_t q) -> double { return e ? phi_in[q] + (double)e[q] : phi_in[q]; };
+        const double Pv = x(k);
+        const double Wv = i > 0 ? x(k - 1) : Pv;
+        const double Ev = i < n - 1 ? x(k + 1) : Pv;
+        const double Sv = j > 0 ? x(k - P) : Pv;
+        const double Nv = j < n - 1 ? x(k + P) : Pv;
+        const double rr = b[k] - ((Ev + Wv - 2.0 * Pv) * ax + (Nv + Sv - 2.0 * Pv) * ay);
+        if (e) phi_out[k] = Pv;
+        r[k] = (T)rr;
+        m = fabs(rr);
+        s = Pv;
     }
-    return hipGetLastError();
-}
-
-// diagnostic: one unguarded colour sweep on the finest level (FP64 arrays)
-__global__ __launch_bounds__(256) void k_gsrb_free(double *e, const double *r, int nx, int ny, int pitch, double ax,
-                                                   double ay, int color)
-{
-    const int t = blockIdx.x * 64 + (threadIdx.x & 63), j = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int i = 2 * t + ((j + color) & 1);
-    if (i >= nx || j >= ny) return;
-    const size_t k = i + (size_t)pitch * j;
-    double s = 0, d = 0;
-    if (i > 0) { s += ax * e[k - 1]; d += ax; }
-    if (i < nx - 1) { s += ax * e[k + 1]; d += ax; }
-    if (j > 0) { s += ay * e[k - pitch]; d += ay; }
-    if (j < ny - 1) { s += ay * e[k + pitch]; d += ay; }
-    e[k] = (s - r[k]) / d;
-}
-
-hipError_t launch_gsrb_fine(const Ctx &c, int color)
-{
-    const Level &L = c.lev[0];
-    dim3 gh(((L.nx + 1) / 2 + 63) / 64, (L.ny + 3) / 4);
-    hipLaunchKernelGGL(k_gsrb_free, gh, dim3(256), 0, c.stream, L.e, (const double *)L.r, L.nx, L.ny, L.pitch, L.ax,
-                       L.ay, color);
-    return hipGetLastError();
-}
-
-hipError_t launch_vcycle(const Ctx &c, int mixed)
-{
-    if (mixed) {
-        float *E[kMaxLevels], *R[kMaxLevels];
-        for (int l = 0; l < c.nlev; ++l) { E[l] = c.lev[l].e32; R[l] = c.lev[l].r32; }
-        return vcycle_impl<float>(c, E, R, c.pinv32);
+    m = block_reduce<true>(m, sh);
+    s = block_reduce<false>(s, sh + 8);
+    if (threadIdx.x == 0) {
+        const size_t bid = blockIdx.x + (size_t)gridDim.x * blockIdx.y;
+        partials[2 * bid] = m;
+        partials[2 * bid + 1] = s;
     }
-    double *E[kMaxLevels], *R[kMaxLevels];
-    for (int l = 0; l < c.nlev; ++l) { E[l] = c.lev[l].e; R[l] = c.lev[l].r; }
-    return vcycle_impl<double>(c, E, R, c.pinv);
 }
 
-// ----------------------------------------------- outer FP64 iteration
 __global__ void k_mg_start(Ctl *ctl)
 {
     ctl->mg_done = 0;
@@ -221,44 +268,6 @@ hipError_t launch_mg_start(const Ctx &c)
     return hipGetLastError();
 }
 
-// r = b - L phi on the fine level (FP64), optionally rounded to FP32 for the
-// mixed V-cycle; block partials of ||r||inf and sum phi
-__global__ __launch_bounds__(256) void k_residual0(const double *phi, const double *b, double *r, float *r32, int n,
-                                                   int P, double ax, double ay, const Ctl *ctl, double *partials)
-{
-    __shared__ double sh[16];
-    if (ctl->mg_done) return;
-    const int i = blockIdx.x * 64 + (threadIdx.x & 63), j = blockIdx.y * 4 + (threadIdx.x >> 6);
-    double m = 0.0, s = 0.0;
-    if (i < n && j < n) {
-        const size_t k = i + (size_t)P * j;
-        const double rr = b[k] - lap_at(phi, i, j, n, n, P, ax, ay);
-        if (r) r[k] = rr;
-        if (r32) r32[k] = (float)rr;
-        m = fabs(rr);
-        s = phi[k];
-    }
-    m = block_reduce_mg<true>(m, sh);
-    s = block_reduce_mg<false>(s, sh + 8);
-    if (threadIdx.x == 0) {
-        const size_t bid = blockIdx.x + (size_t)gridDim.x * blockIdx.y;
-        partials[2 * bid] = m;
-        partials[2 * bid + 1] = s;
-    }
-}
-
-hipError_t launch_residual(const Ctx &c, const double *phi, const double *b, double *r, int *nblocks)
-{
-    const int n = c.g.n;
-    dim3 grid((n + 63) / 64, (n + 3) / 4);
-    if ((int64_t)grid.x * grid.y > kMaxPartials) return hipErrorInvalidValue;
-    *nblocks = grid.x * grid.y;
-    const bool mixed = c.in.precision == AMRESSIF_MIXED;
-    hipLaunchKernelGGL(k_residual0, grid, dim3(256), 0, c.stream, phi, b, mixed ? nullptr : r,
-                       mixed ? c.lev[0].r32 : nullptr, n, c.g.P, c.lev[0].ax, c.lev[0].ay, c.ctl, c.partials);
-    return hipGetLastError();
-}
-
 __global__ __launch_bounds__(1024) void k_fin_resid(const double *partials, int nb, Ctl *ctl, int n)
 {
     __shared__ double sh[32];
@@ -269,8 +278,8 @@ __global__ __launch_bounds__(1024) void k_fin_resid(const double *partials, int 
         m = (pm > m || pm != pm) ? pm : m;
         s += partials[2 * b + 1];
     }
-    m = block_reduce_mg<true>(m, sh);
-    s = block_reduce_mg<false>(s, sh + 16);
+    m = block_reduce<true>(m, sh);
+    s = block_reduce<false>(s, sh + 16);
     if (threadIdx.x == 0) {
         ctl->res_norm = m;
         ctl->phi_sum = s;
@@ -293,26 +302,144 @@ hipError_t launch_fin_resid(const Ctx &c, int nblocks)
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------ launchers
 template <class T>
-__global__ __launch_bounds__(256) void k_add_correction(double *phi, const T *e, int n, int P, const Ctl *ctl)
+static T *lev_e(const Level &L, int which);
+template <>
+double *lev_e<double>(const Level &L, int which) { return which ? L.e2 : L.e; }
+template <>
+float *lev_e<float>(const Level &L, int which) { return which ? L.e32b : L.e32; }
+template <class T>
+static T *lev_r(const Level &L);
+template <>
+double *lev_r<double>(const Level &L) { return L.r; }
+template <>
+float *lev_r<float>(const Level &L) { return L.r32; }
+
+static TailLevels tail_levels(const Ctx &c, int lt, size_t *lds_bytes, size_t elem)
 {
-    if (ctl->mg_done) return;
-    const int i = blockIdx.x * 64 + (threadIdx.x & 63), j = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (i >= n || j >= n) return;
-    const size_t k = i + (size_t)P * j;
-    phi[k] += (double)e[k];
+    TailLevels tl{};
+    tl.nlev = c.nlev - lt;
+    int off = 0;
+    for (int l = 0; l < tl.nlev; ++l) {
+        const Level &L = c.lev[lt + l];
+        tl.nx[l] = L.nx;
+        tl.ny[l] = L.ny;
+        tl.ax[l] = L.ax;
+        tl.ay[l] = L.ay;
+        tl.off[l] = off;
+        off += L.nx * L.ny;
+    }
+    tl.total = off;
+    tl.mb = c.mb;
+    *lds_bytes = 2 * (size_t)off * elem;
+    return tl;
 }
 
-hipError_t launch_add_correction(const Ctx &c, double *phi, int mixed)
+template <class T>
+static hipError_t vcycle_impl(const Ctx &c, const T *pinv)
+{
+    const Ctl *ctl = c.ctl;
+    const int lt = c.tail_level;
+    for (int l = 0; l < lt; ++l) {  // down: smooth from zero, residual + restriction
+        const Level &L = c.lev[l], &Cl = c.lev[l + 1];
+        const T ax = (T)L.ax, ay = (T)L.ay;
+        hipLaunchKernelGGL(k_smooth<T>, dim3((L.nx + SX - 1) / SX, (L.ny + SY - 1) / SY), dim3(256), 0, c.stream,
+                           (const T *)nullptr, (const T *)nullptr, 0, lev_e<T>(L, 0), (const T *)lev_r<T>(L), L.nx,
+                           L.ny, L.pitch, ax, ay, ctl);
+        hipLaunchKernelGGL(k_resid_restrict<T>, dim3((Cl.nx + 63) / 64, (Cl.ny + 3) / 4), dim3(256), 0, c.stream,
+                           (const T *)lev_e<T>(L, 0), (const T *)lev_r<T>(L), L.nx, L.ny, L.pitch, ax, ay,
+                           lev_r<T>(Cl), Cl.nx, Cl.ny, Cl.pitch, ctl);
+    }
+    {
+        size_t lds = 0;
+        const TailLevels tl = tail_levels(c, lt, &lds, sizeof(T));
+        const Level &Lt = c.lev[lt];
+        hipLaunchKernelGGL(k_tail<T>, dim3(1), dim3(1024), lds, c.stream, (const T *)lev_r<T>(Lt), Lt.pitch,
+                           lev_e<T>(Lt, 1), tl, pinv, ctl);
+    }
+    for (int l = lt - 1; l >= 0; --l) {  // up: add the coarse correction, smooth
+        const Level &L = c.lev[l], &Cl = c.lev[l + 1];
+        const T ax = (T)L.ax, ay = (T)L.ay;
+        hipLaunchKernelGGL(k_smooth<T>, dim3((L.nx + SX - 1) / SX, (L.ny + SY - 1) / SY), dim3(256), 0, c.stream,
+                           (const T *)lev_e<T>(L, 0), (const T *)lev_e<T>(Cl, 1), Cl.pitch, lev_e<T>(L, 1),
+                           (const T *)lev_r<T>(L), L.nx, L.ny, L.pitch, ax, ay, ctl);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_vcycle(const Ctx &c, int mixed)
+{
+    return mixed ? vcycle_impl<float>(c, c.pinv32) : vcycle_impl<double>(c, c.pinv);
+}
+
+size_t tail_lds_bytes(const Ctx &c, int mixed)
+{
+    size_t lds = 0;
+    tail_levels(c, c.tail_level, &lds, mixed ? sizeof(float) : sizeof(double));
+    return lds;
+}
+
+// phi_out = phi_in + e_0 (e_0 = the V-cycle's correction; with e null the
+// residual of phi_in alone), then r_0 = b - L phi_out and the norm partials
+hipError_t launch_correct_residual(const Ctx &c, const double *phi_in, double *phi_out, int with_correction,
+                                   int mixed, int *nblocks)
 {
     const int n = c.g.n;
     dim3 grid((n + 63) / 64, (n + 3) / 4);
+    if ((int64_t)grid.x * grid.y > kMaxPartials) return hipErrorInvalidValue;
+    *nblocks = grid.x * grid.y;
+    const Level &L = c.lev[0];
     if (mixed)
-        hipLaunchKernelGGL(k_add_correction<float>, grid, dim3(256), 0, c.stream, phi, (const float *)c.lev[0].e32, n,
-                           c.g.P, c.ctl);
+        hipLaunchKernelGGL(k_correct_residual<float>, grid, dim3(256), 0, c.stream, phi_in, phi_out,
+                           with_correction ? (const float *)L.e32b : (const float *)nullptr, (const double *)c.b,
+                           L.r32, n, c.g.P, L.ax, L.ay, c.ctl, c.partials);
     else
-        hipLaunchKernelGGL(k_add_correction<double>, grid, dim3(256), 0, c.stream, phi, (const double *)c.lev[0].e, n,
-                           c.g.P, c.ctl);
+        hipLaunchKernelGGL(k_correct_residual<double>, grid, dim3(256), 0, c.stream, phi_in, phi_out,
+                           with_correction ? (const double *)L.e2 : (const double *)nullptr, (const double *)c.b,
+                           L.r, n, c.g.P, L.ax, L.ay, c.ctl, c.partials);
+    return hipGetLastError();
+}
+
+// Pick the first level handled by k_tail: the first level of at most 64^2
+// cells whose tail fits the workgroup's LDS (raising the dynamic-LDS limit
+// where the runtime asks for it); fall back to coarser starts.
+hipError_t configure_tail(Ctx &c)
+{
+    const int mixed = c.in.precision == AMRESSIF_MIXED;
+    int dev = 0;
+    hipGetDevice(&dev);
+    int max_lds = 65536;
+    hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    int lt = 0;
+    while (lt < c.nlev - 1 && c.lev[lt].nx > 64) ++lt;
+    for (; lt < c.nlev; ++lt) {
+        c.tail_level = lt;
+        const size_t bytes = tail_lds_bytes(c, mixed);
+        if (bytes > (size_t)max_lds) continue;
+        hipError_t e = mixed ? hipFuncSetAttribute((const void *)k_tail<float>,
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes)
+                             : hipFuncSetAttribute((const void *)k_tail<double>,
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+        if (e == hipSuccess || bytes <= 65536) {
+            (void)hipGetLastError();
+            return hipSuccess;
+        }
+        (void)hipGetLastError();
+    }
+    return hipErrorInvalidValue;
+}
+
+// diagnostic: one fused smoother on the finest level, unguarded (FP64)
+__global__ void k_ctl_clear_mg(Ctl *ctl) { ctl->mg_done = 0; }
+
+hipError_t launch_smooth_fine(const Ctx &c)
+{
+    const Level &L = c.lev[0];
+    hipLaunchKernelGGL(k_ctl_clear_mg, dim3(1), dim3(1), 0, c.stream, c.ctl);
+    hipLaunchKernelGGL(k_smooth<double>, dim3((L.nx + SX - 1) / SX, (L.ny + SY - 1) / SY), dim3(256), 0, c.stream,
+                       (const double *)nullptr, (const double *)nullptr, 0, L.e, (const double *)L.r, L.nx, L.ny,
+                       L.pitch, L.ax, L.ay, (const Ctl *)c.ctl);
     return hipGetLastError();
 }
 

@@ -8,38 +8,63 @@
 // (momentum.cpp:68-87), and on the last stage the pseudo-iteration norm
 // partials (utilities.cpp:421-493).
 //
-// Tiling: a block of 256 threads owns TX x TY = 64 x 16 cells.  Phase 1 builds
-// the cell velocities of the tile plus a 3-cell halo in LDS (ghost cells from
-// their mirror cell, so no ghost storage in HBM); phase 2 gives each thread a
-// 4-row strip of cells and computes the total flux T = -C + V there, with
-// the QUICK face fluxes evaluated from the LDS row/column windows; phase 3
-// reuses the LDS buffer for T and updates the faces.  Tiles are handed to the
-// 8 XCDs in contiguous horizontal bands so halo rows stay in one L2.
+// Work decomposition: one wavefront owns a band of 58 columns (64 lanes: the
+// band plus a 3-lane halo) and a chunk of rows, and marches up the rows.  A
+// rolling 5-row window of cell velocities stays in registers (the y-direction
+// QUICK stencil); x-direction neighbours come from adjacent lanes through DPP
+// wave shifts (no LDS, no barriers).  Each face flux is evaluated once; the
+// next row's global loads are issued one iteration ahead.  Walls: ghost cells
+// come from their mirror lane (x) or mirror row (y) with the cont2cart rules.
 //
-// Two arithmetic forms:
-//  * FP64 (the reference's): u* <- u^k + rk (R(u*) - (1.5/dt) u* + c) with the
+// Precision forms:
+//  * FP64 (the reference's): u* <- u^k + rk (R(u*) - (1.5/dt) u* + c), with the
 //    BDF2 constant c = (1.5/dt) u^n + (0.5/dt)(u^n - u^{n-1}) folded once per step.
-//  * mixed: the stage works on the increment d = u* - u^k in FP32,
-//    d <- rk (R_T(u^k + d) - (1.5/dt) d + w), w = c - G p - (1.5/dt) u^k,
-//    which is the same update rewritten; u^k is carried in FP32 for the stencil
-//    and the step's accumulated change S = u^k - u^n in FP32 (u^n stays FP64).
-#include <type_traits>
-
+//  * mixed (defect correction): per pseudo-iteration, k_stage64 in mode 3 forms
+//    the FP64 residual r = R(u^k) - (1.5/dt) u^k + c of the pseudo base and
+//    k_tangent applies the FP32 tangent-linear stencil R'(u^k) r (QUICK with
+//    the upwind switch frozen at u^k, viscous flux of r).  With J = R' - 1.5/dt
+//    the RK4 stage recursion (coefficients a1..a4) taken to first order in J
+//    gives u^{k+1} = u^k + a4 r + a3 a4 J r = u^k + cr r + cj R' r with
+//    cr = a4 - a3 a4 (1.5/dt), cj = a3 a4.  A converged state (r = 0) is a
+//    fixed point, so the answer is fixed by the FP64 residual.
 #include "device_common.hpp"
 #include "internal.hpp"
 
 namespace amr {
 
-constexpr int TX = 64, TY = 16, NT = 256, RPT = TY / 4;
-constexpr int CW = TX + 5, CH = TY + 5;  // cells: cols i0-3..i0+TX+1, rows j0-3..j0+TY+1
+constexpr int WB = 58;  // output columns per wave (64 lanes = 3 halo + 58 + 3 halo)
+constexpr int RH = 16;  // output rows per wave
+
+// values held by lane - 1 / lane + 1 (DPP wave shifts; the edge lanes get 0)
+__device__ __forceinline__ double lane_prev(double x)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), 0x138, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), 0x138, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double lane_next(double x)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), 0x130, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), 0x130, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float lane_prev(float x)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float lane_next(float x)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xF, 0xF, false));
+}
 
 struct StageArgs {
-    // FP64 form
+    // FP64 form: stencil input, pseudo base, BDF2 constant, pressure, outputs
     const double *us, *vs, *ub, *vb, *cu, *cv, *p;
     double *uo, *vo;
     // mixed form
-    const float *uk, *vk, *ud, *vd;
-    float *dou, *dov, *wu, *wv, *su, *sv, *ukn, *vkn;
+    float *uk, *vk, *ru, *rv;  // FP32 copy of the base and its FP64 residual (stored FP32)
+    double *u64, *v64;         // FP64 pseudo base, updated in place by k_tangent
+    double cr, cj;             // u^{k+1} = u^k + cr r + cj R'(u^k) r
     const double *lid;
     double *partials;
     const Ctl *ctl;
@@ -47,200 +72,308 @@ struct StageArgs {
     double inv_dx, inv_dy, inv_dx2, inv_dy2, inv_ren, c15, rk;
     int bc_lo0, bc_hi0, bc_lo1, bc_hi1;
     double inflow0, inflow1;
-    int mode;   // 0 stage, 1 last stage (+ norm partials), 2 RHS only (FP64)
-    int remap;  // XCD-banded tile order
+    int mode;  // k_stage64: 0 stage, 1 last stage (+ norm), 2 RHS only, 3 mixed residual
+    int nbands, nchunks, remap;
 };
 
-template <bool MIXED>
-__global__ __launch_bounds__(NT) void k_stage(StageArgs a)
+// wave id -> (band, chunk); blocks of 4 waves, handed to the 8 XCDs in
+// contiguous ranges so vertically adjacent chunks share an L2
+__device__ __forceinline__ int wave_id(const StageArgs &a)
 {
-    using R = typename std::conditional<MIXED, float, double>::type;
+    int blk = blockIdx.x;
+    if (a.remap) {
+        const int B = gridDim.x;
+        blk = (blk & 7) * (B >> 3) + (blk >> 3);
+    }
+    return blk * 4 + (threadIdx.x >> 6);
+}
+
+// cell velocities of row jr for this lane's column from raw face loads of the
+// mirror row, with the x ghosts from the mirror lane and the y ghost rule.
+// Homog: the rules without their constant part (an increment of the velocity).
+template <class R, bool Homog>
+__device__ __forceinline__ void finish_cells(const StageArgs &a, int lane, int i, int jr, bool edge, R u0, R u1,
+                                             R v0, R v1, R &U, R &V)
+{
+    const int n = a.n;
+    const int jm = jr < 0 ? -jr - 1 : (jr >= n ? 2 * n - 1 - jr : jr);
+    const bool xin = i >= 0 && i < n;
+    U = R(0.5) * (u0 + u1);
+    V = R(0.5) * (v0 + v1);
+    if (edge) {
+        const int mi = i < 0 ? -i - 1 : 2 * n - 1 - i;
+        int ml = lane + (mi - i);
+        ml = ml < 0 ? 0 : (ml > 63 ? 63 : ml);
+        const R Um = __shfl(U, ml, 64), Vm = __shfl(V, ml, 64);
+        if (!xin) {
+            R gu = 0, gv = 0;
+            if (i >= -2 && i <= n + 1)
+                ghost_rule<R, Homog>(i < 0 ? 0 : 1, i < 0 ? a.bc_lo0 : a.bc_hi0, Um, Vm, R(0), R(0), gu, gv);
+            U = gu;
+            V = gv;
+        }
+    }
+    if (jr != jm && xin) {
+        const R gx = (R)(a.lid ? a.lid[i] : a.inflow0);
+        R gu, gv;
+        ghost_rule<R, Homog>(jr < 0 ? 2 : 3, jr < 0 ? a.bc_lo1 : a.bc_hi1, U, V, gx, (R)a.inflow1, gu, gv);
+        U = gu;
+        V = gv;
+    }
+}
+
+// ============================================================ FP64 stage
+__global__ __launch_bounds__(256) void k_stage64(StageArgs a)
+{
+    using R = double;
     if (a.ctl && a.ctl->mom_done) return;
-    __shared__ R sbuf[2 * CH * CW];
-    __shared__ double sred[16];
-    R(*sU)[CW] = reinterpret_cast<R(*)[CW]>(sbuf);
-    R(*sV)[CW] = reinterpret_cast<R(*)[CW]>(sbuf + CH * CW);
-
-    const int n = a.n, P = a.P, tid = threadIdx.x;
-    int bx = blockIdx.x, by = blockIdx.y;
-    if (a.remap) {  // block b runs on XCD b % 8: give each XCD a contiguous band of tiles
-        const int B = gridDim.x * gridDim.y, b = bx + gridDim.x * by;
-        const int t = (b & 7) * (B >> 3) + (b >> 3);
-        bx = t % gridDim.x;
-        by = t / gridDim.x;
-    }
-    const int i0 = bx * TX, j0 = by * TY;
-
-    auto fu = [&](int i, int j) -> R {  // stencil-input x-face velocity
-        const size_t k = i + (size_t)P * j;
-        if constexpr (MIXED) return a.ud ? a.uk[k] + a.ud[k] : a.uk[k];
-        else return a.us[k];
+    const int lane = threadIdx.x & 63;
+    const int wid = wave_id(a);
+    if (wid >= a.nbands * a.nchunks) return;  // whole wave
+    const int band = wid / a.nchunks, chunk = wid % a.nchunks;
+    const int n = a.n, P = a.P;
+    const int i0 = band * WB, j0 = chunk * RH;
+    const int i = i0 + lane - 3;
+    const bool xin = i >= 0 && i < n, xface = i >= 0 && i <= n;
+    const bool edge = i0 < 3 || i0 + 61 > n;
+    const double *us = a.us, *vs = a.vs;
+    struct Raw {
+        R u0, u1, v0, v1, vn, uw, pc, ub, cu, vb, cv;
     };
-    auto fv = [&](int i, int j) -> R {
-        const size_t k = i + (size_t)P * j;
-        if constexpr (MIXED) return a.vd ? a.vk[k] + a.vd[k] : a.vk[k];
-        else return a.vs[k];
+    auto load = [&](int jc, Raw &r) {  // everything iteration jc consumes
+        const int jr = jc + 2;
+        const int jm = jr < 0 ? -jr - 1 : (jr >= n ? 2 * n - 1 - jr : jr);
+        r.u0 = r.u1 = r.v0 = r.v1 = 0;
+        if (jm >= 0 && jm < n && xin) {
+            const size_t k = i + (size_t)P * jm;
+            r.u0 = us[k]; r.u1 = us[k + 1]; r.v0 = vs[k]; r.v1 = vs[k + P];
+        }
+        r.vn = (xin && jc + 1 >= 0 && jc + 1 <= n) ? vs[i + (size_t)P * (jc + 1)] : R(0);
+        const bool row = jc >= 0 && jc < n;
+        const size_t k = (size_t)(i < 0 ? 0 : i) + (size_t)P * (jc < 0 ? 0 : jc);
+        r.uw = (xface && row) ? us[k] : R(0);
+        r.pc = (xin && row) ? a.p[k] : R(0);
+        const bool out = row && jc >= j0 && a.mode != 2;
+        r.ub = (out && xface && a.mode != 3) ? a.ub[k] : R(0);
+        r.cu = (out && xface) ? a.cu[k] : R(0);
+        r.vb = (out && xin && a.mode != 3) ? a.vb[k] : R(0);
+        r.cv = (out && xin) ? a.cv[k] : R(0);
     };
-
-    // ---- phase 1: cell velocities of tile + halo
-    for (int e = tid; e < CW * CH; e += NT) {
-        const int li = e % CW, lj = e / CW, ci = i0 - 3 + li, cj = j0 - 3 + lj;
-        const bool inx = ci >= 0 && ci < n, iny = cj >= 0 && cj < n;
-        R U = 0, V = 0;
-        if (inx && iny) {
-            U = R(0.5) * (fu(ci, cj) + fu(ci + 1, cj));
-            V = R(0.5) * (fv(ci, cj) + fv(ci, cj + 1));
-        } else if (!inx && iny && ci >= -2 && ci <= n + 1) {
-            const int mi = ci < 0 ? -ci - 1 : 2 * n - 1 - ci;
-            const R um = R(0.5) * (fu(mi, cj) + fu(mi + 1, cj));
-            const R vm = R(0.5) * (fv(mi, cj) + fv(mi, cj + 1));
-            ghost_rule<R>(ci < 0 ? 0 : 1, ci < 0 ? a.bc_lo0 : a.bc_hi0, um, vm, R(0), R(0), U, V);
-        } else if (inx && !iny && cj >= -2 && cj <= n + 1) {
-            const int mj = cj < 0 ? -cj - 1 : 2 * n - 1 - cj;
-            const R um = R(0.5) * (fu(ci, mj) + fu(ci + 1, mj));
-            const R vm = R(0.5) * (fv(ci, mj) + fv(ci, mj + 1));
-            const R gx = (R)(a.lid ? a.lid[ci] : a.inflow0);
-            ghost_rule<R>(cj < 0 ? 2 : 3, cj < 0 ? a.bc_lo1 : a.bc_hi1, um, vm, gx, (R)a.inflow1, U, V);
-        }
-        sU[lj][li] = U;
-        sV[lj][li] = V;
-    }
-    __syncthreads();
-
-    // ---- phase 2: total cell flux T = -C + V on a 4-row strip per thread
-    const R idx = (R)a.inv_dx, idy = (R)a.inv_dy, idx2 = (R)a.inv_dx2, idy2 = (R)a.inv_dy2, iren = (R)a.inv_ren;
-    auto cell_T = [&](int ci, int cj, R &TU, R &TV) {
-        const int li = ci - (i0 - 3), lj = cj - (j0 - 3);
-        const R uw = fu(ci, cj), ue = fu(ci + 1, cj), vsf = fv(ci, cj), vnf = fv(ci, cj + 1);
-        {
-            const R m2 = sU[lj][li - 2], m1 = sU[lj][li - 1], c0 = sU[lj][li], p1 = sU[lj][li + 1], p2 = sU[lj][li + 2];
-            const R s2 = sU[lj - 2][li], s1 = sU[lj - 1][li], n1 = sU[lj + 1][li], n2 = sU[lj + 2][li];
-            const R C = (quick_flux<R>(ue, m1, c0, p1, p2) - quick_flux<R>(uw, m2, m1, c0, p1)) * idx +
-                        (quick_flux<R>(vnf, s1, c0, n1, n2) - quick_flux<R>(vsf, s2, s1, c0, n1)) * idy;
-            const R Vf = ((p1 - R(2) * c0 + m1) * idx2 + (n1 - R(2) * c0 + s1) * idy2) * iren;
-            TU = (R(0) - C) + Vf;
-        }
-        {
-            const R m2 = sV[lj][li - 2], m1 = sV[lj][li - 1], c0 = sV[lj][li], p1 = sV[lj][li + 1], p2 = sV[lj][li + 2];
-            const R s2 = sV[lj - 2][li], s1 = sV[lj - 1][li], n1 = sV[lj + 1][li], n2 = sV[lj + 2][li];
-            const R C = (quick_flux<R>(ue, m1, c0, p1, p2) - quick_flux<R>(uw, m2, m1, c0, p1)) * idx +
-                        (quick_flux<R>(vnf, s1, c0, n1, n2) - quick_flux<R>(vsf, s2, s1, c0, n1)) * idy;
-            const R Vf = ((p1 - R(2) * c0 + m1) * idx2 + (n1 - R(2) * c0 + s1) * idy2) * iren;
-            TV = (R(0) - C) + Vf;
-        }
-    };
-    const int tx = tid & 63, ty = tid >> 6;
-    const int ci = i0 + tx;
-    R tU[RPT], tV[RPT];
+    const R idx = a.inv_dx, idy = a.inv_dy, idx2 = a.inv_dx2, idy2 = a.inv_dy2, iren = a.inv_ren;
+    R wU[5], wV[5];  // rolling window: w[q] = cell (i, jc - 2 + q)
+    {
+        Raw r;
 #pragma unroll
-    for (int k = 0; k < RPT; ++k) {
-        const int cj = j0 + ty * RPT + k;
-        tU[k] = 0;
-        tV[k] = 0;
-        if (ci < n && cj < n) cell_T(ci, cj, tU[k], tV[k]);
+        for (int q = 0; q < 4; ++q) {
+            load(j0 - 5 + q, r);
+            finish_cells<R, false>(a, lane, i, j0 - 3 + q, edge, r.u0, r.u1, r.v0, r.v1, wU[q], wV[q]);
+        }
     }
-    // extra strip: T_U of column i0-1 (threads 0..TY-1), T_V of row j0-1 (threads TY..TY+TX-1)
-    R xT = 0;
-    if (tid < TY) {
-        const int cj = j0 + tid;
-        R tu = 0, tv = 0;
-        if (i0 >= 1 && cj < n) cell_T(i0 - 1, cj, tu, tv);
-        xT = tu;
-    } else if (tid < TY + TX) {
-        const int cc = i0 + tid - TY;
-        R tu = 0, tv = 0;
-        if (j0 >= 1 && cc < n) cell_T(cc, j0 - 1, tu, tv);
-        xT = tv;
-    }
-    __syncthreads();  // every read of sU/sV is done: reuse the buffer for T
-    R(*sTU)[TX + 1] = reinterpret_cast<R(*)[TX + 1]>(sbuf);            // [TY][TX+1], col = ci - (i0-1)
-    R(*sTV)[TX] = reinterpret_cast<R(*)[TX]>(sbuf + TY * (TX + 1));    // [TY+1][TX], row = cj - (j0-1)
-#pragma unroll
-    for (int k = 0; k < RPT; ++k) {
-        sTU[ty * RPT + k][tx + 1] = tU[k];
-        sTV[ty * RPT + k + 1][tx] = tV[k];
-    }
-    if (tid < TY) sTU[tid][0] = xT;
-    else if (tid < TY + TX) sTV[0][tid - TY] = xT;
-    __syncthreads();
-
-    // ---- phase 3: face RHS and update.  Wall-normal faces get R = 0 exactly
-    //      (flux ghost = -mirror, pressure ghost = mirror).
+    R vn = (xin && j0 - 1 >= 0) ? vs[i + (size_t)P * (j0 - 1)] : R(0);  // own y-face of the row
+    R fsU = quick_flux<R>(vn, wU[0], wU[1], wU[2], wU[3]), fsV = quick_flux<R>(vn, wV[0], wV[1], wV[2], wV[3]);
+    R tVprev = 0, pprev = 0;
     double nx_ = 0.0, ny_ = 0.0;
-    const int i = ci;
-    auto x_face = [&](int fi, int fj, R RT) {
-        const size_t k = fi + (size_t)P * fj;
-        const bool wall = fi == 0 || fi == n;
-        if constexpr (!MIXED) {
-            double R_ = 0.0;
-            if (!wall) R_ = -(a.p[k] - a.p[k - 1]) * a.inv_dx + RT;
-            if (a.mode == 2) { a.uo[k] = R_; return; }
-            const double base = a.ub[k];
-            const double unew = base + a.rk * (R_ - a.c15 * a.us[k] + a.cu[k]);
-            a.uo[k] = unew;
-            if (a.mode == 1) { const double d = base - unew; nx_ += fabs(d * d * (wall ? 0.5 : 1.0)); }
-        } else {
-            const float dold = a.ud ? a.ud[k] : 0.f;
-            const float w = a.wu[k];
-            const float dn = (float)a.rk * ((wall ? 0.f : RT) - (float)a.c15 * dold + w);
-            a.dou[k] = dn;
-            if (a.mode == 1) {
-                a.ukn[k] = a.uk[k] + dn;
-                a.wu[k] = w - (float)a.c15 * dn;
-                a.su[k] += dn;
-                const double d = (double)dn;
-                nx_ += d * d * (wall ? 0.5 : 1.0);
+    const bool outlane = lane >= 3 && lane < 3 + WB;
+    const int jend = min(j0 + RH, n);
+    Raw cur, nxt;
+    load(j0 - 1, cur);
+    for (int jc = j0 - 1; jc < jend; ++jc) {
+        if (jc + 1 < jend) load(jc + 1, nxt);  // prefetch
+        finish_cells<R, false>(a, lane, i, jc + 2, edge, cur.u0, cur.u1, cur.v0, cur.v1, wU[4], wV[4]);
+        const R vnf = cur.vn;
+        const R fnU = quick_flux<R>(vnf, wU[1], wU[2], wU[3], wU[4]);
+        const R fnV = quick_flux<R>(vnf, wV[1], wV[2], wV[3], wV[4]);
+        const R Uc = wU[2], Vc = wV[2];
+        const R Um1 = lane_prev(Uc), Um2 = lane_prev(Um1), Up1 = lane_next(Uc);
+        const R Vm1 = lane_prev(Vc), Vm2 = lane_prev(Vm1), Vp1 = lane_next(Vc);
+        const R uw = cur.uw;
+        const R fwU = quick_flux<R>(uw, Um2, Um1, Uc, Up1), fwV = quick_flux<R>(uw, Vm2, Vm1, Vc, Vp1);
+        const R feU = lane_next(fwU), feV = lane_next(fwV);
+        const R CU = (feU - fwU) * idx + (fnU - fsU) * idy;
+        const R CV = (feV - fwV) * idx + (fnV - fsV) * idy;
+        const R VU = ((Up1 - R(2) * Uc + Um1) * idx2 + (wU[3] - R(2) * Uc + wU[1]) * idy2) * iren;
+        const R VV = ((Vp1 - R(2) * Vc + Vm1) * idx2 + (wV[3] - R(2) * Vc + wV[1]) * idy2) * iren;
+        const R TU = (R(0) - CU) + VU, TV = (R(0) - CV) + VV;
+        const R TUw = lane_prev(TU);
+        const R pc = cur.pc, pw = lane_prev(pc);
+        if (jc >= j0 && outlane) {
+            const size_t k = (size_t)(i < 0 ? 0 : i) + (size_t)P * jc;
+            if (xface) {  // x-face (i, jc); wall faces get R = 0 exactly
+                const bool wall = i == 0 || i == n;
+                const double Rx = wall ? 0.0 : -(pc - pw) * a.inv_dx + R(0.5) * (TUw + TU);
+                if (a.mode == 2) a.uo[k] = Rx;
+                else if (a.mode == 3) {
+                    const double r = Rx - a.c15 * uw + cur.cu;
+                    a.uk[k] = (float)uw;
+                    a.ru[k] = (float)r;
+                } else {
+                    const double unew = cur.ub + a.rk * (Rx - a.c15 * uw + cur.cu);
+                    a.uo[k] = unew;
+                    if (a.mode == 1) { const double d = cur.ub - unew; nx_ += fabs(d * d * (wall ? 0.5 : 1.0)); }
+                }
+            }
+            if (xin) {  // y-face (i, jc)
+                const bool wall = jc == 0;
+                const double Ry = wall ? 0.0 : -(pc - pprev) * a.inv_dy + R(0.5) * (tVprev + TV);
+                if (a.mode == 2) a.vo[k] = Ry;
+                else if (a.mode == 3) {
+                    const double r = Ry - a.c15 * vn + cur.cv;
+                    a.vk[k] = (float)vn;
+                    a.rv[k] = (float)r;
+                } else {
+                    const double vnew = cur.vb + a.rk * (Ry - a.c15 * vn + cur.cv);
+                    a.vo[k] = vnew;
+                    if (a.mode == 1) { const double d = cur.vb - vnew; ny_ += fabs(d * d * (wall ? 0.5 : 1.0)); }
+                }
             }
         }
-    };
-    auto y_face = [&](int fi, int fj, R RT) {
-        const size_t k = fi + (size_t)P * fj;
-        const bool wall = fj == 0 || fj == n;
-        if constexpr (!MIXED) {
-            double R_ = 0.0;
-            if (!wall) R_ = -(a.p[k] - a.p[k - P]) * a.inv_dy + RT;
-            if (a.mode == 2) { a.vo[k] = R_; return; }
-            const double base = a.vb[k];
-            const double vnew = base + a.rk * (R_ - a.c15 * a.vs[k] + a.cv[k]);
-            a.vo[k] = vnew;
-            if (a.mode == 1) { const double d = base - vnew; ny_ += fabs(d * d * (wall ? 0.5 : 1.0)); }
-        } else {
-            const float dold = a.vd ? a.vd[k] : 0.f;
-            const float w = a.wv[k];
-            const float dn = (float)a.rk * ((wall ? 0.f : RT) - (float)a.c15 * dold + w);
-            a.dov[k] = dn;
-            if (a.mode == 1) {
-                a.vkn[k] = a.vk[k] + dn;
-                a.wv[k] = w - (float)a.c15 * dn;
-                a.sv[k] += dn;
-                const double d = (double)dn;
-                ny_ += d * d * (wall ? 0.5 : 1.0);
-            }
-        }
-    };
+        fsU = fnU; fsV = fnV; tVprev = TV; pprev = pc; vn = vnf;
 #pragma unroll
-    for (int k = 0; k < RPT; ++k) {
-        const int r = ty * RPT + k, j = j0 + r;
-        if (i <= n && j < n) {
-            const R RT = (i == 0 || i == n) ? R(0) : R(0.5) * (sTU[r][tx] + sTU[r][tx + 1]);
-            x_face(i, j, RT);
-            if (i == n - 1 && i0 + TX == n) x_face(n, j, R(0));  // east wall face
-        }
-        if (i < n && j <= n) {
-            const R RT = (j == 0 || j == n) ? R(0) : R(0.5) * (sTV[r][tx] + sTV[r + 1][tx]);
-            y_face(i, j, RT);
-            if (j == n - 1 && j0 + TY == n) y_face(i, n, R(0));  // north wall face
+        for (int q = 0; q < 4; ++q) { wU[q] = wU[q + 1]; wV[q] = wV[q + 1]; }
+        cur = nxt;
+    }
+    if (jend == n && outlane && xin) {  // north wall row of y-faces: R = 0
+        const size_t k = i + (size_t)P * n;
+        if (a.mode == 2) a.vo[k] = 0.0;
+        else if (a.mode == 3) {
+            a.vk[k] = (float)vn;
+            a.rv[k] = (float)(-a.c15 * vn + a.cv[k]);
+        } else {
+            const double base = a.vb[k];
+            const double vnew = base + a.rk * (0.0 - a.c15 * vn + a.cv[k]);
+            a.vo[k] = vnew;
+            if (a.mode == 1) { const double d = base - vnew; ny_ += fabs(d * d * 0.5); }
         }
     }
     if (a.mode == 1) {
-        const double bx_ = block_reduce<false>(nx_, sred);
-        const double by_ = block_reduce<false>(ny_, sred + 8);
-        if (tid == 0) {
-            const size_t b = blockIdx.x + (size_t)gridDim.x * blockIdx.y;
-            a.partials[2 * b] = bx_;
-            a.partials[2 * b + 1] = by_;
+        for (int o = 32; o > 0; o >>= 1) { nx_ += __shfl_down(nx_, o, 64); ny_ += __shfl_down(ny_, o, 64); }
+        if (lane == 0) { a.partials[2 * wid] = nx_; a.partials[2 * wid + 1] = ny_; }
+    }
+}
+
+// ================================================= FP32 tangent-linear step
+// Tangent-linear flux: derivative of QUICK(vel, X) in direction (dvel, dX)
+// with the upwind choice frozen at vel: vel*phi(dX) + dvel*phi(X).
+__device__ __forceinline__ float quick_tl(float vel, float dvel, float a2, float a1, float b0, float b1, float da2,
+                                          float da1, float db0, float db1)
+{
+    const bool pos = vel >= 0.f;
+    // phi = 3/8 downwind + 6/8 upwind - 1/8 far-upwind
+    const float xd = pos ? b0 : a1, xu = pos ? a1 : b0, xf = pos ? a2 : b1;
+    const float dd = pos ? db0 : da1, du = pos ? da1 : db0, df = pos ? da2 : db1;
+    const float phid = 0.375f * dd + 0.75f * du - 0.125f * df;
+    const float phix = 0.375f * xd + 0.75f * xu - 0.125f * xf;
+    return vel * phid + dvel * phix;
+}
+
+// u^{k+1} = u^k + cr r + cj R'(u^k) r in FP64 with R' r formed in FP32 at the
+// FP32 base; norm partials of the change.
+__global__ __launch_bounds__(256) void k_tangent(StageArgs a)
+{
+    using R = float;
+    if (a.ctl && a.ctl->mom_done) return;
+    const int lane = threadIdx.x & 63;
+    const int wid = wave_id(a);
+    if (wid >= a.nbands * a.nchunks) return;
+    const int band = wid / a.nchunks, chunk = wid % a.nchunks;
+    const int n = a.n, P = a.P;
+    const int i0 = band * WB, j0 = chunk * RH;
+    const int i = i0 + lane - 3;
+    const bool xin = i >= 0 && i < n, xface = i >= 0 && i <= n;
+    const bool edge = i0 < 3 || i0 + 61 > n;
+    const float *uk = a.uk, *vk = a.vk, *ru = a.ru, *rv = a.rv;
+    struct Raw {
+        R u0, u1, v0, v1, d0, d1, e0, e1, vn, dvn, uw, duw;
+    };
+    auto load = [&](int jc, Raw &r) {
+        const int jr = jc + 2;
+        const int jm = jr < 0 ? -jr - 1 : (jr >= n ? 2 * n - 1 - jr : jr);
+        r.u0 = r.u1 = r.v0 = r.v1 = r.d0 = r.d1 = r.e0 = r.e1 = 0;
+        if (jm >= 0 && jm < n && xin) {
+            const size_t k = i + (size_t)P * jm;
+            r.u0 = uk[k]; r.u1 = uk[k + 1]; r.v0 = vk[k]; r.v1 = vk[k + P];
+            r.d0 = ru[k]; r.d1 = ru[k + 1]; r.e0 = rv[k]; r.e1 = rv[k + P];
+        }
+        const bool vok = xin && jc + 1 >= 0 && jc + 1 <= n;
+        r.vn = vok ? vk[i + (size_t)P * (jc + 1)] : R(0);
+        r.dvn = vok ? rv[i + (size_t)P * (jc + 1)] : R(0);
+        const bool row = jc >= 0 && jc < n;
+        const size_t k = (size_t)(i < 0 ? 0 : i) + (size_t)P * (jc < 0 ? 0 : jc);
+        r.uw = (xface && row) ? uk[k] : R(0);
+        r.duw = (xface && row) ? ru[k] : R(0);
+    };
+    const R idx = (R)a.inv_dx, idy = (R)a.inv_dy, idx2 = (R)a.inv_dx2, idy2 = (R)a.inv_dy2, iren = (R)a.inv_ren;
+    R wU[5], wV[5], dU[5], dV[5];
+    {
+        Raw r;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            load(j0 - 5 + q, r);
+            finish_cells<R, false>(a, lane, i, j0 - 3 + q, edge, r.u0, r.u1, r.v0, r.v1, wU[q], wV[q]);
+            finish_cells<R, true>(a, lane, i, j0 - 3 + q, edge, r.d0, r.d1, r.e0, r.e1, dU[q], dV[q]);
         }
     }
+    const bool v0ok = xin && j0 - 1 >= 0;
+    R vn = v0ok ? vk[i + (size_t)P * (j0 - 1)] : R(0), dvn = v0ok ? rv[i + (size_t)P * (j0 - 1)] : R(0);
+    R fsU = quick_tl(vn, dvn, wU[0], wU[1], wU[2], wU[3], dU[0], dU[1], dU[2], dU[3]);
+    R fsV = quick_tl(vn, dvn, wV[0], wV[1], wV[2], wV[3], dV[0], dV[1], dV[2], dV[3]);
+    R tVprev = 0;
+    double nx_ = 0.0, ny_ = 0.0;
+    const bool outlane = lane >= 3 && lane < 3 + WB;
+    const int jend = min(j0 + RH, n);
+    const double cr = a.cr, cj = a.cj;
+    Raw cur, nxt;
+    load(j0 - 1, cur);
+    for (int jc = j0 - 1; jc < jend; ++jc) {
+        if (jc + 1 < jend) load(jc + 1, nxt);
+        finish_cells<R, false>(a, lane, i, jc + 2, edge, cur.u0, cur.u1, cur.v0, cur.v1, wU[4], wV[4]);
+        finish_cells<R, true>(a, lane, i, jc + 2, edge, cur.d0, cur.d1, cur.e0, cur.e1, dU[4], dV[4]);
+        const R fnU = quick_tl(cur.vn, cur.dvn, wU[1], wU[2], wU[3], wU[4], dU[1], dU[2], dU[3], dU[4]);
+        const R fnV = quick_tl(cur.vn, cur.dvn, wV[1], wV[2], wV[3], wV[4], dV[1], dV[2], dV[3], dV[4]);
+        const R Uc = wU[2], Vc = wV[2], dUc = dU[2], dVc = dV[2];
+        const R Um1 = lane_prev(Uc), Um2 = lane_prev(Um1), Up1 = lane_next(Uc);
+        const R Vm1 = lane_prev(Vc), Vm2 = lane_prev(Vm1), Vp1 = lane_next(Vc);
+        const R dUm1 = lane_prev(dUc), dUm2 = lane_prev(dUm1), dUp1 = lane_next(dUc);
+        const R dVm1 = lane_prev(dVc), dVm2 = lane_prev(dVm1), dVp1 = lane_next(dVc);
+        const R fwU = quick_tl(cur.uw, cur.duw, Um2, Um1, Uc, Up1, dUm2, dUm1, dUc, dUp1);
+        const R fwV = quick_tl(cur.uw, cur.duw, Vm2, Vm1, Vc, Vp1, dVm2, dVm1, dVc, dVp1);
+        const R feU = lane_next(fwU), feV = lane_next(fwV);
+        const R CU = (feU - fwU) * idx + (fnU - fsU) * idy;
+        const R CV = (feV - fwV) * idx + (fnV - fsV) * idy;
+        const R VU = ((dUp1 - R(2) * dUc + dUm1) * idx2 + (dU[3] - R(2) * dUc + dU[1]) * idy2) * iren;
+        const R VV = ((dVp1 - R(2) * dVc + dVm1) * idx2 + (dV[3] - R(2) * dVc + dV[1]) * idy2) * iren;
+        const R TU = (R(0) - CU) + VU, TV = (R(0) - CV) + VV;
+        const R TUw = lane_prev(TU);
+        if (jc >= j0 && outlane) {
+            const size_t k = (size_t)(i < 0 ? 0 : i) + (size_t)P * jc;
+            if (xface) {
+                const bool wall = i == 0 || i == n;
+                const float Jr = wall ? 0.f : 0.5f * (TUw + TU);
+                const double du = cr * (double)cur.duw + cj * (double)Jr;
+                a.u64[k] += du;
+                nx_ += du * du * (wall ? 0.5 : 1.0);
+            }
+            if (xin) {
+                const bool wall = jc == 0;
+                const float Jr = wall ? 0.f : 0.5f * (tVprev + TV);
+                const double dv = cr * (double)dvn + cj * (double)Jr;
+                a.v64[k] += dv;
+                ny_ += dv * dv * (wall ? 0.5 : 1.0);
+            }
+        }
+        fsU = fnU; fsV = fnV; tVprev = TV; vn = cur.vn; dvn = cur.dvn;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { wU[q] = wU[q + 1]; wV[q] = wV[q + 1]; dU[q] = dU[q + 1]; dV[q] = dV[q + 1]; }
+        cur = nxt;
+    }
+    if (jend == n && outlane && xin) {  // north wall row: R' r = 0 there
+        const size_t k = i + (size_t)P * n;
+        const double dv = cr * (double)rv[k];
+        a.v64[k] += dv;
+        ny_ += dv * dv * 0.5;
+    }
+    for (int o = 32; o > 0; o >>= 1) { nx_ += __shfl_down(nx_, o, 64); ny_ += __shfl_down(ny_, o, 64); }
+    if (lane == 0) { a.partials[2 * wid] = nx_; a.partials[2 * wid + 1] = ny_; }
 }
 
 static StageArgs base_args(const Ctx &c, double rk, int mode)
@@ -262,14 +395,22 @@ static StageArgs base_args(const Ctx &c, double rk, int mode)
     a.bc_lo0 = g.bc_lo0; a.bc_hi0 = g.bc_hi0; a.bc_lo1 = g.bc_lo1; a.bc_hi1 = g.bc_hi1;
     a.inflow0 = g.inflow0; a.inflow1 = g.inflow1;
     a.mode = mode;
+    a.nbands = (g.n + 1 + WB - 1) / WB;
+    a.nchunks = (g.n + RH - 1) / RH;
     return a;
 }
 
-static dim3 stage_grid(int n, int *remap)
+static dim3 stage_grid(StageArgs &a)
 {
-    dim3 grid((n + TX - 1) / TX, (n + TY - 1) / TY);
-    *remap = ((grid.x * grid.y) % 8 == 0) ? 1 : 0;
-    return grid;
+    const int blocks = (a.nbands * a.nchunks + 3) / 4;
+    a.remap = blocks % 8 == 0 ? 1 : 0;
+    return dim3(blocks);
+}
+
+int stage_partials(const Ctx &c)
+{
+    const int nb = (c.g.n + 1 + WB - 1) / WB, nc = (c.g.n + RH - 1) / RH;
+    return nb * nc;
 }
 
 hipError_t launch_stage(const Ctx &c, const StageLaunch &s, int *nblocks)
@@ -278,52 +419,28 @@ hipError_t launch_stage(const Ctx &c, const StageLaunch &s, int *nblocks)
     a.us = s.us; a.vs = s.vs; a.ub = s.ub; a.vb = s.vb; a.cu = s.cu; a.cv = s.cv; a.p = s.p;
     a.uo = s.uo; a.vo = s.vo;
     if (s.mode == 2 || s.unguarded) a.ctl = nullptr;
-    const dim3 grid = stage_grid(c.g.n, &a.remap);
-    if ((int64_t)grid.x * grid.y > kMaxPartials) return hipErrorInvalidValue;
-    if (nblocks) *nblocks = grid.x * grid.y;
-    hipLaunchKernelGGL(k_stage<false>, grid, dim3(NT), 0, c.stream, a);
+    const dim3 grid = stage_grid(a);
+    if ((int64_t)a.nbands * a.nchunks > kMaxPartials) return hipErrorInvalidValue;
+    if (nblocks) *nblocks = a.nbands * a.nchunks;  // one norm partial per wave
+    hipLaunchKernelGGL(k_stage64, grid, dim3(256), 0, c.stream, a);
     return hipGetLastError();
 }
 
 hipError_t launch_stage_mixed(const Ctx &c, const MixedStageLaunch &s, int *nblocks)
 {
-    StageArgs a = base_args(c, s.rk, s.mode);
-    a.uk = s.uk; a.vk = s.vk; a.ud = s.ud; a.vd = s.vd;
-    a.dou = s.dou; a.dov = s.dov; a.wu = s.wu; a.wv = s.wv; a.su = s.su; a.sv = s.sv;
-    a.ukn = s.ukn; a.vkn = s.vkn;
+    StageArgs a = base_args(c, 0.0, 3);
     if (s.unguarded) a.ctl = nullptr;
-    const dim3 grid = stage_grid(c.g.n, &a.remap);
-    if ((int64_t)grid.x * grid.y > kMaxPartials) return hipErrorInvalidValue;
-    if (nblocks) *nblocks = grid.x * grid.y;
-    hipLaunchKernelGGL(k_stage<true>, grid, dim3(NT), 0, c.stream, a);
-    return hipGetLastError();
-}
-
-// Start of a mixed step: u^k = u^n in FP32, S = 0, w = (0.5/dt)(u^n - u^{n-1}) - G p
-// (= c - G p - (1.5/dt) u^n without the cancellation), velContPrev <- velCont.
-__global__ __launch_bounds__(256) void k_bdf2_mixed(const double *u, double *up, const double *p, float *uk,
-                                                    float *w, float *s, int nx, int ny, int P, int n, int dir,
-                                                    double dt, double inv_h)
-{
-    const int i = blockIdx.x * 64 + (threadIdx.x & 63), j = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (i >= nx || j >= ny) return;
-    const size_t k = i + (size_t)P * j;
-    const double un = u[k], um = up[k];
-    const int f = dir == 0 ? i : j;
-    const double gp = (f == 0 || f == n) ? 0.0 : (p[k] - p[k - (dir == 0 ? 1 : P)]) * inv_h;
-    up[k] = un;
-    uk[k] = (float)un;
-    w[k] = (float)((0.5 / dt) * (un - um) - gp);
-    s[k] = 0.f;
-}
-
-hipError_t launch_bdf2_mixed(const Ctx &c)
-{
-    const int n = c.g.n;
-    hipLaunchKernelGGL(k_bdf2_mixed, dim3((n + 1 + 63) / 64, (n + 3) / 4), dim3(256), 0, c.stream, c.U[0],
-                       c.Uprev[0], c.p, c.uk32[0][0], c.w32[0], c.s32[0], n + 1, n, c.g.P, n, 0, c.g.dt, 1.0 / c.g.dx);
-    hipLaunchKernelGGL(k_bdf2_mixed, dim3((n + 63) / 64, (n + 1 + 3) / 4), dim3(256), 0, c.stream, c.U[1],
-                       c.Uprev[1], c.p, c.uk32[0][1], c.w32[1], c.s32[1], n, n + 1, c.g.P, n, 1, c.g.dt, 1.0 / c.g.dy);
+    const dim3 grid = stage_grid(a);
+    if ((int64_t)a.nbands * a.nchunks > kMaxPartials) return hipErrorInvalidValue;
+    if (nblocks) *nblocks = a.nbands * a.nchunks;
+    a.uk = s.uk; a.vk = s.vk; a.ru = s.ru; a.rv = s.rv;
+    if (s.stage == 0) {  // FP64 residual of the pseudo base
+        a.us = s.u64; a.vs = s.v64; a.cu = s.cu; a.cv = s.cv; a.p = s.p;
+        hipLaunchKernelGGL(k_stage64, grid, dim3(256), 0, c.stream, a);
+    } else {  // FP32 tangent-linear correction, base updated in place
+        a.u64 = s.u64; a.v64 = s.v64; a.cr = s.cr; a.cj = s.cj;
+        hipLaunchKernelGGL(k_tangent, grid, dim3(256), 0, c.stream, a);
+    }
     return hipGetLastError();
 }
 

@@ -178,9 +178,12 @@ int amressif_project(amressif_ctx *ctx, const amr_array4 velStar[2], amr_array4 
  * main.cpp:1173-1175).  Average device time of one kernel at the context's
  * shape and current state, timed with HIP events on the context's stream over
  * `reps` back-to-back launches:
- *   which = 0  fused RK stage of the configured precision (the step's dominant kernel)
- *   which = 1  one red-black Gauss-Seidel colour sweep on the finest level
+ *   which = 0  fused FP64 RK stage (the FP64 step's dominant kernel)
+ *   which = 1  fused finest-level smoother (both colours, from a zero guess)
  *   which = 2  plain device-to-device copy of copy_bytes bytes (bandwidth calibration)
+ *   which = 3  mixed precision: FP64 residual of the pseudo base
+ *   which = 4  mixed precision: FP32 tangent-linear update of the pseudo base
+ * which = 3 and 4 need a context created with precision = 1.
  * bytes_per_launch receives the kernel's algorithmic HBM bytes per launch. */
 int amressif_time_kernel(amressif_ctx *ctx, int which, int reps, int64_t copy_bytes, double *avg_ms,
                          double *bytes_per_launch);

@@ -286,7 +286,10 @@ def test_mixed_matches_fp64_within_1e_6(n, ren, steps):
     pa, pb = a["ctx"][0, 1:-1, 1:-1], b["ctx"][0, 1:-1, 1:-1]
     assert rel(pa - pa.mean(), pb - pb.mean()) < 1e-6
     for s, r in zip(sa, sb):
-        assert abs(s["pseudo_iters"] - r["pseudo_iters"]) <= 1
+        # the mixed pseudo-iteration keeps the RK4 recursion to first order in the
+        # Jacobian: its contraction per iteration is |1 + z + z^2/2| instead of the
+        # RK4 polynomial (z ~ -0.6 here), so it takes up to ~15% more iterations
+        assert r["pseudo_iters"] - 1 <= s["pseudo_iters"] <= 1.25 * r["pseudo_iters"] + 2
         assert s["flags"] == 0
         # the outer Poisson residual and the divergence check stay FP64
         assert s["div_new"] <= 1e-10 * s["div_star"] + 1e-12
@@ -308,8 +311,12 @@ def test_kernel_timing_diagnostic():
     for prec in (api.AMRESSIF_FP64, api.AMRESSIF_MIXED):
         run = api.Run.from_numpy(api.cavity_inputs(256, 1000.0, precision=prec), W.initial_state(256, "seeded"))
         run.advance(1)
-        for which in (0, 1):
+        kinds = (0, 1, 3, 4) if prec == api.AMRESSIF_MIXED else (0, 1)
+        for which in kinds:
             ms, nbytes = run.ctx.time_kernel(which, reps=5)
             assert ms > 0 and nbytes > 0
+        if prec == api.AMRESSIF_FP64:
+            with pytest.raises(api.AmressifError):
+                run.ctx.time_kernel(3, reps=1)
     ms, nbytes = run.ctx.time_kernel(2, reps=5, copy_bytes=1 << 24)
     assert ms > 0 and nbytes == 2 * (1 << 24)

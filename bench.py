@@ -45,6 +45,9 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps timed for cpu_baseline (0: skip)")
     ap.add_argument("--only", choices=["both", "mixed", "fp64"], default="both")
     ap.add_argument("--profile-run", action="store_true", help="short run for rocprofv3: no cpu baseline")
+    ap.add_argument("--kernel-probe", choices=["fp64", "mixed"], default=None,
+                    help="for rocprofv3 --pmc: one step, then a few timed launches of each kernel of that "
+                         "precision and a 64 MiB copy (calibration); prints their HIP-event times")
     return ap.parse_args()
 
 
@@ -76,6 +79,16 @@ def run_precision(api, W, n, ren, precision, state, warmup, steps, torch):
     return run, stats, t1 - t0
 
 
+def kernel_probe(api, W, n, ren, which_prec):
+    prec = api.AMRESSIF_MIXED if which_prec == "mixed" else api.AMRESSIF_FP64
+    run = api.Run.from_numpy(api.cavity_inputs(n, ren, precision=prec), W.initial_state(n, "seeded"))
+    run.advance(1)
+    kinds = (3, 4) if which_prec == "mixed" else (0, 1)
+    res = {KERNELS[k]: run.ctx.time_kernel(k, reps=3)[0] for k in kinds}
+    res["copy 64 MiB"] = run.ctx.time_kernel(2, reps=3, copy_bytes=64 << 20)[0]
+    print(json.dumps({"kernel_probe": which_prec, "n": n, "avg_ms": res}), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -86,6 +99,10 @@ def main():
     import torch
     from amressif_amd import api
     from amressif_amd import workloads as W
+
+    if args.kernel_probe:
+        kernel_probe(api, W, args.n, args.ren, args.kernel_probe)
+        return
 
     n, ren, K, Wu = args.n, args.ren, args.steps, args.warmup
     state = W.initial_state(n, "seeded")

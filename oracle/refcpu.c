@@ -7,7 +7,7 @@
  * __graft_entry__.smoke() and by bench.py's cpu_baseline leg, as the checker
  * or as the timed CPU baseline, and never by the product library.
  *
- * Parity pins (see tests/test_oracle_*.py):
+ * Parity pins (tests/test_oracle.py):
  *   cont2cart ghost rules ........ closed forms (ghost+mirror)/2 == wall value
  *   QUICK half-node flux ......... exact on quadratics, either flow direction
  *   viscous flux ................. Laplacian exact on quadratics

@@ -33,7 +33,7 @@ _lib = ctypes.CDLL(LIB_PATH)
 
 AMRESSIF_FP64 = 0
 AMRESSIF_MIXED = 1
-FLAG_CAP, FLAG_DIVERGED, FLAG_NONFINITE, FLAG_MG_CAP = 1, 2, 4, 8
+FLAG_CAP, FLAG_DIVERGED, FLAG_NONFINITE, FLAG_MG_CAP, FLAG_LIN_CAP = 1, 2, 4, 8, 16
 ERRORS = {-1: "ERR_ARG", -2: "ERR_BC", -3: "ERR_UNSUPPORTED", -4: "ERR_HIP", -5: "ERR_NONFINITE", -6: "ERR_PARSE"}
 
 
@@ -64,6 +64,8 @@ class Inputs(ctypes.Structure):
         ("mg_max_iter", ctypes.c_int),
         ("mg_bottom_max", ctypes.c_int),
         ("precision", ctypes.c_int),
+        ("poisson_solver", ctypes.c_int),
+        ("gmres_restart", ctypes.c_int),
     ]
 
 
@@ -88,6 +90,19 @@ class StepStats(ctypes.Structure):
         ("rhs_sum", ctypes.c_double),
         ("poisson_res", ctypes.c_double),
         ("div_new", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "pad"}
+
+
+class LinStats(ctypes.Structure):
+    _fields_ = [
+        ("lin_iters", ctypes.c_int),
+        ("vcycles", ctypes.c_int),
+        ("flags", ctypes.c_int),
+        ("pad", ctypes.c_int),
+        ("lin_rel", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -119,6 +134,16 @@ _sig = {
     "amressif_project": (_c.c_int, [_c.c_void_p, _A4, _A4, _A4, _P(_c.c_double)]),
     "amressif_time_kernel": (_c.c_int, [_c.c_void_p, _c.c_int, _c.c_int, _c.c_int64, _P(_c.c_double),
                                         _P(_c.c_double)]),
+    "amressif_tape_create": (_c.c_int, [_c.c_void_p, _c.c_int, _P(_c.c_void_p)]),
+    "amressif_tape_destroy": (_c.c_int, [_c.c_void_p]),
+    "amressif_tape_reset": (_c.c_int, [_c.c_void_p]),
+    "amressif_tape_steps": (_c.c_int, [_c.c_void_p]),
+    "amressif_step_fwd": (_c.c_int, [_c.c_void_p, _c.c_int, _c.c_void_p, _P(StepStats)]),
+    "amressif_step_bwd": (_c.c_int, [_c.c_void_p, _c.c_void_p, _c.c_int, _P(LinStats)]),
+    "amressif_step_tlm": (_c.c_int, [_c.c_void_p, _c.c_void_p, _c.c_int, _c.c_int, _P(LinStats)]),
+    "amressif_load_dual": (_c.c_int, [_c.c_void_p, _A4, _A4, _A4, _c.c_void_p]),
+    "amressif_store_dual": (_c.c_int, [_c.c_void_p, _A4, _A4, _A4, _c.c_void_p]),
+    "amressif_set_adjoint_solver": (_c.c_int, [_c.c_void_p, _c.c_double, _c.c_int]),
 }
 for _name, (_res, _args) in _sig.items():
     _f = getattr(_lib, _name)
@@ -284,6 +309,48 @@ class Context:
         self._check(_lib.amressif_time_kernel(self._h, which, reps, copy_bytes, ctypes.byref(ms), ctypes.byref(b)))
         return ms.value, b.value
 
+    # ---- discrete adjoint (forward/backward pair)
+    def tape(self, max_steps: int) -> "Tape":
+        return Tape(self, max_steps)
+
+    def step_fwd(self, nsteps: int, tape: "Tape"):
+        """advance nsteps, recording each step's converged predictor u* on the tape"""
+        st = (StepStats * max(1, nsteps))()
+        self._check(_lib.amressif_step_fwd(self._h, nsteps, tape._h, st))
+        return [st[k].as_dict() for k in range(nsteps)]
+
+    def step_bwd(self, tape: "Tape", nsteps: int):
+        """adjoint of the tape's last nsteps steps (newest first) on the dual state"""
+        st = (LinStats * max(1, nsteps))()
+        self._check(_lib.amressif_step_bwd(self._h, tape._h, nsteps, st))
+        return [st[k].as_dict() for k in range(nsteps)]
+
+    def step_tlm(self, tape: "Tape", first: int, nsteps: int):
+        """tangent-linear of tape steps [first, first + nsteps) on the dual state"""
+        st = (LinStats * max(1, nsteps))()
+        self._check(_lib.amressif_step_tlm(self._h, tape._h, first, nsteps, st))
+        return [st[k].as_dict() for k in range(nsteps)]
+
+    def load_dual(self, du=None, dv=None, dup=None, dvp=None, dp=None, dlid=None):
+        """dual state in: faces (du, dv), (dup, dvp); cells dp (n+2g square); lid dlid (n); None = zeros"""
+        n = self.n
+        a = faces_view(du, dv, n) if du is not None else None
+        b = faces_view(dup, dvp, n) if dup is not None else None
+        c = cell_view(dp, n, (dp.shape[-1] - n) // 2) if dp is not None else None
+        lid = ctypes.c_void_p(dlid.data_ptr()) if dlid is not None else None
+        self._check(_lib.amressif_load_dual(self._h, a, b, ctypes.byref(c) if c is not None else None, lid))
+
+    def store_dual(self, du=None, dv=None, dup=None, dvp=None, dp=None, dlid=None):
+        n = self.n
+        a = faces_view(du, dv, n) if du is not None else None
+        b = faces_view(dup, dvp, n) if dup is not None else None
+        c = cell_view(dp, n, (dp.shape[-1] - n) // 2) if dp is not None else None
+        lid = ctypes.c_void_p(dlid.data_ptr()) if dlid is not None else None
+        self._check(_lib.amressif_store_dual(self._h, a, b, ctypes.byref(c) if c is not None else None, lid))
+
+    def set_adjoint_solver(self, tol: float = 1e-14, max_sweeps: int = 200):
+        self._check(_lib.amressif_set_adjoint_solver(self._h, tol, max_sweeps))
+
     # ---- module functions (reference names)
     def cont2cart(self, u, v, velCart=None):
         """utilities.cpp:16-350"""
@@ -350,6 +417,33 @@ class Context:
         self._check(_lib.amressif_project(self._h, faces_view(ustar, vstar, n), ctypes.byref(uc),
                                           faces_view(u, v, n), ctypes.byref(d)))
         return u, v, d.value
+
+
+class Tape:
+    """Device record of the converged predictors u* of consecutive steps."""
+
+    def __init__(self, ctx: Context, max_steps: int):
+        h = ctypes.c_void_p()
+        ctx._check(_lib.amressif_tape_create(ctx._h, max_steps, ctypes.byref(h)))
+        self._h, self.ctx, self.max_steps = h, ctx, max_steps
+
+    @property
+    def steps(self) -> int:
+        return _lib.amressif_tape_steps(self._h)
+
+    def reset(self):
+        _lib.amressif_tape_reset(self._h)
+
+    def close(self):
+        if self._h and self._h.value:
+            _lib.amressif_tape_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 @dataclass

@@ -26,6 +26,16 @@ struct amressif_ctx {
     amr::Ctx c;
 };
 
+// u* of consecutive recorded steps, 2 face arrays (u then v) per step
+struct amressif_tape {
+    amressif_ctx *ctx = nullptr;
+    hipStream_t stream = nullptr;  // the context's stream (the tape may outlive the context)
+    int cap = 0, count = 0;
+    size_t fsz = 0;  // doubles per face array
+    double *store = nullptr;
+    double *slot(int s) const { return store + (size_t)s * 2 * fsz; }
+};
+
 using amr::Ctx;
 
 namespace amr {
@@ -188,9 +198,44 @@ int ensure_module_scratch(amressif_ctx *ctx)
     return 0;
 }
 
+// ---- FGMRES(m) with a V-cycle preconditioner on c.b (already centred):
+// cycles of guarded Arnoldi steps, polled in batches; the true residual is
+// tested at every cycle start
+int gmres_loop(amressif_ctx *ctx)
+{
+    Ctx &c = ctx->c;
+    const int mixed = c.in.precision == AMRESSIF_MIXED;
+    HIPCHK(amr::launch_mg_start(c));
+    HIPCHK(amr::launch_gm_reset(c));
+    HIPCHK(amr::launch_fill(c, c.phiv[0], (int64_t)(c.g.n + 1) * c.g.P, 0.0));
+    double *x = c.phiv[0];
+    const int cap = std::max(1, c.in.mg_max_iter);
+    for (int cycle = 0; cycle <= cap; ++cycle) {
+        HIPCHK(amr::launch_gm_cycle_start(c, x, c.b, mixed));
+        int j = 0, target = std::max(1, std::min(c.gm_m, c.last_V));
+        for (;;) {
+            while (j < target && j < c.gm_m) {
+                HIPCHK(amr::launch_gm_iteration(c, mixed));
+                ++j;
+            }
+            if (int rc = poll(ctx)) return rc;
+            if (c.ctl_host->mg_done || c.ctl_host->gm_stop || j >= c.gm_m) break;
+            target = j + 2;
+        }
+        if (c.ctl_host->mg_done) break;
+        HIPCHK(amr::launch_gm_cycle_end(c, x));
+    }
+    c.last_V = std::max(1, std::min(c.gm_m, c.ctl_host->mg_iters));
+    c.phi = x;
+    return 0;
+}
+
 // ---- the Poisson solve on c.phi / c.b (b already centred), shared by the
 // step and the module entry point
-int poisson_loop(amressif_ctx *ctx)
+int mg_loop(amressif_ctx *ctx);
+int poisson_loop(amressif_ctx *ctx) { return ctx->c.in.poisson_solver == 1 ? gmres_loop(ctx) : mg_loop(ctx); }
+
+int mg_loop(amressif_ctx *ctx)
 {
     Ctx &c = ctx->c;
     const int mixed = c.in.precision == AMRESSIF_MIXED;
@@ -222,6 +267,8 @@ int poisson_loop(amressif_ctx *ctx)
 // input ping-pongs U -> S1 -> S2 -> S1 and the last stage updates U (the
 // pseudo base u^k) in place.  Mixed: the FP64 residual of U, then the FP32
 // tangent-linear update of U in place (first order in the RK4 recursion).
+int advance_impl(amressif_ctx *ctx, int nsteps, amressif_step_stats *stats, amressif_tape *tape);
+
 int launch_pseudo_iteration(amressif_ctx *ctx, int it, int *nb)
 {
     Ctx &c = ctx->c;
@@ -282,6 +329,8 @@ void amressif_inputs_default(amressif_inputs *in)
     in->mg_max_iter = 2000;
     in->mg_bottom_max = 16;
     in->precision = AMRESSIF_FP64;
+    in->poisson_solver = 0;
+    in->gmres_restart = 30;
 }
 
 double amressif_dt(const amressif_inputs *in)
@@ -388,7 +437,9 @@ int amressif_parse_inputs(const char *path, int argc, const char *const *argv, a
               get_real("amressif.poisson_tol_abs", &out->poisson_tol_abs, 1, false) &&
               get_int("amressif.mg_max_iter", &out->mg_max_iter, 1, false) &&
               get_int("amressif.mg_bottom_max", &out->mg_bottom_max, 1, false) &&
-              get_int("amressif.precision", &out->precision, 1, false);
+              get_int("amressif.precision", &out->precision, 1, false) &&
+              get_int("amressif.poisson_solver", &out->poisson_solver, 1, false) &&
+              get_int("amressif.gmres_restart", &out->gmres_restart, 1, false);
     if (!ok) return seterr(bad);
     return AMRESSIF_OK;
 }
@@ -401,6 +452,9 @@ int amressif_create(const amressif_inputs *in, void *hip_stream, amressif_ctx **
     Ctx &c = ctx->c;
     c.in = *in;
     c.stream = (hipStream_t)hip_stream;
+    // tuning overrides of the stage kernels' decomposition (launch shape only)
+    if (const char *e = std::getenv("AMRESSIF_STAGE_ROWS")) c.stage_rows = std::max(4, std::min(256, std::atoi(e)));
+    if (const char *e = std::getenv("AMRESSIF_STAGE_ORDER")) c.stage_order = std::atoi(e) ? 1 : 0;
     auto bail = [&](int code, const std::string &m) {
         // keep the context so the caller can read the message; it owns nothing yet
         ctx->c.err = m;
@@ -416,6 +470,10 @@ int amressif_create(const amressif_inputs *in, void *hip_stream, amressif_ctx **
     if (in->precision != AMRESSIF_FP64 && in->precision != AMRESSIF_MIXED)
         return bail(AMRESSIF_ERR_ARG, "precision must be 0 (FP64) or 1 (mixed)");
     if (in->mg_bottom_max < 2 || in->mg_bottom_max > 32) return bail(AMRESSIF_ERR_ARG, "mg_bottom_max must be in [2, 32]");
+    if (in->poisson_solver != 0 && in->poisson_solver != 1)
+        return bail(AMRESSIF_ERR_ARG, "poisson_solver must be 0 (multigrid) or 1 (FGMRES)");
+    if (in->poisson_solver == 1 && (in->gmres_restart < 1 || in->gmres_restart > amr::kGmMax))
+        return bail(AMRESSIF_ERR_ARG, "gmres_restart must be in [1, 64]");
     {
         std::string msg;
         if (int rc = check_bc(*in, msg)) return bail(rc, msg);
@@ -489,6 +547,14 @@ int amressif_create(const amressif_inputs *in, void *hip_stream, amressif_ctx **
         if (e1 != hipSuccess || e2 != hipSuccess) return bail(AMRESSIF_ERR_HIP, "pinv upload failed");
         if (amr::configure_tail(c) != hipSuccess) return bail(AMRESSIF_ERR_UNSUPPORTED, "multigrid tail does not fit LDS");
     }
+    if (in->poisson_solver == 1) {  // FGMRES: m+1 Arnoldi vectors, m preconditioned vectors
+        c.gm_m = in->gmres_restart;
+        c.gm_vs = round_up((int)fsz, 64);
+        ALLOC(&c.gmV, c.gm_vs * (size_t)(c.gm_m + 1));
+        ALLOC(&c.gmZ, c.gm_vs * (size_t)c.gm_m);
+        ALLOC(&c.gm_part, amr::gm_partials_doubles(c, c.gm_m));
+        ALLOC(&c.gm, 1);
+    }
     ALLOC(&c.partials, (size_t)amr::kMaxPartials * 4);
     ALLOC(&c.ctl, 1);
     {
@@ -520,6 +586,7 @@ int amressif_destroy(amressif_ctx *ctx)
     for (void *p : c.allocs) hipFree(p);
     if (c.ctl_host) hipHostFree(c.ctl_host);
     if (c.log) hipFree(c.log);
+    delete c.adj;
     delete ctx;
     return AMRESSIF_OK;
 }
@@ -592,6 +659,22 @@ int amressif_store_state(const amressif_ctx *cctx, amr_array4 velCont[2], amr_ar
 
 int amressif_advance(amressif_ctx *ctx, int nsteps, amressif_step_stats *stats)
 {
+    return advance_impl(ctx, nsteps, stats, nullptr);
+}
+
+int amressif_step_fwd(amressif_ctx *ctx, int nsteps, amressif_tape *tape, amressif_step_stats *stats)
+{
+    if (!ctx || !tape || tape->ctx != ctx || nsteps < 0) return fail(ctx, AMRESSIF_ERR_ARG, "bad argument");
+    if (tape->count + nsteps > tape->cap)
+        return fail(ctx, AMRESSIF_ERR_ARG, "tape holds " + std::to_string(tape->cap) + " steps");
+    return advance_impl(ctx, nsteps, stats, tape);
+}
+
+}  // extern "C"
+
+namespace {
+int advance_impl(amressif_ctx *ctx, int nsteps, amressif_step_stats *stats, amressif_tape *tape)
+{
     if (!ctx || nsteps < 0) return fail(ctx, AMRESSIF_ERR_ARG, "bad argument");
     Ctx &c = ctx->c;
     if (nsteps == 0) return AMRESSIF_OK;
@@ -623,6 +706,13 @@ int amressif_advance(amressif_ctx *ctx, int nsteps, amressif_step_stats *stats)
         }
         c.last_P = std::max(1, c.ctl_host->mom_count);
         const int flags = c.ctl_host->flags;
+        if (tape) {  // record the converged predictor u* of this step
+            const size_t bytes = tape->fsz * sizeof(double);
+            HIPCHK(hipMemcpyAsync(tape->slot(tape->count), c.U[0], bytes, hipMemcpyDeviceToDevice, c.stream));
+            HIPCHK(hipMemcpyAsync(tape->slot(tape->count) + tape->fsz, c.U[1], bytes, hipMemcpyDeviceToDevice,
+                                  c.stream));
+            tape->count += 1;
+        }
         // ---- projection: Poisson RHS, MLMG-equivalent solve, update
         HIPCHK(amr::launch_poisson_rhs(c, c.U[0], c.U[1], c.b, &nb));
         HIPCHK(amr::launch_fin_prhs(c, nb));
@@ -643,6 +733,9 @@ int amressif_advance(amressif_ctx *ctx, int nsteps, amressif_step_stats *stats)
     HIPCHK(hipStreamSynchronize(c.stream));
     return status;
 }
+}  // namespace
+
+extern "C" {
 
 // ------------------------------------------------------ module entry points
 int amressif_cont2cart(amressif_ctx *ctx, const amr_array4 velCont[2], amr_array4 *velCart)
@@ -840,7 +933,8 @@ int amressif_time_kernel(amressif_ctx *ctx, int which, int reps, int64_t copy_by
         }
         if (which == 0) {
             amr::StageLaunch a{};
-            a.us = c.U[0]; a.vs = c.U[1]; a.ub = c.U[0]; a.vb = c.U[1];
+            // distinct stencil input and base arrays, as inside a step (S1 holds the step's last stage)
+            a.us = c.S1[0]; a.vs = c.S1[1]; a.ub = c.U[0]; a.vb = c.U[1];
             a.cu = c.C[0]; a.cv = c.C[1]; a.p = c.p;
             a.uo = c.S2[0]; a.vo = c.S2[1];
             a.rk = c.rk[0];
@@ -879,6 +973,260 @@ int amressif_time_kernel(amressif_ctx *ctx, int which, int reps, int64_t copy_by
         static const double per_cell[5] = {72.0, 16.0, 0.0, 56.0, 48.0};
         *bytes_per_launch = which == 2 ? 2.0 * (double)copy_bytes : per_cell[which] * cells;
     }
+    return AMRESSIF_OK;
+}
+
+}  // extern "C"
+
+// ----------------------------------------------------- discrete adjoint
+namespace {
+
+int ensure_adjoint(amressif_ctx *ctx)
+{
+    Ctx &c = ctx->c;
+    if (c.adj) return 0;
+    const int n = c.g.n;
+    if (n < 4) return fail(ctx, AMRESSIF_ERR_UNSUPPORTED, "the adjoint needs n_cell >= 4");
+    c.Pg = round_up(n + 4, 64);
+    const size_t gsz = (size_t)c.Pg * (n + 4), fsz = (size_t)(n + 1) * c.g.P;
+    auto *w = new amr::AdjWork();
+    c.adj = w;
+    std::vector<std::pair<double **, size_t>> bufs = {
+        {&w->XU, gsz}, {&w->XV, gsz}, {&w->DU, gsz}, {&w->DV, gsz}, {&w->F[0], fsz}, {&w->F[1], fsz},
+        {&w->F[2], fsz}, {&w->F[3], fsz}, {&w->Q[0], fsz}, {&w->Q[1], fsz}, {&w->XbU, fsz}, {&w->XbV, fsz},
+        {&w->lidg, (size_t)2 * n}, {&w->L[0][0], fsz}, {&w->L[0][1], fsz}, {&w->L[1][0], fsz}, {&w->L[1][1], fsz},
+        {&c.Du[0], fsz}, {&c.Du[1], fsz}, {&c.Dup[0], fsz}, {&c.Dup[1], fsz}, {&c.Dp, fsz}, {&c.Dlid, (size_t)n}};
+    for (auto &b : bufs) {
+        HIPCHK(amr::dev_alloc_public(c, b.first, b.second));
+        HIPCHK(amr::launch_fill(c, *b.first, (int64_t)b.second, 0.0));
+    }
+    return 0;
+}
+
+// Richardson solve of the tangent (adjoint = false) or adjoint system; the
+// initial guess is in L[0], the solution is returned in *xu, *xv.
+int lin_solve(amressif_ctx *ctx, bool adjoint, const double *dlid, double **xu, double **xv, amressif_lin_stats *st)
+{
+    Ctx &c = ctx->c;
+    amr::AdjWork &w = *c.adj;
+    const int cap = std::max(1, c.adj_cap);
+    HIPCHK(amr::launch_lin_start(c, c.adj_tol, cap));
+    int launched = 0, target = std::max(1, c.last_L);
+    for (;;) {
+        while (launched < target && launched < cap) {
+            const int par = launched & 1;
+            if (adjoint)
+                HIPCHK(amr::launch_adj_sweep(c, w, w.L[par][0], w.L[par][1], w.L[par ^ 1][0], w.L[par ^ 1][1]));
+            else
+                HIPCHK(amr::launch_tl_sweep(c, w, w.L[par][0], w.L[par][1], w.L[par ^ 1][0], w.L[par ^ 1][1], dlid));
+            ++launched;
+        }
+        if (int rc = poll(ctx)) return rc;
+        if (c.ctl_host->lin_done || launched >= cap) break;
+        target = launched + 2;
+    }
+    const int cnt = c.ctl_host->lin_count;
+    c.last_L = std::max(1, cnt);
+    *xu = w.L[cnt & 1][0];
+    *xv = w.L[cnt & 1][1];
+    if (st) {
+        st->lin_iters = cnt;
+        st->lin_rel = c.ctl_host->lin_rel;
+    }
+    return 0;
+}
+
+bool lid_carries_g(const Ctx &c) { return c.g.bc_hi1 != 135 && c.g.bc_hi1 != -135; }
+
+int check_tape(amressif_ctx *ctx, const amressif_tape *tape)
+{
+    if (!tape || tape->ctx != ctx) return fail(ctx, AMRESSIF_ERR_ARG, "tape belongs to another context");
+    if (!ctx->c.adj) return fail(ctx, AMRESSIF_ERR_ARG, "no dual state: create a tape first");
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int amressif_tape_create(amressif_ctx *ctx, int max_steps, amressif_tape **out)
+{
+    if (!ctx || !out || max_steps < 1) return fail(ctx, AMRESSIF_ERR_ARG, "bad argument");
+    *out = nullptr;
+    if (int rc = ensure_adjoint(ctx)) return rc;
+    Ctx &c = ctx->c;
+    auto *t = new amressif_tape();
+    t->ctx = ctx;
+    t->stream = c.stream;
+    t->cap = max_steps;
+    t->fsz = (size_t)(c.g.n + 1) * c.g.P;
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, (size_t)max_steps * 2 * t->fsz * sizeof(double));
+    if (e != hipSuccess) {
+        delete t;
+        return fail(ctx, AMRESSIF_ERR_HIP, std::string("tape allocation: ") + hipGetErrorString(e));
+    }
+    t->store = (double *)p;
+    *out = t;
+    return AMRESSIF_OK;
+}
+
+int amressif_tape_destroy(amressif_tape *tape)
+{
+    if (!tape) return AMRESSIF_OK;
+    hipStreamSynchronize(tape->stream);
+    if (tape->store) hipFree(tape->store);
+    delete tape;
+    return AMRESSIF_OK;
+}
+
+int amressif_tape_reset(amressif_tape *tape)
+{
+    if (!tape) return AMRESSIF_ERR_ARG;
+    tape->count = 0;
+    return AMRESSIF_OK;
+}
+
+int amressif_tape_steps(const amressif_tape *tape) { return tape ? tape->count : 0; }
+
+int amressif_set_adjoint_solver(amressif_ctx *ctx, double tol, int max_sweeps)
+{
+    if (!ctx || !(tol >= 0.0) || max_sweeps < 1) return fail(ctx, AMRESSIF_ERR_ARG, "bad argument");
+    ctx->c.adj_tol = tol;
+    ctx->c.adj_cap = max_sweeps;
+    return AMRESSIF_OK;
+}
+
+int amressif_load_dual(amressif_ctx *ctx, const amr_array4 du[2], const amr_array4 du_prev[2], const amr_array4 *dp,
+                       const double *dlid)
+{
+    if (!ctx) return AMRESSIF_ERR_ARG;
+    if (int rc = ensure_adjoint(ctx)) return rc;
+    Ctx &c = ctx->c;
+    const int64_t fsz = (int64_t)(c.g.n + 1) * c.g.P;
+    for (int d = 0; d < 2; ++d) {
+        if (du) {
+            if (int rc = check_face(ctx, &du[d], d, "du")) return rc;
+            HIPCHK(amr::launch_pack_face(c, du[d], c.Du[d], d));
+        } else {
+            HIPCHK(amr::launch_fill(c, c.Du[d], fsz, 0.0));
+        }
+        if (du_prev) {
+            if (int rc = check_face(ctx, &du_prev[d], d, "du_prev")) return rc;
+            HIPCHK(amr::launch_pack_face(c, du_prev[d], c.Dup[d], d));
+        } else {
+            HIPCHK(amr::launch_fill(c, c.Dup[d], fsz, 0.0));
+        }
+    }
+    if (dp) {
+        if (int rc = check_cell(ctx, dp, 1, "dp")) return rc;
+        HIPCHK(amr::launch_pack_cell(c, *dp, 0, c.Dp));
+    } else {
+        HIPCHK(amr::launch_fill(c, c.Dp, fsz, 0.0));
+    }
+    if (dlid)
+        HIPCHK(hipMemcpyAsync(c.Dlid, dlid, sizeof(double) * c.g.n, hipMemcpyDeviceToDevice, c.stream));
+    else
+        HIPCHK(amr::launch_fill(c, c.Dlid, c.g.n, 0.0));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    return AMRESSIF_OK;
+}
+
+int amressif_store_dual(amressif_ctx *ctx, amr_array4 du[2], amr_array4 du_prev[2], amr_array4 *dp, double *dlid)
+{
+    if (!ctx) return AMRESSIF_ERR_ARG;
+    Ctx &c = ctx->c;
+    if (!c.adj) return fail(ctx, AMRESSIF_ERR_ARG, "no dual state");
+    for (int d = 0; d < 2; ++d) {
+        if (du) {
+            if (int rc = check_face(ctx, &du[d], d, "du")) return rc;
+            HIPCHK(amr::launch_unpack_face(c, c.Du[d], du[d], d));
+        }
+        if (du_prev) {
+            if (int rc = check_face(ctx, &du_prev[d], d, "du_prev")) return rc;
+            HIPCHK(amr::launch_unpack_face(c, c.Dup[d], du_prev[d], d));
+        }
+    }
+    if (dp) {
+        if (int rc = check_cell(ctx, dp, 1, "dp")) return rc;
+        HIPCHK(amr::launch_unpack_cell_shift(c, c.Dp, *dp, 0, 1, nullptr));
+    }
+    if (dlid) HIPCHK(hipMemcpyAsync(dlid, c.Dlid, sizeof(double) * c.g.n, hipMemcpyDeviceToDevice, c.stream));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    return AMRESSIF_OK;
+}
+
+int amressif_step_tlm(amressif_ctx *ctx, const amressif_tape *tape, int first, int nsteps, amressif_lin_stats *stats)
+{
+    if (!ctx) return AMRESSIF_ERR_ARG;
+    if (int rc = check_tape(ctx, tape)) return rc;
+    if (first < 0 || nsteps < 0 || first + nsteps > tape->count)
+        return fail(ctx, AMRESSIF_ERR_ARG, "steps outside the tape");
+    Ctx &c = ctx->c;
+    amr::AdjWork &w = *c.adj;
+    HIPCHK(amr::launch_ctl_set(c, c.in.momentum_tolerance, c.in.poisson_tol_rel, c.in.poisson_tol_abs, c.in.IterNum,
+                               c.in.mg_max_iter));
+    HIPCHK(amr::launch_ctl_reset_step(c));
+    for (int s = 0; s < nsteps; ++s) {
+        amressif_lin_stats st{};
+        const double *ustar = tape->slot(first + s);
+        w.base_u = ustar;
+        w.base_v = ustar + tape->fsz;
+        HIPCHK(amr::launch_cells(c, w.base_u, w.base_v, w.XU, w.XV, 0, nullptr, 0));
+        // du* from (A_u - 1.5/dt) du* = G dp - dc - A_g dg
+        HIPCHK(amr::launch_tl_rhs(c, w, c.Du[0], c.Du[1], c.Dup[0], c.Dup[1], c.Dp, w.L[0][0], w.L[0][1]));
+        double *xu = nullptr, *xv = nullptr;
+        if (int rc = lin_solve(ctx, false, lid_carries_g(c) ? c.Dlid : nullptr, &xu, &xv, &st)) return rc;
+        // projection of du*: the step's own Poisson RHS, solve and update
+        int nb = 0;
+        HIPCHK(amr::launch_poisson_rhs(c, xu, xv, c.b, &nb));
+        HIPCHK(amr::launch_fin_prhs(c, nb));
+        HIPCHK(amr::launch_center_b(c, c.b));
+        if (int rc = poisson_loop(ctx)) return rc;
+        std::swap(c.Du[0], c.Dup[0]);  // du_prev <- du; du is rewritten below
+        std::swap(c.Du[1], c.Dup[1]);
+        HIPCHK(amr::launch_project_ex(c, xu, xv, c.Du[0], c.Du[1], c.Dp, c.phi, 1, &nb));
+        st.vcycles = c.ctl_host->mg_iters;
+        st.flags = c.ctl_host->flags;
+        if (stats) stats[s] = st;
+    }
+    HIPCHK(hipStreamSynchronize(c.stream));
+    return AMRESSIF_OK;
+}
+
+int amressif_step_bwd(amressif_ctx *ctx, amressif_tape *tape, int nsteps, amressif_lin_stats *stats)
+{
+    if (!ctx) return AMRESSIF_ERR_ARG;
+    if (int rc = check_tape(ctx, tape)) return rc;
+    if (nsteps < 0 || nsteps > tape->count) return fail(ctx, AMRESSIF_ERR_ARG, "more steps than the tape holds");
+    Ctx &c = ctx->c;
+    amr::AdjWork &w = *c.adj;
+    HIPCHK(amr::launch_ctl_set(c, c.in.momentum_tolerance, c.in.poisson_tol_rel, c.in.poisson_tol_abs, c.in.IterNum,
+                               c.in.mg_max_iter));
+    HIPCHK(amr::launch_ctl_reset_step(c));
+    for (int s = 0; s < nsteps; ++s) {
+        amressif_lin_stats st{};
+        const double *ustar = tape->slot(tape->count - 1);
+        w.base_u = ustar;
+        w.base_v = ustar + tape->fsz;
+        HIPCHK(amr::launch_cells(c, w.base_u, w.base_v, w.XU, w.XV, 0, nullptr, 0));
+        // projection: phi_bar = C p_bar + (dt/1.5) D u_bar, b_bar = L^+ phi_bar (same solve)
+        HIPCHK(amr::launch_adj_phibar(c, c.Du[0], c.Du[1], c.Dp, c.b));
+        HIPCHK(amr::launch_mean_b(c, c.b));
+        HIPCHK(amr::launch_center_b(c, c.b));
+        if (int rc = poisson_loop(ctx)) return rc;
+        st.vcycles = c.ctl_host->mg_iters;
+        // momentum: (A_u - 1.5/dt)^T s = u_bar - (1.5/dt) G b_bar
+        HIPCHK(amr::launch_adj_rhs(c, w, c.Du[0], c.Du[1], c.phi, w.L[0][0], w.L[0][1]));
+        double *su = nullptr, *sv = nullptr;
+        if (int rc = lin_solve(ctx, true, nullptr, &su, &sv, &st)) return rc;
+        if (lid_carries_g(c)) HIPCHK(amr::launch_adj_lid(c, w, su, sv, c.Dlid));
+        HIPCHK(amr::launch_adj_finish(c, su, sv, c.Du[0], c.Du[1], c.Dup[0], c.Dup[1], c.Dp));
+        st.flags = c.ctl_host->flags;
+        tape->count -= 1;
+        if (stats) stats[s] = st;
+    }
+    HIPCHK(hipStreamSynchronize(c.stream));
     return AMRESSIF_OK;
 }
 

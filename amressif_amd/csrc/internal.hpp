@@ -29,6 +29,10 @@ struct Ctl {
     double div_new;
     double tol_mom, tol_rel, tol_abs;
     int iter_num, mg_max_iter;
+    // linear (tangent / adjoint) Richardson solves
+    int lin_done, lin_count, lin_cap;
+    int gm_stop;  // GMRES: the current Arnoldi cycle is finished (V-cycle kernels skip)
+    double lin_tol, lin_rel;
 };
 
 // Pitched 2D device arrays.  Fine level: x-faces rows j in [0,n), cols
@@ -52,6 +56,27 @@ struct Geometry {
 constexpr int kMaxLevels = 24;
 constexpr int kMaxPartials = 1 << 16;
 
+// Restarted FGMRES state (kernels_gmres.hip), device memory
+constexpr int kGmMax = 64;  // largest restart length
+struct GmState {
+    int j, total, pad0, pad1;  // Arnoldi step in the cycle, steps in the solve
+    double beta, target, scale;
+    double gv[kGmMax + 1], cs[kGmMax], sn[kGmMax], y[kGmMax], c[kGmMax + 1];
+    double H[(kGmMax + 1) * kGmMax];  // Hessenberg, row-major, reduced to R by Givens rotations
+};
+
+// Working arrays of the tangent-linear and adjoint step (kernels_adj.hip)
+struct AdjWork {
+    const double *base_u = nullptr, *base_v = nullptr;  // recorded u* of the step (faces)
+    double *XU = nullptr, *XV = nullptr;                // base cell values, two ghost layers
+    double *DU = nullptr, *DV = nullptr;                // perturbation cell values, two ghost layers
+    double *F[4]{};                                     // face fluxes or flux adjoints (xU, xV, yU, yV)
+    double *Q[2]{};                                     // right-hand side of the linear system (faces)
+    double *XbU = nullptr, *XbV = nullptr;              // cell adjoints (interior)
+    double *lidg = nullptr;                             // north-ghost adjoints of U (2 n)
+    double *L[2][2]{};                                  // Richardson iterates (ping-pong, 2 face arrays)
+};
+
 struct Ctx {
     amressif_inputs in{};
     hipStream_t stream = nullptr;
@@ -63,6 +88,20 @@ struct Ctx {
     // its residual
     float *uk32[2]{}, *r32[2]{};
     double *p = nullptr, *phi = nullptr, *b = nullptr;
+    // dual state of the tangent-linear / adjoint step (faces, faces, cells, lid)
+    double *Du[2]{}, *Dup[2]{}, *Dp = nullptr, *Dlid = nullptr;
+    AdjWork *adj = nullptr;  // their working arrays (allocated on first use)
+    int Pg = 0;              // pitch of the cell arrays with two ghost layers
+    double adj_tol = 1e-14;  // Richardson tolerance of the tangent/adjoint momentum solve
+    int adj_cap = 200;       // and its sweep cap
+    int last_L = 4;          // sweeps of the previous linear solve (batch size)
+    int stage_rows = 16;     // stage kernels: output rows per wave
+    int stage_order = 1;     // stage kernels: 1 = consecutive waves take horizontally adjacent bands
+    // FGMRES Poisson solver (poisson_solver = 1)
+    int gm_m = 0;            // restart length
+    double *gmV = nullptr, *gmZ = nullptr, *gm_part = nullptr;
+    size_t gm_vs = 0;        // doubles between consecutive vectors
+    GmState *gm = nullptr;
     double *phiv[2]{};   // Poisson iterates (ping-pong); phi points at the last solution
     int tail_level = 0;  // first multigrid level handled by the single-workgroup tail kernel
     const double *lid = nullptr;  // caller-owned device lid profile or null
@@ -146,5 +185,30 @@ hipError_t launch_unpack_cell_shift(const Ctx &c, const double *src, const amr_a
 hipError_t launch_copy_bytes(const Ctx &c, const void *src, void *dst, int64_t bytes);
 hipError_t launch_mean_b(const Ctx &c, const double *b);
 hipError_t launch_bdf2_const(const Ctx &c, const double *un, const double *um, double *cc, int dir);
+
+// FGMRES Poisson solver (kernels_gmres.hip)
+size_t gm_partials_doubles(const Ctx &c, int m);
+hipError_t launch_gm_reset(const Ctx &c);
+hipError_t launch_gm_cycle_start(const Ctx &c, const double *x, const double *b, int mixed);
+hipError_t launch_gm_iteration(const Ctx &c, int mixed);
+hipError_t launch_gm_cycle_end(const Ctx &c, double *x);
+
+// tangent-linear / adjoint step (kernels_adj.hip)
+int lin_partials_count(const Ctx &c);
+hipError_t launch_lin_start(const Ctx &c, double tol, int cap);
+hipError_t launch_cells(const Ctx &c, const double *fu, const double *fv, double *XU, double *XV, int homog,
+                        const double *dlid, int guarded);
+hipError_t launch_tl_sweep(const Ctx &c, const AdjWork &w, const double *xu, const double *xv, double *xu_new,
+                           double *xv_new, const double *dlid);
+hipError_t launch_adj_sweep(const Ctx &c, const AdjWork &w, const double *su, const double *sv, double *su_new,
+                            double *sv_new);
+hipError_t launch_adj_lid(const Ctx &c, const AdjWork &w, const double *su, const double *sv, double *glid);
+hipError_t launch_tl_rhs(const Ctx &c, const AdjWork &w, const double *du, const double *dv, const double *dup,
+                         const double *dvp, const double *dp, double *xu, double *xv);
+hipError_t launch_adj_phibar(const Ctx &c, const double *au, const double *av, const double *pb, double *b);
+hipError_t launch_adj_rhs(const Ctx &c, const AdjWork &w, const double *au, const double *av, const double *bb,
+                          double *su, double *sv);
+hipError_t launch_adj_finish(const Ctx &c, const double *su, const double *sv, double *au, double *av, double *aup,
+                             double *avp, double *pb);
 
 }  // namespace amr

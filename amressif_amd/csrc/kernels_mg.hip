@@ -34,39 +34,60 @@ constexpr int SW = SX + 2 * SHALO, SHT = SY + 2 * SHALO;  // 72 x 24 LDS region
 
 // x_out = GSRB^2(x_in + P(ec)) for L x = r on one level.  x_in null: start
 // from zero; ec null: no coarse correction.  Red (i+j even) then black, twice.
+// The 72 x 24 LDS region is split into horizontal cell pairs (one red, one
+// black); thread t owns pairs t, t + 256, ... so every half-sweep is a fixed
+// loop of at most four updates per thread with no index division.  Cells of
+// one colour read only the other colour, so the update order inside a
+// half-sweep does not change the result.
+constexpr int SPAIRS = SW / 2 * SHT;                   // 864 pairs
+constexpr int SPER = (SPAIRS + 255) / 256;             // pairs per thread
+
 template <class T>
 __global__ __launch_bounds__(256) void k_smooth(const T *xin, const T *ec, int pc, T *xout, const T *rhs, int nx,
                                                 int ny, int pitch, T ax, T ay, const Ctl *ctl)
 {
-    if (ctl->mg_done) return;
+    if (ctl->mg_done | ctl->gm_stop) return;
     __shared__ T sx[SHT][SW], sr[SHT][SW];
     const int tid = threadIdx.x;
     const int i0 = blockIdx.x * SX, j0 = blockIdx.y * SY;
-    for (int e = tid; e < SW * SHT; e += 256) {
-        const int li = e % SW, lj = e / SW, i = i0 - SHALO + li, j = j0 - SHALO + lj;
-        T x = 0, r = 0;
-        if (i >= 0 && i < nx && j >= 0 && j < ny) {
-            const size_t k = i + (size_t)pitch * j;
-            if (xin) x = xin[k];
-            if (ec) x += ec[(i >> 1) + (size_t)pc * (j >> 1)];
-            r = rhs[k];
+    int pl[SPER], pj[SPER];  // pair column (cell 2 pl) and row in the region; -1: none
+#pragma unroll
+    for (int q = 0; q < SPER; ++q) {
+        const int p = tid + 256 * q;
+        pj[q] = p < SPAIRS ? p / (SW / 2) : -1;
+        pl[q] = p % (SW / 2);
+    }
+#pragma unroll
+    for (int q = 0; q < SPER; ++q) {
+        if (pj[q] < 0) continue;
+        const int lj = pj[q], j = j0 - SHALO + lj;
+#pragma unroll
+        for (int c2 = 0; c2 < 2; ++c2) {
+            const int li = 2 * pl[q] + c2, i = i0 - SHALO + li;
+            T x = 0, r = 0;
+            if (i >= 0 && i < nx && j >= 0 && j < ny) {
+                const size_t k = i + (size_t)pitch * j;
+                if (xin) x = xin[k];
+                if (ec) x += ec[(i >> 1) + (size_t)pc * (j >> 1)];
+                r = rhs[k];
+            }
+            sx[lj][li] = x;
+            sr[lj][li] = r;
         }
-        sx[lj][li] = x;
-        sr[lj][li] = r;
     }
     __syncthreads();
     const T inv_d = T(1) / (T(2) * ax + T(2) * ay);
+#pragma unroll
     for (int h = 0; h < 4; ++h) {
-        const int color = h & 1;
-        const int lo = 1 + h, W = SW - 2 * lo, H = SHT - 2 * lo, half = W / 2;
-        for (int e = tid; e < half * H; e += 256) {
-            const int lj = lo + e / half, kk = e % half;
-            const int j = j0 - SHALO + lj;
-            // first column of this colour in the row
-            const int istart = i0 - SHALO + lo;
-            const int par = ((istart + j) & 1) == color ? 0 : 1;
-            const int li = lo + 2 * kk + par, i = i0 - SHALO + li;
-            if (li >= lo + W || i < 0 || i >= nx || j < 0 || j >= ny) continue;
+        const int color = h & 1, lo = 1 + h;
+#pragma unroll
+        for (int q = 0; q < SPER; ++q) {
+            const int lj = pj[q];
+            // i0 - SHALO and j0 - SHALO are even: the colour of (li, lj) is (li + lj) & 1
+            const int li = 2 * pl[q] + ((lj & 1) ^ color);
+            if (lj < lo || lj >= SHT - lo || li < lo || li >= SW - lo) continue;
+            const int i = i0 - SHALO + li, j = j0 - SHALO + lj;
+            if (i < 0 || i >= nx || j < 0 || j >= ny) continue;
             T s = 0;
             if (i > 0 && i < nx - 1 && j > 0 && j < ny - 1) {
                 s = ax * sx[lj][li - 1] + ax * sx[lj][li + 1] + ay * sx[lj - 1][li] + ay * sx[lj + 1][li];
@@ -106,7 +127,7 @@ template <class T>
 __global__ __launch_bounds__(256) void k_resid_restrict(const T *e, const T *r, int nx, int ny, int pf, T ax, T ay,
                                                         T *rc, int ncx, int ncy, int pc, const Ctl *ctl)
 {
-    if (ctl->mg_done) return;
+    if (ctl->mg_done | ctl->gm_stop) return;
     const int I = blockIdx.x * 64 + (threadIdx.x & 63), J = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (I >= ncx || J >= ncy) return;
     const int i = 2 * I, j = 2 * J;
@@ -158,7 +179,7 @@ template <class T>
 __global__ __launch_bounds__(1024) void k_tail(const T *rtop, int ptop, T *etop, TailLevels tl, const T *pinv,
                                                const Ctl *ctl)
 {
-    if (ctl->mg_done) return;
+    if (ctl->mg_done | ctl->gm_stop) return;
     extern __shared__ unsigned char smem_raw[];
     T *E = reinterpret_cast<T *>(smem_raw);
     T *Rr = E + tl.total;
@@ -254,6 +275,7 @@ __global__ __launch_bounds__(256) void k_correct_residual(const double *phi_in, 
 __global__ void k_mg_start(Ctl *ctl)
 {
     ctl->mg_done = 0;
+    ctl->gm_stop = 0;
     ctl->mg_iters = 0;
     ctl->mg_started = 0;
     ctl->res_norm = 0.0;

@@ -33,7 +33,6 @@
 namespace amr {
 
 constexpr int WB = 58;  // output columns per wave (64 lanes = 3 halo + 58 + 3 halo)
-constexpr int RH = 16;  // output rows per wave
 
 // values held by lane - 1 / lane + 1 (DPP wave shifts; the edge lanes get 0)
 __device__ __forceinline__ double lane_prev(double x)
@@ -74,6 +73,8 @@ struct StageArgs {
     double inflow0, inflow1;
     int mode;  // k_stage64: 0 stage, 1 last stage (+ norm), 2 RHS only, 3 mixed residual
     int nbands, nchunks, remap;
+    int rows;   // output rows per wave (chunk height)
+    int order;  // wave -> tile: 0 band-major (chunks of one band consecutive), 1 chunk-major
 };
 
 // wave id -> (band, chunk); blocks of 4 waves, handed to the 8 XCDs in
@@ -130,12 +131,14 @@ __global__ __launch_bounds__(256) void k_stage64(StageArgs a)
     const int lane = threadIdx.x & 63;
     const int wid = wave_id(a);
     if (wid >= a.nbands * a.nchunks) return;  // whole wave
-    const int band = wid / a.nchunks, chunk = wid % a.nchunks;
+    const int band = a.order ? wid % a.nbands : wid / a.nchunks;
+    const int chunk = a.order ? wid / a.nbands : wid % a.nchunks;
     const int n = a.n, P = a.P;
-    const int i0 = band * WB, j0 = chunk * RH;
+    const int i0 = band * WB, j0 = chunk * a.rows;
     const int i = i0 + lane - 3;
     const bool xin = i >= 0 && i < n, xface = i >= 0 && i <= n;
     const bool edge = i0 < 3 || i0 + 61 > n;
+    const bool outlane = lane >= 3 && lane < 3 + WB;
     const double *us = a.us, *vs = a.vs;
     struct Raw {
         R u0, u1, v0, v1, vn, uw, pc, ub, cu, vb, cv;
@@ -153,7 +156,7 @@ __global__ __launch_bounds__(256) void k_stage64(StageArgs a)
         const size_t k = (size_t)(i < 0 ? 0 : i) + (size_t)P * (jc < 0 ? 0 : jc);
         r.uw = (xface && row) ? us[k] : R(0);
         r.pc = (xin && row) ? a.p[k] : R(0);
-        const bool out = row && jc >= j0 && a.mode != 2;
+        const bool out = row && jc >= j0 && a.mode != 2 && outlane;
         r.ub = (out && xface && a.mode != 3) ? a.ub[k] : R(0);
         r.cu = (out && xface) ? a.cu[k] : R(0);
         r.vb = (out && xin && a.mode != 3) ? a.vb[k] : R(0);
@@ -173,8 +176,7 @@ __global__ __launch_bounds__(256) void k_stage64(StageArgs a)
     R fsU = quick_flux<R>(vn, wU[0], wU[1], wU[2], wU[3]), fsV = quick_flux<R>(vn, wV[0], wV[1], wV[2], wV[3]);
     R tVprev = 0, pprev = 0;
     double nx_ = 0.0, ny_ = 0.0;
-    const bool outlane = lane >= 3 && lane < 3 + WB;
-    const int jend = min(j0 + RH, n);
+    const int jend = min(j0 + a.rows, n);
     Raw cur, nxt;
     load(j0 - 1, cur);
     for (int jc = j0 - 1; jc < jend; ++jc) {
@@ -275,9 +277,10 @@ __global__ __launch_bounds__(256) void k_tangent(StageArgs a)
     const int lane = threadIdx.x & 63;
     const int wid = wave_id(a);
     if (wid >= a.nbands * a.nchunks) return;
-    const int band = wid / a.nchunks, chunk = wid % a.nchunks;
+    const int band = a.order ? wid % a.nbands : wid / a.nchunks;
+    const int chunk = a.order ? wid / a.nbands : wid % a.nchunks;
     const int n = a.n, P = a.P;
-    const int i0 = band * WB, j0 = chunk * RH;
+    const int i0 = band * WB, j0 = chunk * a.rows;
     const int i = i0 + lane - 3;
     const bool xin = i >= 0 && i < n, xface = i >= 0 && i <= n;
     const bool edge = i0 < 3 || i0 + 61 > n;
@@ -320,7 +323,7 @@ __global__ __launch_bounds__(256) void k_tangent(StageArgs a)
     R tVprev = 0;
     double nx_ = 0.0, ny_ = 0.0;
     const bool outlane = lane >= 3 && lane < 3 + WB;
-    const int jend = min(j0 + RH, n);
+    const int jend = min(j0 + a.rows, n);
     const double cr = a.cr, cj = a.cj;
     Raw cur, nxt;
     load(j0 - 1, cur);
@@ -395,8 +398,10 @@ static StageArgs base_args(const Ctx &c, double rk, int mode)
     a.bc_lo0 = g.bc_lo0; a.bc_hi0 = g.bc_hi0; a.bc_lo1 = g.bc_lo1; a.bc_hi1 = g.bc_hi1;
     a.inflow0 = g.inflow0; a.inflow1 = g.inflow1;
     a.mode = mode;
+    a.rows = c.stage_rows;
+    a.order = c.stage_order;
     a.nbands = (g.n + 1 + WB - 1) / WB;
-    a.nchunks = (g.n + RH - 1) / RH;
+    a.nchunks = (g.n + a.rows - 1) / a.rows;
     return a;
 }
 
@@ -407,11 +412,6 @@ static dim3 stage_grid(StageArgs &a)
     return dim3(blocks);
 }
 
-int stage_partials(const Ctx &c)
-{
-    const int nb = (c.g.n + 1 + WB - 1) / WB, nc = (c.g.n + RH - 1) / RH;
-    return nb * nc;
-}
 
 hipError_t launch_stage(const Ctx &c, const StageLaunch &s, int *nblocks)
 {

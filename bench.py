@@ -45,6 +45,9 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps timed for cpu_baseline (0: skip)")
     ap.add_argument("--only", choices=["both", "mixed", "fp64"], default="both")
     ap.add_argument("--profile-run", action="store_true", help="short run for rocprofv3: no cpu baseline")
+    ap.add_argument("--adjoint-n", type=int, default=512, help="adjoint leg grid (0: skip the leg)")
+    ap.add_argument("--adjoint-steps", type=int, default=1000)
+    ap.add_argument("--adjoint-every", type=int, default=32, help="checkpoint interval of the adjoint leg")
     ap.add_argument("--kernel-probe", choices=["fp64", "mixed"], default=None,
                     help="for rocprofv3 --pmc: one step, then a few timed launches of each kernel of that "
                          "precision and a 64 MiB copy (calibration); prints their HIP-event times")
@@ -77,6 +80,40 @@ def run_precision(api, W, n, ren, precision, state, warmup, steps, torch):
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     return run, stats, t1 - t0
+
+
+def adjoint_leg(api, W, n, steps, every, torch):
+    """Lid-and-initial-velocity gradient of the lower-left kinetic energy after
+    `steps` FP64 steps (512^2, Re = 1000 by default): forward with checkpoints
+    every `every` steps, then the backward sweep (segment recomputation + adjoint)."""
+    from amressif_amd.adjoint import CheckpointedRollout, kinetic_energy_lower_left
+    inp = api.cavity_inputs(n, 1000.0)
+    ro = CheckpointedRollout(inp, steps, every=every)
+    lid = torch.tensor(W.lid_profile(n), device="cuda")
+    st = W.initial_state(n, "seeded")
+    u0, v0 = torch.tensor(st["u"], device="cuda"), torch.tensor(st["v"], device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    u, v = ro.forward(lid, u0, v0)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    u.requires_grad_(True)
+    v.requires_grad_(True)
+    J = kinetic_energy_lower_left(u, v)
+    gu, gv = torch.autograd.grad(J, (u, v))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    glid, gu0, gv0 = ro.backward(gu, gv)
+    torch.cuda.synchronize()
+    t_bwd = time.perf_counter() - t0
+    import numpy as np
+    return {"workload": f"lid-driven cavity {n}x{n}, Re=1000, seeded start + seeded lid profile, {steps} FP64 "
+                        f"steps, J = lower-left kinetic energy, checkpoints every {every} steps",
+            "forward_s": t_fwd, "backward_s": t_bwd, "ratio": t_bwd / t_fwd, "J": float(J),
+            "grad_lid_norm": float(glid.norm()), "grad_u0_norm": float(torch.sqrt((gu0 ** 2).sum() + (gv0 ** 2).sum())),
+            "mean_pseudo_iters": float(np.mean([s["pseudo_iters"] for s in ro.fwd_stats])),
+            "mean_adjoint_sweeps": float(np.mean([s["lin_iters"] for s in ro.bwd_stats])),
+            "adjoint_flags": int(np.bitwise_or.reduce([s["flags"] for s in ro.bwd_stats]))}
 
 
 def kernel_probe(api, W, n, ren, which_prec):
@@ -153,6 +190,10 @@ def main():
     if "fp64" in runs:
         extra["fp64_stage"] = roofline if head == "fp64" else kernel_roofline(runs["fp64"], 0, "stage_fp64")
 
+    adj = None
+    if args.adjoint_n > 0 and not args.profile_run:
+        adj = adjoint_leg(api, W, args.adjoint_n, args.adjoint_steps, args.adjoint_every, torch)
+
     cpu = None
     if args.cpu_steps > 0 and not args.profile_run:
         from oracle.refcpu import Oracle
@@ -188,7 +229,8 @@ def main():
         "mixed_vs_fp64_speedup": (out["mixed"]["mcell_steps_per_s"] / out["fp64"]["mcell_steps_per_s"]
                                   if "mixed" in out and "fp64" in out else None),
         "mixed_vs_fp64_rel_err": err,
-        "adjoint_forward_ratio": None,
+        "adjoint_forward_ratio": adj["ratio"] if adj else None,
+        "adjoint": adj,
         "roofline": roofline,
         "kernel_rooflines": extra,
         "cpu_baseline": cpu,

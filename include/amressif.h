@@ -44,6 +44,7 @@ extern "C" {
 #define AMRESSIF_FLAG_DIVERGED 2  /* normError > 1e2 (main.cpp:1035-1039) */
 #define AMRESSIF_FLAG_NONFINITE 4 /* NaN/Inf norm: the reference's while test exits silently (main.cpp:898) */
 #define AMRESSIF_FLAG_MG_CAP 8    /* Poisson iteration cap reached before tolerance */
+#define AMRESSIF_FLAG_LIN_CAP 16  /* tangent/adjoint momentum solve reached its sweep cap */
 
 /* The reference's ParmParse keys (Source/src1_rever/main.cpp:89-157) plus the
  * solver settings it hard-codes (poisson.cpp:159-182) and this build's
@@ -76,6 +77,10 @@ typedef struct {
     int mg_bottom_max;          /* coarsen while the level is larger than this and even; 16 */
     /* this build */
     int precision;              /* AMRESSIF_FP64 or AMRESSIF_MIXED */
+    int poisson_solver;         /* 0: multigrid V-cycle iteration (the MLMG solve, default);
+                                   1: FGMRES(gmres_restart) right-preconditioned by one V-cycle
+                                   (the solver GMRES_Poisson.H:14-76 sets up; same operator) */
+    int gmres_restart;          /* restart length, 1..64; default 30 */
 } amressif_inputs;
 
 /* AMReX Array4 view of one box: ptr addresses element (lo[0], lo[1], lo[2], comp 0);
@@ -172,6 +177,59 @@ int amressif_poisson_solve(amressif_ctx *ctx, amr_array4 *sol, const amr_array4 
  * velCont = velStar - G phi dt/1.5, p += phi.  Returns ||D(velCont)||inf. */
 int amressif_project(amressif_ctx *ctx, const amr_array4 velStar[2], amr_array4 *userCtx,
                      amr_array4 velCont[2], double *div_linf);
+
+/* ------------------------------------------------- discrete adjoint
+ * No reference counterpart (RV has no adjoint).  The step is the map
+ * (u^n, u^{n-1}, p^n; lid g) -> (u^{n+1}, u^n, p^{n+1}) of amressif_advance.
+ * Its linearisation is taken at the converged momentum predictor u* (QUICK
+ * upwind switch frozen at u*), so the tangent and adjoint are those of the
+ * converged step: the momentum system (A_u - 1.5/dt) du* = G dp - dc - A_g dg
+ * and its transpose are solved by Richardson sweeps to rel. change
+ * <= adjoint_tol; the projection reuses the step's multigrid solve (the
+ * Neumann Laplacian is symmetric).
+ *
+ * A tape holds the u* of consecutive forward steps (2 face arrays per step,
+ * device memory owned by the tape).  Checkpointing is the caller's: restore a
+ * checkpoint with amressif_load_state, re-run its segment with
+ * amressif_step_fwd into a reset tape, then amressif_step_bwd over it.
+ *
+ * The dual state (faces du[2], du_prev[2], cells dp, lid dlid) lives in the
+ * context: a perturbation for amressif_step_tlm, an adjoint for
+ * amressif_step_bwd (which accumulates into dlid). */
+typedef struct amressif_tape amressif_tape;
+
+typedef struct {
+    int lin_iters;   /* Richardson sweeps of the momentum system */
+    int vcycles;     /* V-cycles of the projection solve */
+    int flags;       /* AMRESSIF_FLAG_LIN_CAP, AMRESSIF_FLAG_MG_CAP, AMRESSIF_FLAG_NONFINITE */
+    int pad;
+    double lin_rel;  /* last relative change of the momentum solve */
+} amressif_lin_stats;
+
+int amressif_tape_create(amressif_ctx *ctx, int max_steps, amressif_tape **out);
+int amressif_tape_destroy(amressif_tape *tape);
+int amressif_tape_reset(amressif_tape *tape);
+int amressif_tape_steps(const amressif_tape *tape);
+/* Forward of the pair: amressif_advance, also appending each step's u* to the
+ * tape (ERR_ARG if the tape would overflow). */
+int amressif_step_fwd(amressif_ctx *ctx, int nsteps, amressif_tape *tape, amressif_step_stats *stats);
+/* Backward of the pair: the adjoint of the tape's last nsteps steps, newest
+ * first, applied to the dual state; the steps are removed from the tape.
+ * stats: host array of nsteps entries (newest first) or NULL. */
+int amressif_step_bwd(amressif_ctx *ctx, amressif_tape *tape, int nsteps, amressif_lin_stats *stats);
+/* Tangent-linear of tape steps [first, first + nsteps) in forward order,
+ * applied to the dual state (dlid is the constant lid perturbation). */
+int amressif_step_tlm(amressif_ctx *ctx, const amressif_tape *tape, int first, int nsteps,
+                      amressif_lin_stats *stats);
+/* Dual state I/O: faces du[2], du_prev[2] (0 ghosts), cells dp (1 comp,
+ * >= 0 ghosts), dlid (n_cell doubles, device).  NULL loads zeros / skips. */
+int amressif_load_dual(amressif_ctx *ctx, const amr_array4 du[2], const amr_array4 du_prev[2],
+                       const amr_array4 *dp, const double *dlid);
+int amressif_store_dual(amressif_ctx *ctx, amr_array4 du[2], amr_array4 du_prev[2], amr_array4 *dp,
+                        double *dlid);
+/* Richardson tolerance and sweep cap of the tangent/adjoint momentum solve
+ * (defaults 1e-14 and 200). */
+int amressif_set_adjoint_solver(amressif_ctx *ctx, double tol, int max_sweeps);
 
 /* ------------------------------------------------------------ diagnostics
  * No reference counterpart (the reference prints per-step wall time only,

@@ -21,6 +21,7 @@ import ctypes
 import os
 from dataclasses import dataclass
 
+import numpy as np
 import torch
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -109,6 +110,21 @@ class LinStats(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_ if f != "pad"}
 
 
+class AmrStats(ctypes.Structure):
+    _fields_ = [
+        ("coarse_iters", ctypes.c_int),
+        ("fine_iters", ctypes.c_int),
+        ("poisson_iters", ctypes.c_int),
+        ("flags", ctypes.c_int),
+        ("div_star", ctypes.c_double),
+        ("div_new", ctypes.c_double),
+        ("poisson_res", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 _P = ctypes.POINTER
 _A4 = _P(Array4)
 _c = ctypes
@@ -144,6 +160,13 @@ _sig = {
     "amressif_load_dual": (_c.c_int, [_c.c_void_p, _A4, _A4, _A4, _c.c_void_p]),
     "amressif_store_dual": (_c.c_int, [_c.c_void_p, _A4, _A4, _A4, _c.c_void_p]),
     "amressif_set_adjoint_solver": (_c.c_int, [_c.c_void_p, _c.c_double, _c.c_int]),
+    "amressif_amr_create": (_c.c_int, [_P(Inputs), _c.c_int, _c.c_void_p, _P(_c.c_void_p)]),
+    "amressif_amr_destroy": (_c.c_int, [_c.c_void_p]),
+    "amressif_amr_last_error": (_c.c_char_p, [_c.c_void_p]),
+    "amressif_amr_dt": (_c.c_int, [_c.c_void_p, _P(_c.c_double)]),
+    "amressif_amr_load_state": (_c.c_int, [_c.c_void_p, _A4, _A4, _A4, _A4]),
+    "amressif_amr_store_state": (_c.c_int, [_c.c_void_p, _A4, _A4, _A4, _A4]),
+    "amressif_amr_advance": (_c.c_int, [_c.c_void_p, _c.c_int, _P(AmrStats)]),
 }
 for _name, (_res, _args) in _sig.items():
     _f = getattr(_lib, _name)
@@ -437,6 +460,80 @@ class Tape:
     def close(self):
         if self._h and self._h.value:
             _lib.amressif_tape_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class TwoLevel:
+    """Coarse n0^2 level + a ratio-2 patch over the coarse rows [jc0, n0)
+    (amressif_amr_*), state held in torch device tensors:
+    u0 (n0, n0+1), v0 (n0+1, n0), ctx0 (2, n0+2, n0+2); u1 (ny, 2n0+1),
+    v1 (ny+1, 2n0), p1 (ny, 2n0) with ny = 2 (n0 - jc0)."""
+
+    def __init__(self, inp: Inputs, jc0: int, state: dict, device="cuda"):
+        if not torch.cuda.is_available():
+            raise RuntimeError("amressif_amd needs a GPU: no HIP device is visible")
+        self.n0, self.jc0 = inp.n_cell, jc0
+        self.nx, self.ny = 2 * self.n0, 2 * (self.n0 - jc0)
+        h = ctypes.c_void_p()
+        rc = _lib.amressif_amr_create(ctypes.byref(inp), jc0, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream),
+                                      ctypes.byref(h))
+        self._h = h
+        if rc:
+            msg = _lib.amressif_amr_last_error(h).decode() if h.value else ""
+            _lib.amressif_amr_destroy(h)
+            self._h = ctypes.c_void_p()
+            raise AmressifError(rc, msg)
+        t = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device=device)  # noqa: E731
+        n0, nx, ny = self.n0, self.nx, self.ny
+        self.u0, self.v0 = t(state["u0"]), t(state["v0"])
+        self.ctx0 = t(state.get("ctx0", np.zeros((2, n0 + 2, n0 + 2))))
+        self.u1, self.v1 = t(state["u1"]), t(state["v1"])
+        self.p1 = t(state.get("p1", np.zeros((ny, nx))))
+        self._check(_lib.amressif_amr_load_state(self._h, faces_view(self.u0, self.v0, n0),
+                                                 ctypes.byref(cell_view(self.ctx0, n0, 1)), self._fine_faces(),
+                                                 ctypes.byref(self._fine_p())))
+
+    def _fine_faces(self):
+        arr = (Array4 * 2)()
+        arr[0] = _a4(self.u1, (0, 0), (self.nx, self.ny - 1), 1)
+        arr[1] = _a4(self.v1, (0, 0), (self.nx - 1, self.ny), 1)
+        return arr
+
+    def _fine_p(self):
+        return _a4(self.p1, (0, 0), (self.nx - 1, self.ny - 1), 1)
+
+    def _check(self, rc):
+        if rc:
+            raise AmressifError(rc, _lib.amressif_amr_last_error(self._h).decode())
+
+    @property
+    def dt(self) -> float:
+        d = ctypes.c_double()
+        self._check(_lib.amressif_amr_dt(self._h, ctypes.byref(d)))
+        return d.value
+
+    def advance(self, nsteps: int):
+        st = (AmrStats * max(1, nsteps))()
+        self._check(_lib.amressif_amr_advance(self._h, nsteps, st))
+        return [st[k].as_dict() for k in range(nsteps)]
+
+    def fetch(self):
+        n0 = self.n0
+        self._check(_lib.amressif_amr_store_state(self._h, faces_view(self.u0, self.v0, n0),
+                                                  ctypes.byref(cell_view(self.ctx0, n0, 1)), self._fine_faces(),
+                                                  ctypes.byref(self._fine_p())))
+        return {"u0": self.u0.cpu().numpy(), "v0": self.v0.cpu().numpy(), "ctx0": self.ctx0.cpu().numpy(),
+                "u1": self.u1.cpu().numpy(), "v1": self.v1.cpu().numpy(), "p1": self.p1.cpu().numpy()}
+
+    def close(self):
+        if self._h and self._h.value:
+            _lib.amressif_amr_destroy(self._h)
             self._h = ctypes.c_void_p()
 
     def __del__(self):

@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libamressif_hip.so")
-SOURCES = ["kernels_stage.hip", "kernels_step.hip", "kernels_mg.hip", "kernels_adj.hip", "kernels_gmres.hip", "api.cpp"]
+SOURCES = ["kernels_stage.hip", "kernels_step.hip", "kernels_mg.hip", "kernels_adj.hip", "kernels_gmres.hip", "kernels_amr.hip", "api.cpp", "amr_two_level.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

@@ -22,9 +22,6 @@
 
 #include "internal.hpp"
 
-struct amressif_ctx {
-    amr::Ctx c;
-};
 
 // u* of consecutive recorded steps, 2 face arrays (u then v) per step
 struct amressif_tape {
@@ -268,6 +265,43 @@ int mg_loop(amressif_ctx *ctx)
 // pseudo base u^k) in place.  Mixed: the FP64 residual of U, then the FP32
 // tangent-linear update of U in place (first order in the RK4 recursion).
 int advance_impl(amressif_ctx *ctx, int nsteps, amressif_step_stats *stats, amressif_tape *tape);
+int launch_pseudo_iteration(amressif_ctx *ctx, int it, int *nb);
+}  // namespace
+
+// BDF2 history and the RK4 dual-time pseudo-iterations of one step
+// (main.cpp:879-884, 898-1040): c.U holds the converged predictor u* after it,
+// the control block copy c.ctl_host its count and flags
+int amr_momentum_predictor(amressif_ctx *ctx)
+{
+    Ctx &c = ctx->c;
+    HIPCHK(amr::launch_ctl_reset_step(c));
+    HIPCHK(amr::launch_bdf2(c));
+    int launched = 0, target = std::max(1, c.last_P), nb = 0;
+    const int cap = c.in.IterNum + 1;
+    for (;;) {
+        while (launched < target && launched < cap) {
+            if (int rc = launch_pseudo_iteration(ctx, launched, &nb)) return rc;
+            HIPCHK(amr::launch_fin_pseudo(c, nb));
+            ++launched;
+        }
+        if (int rc = poll(ctx)) return rc;
+        if (c.ctl_host->mom_done || launched >= cap) break;
+        target = launched + 2;
+    }
+    c.last_P = std::max(1, c.ctl_host->mom_count);
+    return 0;
+}
+
+int amr_poll(amressif_ctx *ctx) { return poll(ctx); }
+int amr_set_ctl(amressif_ctx *ctx)
+{
+    Ctx &c = ctx->c;
+    HIPCHK(amr::launch_ctl_set(c, c.in.momentum_tolerance, c.in.poisson_tol_rel, c.in.poisson_tol_abs, c.in.IterNum,
+                               c.in.mg_max_iter));
+    return 0;
+}
+
+namespace {
 
 int launch_pseudo_iteration(amressif_ctx *ctx, int it, int *nb)
 {
@@ -454,6 +488,8 @@ int amressif_create(const amressif_inputs *in, void *hip_stream, amressif_ctx **
     c.stream = (hipStream_t)hip_stream;
     // tuning overrides of the stage kernels' decomposition (launch shape only)
     if (const char *e = std::getenv("AMRESSIF_STAGE_ROWS")) c.stage_rows = std::max(4, std::min(256, std::atoi(e)));
+    if (const char *e = std::getenv("AMRESSIF_TANGENT_ROWS"))
+        c.tangent_rows = std::max(4, std::min(256, std::atoi(e)));
     if (const char *e = std::getenv("AMRESSIF_STAGE_ORDER")) c.stage_order = std::atoi(e) ? 1 : 0;
     auto bail = [&](int code, const std::string &m) {
         // keep the context so the caller can read the message; it owns nothing yet
@@ -689,22 +725,8 @@ int advance_impl(amressif_ctx *ctx, int nsteps, amressif_step_stats *stats, amre
     int status = AMRESSIF_OK;
     const bool mixed = c.in.precision == AMRESSIF_MIXED;
     for (int s = 0; s < nsteps; ++s) {
-        HIPCHK(amr::launch_ctl_reset_step(c));
-        HIPCHK(amr::launch_bdf2(c));
-        // ---- momentum: RK4 dual-time pseudo-iterations (main.cpp:898-1040)
-        int launched = 0, target = std::max(1, c.last_P), nb = 0;
-        const int cap = c.in.IterNum + 1;
-        for (;;) {
-            while (launched < target && launched < cap) {
-                if (int rc = launch_pseudo_iteration(ctx, launched, &nb)) return rc;
-                HIPCHK(amr::launch_fin_pseudo(c, nb));
-                ++launched;
-            }
-            if (int rc = poll(ctx)) return rc;
-            if (c.ctl_host->mom_done || launched >= cap) break;
-            target = launched + 2;
-        }
-        c.last_P = std::max(1, c.ctl_host->mom_count);
+        int nb = 0;
+        if (int rc = amr_momentum_predictor(ctx)) return rc;
         const int flags = c.ctl_host->flags;
         if (tape) {  // record the converged predictor u* of this step
             const size_t bytes = tape->fsz * sizeof(double);

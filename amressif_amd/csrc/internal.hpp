@@ -95,7 +95,8 @@ struct Ctx {
     double adj_tol = 1e-14;  // Richardson tolerance of the tangent/adjoint momentum solve
     int adj_cap = 200;       // and its sweep cap
     int last_L = 4;          // sweeps of the previous linear solve (batch size)
-    int stage_rows = 16;     // stage kernels: output rows per wave
+    int stage_rows = 8;      // k_stage64: output rows per wave
+    int tangent_rows = 16;   // k_tangent: output rows per wave
     int stage_order = 1;     // stage kernels: 1 = consecutive waves take horizontally adjacent bands
     // FGMRES Poisson solver (poisson_solver = 1)
     int gm_m = 0;            // restart length
@@ -193,6 +194,48 @@ hipError_t launch_gm_cycle_start(const Ctx &c, const double *x, const double *b,
 hipError_t launch_gm_iteration(const Ctx &c, int mixed);
 hipError_t launch_gm_cycle_end(const Ctx &c, double *x);
 
+// two-level composite step (kernels_amr.hip): coarse level n0 x n0, fine patch
+// over the coarse rows [jc0, n0) at ratio 2
+struct AmrGeom {
+    int n0, P0, jc0;        // coarse cells per side, pitch, first covered row
+    int nx, ny, P1, Pg1;    // fine patch cells, pitch, ghosted-cell pitch
+    double h0, h1;          // spacings
+    double iren, c15;
+    int bc_w, bc_e, bc_n;   // wall codes of the patch sides (the coarse domain's)
+    double inflow0, inflow1;
+};
+int amr_flat_blocks(const AmrGeom &a);
+hipError_t amr_fine_cells(const AmrGeom &a, hipStream_t s, const double *u, const double *v, const double *u0,
+                          const double *v0, double *XU, double *XV);
+hipError_t amr_fine_interface(const AmrGeom &a, hipStream_t s, double *v, const double *v0);
+hipError_t amr_fine_stage(const AmrGeom &a, hipStream_t s, const double *us, const double *vs, const double *ub,
+                          const double *vb, const double *cu, const double *cv, const double *p, const double *XU,
+                          const double *XV, double *uo, double *vo, double rk, int mode, double *partials,
+                          const Ctl *ctl, int *nblocks);
+hipError_t amr_fine_fin(const AmrGeom &a, hipStream_t s, const double *partials, int nb, Ctl *ctl);
+hipError_t amr_fine_ctl_reset(hipStream_t s, Ctl *ctl, double tol, int iter_num);
+hipError_t amr_fine_bdf2(const AmrGeom &a, hipStream_t s, double *u, double *up, double *c, int dir, double dt);
+hipError_t amr_apply(const AmrGeom &a, hipStream_t s, const double *x0, const double *x1, const double *b0,
+                     const double *b1, double *y0, double *y1, double *partials);
+hipError_t amr_dots(const AmrGeom &a, hipStream_t s, const double *w, const double *V, size_t vs, int nv,
+                    size_t off1, double *partials);
+hipError_t amr_combine(const AmrGeom &a, hipStream_t s, double *w, double alpha, const double *V, size_t vs, int nv,
+                       const double *coef, size_t off1);
+hipError_t amr_wsum(const AmrGeom &a, hipStream_t s, const double *x0, const double *x1, double *partials);
+hipError_t amr_shift(const AmrGeom &a, hipStream_t s, double *x0, double *x1, const double *mean);
+hipError_t amr_restrict(const AmrGeom &a, hipStream_t s, const double *r0, const double *r1, void *rc, int mixed);
+hipError_t amr_prolong(const AmrGeom &a, hipStream_t s, const void *e, double *z0, double *z1, int mixed);
+hipError_t amr_fine_gs(const AmrGeom &a, hipStream_t s, const double *z0, double *z1, const double *r1, int color);
+hipError_t amr_rhs(const AmrGeom &a, hipStream_t s, double dt, const double *u0, const double *v0, const double *u1,
+                   const double *v1, double *b0, double *b1, double *partials);
+hipError_t amr_project(const AmrGeom &a, hipStream_t s, double dt, const double *phi0, const double *phi1, double *u0,
+                       double *v0, double *u1, double *v1, double *p0, double *p1);
+hipError_t amr_average_down(const AmrGeom &a, hipStream_t s, const double *u1, const double *v1, const double *p1,
+                            double *u0, double *v0, double *p0);
+hipError_t launch_reduce_cols(hipStream_t s, const double *partials, int nblk, int ncol, int is_max, double *out);
+hipError_t launch_pack_rect(hipStream_t s, const amr_array4 &v, double *dst, int P, int nx, int ny);
+hipError_t launch_unpack_rect(hipStream_t s, const double *src, const amr_array4 &v, int P, int nx, int ny);
+
 // tangent-linear / adjoint step (kernels_adj.hip)
 int lin_partials_count(const Ctx &c);
 hipError_t launch_lin_start(const Ctx &c, double tol, int cap);
@@ -212,3 +255,12 @@ hipError_t launch_adj_finish(const Ctx &c, const double *su, const double *sv, d
                              double *avp, double *pb);
 
 }  // namespace amr
+
+// the C-ABI context handle (include/amressif.h) and host helpers shared by
+// the single-level driver (api.cpp) and the two-level driver (amr_two_level.cpp)
+struct amressif_ctx {
+    amr::Ctx c;
+};
+int amr_momentum_predictor(amressif_ctx *ctx);
+int amr_poll(amressif_ctx *ctx);
+int amr_set_ctl(amressif_ctx *ctx);

@@ -379,7 +379,7 @@ __global__ __launch_bounds__(256) void k_tangent(StageArgs a)
     if (lane == 0) { a.partials[2 * wid] = nx_; a.partials[2 * wid + 1] = ny_; }
 }
 
-static StageArgs base_args(const Ctx &c, double rk, int mode)
+static StageArgs base_args(const Ctx &c, double rk, int mode, int rows)
 {
     const Geometry &g = c.g;
     StageArgs a{};
@@ -398,7 +398,7 @@ static StageArgs base_args(const Ctx &c, double rk, int mode)
     a.bc_lo0 = g.bc_lo0; a.bc_hi0 = g.bc_hi0; a.bc_lo1 = g.bc_lo1; a.bc_hi1 = g.bc_hi1;
     a.inflow0 = g.inflow0; a.inflow1 = g.inflow1;
     a.mode = mode;
-    a.rows = c.stage_rows;
+    a.rows = rows;
     a.order = c.stage_order;
     a.nbands = (g.n + 1 + WB - 1) / WB;
     a.nchunks = (g.n + a.rows - 1) / a.rows;
@@ -415,7 +415,7 @@ static dim3 stage_grid(StageArgs &a)
 
 hipError_t launch_stage(const Ctx &c, const StageLaunch &s, int *nblocks)
 {
-    StageArgs a = base_args(c, s.rk, s.mode);
+    StageArgs a = base_args(c, s.rk, s.mode, c.stage_rows);
     a.us = s.us; a.vs = s.vs; a.ub = s.ub; a.vb = s.vb; a.cu = s.cu; a.cv = s.cv; a.p = s.p;
     a.uo = s.uo; a.vo = s.vo;
     if (s.mode == 2 || s.unguarded) a.ctl = nullptr;
@@ -428,7 +428,7 @@ hipError_t launch_stage(const Ctx &c, const StageLaunch &s, int *nblocks)
 
 hipError_t launch_stage_mixed(const Ctx &c, const MixedStageLaunch &s, int *nblocks)
 {
-    StageArgs a = base_args(c, 0.0, 3);
+    StageArgs a = base_args(c, 0.0, 3, s.stage == 0 ? c.stage_rows : c.tangent_rows);
     if (s.unguarded) a.ctl = nullptr;
     const dim3 grid = stage_grid(a);
     if ((int64_t)a.nbands * a.nchunks > kMaxPartials) return hipErrorInvalidValue;

@@ -85,6 +85,34 @@ def lid_profile(n: int, seed: int = SEED + 1, amp: float = 0.05, passes: int = 4
     return np.ascontiguousarray(g)
 
 
+def _psi_nodes(xs: np.ndarray, ys: np.ndarray, seed: int, kmax: int, sigma: float) -> np.ndarray:
+    """psi[j, i] at the nodes (xs[i], ys[j]) of the seeded streamfunction."""
+    a = streamfunction_coeffs(seed, kmax, sigma)
+    k = np.arange(1, kmax + 1)
+    sx = np.sin(np.pi * np.outer(k, xs))
+    sy = np.sin(np.pi * np.outer(k, ys))
+    psi = (sx.T @ a @ sy).T * np.outer(np.sin(np.pi * ys) ** 2, np.sin(np.pi * xs) ** 2)
+    return psi
+
+
+def seeded_two_level(n0: int, jc0: int, seed: int = SEED, kmax: int = 8, sigma: float = 0.1):
+    """The seeded field on a coarse n0^2 grid and on a ratio-2 patch over the
+    coarse rows [jc0, n0): both levels difference the same nodal psi, so the
+    patch is discretely divergence-free, its wall-normal faces are 0 and every
+    coarse face under the patch is the mean of the fine faces on it."""
+    u0, v0 = seeded_velocity(n0, seed, kmax, sigma)
+    nx, ny = 2 * n0, 2 * (n0 - jc0)
+    h = 1.0 / nx
+    xs = np.arange(nx + 1) * h
+    ys = jc0 / n0 + np.arange(ny + 1) * h
+    psi = _psi_nodes(xs, ys, seed, kmax, sigma)
+    psi[:, 0] = psi[:, -1] = 0.0  # west / east walls
+    psi[-1, :] = 0.0              # north wall
+    u1 = np.ascontiguousarray((psi[1:, :] - psi[:-1, :]) / h)   # (ny, nx+1)
+    v1 = np.ascontiguousarray(-(psi[:, 1:] - psi[:, :-1]) / h)  # (ny+1, nx)
+    return {"u0": u0, "v0": v0, "u1": u1, "v1": v1}
+
+
 def initial_state(n: int, init: str = "zero", seed: int = SEED):
     """t = 0 state: faces u, v, their previous-step copies (velContPrev =
     velCont, so the BDF2 history term is 0 at step 1) and userCtx = 0."""

@@ -231,6 +231,41 @@ int amressif_store_dual(amressif_ctx *ctx, amr_array4 du[2], amr_array4 du_prev[
  * (defaults 1e-14 and 200). */
 int amressif_set_adjoint_solver(amressif_ctx *ctx, double tol, int max_sweeps);
 
+/* ------------------------------------------------- two-level composite step
+ * No reference counterpart (RV is single-level).  A coarse n_cell^2 level
+ * plus one refined patch (ratio 2) over the coarse rows [patch_jlo, n_cell),
+ * full width, so its W/E/N sides are the domain walls and its south side the
+ * coarse/fine interface.  One dt (from the fine spacing, or fixed_dt) for both
+ * levels.  Per step: coarse predictor (the single-level dual-time loop), fine
+ * predictor with ghosts and interface faces from the coarse predictor, average
+ * down, composite projection (FGMRES on the composite Neumann operator with a
+ * coarse V-cycle + patch Gauss-Seidel preconditioner; flux matching across the
+ * interface), update, average down.  Fine arrays: x-faces (2n+1) x ny,
+ * y-faces 2n x (ny+1), p 2n x ny with ny = 2 (n_cell - patch_jlo). */
+typedef struct amressif_amr amressif_amr;
+
+typedef struct {
+    int coarse_iters;   /* coarse pseudo-iterations */
+    int fine_iters;     /* fine pseudo-iterations */
+    int poisson_iters;  /* FGMRES iterations of the composite solve */
+    int flags;          /* AMRESSIF_FLAG_* */
+    double div_star;    /* composite ||D(u*)||inf */
+    double div_new;     /* composite ||D(u^{n+1})||inf */
+    double poisson_res; /* final ||r||inf / ||b||inf */
+} amressif_amr_stats;
+
+int amressif_amr_create(const amressif_inputs *in, int patch_jlo, void *hip_stream, amressif_amr **out);
+int amressif_amr_destroy(amressif_amr *amr);
+const char *amressif_amr_last_error(const amressif_amr *amr);
+int amressif_amr_dt(const amressif_amr *amr, double *dt);
+/* coarse faces + coarse userCtx (p, phi) as amressif_load_state; fine faces and
+ * fine p (1 comp, >= 0 ghosts; NULL = 0).  velContPrev = velCont on both. */
+int amressif_amr_load_state(amressif_amr *amr, const amr_array4 coarse_vel[2], const amr_array4 *coarse_ctx,
+                            const amr_array4 fine_vel[2], const amr_array4 *fine_p);
+int amressif_amr_store_state(amressif_amr *amr, amr_array4 coarse_vel[2], amr_array4 *coarse_ctx,
+                             amr_array4 fine_vel[2], amr_array4 *fine_p);
+int amressif_amr_advance(amressif_amr *amr, int nsteps, amressif_amr_stats *stats);
+
 /* ------------------------------------------------------------ diagnostics
  * No reference counterpart (the reference prints per-step wall time only,
  * main.cpp:1173-1175).  Average device time of one kernel at the context's

@@ -15,7 +15,16 @@ Both precisions are run from the same initial state:
               (the headline `value`)
   * fp64   -- the plain FP64 step
 and the mixed result is compared with the FP64 one per field after the same
-number of steps.  One JSON line is printed.
+number of steps.  Beside them, two more legs are timed: the discrete adjoint
+of a 512^2, Re = 1000 rollout (backward sweep with checkpoint recomputation
+over the forward time) and the two-level composite step (512^2 coarse level +
+a ratio-2 patch over the top quarter).  One JSON line is printed.
+
+roofline: algorithmic bytes per launch of the headline run's dominant kernel
+over its HIP-event launch time; traffic: rocprofv3 FETCH_SIZE + WRITE_SIZE per
+launch from profiles/pmc_summary.txt, scaled by a 64 MiB copy.
+fp64_step_roofline: algorithmic bytes of a whole FP64 step (model in
+step_bytes_fp64) over the measured step time.
 """
 from __future__ import annotations
 
@@ -48,6 +57,8 @@ def parse():
     ap.add_argument("--adjoint-n", type=int, default=512, help="adjoint leg grid (0: skip the leg)")
     ap.add_argument("--adjoint-steps", type=int, default=1000)
     ap.add_argument("--adjoint-every", type=int, default=32, help="checkpoint interval of the adjoint leg")
+    ap.add_argument("--amr-n", type=int, default=512, help="two-level leg: coarse grid (0: skip the leg)")
+    ap.add_argument("--amr-steps", type=int, default=100)
     ap.add_argument("--kernel-probe", choices=["fp64", "mixed"], default=None,
                     help="for rocprofv3 --pmc: one step, then a few timed launches of each kernel of that "
                          "precision and a 64 MiB copy (calibration); prints their HIP-event times")
@@ -67,6 +78,20 @@ def read_traffic(kernel_key: str):
             vals[m.group(1)] = float(m.group(2))
     key = f"{kernel_key}.traffic_bytes"
     return vals.get(key)
+
+
+def step_bytes_fp64(n, pseudo_iters, vcycles):
+    """Algorithmic HBM bytes of one FP64 step at n^2 cells: 4 fused stages of
+    72 B/cell per pseudo-iteration; per V-cycle, on every level, the
+    pre-smoother from zero (r in, e out: 16 B), residual + restriction (e, r in:
+    16 B, coarse r out: 2 B) and the post-smoother (e, r, coarse e in, e out:
+    26 B), i.e. 60 B/cell times 4/3 for the hierarchy, plus the FP64 correction
+    and residual on the finest level (phi, e, b in; phi, r out: 40 B); once per
+    step BDF2 (u, uprev in; c, uprev out: 32 B), Poisson RHS (u, v in, b out:
+    24 B), centring (16 B) and the projection (u*, v*, phi, p in; u, v, p out:
+    56 B)."""
+    cells = float(n) * n
+    return cells * (pseudo_iters * 4 * 72 + vcycles * (60 * 4 / 3 + 40) + 32 + 24 + 16 + 56)
 
 
 def run_precision(api, W, n, ren, precision, state, warmup, steps, torch):
@@ -109,11 +134,35 @@ def adjoint_leg(api, W, n, steps, every, torch):
     import numpy as np
     return {"workload": f"lid-driven cavity {n}x{n}, Re=1000, seeded start + seeded lid profile, {steps} FP64 "
                         f"steps, J = lower-left kinetic energy, checkpoints every {every} steps",
-            "forward_s": t_fwd, "backward_s": t_bwd, "ratio": t_bwd / t_fwd, "J": float(J),
+            "forward_s": t_fwd, "backward_s": t_bwd, "ratio": t_bwd / t_fwd, "J": float(J.detach()),
             "grad_lid_norm": float(glid.norm()), "grad_u0_norm": float(torch.sqrt((gu0 ** 2).sum() + (gv0 ** 2).sum())),
             "mean_pseudo_iters": float(np.mean([s["pseudo_iters"] for s in ro.fwd_stats])),
             "mean_adjoint_sweeps": float(np.mean([s["lin_iters"] for s in ro.bwd_stats])),
             "adjoint_flags": int(np.bitwise_or.reduce([s["flags"] for s in ro.bwd_stats]))}
+
+
+def amr_leg(api, W, n0, steps, torch):
+    """Two-level forward step, mixed precision: coarse n0^2, ratio-2 patch over
+    the top quarter (coarse rows [3 n0 / 4, n0)), Re = 2000, seeded field."""
+    import numpy as np
+    jc0 = 3 * n0 // 4
+    inp = api.cavity_inputs(n0, 2000.0, precision=api.AMRESSIF_MIXED)
+    tl = api.TwoLevel(inp, jc0, W.seeded_two_level(n0, jc0))
+    tl.advance(2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = tl.advance(steps)
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    cells = n0 * jc0 + (2 * n0) * (2 * (n0 - jc0))
+    return {"workload": f"two-level cavity: coarse {n0}x{n0} + ratio-2 patch {2 * n0}x{2 * (n0 - jc0)} over the top "
+                        f"quarter, Re=2000, seeded start, mixed precision, composite FGMRES projection, {steps} steps",
+            "composite_cells": cells, "mcell_steps_per_s": cells * steps / sec / 1e6, "ms_per_step": 1e3 * sec / steps,
+            "mean_coarse_iters": float(np.mean([x["coarse_iters"] for x in st])),
+            "mean_fine_iters": float(np.mean([x["fine_iters"] for x in st])),
+            "mean_poisson_iters": float(np.mean([x["poisson_iters"] for x in st])),
+            "max_div_ratio": float(max(x["div_new"] / max(x["div_star"], 1e-300) for x in st)),
+            "flags": int(np.bitwise_or.reduce([x["flags"] for x in st]))}
 
 
 def kernel_probe(api, W, n, ren, which_prec):
@@ -181,6 +230,11 @@ def main():
 
     head = "mixed" if "mixed" in runs else "fp64"
     extra = {}
+    if "fp64" in out:
+        o = out["fp64"]
+        nbytes = step_bytes_fp64(n, o["pseudo_iters"], o["vcycles"])
+        extra["fp64_step_roofline"] = {"bytes_per_step": nbytes, "achieved": nbytes / (o["ms_per_step"] * 1e-3) / 1e9,
+                                       "unit": "GB/s", "frac": nbytes / (o["ms_per_step"] * 1e-3) / HBM_PEAK}
     if head == "mixed":
         cand = [kernel_roofline(runs["mixed"], w, nm) for w, nm in ((3, "residual_mixed"), (4, "tangent_mixed"))]
         roofline = max(cand, key=lambda r: r["avg_launch_ms"])
@@ -193,6 +247,10 @@ def main():
     adj = None
     if args.adjoint_n > 0 and not args.profile_run:
         adj = adjoint_leg(api, W, args.adjoint_n, args.adjoint_steps, args.adjoint_every, torch)
+
+    amr = None
+    if args.amr_n > 0 and not args.profile_run:
+        amr = amr_leg(api, W, args.amr_n, args.amr_steps, torch)
 
     cpu = None
     if args.cpu_steps > 0 and not args.profile_run:
@@ -231,6 +289,7 @@ def main():
         "mixed_vs_fp64_rel_err": err,
         "adjoint_forward_ratio": adj["ratio"] if adj else None,
         "adjoint": adj,
+        "two_level": amr,
         "roofline": roofline,
         "kernel_rooflines": extra,
         "cpu_baseline": cpu,

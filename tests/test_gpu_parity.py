@@ -211,6 +211,18 @@ def test_advance_cavity128_re100_zero_start():
     assert rel(got["vc"][:, 2:-2, 2:-2], ref["vc"][:, 2:-2, 2:-2]) < 1e-10
 
 
+def test_c1_cavity128_re100_full_200_steps():
+    # the plain CPU-reference workload in full: 128^2, Re = 100, from rest, 200 steps
+    ref, rs, got, st = run_both(128, 100.0, 200)
+    check_step_parity(ref, rs, got, st)
+
+
+def test_c2_cavity1024_re1000_first_steps():
+    # the FP64 GPU workload at its full size (1024^2, Re = 1000, seeded), first steps
+    ref, rs, got, st = run_both(1024, 1000.0, 2, init="seeded")
+    check_step_parity(ref, rs, got, st)
+
+
 def test_advance_seeded_256_re1000():
     ref, rs, got, st = run_both(256, 1000.0, 6, init="seeded")
     check_step_parity(ref, rs, got, st)

@@ -67,6 +67,7 @@ class Inputs(ctypes.Structure):
         ("precision", ctypes.c_int),
         ("poisson_solver", ctypes.c_int),
         ("gmres_restart", ctypes.c_int),
+        ("mixed_momentum", ctypes.c_int),
     ]
 
 

@@ -260,10 +260,12 @@ int mg_loop(amressif_ctx *ctx)
     return 0;
 }
 
-// One RK4 pseudo-iteration.  FP64: four guarded stage launches; the stencil
-// input ping-pongs U -> S1 -> S2 -> S1 and the last stage updates U (the
-// pseudo base u^k) in place.  Mixed: the FP64 residual of U, then the FP32
-// tangent-linear update of U in place (first order in the RK4 recursion).
+// One pseudo-iteration.  FP64: the RK4 dual-time sweep, four guarded stage
+// launches; the stencil input ping-pongs U -> S1 -> S2 -> S1 and the last stage
+// updates U (the pseudo base u^k) in place.  Mixed: the FP64 residual r of U,
+// then the FP32 tangent-linear correction of U in place: by default the
+// two-term Neumann series of the inverse Jacobian (iterative refinement of the
+// implicit BDF2 system), or the RK4 recursion to first order in the Jacobian.
 int advance_impl(amressif_ctx *ctx, int nsteps, amressif_step_stats *stats, amressif_tape *tape);
 int launch_pseudo_iteration(amressif_ctx *ctx, int it, int *nb);
 }  // namespace
@@ -311,8 +313,16 @@ int launch_pseudo_iteration(amressif_ctx *ctx, int it, int *nb)
         amr::MixedStageLaunch a{};
         a.u64 = c.U[0]; a.v64 = c.U[1]; a.cu = c.C[0]; a.cv = c.C[1]; a.p = c.p;
         a.uk = c.uk32[0]; a.vk = c.uk32[1]; a.ru = c.r32[0]; a.rv = c.r32[1];
-        a.cj = c.rk[2] * c.rk[3];
-        a.cr = c.rk[3] - a.cj * (1.5 / c.g.dt);
+        const double c15 = 1.5 / c.g.dt;
+        if (c.in.mixed_momentum == 1) {
+            // Newton correction with J^{-1} ~ -(1/c15)(I + R'/c15): u += r/c15 + R'(r)/c15^2
+            a.cr = 1.0 / c15;
+            a.cj = 1.0 / (c15 * c15);
+        } else {
+            // RK4 dual-time recursion to first order in J = R' - c15
+            a.cj = c.rk[2] * c.rk[3];
+            a.cr = c.rk[3] - a.cj * c15;
+        }
         a.stage = 0;
         HIPCHK(amr::launch_stage_mixed(c, a, nb));
         a.stage = 1;
@@ -365,6 +375,7 @@ void amressif_inputs_default(amressif_inputs *in)
     in->precision = AMRESSIF_FP64;
     in->poisson_solver = 0;
     in->gmres_restart = 30;
+    in->mixed_momentum = 1;
 }
 
 double amressif_dt(const amressif_inputs *in)
@@ -473,7 +484,8 @@ int amressif_parse_inputs(const char *path, int argc, const char *const *argv, a
               get_int("amressif.mg_bottom_max", &out->mg_bottom_max, 1, false) &&
               get_int("amressif.precision", &out->precision, 1, false) &&
               get_int("amressif.poisson_solver", &out->poisson_solver, 1, false) &&
-              get_int("amressif.gmres_restart", &out->gmres_restart, 1, false);
+              get_int("amressif.gmres_restart", &out->gmres_restart, 1, false) &&
+              get_int("amressif.mixed_momentum", &out->mixed_momentum, 1, false);
     if (!ok) return seterr(bad);
     return AMRESSIF_OK;
 }
@@ -506,6 +518,8 @@ int amressif_create(const amressif_inputs *in, void *hip_stream, amressif_ctx **
     if (in->precision != AMRESSIF_FP64 && in->precision != AMRESSIF_MIXED)
         return bail(AMRESSIF_ERR_ARG, "precision must be 0 (FP64) or 1 (mixed)");
     if (in->mg_bottom_max < 2 || in->mg_bottom_max > 32) return bail(AMRESSIF_ERR_ARG, "mg_bottom_max must be in [2, 32]");
+    if (in->mixed_momentum != 0 && in->mixed_momentum != 1)
+        return bail(AMRESSIF_ERR_ARG, "mixed_momentum must be 0 (dual-time form) or 1 (iterative refinement)");
     if (in->poisson_solver != 0 && in->poisson_solver != 1)
         return bail(AMRESSIF_ERR_ARG, "poisson_solver must be 0 (multigrid) or 1 (FGMRES)");
     if (in->poisson_solver == 1 && (in->gmres_restart < 1 || in->gmres_restart > amr::kGmMax))

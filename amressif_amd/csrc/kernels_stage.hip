@@ -19,14 +19,17 @@
 // Precision forms:
 //  * FP64 (the reference's): u* <- u^k + rk (R(u*) - (1.5/dt) u* + c), with the
 //    BDF2 constant c = (1.5/dt) u^n + (0.5/dt)(u^n - u^{n-1}) folded once per step.
-//  * mixed (defect correction): per pseudo-iteration, k_stage64 in mode 3 forms
-//    the FP64 residual r = R(u^k) - (1.5/dt) u^k + c of the pseudo base and
-//    k_tangent applies the FP32 tangent-linear stencil R'(u^k) r (QUICK with
-//    the upwind switch frozen at u^k, viscous flux of r).  With J = R' - 1.5/dt
-//    the RK4 stage recursion (coefficients a1..a4) taken to first order in J
-//    gives u^{k+1} = u^k + a4 r + a3 a4 J r = u^k + cr r + cj R' r with
-//    cr = a4 - a3 a4 (1.5/dt), cj = a3 a4.  A converged state (r = 0) is a
-//    fixed point, so the answer is fixed by the FP64 residual.
+//  * mixed (iterative refinement): per iteration, k_stage64 in mode 3 forms
+//    the FP64 residual r = R(u^k) - (1.5/dt) u^k + c of the implicit BDF2
+//    system and k_tangent applies the FP32 tangent-linear stencil R'(u^k) r
+//    (QUICK with the upwind switch frozen at u^k, viscous flux of r), then
+//    u^{k+1} = u^k + cr r + cj R' r in FP64.  With J = R' - c, c = 1.5/dt,
+//    the default is the Newton correction with the two-term Neumann series
+//    J^{-1} ~ -(1/c)(I + R'/c): cr = 1/c, cj = 1/c^2 (|R'|/c is ~1e-4 at
+//    2048^2, so a few iterations converge).  The alternative keeps the RK4
+//    dual-time recursion (a1..a4) to first order in J: cr = a4 - a3 a4 c,
+//    cj = a3 a4.  A converged state (r = 0) is a fixed point of both, so the
+//    answer is fixed by the FP64 residual.
 #include "device_common.hpp"
 #include "internal.hpp"
 

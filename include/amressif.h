@@ -81,6 +81,11 @@ typedef struct {
                                    1: FGMRES(gmres_restart) right-preconditioned by one V-cycle
                                    (the solver GMRES_Poisson.H:14-76 sets up; same operator) */
     int gmres_restart;          /* restart length, 1..64; default 30 */
+    int mixed_momentum;         /* precision = 1 only: the momentum solve of the implicit BDF2 system.
+                                   1 (default): FP64 iterative refinement, each correction the FP32
+                                   two-term Neumann series -(1/c)(I + R'/c) of the inverse Jacobian,
+                                   c = 1.5/dt; 0: the RK4 dual-time recursion to first order in
+                                   the Jacobian (the reference's pseudo-time structure) */
 } amressif_inputs;
 
 /* AMReX Array4 view of one box: ptr addresses element (lo[0], lo[1], lo[2], comp 0);

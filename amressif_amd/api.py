@@ -68,6 +68,7 @@ class Inputs(ctypes.Structure):
         ("poisson_solver", ctypes.c_int),
         ("gmres_restart", ctypes.c_int),
         ("mixed_momentum", ctypes.c_int),
+        ("poisson_warm_start", ctypes.c_int),
     ]
 
 

@@ -229,25 +229,34 @@ int gmres_loop(amressif_ctx *ctx)
 
 // ---- the Poisson solve on c.phi / c.b (b already centred), shared by the
 // step and the module entry point
-int mg_loop(amressif_ctx *ctx);
-int poisson_loop(amressif_ctx *ctx) { return ctx->c.in.poisson_solver == 1 ? gmres_loop(ctx) : mg_loop(ctx); }
+int mg_loop(amressif_ctx *ctx, bool warm);
+// warm: start from the previous step's solution (the stopping test stays
+// relative to ||b||inf); cold: from zero, as poisson.cpp:63
+int poisson_loop(amressif_ctx *ctx, bool warm = false)
+{
+    return ctx->c.in.poisson_solver == 1 ? gmres_loop(ctx) : mg_loop(ctx, warm);
+}
 
-int mg_loop(amressif_ctx *ctx)
+int mg_loop(amressif_ctx *ctx, bool warm)
 {
     Ctx &c = ctx->c;
     const int mixed = c.in.precision == AMRESSIF_MIXED;
     int nb = 0;
     HIPCHK(amr::launch_mg_start(c));
-    HIPCHK(amr::launch_fill(c, c.phiv[0], (int64_t)(c.g.n + 1) * c.g.P, 0.0));
-    HIPCHK(amr::launch_correct_residual(c, c.phiv[0], nullptr, 0, mixed, &nb));  // r = b, ||b||
+    if (warm) {
+        if (c.phi != c.phiv[0]) std::swap(c.phiv[0], c.phiv[1]);  // phiv[0] <- the previous solution
+        HIPCHK(amr::launch_b_norm(c));
+    } else {
+        HIPCHK(amr::launch_fill(c, c.phiv[0], (int64_t)(c.g.n + 1) * c.g.P, 0.0));
+    }
+    HIPCHK(amr::launch_correct_residual(c, c.phiv[0], nullptr, 0, mixed, &nb));  // r = b - L phi0
     HIPCHK(amr::launch_fin_resid(c, nb));
     int launched = 0, target = std::max(1, c.last_V);
     const int cap = std::max(1, c.in.mg_max_iter);
     for (;;) {
         while (launched < target && launched < cap) {
             const int par = launched & 1;  // executed cycles always follow this order
-            HIPCHK(amr::launch_vcycle(c, mixed));
-            HIPCHK(amr::launch_correct_residual(c, c.phiv[par], c.phiv[par ^ 1], 1, mixed, &nb));
+            HIPCHK(amr::launch_vcycle_step(c, mixed, par, c.phiv[par], c.phiv[par ^ 1], &nb));
             HIPCHK(amr::launch_fin_resid(c, nb));
             ++launched;
         }
@@ -376,6 +385,7 @@ void amressif_inputs_default(amressif_inputs *in)
     in->poisson_solver = 0;
     in->gmres_restart = 30;
     in->mixed_momentum = 1;
+    in->poisson_warm_start = -1;
 }
 
 double amressif_dt(const amressif_inputs *in)
@@ -485,7 +495,8 @@ int amressif_parse_inputs(const char *path, int argc, const char *const *argv, a
               get_int("amressif.precision", &out->precision, 1, false) &&
               get_int("amressif.poisson_solver", &out->poisson_solver, 1, false) &&
               get_int("amressif.gmres_restart", &out->gmres_restart, 1, false) &&
-              get_int("amressif.mixed_momentum", &out->mixed_momentum, 1, false);
+              get_int("amressif.mixed_momentum", &out->mixed_momentum, 1, false) &&
+              get_int("amressif.poisson_warm_start", &out->poisson_warm_start, 1, false);
     if (!ok) return seterr(bad);
     return AMRESSIF_OK;
 }
@@ -518,6 +529,8 @@ int amressif_create(const amressif_inputs *in, void *hip_stream, amressif_ctx **
     if (in->precision != AMRESSIF_FP64 && in->precision != AMRESSIF_MIXED)
         return bail(AMRESSIF_ERR_ARG, "precision must be 0 (FP64) or 1 (mixed)");
     if (in->mg_bottom_max < 2 || in->mg_bottom_max > 32) return bail(AMRESSIF_ERR_ARG, "mg_bottom_max must be in [2, 32]");
+    if (in->poisson_warm_start < -1 || in->poisson_warm_start > 1)
+        return bail(AMRESSIF_ERR_ARG, "poisson_warm_start must be -1 (auto), 0 or 1");
     if (in->mixed_momentum != 0 && in->mixed_momentum != 1)
         return bail(AMRESSIF_ERR_ARG, "mixed_momentum must be 0 (dual-time form) or 1 (iterative refinement)");
     if (in->poisson_solver != 0 && in->poisson_solver != 1)
@@ -575,9 +588,10 @@ int amressif_create(const amressif_inputs *in, void *hip_stream, amressif_ctx **
             const size_t lsz = (size_t)L.pitch * (ny + 1);
             if (in->precision == AMRESSIF_MIXED) {
                 ALLOC(&L.e32, lsz); ALLOC(&L.e32b, lsz); ALLOC(&L.r32, lsz);
-                if (l == 0) { ALLOC(&L.e, lsz); ALLOC(&L.r, lsz); }  // diagnostics only
+                if (l == 0) { ALLOC(&L.e, lsz); ALLOC(&L.r, lsz); ALLOC(&L.r32b, lsz); }  // e, r: diagnostics only
             } else {
                 ALLOC(&L.e, lsz); ALLOC(&L.e2, lsz); ALLOC(&L.r, lsz);
+                if (l == 0) ALLOC(&L.rb, lsz);
             }
             ++l;
             if (nx > in->mg_bottom_max && ny > in->mg_bottom_max && nx % 2 == 0 && ny % 2 == 0 && l < amr::kMaxLevels) {
@@ -738,6 +752,7 @@ int advance_impl(amressif_ctx *ctx, int nsteps, amressif_step_stats *stats, amre
                                c.in.mg_max_iter));
     int status = AMRESSIF_OK;
     const bool mixed = c.in.precision == AMRESSIF_MIXED;
+    const bool warm = c.in.poisson_warm_start == 1 || (c.in.poisson_warm_start == -1 && mixed);
     for (int s = 0; s < nsteps; ++s) {
         int nb = 0;
         if (int rc = amr_momentum_predictor(ctx)) return rc;
@@ -753,7 +768,7 @@ int advance_impl(amressif_ctx *ctx, int nsteps, amressif_step_stats *stats, amre
         HIPCHK(amr::launch_poisson_rhs(c, c.U[0], c.U[1], c.b, &nb));
         HIPCHK(amr::launch_fin_prhs(c, nb));
         HIPCHK(amr::launch_center_b(c, c.b));
-        if (int rc = poisson_loop(ctx)) return rc;
+        if (int rc = poisson_loop(ctx, warm)) return rc;
         HIPCHK(amr::launch_project(c, c.U[0], c.U[1], c.S1[0], c.S1[1], &nb));
         HIPCHK(amr::launch_fin_project(c, nb, s));
         std::swap(c.U[0], c.S1[0]);

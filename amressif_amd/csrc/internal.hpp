@@ -22,7 +22,8 @@ struct Ctl {
     int mg_done;    // Poisson solve converged (or capped)
     int mg_iters;   // V-cycles done
     int mg_started; // first residual evaluated
-    int pad0, pad1;
+    int b_norm_fixed;  // ||b||inf measured before the first residual (warm start)
+    int pad1;
     double norm_error;
     double div_star, rhs_sum, b_mean;
     double b_norm, res_norm, phi_sum, phi_mean;
@@ -43,6 +44,8 @@ struct Level {
     double ax, ay;   // 1/dx^2, 1/dy^2
     double *e, *e2, *r;   // FP64 correction (pre- and post-smoothing buffers) and right-hand side
     float *e32, *e32b, *r32; // the same in FP32 for the mixed V-cycle
+    double *rb;              // level 0: second right-hand-side buffer (the fused outer step ping-pongs)
+    float *r32b;
 };
 
 struct Geometry {
@@ -157,8 +160,10 @@ hipError_t launch_poisson_rhs(const Ctx &c, const double *u, const double *v, do
 hipError_t launch_fin_prhs(const Ctx &c, int nblocks);
 hipError_t launch_center_b(const Ctx &c, double *b);
 hipError_t launch_mg_start(const Ctx &c);
+hipError_t launch_b_norm(const Ctx &c);
 hipError_t launch_fin_resid(const Ctx &c, int nblocks);
 hipError_t launch_vcycle(const Ctx &c, int mixed);
+hipError_t launch_vcycle_step(const Ctx &c, int mixed, int par, const double *phi_in, double *phi_out, int *nblocks);
 hipError_t launch_correct_residual(const Ctx &c, const double *phi_in, double *phi_out, int with_correction,
                                    int mixed, int *nblocks);
 size_t tail_lds_bytes(const Ctx &c, int mixed);

@@ -29,41 +29,66 @@
 namespace amr {
 
 // --------------------------------------------------------- fused smoother
-constexpr int SX = 64, SY = 16, SHALO = 4;
-constexpr int SW = SX + 2 * SHALO, SHT = SY + 2 * SHALO;  // 72 x 24 LDS region
+constexpr int SX = 64, SY = 16;  // output tile
 
-// x_out = GSRB^2(x_in + P(ec)) for L x = r on one level.  x_in null: start
-// from zero; ec null: no coarse correction.  Red (i+j even) then black, twice.
-// The 72 x 24 LDS region is split into horizontal cell pairs (one red, one
-// black); thread t owns pairs t, t + 256, ... so every half-sweep is a fixed
-// loop of at most four updates per thread with no index division.  Cells of
-// one colour read only the other colour, so the update order inside a
-// half-sweep does not change the result.
-constexpr int SPAIRS = SW / 2 * SHT;                   // 864 pairs
-constexpr int SPER = (SPAIRS + 255) / 256;             // pairs per thread
+// x_out = GSRB^2(x_in + P(ec)) for L x = r on one level: red (i+j even) then
+// black, twice, in an LDS region of the tile plus a halo of H cells.  The
+// region is split into horizontal cell pairs (one red, one black); thread t
+// owns pairs t, t + 256, ... so every half-sweep is a fixed loop of a few
+// updates per thread with no index division.  Cells of one colour read only
+// the other colour, so the update order inside a half-sweep does not change
+// the result.  Each half-sweep invalidates one more ring of the region, so
+// after the four the region minus a ring of 4 is exact.
+//   MODE 0 (H = 4): x_in null starts from zero, ec null adds no correction.
+//   MODE 1 (H = 5, the down leg): from zero; the exact ring around the tile
+//     also gives the tile's residual r - L x, averaged over each 2x2 block into
+//     the coarse right-hand side rc (the separate residual + restriction pass
+//     with the same arithmetic).
+//   MODE 2 (H = 5, the up leg on the finest level): the correction e is added
+//     to the FP64 iterate, phi_out = phi_in + e, and the next residual
+//     r = b - L phi_out is formed in FP64 (stored in T), with block partials of
+//     ||r||inf and sum phi_out -- the FP64 outer step fused into the smoother.
+template <int H>
+struct SmoothShape {
+    static constexpr int W = SX + 2 * H, HT = SY + 2 * H;
+    static constexpr int PAIRS = W / 2 * HT;
+    static constexpr int PER = (PAIRS + 255) / 256;
+};
 
-template <class T>
+struct OuterArgs {  // MODE 2
+    const double *phi_in;
+    double *phi_out;
+    const double *b;
+    double *partials;
+    double ax, ay;  // the FP64 operator's coefficients (not the V-cycle's T-rounded ones)
+};
+
+template <class T, int MODE>
 __global__ __launch_bounds__(256) void k_smooth(const T *xin, const T *ec, int pc, T *xout, const T *rhs, int nx,
-                                                int ny, int pitch, T ax, T ay, const Ctl *ctl)
+                                                int ny, int pitch, T ax, T ay, T *rc, int ncx, int ncy, int pcc,
+                                                OuterArgs o, const Ctl *ctl)
 {
+    constexpr int H = MODE == 0 ? 4 : 5;
+    using S = SmoothShape<H>;
+    constexpr int SW = S::W, SHT = S::HT;
     if (ctl->mg_done | ctl->gm_stop) return;
     __shared__ T sx[SHT][SW], sr[SHT][SW];
     const int tid = threadIdx.x;
     const int i0 = blockIdx.x * SX, j0 = blockIdx.y * SY;
-    int pl[SPER], pj[SPER];  // pair column (cell 2 pl) and row in the region; -1: none
+    int pl[S::PER], pj[S::PER];  // pair column (cell 2 pl) and row in the region; -1: none
 #pragma unroll
-    for (int q = 0; q < SPER; ++q) {
+    for (int q = 0; q < S::PER; ++q) {
         const int p = tid + 256 * q;
-        pj[q] = p < SPAIRS ? p / (SW / 2) : -1;
+        pj[q] = p < S::PAIRS ? p / (SW / 2) : -1;
         pl[q] = p % (SW / 2);
     }
 #pragma unroll
-    for (int q = 0; q < SPER; ++q) {
+    for (int q = 0; q < S::PER; ++q) {
         if (pj[q] < 0) continue;
-        const int lj = pj[q], j = j0 - SHALO + lj;
+        const int lj = pj[q], j = j0 - H + lj;
 #pragma unroll
         for (int c2 = 0; c2 < 2; ++c2) {
-            const int li = 2 * pl[q] + c2, i = i0 - SHALO + li;
+            const int li = 2 * pl[q] + c2, i = i0 - H + li;
             T x = 0, r = 0;
             if (i >= 0 && i < nx && j >= 0 && j < ny) {
                 const size_t k = i + (size_t)pitch * j;
@@ -81,12 +106,12 @@ __global__ __launch_bounds__(256) void k_smooth(const T *xin, const T *ec, int p
     for (int h = 0; h < 4; ++h) {
         const int color = h & 1, lo = 1 + h;
 #pragma unroll
-        for (int q = 0; q < SPER; ++q) {
+        for (int q = 0; q < S::PER; ++q) {
             const int lj = pj[q];
-            // i0 - SHALO and j0 - SHALO are even: the colour of (li, lj) is (li + lj) & 1
+            // i0 - H and j0 - H have the parity of H: the colour of (li, lj) is (li + lj + 2H) & 1
             const int li = 2 * pl[q] + ((lj & 1) ^ color);
             if (lj < lo || lj >= SHT - lo || li < lo || li >= SW - lo) continue;
-            const int i = i0 - SHALO + li, j = j0 - SHALO + lj;
+            const int i = i0 - H + li, j = j0 - H + lj;
             if (i < 0 || i >= nx || j < 0 || j >= ny) continue;
             T s = 0;
             if (i > 0 && i < nx - 1 && j > 0 && j < ny - 1) {
@@ -103,9 +128,55 @@ __global__ __launch_bounds__(256) void k_smooth(const T *xin, const T *ec, int p
         }
         __syncthreads();
     }
-    for (int e = tid; e < SX * SY; e += 256) {
-        const int li = SHALO + (e % SX), lj = SHALO + (e / SX), i = i0 + (e % SX), j = j0 + (e / SX);
-        if (i < nx && j < ny) xout[i + (size_t)pitch * j] = sx[lj][li];
+    if (MODE != 2) {
+        for (int e = tid; e < SX * SY; e += 256) {
+            const int li = H + (e % SX), lj = H + (e / SX), i = i0 + (e % SX), j = j0 + (e / SX);
+            if (i < nx && j < ny) xout[i + (size_t)pitch * j] = sx[lj][li];
+        }
+    }
+    if (MODE == 1) {  // residual of the tile, 2x2 averages: one coarse cell per thread (32 x 8)
+        const int I = (i0 >> 1) + (tid & 31), J = (j0 >> 1) + (tid >> 5);
+        const int i = 2 * I, j = 2 * J;
+        if (I < ncx && J < ncy) {
+            auto res = [&](int gi, int gj) {
+                const int li = gi - i0 + H, lj = gj - j0 + H;
+                const T P = sx[lj][li];
+                const T W = gi > 0 ? sx[lj][li - 1] : P;
+                const T E = gi < nx - 1 ? sx[lj][li + 1] : P;
+                const T Sv = gj > 0 ? sx[lj - 1][li] : P;
+                const T N = gj < ny - 1 ? sx[lj + 1][li] : P;
+                return sr[lj][li] - ((E + W - T(2) * P) * ax + (N + Sv - T(2) * P) * ay);
+            };
+            const T t00 = res(i, j), t10 = res(i + 1, j), t01 = res(i, j + 1), t11 = res(i + 1, j + 1);
+            rc[I + (size_t)pcc * J] = T(0.25) * (((t00 + t10) + t01) + t11);
+        }
+    }
+    if (MODE == 2) {  // phi_out = phi_in + e, r = b - L phi_out (FP64), partials
+        __shared__ double shr[16];
+        double m = 0.0, sm = 0.0;
+        for (int e = tid; e < SX * SY; e += 256) {
+            const int li = H + (e % SX), lj = H + (e / SX), i = i0 + (e % SX), j = j0 + (e / SX);
+            if (i >= nx || j >= ny) continue;
+            const size_t k = i + (size_t)pitch * j;
+            auto x = [&](size_t q, int dl, int dj) -> double { return o.phi_in[q] + (double)sx[lj + dj][li + dl]; };
+            const double Pv = x(k, 0, 0);
+            const double Wv = i > 0 ? x(k - 1, -1, 0) : Pv;
+            const double Ev = i < nx - 1 ? x(k + 1, 1, 0) : Pv;
+            const double Sv = j > 0 ? x(k - pitch, 0, -1) : Pv;
+            const double Nv = j < ny - 1 ? x(k + pitch, 0, 1) : Pv;
+            const double rr = o.b[k] - ((Ev + Wv - 2.0 * Pv) * o.ax + (Nv + Sv - 2.0 * Pv) * o.ay);
+            o.phi_out[k] = Pv;
+            xout[k] = (T)rr;  // the next residual
+            m = (fabs(rr) > m || rr != rr) ? fabs(rr) : m;
+            sm += Pv;
+        }
+        m = block_reduce<true>(m, shr);
+        sm = block_reduce<false>(sm, shr + 8);
+        if (tid == 0) {
+            const size_t bid = blockIdx.x + (size_t)gridDim.x * blockIdx.y;
+            o.partials[2 * bid] = m;
+            o.partials[2 * bid + 1] = sm;
+        }
     }
 }
 
@@ -119,23 +190,6 @@ __device__ __forceinline__ T lap_at(const T *e, int i, int j, int nx, int ny, in
     const T S = j > 0 ? e[k - pitch] : P;
     const T N = j < ny - 1 ? e[k + pitch] : P;
     return (E + W - T(2) * P) * ax + (N + S - T(2) * P) * ay;
-}
-
-// residual of the correction equation, averaged over each 2x2 block into the
-// coarse right-hand side
-template <class T>
-__global__ __launch_bounds__(256) void k_resid_restrict(const T *e, const T *r, int nx, int ny, int pf, T ax, T ay,
-                                                        T *rc, int ncx, int ncy, int pc, const Ctl *ctl)
-{
-    if (ctl->mg_done | ctl->gm_stop) return;
-    const int I = blockIdx.x * 64 + (threadIdx.x & 63), J = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (I >= ncx || J >= ncy) return;
-    const int i = 2 * I, j = 2 * J;
-    const T t00 = r[i + (size_t)pf * j] - lap_at(e, i, j, nx, ny, pf, ax, ay);
-    const T t10 = r[i + 1 + (size_t)pf * j] - lap_at(e, i + 1, j, nx, ny, pf, ax, ay);
-    const T t01 = r[i + (size_t)pf * (j + 1)] - lap_at(e, i, j + 1, nx, ny, pf, ax, ay);
-    const T t11 = r[i + 1 + (size_t)pf * (j + 1)] - lap_at(e, i + 1, j + 1, nx, ny, pf, ax, ay);
-    rc[I + (size_t)pc * J] = T(0.25) * (((t00 + t10) + t01) + t11);
 }
 
 // ------------------------------------------------------ single-block tail
@@ -209,13 +263,24 @@ __global__ __launch_bounds__(1024) void k_tail(const T *rtop, int ptop, T *etop,
         }
         __syncthreads();
     }
-    {  // bottom: e = pinv r (pinv symmetric: read it column-wise for coalescing)
+    {  // bottom: e = pinv r (pinv symmetric: read it column-wise for coalescing);
+       // up to four threads share a row, each over a quarter of the columns
         const int b = tl.nlev - 1, m = tl.mb;
         T *Eb = E + tl.off[b];
         const T *Rb = Rr + tl.off[b];
-        for (int row = tid; row < m; row += nt) {
+        T *Pb = Rr + tl.total;  // parts x m partial sums
+        const int parts = m > 0 ? max(1, min(4, nt / m)) : 1;
+        for (int q = tid; q < m * parts; q += nt) {
+            const int row = q % m, part = q / m;
+            const int k0 = part * m / parts, k1 = (part + 1) * m / parts;
             T s = 0;
-            for (int k = 0; k < m; ++k) s += pinv[(size_t)k * m + row] * Rb[k];
+            for (int k = k0; k < k1; ++k) s += pinv[(size_t)k * m + row] * Rb[k];
+            Pb[q] = s;
+        }
+        __syncthreads();
+        for (int row = tid; row < m; row += nt) {
+            T s = Pb[row];
+            for (int part = 1; part < parts; ++part) s += Pb[row + part * m];
             Eb[row] = s;
         }
         __syncthreads();
@@ -278,10 +343,47 @@ __global__ void k_mg_start(Ctl *ctl)
     ctl->gm_stop = 0;
     ctl->mg_iters = 0;
     ctl->mg_started = 0;
+    ctl->b_norm_fixed = 0;
     ctl->res_norm = 0.0;
     ctl->b_norm = 0.0;
     ctl->phi_sum = 0.0;
     ctl->phi_mean = 0.0;
+}
+
+// ||b||inf of the (centred) right-hand side, for a solve that starts from a
+// nonzero guess: the stopping test stays relative to ||b||inf
+__global__ __launch_bounds__(256) void k_absmax_partials(const double *b, int n, int P, double *partials)
+{
+    __shared__ double sh[16];
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63), j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const double m = (i < n && j < n) ? fabs(b[i + (size_t)P * j]) : 0.0;
+    const double r = block_reduce<true>(m, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x + (size_t)gridDim.x * blockIdx.y] = r;
+}
+
+__global__ __launch_bounds__(1024) void k_fin_b_norm(const double *partials, int nb, Ctl *ctl)
+{
+    __shared__ double sh[32];
+    double m = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+        const double p = partials[b];
+        m = (p > m || p != p) ? p : m;
+    }
+    m = block_reduce<true>(m, sh);
+    if (threadIdx.x == 0) {
+        ctl->b_norm = m;
+        ctl->b_norm_fixed = 1;
+    }
+}
+
+hipError_t launch_b_norm(const Ctx &c)
+{
+    const int n = c.g.n;
+    const dim3 grid((n + 63) / 64, (n + 3) / 4);
+    if ((int64_t)grid.x * grid.y > kMaxPartials) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_absmax_partials, grid, dim3(256), 0, c.stream, c.b, n, c.g.P, c.partials);
+    hipLaunchKernelGGL(k_fin_b_norm, dim3(1), dim3(1024), 0, c.stream, c.partials, (int)(grid.x * grid.y), c.ctl);
+    return hipGetLastError();
 }
 
 hipError_t launch_mg_start(const Ctx &c)
@@ -307,7 +409,7 @@ __global__ __launch_bounds__(1024) void k_fin_resid(const double *partials, int 
         ctl->phi_sum = s;
         ctl->phi_mean = s / ((double)n * (double)n);
         if (!ctl->mg_started) {
-            ctl->b_norm = m;  // phi = 0 on the first evaluation
+            if (!ctl->b_norm_fixed) ctl->b_norm = m;  // phi = 0 on the first evaluation: r = b
             ctl->mg_started = 1;
         } else {
             ctl->mg_iters += 1;
@@ -354,45 +456,80 @@ static TailLevels tail_levels(const Ctx &c, int lt, size_t *lds_bytes, size_t el
     }
     tl.total = off;
     tl.mb = c.mb;
-    *lds_bytes = 2 * (size_t)off * elem;
+    *lds_bytes = (2 * (size_t)off + 4 * (size_t)c.mb) * elem;  // e, r of every level + bottom partial sums
     return tl;
 }
 
 template <class T>
-static hipError_t vcycle_impl(const Ctx &c, const T *pinv)
+static dim3 tile_grid(const Level &L) { return dim3((L.nx + SX - 1) / SX, (L.ny + SY - 1) / SY); }
+
+// One V-cycle on the correction from zero.  r0 is the level-0 right-hand side;
+// with fuse set (and level 0 smoothed by its own kernels) the last up-leg
+// smoother also applies the correction to the FP64 iterate and writes the next
+// level-0 right-hand side to r0_next with its norm partials.
+template <class T>
+static hipError_t vcycle_impl(const Ctx &c, const T *pinv, const T *r0, const OuterArgs *fuse, T *r0_next)
 {
     const Ctl *ctl = c.ctl;
     const int lt = c.tail_level;
-    for (int l = 0; l < lt; ++l) {  // down: smooth from zero, residual + restriction
+    const OuterArgs none{};
+    for (int l = 0; l < lt; ++l) {  // down: smooth from zero, fused residual + restriction
         const Level &L = c.lev[l], &Cl = c.lev[l + 1];
-        const T ax = (T)L.ax, ay = (T)L.ay;
-        hipLaunchKernelGGL(k_smooth<T>, dim3((L.nx + SX - 1) / SX, (L.ny + SY - 1) / SY), dim3(256), 0, c.stream,
-                           (const T *)nullptr, (const T *)nullptr, 0, lev_e<T>(L, 0), (const T *)lev_r<T>(L), L.nx,
-                           L.ny, L.pitch, ax, ay, ctl);
-        hipLaunchKernelGGL(k_resid_restrict<T>, dim3((Cl.nx + 63) / 64, (Cl.ny + 3) / 4), dim3(256), 0, c.stream,
-                           (const T *)lev_e<T>(L, 0), (const T *)lev_r<T>(L), L.nx, L.ny, L.pitch, ax, ay,
-                           lev_r<T>(Cl), Cl.nx, Cl.ny, Cl.pitch, ctl);
+        hipLaunchKernelGGL((k_smooth<T, 1>), tile_grid<T>(L), dim3(256), 0, c.stream, (const T *)nullptr,
+                           (const T *)nullptr, 0, lev_e<T>(L, 0), l == 0 ? r0 : (const T *)lev_r<T>(L), L.nx, L.ny,
+                           L.pitch, (T)L.ax, (T)L.ay, lev_r<T>(Cl), Cl.nx, Cl.ny, Cl.pitch, none, ctl);
     }
     {
         size_t lds = 0;
         const TailLevels tl = tail_levels(c, lt, &lds, sizeof(T));
         const Level &Lt = c.lev[lt];
-        hipLaunchKernelGGL(k_tail<T>, dim3(1), dim3(1024), lds, c.stream, (const T *)lev_r<T>(Lt), Lt.pitch,
-                           lev_e<T>(Lt, 1), tl, pinv, ctl);
+        hipLaunchKernelGGL(k_tail<T>, dim3(1), dim3(1024), lds, c.stream, lt == 0 ? r0 : (const T *)lev_r<T>(Lt),
+                           Lt.pitch, lev_e<T>(Lt, 1), tl, pinv, ctl);
     }
     for (int l = lt - 1; l >= 0; --l) {  // up: add the coarse correction, smooth
         const Level &L = c.lev[l], &Cl = c.lev[l + 1];
-        const T ax = (T)L.ax, ay = (T)L.ay;
-        hipLaunchKernelGGL(k_smooth<T>, dim3((L.nx + SX - 1) / SX, (L.ny + SY - 1) / SY), dim3(256), 0, c.stream,
-                           (const T *)lev_e<T>(L, 0), (const T *)lev_e<T>(Cl, 1), Cl.pitch, lev_e<T>(L, 1),
-                           (const T *)lev_r<T>(L), L.nx, L.ny, L.pitch, ax, ay, ctl);
+        const T *rl = l == 0 ? r0 : (const T *)lev_r<T>(L);
+        if (l == 0 && fuse)
+            hipLaunchKernelGGL((k_smooth<T, 2>), tile_grid<T>(L), dim3(256), 0, c.stream, (const T *)lev_e<T>(L, 0),
+                               (const T *)lev_e<T>(Cl, 1), Cl.pitch, r0_next, rl, L.nx, L.ny, L.pitch, (T)L.ax,
+                               (T)L.ay, (T *)nullptr, 0, 0, 0, *fuse, ctl);
+        else
+            hipLaunchKernelGGL((k_smooth<T, 0>), tile_grid<T>(L), dim3(256), 0, c.stream, (const T *)lev_e<T>(L, 0),
+                               (const T *)lev_e<T>(Cl, 1), Cl.pitch, lev_e<T>(L, 1), rl, L.nx, L.ny, L.pitch,
+                               (T)L.ax, (T)L.ay, (T *)nullptr, 0, 0, 0, none, ctl);
     }
     return hipGetLastError();
 }
 
 hipError_t launch_vcycle(const Ctx &c, int mixed)
 {
-    return mixed ? vcycle_impl<float>(c, c.pinv32) : vcycle_impl<double>(c, c.pinv);
+    const Level &L0 = c.lev[0];
+    return mixed ? vcycle_impl<float>(c, c.pinv32, L0.r32, nullptr, nullptr)
+                 : vcycle_impl<double>(c, c.pinv, L0.r, nullptr, nullptr);
+}
+
+// One outer step of the Poisson iteration: V-cycle on the level-0 residual
+// buffer `par`, phi_out = phi_in + e, the next residual into buffer par ^ 1 and
+// its norm partials (then launch_fin_resid).  Falls back to the separate
+// correction kernel when level 0 is inside the single-workgroup tail.
+hipError_t launch_vcycle_step(const Ctx &c, int mixed, int par, const double *phi_in, double *phi_out, int *nblocks)
+{
+    const Level &L0 = c.lev[0];
+    if (c.tail_level == 0) {
+        hipError_t e = launch_vcycle(c, mixed);
+        if (e != hipSuccess) return e;
+        return launch_correct_residual(c, phi_in, phi_out, 1, mixed, nblocks);
+    }
+    const dim3 g = tile_grid<double>(L0);
+    if ((int64_t)g.x * g.y > kMaxPartials) return hipErrorInvalidValue;
+    *nblocks = (int)(g.x * g.y);
+    OuterArgs o{phi_in, phi_out, c.b, c.partials, L0.ax, L0.ay};
+    if (mixed) {
+        float *rb[2] = {L0.r32, L0.r32b};
+        return vcycle_impl<float>(c, c.pinv32, rb[par], &o, rb[par ^ 1]);
+    }
+    double *rb[2] = {L0.r, L0.rb};
+    return vcycle_impl<double>(c, c.pinv, rb[par], &o, rb[par ^ 1]);
 }
 
 size_t tail_lds_bytes(const Ctx &c, int mixed)
@@ -459,9 +596,10 @@ hipError_t launch_smooth_fine(const Ctx &c)
 {
     const Level &L = c.lev[0];
     hipLaunchKernelGGL(k_ctl_clear_mg, dim3(1), dim3(1), 0, c.stream, c.ctl);
-    hipLaunchKernelGGL(k_smooth<double>, dim3((L.nx + SX - 1) / SX, (L.ny + SY - 1) / SY), dim3(256), 0, c.stream,
-                       (const double *)nullptr, (const double *)nullptr, 0, L.e, (const double *)L.r, L.nx, L.ny,
-                       L.pitch, L.ax, L.ay, (const Ctl *)c.ctl);
+    const OuterArgs none{};
+    hipLaunchKernelGGL((k_smooth<double, 0>), tile_grid<double>(L), dim3(256), 0, c.stream, (const double *)nullptr,
+                       (const double *)nullptr, 0, L.e, (const double *)L.r, L.nx, L.ny, L.pitch, L.ax, L.ay,
+                       (double *)nullptr, 0, 0, 0, none, (const Ctl *)c.ctl);
     return hipGetLastError();
 }
 

@@ -191,20 +191,23 @@ def run_both(n, ren, steps, init="zero", **kw):
     return ref, ref_stats, got, stats
 
 
+def div_floor(state):
+    # the round-off floor of the discrete divergence: a few ulps of |u| / h
+    n = state["u"].shape[0]
+    return max(1e-12, 16 * np.finfo(np.float64).eps * max(np.abs(state["u"]).max(), np.abs(state["v"]).max()) * n)
+
+
 def check_step_parity(ref, ref_stats, got, stats, tol=1e-10):
     assert rel(got["u"], ref["u"]) < tol
     assert rel(got["v"], ref["v"]) < tol
     p, pr = got["ctx"][0, 1:-1, 1:-1], ref["ctx"][0, 1:-1, 1:-1]
     assert rel(p - p.mean(), pr - pr.mean()) < tol
     assert rel(got["uprev"], ref["uprev"]) < tol
-    # divergence bar: the solver tolerance relative to ||D u*||, above the
-    # round-off floor of the discrete divergence (a few ulps of |u| / h)
-    n = got["u"].shape[0]
-    floor = 16 * np.finfo(np.float64).eps * max(np.abs(got["u"]).max(), np.abs(got["v"]).max()) * n
+    floor = div_floor(got)
     for s, r in zip(stats, ref_stats):
         assert s["pseudo_iters"] == r["pseudo_iters"]
         assert s["flags"] == 0
-        assert s["div_new"] <= 1e-10 * s["div_star"] + max(1e-12, floor)
+        assert s["div_new"] <= 1e-10 * s["div_star"] + floor
     # wall-normal faces stay exactly zero
     assert np.all(got["u"][:, [0, -1]] == 0.0) and np.all(got["v"][[0, -1], :] == 0.0)
 
@@ -225,6 +228,14 @@ def test_c2_cavity1024_re1000_first_steps():
     # the FP64 GPU workload at its full size (1024^2, Re = 1000, seeded), first steps
     ref, rs, got, st = run_both(1024, 1000.0, 2, init="seeded")
     check_step_parity(ref, rs, got, st)
+
+
+def test_advance_warm_started_poisson_matches_oracle():
+    # the warm-started solve stops on the same ||r||inf <= 1e-10 ||b||inf test
+    ref, rs, got, st = run_both(128, 400.0, 12, init="seeded", poisson_warm_start=1)
+    check_step_parity(ref, rs, got, st)
+    _, _, _, cold = run_both(128, 400.0, 12, init="seeded", poisson_warm_start=0)
+    assert sum(s["vcycles"] for s in st[2:]) < sum(s["vcycles"] for s in cold[2:])
 
 
 def test_advance_seeded_256_re1000():
@@ -313,7 +324,7 @@ def test_mixed_matches_fp64_within_1e_6(n, ren, steps, mm):
             assert 1 <= s["pseudo_iters"] <= 6 and s["pseudo_iters"] < r["pseudo_iters"]
         assert s["flags"] == 0
         # the outer Poisson residual and the divergence check stay FP64
-        assert s["div_new"] <= 1e-10 * s["div_star"] + 1e-12
+        assert s["div_new"] <= 1e-10 * s["div_star"] + div_floor(a)
     assert np.all(a["u"][:, [0, -1]] == 0.0) and np.all(a["v"][[0, -1], :] == 0.0)
 
 

@@ -230,8 +230,10 @@ int gmres_loop(amressif_ctx *ctx)
 // ---- the Poisson solve on c.phi / c.b (b already centred), shared by the
 // step and the module entry point
 int mg_loop(amressif_ctx *ctx, bool warm);
-// warm: start from the previous step's solution (the stopping test stays
-// relative to ||b||inf); cold: from zero, as poisson.cpp:63
+// warm: start from the linear extrapolation 2 phi^{n-1} - phi^{n-2} of the
+// previous steps' solutions (the previous solution alone right after a state
+// load); the stopping test stays relative to ||b||inf.  cold: from zero, as
+// poisson.cpp:63
 int poisson_loop(amressif_ctx *ctx, bool warm = false)
 {
     return ctx->c.in.poisson_solver == 1 ? gmres_loop(ctx) : mg_loop(ctx, warm);
@@ -244,7 +246,12 @@ int mg_loop(amressif_ctx *ctx, bool warm)
     int nb = 0;
     HIPCHK(amr::launch_mg_start(c));
     if (warm) {
-        if (c.phi != c.phiv[0]) std::swap(c.phiv[0], c.phiv[1]);  // phiv[0] <- the previous solution
+        if (!c.phih) HIPCHK(amr::dev_alloc_public(c, &c.phih, (size_t)(c.g.n + 1) * c.g.P));
+        // the guess goes into the buffer that does not hold phi^{n-1}; phih <- phi^{n-1}
+        double *guess = c.phi == c.phiv[0] ? c.phiv[1] : c.phiv[0];
+        HIPCHK(amr::launch_extrapolate(c, c.phi, c.phih, guess, c.phih_valid));
+        c.phih_valid = 1;
+        if (guess != c.phiv[0]) std::swap(c.phiv[0], c.phiv[1]);
         HIPCHK(amr::launch_b_norm(c));
     } else {
         HIPCHK(amr::launch_fill(c, c.phiv[0], (int64_t)(c.g.n + 1) * c.g.P, 0.0));
@@ -688,6 +695,7 @@ int amressif_load_state(amressif_ctx *ctx, const amr_array4 velCont[2], const am
         HIPCHK(amr::launch_fill(c, c.phi, (int64_t)(c.g.n + 1) * c.g.P, 0.0));
     }
     HIPCHK(amr::launch_mg_start(c));  // phi_mean = 0 for the loaded phi
+    c.phih_valid = 0;                  // no solution history for the warm start
     HIPCHK(hipStreamSynchronize(c.stream));
     return AMRESSIF_OK;
 }

@@ -107,6 +107,8 @@ struct Ctx {
     size_t gm_vs = 0;        // doubles between consecutive vectors
     GmState *gm = nullptr;
     double *phiv[2]{};   // Poisson iterates (ping-pong); phi points at the last solution
+    double *phih = nullptr;  // warm start: the solution of the step before (allocated on first use)
+    int phih_valid = 0;
     int tail_level = 0;  // first multigrid level handled by the single-workgroup tail kernel
     const double *lid = nullptr;  // caller-owned device lid profile or null
     Level lev[kMaxLevels]{};
@@ -161,6 +163,7 @@ hipError_t launch_fin_prhs(const Ctx &c, int nblocks);
 hipError_t launch_center_b(const Ctx &c, double *b);
 hipError_t launch_mg_start(const Ctx &c);
 hipError_t launch_b_norm(const Ctx &c);
+hipError_t launch_extrapolate(const Ctx &c, const double *phi1, double *phi2, double *guess, int extrapolate);
 hipError_t launch_fin_resid(const Ctx &c, int nblocks);
 hipError_t launch_vcycle(const Ctx &c, int mixed);
 hipError_t launch_vcycle_step(const Ctx &c, int mixed, int par, const double *phi_in, double *phi_out, int *nblocks);

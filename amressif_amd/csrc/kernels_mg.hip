@@ -386,6 +386,26 @@ hipError_t launch_b_norm(const Ctx &c)
     return hipGetLastError();
 }
 
+// warm-start guess: 2 phi1 - phi2 (phi1 alone without history), then phi2 <- phi1
+__global__ __launch_bounds__(256) void k_extrapolate(const double *phi1, double *phi2, double *guess, int n, int P,
+                                                     int extrapolate)
+{
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63), j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (i >= n || j >= n) return;
+    const size_t k = i + (size_t)P * j;
+    const double a = phi1[k];
+    guess[k] = extrapolate ? 2.0 * a - phi2[k] : a;
+    phi2[k] = a;
+}
+
+hipError_t launch_extrapolate(const Ctx &c, const double *phi1, double *phi2, double *guess, int extrapolate)
+{
+    const int n = c.g.n;
+    hipLaunchKernelGGL(k_extrapolate, dim3((n + 63) / 64, (n + 3) / 4), dim3(256), 0, c.stream, phi1, phi2, guess, n,
+                       c.g.P, extrapolate);
+    return hipGetLastError();
+}
+
 hipError_t launch_mg_start(const Ctx &c)
 {
     hipLaunchKernelGGL(k_mg_start, dim3(1), dim3(1), 0, c.stream, c.ctl);

@@ -71,7 +71,6 @@ __global__ __launch_bounds__(256) void k_smooth(const T *xin, const T *ec, int p
     constexpr int H = MODE == 0 ? 4 : 5;
     using S = SmoothShape<H>;
     constexpr int SW = S::W, SHT = S::HT;
-    if (ctl->mg_done | ctl->gm_stop) return;
     __shared__ T sx[SHT][SW], sr[SHT][SW];
     const int tid = threadIdx.x;
     const int i0 = blockIdx.x * SX, j0 = blockIdx.y * SY;
@@ -82,22 +81,34 @@ __global__ __launch_bounds__(256) void k_smooth(const T *xin, const T *ec, int p
         pj[q] = p < S::PAIRS ? p / (SW / 2) : -1;
         pl[q] = p % (SW / 2);
     }
+    // the loads are issued before the control flag is read, so their latency
+    // overlaps the flag's (all addresses are in bounds either way)
+    T xv[S::PER][2], rv[S::PER][2];
 #pragma unroll
     for (int q = 0; q < S::PER; ++q) {
-        if (pj[q] < 0) continue;
         const int lj = pj[q], j = j0 - H + lj;
 #pragma unroll
         for (int c2 = 0; c2 < 2; ++c2) {
             const int li = 2 * pl[q] + c2, i = i0 - H + li;
             T x = 0, r = 0;
-            if (i >= 0 && i < nx && j >= 0 && j < ny) {
+            if (lj >= 0 && i >= 0 && i < nx && j >= 0 && j < ny) {
                 const size_t k = i + (size_t)pitch * j;
                 if (xin) x = xin[k];
                 if (ec) x += ec[(i >> 1) + (size_t)pc * (j >> 1)];
                 r = rhs[k];
             }
-            sx[lj][li] = x;
-            sr[lj][li] = r;
+            xv[q][c2] = x;
+            rv[q][c2] = r;
+        }
+    }
+    if (ctl->mg_done | ctl->gm_stop) return;
+#pragma unroll
+    for (int q = 0; q < S::PER; ++q) {
+        if (pj[q] < 0) continue;
+#pragma unroll
+        for (int c2 = 0; c2 < 2; ++c2) {
+            sx[pj[q]][2 * pl[q] + c2] = xv[q][c2];
+            sr[pj[q]][2 * pl[q] + c2] = rv[q][c2];
         }
     }
     __syncthreads();
@@ -202,13 +213,18 @@ struct TailLevels {
     int mb;            // bottom cells
 };
 
+// e / d for 0 <= e < 2^20 and d <= 64 through a float reciprocal: the exact
+// quotient's fractional part stays >= 0.5/d away from an integer
+__device__ __forceinline__ int qdiv(int e, float inv) { return (int)(((float)e + 0.5f) * inv); }
+
 template <class T>
 __device__ void tail_gsrb(T *E, const T *Rr, int nx, int ny, T ax, T ay, int color)
 {
     const T inv_d = T(1) / (T(2) * ax + T(2) * ay);
     const int half = (nx + 1) / 2;
+    const float ih = 1.0f / (float)half;
     for (int e = threadIdx.x; e < half * ny; e += blockDim.x) {
-        const int j = e / half, i = 2 * (e % half) + ((j + color) & 1);
+        const int j = qdiv(e, ih), i = 2 * (e - j * half) + ((j + color) & 1);
         if (i >= nx) continue;
         const int k = i + nx * j;
         T s = 0;
@@ -233,14 +249,24 @@ template <class T>
 __global__ __launch_bounds__(1024) void k_tail(const T *rtop, int ptop, T *etop, TailLevels tl, const T *pinv,
                                                const Ctl *ctl)
 {
-    if (ctl->mg_done | ctl->gm_stop) return;
     extern __shared__ unsigned char smem_raw[];
     T *E = reinterpret_cast<T *>(smem_raw);
     T *Rr = E + tl.total;
     const int tid = threadIdx.x, nt = blockDim.x;
-    {
+    {  // top level r (at most 64^2 = 4 per thread), loaded before the control flag is read
         const int nx = tl.nx[0], ny = tl.ny[0];
-        for (int k = tid; k < nx * ny; k += nt) Rr[k] = rtop[(k % nx) + (size_t)ptop * (k / nx)];
+        const float inx = 1.0f / (float)nx;
+        T v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int k = tid + q * nt, j = qdiv(k, inx);
+            v[q] = k < nx * ny ? rtop[(k - j * nx) + (size_t)ptop * j] : T(0);
+        }
+        if (ctl->mg_done | ctl->gm_stop) return;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (tid + q * nt < nx * ny) Rr[tid + q * nt] = v[q];
+        for (int k = tid + 4 * nt; k < nx * ny; k += nt) Rr[k] = rtop[(k % nx) + (size_t)ptop * (k / nx)];
     }
     __syncthreads();
     for (int l = 0; l < tl.nlev - 1; ++l) {
@@ -253,8 +279,9 @@ __global__ __launch_bounds__(1024) void k_tail(const T *rtop, int ptop, T *etop,
         for (int s = 0; s < 2; ++s) { tail_gsrb(El, Rl, nx, ny, ax, ay, 0); tail_gsrb(El, Rl, nx, ny, ax, ay, 1); }
         const int cx = tl.nx[l + 1], cy = tl.ny[l + 1];
         T *Rc = Rr + tl.off[l + 1];
+        const float icx = 1.0f / (float)cx;
         for (int k = tid; k < cx * cy; k += nt) {
-            const int I = k % cx, J = k / cx, i = 2 * I, j = 2 * J;
+            const int J = qdiv(k, icx), I = k - J * cx, i = 2 * I, j = 2 * J;
             const T t00 = Rl[i + nx * j] - lap_at(El, i, j, nx, ny, nx, ax, ay);
             const T t10 = Rl[i + 1 + nx * j] - lap_at(El, i + 1, j, nx, ny, nx, ax, ay);
             const T t01 = Rl[i + nx * (j + 1)] - lap_at(El, i, j + 1, nx, ny, nx, ax, ay);
@@ -291,7 +318,11 @@ __global__ __launch_bounds__(1024) void k_tail(const T *rtop, int ptop, T *etop,
         const T *Ec = E + tl.off[l + 1];
         const T *Rl = Rr + tl.off[l];
         const T ax = (T)tl.ax[l], ay = (T)tl.ay[l];
-        for (int k = tid; k < nx * ny; k += nt) El[k] += Ec[((k % nx) >> 1) + cx * ((k / nx) >> 1)];
+        const float inx = 1.0f / (float)nx;
+        for (int k = tid; k < nx * ny; k += nt) {
+            const int j = qdiv(k, inx);
+            El[k] += Ec[((k - j * nx) >> 1) + cx * (j >> 1)];
+        }
         __syncthreads();
         for (int s = 0; s < 2; ++s) { tail_gsrb(El, Rl, nx, ny, ax, ay, 0); tail_gsrb(El, Rl, nx, ny, ax, ay, 1); }
     }

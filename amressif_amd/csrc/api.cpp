@@ -971,10 +971,10 @@ int amressif_time_kernel(amressif_ctx *ctx, int which, int reps, int64_t copy_by
         HIPCHK(hipMalloc(&src, (size_t)copy_bytes));
         HIPCHK(hipMalloc(&dst, (size_t)copy_bytes));
         HIPCHK(hipMemsetAsync(src, 0, (size_t)copy_bytes, c.stream));
-    } else if (which < 0 || which > 4) {
-        return fail(ctx, AMRESSIF_ERR_ARG, "which must be 0..4");
+    } else if (which < 0 || which > 5) {
+        return fail(ctx, AMRESSIF_ERR_ARG, "which must be 0..5");
     }
-    if (which >= 3 && c.in.precision != AMRESSIF_MIXED)
+    if ((which == 3 || which == 4) && c.in.precision != AMRESSIF_MIXED)
         return fail(ctx, AMRESSIF_ERR_ARG, "which = 3, 4 need a mixed-precision context");
     auto launch = [&]() -> hipError_t {
         if (which == 3 || which == 4) {  // the two kernels of a mixed pseudo-iteration
@@ -1002,7 +1002,7 @@ int amressif_time_kernel(amressif_ctx *ctx, int which, int reps, int64_t copy_by
             int nb = 0;
             return amr::launch_stage(c, a, &nb);
         }
-        if (which == 1) return amr::launch_smooth_fine(c);
+        if (which == 1 || which == 5) return amr::launch_smooth_fine(c, which, c.S2[0]);
         return amr::launch_copy_bytes(c, src, dst, copy_bytes);
     };
     HIPCHK(launch());  // warm-up
@@ -1021,15 +1021,18 @@ int amressif_time_kernel(amressif_ctx *ctx, int which, int reps, int64_t copy_by
     if (dst) hipFree(dst);
     *avg_ms = (double)ms / reps;
     if (bytes_per_launch) {
-        // algorithmic bytes per cell:
+        // algorithmic bytes per cell (t = 4 or 8: the V-cycle precision):
         // 0 FP64 stage: reads u*, v*, u^k, v^k, c_u, c_v, p, writes u*, v* (9 x 8 B);
-        // 1 fused fine smoother from zero: reads r, writes e (2 x 8 B);
+        // 1 finest down-leg smoother: reads r, writes e (2 t), the coarse r (t / 4);
         // 2 copy: read + write of copy_bytes;
         // 3 mixed residual: reads u^k, v^k, c_u, c_v, p (5 x 8 B), writes the FP32 base and
         //   residual (4 x 4 B);
         // 4 tangent-linear update: reads the FP32 base and residual (4 x 4 B), reads and
-        //   writes u^k, v^k (4 x 8 B)
-        static const double per_cell[5] = {72.0, 16.0, 0.0, 56.0, 48.0};
+        //   writes u^k, v^k (4 x 8 B);
+        // 5 finest up-leg smoother + FP64 outer step: reads e, r (2 t), the coarse e (t / 4),
+        //   phi, b (2 x 8 B), writes phi (8 B) and the next r (t)
+        const double t = c.in.precision == AMRESSIF_MIXED ? 4.0 : 8.0;
+        const double per_cell[6] = {72.0, 2.25 * t, 0.0, 56.0, 48.0, 3.25 * t + 24.0};
         *bytes_per_launch = which == 2 ? 2.0 * (double)copy_bytes : per_cell[which] * cells;
     }
     return AMRESSIF_OK;

@@ -171,7 +171,7 @@ hipError_t launch_correct_residual(const Ctx &c, const double *phi_in, double *p
                                    int mixed, int *nblocks);
 size_t tail_lds_bytes(const Ctx &c, int mixed);
 hipError_t configure_tail(Ctx &c);
-hipError_t launch_smooth_fine(const Ctx &c);
+hipError_t launch_smooth_fine(const Ctx &c, int kind, double *scratch_phi);
 hipError_t launch_project(const Ctx &c, const double *us, const double *vs, double *uo, double *vo, int *nblocks,
                           const float *au = nullptr, const float *av = nullptr);
 hipError_t launch_fin_project(const Ctx &c, int nblocks, int step_index);

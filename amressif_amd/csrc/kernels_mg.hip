@@ -643,15 +643,36 @@ hipError_t configure_tail(Ctx &c)
 // diagnostic: one fused smoother on the finest level, unguarded (FP64)
 __global__ void k_ctl_clear_mg(Ctl *ctl) { ctl->mg_done = 0; }
 
-hipError_t launch_smooth_fine(const Ctx &c)
+// diagnostic: one finest-level smoother of the context's V-cycle precision,
+// unguarded (the control flag is cleared first), on the buffers of the last
+// solve: kind 1 the down-leg smoother with the fused restriction, kind 5 the
+// up-leg smoother with the fused FP64 outer step (writing to scratch: phi_out
+// into scratch_phi, the next residual into the second level-0 buffer)
+template <class T>
+static hipError_t smooth_probe(const Ctx &c, int kind, T *r0b, double *scratch_phi)
 {
-    const Level &L = c.lev[0];
-    hipLaunchKernelGGL(k_ctl_clear_mg, dim3(1), dim3(1), 0, c.stream, c.ctl);
+    const Level &L = c.lev[0], &Cl = c.lev[1];
     const OuterArgs none{};
-    hipLaunchKernelGGL((k_smooth<double, 0>), tile_grid<double>(L), dim3(256), 0, c.stream, (const double *)nullptr,
-                       (const double *)nullptr, 0, L.e, (const double *)L.r, L.nx, L.ny, L.pitch, L.ax, L.ay,
-                       (double *)nullptr, 0, 0, 0, none, (const Ctl *)c.ctl);
+    if (kind == 1)
+        hipLaunchKernelGGL((k_smooth<T, 1>), tile_grid<T>(L), dim3(256), 0, c.stream, (const T *)nullptr,
+                           (const T *)nullptr, 0, lev_e<T>(L, 0), (const T *)lev_r<T>(L), L.nx, L.ny, L.pitch,
+                           (T)L.ax, (T)L.ay, lev_r<T>(Cl), Cl.nx, Cl.ny, Cl.pitch, none, (const Ctl *)c.ctl);
+    else {
+        OuterArgs o{c.phi, scratch_phi, c.b, c.partials, L.ax, L.ay};
+        hipLaunchKernelGGL((k_smooth<T, 2>), tile_grid<T>(L), dim3(256), 0, c.stream, (const T *)lev_e<T>(L, 0),
+                           (const T *)lev_e<T>(Cl, 1), Cl.pitch, r0b, (const T *)lev_r<T>(L), L.nx, L.ny, L.pitch,
+                           (T)L.ax, (T)L.ay, (T *)nullptr, 0, 0, 0, o, (const Ctl *)c.ctl);
+    }
     return hipGetLastError();
+}
+
+hipError_t launch_smooth_fine(const Ctx &c, int kind, double *scratch_phi)
+{
+    if (c.tail_level == 0) return hipErrorInvalidValue;  // level 0 has no smoother kernel
+    hipLaunchKernelGGL(k_ctl_clear_mg, dim3(1), dim3(1), 0, c.stream, c.ctl);
+    const Level &L = c.lev[0];
+    return c.in.precision == AMRESSIF_MIXED ? smooth_probe<float>(c, kind, L.r32b, scratch_phi)
+                                            : smooth_probe<double>(c, kind, L.rb, scratch_phi);
 }
 
 }  // namespace amr

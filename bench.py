@@ -40,8 +40,10 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 # timing kinds of amressif_time_kernel -> kernel symbol
-KERNELS = {0: "amr::k_stage64 (FP64 stage)", 1: "amr::k_smooth (finest level)", 3: "amr::k_stage64 (mixed residual)",
-           4: "amr::k_tangent"}
+KERNELS = {0: "amr::k_stage64 (FP64 stage)", 1: "amr::k_smooth<T, 1> (finest down-leg smoother + restriction)",
+           3: "amr::k_stage64 (mixed residual)", 4: "amr::k_tangent",
+           5: "amr::k_smooth<T, 2> (finest up-leg smoother + FP64 outer step)"}
+PMC_KEYS = {0: "stage_fp64", 3: "residual_mixed", 4: "tangent_mixed"}
 
 
 def parse():
@@ -169,7 +171,7 @@ def kernel_probe(api, W, n, ren, which_prec):
     prec = api.AMRESSIF_MIXED if which_prec == "mixed" else api.AMRESSIF_FP64
     run = api.Run.from_numpy(api.cavity_inputs(n, ren, precision=prec), W.initial_state(n, "seeded"))
     run.advance(1)
-    kinds = (3, 4) if which_prec == "mixed" else (0, 1)
+    kinds = (3, 4, 1, 5) if which_prec == "mixed" else (0, 1, 5)
     res = {KERNELS[k]: run.ctx.time_kernel(k, reps=3)[0] for k in kinds}
     res["copy 64 MiB"] = run.ctx.time_kernel(2, reps=3, copy_bytes=64 << 20)[0]
     print(json.dumps({"kernel_probe": which_prec, "n": n, "avg_ms": res}), flush=True)
@@ -228,6 +230,21 @@ def main():
                 "frac": achieved / HBM_PEAK, "traffic": read_traffic(f"{name}_{n}"), "kernel": KERNELS[which],
                 "avg_launch_ms": ms, "algorithmic_bytes_per_launch": nbytes}
 
+    def dominant(name, kinds):
+        """the kernel with the largest time per step among the per-iteration kernels
+        (launches per step: the pseudo-iteration kernels once per iteration, the
+        finest-level smoothers once per V-cycle)"""
+        o = out[name]
+        cand = []
+        for w in kinds:
+            r = kernel_roofline(runs[name], w, f"{PMC_KEYS.get(w, 'smooth%d' % w)}_{name}" if w in (1, 5)
+                                else PMC_KEYS[w])
+            per_step = o["pseudo_iters"] * (4 if w == 0 else 1) if w in (0, 3, 4) else o["vcycles"]
+            r["launches_per_step"] = per_step
+            r["ms_per_step"] = per_step * r["avg_launch_ms"]
+            cand.append(r)
+        return max(cand, key=lambda r: r["ms_per_step"]), cand
+
     head = "mixed" if "mixed" in runs else "fp64"
     extra = {}
     if "fp64" in out:
@@ -236,13 +253,11 @@ def main():
         extra["fp64_step_roofline"] = {"bytes_per_step": nbytes, "achieved": nbytes / (o["ms_per_step"] * 1e-3) / 1e9,
                                        "unit": "GB/s", "frac": nbytes / (o["ms_per_step"] * 1e-3) / HBM_PEAK}
     if head == "mixed":
-        cand = [kernel_roofline(runs["mixed"], w, nm) for w, nm in ((3, "residual_mixed"), (4, "tangent_mixed"))]
-        roofline = max(cand, key=lambda r: r["avg_launch_ms"])
-        extra["mixed_kernels"] = cand
+        roofline, extra["mixed_kernels"] = dominant("mixed", (3, 4, 1, 5))
     else:
-        roofline = kernel_roofline(runs["fp64"], 0, "stage_fp64")
-    if "fp64" in runs:
-        extra["fp64_stage"] = roofline if head == "fp64" else kernel_roofline(runs["fp64"], 0, "stage_fp64")
+        roofline, extra["fp64_kernels"] = dominant("fp64", (0, 1, 5))
+    if "fp64" in runs and head != "fp64":
+        _, extra["fp64_kernels"] = dominant("fp64", (0, 1, 5))
 
     adj = None
     if args.adjoint_n > 0 and not args.profile_run:

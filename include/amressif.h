@@ -281,10 +281,11 @@ int amressif_amr_advance(amressif_amr *amr, int nsteps, amressif_amr_stats *stat
  * shape and current state, timed with HIP events on the context's stream over
  * `reps` back-to-back launches:
  *   which = 0  fused FP64 RK stage (the FP64 step's dominant kernel)
- *   which = 1  fused finest-level smoother (both colours, from a zero guess)
+ *   which = 1  finest-level down-leg smoother with the fused restriction (V-cycle precision)
  *   which = 2  plain device-to-device copy of copy_bytes bytes (bandwidth calibration)
  *   which = 3  mixed precision: FP64 residual of the pseudo base
  *   which = 4  mixed precision: FP32 tangent-linear update of the pseudo base
+ *   which = 5  finest-level up-leg smoother with the fused FP64 outer step (V-cycle precision)
  * which = 3 and 4 need a context created with precision = 1.
  * bytes_per_launch receives the kernel's algorithmic HBM bytes per launch. */
 int amressif_time_kernel(amressif_ctx *ctx, int which, int reps, int64_t copy_bytes, double *avg_ms,

@@ -227,37 +227,45 @@ int gmres_loop(amressif_ctx *ctx)
     return 0;
 }
 
-// ---- the Poisson solve on c.phi / c.b (b already centred), shared by the
-// step and the module entry point
-int mg_loop(amressif_ctx *ctx, bool warm);
+// ---- the Poisson solve on c.phi / c.b, shared by the step and the module
+// entry point
+int mg_loop(amressif_ctx *ctx, bool warm, bool center);
 // warm: start from the linear extrapolation 2 phi^{n-1} - phi^{n-2} of the
 // previous steps' solutions (the previous solution alone right after a state
 // load); the stopping test stays relative to ||b||inf.  cold: from zero, as
-// poisson.cpp:63
-int poisson_loop(amressif_ctx *ctx, bool warm = false)
+// poisson.cpp:63.  center: b still holds its mean (ctl->b_mean), removed here.
+int poisson_loop(amressif_ctx *ctx, bool warm = false, bool center = false)
 {
-    return ctx->c.in.poisson_solver == 1 ? gmres_loop(ctx) : mg_loop(ctx, warm);
+    if (ctx->c.in.poisson_solver == 1) {
+        if (center) HIPCHK(amr::launch_center_b(ctx->c, ctx->c.b));
+        return gmres_loop(ctx);
+    }
+    return mg_loop(ctx, warm, center);
 }
 
-int mg_loop(amressif_ctx *ctx, bool warm)
+int mg_loop(amressif_ctx *ctx, bool warm, bool center)
 {
     Ctx &c = ctx->c;
     const int mixed = c.in.precision == AMRESSIF_MIXED;
-    int nb = 0;
+    int nb = 0, mode = 0;
+    const double *phi1 = nullptr, *phi2 = nullptr;
     HIPCHK(amr::launch_mg_start(c));
     if (warm) {
         if (!c.phih) HIPCHK(amr::dev_alloc_public(c, &c.phih, (size_t)(c.g.n + 1) * c.g.P));
-        // the guess goes into the buffer that does not hold phi^{n-1}; phih <- phi^{n-1}
-        double *guess = c.phi == c.phiv[0] ? c.phiv[1] : c.phiv[0];
-        HIPCHK(amr::launch_extrapolate(c, c.phi, c.phih, guess, c.phih_valid));
+        // three buffers, rotated instead of copied: X = phi^{n-1} (the last
+        // solution, one of the solve's pair), Y = phi^{n-2}, F = the other of
+        // the pair.  The guess goes into F and the solve ping-pongs F <-> Y;
+        // X becomes phi^{n-2} of the next step.
+        double *X = c.phi, *Y = c.phih, *F = c.phi == c.phiv[0] ? c.phiv[1] : c.phiv[0];
+        mode = c.phih_valid ? 2 : 1;
+        phi1 = X;
+        phi2 = Y;
+        c.phiv[0] = F;
+        c.phiv[1] = Y;
+        c.phih = X;
         c.phih_valid = 1;
-        if (guess != c.phiv[0]) std::swap(c.phiv[0], c.phiv[1]);
-        HIPCHK(amr::launch_b_norm(c));
-    } else {
-        HIPCHK(amr::launch_fill(c, c.phiv[0], (int64_t)(c.g.n + 1) * c.g.P, 0.0));
     }
-    HIPCHK(amr::launch_correct_residual(c, c.phiv[0], nullptr, 0, mixed, &nb));  // r = b - L phi0
-    HIPCHK(amr::launch_fin_resid(c, nb));
+    HIPCHK(amr::launch_poisson_start(c, mode, center ? 1 : 0, phi1, phi2, c.phiv[0], mixed));  // + its finalize
     int launched = 0, target = std::max(1, c.last_V);
     const int cap = std::max(1, c.in.mg_max_iter);
     for (;;) {
@@ -387,7 +395,7 @@ void amressif_inputs_default(amressif_inputs *in)
     in->poisson_tol_rel = 1.0e-10;
     in->poisson_tol_abs = 0.0;
     in->mg_max_iter = 2000;
-    in->mg_bottom_max = 16;
+    in->mg_bottom_max = 4;
     in->precision = AMRESSIF_FP64;
     in->poisson_solver = 0;
     in->gmres_restart = 30;
@@ -521,6 +529,17 @@ int amressif_create(const amressif_inputs *in, void *hip_stream, amressif_ctx **
     if (const char *e = std::getenv("AMRESSIF_TANGENT_ROWS"))
         c.tangent_rows = std::max(4, std::min(256, std::atoi(e)));
     if (const char *e = std::getenv("AMRESSIF_STAGE_ORDER")) c.stage_order = std::atoi(e) ? 1 : 0;
+    // and of the multigrid smoothers (kernel choice and rows per wave: 4, 8, 16 or 32)
+    if (const char *e = std::getenv("AMRESSIF_MG_SMOOTHER")) c.mg_smoother = std::atoi(e) ? 1 : 0;
+    auto rows_ok = [](int r) { return r == 4 || r == 8 || r == 16 || r == 32; };
+    for (int t = 0; t < 2; ++t)
+        if (const char *e = std::getenv(t ? "AMRESSIF_MG_ROWS32" : "AMRESSIF_MG_ROWS64")) {
+            int v[3];
+            if (std::sscanf(e, "%d,%d,%d", &v[0], &v[1], &v[2]) == 3 && rows_ok(v[0]) && rows_ok(v[1]) && rows_ok(v[2]))
+                for (int m = 0; m < 3; ++m) c.mg_rows[t][m] = v[m];
+        }
+    if (const char *e = std::getenv("AMRESSIF_MG_ROWS_COARSE"))
+        if (rows_ok(std::atoi(e))) c.mg_rows_coarse = std::atoi(e);
     auto bail = [&](int code, const std::string &m) {
         // keep the context so the caller can read the message; it owns nothing yet
         ctx->c.err = m;
@@ -775,8 +794,7 @@ int advance_impl(amressif_ctx *ctx, int nsteps, amressif_step_stats *stats, amre
         // ---- projection: Poisson RHS, MLMG-equivalent solve, update
         HIPCHK(amr::launch_poisson_rhs(c, c.U[0], c.U[1], c.b, &nb));
         HIPCHK(amr::launch_fin_prhs(c, nb));
-        HIPCHK(amr::launch_center_b(c, c.b));
-        if (int rc = poisson_loop(ctx, warm)) return rc;
+        if (int rc = poisson_loop(ctx, warm, true)) return rc;  // centres b
         HIPCHK(amr::launch_project(c, c.U[0], c.U[1], c.S1[0], c.S1[1], &nb));
         HIPCHK(amr::launch_fin_project(c, nb, s));
         std::swap(c.U[0], c.S1[0]);

@@ -76,13 +76,15 @@ __device__ __forceinline__ double lane_next(double x)
     const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), 0x130, 0xF, 0xF, false);
     return __hiloint2double(hi, lo);
 }
+// (float: bound_ctrl zero-fills the edge lanes, which lets the compiler fold
+// the shift into the consuming VALU instruction)
 __device__ __forceinline__ float lane_prev(float x)
 {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x138, 0xF, 0xF, true));
 }
 __device__ __forceinline__ float lane_next(float x)
 {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x130, 0xF, 0xF, true));
 }
 
 }  // namespace amr

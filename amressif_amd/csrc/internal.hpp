@@ -101,6 +101,13 @@ struct Ctx {
     int stage_rows = 8;      // k_stage64: output rows per wave
     int tangent_rows = 16;   // k_tangent: output rows per wave
     int stage_order = 1;     // stage kernels: 1 = consecutive waves take horizontally adjacent bands
+    // multigrid smoothers of the levels above the tail: 1 = wave-column kernel
+    // (registers + DPP), 0 = LDS tile kernel; output rows per wave of the
+    // wave-column kernel by [FP64, FP32][up leg, down leg, finest up leg], and
+    // on the levels below the finest
+    int mg_smoother = 1;
+    int mg_rows[2][3] = {{8, 8, 8}, {8, 8, 4}};
+    int mg_rows_coarse = 4;
     // FGMRES Poisson solver (poisson_solver = 1)
     int gm_m = 0;            // restart length
     double *gmV = nullptr, *gmZ = nullptr, *gm_part = nullptr;
@@ -162,8 +169,9 @@ hipError_t launch_poisson_rhs(const Ctx &c, const double *u, const double *v, do
 hipError_t launch_fin_prhs(const Ctx &c, int nblocks);
 hipError_t launch_center_b(const Ctx &c, double *b);
 hipError_t launch_mg_start(const Ctx &c);
-hipError_t launch_b_norm(const Ctx &c);
-hipError_t launch_extrapolate(const Ctx &c, const double *phi1, double *phi2, double *guess, int extrapolate);
+// first residual of a Poisson solve (see kernels_mg.hip), with its finalize
+hipError_t launch_poisson_start(const Ctx &c, int mode, int center, const double *phi1, const double *phi2,
+                                double *phi0, int mixed);
 hipError_t launch_fin_resid(const Ctx &c, int nblocks);
 hipError_t launch_vcycle(const Ctx &c, int mixed);
 hipError_t launch_vcycle_step(const Ctx &c, int mixed, int par, const double *phi_in, double *phi_out, int *nblocks);

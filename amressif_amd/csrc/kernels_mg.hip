@@ -191,6 +191,208 @@ __global__ __launch_bounds__(256) void k_smooth(const T *xin, const T *ec, int p
     }
 }
 
+// ------------------------------------------------- wave-column smoother
+// The same four red-black half-sweeps (and the same fused restriction / FP64
+// outer step per MODE) with no LDS and no barrier: one wavefront owns 64
+// adjacent columns, the OC = 64 - 2H output columns plus an H-cell halo on
+// each side, and NR = R + 2H rows of them in registers (R output rows plus the
+// halo).  x-neighbours come from the adjacent lanes by DPP, y-neighbours are
+// the registers of the rows above and below.  Each half-sweep leaves one more
+// ring of lanes and rows stale; it skips the rows that are stale already.
+// Cells outside the domain hold 0 and are never updated, so the Neumann walls
+// only change the diagonal: x = (ax (W + E) + ay (S + N) - r) / d with
+// d = cw + ce + cs + cn (ax or 0 per lane, ay or 0 per row).  The sweeps use the
+// right-hand side pre-scaled by 1/d and the coefficients ax/d, ay/d.
+// Waves are numbered chunk-major (adjacent waves take adjacent bands of one row
+// chunk) and workgroups are handed to the 8 XCDs in contiguous ranges, so the
+// halo lines two waves share are read through one XCD's L2.
+__device__ __forceinline__ double wave_max(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        const double w = __shfl_xor(v, o, 64);
+        v = (w > v || w != w) ? w : v;
+    }
+    return v;
+}
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// four red-black half-sweeps of a register window whose first row jb has the
+// parity of H (the window starts H rows below an even row): row q holds the
+// cells of colour c in the lanes of parity (q + H + c) & 1.  rs = r / d.
+// With YINT every updated row is an interior row of the domain.
+template <class T, int NR, int H, bool YINT>
+__device__ __forceinline__ void gsrb4(T (&x)[NR], const T (&rs)[NR], int jb, int ny, T a_mid, T b_mid, T a_edge,
+                                      T b_edge, bool upd_even, bool upd_odd)
+{
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+#pragma unroll
+        for (int q = 1 + h; q < NR - 1 - h; ++q) {
+            const int j = jb + q;
+            T a = a_mid, b = b_mid;
+            bool upd = ((q + H + h) & 1) ? upd_odd : upd_even;
+            if (!YINT) {
+                const bool edge = j == 0 || j == ny - 1;
+                a = edge ? a_edge : a_mid;
+                b = edge ? b_edge : b_mid;
+                upd = upd && j >= 0 && j < ny;
+            }
+            const T we = lane_prev(x[q]) + lane_next(x[q]);
+            const T sn = x[q - 1] + x[q + 1];
+            const T nv = a * we + (b * sn - rs[q]);
+            x[q] = upd ? nv : x[q];
+        }
+    }
+}
+
+template <class T, int MODE, int R>
+__global__ __launch_bounds__(256) void k_gsrb(const T *xin, const T *ec, int pc, T *xout, const T *rhs, int nx,
+                                              int ny, int pitch, T ax, T ay, T *rc, int ncx, int ncy, int pcc,
+                                              OuterArgs o, const Ctl *ctl, int nbands, int nwaves)
+{
+    constexpr int H = MODE == 0 ? 4 : 5;
+    constexpr int OC = 64 - 2 * H, NR = R + 2 * H;
+    const int lane = threadIdx.x & 63;
+    const int nblk = gridDim.x;
+    const int blk = (nblk & 7) == 0 ? (blockIdx.x & 7) * (nblk >> 3) + (blockIdx.x >> 3) : blockIdx.x;
+    const int w = blk * 4 + (threadIdx.x >> 6);
+    // a wave past the last one exits at once; in MODE 2 it stays for the
+    // workgroup's reduction (the only barrier) and touches no memory
+    const bool live = w < nwaves;
+    if (MODE != 2 && !live) return;
+    const int band = w % nbands, chunk = w / nbands;
+    const int i0 = band * OC, j0 = chunk * R, jb = j0 - H;  // jb: the row of register 0
+    const int i = i0 - H + lane;
+    const bool in_i = live && i >= 0 && i < nx;
+    // loads first (MODE 1 starts from zero); their latency overlaps the flag's.
+    // Running offsets (dereferenced only in range) keep one address live.
+    const bool yin = jb >= 0 && jb + NR <= ny;  // every window row is a domain row
+    T x[NR], r[NR];
+    {
+        int k = i + pitch * jb, kc = (i >> 1) + pc * (jb >> 1);
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+            const int j = jb + q;
+            T xv = 0, rv = 0;
+            if (in_i && (yin || (j >= 0 && j < ny))) {
+                rv = rhs[k];
+                if (MODE != 1) xv = xin[k] + ec[kc];
+            }
+            x[q] = xv;
+            r[q] = rv;
+            k += pitch;
+            if (((q + 1 + H) & 1) == 0) kc += pc;  // row jb + q + 1 is even: next coarse row
+        }
+    }
+    // MODE 2: the FP64 iterate on the output rows and one row either side, b on the output rows
+    constexpr int NP = MODE == 2 ? R + 2 : 1, NB = MODE == 2 ? R : 1;
+    double ph[NP], bb[NB];
+    if (MODE == 2) {
+        int k = i + pitch * (j0 - 1);
+#pragma unroll
+        for (int t = 0; t < NP; ++t) {
+            const int j = j0 - 1 + t;
+            ph[t] = (in_i && j >= 0 && j < ny) ? o.phi_in[k] : 0.0;
+            if (t >= 1 && t <= NB) bb[t - 1] = (in_i && j < ny) ? o.b[k] : 0.0;
+            k += pitch;
+        }
+    }
+    if (ctl->mg_done | ctl->gm_stop) return;
+    const T cw = i > 0 ? ax : T(0), ce = i < nx - 1 ? ax : T(0);
+    const T d_mid = cw + ce + T(2) * ay, d_edge = cw + ce + ay;  // interior rows, the two wall rows
+    const T inv_mid = T(1) / d_mid, inv_edge = T(1) / d_edge;
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {  // r -> r / d
+        const int j = jb + q;
+        r[q] *= (j > 0 && j < ny - 1) ? inv_mid : inv_edge;
+    }
+    const bool odd = (i & 1) != 0, upd_odd = in_i && odd, upd_even = in_i && !odd;
+    // most windows hold no wall row and no row outside the domain: their
+    // sweeps need no per-row coefficients
+    if (yin)
+        gsrb4<T, NR, H, true>(x, r, jb, ny, ax * inv_mid, ay * inv_mid, ax * inv_edge, ay * inv_edge, upd_even,
+                              upd_odd);
+    else
+        gsrb4<T, NR, H, false>(x, r, jb, ny, ax * inv_mid, ay * inv_mid, ax * inv_edge, ay * inv_edge, upd_even,
+                               upd_odd);
+    const bool out_lane = lane >= H && lane < H + OC && i < nx;
+    if (MODE != 2) {
+#pragma unroll
+        for (int q = H; q < H + R; ++q) {
+            const int j = jb + q;
+            if (out_lane && j < ny) xout[i + (size_t)pitch * j] = x[q];
+        }
+    }
+    if (MODE == 1) {  // residual of the output cells, 2x2 averages into the coarse right-hand side
+        auto res = [&](int q) {
+            const int j = jb + q;
+            const T P = x[q];
+            const T Wl = lane_prev(P), El = lane_next(P);
+            const T Wv = i > 0 ? Wl : P;
+            const T Ev = i < nx - 1 ? El : P;
+            const T Sv = j > 0 ? x[q - 1] : P;
+            const T Nv = j < ny - 1 ? x[q + 1] : P;
+            const T rq = r[q] * ((j > 0 && j < ny - 1) ? d_mid : d_edge);  // r / d * d
+            return rq - ((Ev + Wv - T(2) * P) * ax + (Nv + Sv - T(2) * P) * ay);
+        };
+#pragma unroll
+        for (int m = 0; m < R / 2; ++m) {
+            const int q = H + 2 * m, j = jb + q;
+            const T t00 = res(q), t01 = res(q + 1);
+            const T t10 = lane_next(t00), t11 = lane_next(t01);
+            const int I = i >> 1, J = j >> 1;
+            if (out_lane && !odd && I < ncx && J < ncy)
+                rc[I + (size_t)pcc * J] = T(0.25) * (((t00 + t10) + t01) + t11);
+        }
+    }
+    if (MODE == 2) {  // phi_out = phi_in + e, r = b - L phi_out (FP64), wave partials
+        double P[NP];
+#pragma unroll
+        for (int t = 0; t < NP; ++t) P[t] = ph[t] + (double)x[H - 1 + t];
+        double m = 0.0, sm = 0.0;
+#pragma unroll
+        for (int t = 1; t <= R; ++t) {
+            const int j = j0 - 1 + t;
+            const double Pv = P[t];
+            const double Wl = lane_prev(Pv), El = lane_next(Pv);
+            const double Wv = i > 0 ? Wl : Pv;
+            const double Ev = i < nx - 1 ? El : Pv;
+            const double Sv = j > 0 ? P[t - 1] : Pv;
+            const double Nv = j < ny - 1 ? P[t + 1] : Pv;
+            const double rr = bb[t - 1] - ((Ev + Wv - 2.0 * Pv) * o.ax + (Nv + Sv - 2.0 * Pv) * o.ay);
+            if (out_lane && j < ny) {
+                const size_t k = i + (size_t)pitch * j;
+                o.phi_out[k] = Pv;
+                xout[k] = (T)rr;  // the next residual
+                m = (fabs(rr) > m || rr != rr) ? fabs(rr) : m;
+                sm += Pv;
+            }
+        }
+        m = wave_max(m);
+        sm = wave_sum(sm);
+        __shared__ double shm[4], shs[4];
+        const int wv = threadIdx.x >> 6;
+        if (lane == 0) {
+            shm[wv] = m;
+            shs[wv] = sm;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {  // one partial per workgroup, combined in wave order
+            double mm = shm[0], ss = shs[0];
+            for (int q = 1; q < 4; ++q) {
+                mm = (shm[q] > mm || shm[q] != shm[q]) ? shm[q] : mm;
+                ss += shs[q];
+            }
+            o.partials[2 * (size_t)blockIdx.x] = mm;
+            o.partials[2 * (size_t)blockIdx.x + 1] = ss;
+        }
+    }
+}
+
 template <class T>
 __device__ __forceinline__ T lap_at(const T *e, int i, int j, int nx, int ny, int pitch, T ax, T ay)
 {
@@ -381,59 +583,90 @@ __global__ void k_mg_start(Ctl *ctl)
     ctl->phi_mean = 0.0;
 }
 
-// ||b||inf of the (centred) right-hand side, for a solve that starts from a
-// nonzero guess: the stopping test stays relative to ||b||inf
-__global__ __launch_bounds__(256) void k_absmax_partials(const double *b, int n, int P, double *partials)
+// Start of the Poisson iteration, one pass: b <- b - mean(b) in place (center),
+// the starting iterate phi0 = 0 (mode 0), phi1 (mode 1) or the extrapolation
+// 2 phi1 - phi2 (mode 2), r0 = b - L phi0 in FP64 (stored T), and block
+// partials of ||r0||inf, sum phi0 and ||b||inf (the stopping test stays
+// relative to ||b||inf whatever the start).
+template <class T>
+__global__ __launch_bounds__(256) void k_poisson_start(double *b, const double *phi1, const double *phi2,
+                                                       double *phi0, T *r, int n, int P, double ax, double ay,
+                                                       int mode, int center, const Ctl *ctl, double *partials)
 {
-    __shared__ double sh[16];
+    __shared__ double sh[24];
     const int i = blockIdx.x * 64 + (threadIdx.x & 63), j = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const double m = (i < n && j < n) ? fabs(b[i + (size_t)P * j]) : 0.0;
-    const double r = block_reduce<true>(m, sh);
-    if (threadIdx.x == 0) partials[blockIdx.x + (size_t)gridDim.x * blockIdx.y] = r;
-}
-
-__global__ __launch_bounds__(1024) void k_fin_b_norm(const double *partials, int nb, Ctl *ctl)
-{
-    __shared__ double sh[32];
-    double m = 0.0;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-        const double p = partials[b];
-        m = (p > m || p != p) ? p : m;
+    double mr = 0.0, sp = 0.0, mb = 0.0;
+    if (i < n && j < n) {
+        const size_t k = i + (size_t)P * j;
+        auto g = [&](size_t q) -> double {
+            return mode == 0 ? 0.0 : mode == 1 ? phi1[q] : 2.0 * phi1[q] - phi2[q];
+        };
+        const double bc = center ? b[k] - ctl->b_mean : b[k];
+        const double Pv = g(k);
+        const double Wv = i > 0 ? g(k - 1) : Pv;
+        const double Ev = i < n - 1 ? g(k + 1) : Pv;
+        const double Sv = j > 0 ? g(k - P) : Pv;
+        const double Nv = j < n - 1 ? g(k + P) : Pv;
+        const double rr = bc - ((Ev + Wv - 2.0 * Pv) * ax + (Nv + Sv - 2.0 * Pv) * ay);
+        if (center) b[k] = bc;
+        phi0[k] = Pv;
+        r[k] = (T)rr;
+        mr = fabs(rr);
+        sp = Pv;
+        mb = fabs(bc);
     }
-    m = block_reduce<true>(m, sh);
+    mr = block_reduce<true>(mr, sh);
+    sp = block_reduce<false>(sp, sh + 8);
+    mb = block_reduce<true>(mb, sh + 16);
     if (threadIdx.x == 0) {
-        ctl->b_norm = m;
-        ctl->b_norm_fixed = 1;
+        const size_t bid = blockIdx.x + (size_t)gridDim.x * blockIdx.y;
+        partials[3 * bid] = mr;
+        partials[3 * bid + 1] = sp;
+        partials[3 * bid + 2] = mb;
     }
 }
 
-hipError_t launch_b_norm(const Ctx &c)
+// the first residual's norm and ||b||inf, and the stopping test on it
+__global__ __launch_bounds__(1024) void k_fin_start(const double *partials, int nb, Ctl *ctl, int n)
+{
+    __shared__ double sh[48];
+    double mr = 0.0, sp = 0.0, mb = 0.0;
+    for (int q = threadIdx.x; q < nb; q += blockDim.x) {
+        const double a = partials[3 * q], c = partials[3 * q + 2];
+        mr = (a > mr || a != a) ? a : mr;
+        sp += partials[3 * q + 1];
+        mb = (c > mb || c != c) ? c : mb;
+    }
+    mr = block_reduce<true>(mr, sh);
+    sp = block_reduce<false>(sp, sh + 16);
+    mb = block_reduce<true>(mb, sh + 32);
+    if (threadIdx.x == 0) {
+        ctl->b_norm = mb;
+        ctl->b_norm_fixed = 1;
+        ctl->res_norm = mr;
+        ctl->phi_sum = sp;
+        ctl->phi_mean = sp / ((double)n * (double)n);
+        ctl->mg_started = 1;
+        const double thr = fmax(ctl->tol_abs, ctl->tol_rel * mb);
+        if (mr <= thr || !(mr == mr)) ctl->mg_done = 1;
+    }
+}
+
+hipError_t launch_poisson_start(const Ctx &c, int mode, int center, const double *phi1, const double *phi2,
+                                double *phi0, int mixed)
 {
     const int n = c.g.n;
     const dim3 grid((n + 63) / 64, (n + 3) / 4);
-    if ((int64_t)grid.x * grid.y > kMaxPartials) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_absmax_partials, grid, dim3(256), 0, c.stream, c.b, n, c.g.P, c.partials);
-    hipLaunchKernelGGL(k_fin_b_norm, dim3(1), dim3(1024), 0, c.stream, c.partials, (int)(grid.x * grid.y), c.ctl);
-    return hipGetLastError();
-}
-
-// warm-start guess: 2 phi1 - phi2 (phi1 alone without history), then phi2 <- phi1
-__global__ __launch_bounds__(256) void k_extrapolate(const double *phi1, double *phi2, double *guess, int n, int P,
-                                                     int extrapolate)
-{
-    const int i = blockIdx.x * 64 + (threadIdx.x & 63), j = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (i >= n || j >= n) return;
-    const size_t k = i + (size_t)P * j;
-    const double a = phi1[k];
-    guess[k] = extrapolate ? 2.0 * a - phi2[k] : a;
-    phi2[k] = a;
-}
-
-hipError_t launch_extrapolate(const Ctx &c, const double *phi1, double *phi2, double *guess, int extrapolate)
-{
-    const int n = c.g.n;
-    hipLaunchKernelGGL(k_extrapolate, dim3((n + 63) / 64, (n + 3) / 4), dim3(256), 0, c.stream, phi1, phi2, guess, n,
-                       c.g.P, extrapolate);
+    const int nb = (int)(grid.x * grid.y);
+    if (nb > kMaxPartials) return hipErrorInvalidValue;
+    const Level &L = c.lev[0];
+    if (mixed)
+        hipLaunchKernelGGL(k_poisson_start<float>, grid, dim3(256), 0, c.stream, c.b, phi1, phi2, phi0, L.r32, n,
+                           c.g.P, L.ax, L.ay, mode, center, (const Ctl *)c.ctl, c.partials);
+    else
+        hipLaunchKernelGGL(k_poisson_start<double>, grid, dim3(256), 0, c.stream, c.b, phi1, phi2, phi0, L.r, n,
+                           c.g.P, L.ax, L.ay, mode, center, (const Ctl *)c.ctl, c.partials);
+    hipLaunchKernelGGL(k_fin_start, dim3(1), dim3(1024), 0, c.stream, (const double *)c.partials, nb, c.ctl, n);
     return hipGetLastError();
 }
 
@@ -511,8 +744,68 @@ static TailLevels tail_levels(const Ctx &c, int lt, size_t *lds_bytes, size_t el
     return tl;
 }
 
-template <class T>
 static dim3 tile_grid(const Level &L) { return dim3((L.nx + SX - 1) / SX, (L.ny + SY - 1) / SY); }
+
+// launch shape of one smoother on level l: the LDS tile kernel (one partial per
+// workgroup in MODE 2) or the wave-column kernel (one per wave)
+struct SmoothShapeRt {
+    dim3 grid;
+    int nbands = 0, nwaves = 0, rows = 0, wave = 0;
+    int parts = 0;  // MODE 2 norm partials: one per workgroup
+};
+
+template <class T>
+static SmoothShapeRt smooth_shape(const Ctx &c, int l, int mode)
+{
+    SmoothShapeRt s;
+    const Level &L = c.lev[l];
+    s.wave = c.mg_smoother;
+    if (!s.wave) {
+        s.grid = tile_grid(L);
+        s.nwaves = s.parts = (int)(s.grid.x * s.grid.y);
+        return s;
+    }
+    const int H = mode == 0 ? 4 : 5, OC = 64 - 2 * H;
+    s.rows = l == 0 ? c.mg_rows[sizeof(T) == 4][mode] : c.mg_rows_coarse;
+    s.nbands = (L.nx + OC - 1) / OC;
+    s.nwaves = s.nbands * ((L.ny + s.rows - 1) / s.rows);
+    s.grid = dim3(((s.nwaves + 3) / 4 + 7) / 8 * 8);  // a multiple of 8: the XCD remap applies
+    s.parts = (int)s.grid.x;
+    return s;
+}
+
+template <class T, int MODE, int R>
+static void launch_gsrb(const Ctx &c, const SmoothShapeRt &s, const T *xin, const T *ec, int pc, T *xout,
+                        const T *rhs, const Level &L, T *rc, const Level *Cl, const OuterArgs &o)
+{
+    hipLaunchKernelGGL((k_gsrb<T, MODE, R>), s.grid, dim3(256), 0, c.stream, xin, ec, pc, xout, rhs, L.nx, L.ny,
+                       L.pitch, (T)L.ax, (T)L.ay, rc, Cl ? Cl->nx : 0, Cl ? Cl->ny : 0, Cl ? Cl->pitch : 0, o,
+                       (const Ctl *)c.ctl, s.nbands, s.nwaves);
+}
+
+// one smoother on level l (MODE 1: from zero with the fused restriction into
+// level l + 1; MODE 0: from xin + P(ec); MODE 2: MODE 0 plus the FP64 outer step)
+template <class T, int MODE>
+static void launch_smoother(const Ctx &c, int l, const T *xin, const T *ec, int pc, T *xout, const T *rhs,
+                            const OuterArgs &o)
+{
+    const Level &L = c.lev[l];
+    const Level *Cl = MODE == 1 ? &c.lev[l + 1] : nullptr;
+    T *rc = MODE == 1 ? lev_r<T>(*Cl) : (T *)nullptr;
+    const SmoothShapeRt s = smooth_shape<T>(c, l, MODE);
+    if (!s.wave) {
+        hipLaunchKernelGGL((k_smooth<T, MODE>), s.grid, dim3(256), 0, c.stream, xin, ec, pc, xout, rhs, L.nx, L.ny,
+                           L.pitch, (T)L.ax, (T)L.ay, rc, Cl ? Cl->nx : 0, Cl ? Cl->ny : 0, Cl ? Cl->pitch : 0, o,
+                           (const Ctl *)c.ctl);
+        return;
+    }
+    switch (s.rows) {
+    case 4: launch_gsrb<T, MODE, 4>(c, s, xin, ec, pc, xout, rhs, L, rc, Cl, o); break;
+    case 8: launch_gsrb<T, MODE, 8>(c, s, xin, ec, pc, xout, rhs, L, rc, Cl, o); break;
+    case 16: launch_gsrb<T, MODE, 16>(c, s, xin, ec, pc, xout, rhs, L, rc, Cl, o); break;
+    default: launch_gsrb<T, MODE, 32>(c, s, xin, ec, pc, xout, rhs, L, rc, Cl, o); break;
+    }
+}
 
 // One V-cycle on the correction from zero.  r0 is the level-0 right-hand side;
 // with fuse set (and level 0 smoothed by its own kernels) the last up-leg
@@ -524,12 +817,9 @@ static hipError_t vcycle_impl(const Ctx &c, const T *pinv, const T *r0, const Ou
     const Ctl *ctl = c.ctl;
     const int lt = c.tail_level;
     const OuterArgs none{};
-    for (int l = 0; l < lt; ++l) {  // down: smooth from zero, fused residual + restriction
-        const Level &L = c.lev[l], &Cl = c.lev[l + 1];
-        hipLaunchKernelGGL((k_smooth<T, 1>), tile_grid<T>(L), dim3(256), 0, c.stream, (const T *)nullptr,
-                           (const T *)nullptr, 0, lev_e<T>(L, 0), l == 0 ? r0 : (const T *)lev_r<T>(L), L.nx, L.ny,
-                           L.pitch, (T)L.ax, (T)L.ay, lev_r<T>(Cl), Cl.nx, Cl.ny, Cl.pitch, none, ctl);
-    }
+    for (int l = 0; l < lt; ++l)  // down: smooth from zero, fused residual + restriction
+        launch_smoother<T, 1>(c, l, (const T *)nullptr, (const T *)nullptr, 0, lev_e<T>(c.lev[l], 0),
+                              l == 0 ? r0 : (const T *)lev_r<T>(c.lev[l]), none);
     {
         size_t lds = 0;
         const TailLevels tl = tail_levels(c, lt, &lds, sizeof(T));
@@ -541,13 +831,11 @@ static hipError_t vcycle_impl(const Ctx &c, const T *pinv, const T *r0, const Ou
         const Level &L = c.lev[l], &Cl = c.lev[l + 1];
         const T *rl = l == 0 ? r0 : (const T *)lev_r<T>(L);
         if (l == 0 && fuse)
-            hipLaunchKernelGGL((k_smooth<T, 2>), tile_grid<T>(L), dim3(256), 0, c.stream, (const T *)lev_e<T>(L, 0),
-                               (const T *)lev_e<T>(Cl, 1), Cl.pitch, r0_next, rl, L.nx, L.ny, L.pitch, (T)L.ax,
-                               (T)L.ay, (T *)nullptr, 0, 0, 0, *fuse, ctl);
+            launch_smoother<T, 2>(c, l, (const T *)lev_e<T>(L, 0), (const T *)lev_e<T>(Cl, 1), Cl.pitch, r0_next, rl,
+                                  *fuse);
         else
-            hipLaunchKernelGGL((k_smooth<T, 0>), tile_grid<T>(L), dim3(256), 0, c.stream, (const T *)lev_e<T>(L, 0),
-                               (const T *)lev_e<T>(Cl, 1), Cl.pitch, lev_e<T>(L, 1), rl, L.nx, L.ny, L.pitch,
-                               (T)L.ax, (T)L.ay, (T *)nullptr, 0, 0, 0, none, ctl);
+            launch_smoother<T, 0>(c, l, (const T *)lev_e<T>(L, 0), (const T *)lev_e<T>(Cl, 1), Cl.pitch,
+                                  lev_e<T>(L, 1), rl, none);
     }
     return hipGetLastError();
 }
@@ -571,9 +859,9 @@ hipError_t launch_vcycle_step(const Ctx &c, int mixed, int par, const double *ph
         if (e != hipSuccess) return e;
         return launch_correct_residual(c, phi_in, phi_out, 1, mixed, nblocks);
     }
-    const dim3 g = tile_grid<double>(L0);
-    if ((int64_t)g.x * g.y > kMaxPartials) return hipErrorInvalidValue;
-    *nblocks = (int)(g.x * g.y);
+    const int parts = mixed ? smooth_shape<float>(c, 0, 2).parts : smooth_shape<double>(c, 0, 2).parts;
+    if (parts > kMaxPartials) return hipErrorInvalidValue;
+    *nblocks = parts;
     OuterArgs o{phi_in, phi_out, c.b, c.partials, L0.ax, L0.ay};
     if (mixed) {
         float *rb[2] = {L0.r32, L0.r32b};
@@ -652,16 +940,14 @@ template <class T>
 static hipError_t smooth_probe(const Ctx &c, int kind, T *r0b, double *scratch_phi)
 {
     const Level &L = c.lev[0], &Cl = c.lev[1];
-    const OuterArgs none{};
     if (kind == 1)
-        hipLaunchKernelGGL((k_smooth<T, 1>), tile_grid<T>(L), dim3(256), 0, c.stream, (const T *)nullptr,
-                           (const T *)nullptr, 0, lev_e<T>(L, 0), (const T *)lev_r<T>(L), L.nx, L.ny, L.pitch,
-                           (T)L.ax, (T)L.ay, lev_r<T>(Cl), Cl.nx, Cl.ny, Cl.pitch, none, (const Ctl *)c.ctl);
+        launch_smoother<T, 1>(c, 0, (const T *)nullptr, (const T *)nullptr, 0, lev_e<T>(L, 0),
+                              (const T *)lev_r<T>(L), OuterArgs{});
     else {
+        if (smooth_shape<T>(c, 0, 2).parts > kMaxPartials) return hipErrorInvalidValue;
         OuterArgs o{c.phi, scratch_phi, c.b, c.partials, L.ax, L.ay};
-        hipLaunchKernelGGL((k_smooth<T, 2>), tile_grid<T>(L), dim3(256), 0, c.stream, (const T *)lev_e<T>(L, 0),
-                           (const T *)lev_e<T>(Cl, 1), Cl.pitch, r0b, (const T *)lev_r<T>(L), L.nx, L.ny, L.pitch,
-                           (T)L.ax, (T)L.ay, (T *)nullptr, 0, 0, 0, o, (const Ctl *)c.ctl);
+        launch_smoother<T, 2>(c, 0, (const T *)lev_e<T>(L, 0), (const T *)lev_e<T>(Cl, 1), Cl.pitch, r0b,
+                              (const T *)lev_r<T>(L), o);
     }
     return hipGetLastError();
 }

@@ -74,7 +74,7 @@ typedef struct {
     double poisson_tol_rel;     /* 1e-10 (poisson.cpp:181) */
     double poisson_tol_abs;     /* 0 (poisson.cpp:182) */
     int mg_max_iter;            /* 2000 (poisson.cpp:159) */
-    int mg_bottom_max;          /* coarsen while the level is larger than this and even; 16 */
+    int mg_bottom_max;          /* coarsen while the level is larger than this and even; 4 */
     /* this build */
     int precision;              /* AMRESSIF_FP64 or AMRESSIF_MIXED */
     int poisson_solver;         /* 0: multigrid V-cycle iteration (the MLMG solve, default);

@@ -1,0 +1,84 @@
+"""GPU checks of the multigrid smoothers.
+
+The wave-column smoother (the column window in registers, x-neighbours by DPP
+lane shifts) and the LDS tile smoother run the same red-black Gauss-Seidel
+half-sweeps, the same fused residual + restriction and the same fused FP64
+outer step.  So after a fixed number of V-cycles a Poisson solve with either
+has the same residual and the same iterate up to rounding, for every window
+height (output rows per wave).  Sizes cover partial last bands and row chunks
+(100, 200 and 400 are not multiples of the band widths 54 and 56, and 100 and
+200 not of every row chunk) as well as the 2048^2 workload's hierarchy.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from test_gpu_parity import _api, cuda, rel
+
+pytestmark = pytest.mark.gpu
+
+ENV_KEYS = ("AMRESSIF_MG_SMOOTHER", "AMRESSIF_MG_ROWS32", "AMRESSIF_MG_ROWS64", "AMRESSIF_MG_ROWS_COARSE")
+
+
+def _solve(n, precision, b, cycles, **env):
+    """cycles V-cycles of the step's Poisson iteration from zero (tolerance 0,
+    cycle cap), with the smoother settings in env; returns (phi, ||r||/||b||)"""
+    api = _api()
+    saved = {k: os.environ.get(k) for k in ENV_KEYS}
+    for k in ENV_KEYS:
+        os.environ.pop(k, None)
+    os.environ.update(env)
+    try:
+        ctx = api.Context(api.cavity_inputs(n, 100.0, precision=precision, mg_max_iter=cycles))
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    sol, it, res = ctx.poisson_advance(cuda(b), tol_rel=0.0)
+    assert it == cycles
+    return sol.cpu().numpy()[1:-1, 1:-1], res
+
+
+def _rhs(n, seed):
+    rng = np.random.default_rng(seed)
+    b = np.zeros((n + 2, n + 2))
+    b[1:-1, 1:-1] = rng.standard_normal((n, n))
+    return b
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("n", [100, 200, 400, 1024])
+def test_wave_column_smoother_matches_tile_smoother(n, precision):
+    b = _rhs(n, 11 + n)
+    ref, res_ref = _solve(n, precision, b, 3, AMRESSIF_MG_SMOOTHER="0")
+    got, res = _solve(n, precision, b, 3, AMRESSIF_MG_SMOOTHER="1")
+    # three V-cycles reduce the residual by orders of magnitude; the two
+    # kernels differ only in rounding: ~1e-16 (FP64) or ~1e-7 (FP32 V-cycle)
+    # of the correction per cycle, and ||r|| / ||b|| by as much
+    assert res_ref < 1e-2
+    tol = 1e-10 if precision == 0 else 1e-5
+    assert abs(res - res_ref) <= (1e-12 if precision == 0 else 1e-5), (res, res_ref)
+    assert rel(got, ref) < tol
+
+
+@pytest.mark.parametrize("rows", ["4,4,4", "8,8,8", "16,16,16", "32,32,8", "8,32,16"])
+def test_wave_column_window_heights_agree(rows):
+    n = 200
+    b = _rhs(n, 5)
+    ref, res_ref = _solve(n, 0, b, 3, AMRESSIF_MG_SMOOTHER="0")
+    for coarse in ("4", "8", "32"):
+        got, res = _solve(n, 0, b, 3, AMRESSIF_MG_SMOOTHER="1", AMRESSIF_MG_ROWS64=rows,
+                          AMRESSIF_MG_ROWS_COARSE=coarse)
+        assert abs(res - res_ref) <= 1e-12, (rows, coarse)
+        assert rel(got, ref) < 1e-10, (rows, coarse)
+
+
+def test_wave_column_smoother_converges_at_2048():
+    n = 2048
+    b = _rhs(n, 3)
+    for precision in (0, 1):
+        _, res = _solve(n, precision, b, 6, AMRESSIF_MG_SMOOTHER="1")
+        assert res < 1e-6, (precision, res)

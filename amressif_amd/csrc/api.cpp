@@ -229,7 +229,7 @@ int gmres_loop(amressif_ctx *ctx)
 
 // ---- the Poisson solve on c.phi / c.b, shared by the step and the module
 // entry point
-int mg_loop(amressif_ctx *ctx, bool warm, bool center);
+int mg_loop(amressif_ctx *ctx, bool warm, bool center, const double *us, const double *vs);
 // warm: start from the linear extrapolation 2 phi^{n-1} - phi^{n-2} of the
 // previous steps' solutions (the previous solution alone right after a state
 // load); the stopping test stays relative to ||b||inf.  cold: from zero, as
@@ -240,10 +240,25 @@ int poisson_loop(amressif_ctx *ctx, bool warm = false, bool center = false)
         if (center) HIPCHK(amr::launch_center_b(ctx->c, ctx->c.b));
         return gmres_loop(ctx);
     }
-    return mg_loop(ctx, warm, center);
+    return mg_loop(ctx, warm, center, nullptr, nullptr);
 }
 
-int mg_loop(amressif_ctx *ctx, bool warm, bool center)
+// the step's projection solve: right-hand side (1.5/dt) D(u*) of the
+// predictor faces us, vs (fn_rhs.cpp:71-125), centred, then the solve; the
+// multigrid iteration forms the right-hand side inside its first pass
+int poisson_loop_step(amressif_ctx *ctx, bool warm, const double *us, const double *vs)
+{
+    Ctx &c = ctx->c;
+    if (c.in.poisson_solver == 1) {
+        int nb = 0;
+        HIPCHK(amr::launch_poisson_rhs(c, us, vs, c.b, &nb));
+        HIPCHK(amr::launch_fin_prhs(c, nb));
+        return poisson_loop(ctx, warm, true);
+    }
+    return mg_loop(ctx, warm, false, us, vs);
+}
+
+int mg_loop(amressif_ctx *ctx, bool warm, bool center, const double *us, const double *vs)
 {
     Ctx &c = ctx->c;
     const int mixed = c.in.precision == AMRESSIF_MIXED;
@@ -265,7 +280,7 @@ int mg_loop(amressif_ctx *ctx, bool warm, bool center)
         c.phih = X;
         c.phih_valid = 1;
     }
-    HIPCHK(amr::launch_poisson_start(c, mode, center ? 1 : 0, phi1, phi2, c.phiv[0], mixed));  // + its finalize
+    HIPCHK(amr::launch_poisson_start(c, mode, center ? 1 : 0, phi1, phi2, c.phiv[0], mixed, us, vs));  // + finalize
     int launched = 0, target = std::max(1, c.last_V);
     const int cap = std::max(1, c.in.mg_max_iter);
     for (;;) {
@@ -792,9 +807,7 @@ int advance_impl(amressif_ctx *ctx, int nsteps, amressif_step_stats *stats, amre
             tape->count += 1;
         }
         // ---- projection: Poisson RHS, MLMG-equivalent solve, update
-        HIPCHK(amr::launch_poisson_rhs(c, c.U[0], c.U[1], c.b, &nb));
-        HIPCHK(amr::launch_fin_prhs(c, nb));
-        if (int rc = poisson_loop(ctx, warm, true)) return rc;  // centres b
+        if (int rc = poisson_loop_step(ctx, warm, c.U[0], c.U[1])) return rc;
         HIPCHK(amr::launch_project(c, c.U[0], c.U[1], c.S1[0], c.S1[1], &nb));
         HIPCHK(amr::launch_fin_project(c, nb, s));
         std::swap(c.U[0], c.S1[0]);

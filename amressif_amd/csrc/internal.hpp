@@ -171,7 +171,7 @@ hipError_t launch_center_b(const Ctx &c, double *b);
 hipError_t launch_mg_start(const Ctx &c);
 // first residual of a Poisson solve (see kernels_mg.hip), with its finalize
 hipError_t launch_poisson_start(const Ctx &c, int mode, int center, const double *phi1, const double *phi2,
-                                double *phi0, int mixed);
+                                double *phi0, int mixed, const double *us = nullptr, const double *vs = nullptr);
 hipError_t launch_fin_resid(const Ctx &c, int nblocks);
 hipError_t launch_vcycle(const Ctx &c, int mixed);
 hipError_t launch_vcycle_step(const Ctx &c, int mixed, int par, const double *phi_in, double *phi_out, int *nblocks);

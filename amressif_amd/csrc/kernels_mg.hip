@@ -583,32 +583,49 @@ __global__ void k_mg_start(Ctl *ctl)
     ctl->phi_mean = 0.0;
 }
 
-// Start of the Poisson iteration, one pass: b <- b - mean(b) in place (center),
-// the starting iterate phi0 = 0 (mode 0), phi1 (mode 1) or the extrapolation
-// 2 phi1 - phi2 (mode 2), r0 = b - L phi0 in FP64 (stored T), and block
-// partials of ||r0||inf, sum phi0 and ||b||inf (the stopping test stays
-// relative to ||b||inf whatever the start).
-template <class T>
-__global__ __launch_bounds__(256) void k_poisson_start(double *b, const double *phi1, const double *phi2,
+// Start of the Poisson iteration, one pass.  The right-hand side is either b
+// (b <- b - mean(b) in place when center is set) or, with FROM_U, formed here
+// from the predictor faces u*, v* as b = (1.5/dt) D(u*) - mean (the step's
+// poisson_righthand_side_calc, fn_rhs.cpp:71-125, with the mean from
+// k_flux_mean) and stored.  Then the starting iterate phi0 = 0 (mode 0), phi1
+// (mode 1) or the extrapolation 2 phi1 - phi2 (mode 2) of the two previous
+// solutions (newest first), r0 = b - L phi0 in FP64 (stored T), and block
+// partials of ||r0||inf, sum phi0, ||b||inf and, with FROM_U, ||D(u*)||inf and
+// sum (1.5/dt) D(u*) (the stopping test stays relative to ||b||inf whatever
+// the start).
+constexpr int kStartParts = 5;
+
+template <class T, bool FROM_U>
+__global__ __launch_bounds__(256) void k_poisson_start(double *b, const double *u, const double *v, double dx,
+                                                       double dy, double dt, const double *phi1, const double *phi2,
                                                        double *phi0, T *r, int n, int P, double ax, double ay,
                                                        int mode, int center, const Ctl *ctl, double *partials)
 {
-    __shared__ double sh[24];
+    __shared__ double sh[40];
     const int i = blockIdx.x * 64 + (threadIdx.x & 63), j = blockIdx.y * 4 + (threadIdx.x >> 6);
-    double mr = 0.0, sp = 0.0, mb = 0.0;
+    double mr = 0.0, sp = 0.0, mb = 0.0, md = 0.0, sb = 0.0;
     if (i < n && j < n) {
         const size_t k = i + (size_t)P * j;
         auto g = [&](size_t q) -> double {
             return mode == 0 ? 0.0 : mode == 1 ? phi1[q] : 2.0 * phi1[q] - phi2[q];
         };
-        const double bc = center ? b[k] - ctl->b_mean : b[k];
+        double bc;
+        if (FROM_U) {
+            const double d = (u[k + 1] - u[k]) / dx + (v[k + P] - v[k]) / dy;
+            const double braw = (1.5 / dt) * d;
+            bc = braw - ctl->b_mean;
+            md = fabs(d);
+            sb = braw;
+        } else {
+            bc = center ? b[k] - ctl->b_mean : b[k];
+        }
         const double Pv = g(k);
         const double Wv = i > 0 ? g(k - 1) : Pv;
         const double Ev = i < n - 1 ? g(k + 1) : Pv;
         const double Sv = j > 0 ? g(k - P) : Pv;
         const double Nv = j < n - 1 ? g(k + P) : Pv;
         const double rr = bc - ((Ev + Wv - 2.0 * Pv) * ax + (Nv + Sv - 2.0 * Pv) * ay);
-        if (center) b[k] = bc;
+        if (FROM_U || center) b[k] = bc;
         phi0[k] = Pv;
         r[k] = (T)rr;
         mr = fabs(rr);
@@ -618,29 +635,60 @@ __global__ __launch_bounds__(256) void k_poisson_start(double *b, const double *
     mr = block_reduce<true>(mr, sh);
     sp = block_reduce<false>(sp, sh + 8);
     mb = block_reduce<true>(mb, sh + 16);
+    if (FROM_U) {
+        md = block_reduce<true>(md, sh + 24);
+        sb = block_reduce<false>(sb, sh + 32);
+    }
     if (threadIdx.x == 0) {
-        const size_t bid = blockIdx.x + (size_t)gridDim.x * blockIdx.y;
-        partials[3 * bid] = mr;
-        partials[3 * bid + 1] = sp;
-        partials[3 * bid + 2] = mb;
+        double *o = partials + (size_t)kStartParts * (blockIdx.x + (size_t)gridDim.x * blockIdx.y);
+        o[0] = mr;
+        o[1] = sp;
+        o[2] = mb;
+        o[3] = md;
+        o[4] = sb;
     }
 }
 
-// the first residual's norm and ||b||inf, and the stopping test on it
-__global__ __launch_bounds__(1024) void k_fin_start(const double *partials, int nb, Ctl *ctl, int n)
+// (1.5/dt) times the mean cell divergence of u, v from the boundary faces alone:
+// the cell divergences telescope to the net flux through the walls
+__global__ __launch_bounds__(1024) void k_flux_mean(const double *u, const double *v, int n, int P, double dx,
+                                                    double dy, double dt, Ctl *ctl)
 {
-    __shared__ double sh[48];
-    double mr = 0.0, sp = 0.0, mb = 0.0;
+    __shared__ double sh[16];
+    double s = 0.0;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        const size_t row = (size_t)P * j;
+        s += (u[row + n] - u[row]) / dx;
+    }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += (v[(size_t)P * n + i] - v[i]) / dy;
+    s = block_reduce<false>(s, sh);
+    if (threadIdx.x == 0) ctl->b_mean = (1.5 / dt) * s / ((double)n * (double)n);
+}
+
+// the first residual's norm and ||b||inf (and with from_u the divergence
+// statistics of u*), and the stopping test on it
+__global__ __launch_bounds__(1024) void k_fin_start(const double *partials, int nb, Ctl *ctl, int n, int from_u)
+{
+    __shared__ double sh[80];
+    double mr = 0.0, sp = 0.0, mb = 0.0, md = 0.0, sb = 0.0;
     for (int q = threadIdx.x; q < nb; q += blockDim.x) {
-        const double a = partials[3 * q], c = partials[3 * q + 2];
-        mr = (a > mr || a != a) ? a : mr;
-        sp += partials[3 * q + 1];
-        mb = (c > mb || c != c) ? c : mb;
+        const double *o = partials + (size_t)kStartParts * q;
+        mr = (o[0] > mr || o[0] != o[0]) ? o[0] : mr;
+        sp += o[1];
+        mb = (o[2] > mb || o[2] != o[2]) ? o[2] : mb;
+        md = (o[3] > md || o[3] != o[3]) ? o[3] : md;
+        sb += o[4];
     }
     mr = block_reduce<true>(mr, sh);
     sp = block_reduce<false>(sp, sh + 16);
     mb = block_reduce<true>(mb, sh + 32);
+    md = block_reduce<true>(md, sh + 48);
+    sb = block_reduce<false>(sb, sh + 64);
     if (threadIdx.x == 0) {
+        if (from_u) {
+            ctl->div_star = md;
+            ctl->rhs_sum = sb;
+        }
         ctl->b_norm = mb;
         ctl->b_norm_fixed = 1;
         ctl->res_norm = mr;
@@ -653,20 +701,28 @@ __global__ __launch_bounds__(1024) void k_fin_start(const double *partials, int 
 }
 
 hipError_t launch_poisson_start(const Ctx &c, int mode, int center, const double *phi1, const double *phi2,
-                                double *phi0, int mixed)
+                                double *phi0, int mixed, const double *us, const double *vs)
 {
     const int n = c.g.n;
     const dim3 grid((n + 63) / 64, (n + 3) / 4);
     const int nb = (int)(grid.x * grid.y);
-    if (nb > kMaxPartials) return hipErrorInvalidValue;
+    if ((int64_t)nb * kStartParts > (int64_t)kMaxPartials * 4) return hipErrorInvalidValue;
     const Level &L = c.lev[0];
-    if (mixed)
-        hipLaunchKernelGGL(k_poisson_start<float>, grid, dim3(256), 0, c.stream, c.b, phi1, phi2, phi0, L.r32, n,
-                           c.g.P, L.ax, L.ay, mode, center, (const Ctl *)c.ctl, c.partials);
-    else
-        hipLaunchKernelGGL(k_poisson_start<double>, grid, dim3(256), 0, c.stream, c.b, phi1, phi2, phi0, L.r, n,
-                           c.g.P, L.ax, L.ay, mode, center, (const Ctl *)c.ctl, c.partials);
-    hipLaunchKernelGGL(k_fin_start, dim3(1), dim3(1024), 0, c.stream, (const double *)c.partials, nb, c.ctl, n);
+    const double dx = c.g.dx, dy = c.g.dy, dt = c.g.dt;
+    if (us) hipLaunchKernelGGL(k_flux_mean, dim3(1), dim3(1024), 0, c.stream, us, vs, n, c.g.P, dx, dy, dt, c.ctl);
+#define AMR_START(T, FU, RR)                                                                                         \
+    hipLaunchKernelGGL((k_poisson_start<T, FU>), grid, dim3(256), 0, c.stream, c.b, us, vs, dx, dy, dt, phi1, phi2,   \
+                       phi0, RR, n, c.g.P, L.ax, L.ay, mode, center, (const Ctl *)c.ctl, c.partials)
+    if (mixed) {
+        if (us) AMR_START(float, true, L.r32);
+        else AMR_START(float, false, L.r32);
+    } else {
+        if (us) AMR_START(double, true, L.r);
+        else AMR_START(double, false, L.r);
+    }
+#undef AMR_START
+    hipLaunchKernelGGL(k_fin_start, dim3(1), dim3(1024), 0, c.stream, (const double *)c.partials, nb, c.ctl, n,
+                       us ? 1 : 0);
     return hipGetLastError();
 }
 

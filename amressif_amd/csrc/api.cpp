@@ -545,7 +545,7 @@ int amressif_create(const amressif_inputs *in, void *hip_stream, amressif_ctx **
         c.tangent_rows = std::max(4, std::min(256, std::atoi(e)));
     if (const char *e = std::getenv("AMRESSIF_STAGE_ORDER")) c.stage_order = std::atoi(e) ? 1 : 0;
     // and of the multigrid smoothers (kernel choice and rows per wave: 4, 8, 16 or 32)
-    if (const char *e = std::getenv("AMRESSIF_MG_SMOOTHER")) c.mg_smoother = std::atoi(e) ? 1 : 0;
+    if (const char *e = std::getenv("AMRESSIF_MG_SMOOTHER")) c.mg_smoother = std::max(0, std::min(2, std::atoi(e)));
     auto rows_ok = [](int r) { return r == 4 || r == 8 || r == 16 || r == 32; };
     for (int t = 0; t < 2; ++t)
         if (const char *e = std::getenv(t ? "AMRESSIF_MG_ROWS32" : "AMRESSIF_MG_ROWS64")) {

@@ -101,11 +101,12 @@ struct Ctx {
     int stage_rows = 8;      // k_stage64: output rows per wave
     int tangent_rows = 16;   // k_tangent: output rows per wave
     int stage_order = 1;     // stage kernels: 1 = consecutive waves take horizontally adjacent bands
-    // multigrid smoothers of the levels above the tail: 1 = wave-column kernel
-    // (registers + DPP), 0 = LDS tile kernel; output rows per wave of the
-    // wave-column kernel by [FP64, FP32][up leg, down leg, finest up leg], and
-    // on the levels below the finest
-    int mg_smoother = 1;
+    // multigrid smoothers of the levels above the tail: 2 = wave-column kernel
+    // with two columns per lane, 1 = one column per lane (both registers +
+    // DPP), 0 = LDS tile kernel; output rows per wave of the wave-column
+    // kernels by [FP64, FP32][up leg, down leg, finest up leg], and on the
+    // levels below the finest
+    int mg_smoother = 2;
     int mg_rows[2][3] = {{8, 8, 8}, {8, 8, 4}};
     int mg_rows_coarse = 4;
     // FGMRES Poisson solver (poisson_solver = 1)

@@ -24,6 +24,7 @@
 #include "device_common.hpp"
 #include "internal.hpp"
 
+#include <algorithm>
 #include <cmath>
 
 namespace amr {
@@ -370,6 +371,236 @@ __global__ __launch_bounds__(256) void k_gsrb(const T *xin, const T *ec, int pc,
                 xout[k] = (T)rr;  // the next residual
                 m = (fabs(rr) > m || rr != rr) ? fabs(rr) : m;
                 sm += Pv;
+            }
+        }
+        m = wave_max(m);
+        sm = wave_sum(sm);
+        __shared__ double shm[4], shs[4];
+        const int wv = threadIdx.x >> 6;
+        if (lane == 0) {
+            shm[wv] = m;
+            shs[wv] = sm;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {  // one partial per workgroup, combined in wave order
+            double mm = shm[0], ss = shs[0];
+            for (int q = 1; q < 4; ++q) {
+                mm = (shm[q] > mm || shm[q] != shm[q]) ? shm[q] : mm;
+                ss += shs[q];
+            }
+            o.partials[2 * (size_t)blockIdx.x] = mm;
+            o.partials[2 * (size_t)blockIdx.x + 1] = ss;
+        }
+    }
+}
+
+// ------------------------------------------ paired-column wave smoother
+// The wave-column smoother with two adjacent columns per lane: lane l holds
+// the even column c = c0 + 2l (x0) and the odd column c + 1 (x1) of 128
+// columns (HX halo columns either side, HX even).  In a row, one colour lies in
+// the x0 cells and the other in the x1 cells, so each half-sweep updates one
+// fixed register per row in every lane (no select, no idle lane): an x0 cell
+// reads its west neighbour from lane l - 1's x1 (DPP) and its east one from
+// the lane's own x1, an x1 cell the reverse.  Cells outside the domain hold 0
+// and get zero coefficients, so they stay 0.  The 2x2 restriction of the down
+// leg is lane-local (a coarse cell is one lane's two columns in two rows).
+template <class T>
+struct Pair;
+template <>
+struct Pair<float> {
+    using V = float2;
+};
+template <>
+struct Pair<double> {
+    using V = double2;
+};
+
+template <class T, int NR, int H, bool YINT>
+__device__ __forceinline__ void gsrb4_pair(T (&x0)[NR], T (&x1)[NR], const T (&rs0)[NR], const T (&rs1)[NR], int jb,
+                                           int ny, const T (&A)[2][2], const T (&B)[2][2], bool in_i)
+{
+    // A[cell][edge], B[cell][edge]: ax/d and ay/d of the even (0) and odd (1)
+    // cell on an interior (0) or wall (1) row; zero outside the domain
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+#pragma unroll
+        for (int q = 1 + h; q < NR - 1 - h; ++q) {
+            const int j = jb + q;
+            // the even cell of row q has colour (q + H) & 1: it is updated when that is h's colour
+            const bool even = ((q + H + h) & 1) == 0;
+            const int cell = even ? 0 : 1;
+            T a = A[cell][0], b = B[cell][0];
+            if (!YINT) {
+                const bool edge = j == 0 || j == ny - 1, rin = j >= 0 && j < ny;
+                a = rin ? (edge ? A[cell][1] : A[cell][0]) : T(0);
+                b = rin ? (edge ? B[cell][1] : B[cell][0]) : T(0);
+            }
+            if (even) {
+                const T we = lane_prev(x1[q]) + x1[q];
+                const T sn = x0[q - 1] + x0[q + 1];
+                x0[q] = a * we + (b * sn - rs0[q]);
+            } else {
+                const T we = x0[q] + lane_next(x0[q]);
+                const T sn = x1[q - 1] + x1[q + 1];
+                x1[q] = a * we + (b * sn - rs1[q]);
+            }
+        }
+    }
+    (void)in_i;
+}
+
+template <class T, int MODE, int R>
+__global__ __launch_bounds__(256) void k_gsrb2(const T *xin, const T *ec, int pc, T *xout, const T *rhs, int nx,
+                                               int ny, int pitch, T ax, T ay, T *rc, int ncx, int ncy, int pcc,
+                                               OuterArgs o, const Ctl *ctl, int nbands, int nwaves)
+{
+    using V = typename Pair<T>::V;
+    constexpr int H = MODE == 0 ? 4 : 5, HX = MODE == 0 ? 4 : 6;
+    constexpr int OC = 128 - 2 * HX, NR = R + 2 * H;
+    const int lane = threadIdx.x & 63;
+    const int nblk = gridDim.x;
+    const int blk = (nblk & 7) == 0 ? (blockIdx.x & 7) * (nblk >> 3) + (blockIdx.x >> 3) : blockIdx.x;
+    const int w = blk * 4 + (threadIdx.x >> 6);
+    // a wave past the last one exits at once; in MODE 2 it stays for the
+    // workgroup's reduction (the only barrier) and touches no memory
+    const bool live = w < nwaves;
+    if (MODE != 2 && !live) return;
+    const int band = w % nbands, chunk = w / nbands;
+    const int i0 = band * OC, j0 = chunk * R, jb = j0 - H;
+    const int ca = i0 - HX + 2 * lane;  // the lane's even column; nx is even, so both or neither are inside
+    const bool in_i = live && ca >= 0 && ca < nx;
+    const bool yin = jb >= 0 && jb + NR <= ny;
+    T x0[NR], x1[NR], r0[NR], r1[NR];
+    {
+        int k = ca + pitch * jb, kc = (ca >> 1) + pc * (jb >> 1);
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+            const int j = jb + q;
+            V xv = {0, 0}, rv = {0, 0};
+            if (in_i && (yin || (j >= 0 && j < ny))) {
+                rv = *reinterpret_cast<const V *>(rhs + k);
+                if (MODE != 1) {
+                    xv = *reinterpret_cast<const V *>(xin + k);
+                    const T e = ec[kc];
+                    xv.x += e;
+                    xv.y += e;
+                }
+            }
+            x0[q] = xv.x;
+            x1[q] = xv.y;
+            r0[q] = rv.x;
+            r1[q] = rv.y;
+            k += pitch;
+            if (((q + 1 + H) & 1) == 0) kc += pc;  // row jb + q + 1 is even: next coarse row
+        }
+    }
+    constexpr int NP = MODE == 2 ? R + 2 : 1, NB = MODE == 2 ? R : 1;
+    double ph0[NP], ph1[NP], bb0[NB], bb1[NB];
+    if (MODE == 2) {
+        int k = ca + pitch * (j0 - 1);
+#pragma unroll
+        for (int t = 0; t < NP; ++t) {
+            const int j = j0 - 1 + t;
+            double2 pv = {0.0, 0.0};
+            if (in_i && j >= 0 && j < ny) pv = *reinterpret_cast<const double2 *>(o.phi_in + k);
+            ph0[t] = pv.x;
+            ph1[t] = pv.y;
+            if (t >= 1 && t <= NB) {
+                double2 bv = {0.0, 0.0};
+                if (in_i && j < ny) bv = *reinterpret_cast<const double2 *>(o.b + k);
+                bb0[t - 1] = bv.x;
+                bb1[t - 1] = bv.y;
+            }
+            k += pitch;
+        }
+    }
+    if (ctl->mg_done | ctl->gm_stop) return;
+    // per cell: d = cw + ce + cs + cn; the even cell has a west wall at column 0,
+    // the odd one an east wall at column nx - 1
+    const T cw0 = ca > 0 ? ax : T(0), ce1 = ca + 1 < nx - 1 ? ax : T(0);
+    const T d0m = cw0 + ax + T(2) * ay, d0e = cw0 + ax + ay, d1m = ax + ce1 + T(2) * ay, d1e = ax + ce1 + ay;
+    const T i0m = T(1) / d0m, i0e = T(1) / d0e, i1m = T(1) / d1m, i1e = T(1) / d1e;
+    T A[2][2], B[2][2];
+    A[0][0] = in_i ? ax * i0m : T(0);
+    A[0][1] = in_i ? ax * i0e : T(0);
+    A[1][0] = in_i ? ax * i1m : T(0);
+    A[1][1] = in_i ? ax * i1e : T(0);
+    B[0][0] = in_i ? ay * i0m : T(0);
+    B[0][1] = in_i ? ay * i0e : T(0);
+    B[1][0] = in_i ? ay * i1m : T(0);
+    B[1][1] = in_i ? ay * i1e : T(0);
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {  // r -> r / d
+        const int j = jb + q;
+        const bool mid = j > 0 && j < ny - 1;
+        r0[q] *= mid ? i0m : i0e;
+        r1[q] *= mid ? i1m : i1e;
+    }
+    if (yin)
+        gsrb4_pair<T, NR, H, true>(x0, x1, r0, r1, jb, ny, A, B, in_i);
+    else
+        gsrb4_pair<T, NR, H, false>(x0, x1, r0, r1, jb, ny, A, B, in_i);
+    const bool out_lane = in_i && lane >= HX / 2 && lane < HX / 2 + OC / 2;
+    if (MODE != 2) {
+#pragma unroll
+        for (int q = H; q < H + R; ++q) {
+            const int j = jb + q;
+            if (out_lane && j < ny) *reinterpret_cast<V *>(xout + ca + (size_t)pitch * j) = V{x0[q], x1[q]};
+        }
+    }
+    if (MODE == 1) {  // residual of the output cells, 2x2 averages (lane-local) into the coarse right-hand side
+        auto res = [&](int q, T &t0, T &t1) {
+            const int j = jb + q;
+            const bool mid = j > 0 && j < ny - 1;
+            const T P0 = x0[q], P1 = x1[q];
+            // the lane shifts run in every lane (a shift inside a branch would
+            // read lanes that the branch switched off)
+            const T Wl = lane_prev(P1), El = lane_next(P0);
+            const T W0 = ca > 0 ? Wl : P0;          // west of the even cell
+            const T E1 = ca + 1 < nx - 1 ? El : P1;  // east of the odd cell
+            const T S0 = j > 0 ? x0[q - 1] : P0, N0 = j < ny - 1 ? x0[q + 1] : P0;
+            const T S1 = j > 0 ? x1[q - 1] : P1, N1 = j < ny - 1 ? x1[q + 1] : P1;
+            const T rq0 = r0[q] * (mid ? d0m : d0e), rq1 = r1[q] * (mid ? d1m : d1e);
+            t0 = rq0 - ((P1 + W0 - T(2) * P0) * ax + (N0 + S0 - T(2) * P0) * ay);
+            t1 = rq1 - ((E1 + P0 - T(2) * P1) * ax + (N1 + S1 - T(2) * P1) * ay);
+        };
+#pragma unroll
+        for (int m = 0; m < R / 2; ++m) {
+            const int q = H + 2 * m, j = jb + q;
+            T t00, t10, t01, t11;
+            res(q, t00, t10);
+            res(q + 1, t01, t11);
+            const int I = ca >> 1, J = j >> 1;
+            if (out_lane && I < ncx && J < ncy) rc[I + (size_t)pcc * J] = T(0.25) * (((t00 + t10) + t01) + t11);
+        }
+    }
+    if (MODE == 2) {  // phi_out = phi_in + e, r = b - L phi_out (FP64), one partial per workgroup
+        double P0[NP], P1[NP];
+#pragma unroll
+        for (int t = 0; t < NP; ++t) {
+            P0[t] = ph0[t] + (double)x0[H - 1 + t];
+            P1[t] = ph1[t] + (double)x1[H - 1 + t];
+        }
+        double m = 0.0, sm = 0.0;
+#pragma unroll
+        for (int t = 1; t <= R; ++t) {
+            const int j = j0 - 1 + t;
+            const double p0 = P0[t], p1 = P1[t];
+            const double Wl = lane_prev(p1), El = lane_next(p0);
+            const double W0 = ca > 0 ? Wl : p0;
+            const double E1 = ca + 1 < nx - 1 ? El : p1;
+            const double S0 = j > 0 ? P0[t - 1] : p0, N0 = j < ny - 1 ? P0[t + 1] : p0;
+            const double S1 = j > 0 ? P1[t - 1] : p1, N1 = j < ny - 1 ? P1[t + 1] : p1;
+            const double rr0 = bb0[t - 1] - ((p1 + W0 - 2.0 * p0) * o.ax + (N0 + S0 - 2.0 * p0) * o.ay);
+            const double rr1 = bb1[t - 1] - ((E1 + p0 - 2.0 * p1) * o.ax + (N1 + S1 - 2.0 * p1) * o.ay);
+            if (out_lane && j < ny) {
+                const size_t k = ca + (size_t)pitch * j;
+                *reinterpret_cast<double2 *>(o.phi_out + k) = double2{p0, p1};
+                *reinterpret_cast<V *>(xout + k) = V{(T)rr0, (T)rr1};  // the next residual
+                m = (fabs(rr0) > m || rr0 != rr0) ? fabs(rr0) : m;
+                m = (fabs(rr1) > m || rr1 != rr1) ? fabs(rr1) : m;
+                sm += p0;
+                sm += p1;
             }
         }
         m = wave_max(m);
@@ -821,8 +1052,12 @@ static SmoothShapeRt smooth_shape(const Ctx &c, int l, int mode)
         s.nwaves = s.parts = (int)(s.grid.x * s.grid.y);
         return s;
     }
-    const int H = mode == 0 ? 4 : 5, OC = 64 - 2 * H;
+    const int OC = c.mg_smoother == 2 ? (mode == 0 ? 120 : 116) : (mode == 0 ? 56 : 54);
     s.rows = l == 0 ? c.mg_rows[sizeof(T) == 4][mode] : c.mg_rows_coarse;
+    if (c.mg_smoother == 2) {  // the paired-column kernel's register limit (pair_rows_max)
+        const int cap = sizeof(T) == 4 ? (mode == 2 ? 8 : 16) : (mode == 2 ? 4 : 8);
+        s.rows = std::min(s.rows, cap);
+    }
     s.nbands = (L.nx + OC - 1) / OC;
     s.nwaves = s.nbands * ((L.ny + s.rows - 1) / s.rows);
     s.grid = dim3(((s.nwaves + 3) / 4 + 7) / 8 * 8);  // a multiple of 8: the XCD remap applies
@@ -830,13 +1065,27 @@ static SmoothShapeRt smooth_shape(const Ctx &c, int l, int mode)
     return s;
 }
 
+// largest window of the paired-column kernel that fits the registers without
+// spilling: the shape falls back to it
+template <class T, int MODE>
+constexpr int pair_rows_max()
+{
+    return sizeof(T) == 4 ? (MODE == 2 ? 8 : 16) : (MODE == 2 ? 4 : 8);
+}
+
 template <class T, int MODE, int R>
 static void launch_gsrb(const Ctx &c, const SmoothShapeRt &s, const T *xin, const T *ec, int pc, T *xout,
                         const T *rhs, const Level &L, T *rc, const Level *Cl, const OuterArgs &o)
 {
-    hipLaunchKernelGGL((k_gsrb<T, MODE, R>), s.grid, dim3(256), 0, c.stream, xin, ec, pc, xout, rhs, L.nx, L.ny,
-                       L.pitch, (T)L.ax, (T)L.ay, rc, Cl ? Cl->nx : 0, Cl ? Cl->ny : 0, Cl ? Cl->pitch : 0, o,
-                       (const Ctl *)c.ctl, s.nbands, s.nwaves);
+    constexpr int R2 = R < pair_rows_max<T, MODE>() ? R : pair_rows_max<T, MODE>();
+    if (s.wave == 2)
+        hipLaunchKernelGGL((k_gsrb2<T, MODE, R2>), s.grid, dim3(256), 0, c.stream, xin, ec, pc, xout, rhs, L.nx, L.ny,
+                           L.pitch, (T)L.ax, (T)L.ay, rc, Cl ? Cl->nx : 0, Cl ? Cl->ny : 0, Cl ? Cl->pitch : 0, o,
+                           (const Ctl *)c.ctl, s.nbands, s.nwaves);
+    else
+        hipLaunchKernelGGL((k_gsrb<T, MODE, R>), s.grid, dim3(256), 0, c.stream, xin, ec, pc, xout, rhs, L.nx, L.ny,
+                           L.pitch, (T)L.ax, (T)L.ay, rc, Cl ? Cl->nx : 0, Cl ? Cl->ny : 0, Cl ? Cl->pitch : 0, o,
+                           (const Ctl *)c.ctl, s.nbands, s.nwaves);
 }
 
 // one smoother on level l (MODE 1: from zero with the fused restriction into

@@ -1,9 +1,9 @@
 """GPU checks of the multigrid smoothers.
 
-The wave-column smoother (the column window in registers, x-neighbours by DPP
-lane shifts) and the LDS tile smoother run the same red-black Gauss-Seidel
-half-sweeps, the same fused residual + restriction and the same fused FP64
-outer step.  So after a fixed number of V-cycles a Poisson solve with either
+The wave-column smoothers (the column window in registers, x-neighbours by DPP
+lane shifts; one or two columns per lane) and the LDS tile smoother run the
+same red-black Gauss-Seidel half-sweeps, the same fused residual + restriction
+and the same fused FP64 outer step.  So after a fixed number of V-cycles a Poisson solve with either
 has the same residual and the same iterate up to rounding, for every window
 height (output rows per wave).  Sizes cover partial last bands and row chunks
 (100, 200 and 400 are not multiples of the band widths 54 and 56, and 100 and
@@ -49,12 +49,13 @@ def _rhs(n, seed):
     return b
 
 
+@pytest.mark.parametrize("smoother", ["1", "2"])
 @pytest.mark.parametrize("precision", [0, 1])
 @pytest.mark.parametrize("n", [100, 200, 400, 1024])
-def test_wave_column_smoother_matches_tile_smoother(n, precision):
+def test_wave_column_smoother_matches_tile_smoother(n, precision, smoother):
     b = _rhs(n, 11 + n)
     ref, res_ref = _solve(n, precision, b, 3, AMRESSIF_MG_SMOOTHER="0")
-    got, res = _solve(n, precision, b, 3, AMRESSIF_MG_SMOOTHER="1")
+    got, res = _solve(n, precision, b, 3, AMRESSIF_MG_SMOOTHER=smoother)
     # three V-cycles reduce the residual by orders of magnitude; the two
     # kernels differ only in rounding: ~1e-16 (FP64) or ~1e-7 (FP32 V-cycle)
     # of the correction per cycle, and ||r|| / ||b|| by as much
@@ -64,21 +65,23 @@ def test_wave_column_smoother_matches_tile_smoother(n, precision):
     assert rel(got, ref) < tol
 
 
+@pytest.mark.parametrize("smoother", ["1", "2"])
 @pytest.mark.parametrize("rows", ["4,4,4", "8,8,8", "16,16,16", "32,32,8", "8,32,16"])
-def test_wave_column_window_heights_agree(rows):
+def test_wave_column_window_heights_agree(rows, smoother):
     n = 200
     b = _rhs(n, 5)
     ref, res_ref = _solve(n, 0, b, 3, AMRESSIF_MG_SMOOTHER="0")
     for coarse in ("4", "8", "32"):
-        got, res = _solve(n, 0, b, 3, AMRESSIF_MG_SMOOTHER="1", AMRESSIF_MG_ROWS64=rows,
+        got, res = _solve(n, 0, b, 3, AMRESSIF_MG_SMOOTHER=smoother, AMRESSIF_MG_ROWS64=rows,
                           AMRESSIF_MG_ROWS_COARSE=coarse)
         assert abs(res - res_ref) <= 1e-12, (rows, coarse)
         assert rel(got, ref) < 1e-10, (rows, coarse)
 
 
-def test_wave_column_smoother_converges_at_2048():
+@pytest.mark.parametrize("smoother", ["1", "2"])
+def test_wave_column_smoother_converges_at_2048(smoother):
     n = 2048
     b = _rhs(n, 3)
     for precision in (0, 1):
-        _, res = _solve(n, precision, b, 6, AMRESSIF_MG_SMOOTHER="1")
+        _, res = _solve(n, precision, b, 6, AMRESSIF_MG_SMOOTHER=smoother)
         assert res < 1e-6, (precision, res)

@@ -31,6 +31,26 @@ __device__ inline double block_reduce(double v, double *sh)
     return r;
 }
 
+// Cell tiles of 64 x 16 for a workgroup of 256 threads (four passes of 64 x 4)
+// on a one-dimensional grid whose size is a multiple of 8.  Workgroup b runs
+// on XCD b mod 8; the map hands each XCD a contiguous range of tiles in
+// row-major order, so the tiles above and below a tile, whose rows a stencil
+// reads again, are mostly on the same XCD and its L2.
+struct CellTile {
+    int i0, j0;
+    bool live;  // false: a padding workgroup past the last tile
+};
+__device__ inline CellTile cell_tile(int ntx, int nty)
+{
+    const int B = gridDim.x;
+    const int blk = (blockIdx.x & 7) * (B >> 3) + (blockIdx.x >> 3);
+    CellTile t;
+    t.i0 = (blk % ntx) * 64;
+    t.j0 = (blk / ntx) * 16;
+    t.live = blk / ntx < nty;
+    return t;
+}
+
 // QUICK half-node flux of one face (fn_flux_calc.cpp:66-73): vel is the face
 // velocity, a2 = U(i-2), a1 = U(i-1), b0 = U(i), b1 = U(i+1) along the face
 // normal.  ud = 0.5 vel + |0.5 vel| = max(vel, 0), up = min(vel, 0).

@@ -830,25 +830,30 @@ template <class T, bool FROM_U>
 __global__ __launch_bounds__(256) void k_poisson_start(double *b, const double *u, const double *v, double dx,
                                                        double dy, double dt, const double *phi1, const double *phi2,
                                                        double *phi0, T *r, int n, int P, double ax, double ay,
-                                                       int mode, int center, const Ctl *ctl, double *partials)
+                                                       int mode, int center, const Ctl *ctl, double *partials,
+                                                       int ntx, int nty)
 {
     __shared__ double sh[40];
-    const int i = blockIdx.x * 64 + (threadIdx.x & 63), j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const CellTile t = cell_tile(ntx, nty);
     double mr = 0.0, sp = 0.0, mb = 0.0, md = 0.0, sb = 0.0;
-    if (i < n && j < n) {
+    const double bmean = FROM_U || center ? ctl->b_mean : 0.0;
+    auto g = [&](size_t q) -> double {
+        return mode == 0 ? 0.0 : mode == 1 ? phi1[q] : 2.0 * phi1[q] - phi2[q];
+    };
+#pragma unroll
+    for (int pass = 0; pass < 4; ++pass) {
+        const int i = t.i0 + (threadIdx.x & 63), j = t.j0 + 4 * pass + (threadIdx.x >> 6);
+        if (!t.live || i >= n || j >= n) continue;
         const size_t k = i + (size_t)P * j;
-        auto g = [&](size_t q) -> double {
-            return mode == 0 ? 0.0 : mode == 1 ? phi1[q] : 2.0 * phi1[q] - phi2[q];
-        };
         double bc;
         if (FROM_U) {
             const double d = (u[k + 1] - u[k]) / dx + (v[k + P] - v[k]) / dy;
             const double braw = (1.5 / dt) * d;
-            bc = braw - ctl->b_mean;
-            md = fabs(d);
-            sb = braw;
+            bc = braw - bmean;
+            md = (fabs(d) > md || d != d) ? fabs(d) : md;
+            sb += braw;
         } else {
-            bc = center ? b[k] - ctl->b_mean : b[k];
+            bc = b[k] - bmean;
         }
         const double Pv = g(k);
         const double Wv = i > 0 ? g(k - 1) : Pv;
@@ -859,9 +864,9 @@ __global__ __launch_bounds__(256) void k_poisson_start(double *b, const double *
         if (FROM_U || center) b[k] = bc;
         phi0[k] = Pv;
         r[k] = (T)rr;
-        mr = fabs(rr);
-        sp = Pv;
-        mb = fabs(bc);
+        mr = (fabs(rr) > mr || rr != rr) ? fabs(rr) : mr;
+        sp += Pv;
+        mb = (fabs(bc) > mb || bc != bc) ? fabs(bc) : mb;
     }
     mr = block_reduce<true>(mr, sh);
     sp = block_reduce<false>(sp, sh + 8);
@@ -871,7 +876,7 @@ __global__ __launch_bounds__(256) void k_poisson_start(double *b, const double *
         sb = block_reduce<false>(sb, sh + 32);
     }
     if (threadIdx.x == 0) {
-        double *o = partials + (size_t)kStartParts * (blockIdx.x + (size_t)gridDim.x * blockIdx.y);
+        double *o = partials + (size_t)kStartParts * blockIdx.x;
         o[0] = mr;
         o[1] = sp;
         o[2] = mb;
@@ -935,15 +940,16 @@ hipError_t launch_poisson_start(const Ctx &c, int mode, int center, const double
                                 double *phi0, int mixed, const double *us, const double *vs)
 {
     const int n = c.g.n;
-    const dim3 grid((n + 63) / 64, (n + 3) / 4);
-    const int nb = (int)(grid.x * grid.y);
+    const int ntx = (n + 63) / 64, nty = (n + 15) / 16;
+    const int nb = (ntx * nty + 7) / 8 * 8;  // 64 x 16 tiles, a multiple of 8 workgroups
     if ((int64_t)nb * kStartParts > (int64_t)kMaxPartials * 4) return hipErrorInvalidValue;
+    const dim3 grid(nb);
     const Level &L = c.lev[0];
     const double dx = c.g.dx, dy = c.g.dy, dt = c.g.dt;
     if (us) hipLaunchKernelGGL(k_flux_mean, dim3(1), dim3(1024), 0, c.stream, us, vs, n, c.g.P, dx, dy, dt, c.ctl);
 #define AMR_START(T, FU, RR)                                                                                         \
     hipLaunchKernelGGL((k_poisson_start<T, FU>), grid, dim3(256), 0, c.stream, c.b, us, vs, dx, dy, dt, phi1, phi2,   \
-                       phi0, RR, n, c.g.P, L.ax, L.ay, mode, center, (const Ctl *)c.ctl, c.partials)
+                       phi0, RR, n, c.g.P, L.ax, L.ay, mode, center, (const Ctl *)c.ctl, c.partials, ntx, nty)
     if (mixed) {
         if (us) AMR_START(float, true, L.r32);
         else AMR_START(float, false, L.r32);

@@ -262,12 +262,16 @@ hipError_t launch_center_b(const Ctx &c, double *b)
 __global__ __launch_bounds__(256) void k_project(const double *us, const double *vs, const float *au,
                                                  const float *av, double *uo, double *vo, double *p,
                                                  const double *phi, int n, int P, double dx, double dy, double dt,
-                                                 const Ctl *ctl, int use_mean, double *partials)
+                                                 const Ctl *ctl, int use_mean, double *partials, int ntx, int nty)
 {
     __shared__ double sh[16];
-    const int i = blockIdx.x * 64 + (threadIdx.x & 63), j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const CellTile t = cell_tile(ntx, nty);
+    const double pm = use_mean ? ctl->phi_mean : 0.0;
     double m = 0.0;
-    if (i < n && j < n) {
+#pragma unroll
+    for (int pass = 0; pass < 4; ++pass) {
+        const int i = t.i0 + (threadIdx.x & 63), j = t.j0 + 4 * pass + (threadIdx.x >> 6);
+        if (!t.live || i >= n || j >= n) continue;
         const size_t k = i + (size_t)P * j;
         const double ph = phi[k];
         auto ux = [&](int fi) -> double {  // new x-face velocity at (fi, j)
@@ -289,23 +293,24 @@ __global__ __launch_bounds__(256) void k_project(const double *us, const double 
         vo[k] = vsth;
         if (i == n - 1) uo[k + 1] = ue;
         if (j == n - 1) vo[k + P] = vn;
-        const double pm = use_mean ? ctl->phi_mean : 0.0;
         p[k] = p[k] + (ph - pm);
-        m = fabs((ue - uw) / dx + (vn - vsth) / dy);
+        const double dv = fabs((ue - uw) / dx + (vn - vsth) / dy);
+        m = (dv > m || dv != dv) ? dv : m;
     }
     m = block_reduce<true>(m, sh);
-    if (threadIdx.x == 0) partials[blockIdx.x + (size_t)gridDim.x * blockIdx.y] = m;
+    if (threadIdx.x == 0) partials[blockIdx.x] = m;
 }
 
 hipError_t launch_project_ex(const Ctx &c, const double *us, const double *vs, double *uo, double *vo, double *p,
                              const double *phi, int use_mean, int *nblocks, const float *au, const float *av)
 {
     const int n = c.g.n;
-    dim3 grid((n + 63) / 64, (n + 3) / 4);
-    if ((int64_t)grid.x * grid.y > kMaxPartials) return hipErrorInvalidValue;
-    *nblocks = grid.x * grid.y;
-    hipLaunchKernelGGL(k_project, grid, dim3(256), 0, c.stream, us, vs, au, av, uo, vo, p, phi, n, c.g.P, c.g.dx,
-                       c.g.dy, c.g.dt, c.ctl, use_mean, c.partials);
+    const int ntx = (n + 63) / 64, nty = (n + 15) / 16;
+    const int nb = (ntx * nty + 7) / 8 * 8;  // 64 x 16 tiles, a multiple of 8 workgroups
+    if (nb > kMaxPartials) return hipErrorInvalidValue;
+    *nblocks = nb;
+    hipLaunchKernelGGL(k_project, dim3(nb), dim3(256), 0, c.stream, us, vs, au, av, uo, vo, p, phi, n, c.g.P, c.g.dx,
+                       c.g.dy, c.g.dt, c.ctl, use_mean, c.partials, ntx, nty);
     return hipGetLastError();
 }
 

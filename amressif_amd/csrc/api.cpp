@@ -312,11 +312,16 @@ int launch_pseudo_iteration(amressif_ctx *ctx, int it, int *nb);
 // BDF2 history and the RK4 dual-time pseudo-iterations of one step
 // (main.cpp:879-884, 898-1040): c.U holds the converged predictor u* after it,
 // the control block copy c.ctl_host its count and flags
-int amr_momentum_predictor(amressif_ctx *ctx)
+int amr_momentum_predictor(amressif_ctx *ctx, bool bdf_ready)
 {
     Ctx &c = ctx->c;
     HIPCHK(amr::launch_ctl_reset_step(c));
-    HIPCHK(amr::launch_bdf2(c));
+    if (bdf_ready) {  // the previous projection formed C and copied u^n to S2
+        std::swap(c.Uprev[0], c.S2[0]);
+        std::swap(c.Uprev[1], c.S2[1]);
+    } else {
+        HIPCHK(amr::launch_bdf2(c));
+    }
     int launched = 0, target = std::max(1, c.last_P), nb = 0;
     const int cap = c.in.IterNum + 1;
     for (;;) {
@@ -795,9 +800,11 @@ int advance_impl(amressif_ctx *ctx, int nsteps, amressif_step_stats *stats, amre
     int status = AMRESSIF_OK;
     const bool mixed = c.in.precision == AMRESSIF_MIXED;
     const bool warm = c.in.poisson_warm_start == 1 || (c.in.poisson_warm_start == -1 && mixed);
+    bool bdf_ready = false;  // C and S2 hold the next step's BDF2 constant and u^n
     for (int s = 0; s < nsteps; ++s) {
         int nb = 0;
-        if (int rc = amr_momentum_predictor(ctx)) return rc;
+        if (int rc = amr_momentum_predictor(ctx, bdf_ready)) return rc;
+        bdf_ready = false;
         const int flags = c.ctl_host->flags;
         if (tape) {  // record the converged predictor u* of this step
             const size_t bytes = tape->fsz * sizeof(double);
@@ -808,8 +815,13 @@ int advance_impl(amressif_ctx *ctx, int nsteps, amressif_step_stats *stats, amre
         }
         // ---- projection: Poisson RHS, MLMG-equivalent solve, update
         if (int rc = poisson_loop_step(ctx, warm, c.U[0], c.U[1])) return rc;
-        HIPCHK(amr::launch_project(c, c.U[0], c.U[1], c.S1[0], c.S1[1], &nb));
+        // between two steps of this call the projection also forms the next
+        // step's BDF2 constant (main.cpp:879-884); the last step leaves
+        // C and velContPrev as the reference does
+        const bool fold = s + 1 < nsteps && !(flags & AMRESSIF_FLAG_NONFINITE);
+        HIPCHK(amr::launch_project(c, c.U[0], c.U[1], c.S1[0], c.S1[1], &nb, nullptr, nullptr, fold ? 1 : 0));
         HIPCHK(amr::launch_fin_project(c, nb, s));
+        bdf_ready = fold;
         std::swap(c.U[0], c.S1[0]);
         std::swap(c.U[1], c.S1[1]);
         if (flags & AMRESSIF_FLAG_NONFINITE) {

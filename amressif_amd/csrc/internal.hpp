@@ -181,8 +181,10 @@ hipError_t launch_correct_residual(const Ctx &c, const double *phi_in, double *p
 size_t tail_lds_bytes(const Ctx &c, int mixed);
 hipError_t configure_tail(Ctx &c);
 hipError_t launch_smooth_fine(const Ctx &c, int kind, double *scratch_phi);
+// next_bdf: also form the next step's BDF2 constant in C and a copy of the new
+// velocity in S2 (that step then takes S2 as u^n and skips launch_bdf2)
 hipError_t launch_project(const Ctx &c, const double *us, const double *vs, double *uo, double *vo, int *nblocks,
-                          const float *au = nullptr, const float *av = nullptr);
+                          const float *au = nullptr, const float *av = nullptr, int next_bdf = 0);
 hipError_t launch_fin_project(const Ctx &c, int nblocks, int step_index);
 hipError_t launch_cont2cart_out(const Ctx &c, const double *u, const double *v, const amr_array4 &vc);
 hipError_t launch_pack_face(const Ctx &c, const amr_array4 &a, double *dst, int dir);
@@ -197,7 +199,7 @@ hipError_t launch_ctl_set(const Ctx &c, double tol_mom, double tol_rel, double t
                           int mg_max_iter);
 hipError_t launch_project_ex(const Ctx &c, const double *us, const double *vs, double *uo, double *vo, double *p,
                              const double *phi, int use_mean, int *nblocks, const float *au = nullptr,
-                             const float *av = nullptr);
+                             const float *av = nullptr, int next_bdf = 0);
 hipError_t launch_unpack_cell_shift(const Ctx &c, const double *src, const amr_array4 &a, int comp,
                                     int fill_ghosts, const double *shift);
 hipError_t launch_copy_bytes(const Ctx &c, const void *src, void *dst, int64_t bytes);
@@ -278,6 +280,6 @@ hipError_t launch_adj_finish(const Ctx &c, const double *su, const double *sv, d
 struct amressif_ctx {
     amr::Ctx c;
 };
-int amr_momentum_predictor(amressif_ctx *ctx);
+int amr_momentum_predictor(amressif_ctx *ctx, bool bdf_ready = false);
 int amr_poll(amressif_ctx *ctx);
 int amr_set_ctl(amressif_ctx *ctx);

@@ -259,10 +259,20 @@ hipError_t launch_center_b(const Ctx &c, double *b)
 // update_solution (poisson.cpp:192-392): u = u* - (G_x phi) dt / 1.5 on
 // interior faces (wall faces keep u*: the Neumann ghost makes G phi = 0),
 // p <- p + (phi - mean phi), and the partial ||D(u^{n+1})||inf.
+// With the next step's BDF2 folded in (nx.cu non-null): on every face the
+// constant of the next step, c = (1.5/dt) u^{n+1} + (0.5/dt)(u^{n+1} - u^n)
+// with u^n from nx.up (as k_bdf2 forms it), and a copy of u^{n+1} to nx.w,
+// which becomes that step's u^n; nx.up itself still holds u^n afterwards.
+struct NextBdf {
+    const double *up_u, *up_v;
+    double *cu, *cv, *wu, *wv;
+};
+
 __global__ __launch_bounds__(256) void k_project(const double *us, const double *vs, const float *au,
                                                  const float *av, double *uo, double *vo, double *p,
                                                  const double *phi, int n, int P, double dx, double dy, double dt,
-                                                 const Ctl *ctl, int use_mean, double *partials, int ntx, int nty)
+                                                 const Ctl *ctl, int use_mean, double *partials, int ntx, int nty,
+                                                 NextBdf nx)
 {
     __shared__ double sh[16];
     const CellTile t = cell_tile(ntx, nty);
@@ -289,10 +299,18 @@ __global__ __launch_bounds__(256) void k_project(const double *us, const double 
             return vstar - (g * dt / 1.5);
         };
         const double uw = ux(i), ue = ux(i + 1), vsth = vy(j), vn = vy(j + 1);
-        uo[k] = uw;
-        vo[k] = vsth;
-        if (i == n - 1) uo[k + 1] = ue;
-        if (j == n - 1) vo[k + P] = vn;
+        auto put = [&](double *o, const double *up, double *cc, double *wc, size_t f, double un) {
+            o[f] = un;
+            if (cc) {
+                const double um = up[f];
+                cc[f] = (1.5 / dt) * un + (0.5 / dt) * (un - um);
+                wc[f] = un;
+            }
+        };
+        put(uo, nx.up_u, nx.cu, nx.wu, k, uw);
+        put(vo, nx.up_v, nx.cv, nx.wv, k, vsth);
+        if (i == n - 1) put(uo, nx.up_u, nx.cu, nx.wu, k + 1, ue);
+        if (j == n - 1) put(vo, nx.up_v, nx.cv, nx.wv, k + P, vn);
         p[k] = p[k] + (ph - pm);
         const double dv = fabs((ue - uw) / dx + (vn - vsth) / dy);
         m = (dv > m || dv != dv) ? dv : m;
@@ -302,22 +320,24 @@ __global__ __launch_bounds__(256) void k_project(const double *us, const double 
 }
 
 hipError_t launch_project_ex(const Ctx &c, const double *us, const double *vs, double *uo, double *vo, double *p,
-                             const double *phi, int use_mean, int *nblocks, const float *au, const float *av)
+                             const double *phi, int use_mean, int *nblocks, const float *au, const float *av,
+                             int next_bdf)
 {
+    const NextBdf nx = next_bdf ? NextBdf{c.Uprev[0], c.Uprev[1], c.C[0], c.C[1], c.S2[0], c.S2[1]} : NextBdf{};
     const int n = c.g.n;
     const int ntx = (n + 63) / 64, nty = (n + 15) / 16;
     const int nb = (ntx * nty + 7) / 8 * 8;  // 64 x 16 tiles, a multiple of 8 workgroups
     if (nb > kMaxPartials) return hipErrorInvalidValue;
     *nblocks = nb;
     hipLaunchKernelGGL(k_project, dim3(nb), dim3(256), 0, c.stream, us, vs, au, av, uo, vo, p, phi, n, c.g.P, c.g.dx,
-                       c.g.dy, c.g.dt, c.ctl, use_mean, c.partials, ntx, nty);
+                       c.g.dy, c.g.dt, c.ctl, use_mean, c.partials, ntx, nty, nx);
     return hipGetLastError();
 }
 
 hipError_t launch_project(const Ctx &c, const double *us, const double *vs, double *uo, double *vo, int *nblocks,
-                          const float *au, const float *av)
+                          const float *au, const float *av, int next_bdf)
 {
-    return launch_project_ex(c, us, vs, uo, vo, c.p, c.phi, 1, nblocks, au, av);
+    return launch_project_ex(c, us, vs, uo, vo, c.p, c.phi, 1, nblocks, au, av, next_bdf);
 }
 
 __global__ __launch_bounds__(1024) void k_fin_project(const double *partials, int nb, Ctl *ctl,

@@ -551,6 +551,7 @@ int amressif_create(const amressif_inputs *in, void *hip_stream, amressif_ctx **
     if (const char *e = std::getenv("AMRESSIF_STAGE_ORDER")) c.stage_order = std::atoi(e) ? 1 : 0;
     // and of the multigrid smoothers (kernel choice and rows per wave: 4, 8, 16 or 32)
     if (const char *e = std::getenv("AMRESSIF_MG_SMOOTHER")) c.mg_smoother = std::max(0, std::min(2, std::atoi(e)));
+    if (const char *e = std::getenv("AMRESSIF_TAIL_DCT")) c.tail_dct = std::atoi(e) ? 1 : 0;
     auto rows_ok = [](int r) { return r == 4 || r == 8 || r == 16 || r == 32; };
     for (int t = 0; t < 2; ++t)
         if (const char *e = std::getenv(t ? "AMRESSIF_MG_ROWS32" : "AMRESSIF_MG_ROWS64")) {
@@ -656,6 +657,32 @@ int amressif_create(const amressif_inputs *in, void *hip_stream, amressif_ctx **
         hipError_t e2 = hipMemcpy(c.pinv32, pinv32.data(), pinv32.size() * 4, hipMemcpyHostToDevice);
         if (e1 != hipSuccess || e2 != hipSuccess) return bail(AMRESSIF_ERR_HIP, "pinv upload failed");
         if (amr::configure_tail(c) != hipSuccess) return bail(AMRESSIF_ERR_UNSUPPORTED, "multigrid tail does not fit LDS");
+        const amr::Level &T = c.lev[c.tail_level];
+        if (c.tail_dct && T.nx <= 64 && T.ny <= 64) {
+            // orthonormal DCT-II tables of the top tail level and the inverse
+            // eigenvalues of its Neumann operator (zero for the constant mode)
+            const int K = 64;
+            std::vector<double> tab(3 * K * K, 0.0);
+            const double pi = 3.14159265358979323846;
+            auto fill = [&](double *C, int m) {
+                for (int i = 0; i < m; ++i)
+                    for (int k = 0; k < m; ++k)
+                        C[i * K + k] = std::sqrt((k == 0 ? 1.0 : 2.0) / m) * std::cos(pi * k * (2 * i + 1) / (2.0 * m));
+            };
+            fill(tab.data(), T.nx);
+            fill(tab.data() + K * K, T.ny);
+            for (int mm = 0; mm < T.ny; ++mm)
+                for (int k = 0; k < T.nx; ++k) {
+                    const double lam = -T.ax * (2.0 - 2.0 * std::cos(pi * k / T.nx)) -
+                                       T.ay * (2.0 - 2.0 * std::cos(pi * mm / T.ny));
+                    tab[2 * K * K + mm * K + k] = (k == 0 && mm == 0) ? 0.0 : 1.0 / lam;
+                }
+            std::vector<float> tab32(tab.begin(), tab.end());
+            ALLOC(&c.dct_tab, tab.size()); ALLOC(&c.dct_tab32, tab.size());
+            hipError_t e3 = hipMemcpy(c.dct_tab, tab.data(), tab.size() * 8, hipMemcpyHostToDevice);
+            hipError_t e4 = hipMemcpy(c.dct_tab32, tab32.data(), tab32.size() * 4, hipMemcpyHostToDevice);
+            if (e3 != hipSuccess || e4 != hipSuccess) return bail(AMRESSIF_ERR_HIP, "DCT table upload failed");
+        }
     }
     if (in->poisson_solver == 1) {  // FGMRES: m+1 Arnoldi vectors, m preconditioned vectors
         c.gm_m = in->gmres_restart;

@@ -118,6 +118,11 @@ struct Ctx {
     double *phih = nullptr;  // warm start: the solution of the step before (allocated on first use)
     int phih_valid = 0;
     int tail_level = 0;  // first multigrid level handled by the single-workgroup tail kernel
+    // exact solve of that level by the DCT (k_tail_dct): cosine and inverse
+    // eigenvalue tables (null: the tail runs the V-cycle down to the bottom)
+    int tail_dct = 1;
+    double *dct_tab = nullptr;
+    float *dct_tab32 = nullptr;
     const double *lid = nullptr;  // caller-owned device lid profile or null
     Level lev[kMaxLevels]{};
     int nlev = 0;

@@ -765,6 +765,94 @@ __global__ __launch_bounds__(1024) void k_tail(const T *rtop, int ptop, T *etop,
     }
 }
 
+// ------------------------------------------------- exact tail (DCT solve)
+// The levels of at most 64 x 64 cells can be replaced by the exact Neumann
+// solve on the top one: the 5-point operator with mirror walls is diagonal in
+// the orthonormal DCT-II basis, L = Cy D Cx^T with D(m, k) = -ax (2 - 2 cos(pi
+// k / nx)) - ay (2 - 2 cos(pi m / ny)), so e = Cy (D^+ (Cy^T r Cx)) Cx^T, the
+// (0, 0) mode (the constant) set to zero as the pseudo-inverse does.  Four
+// 64 x 64 x 64 matrix products in one workgroup (zero padding to 64), each
+// thread a 4 x 4 block of the product; the cosine tables are staged in LDS
+// with a padded row stride against bank conflicts.
+constexpr int kDct = 64, kDctLd = 65;
+
+// table layout (global): Cx [64 x 64], Cy [64 x 64], D^+ [64 (m) x 64 (k)]
+template <class T, bool TA, bool TB>
+__device__ __forceinline__ void dct_gemm(const T *A, const T *B, T (&acc)[4][4])
+{
+    const int t = threadIdx.x, a0 = 4 * (t >> 4), b0 = 4 * (t & 15);
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[r][q] = T(0);
+#pragma unroll 4
+    for (int c = 0; c < kDct; ++c) {
+        T av[4], bv[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) av[r] = TA ? A[c * kDctLd + a0 + r] : A[(a0 + r) * kDctLd + c];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) bv[q] = TB ? B[(b0 + q) * kDctLd + c] : B[c * kDctLd + b0 + q];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[r][q] += av[r] * bv[q];
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_tail_dct(const T *rtop, int ptop, T *etop, int nx, int ny, const T *tab,
+                                                  const Ctl *ctl)
+{
+    __shared__ T Cx[kDct * kDctLd], Cy[kDct * kDctLd], B0[kDct * kDctLd], B1[kDct * kDctLd];
+    const int t = threadIdx.x, a0 = 4 * (t >> 4), b0 = 4 * (t & 15);
+    // loads first (the tables and r, zero-padded to 64 x 64); their latency overlaps the flag's
+    T vx[16], vy[16], vr[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int e = t + 256 * q, row = e >> 6, col = e & 63;
+        vx[q] = tab[e];
+        vy[q] = tab[kDct * kDct + e];
+        vr[q] = row < ny && col < nx ? rtop[col + (size_t)ptop * row] : T(0);
+    }
+    if (ctl->mg_done | ctl->gm_stop) return;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int e = t + 256 * q, row = e >> 6, col = e & 63;
+        Cx[row * kDctLd + col] = vx[q];
+        Cy[row * kDctLd + col] = vy[q];
+        B0[row * kDctLd + col] = vr[q];
+    }
+    __syncthreads();
+    T acc[4][4];
+    dct_gemm<T, false, false>(B0, Cx, acc);  // T1 = R Cx
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) B1[(a0 + r) * kDctLd + b0 + q] = acc[r][q];
+    __syncthreads();
+    dct_gemm<T, true, false>(Cy, B1, acc);  // S = Cy^T T1, then S D^+
+    const T *dinv = tab + 2 * kDct * kDct;
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) B0[(a0 + r) * kDctLd + b0 + q] = acc[r][q] * dinv[(a0 + r) * kDct + b0 + q];
+    __syncthreads();
+    dct_gemm<T, false, false>(Cy, B0, acc);  // T2 = Cy S
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) B1[(a0 + r) * kDctLd + b0 + q] = acc[r][q];
+    __syncthreads();
+    dct_gemm<T, false, true>(B1, Cx, acc);  // E = T2 Cx^T
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int row = a0 + r, col = b0 + q;
+            if (row < ny && col < nx) etop[col + (size_t)ptop * row] = acc[r][q];
+        }
+}
+
 // ------------------------------------------------ FP64 outer iteration
 // phi_out = phi_in + e (FP64; e null: no correction and no write),
 // r = b - L phi_out in FP64, stored in T for the next V-cycle; block
@@ -1132,11 +1220,18 @@ static hipError_t vcycle_impl(const Ctx &c, const T *pinv, const T *r0, const Ou
         launch_smoother<T, 1>(c, l, (const T *)nullptr, (const T *)nullptr, 0, lev_e<T>(c.lev[l], 0),
                               l == 0 ? r0 : (const T *)lev_r<T>(c.lev[l]), none);
     {
-        size_t lds = 0;
-        const TailLevels tl = tail_levels(c, lt, &lds, sizeof(T));
         const Level &Lt = c.lev[lt];
-        hipLaunchKernelGGL(k_tail<T>, dim3(1), dim3(1024), lds, c.stream, lt == 0 ? r0 : (const T *)lev_r<T>(Lt),
-                           Lt.pitch, lev_e<T>(Lt, 1), tl, pinv, ctl);
+        const T *rt = lt == 0 ? r0 : (const T *)lev_r<T>(Lt);
+        const T *tab = sizeof(T) == 4 ? (const T *)(const void *)c.dct_tab32 : (const T *)(const void *)c.dct_tab;
+        if (tab) {  // the exact solve of the top tail level
+            hipLaunchKernelGGL(k_tail_dct<T>, dim3(1), dim3(256), 0, c.stream, rt, Lt.pitch, lev_e<T>(Lt, 1), Lt.nx,
+                               Lt.ny, tab, ctl);
+        } else {
+            size_t lds = 0;
+            const TailLevels tl = tail_levels(c, lt, &lds, sizeof(T));
+            hipLaunchKernelGGL(k_tail<T>, dim3(1), dim3(1024), lds, c.stream, rt, Lt.pitch, lev_e<T>(Lt, 1), tl, pinv,
+                               ctl);
+        }
     }
     for (int l = lt - 1; l >= 0; --l) {  // up: add the coarse correction, smooth
         const Level &L = c.lev[l], &Cl = c.lev[l + 1];

@@ -18,11 +18,12 @@ from test_gpu_parity import _api, cuda, rel
 
 pytestmark = pytest.mark.gpu
 
-ENV_KEYS = ("AMRESSIF_MG_SMOOTHER", "AMRESSIF_MG_ROWS32", "AMRESSIF_MG_ROWS64", "AMRESSIF_MG_ROWS_COARSE")
+ENV_KEYS = ("AMRESSIF_MG_SMOOTHER", "AMRESSIF_MG_ROWS32", "AMRESSIF_MG_ROWS64", "AMRESSIF_MG_ROWS_COARSE",
+            "AMRESSIF_TAIL_DCT")
 
 
-def _solve(n, precision, b, cycles, **env):
-    """cycles V-cycles of the step's Poisson iteration from zero (tolerance 0,
+def _solve(n, precision, b, cycles, tol=0.0, **env):
+    """cycles V-cycles of the step's Poisson iteration from zero (tolerance tol,
     cycle cap), with the smoother settings in env; returns (phi, ||r||/||b||)"""
     api = _api()
     saved = {k: os.environ.get(k) for k in ENV_KEYS}
@@ -37,8 +38,8 @@ def _solve(n, precision, b, cycles, **env):
                 os.environ.pop(k, None)
             else:
                 os.environ[k] = v
-    sol, it, res = ctx.poisson_advance(cuda(b), tol_rel=0.0)
-    assert it == cycles
+    sol, it, res = ctx.poisson_advance(cuda(b), tol_rel=tol)
+    assert it == cycles if tol == 0.0 else it <= cycles
     return sol.cpu().numpy()[1:-1, 1:-1], res
 
 
@@ -85,3 +86,19 @@ def test_wave_column_smoother_converges_at_2048(smoother):
     for precision in (0, 1):
         _, res = _solve(n, precision, b, 6, AMRESSIF_MG_SMOOTHER=smoother)
         assert res < 1e-6, (precision, res)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("n", [64, 100, 256, 1024])
+def test_exact_tail_solve_matches_tail_v_cycle(n, precision):
+    """The levels of at most 64^2 cells solved exactly (DCT) instead of by the
+    V-cycle below them: the same converged solution, in no more V-cycles, and
+    each cycle reduces the residual at least as much."""
+    b = _rhs(n, 23 + n)
+    ref, res_ref = _solve(n, precision, b, 60, tol=1e-10, AMRESSIF_TAIL_DCT="0")
+    got, res = _solve(n, precision, b, 60, tol=1e-10, AMRESSIF_TAIL_DCT="1")
+    assert res <= 1e-10 and res_ref <= 1e-10
+    assert rel(got, ref) < 1e-8
+    _, r3_ref = _solve(n, precision, b, 3, AMRESSIF_TAIL_DCT="0")
+    _, r3 = _solve(n, precision, b, 3, AMRESSIF_TAIL_DCT="1")
+    assert r3 <= 1.5 * r3_ref, (r3, r3_ref)

@@ -258,29 +258,52 @@ int poisson_loop_step(amressif_ctx *ctx, bool warm, const double *us, const doub
     return mg_loop(ctx, warm, false, us, vs);
 }
 
-int mg_loop(amressif_ctx *ctx, bool warm, bool center, const double *us, const double *vs)
+// warm-start buffers of one solve (see mg_prepare)
+struct WarmStart {
+    int mode = 0;  // k_poisson_start: 0 from zero, 1 phi^{n-1}, 2 extrapolation
+    const double *phi1 = nullptr, *phi2 = nullptr;
+};
+
+WarmStart mg_prepare(amressif_ctx *ctx, bool warm, int *rc)
+{
+    Ctx &c = ctx->c;
+    WarmStart w;
+    *rc = 0;
+    if (!warm) return w;
+    if (!c.phih) {
+        const hipError_t e = amr::dev_alloc_public(c, &c.phih, (size_t)(c.g.n + 1) * c.g.P);
+        if (e != hipSuccess) {
+            *rc = fail(ctx, AMRESSIF_ERR_HIP, hipGetErrorString(e));
+            return w;
+        }
+    }
+    // three buffers, rotated instead of copied: X = phi^{n-1} (the last
+    // solution, one of the solve's pair), Y = phi^{n-2}, F = the other of
+    // the pair.  The guess goes into F and the solve ping-pongs F <-> Y;
+    // X becomes phi^{n-2} of the next step.
+    double *X = c.phi, *Y = c.phih, *F = c.phi == c.phiv[0] ? c.phiv[1] : c.phiv[0];
+    w.mode = c.phih_valid ? 2 : 1;
+    w.phi1 = X;
+    w.phi2 = Y;
+    c.phiv[0] = F;
+    c.phiv[1] = Y;
+    c.phih = X;
+    c.phih_valid = 1;
+    return w;
+}
+
+// The multigrid iteration from the prepared start.  require_mom: enqueued
+// before the host saw the pseudo-iteration loop end; *skipped reports that it
+// had not (every kernel of the phase then did nothing) and the caller runs it
+// again once the loop has ended.
+int mg_run(amressif_ctx *ctx, const WarmStart &w, bool center, const double *us, const double *vs,
+           bool require_mom, bool *skipped)
 {
     Ctx &c = ctx->c;
     const int mixed = c.in.precision == AMRESSIF_MIXED;
-    int nb = 0, mode = 0;
-    const double *phi1 = nullptr, *phi2 = nullptr;
-    HIPCHK(amr::launch_mg_start(c));
-    if (warm) {
-        if (!c.phih) HIPCHK(amr::dev_alloc_public(c, &c.phih, (size_t)(c.g.n + 1) * c.g.P));
-        // three buffers, rotated instead of copied: X = phi^{n-1} (the last
-        // solution, one of the solve's pair), Y = phi^{n-2}, F = the other of
-        // the pair.  The guess goes into F and the solve ping-pongs F <-> Y;
-        // X becomes phi^{n-2} of the next step.
-        double *X = c.phi, *Y = c.phih, *F = c.phi == c.phiv[0] ? c.phiv[1] : c.phiv[0];
-        mode = c.phih_valid ? 2 : 1;
-        phi1 = X;
-        phi2 = Y;
-        c.phiv[0] = F;
-        c.phiv[1] = Y;
-        c.phih = X;
-        c.phih_valid = 1;
-    }
-    HIPCHK(amr::launch_poisson_start(c, mode, center ? 1 : 0, phi1, phi2, c.phiv[0], mixed, us, vs));  // + finalize
+    int nb = 0;
+    HIPCHK(amr::launch_mg_start(c, require_mom ? 1 : 0));
+    HIPCHK(amr::launch_poisson_start(c, w.mode, center ? 1 : 0, w.phi1, w.phi2, c.phiv[0], mixed, us, vs));
     int launched = 0, target = std::max(1, c.last_V);
     const int cap = std::max(1, c.in.mg_max_iter);
     for (;;) {
@@ -291,12 +314,24 @@ int mg_loop(amressif_ctx *ctx, bool warm, bool center, const double *us, const d
             ++launched;
         }
         if (int rc = poll(ctx)) return rc;
+        if (require_mom && c.ctl_host->mg_skip) {
+            *skipped = true;
+            return 0;
+        }
         if (c.ctl_host->mg_done || launched >= cap) break;
         target = launched + 2;
     }
     c.last_V = std::max(1, c.ctl_host->mg_iters);
     c.phi = c.phiv[c.ctl_host->mg_iters & 1];
     return 0;
+}
+
+int mg_loop(amressif_ctx *ctx, bool warm, bool center, const double *us, const double *vs)
+{
+    int rc = 0;
+    const WarmStart w = mg_prepare(ctx, warm, &rc);
+    if (rc) return rc;
+    return mg_run(ctx, w, center, us, vs, false, nullptr);
 }
 
 // One pseudo-iteration.  FP64: the RK4 dual-time sweep, four guarded stage
@@ -312,7 +347,7 @@ int launch_pseudo_iteration(amressif_ctx *ctx, int it, int *nb);
 // BDF2 history and the RK4 dual-time pseudo-iterations of one step
 // (main.cpp:879-884, 898-1040): c.U holds the converged predictor u* after it,
 // the control block copy c.ctl_host its count and flags
-int amr_momentum_predictor(amressif_ctx *ctx, bool bdf_ready)
+int amr_momentum_begin(amressif_ctx *ctx, bool bdf_ready, int *launched)
 {
     Ctx &c = ctx->c;
     HIPCHK(amr::launch_ctl_reset_step(c));
@@ -322,8 +357,20 @@ int amr_momentum_predictor(amressif_ctx *ctx, bool bdf_ready)
     } else {
         HIPCHK(amr::launch_bdf2(c));
     }
-    int launched = 0, target = std::max(1, c.last_P), nb = 0;
+    const int target = std::min(std::max(1, c.last_P), c.in.IterNum + 1);
+    int nb = 0;
+    for (*launched = 0; *launched < target; ++*launched) {
+        if (int rc = launch_pseudo_iteration(ctx, *launched, &nb)) return rc;
+        HIPCHK(amr::launch_fin_pseudo(c, nb));
+    }
+    return 0;
+}
+
+int amr_momentum_finish(amressif_ctx *ctx, int launched)
+{
+    Ctx &c = ctx->c;
     const int cap = c.in.IterNum + 1;
+    int target = launched, nb = 0;
     for (;;) {
         while (launched < target && launched < cap) {
             if (int rc = launch_pseudo_iteration(ctx, launched, &nb)) return rc;
@@ -336,6 +383,13 @@ int amr_momentum_predictor(amressif_ctx *ctx, bool bdf_ready)
     }
     c.last_P = std::max(1, c.ctl_host->mom_count);
     return 0;
+}
+
+int amr_momentum_predictor(amressif_ctx *ctx, bool bdf_ready)
+{
+    int launched = 0;
+    if (int rc = amr_momentum_begin(ctx, bdf_ready, &launched)) return rc;
+    return amr_momentum_finish(ctx, launched);
 }
 
 int amr_poll(amressif_ctx *ctx) { return poll(ctx); }
@@ -828,10 +882,31 @@ int advance_impl(amressif_ctx *ctx, int nsteps, amressif_step_stats *stats, amre
     const bool mixed = c.in.precision == AMRESSIF_MIXED;
     const bool warm = c.in.poisson_warm_start == 1 || (c.in.poisson_warm_start == -1 && mixed);
     bool bdf_ready = false;  // C and S2 hold the next step's BDF2 constant and u^n
+    // Without a tape and with the multigrid solver, the Poisson phase is
+    // enqueued right behind the first batch of pseudo-iterations, with no
+    // host wait between them; its kernels do nothing if that batch did not
+    // end the loop, and the phase then runs again after the loop has ended.
+    const bool speculate = !tape && c.in.poisson_solver != 1;
     for (int s = 0; s < nsteps; ++s) {
         int nb = 0;
-        if (int rc = amr_momentum_predictor(ctx, bdf_ready)) return rc;
-        bdf_ready = false;
+        if (speculate) {
+            int launched = 0, rc = 0;
+            if ((rc = amr_momentum_begin(ctx, bdf_ready, &launched))) return rc;
+            bdf_ready = false;
+            const WarmStart w = mg_prepare(ctx, warm, &rc);
+            if (rc) return rc;
+            bool skipped = false;
+            if ((rc = mg_run(ctx, w, false, c.U[0], c.U[1], true, &skipped))) return rc;
+            if (skipped) {
+                if ((rc = amr_momentum_finish(ctx, launched))) return rc;
+                if ((rc = mg_run(ctx, w, false, c.U[0], c.U[1], false, nullptr))) return rc;
+            } else {
+                c.last_P = std::max(1, c.ctl_host->mom_count);
+            }
+        } else {
+            if (int rc = amr_momentum_predictor(ctx, bdf_ready)) return rc;
+            bdf_ready = false;
+        }
         const int flags = c.ctl_host->flags;
         if (tape) {  // record the converged predictor u* of this step
             const size_t bytes = tape->fsz * sizeof(double);
@@ -841,7 +916,8 @@ int advance_impl(amressif_ctx *ctx, int nsteps, amressif_step_stats *stats, amre
             tape->count += 1;
         }
         // ---- projection: Poisson RHS, MLMG-equivalent solve, update
-        if (int rc = poisson_loop_step(ctx, warm, c.U[0], c.U[1])) return rc;
+        if (!speculate)
+            if (int rc = poisson_loop_step(ctx, warm, c.U[0], c.U[1])) return rc;
         // between two steps of this call the projection also forms the next
         // step's BDF2 constant (main.cpp:879-884); the last step leaves
         // C and velContPrev as the reference does

@@ -33,6 +33,7 @@ struct Ctl {
     // linear (tangent / adjoint) Richardson solves
     int lin_done, lin_count, lin_cap;
     int gm_stop;  // GMRES: the current Arnoldi cycle is finished (V-cycle kernels skip)
+    int mg_skip;  // the step's Poisson phase ran before the pseudo-iterations ended: skipped
     double lin_tol, lin_rel;
 };
 
@@ -174,7 +175,7 @@ hipError_t launch_poisson_rhs(const Ctx &c, const double *u, const double *v, do
                               const float *au = nullptr, const float *av = nullptr);
 hipError_t launch_fin_prhs(const Ctx &c, int nblocks);
 hipError_t launch_center_b(const Ctx &c, double *b);
-hipError_t launch_mg_start(const Ctx &c);
+hipError_t launch_mg_start(const Ctx &c, int require_mom = 0);
 // first residual of a Poisson solve (see kernels_mg.hip), with its finalize
 hipError_t launch_poisson_start(const Ctx &c, int mode, int center, const double *phi1, const double *phi2,
                                 double *phi0, int mixed, const double *us = nullptr, const double *vs = nullptr);
@@ -286,5 +287,9 @@ struct amressif_ctx {
     amr::Ctx c;
 };
 int amr_momentum_predictor(amressif_ctx *ctx, bool bdf_ready = false);
+// its two halves: the step's set-up and first batch of pseudo-iterations (no
+// host wait), then the polled continuation until the loop ends
+int amr_momentum_begin(amressif_ctx *ctx, bool bdf_ready, int *launched);
+int amr_momentum_finish(amressif_ctx *ctx, int launched);
 int amr_poll(amressif_ctx *ctx);
 int amr_set_ctl(amressif_ctx *ctx);

@@ -889,9 +889,14 @@ __global__ __launch_bounds__(256) void k_correct_residual(const double *phi_in, 
     }
 }
 
-__global__ void k_mg_start(Ctl *ctl)
+// require_mom: the step's Poisson phase was enqueued before the host saw the
+// pseudo-iteration loop end; if it has not ended, the whole phase is skipped
+// (mg_done and mg_skip set) and the host runs it again later
+__global__ void k_mg_start(Ctl *ctl, int require_mom)
 {
-    ctl->mg_done = 0;
+    const int skip = require_mom && !ctl->mom_done;
+    ctl->mg_done = skip;
+    ctl->mg_skip = skip;
     ctl->gm_stop = 0;
     ctl->mg_iters = 0;
     ctl->mg_started = 0;
@@ -922,6 +927,7 @@ __global__ __launch_bounds__(256) void k_poisson_start(double *b, const double *
                                                        int ntx, int nty)
 {
     __shared__ double sh[40];
+    if (ctl->mg_done) return;  // a skipped phase (k_mg_start)
     const CellTile t = cell_tile(ntx, nty);
     double mr = 0.0, sp = 0.0, mb = 0.0, md = 0.0, sb = 0.0;
     const double bmean = FROM_U || center ? ctl->b_mean : 0.0;
@@ -979,6 +985,7 @@ __global__ __launch_bounds__(1024) void k_flux_mean(const double *u, const doubl
                                                     double dy, double dt, Ctl *ctl)
 {
     __shared__ double sh[16];
+    if (ctl->mg_done) return;  // a skipped phase (k_mg_start)
     double s = 0.0;
     for (int j = threadIdx.x; j < n; j += blockDim.x) {
         const size_t row = (size_t)P * j;
@@ -994,6 +1001,7 @@ __global__ __launch_bounds__(1024) void k_flux_mean(const double *u, const doubl
 __global__ __launch_bounds__(1024) void k_fin_start(const double *partials, int nb, Ctl *ctl, int n, int from_u)
 {
     __shared__ double sh[80];
+    if (ctl->mg_done) return;  // a skipped phase (k_mg_start)
     double mr = 0.0, sp = 0.0, mb = 0.0, md = 0.0, sb = 0.0;
     for (int q = threadIdx.x; q < nb; q += blockDim.x) {
         const double *o = partials + (size_t)kStartParts * q;
@@ -1051,9 +1059,9 @@ hipError_t launch_poisson_start(const Ctx &c, int mode, int center, const double
     return hipGetLastError();
 }
 
-hipError_t launch_mg_start(const Ctx &c)
+hipError_t launch_mg_start(const Ctx &c, int require_mom)
 {
-    hipLaunchKernelGGL(k_mg_start, dim3(1), dim3(1), 0, c.stream, c.ctl);
+    hipLaunchKernelGGL(k_mg_start, dim3(1), dim3(1), 0, c.stream, c.ctl, require_mom);
     return hipGetLastError();
 }
 

@@ -606,6 +606,7 @@ int amressif_create(const amressif_inputs *in, void *hip_stream, amressif_ctx **
     // and of the multigrid smoothers (kernel choice and rows per wave: 4, 8, 16 or 32)
     if (const char *e = std::getenv("AMRESSIF_MG_SMOOTHER")) c.mg_smoother = std::max(0, std::min(2, std::atoi(e)));
     if (const char *e = std::getenv("AMRESSIF_TAIL_DCT")) c.tail_dct = std::atoi(e) ? 1 : 0;
+    if (const char *e = std::getenv("AMRESSIF_TAIL_EXT")) c.tail_ext = std::atoi(e) ? 1 : 0;
     auto rows_ok = [](int r) { return r == 4 || r == 8 || r == 16 || r == 32; };
     for (int t = 0; t < 2; ++t)
         if (const char *e = std::getenv(t ? "AMRESSIF_MG_ROWS32" : "AMRESSIF_MG_ROWS64")) {
@@ -736,6 +737,11 @@ int amressif_create(const amressif_inputs *in, void *hip_stream, amressif_ctx **
             hipError_t e3 = hipMemcpy(c.dct_tab, tab.data(), tab.size() * 8, hipMemcpyHostToDevice);
             hipError_t e4 = hipMemcpy(c.dct_tab32, tab32.data(), tab32.size() * 4, hipMemcpyHostToDevice);
             if (e3 != hipSuccess || e4 != hipSuccess) return bail(AMRESSIF_ERR_HIP, "DCT table upload failed");
+            const amr::Level *L1 = c.tail_level >= 2 ? &c.lev[c.tail_level - 1] : nullptr;
+            c.tail_ext = c.tail_ext && in->precision == AMRESSIF_MIXED && L1 && L1->nx == 2 * T.nx &&
+                         L1->ny == 2 * T.ny && L1->nx <= 128 && (L1->ny == 32 || L1->ny == 64 || L1->ny == 128);
+        } else {
+            c.tail_ext = 0;
         }
     }
     if (in->poisson_solver == 1) {  // FGMRES: m+1 Arnoldi vectors, m preconditioned vectors

@@ -124,6 +124,9 @@ struct Ctx {
     int tail_dct = 1;
     double *dct_tab = nullptr;
     float *dct_tab32 = nullptr;
+    // FP32 V-cycle: the level above it smoothed inside the same kernel
+    // (k_tail_dct_ext; 32, 64 or 128 rows, at most 128 columns)
+    int tail_ext = 1;
     const double *lid = nullptr;  // caller-owned device lid profile or null
     Level lev[kMaxLevels]{};
     int nlev = 0;

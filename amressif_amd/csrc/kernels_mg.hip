@@ -799,6 +799,34 @@ __device__ __forceinline__ void dct_gemm(const T *A, const T *B, T (&acc)[4][4])
     }
 }
 
+// the DCT solve of the workgroup: R in B0 (zero outside ny x nx), the tables
+// in Cx, Cy (LDS) and dinv (global); the 4 x 4 block of E of this thread in
+// acc.  B0 and B1 are overwritten.
+template <class T>
+__device__ __forceinline__ void dct_solve(const T *Cx, const T *Cy, T *B0, T *B1, const T *dinv, T (&acc)[4][4])
+{
+    const int t = threadIdx.x, a0 = 4 * (t >> 4), b0 = 4 * (t & 15);
+    dct_gemm<T, false, false>(B0, Cx, acc);  // T1 = R Cx
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) B1[(a0 + r) * kDctLd + b0 + q] = acc[r][q];
+    __syncthreads();
+    dct_gemm<T, true, false>(Cy, B1, acc);  // S = Cy^T T1, then S D^+
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) B0[(a0 + r) * kDctLd + b0 + q] = acc[r][q] * dinv[(a0 + r) * kDct + b0 + q];
+    __syncthreads();
+    dct_gemm<T, false, false>(Cy, B0, acc);  // T2 = Cy S
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) B1[(a0 + r) * kDctLd + b0 + q] = acc[r][q];
+    __syncthreads();
+    dct_gemm<T, false, true>(B1, Cx, acc);  // E = T2 Cx^T
+}
+
 template <class T>
 __global__ __launch_bounds__(256) void k_tail_dct(const T *rtop, int ptop, T *etop, int nx, int ny, const T *tab,
                                                   const Ctl *ctl)
@@ -824,26 +852,7 @@ __global__ __launch_bounds__(256) void k_tail_dct(const T *rtop, int ptop, T *et
     }
     __syncthreads();
     T acc[4][4];
-    dct_gemm<T, false, false>(B0, Cx, acc);  // T1 = R Cx
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) B1[(a0 + r) * kDctLd + b0 + q] = acc[r][q];
-    __syncthreads();
-    dct_gemm<T, true, false>(Cy, B1, acc);  // S = Cy^T T1, then S D^+
-    const T *dinv = tab + 2 * kDct * kDct;
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) B0[(a0 + r) * kDctLd + b0 + q] = acc[r][q] * dinv[(a0 + r) * kDct + b0 + q];
-    __syncthreads();
-    dct_gemm<T, false, false>(Cy, B0, acc);  // T2 = Cy S
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) B1[(a0 + r) * kDctLd + b0 + q] = acc[r][q];
-    __syncthreads();
-    dct_gemm<T, false, true>(B1, Cx, acc);  // E = T2 Cx^T
+    dct_solve<T>(Cx, Cy, B0, B1, tab + 2 * kDct * kDct, acc);
 #pragma unroll
     for (int r = 0; r < 4; ++r)
 #pragma unroll
@@ -851,6 +860,162 @@ __global__ __launch_bounds__(256) void k_tail_dct(const T *rtop, int ptop, T *et
             const int row = a0 + r, col = b0 + q;
             if (row < ny && col < nx) etop[col + (size_t)ptop * row] = acc[r][q];
         }
+}
+
+// The level above the DCT level (twice its size: ny1 = 4 R1 rows, nx1 <= 128
+// columns) in the same workgroup, FP32: wave w owns rows [w R1, (w + 1) R1)
+// and every column (two per lane, as k_gsrb2), and trades its first and last
+// row with the neighbouring waves through LDS after every half-sweep (double
+// buffered: one barrier per exchange).  Down leg: two GSRB sweeps from zero,
+// the residual and its 2x2 average as the DCT input; then the DCT solve; up
+// leg: the coarse correction added, two sweeps, the level's correction stored.
+// Replaces that level's two smoother launches.
+template <int R1>
+__device__ __forceinline__ void ext_sweeps(float (&x0)[R1 + 2], float (&x1)[R1 + 2], const float (&rs0)[R1],
+                                           const float (&rs1)[R1], const float (&A)[2][2], const float (&B)[2][2],
+                                           int j0, int ny1, float (*xch)[4][2][128], int &par, int w, int lane,
+                                           bool last_exchange)
+{
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+#pragma unroll
+        for (int q = 1; q <= R1; ++q) {
+            const int j = j0 + q - 1;
+            const bool even = ((q + 1 + h) & 1) == 0;  // j0 is even: row j has the parity of q - 1
+            const int cell = even ? 0 : 1;
+            const bool edge = j == 0 || j == ny1 - 1;
+            const float a = edge ? A[cell][1] : A[cell][0], b = edge ? B[cell][1] : B[cell][0];
+            if (even) {
+                const float we = lane_prev(x1[q]) + x1[q];
+                const float sn = x0[q - 1] + x0[q + 1];
+                x0[q] = a * we + (b * sn - rs0[q - 1]);
+            } else {
+                const float we = x0[q] + lane_next(x0[q]);
+                const float sn = x1[q - 1] + x1[q + 1];
+                x1[q] = a * we + (b * sn - rs1[q - 1]);
+            }
+        }
+        if (h < 3 || last_exchange) {
+            xch[par][w][0][2 * lane] = x0[1];
+            xch[par][w][0][2 * lane + 1] = x1[1];
+            xch[par][w][1][2 * lane] = x0[R1];
+            xch[par][w][1][2 * lane + 1] = x1[R1];
+            __syncthreads();
+            if (w > 0) {
+                x0[0] = xch[par][w - 1][1][2 * lane];
+                x1[0] = xch[par][w - 1][1][2 * lane + 1];
+            }
+            if (w < 3) {
+                x0[R1 + 1] = xch[par][w + 1][0][2 * lane];
+                x1[R1 + 1] = xch[par][w + 1][0][2 * lane + 1];
+            }
+            par ^= 1;
+        }
+    }
+}
+
+template <int R1>
+__global__ __launch_bounds__(256) void k_tail_dct_ext(const float *r1g, int p1, float *e1g, int nx1, float ax1,
+                                                      float ay1, int nx, int ny, const float *tab, const Ctl *ctl)
+{
+    __shared__ float Cx[kDct * kDctLd], Cy[kDct * kDctLd], B0[kDct * kDctLd], B1[kDct * kDctLd];
+    __shared__ float xch[2][4][2][128];  // [buffer][wave][first, last own row][column]
+    constexpr int ny1 = 4 * R1;
+    const int t = threadIdx.x, w = t >> 6, lane = t & 63, a0 = 4 * (t >> 4), b0 = 4 * (t & 15);
+    const int j0 = w * R1, ca = 2 * lane;
+    const bool in_i = ca < nx1;
+    // loads first: the tables and the own rows of r; their latency overlaps the flag's
+    float vx[16], vy[16], r0[R1], r1[R1];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        vx[q] = tab[t + 256 * q];
+        vy[q] = tab[kDct * kDct + t + 256 * q];
+    }
+#pragma unroll
+    for (int q = 0; q < R1; ++q) {
+        float2 v = {0.f, 0.f};
+        if (in_i) v = *reinterpret_cast<const float2 *>(r1g + ca + (size_t)p1 * (j0 + q));
+        r0[q] = v.x;
+        r1[q] = v.y;
+    }
+    if (ctl->mg_done | ctl->gm_stop) return;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int e = t + 256 * q, row = e >> 6, col = e & 63;
+        Cx[row * kDctLd + col] = vx[q];
+        Cy[row * kDctLd + col] = vy[q];
+        B0[row * kDctLd + col] = 0.f;  // the DCT input, zero outside ny x nx
+    }
+    const float cw0 = ca > 0 ? ax1 : 0.f, ce1 = ca + 1 < nx1 - 1 ? ax1 : 0.f;
+    const float d0m = cw0 + ax1 + 2.f * ay1, d0e = cw0 + ax1 + ay1, d1m = ax1 + ce1 + 2.f * ay1,
+                d1e = ax1 + ce1 + ay1;
+    const float i0m = 1.f / d0m, i0e = 1.f / d0e, i1m = 1.f / d1m, i1e = 1.f / d1e;
+    float A[2][2], B[2][2];
+    A[0][0] = in_i ? ax1 * i0m : 0.f;
+    A[0][1] = in_i ? ax1 * i0e : 0.f;
+    A[1][0] = in_i ? ax1 * i1m : 0.f;
+    A[1][1] = in_i ? ax1 * i1e : 0.f;
+    B[0][0] = in_i ? ay1 * i0m : 0.f;
+    B[0][1] = in_i ? ay1 * i0e : 0.f;
+    B[1][0] = in_i ? ay1 * i1m : 0.f;
+    B[1][1] = in_i ? ay1 * i1e : 0.f;
+#pragma unroll
+    for (int q = 0; q < R1; ++q) {  // r -> r / d
+        const int j = j0 + q;
+        const bool mid = j > 0 && j < ny1 - 1;
+        r0[q] *= mid ? i0m : i0e;
+        r1[q] *= mid ? i1m : i1e;
+    }
+    float x0[R1 + 2], x1[R1 + 2];  // rows j0 - 1 .. j0 + R1 (the outer two from the neighbours)
+#pragma unroll
+    for (int q = 0; q < R1 + 2; ++q) x0[q] = x1[q] = 0.f;
+    int par = 0;
+    __syncthreads();  // B0 cleared before the restriction writes it
+    ext_sweeps<R1>(x0, x1, r0, r1, A, B, j0, ny1, xch, par, w, lane, true);
+    // residual of the own rows, 2x2 averages (lane-local) into the DCT input
+#pragma unroll
+    for (int m = 0; m < R1 / 2; ++m) {
+        float t4[4];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int q = 1 + 2 * m + u, j = j0 + q - 1;
+            const bool mid = j > 0 && j < ny1 - 1;
+            const float P0 = x0[q], P1 = x1[q];
+            const float Wl = lane_prev(P1), El = lane_next(P0);
+            const float W0 = ca > 0 ? Wl : P0, E1 = ca + 1 < nx1 - 1 ? El : P1;
+            const float S0 = j > 0 ? x0[q - 1] : P0, N0 = j < ny1 - 1 ? x0[q + 1] : P0;
+            const float S1 = j > 0 ? x1[q - 1] : P1, N1 = j < ny1 - 1 ? x1[q + 1] : P1;
+            const float rq0 = r0[q - 1] * (mid ? d0m : d0e), rq1 = r1[q - 1] * (mid ? d1m : d1e);
+            t4[2 * u] = rq0 - ((P1 + W0 - 2.f * P0) * ax1 + (N0 + S0 - 2.f * P0) * ay1);
+            t4[2 * u + 1] = rq1 - ((E1 + P0 - 2.f * P1) * ax1 + (N1 + S1 - 2.f * P1) * ay1);
+        }
+        const int I = lane, J = j0 / 2 + m;
+        if (in_i && I < nx && J < ny) B0[J * kDctLd + I] = 0.25f * (((t4[0] + t4[1]) + t4[2]) + t4[3]);
+    }
+    __syncthreads();
+    float acc[4][4];
+    dct_solve<float>(Cx, Cy, B0, B1, tab + 2 * kDct * kDct, acc);
+    __syncthreads();  // every wave is past the last product before B0 takes the correction
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) B0[(a0 + r) * kDctLd + b0 + q] = acc[r][q];
+    __syncthreads();
+    // up leg: add the coarse correction (the halo rows too: they hold the
+    // neighbours' down-leg values), two sweeps, store the own rows
+#pragma unroll
+    for (int q = 0; q < R1 + 2; ++q) {
+        const int j = j0 + q - 1;
+        if (j >= 0 && j < ny1 && in_i) {
+            const float e = B0[(j >> 1) * kDctLd + lane];
+            x0[q] += e;
+            x1[q] += e;
+        }
+    }
+    ext_sweeps<R1>(x0, x1, r0, r1, A, B, j0, ny1, xch, par, w, lane, false);
+#pragma unroll
+    for (int q = 1; q <= R1; ++q)
+        if (in_i) *reinterpret_cast<float2 *>(e1g + ca + (size_t)p1 * (j0 + q - 1)) = float2{x0[q], x1[q]};
 }
 
 // ------------------------------------------------ FP64 outer iteration
@@ -1222,12 +1387,24 @@ template <class T>
 static hipError_t vcycle_impl(const Ctx &c, const T *pinv, const T *r0, const OuterArgs *fuse, T *r0_next)
 {
     const Ctl *ctl = c.ctl;
-    const int lt = c.tail_level;
+    const int ext = sizeof(T) == 4 && c.tail_ext && c.dct_tab32;  // level tail_level - 1 joins the tail kernel
+    const int lt = c.tail_level - ext;
     const OuterArgs none{};
     for (int l = 0; l < lt; ++l)  // down: smooth from zero, fused residual + restriction
         launch_smoother<T, 1>(c, l, (const T *)nullptr, (const T *)nullptr, 0, lev_e<T>(c.lev[l], 0),
                               l == 0 ? r0 : (const T *)lev_r<T>(c.lev[l]), none);
-    {
+    if (ext) {
+        const Level &L1 = c.lev[lt], &Lt = c.lev[lt + 1];
+        const float *r1 = (const float *)(const void *)lev_r<T>(L1);
+        float *e1 = (float *)(void *)lev_e<T>(L1, 1);
+#define AMR_EXT(R1)                                                                                                  \
+    hipLaunchKernelGGL(k_tail_dct_ext<R1>, dim3(1), dim3(256), 0, c.stream, r1, L1.pitch, e1, L1.nx, (float)L1.ax,    \
+                       (float)L1.ay, Lt.nx, Lt.ny, (const float *)c.dct_tab32, ctl)
+        if (L1.ny == 32) AMR_EXT(8);
+        else if (L1.ny == 64) AMR_EXT(16);
+        else AMR_EXT(32);
+#undef AMR_EXT
+    } else {
         const Level &Lt = c.lev[lt];
         const T *rt = lt == 0 ? r0 : (const T *)lev_r<T>(Lt);
         const T *tab = sizeof(T) == 4 ? (const T *)(const void *)c.dct_tab32 : (const T *)(const void *)c.dct_tab;

@@ -5,7 +5,7 @@ Workload (the one the headline metric is quoted on): unit-square cavity,
 2048 x 2048 cells, Re = 5000, lid u = 1, seeded divergence-free start, cfl 0.5
 (dt = 0.5 / (4 n^2)), momentum tolerance 1e-10, Poisson tolerance 1e-10.
 
-A "step" is one full time step (RK4 dual-time pseudo-iterations to
+A "step" is one full time step (pseudo-iterations of the momentum predictor to
 convergence, Poisson RHS, multigrid projection, update) over the whole grid.
 The state is resident in HBM before the timed region; W warm-up steps run
 first, then exactly K steps are timed between device synchronisations.
@@ -40,9 +40,9 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 # timing kinds of amressif_time_kernel -> kernel symbol
-KERNELS = {0: "amr::k_stage64 (FP64 stage)", 1: "amr::k_smooth<T, 1> (finest down-leg smoother + restriction)",
+KERNELS = {0: "amr::k_stage64 (FP64 stage)", 1: "amr::k_gsrb2<T, 1, R> (finest down-leg smoother + restriction)",
            3: "amr::k_stage64 (mixed residual)", 4: "amr::k_tangent",
-           5: "amr::k_smooth<T, 2> (finest up-leg smoother + FP64 outer step)"}
+           5: "amr::k_gsrb2<T, 2, R> (finest up-leg smoother + FP64 outer step)"}
 PMC_KEYS = {0: "stage_fp64", 3: "residual_mixed", 4: "tangent_mixed"}
 
 
@@ -83,17 +83,18 @@ def read_traffic(kernel_key: str):
 
 
 def step_bytes_fp64(n, pseudo_iters, vcycles):
-    """Algorithmic HBM bytes of one FP64 step at n^2 cells: 4 fused stages of
-    72 B/cell per pseudo-iteration; per V-cycle, on every level, the
-    pre-smoother from zero (r in, e out: 16 B), residual + restriction (e, r in:
-    16 B, coarse r out: 2 B) and the post-smoother (e, r, coarse e in, e out:
-    26 B), i.e. 60 B/cell times 4/3 for the hierarchy, plus the FP64 correction
-    and residual on the finest level (phi, e, b in; phi, r out: 40 B); once per
-    step BDF2 (u, uprev in; c, uprev out: 32 B), Poisson RHS (u, v in, b out:
-    24 B), centring (16 B) and the projection (u*, v*, phi, p in; u, v, p out:
-    56 B)."""
+    """Algorithmic HBM bytes of one FP64 step at n^2 cells, kernel by kernel:
+    4 fused RK4 stages of 72 B/cell per pseudo-iteration; per V-cycle the
+    finest down-leg smoother with the fused restriction (r in, e out, coarse
+    r out: 18 B), the finest up-leg smoother with the fused FP64 outer step
+    (e, r, coarse e, phi, b in; phi, next r out: 50 B) and the coarser levels
+    (down 18 B + up 26 B per cell of the level: 44/3 B per finest cell over the
+    hierarchy; the single-workgroup tail is negligible); once per step the
+    Poisson start (u*, v* in; b, phi0, r0 out: 40 B) and the projection with
+    the next step's BDF2 constant (u*, v*, phi, p, u^n in; u, v, p, c and the
+    copy of u out: 104 B)."""
     cells = float(n) * n
-    return cells * (pseudo_iters * 4 * 72 + vcycles * (60 * 4 / 3 + 40) + 32 + 24 + 16 + 56)
+    return cells * (pseudo_iters * 4 * 72 + vcycles * (18 + 50 + 44 / 3) + 40 + 104)
 
 
 def run_precision(api, W, n, ren, precision, state, warmup, steps, torch):

@@ -86,10 +86,11 @@ typedef struct {
                                    two-term Neumann series -(1/c)(I + R'/c) of the inverse Jacobian,
                                    c = 1.5/dt; 0: the RK4 dual-time recursion to first order in
                                    the Jacobian (the reference's pseudo-time structure) */
-    int poisson_warm_start;     /* the step's Poisson solve starts from the previous step's phi
-                                   instead of 0 (poisson.cpp:63); the stopping test stays
-                                   ||r||inf <= tol_rel ||b||inf.  -1 (default): on for precision = 1,
-                                   off for FP64; 0 off; 1 on */
+    int poisson_warm_start;     /* the step's Poisson solve starts from the linear extrapolation
+                                   2 phi^{n-1} - phi^{n-2} of the previous steps' solutions (phi^{n-1}
+                                   alone right after a state load) instead of 0 (poisson.cpp:63);
+                                   the stopping test stays ||r||inf <= tol_rel ||b||inf.
+                                   -1 (default): on for precision = 1, off for FP64; 0 off; 1 on */
 } amressif_inputs;
 
 /* AMReX Array4 view of one box: ptr addresses element (lo[0], lo[1], lo[2], comp 0);

@@ -105,9 +105,13 @@ __device__ __forceinline__ void finish_cells(const StageArgs &a, int lane, int i
 }
 
 // ============================================================ FP64 stage
+// MODE: the launch's a.mode (0 stage, 1 last stage + norm, 2 RHS only, 3 mixed
+// residual) as a template argument, so each form compiles to its own kernel
+template <int MODE>
 __global__ __launch_bounds__(256) void k_stage64(StageArgs a)
 {
     using R = double;
+    a.mode = MODE;
     if (a.ctl && a.ctl->mom_done) return;
     const int lane = threadIdx.x & 63;
     const int wid = wave_id(a);
@@ -403,7 +407,12 @@ hipError_t launch_stage(const Ctx &c, const StageLaunch &s, int *nblocks)
     const dim3 grid = stage_grid(a);
     if ((int64_t)a.nbands * a.nchunks > kMaxPartials) return hipErrorInvalidValue;
     if (nblocks) *nblocks = a.nbands * a.nchunks;  // one norm partial per wave
-    hipLaunchKernelGGL(k_stage64, grid, dim3(256), 0, c.stream, a);
+    switch (s.mode) {
+    case 0: hipLaunchKernelGGL(k_stage64<0>, grid, dim3(256), 0, c.stream, a); break;
+    case 1: hipLaunchKernelGGL(k_stage64<1>, grid, dim3(256), 0, c.stream, a); break;
+    case 2: hipLaunchKernelGGL(k_stage64<2>, grid, dim3(256), 0, c.stream, a); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
@@ -417,7 +426,7 @@ hipError_t launch_stage_mixed(const Ctx &c, const MixedStageLaunch &s, int *nblo
     a.uk = s.uk; a.vk = s.vk; a.ru = s.ru; a.rv = s.rv;
     if (s.stage == 0) {  // FP64 residual of the pseudo base
         a.us = s.u64; a.vs = s.v64; a.cu = s.cu; a.cv = s.cv; a.p = s.p;
-        hipLaunchKernelGGL(k_stage64, grid, dim3(256), 0, c.stream, a);
+        hipLaunchKernelGGL(k_stage64<3>, grid, dim3(256), 0, c.stream, a);
     } else {  // FP32 tangent-linear correction, base updated in place
         a.u64 = s.u64; a.v64 = s.v64; a.cr = s.cr; a.cj = s.cj;
         hipLaunchKernelGGL(k_tangent, grid, dim3(256), 0, c.stream, a);

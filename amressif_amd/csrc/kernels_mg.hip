@@ -771,16 +771,19 @@ __global__ __launch_bounds__(1024) void k_tail(const T *rtop, int ptop, T *etop,
 // the orthonormal DCT-II basis, L = Cy D Cx^T with D(m, k) = -ax (2 - 2 cos(pi
 // k / nx)) - ay (2 - 2 cos(pi m / ny)), so e = Cy (D^+ (Cy^T r Cx)) Cx^T, the
 // (0, 0) mode (the constant) set to zero as the pseudo-inverse does.  Four
-// 64 x 64 x 64 matrix products in one workgroup (zero padding to 64), each
-// thread a 4 x 4 block of the product; the cosine tables are staged in LDS
-// with a padded row stride against bank conflicts.
+// 64 x 64 x 64 matrix products in one workgroup (zero padding to 64): on the
+// matrix cores in FP32, on the VALU in FP64; the cosine tables are staged in
+// LDS with a padded row stride against bank conflicts.
 constexpr int kDct = 64, kDctLd = 65;
 
 // table layout (global): Cx [64 x 64], Cy [64 x 64], D^+ [64 (m) x 64 (k)]
+// O = op(A) op(B) (each 64 x 64 in LDS at stride kDctLd), times scale (global,
+// 64 x 64) elementwise when it is given.  FP64: VALU, each thread a 4 x 4 block.
 template <class T, bool TA, bool TB>
-__device__ __forceinline__ void dct_gemm(const T *A, const T *B, T (&acc)[4][4])
+__device__ __forceinline__ void dct_gemm(const T *A, const T *B, T *O, const T *scale)
 {
     const int t = threadIdx.x, a0 = 4 * (t >> 4), b0 = 4 * (t & 15);
+    T acc[4][4];
 #pragma unroll
     for (int r = 0; r < 4; ++r)
 #pragma unroll
@@ -797,34 +800,69 @@ __device__ __forceinline__ void dct_gemm(const T *A, const T *B, T (&acc)[4][4])
 #pragma unroll
             for (int q = 0; q < 4; ++q) acc[r][q] += av[r] * bv[q];
     }
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            O[(a0 + r) * kDctLd + b0 + q] = scale ? acc[r][q] * scale[(a0 + r) * kDct + b0 + q] : acc[r][q];
+}
+
+// FP32: matrix cores.  v_mfma_f32_16x16x4f32 multiplies a 16 x 4 block of A
+// (lane l holds A[l % 16][l / 16]) by a 4 x 16 block of B (lane l holds
+// B[l / 16][l % 16]) into a 16 x 16 block of D (lane l holds D[4 (l / 16) + v]
+// [l % 16], v = 0..3).  Wave w computes rows 16w..16w+15 of O as four such
+// blocks, stepping through k four at a time.
+typedef float dct_v4f __attribute__((ext_vector_type(4)));
+template <bool TA, bool TB>
+__device__ __forceinline__ void dct_gemm_mfma(const float *A, const float *B, float *O, const float *scale)
+{
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6, li = l & 15, lk = l >> 4;
+    dct_v4f acc[4];
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) acc[cb] = dct_v4f{0.f, 0.f, 0.f, 0.f};
+    const int ar = 16 * w + li;
+#pragma unroll 4
+    for (int k0 = 0; k0 < kDct; k0 += 4) {
+        const int kk = k0 + lk;
+        const float a = TA ? A[kk * kDctLd + ar] : A[ar * kDctLd + kk];
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) {
+            const int bc = 16 * cb + li;
+            const float b = TB ? B[bc * kDctLd + kk] : B[kk * kDctLd + bc];
+            acc[cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[cb], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int row = 16 * w + 4 * lk + v, col = 16 * cb + li;
+            O[row * kDctLd + col] = scale ? acc[cb][v] * scale[row * kDct + col] : acc[cb][v];
+        }
+}
+
+template <class T, bool TA, bool TB>
+__device__ __forceinline__ void dct_product(const T *A, const T *B, T *O, const T *scale)
+{
+    if constexpr (sizeof(T) == 4)
+        dct_gemm_mfma<TA, TB>((const float *)A, (const float *)B, (float *)O, (const float *)scale);
+    else
+        dct_gemm<T, TA, TB>(A, B, O, scale);
 }
 
 // the DCT solve of the workgroup: R in B0 (zero outside ny x nx), the tables
-// in Cx, Cy (LDS) and dinv (global); the 4 x 4 block of E of this thread in
-// acc.  B0 and B1 are overwritten.
+// in Cx, Cy (LDS) and dinv (global); E in B0 on return (B1 overwritten)
 template <class T>
-__device__ __forceinline__ void dct_solve(const T *Cx, const T *Cy, T *B0, T *B1, const T *dinv, T (&acc)[4][4])
+__device__ __forceinline__ void dct_solve(const T *Cx, const T *Cy, T *B0, T *B1, const T *dinv)
 {
-    const int t = threadIdx.x, a0 = 4 * (t >> 4), b0 = 4 * (t & 15);
-    dct_gemm<T, false, false>(B0, Cx, acc);  // T1 = R Cx
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) B1[(a0 + r) * kDctLd + b0 + q] = acc[r][q];
+    dct_product<T, false, false>(B0, Cx, B1, (const T *)nullptr);  // T1 = R Cx
     __syncthreads();
-    dct_gemm<T, true, false>(Cy, B1, acc);  // S = Cy^T T1, then S D^+
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) B0[(a0 + r) * kDctLd + b0 + q] = acc[r][q] * dinv[(a0 + r) * kDct + b0 + q];
+    dct_product<T, true, false>(Cy, B1, B0, dinv);  // S = (Cy^T T1) D^+
     __syncthreads();
-    dct_gemm<T, false, false>(Cy, B0, acc);  // T2 = Cy S
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) B1[(a0 + r) * kDctLd + b0 + q] = acc[r][q];
+    dct_product<T, false, false>(Cy, B0, B1, (const T *)nullptr);  // T2 = Cy S
     __syncthreads();
-    dct_gemm<T, false, true>(B1, Cx, acc);  // E = T2 Cx^T
+    dct_product<T, false, true>(B1, Cx, B0, (const T *)nullptr);  // E = T2 Cx^T
+    __syncthreads();
 }
 
 template <class T>
@@ -832,7 +870,7 @@ __global__ __launch_bounds__(256) void k_tail_dct(const T *rtop, int ptop, T *et
                                                   const Ctl *ctl)
 {
     __shared__ T Cx[kDct * kDctLd], Cy[kDct * kDctLd], B0[kDct * kDctLd], B1[kDct * kDctLd];
-    const int t = threadIdx.x, a0 = 4 * (t >> 4), b0 = 4 * (t & 15);
+    const int t = threadIdx.x;
     // loads first (the tables and r, zero-padded to 64 x 64); their latency overlaps the flag's
     T vx[16], vy[16], vr[16];
 #pragma unroll
@@ -851,15 +889,12 @@ __global__ __launch_bounds__(256) void k_tail_dct(const T *rtop, int ptop, T *et
         B0[row * kDctLd + col] = vr[q];
     }
     __syncthreads();
-    T acc[4][4];
-    dct_solve<T>(Cx, Cy, B0, B1, tab + 2 * kDct * kDct, acc);
+    dct_solve<T>(Cx, Cy, B0, B1, tab + 2 * kDct * kDct);
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int row = a0 + r, col = b0 + q;
-            if (row < ny && col < nx) etop[col + (size_t)ptop * row] = acc[r][q];
-        }
+    for (int q = 0; q < 16; ++q) {
+        const int e = t + 256 * q, row = e >> 6, col = e & 63;
+        if (row < ny && col < nx) etop[col + (size_t)ptop * row] = B0[row * kDctLd + col];
+    }
 }
 
 // The level above the DCT level (twice its size: ny1 = 4 R1 rows, nx1 <= 128
@@ -921,7 +956,7 @@ __global__ __launch_bounds__(256) void k_tail_dct_ext(const float *r1g, int p1, 
     __shared__ float Cx[kDct * kDctLd], Cy[kDct * kDctLd], B0[kDct * kDctLd], B1[kDct * kDctLd];
     __shared__ float xch[2][4][2][128];  // [buffer][wave][first, last own row][column]
     constexpr int ny1 = 4 * R1;
-    const int t = threadIdx.x, w = t >> 6, lane = t & 63, a0 = 4 * (t >> 4), b0 = 4 * (t & 15);
+    const int t = threadIdx.x, w = t >> 6, lane = t & 63;
     const int j0 = w * R1, ca = 2 * lane;
     const bool in_i = ca < nx1;
     // loads first: the tables and the own rows of r; their latency overlaps the flag's
@@ -993,14 +1028,7 @@ __global__ __launch_bounds__(256) void k_tail_dct_ext(const float *r1g, int p1, 
         if (in_i && I < nx && J < ny) B0[J * kDctLd + I] = 0.25f * (((t4[0] + t4[1]) + t4[2]) + t4[3]);
     }
     __syncthreads();
-    float acc[4][4];
-    dct_solve<float>(Cx, Cy, B0, B1, tab + 2 * kDct * kDct, acc);
-    __syncthreads();  // every wave is past the last product before B0 takes the correction
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) B0[(a0 + r) * kDctLd + b0 + q] = acc[r][q];
-    __syncthreads();
+    dct_solve<float>(Cx, Cy, B0, B1, tab + 2 * kDct * kDct);  // the coarse correction in B0
     // up leg: add the coarse correction (the halo rows too: they hold the
     // neighbours' down-leg values), two sweeps, store the own rows
 #pragma unroll

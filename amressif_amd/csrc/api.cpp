@@ -603,6 +603,7 @@ int amressif_create(const amressif_inputs *in, void *hip_stream, amressif_ctx **
     if (const char *e = std::getenv("AMRESSIF_TANGENT_ROWS"))
         c.tangent_rows = std::max(4, std::min(256, std::atoi(e)));
     if (const char *e = std::getenv("AMRESSIF_STAGE_ORDER")) c.stage_order = std::atoi(e) ? 1 : 0;
+    if (const char *e = std::getenv("AMRESSIF_STAGE_ALT")) c.stage_alt = std::atoi(e) ? 1 : 0;
     // and of the multigrid smoothers (kernel choice and rows per wave: 4, 8, 16 or 32)
     if (const char *e = std::getenv("AMRESSIF_MG_SMOOTHER")) c.mg_smoother = std::max(0, std::min(2, std::atoi(e)));
     if (const char *e = std::getenv("AMRESSIF_TAIL_DCT")) c.tail_dct = std::atoi(e) ? 1 : 0;

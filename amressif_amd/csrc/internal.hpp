@@ -102,6 +102,7 @@ struct Ctx {
     int stage_rows = 8;      // k_stage64: output rows per wave
     int tangent_rows = 16;   // k_tangent: output rows per wave
     int stage_order = 1;     // stage kernels: 1 = consecutive waves take horizontally adjacent bands
+    int stage_alt = 1;       // k_stage64: odd row chunks march down
     // multigrid smoothers of the levels above the tail: 2 = wave-column kernel
     // with two columns per lane, 1 = one column per lane (both registers +
     // DPP), 0 = LDS tile kernel; output rows per wave of the wave-column

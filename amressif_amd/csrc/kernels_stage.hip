@@ -56,6 +56,7 @@ struct StageArgs {
     int nbands, nchunks, remap;
     int rows;   // output rows per wave (chunk height)
     int order;  // wave -> tile: 0 band-major (chunks of one band consecutive), 1 chunk-major
+    int alt;    // k_stage64: odd chunks march down (shared halo rows read at the same time)
 };
 
 // wave id -> (band, chunk); blocks of 4 waves, handed to the 8 XCDs in
@@ -149,6 +150,125 @@ __global__ __launch_bounds__(256) void k_stage64(StageArgs a)
     };
     const R idx = a.inv_dx, idy = a.inv_dy, idx2 = a.inv_dx2, idy2 = a.inv_dy2, iren = a.inv_ren;
     R wU[5], wV[5];  // rolling window: w[q] = cell (i, jc - 2 + q)
+    const int jend = min(j0 + a.rows, n);
+    if (a.alt && (chunk & 1)) {
+        // Odd chunks march down (even ones up): the rows a chunk shares with
+        // its neighbours as stencil halo are then read by both at about the same
+        // time (both at their start or both at their end), so the second read
+        // is served by L2.  Same arithmetic per face as the upward march; the
+        // y-face (i, jc + 1) is written at row jc from the values carried from
+        // row jc + 1.
+        struct RawD {
+            R u0, u1, v0, v1, vs, uw, pc, ub, cu, vb, cv;
+        };
+        auto cells = [&](int jr, RawD &r) {  // face values of cell row jr (its mirror row outside)
+            const int jm = jr < 0 ? -jr - 1 : (jr >= n ? 2 * n - 1 - jr : jr);
+            r.u0 = r.u1 = r.v0 = r.v1 = 0;
+            if (jm >= 0 && jm < n && xin) {
+                const size_t k = i + (size_t)P * jm;
+                r.u0 = us[k]; r.u1 = us[k + 1]; r.v0 = vs[k]; r.v1 = vs[k + P];
+            }
+        };
+        auto loadd = [&](int jc, RawD &r) {  // everything iteration jc consumes
+            cells(jc - 2, r);
+            r.vs = (xin && jc >= 0 && jc <= n) ? vs[i + (size_t)P * jc] : R(0);  // south y-face of row jc
+            const bool row = jc >= 0 && jc < n;
+            const size_t k = (size_t)(i < 0 ? 0 : i) + (size_t)P * (jc < 0 ? 0 : jc);
+            r.uw = (xface && row) ? us[k] : R(0);
+            r.pc = (xin && row) ? a.p[k] : R(0);
+            const bool out = row && jc >= j0 && MODE != 2 && outlane;
+            r.ub = (out && xface && MODE != 3) ? a.ub[k] : R(0);
+            r.cu = (out && xface) ? a.cu[k] : R(0);
+            r.vb = (out && xin && MODE != 3) ? a.vb[k] : R(0);
+            r.cv = (out && xin) ? a.cv[k] : R(0);
+        };
+        {
+            RawD r;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {  // rows jend + 1 .. jend - 2 into w[4] .. w[1]
+                cells(jend + 1 - q, r);
+                finish_cells<R, false>(a, lane, i, jend + 1 - q, edge, r.u0, r.u1, r.v0, r.v1, wU[4 - q], wV[4 - q]);
+            }
+        }
+        const R vtop = (xin && jend <= n) ? vs[i + (size_t)P * jend] : R(0);  // y-face (i, jend)
+        R fnU = quick_flux<R>(vtop, wU[1], wU[2], wU[3], wU[4]), fnV = quick_flux<R>(vtop, wV[1], wV[2], wV[3], wV[4]);
+        R tVup = 0, pup = 0, vup = vtop, vbup = 0, cvup = 0;  // row jc + 1's TV, p, south-face v, base, constant
+        double nx_ = 0.0, ny_ = 0.0;
+        RawD cur, nxt;
+        loadd(jend - 1, cur);
+        for (int jc = jend - 1; jc >= j0 - 1; --jc) {
+            if (jc - 1 >= j0 - 1) loadd(jc - 1, nxt);  // prefetch
+            finish_cells<R, false>(a, lane, i, jc - 2, edge, cur.u0, cur.u1, cur.v0, cur.v1, wU[0], wV[0]);
+            const R vsf = cur.vs;
+            const R fsU = quick_flux<R>(vsf, wU[0], wU[1], wU[2], wU[3]);
+            const R fsV = quick_flux<R>(vsf, wV[0], wV[1], wV[2], wV[3]);
+            const R Uc = wU[2], Vc = wV[2];
+            const R Um1 = lane_prev(Uc), Um2 = lane_prev(Um1), Up1 = lane_next(Uc);
+            const R Vm1 = lane_prev(Vc), Vm2 = lane_prev(Vm1), Vp1 = lane_next(Vc);
+            const R uw = cur.uw;
+            const R fwU = quick_flux<R>(uw, Um2, Um1, Uc, Up1), fwV = quick_flux<R>(uw, Vm2, Vm1, Vc, Vp1);
+            const R feU = lane_next(fwU), feV = lane_next(fwV);
+            const R CU = (feU - fwU) * idx + (fnU - fsU) * idy;
+            const R CV = (feV - fwV) * idx + (fnV - fsV) * idy;
+            const R VU = ((Up1 - R(2) * Uc + Um1) * idx2 + (wU[3] - R(2) * Uc + wU[1]) * idy2) * iren;
+            const R VV = ((Vp1 - R(2) * Vc + Vm1) * idx2 + (wV[3] - R(2) * Vc + wV[1]) * idy2) * iren;
+            const R TU = (R(0) - CU) + VU, TV = (R(0) - CV) + VV;
+            const R TUw = lane_prev(TU);
+            const R pc = cur.pc, pw = lane_prev(pc);
+            if (jc >= j0 && outlane && xface) {  // x-face (i, jc); wall faces get R = 0 exactly
+                const size_t k = (size_t)(i < 0 ? 0 : i) + (size_t)P * jc;
+                const bool wall = i == 0 || i == n;
+                const double Rx = wall ? 0.0 : -(pc - pw) * a.inv_dx + R(0.5) * (TUw + TU);
+                if (MODE == 2) a.uo[k] = Rx;
+                else if (MODE == 3) {
+                    const double r = Rx - a.c15 * uw + cur.cu;
+                    a.uk[k] = (float)uw;
+                    a.ru[k] = (float)r;
+                } else {
+                    const double unew = cur.ub + a.rk * (Rx - a.c15 * uw + cur.cu);
+                    a.uo[k] = unew;
+                    if (MODE == 1) { const double d = cur.ub - unew; nx_ += fabs(d * d * (wall ? 0.5 : 1.0)); }
+                }
+            }
+            if (jc + 1 < jend && jc + 1 >= j0 && outlane && xin) {  // y-face (i, jc + 1)
+                const size_t k = i + (size_t)P * (jc + 1);
+                const bool wall = jc + 1 == 0;
+                const double Ry = wall ? 0.0 : -(pup - pc) * a.inv_dy + R(0.5) * (TV + tVup);
+                if (MODE == 2) a.vo[k] = Ry;
+                else if (MODE == 3) {
+                    const double r = Ry - a.c15 * vup + cvup;
+                    a.vk[k] = (float)vup;
+                    a.rv[k] = (float)r;
+                } else {
+                    const double vnew = vbup + a.rk * (Ry - a.c15 * vup + cvup);
+                    a.vo[k] = vnew;
+                    if (MODE == 1) { const double d = vbup - vnew; ny_ += fabs(d * d * (wall ? 0.5 : 1.0)); }
+                }
+            }
+            fnU = fsU; fnV = fsV; tVup = TV; pup = pc; vup = vsf; vbup = cur.vb; cvup = cur.cv;
+#pragma unroll
+            for (int q = 4; q > 0; --q) { wU[q] = wU[q - 1]; wV[q] = wV[q - 1]; }
+            cur = nxt;
+        }
+        if (jend == n && outlane && xin) {  // north wall row of y-faces: R = 0
+            const size_t k = i + (size_t)P * n;
+            if (MODE == 2) a.vo[k] = 0.0;
+            else if (MODE == 3) {
+                a.vk[k] = (float)vtop;
+                a.rv[k] = (float)(-a.c15 * vtop + a.cv[k]);
+            } else {
+                const double base = a.vb[k];
+                const double vnew = base + a.rk * (0.0 - a.c15 * vtop + a.cv[k]);
+                a.vo[k] = vnew;
+                if (MODE == 1) { const double d = base - vnew; ny_ += fabs(d * d * 0.5); }
+            }
+        }
+        if (MODE == 1) {
+            for (int o = 32; o > 0; o >>= 1) { nx_ += __shfl_down(nx_, o, 64); ny_ += __shfl_down(ny_, o, 64); }
+            if (lane == 0) { a.partials[2 * wid] = nx_; a.partials[2 * wid + 1] = ny_; }
+        }
+        return;
+    }
     {
         Raw r;
 #pragma unroll
@@ -161,7 +281,6 @@ __global__ __launch_bounds__(256) void k_stage64(StageArgs a)
     R fsU = quick_flux<R>(vn, wU[0], wU[1], wU[2], wU[3]), fsV = quick_flux<R>(vn, wV[0], wV[1], wV[2], wV[3]);
     R tVprev = 0, pprev = 0;
     double nx_ = 0.0, ny_ = 0.0;
-    const int jend = min(j0 + a.rows, n);
     Raw cur, nxt;
     load(j0 - 1, cur);
     for (int jc = j0 - 1; jc < jend; ++jc) {
@@ -385,6 +504,7 @@ static StageArgs base_args(const Ctx &c, double rk, int mode, int rows)
     a.mode = mode;
     a.rows = rows;
     a.order = c.stage_order;
+    a.alt = c.stage_alt;
     a.nbands = (g.n + 1 + WB - 1) / WB;
     a.nchunks = (g.n + a.rows - 1) / a.rows;
     return a;

@@ -13,8 +13,10 @@
 // rolling 5-row window of cell velocities stays in registers (the y-direction
 // QUICK stencil); x-direction neighbours come from adjacent lanes through DPP
 // wave shifts (no LDS, no barriers).  Each face flux is evaluated once; the
-// next row's global loads are issued one iteration ahead.  Walls: ghost cells
-// come from their mirror lane (x) or mirror row (y) with the cont2cart rules.
+// next row's global loads are issued one iteration ahead and (k_stage64) a
+// row's stores one iteration behind, so each row waits for memory once.
+// Walls: ghost cells come from their mirror lane (x) or mirror row (y) with
+// the cont2cart rules.
 //
 // Precision forms:
 //  * FP64 (the reference's): u* <- u^k + rk (R(u*) - (1.5/dt) u* + c), with the
@@ -74,9 +76,11 @@ __device__ __forceinline__ int wave_id(const StageArgs &a)
 // cell velocities of row jr for this lane's column from raw face loads of the
 // mirror row, with the x ghosts from the mirror lane and the y ghost rule.
 // Homog: the rules without their constant part (an increment of the velocity).
+// gx: the lid (or inflow) velocity of this lane's column, loaded once per wave
+// so that the row loops hold no loads besides their prefetch.
 template <class R, bool Homog>
 __device__ __forceinline__ void finish_cells(const StageArgs &a, int lane, int i, int jr, bool edge, R u0, R u1,
-                                             R v0, R v1, R &U, R &V)
+                                             R v0, R v1, R gx, R &U, R &V)
 {
     const int n = a.n;
     const int jm = jr < 0 ? -jr - 1 : (jr >= n ? 2 * n - 1 - jr : jr);
@@ -97,7 +101,6 @@ __device__ __forceinline__ void finish_cells(const StageArgs &a, int lane, int i
         }
     }
     if (jr != jm && xin) {
-        const R gx = (R)(a.lid ? a.lid[i] : a.inflow0);
         R gu, gv;
         ghost_rule<R, Homog>(jr < 0 ? 2 : 3, jr < 0 ? a.bc_lo1 : a.bc_hi1, U, V, gx, (R)a.inflow1, gu, gv);
         U = gu;
@@ -126,16 +129,18 @@ __global__ __launch_bounds__(256) void k_stage64(StageArgs a)
     const bool edge = i0 < 3 || i0 + 61 > n;
     const bool outlane = lane >= 3 && lane < 3 + WB;
     const double *us = a.us, *vs = a.vs;
+    const R gx = (xin && a.lid) ? a.lid[i] : a.inflow0;
     struct Raw {
-        R u0, u1, v0, v1, vn, uw, pc, ub, cu, vb, cv;
+        R u0, v0, v1, vn, uw, pc, ub, cu, vb, cv;  // u(i + 1) = u0 of lane + 1
     };
     auto load = [&](int jc, Raw &r) {  // everything iteration jc consumes
         const int jr = jc + 2;
         const int jm = jr < 0 ? -jr - 1 : (jr >= n ? 2 * n - 1 - jr : jr);
-        r.u0 = r.u1 = r.v0 = r.v1 = 0;
-        if (jm >= 0 && jm < n && xin) {
+        r.u0 = r.v0 = r.v1 = 0;
+        if (jm >= 0 && jm < n && xface) {
             const size_t k = i + (size_t)P * jm;
-            r.u0 = us[k]; r.u1 = us[k + 1]; r.v0 = vs[k]; r.v1 = vs[k + P];
+            r.u0 = us[k];
+            if (xin) { r.v0 = vs[k]; r.v1 = vs[k + P]; }
         }
         r.vn = (xin && jc + 1 >= 0 && jc + 1 <= n) ? vs[i + (size_t)P * (jc + 1)] : R(0);
         const bool row = jc >= 0 && jc < n;
@@ -159,14 +164,15 @@ __global__ __launch_bounds__(256) void k_stage64(StageArgs a)
         // y-face (i, jc + 1) is written at row jc from the values carried from
         // row jc + 1.
         struct RawD {
-            R u0, u1, v0, v1, vs, uw, pc, ub, cu, vb, cv;
+            R u0, v0, v1, vs, uw, pc, ub, cu, vb, cv;
         };
         auto cells = [&](int jr, RawD &r) {  // face values of cell row jr (its mirror row outside)
             const int jm = jr < 0 ? -jr - 1 : (jr >= n ? 2 * n - 1 - jr : jr);
-            r.u0 = r.u1 = r.v0 = r.v1 = 0;
-            if (jm >= 0 && jm < n && xin) {
+            r.u0 = r.v0 = r.v1 = 0;
+            if (jm >= 0 && jm < n && xface) {
                 const size_t k = i + (size_t)P * jm;
-                r.u0 = us[k]; r.u1 = us[k + 1]; r.v0 = vs[k]; r.v1 = vs[k + P];
+                r.u0 = us[k];
+                if (xin) { r.v0 = vs[k]; r.v1 = vs[k + P]; }
             }
         };
         auto loadd = [&](int jc, RawD &r) {  // everything iteration jc consumes
@@ -187,18 +193,37 @@ __global__ __launch_bounds__(256) void k_stage64(StageArgs a)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {  // rows jend + 1 .. jend - 2 into w[4] .. w[1]
                 cells(jend + 1 - q, r);
-                finish_cells<R, false>(a, lane, i, jend + 1 - q, edge, r.u0, r.u1, r.v0, r.v1, wU[4 - q], wV[4 - q]);
+                finish_cells<R, false>(a, lane, i, jend + 1 - q, edge, r.u0, lane_next(r.u0), r.v0, r.v1, gx, wU[4 - q],
+                                       wV[4 - q]);
             }
         }
         const R vtop = (xin && jend <= n) ? vs[i + (size_t)P * jend] : R(0);  // y-face (i, jend)
         R fnU = quick_flux<R>(vtop, wU[1], wU[2], wU[3], wU[4]), fnV = quick_flux<R>(vtop, wV[1], wV[2], wV[3], wV[4]);
         R tVup = 0, pup = 0, vup = vtop, vbup = 0, cvup = 0;  // row jc + 1's TV, p, south-face v, base, constant
         double nx_ = 0.0, ny_ = 0.0;
+        // deferred stores, as in the upward march below: x-face (i, jc) and
+        // y-face (i, jc + 1) of row jc go out at the top of iteration jc - 1
+        unsigned skx = 0, sky = 0;  // n <= 16384: every index fits 32 bits
+        bool sxf = false, syf = false;
+        double sx = 0.0, sy = 0.0;
+        float fx0 = 0.f, fx1 = 0.f, fy0 = 0.f, fy1 = 0.f;
+        auto flush = [&]() {
+            if (sxf) {
+                if (MODE == 3) { a.uk[skx] = fx0; a.ru[skx] = fx1; }
+                else a.uo[skx] = sx;
+            }
+            if (syf) {
+                if (MODE == 3) { a.vk[sky] = fy0; a.rv[sky] = fy1; }
+                else a.vo[sky] = sy;
+            }
+        };
         RawD cur, nxt;
         loadd(jend - 1, cur);
         for (int jc = jend - 1; jc >= j0 - 1; --jc) {
             if (jc - 1 >= j0 - 1) loadd(jc - 1, nxt);  // prefetch
-            finish_cells<R, false>(a, lane, i, jc - 2, edge, cur.u0, cur.u1, cur.v0, cur.v1, wU[0], wV[0]);
+            flush();
+            sxf = syf = false;
+            finish_cells<R, false>(a, lane, i, jc - 2, edge, cur.u0, lane_next(cur.u0), cur.v0, cur.v1, gx, wU[0], wV[0]);
             const R vsf = cur.vs;
             const R fsU = quick_flux<R>(vsf, wU[0], wU[1], wU[2], wU[3]);
             const R fsV = quick_flux<R>(vsf, wV[0], wV[1], wV[2], wV[3]);
@@ -216,32 +241,32 @@ __global__ __launch_bounds__(256) void k_stage64(StageArgs a)
             const R TUw = lane_prev(TU);
             const R pc = cur.pc, pw = lane_prev(pc);
             if (jc >= j0 && outlane && xface) {  // x-face (i, jc); wall faces get R = 0 exactly
-                const size_t k = (size_t)(i < 0 ? 0 : i) + (size_t)P * jc;
+                skx = (unsigned)(i < 0 ? 0 : i) + (unsigned)P * jc;
                 const bool wall = i == 0 || i == n;
                 const double Rx = wall ? 0.0 : -(pc - pw) * a.inv_dx + R(0.5) * (TUw + TU);
-                if (MODE == 2) a.uo[k] = Rx;
+                sxf = true;
+                if (MODE == 2) sx = Rx;
                 else if (MODE == 3) {
-                    const double r = Rx - a.c15 * uw + cur.cu;
-                    a.uk[k] = (float)uw;
-                    a.ru[k] = (float)r;
+                    fx0 = (float)uw;
+                    fx1 = (float)(Rx - a.c15 * uw + cur.cu);
                 } else {
                     const double unew = cur.ub + a.rk * (Rx - a.c15 * uw + cur.cu);
-                    a.uo[k] = unew;
+                    sx = unew;
                     if (MODE == 1) { const double d = cur.ub - unew; nx_ += fabs(d * d * (wall ? 0.5 : 1.0)); }
                 }
             }
             if (jc + 1 < jend && jc + 1 >= j0 && outlane && xin) {  // y-face (i, jc + 1)
-                const size_t k = i + (size_t)P * (jc + 1);
+                sky = (unsigned)i + (unsigned)P * (jc + 1);
                 const bool wall = jc + 1 == 0;
                 const double Ry = wall ? 0.0 : -(pup - pc) * a.inv_dy + R(0.5) * (TV + tVup);
-                if (MODE == 2) a.vo[k] = Ry;
+                syf = true;
+                if (MODE == 2) sy = Ry;
                 else if (MODE == 3) {
-                    const double r = Ry - a.c15 * vup + cvup;
-                    a.vk[k] = (float)vup;
-                    a.rv[k] = (float)r;
+                    fy0 = (float)vup;
+                    fy1 = (float)(Ry - a.c15 * vup + cvup);
                 } else {
                     const double vnew = vbup + a.rk * (Ry - a.c15 * vup + cvup);
-                    a.vo[k] = vnew;
+                    sy = vnew;
                     if (MODE == 1) { const double d = vbup - vnew; ny_ += fabs(d * d * (wall ? 0.5 : 1.0)); }
                 }
             }
@@ -250,6 +275,7 @@ __global__ __launch_bounds__(256) void k_stage64(StageArgs a)
             for (int q = 4; q > 0; --q) { wU[q] = wU[q - 1]; wV[q] = wV[q - 1]; }
             cur = nxt;
         }
+        flush();
         if (jend == n && outlane && xin) {  // north wall row of y-faces: R = 0
             const size_t k = i + (size_t)P * n;
             if (MODE == 2) a.vo[k] = 0.0;
@@ -274,18 +300,40 @@ __global__ __launch_bounds__(256) void k_stage64(StageArgs a)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             load(j0 - 5 + q, r);
-            finish_cells<R, false>(a, lane, i, j0 - 3 + q, edge, r.u0, r.u1, r.v0, r.v1, wU[q], wV[q]);
+            finish_cells<R, false>(a, lane, i, j0 - 3 + q, edge, r.u0, lane_next(r.u0), r.v0, r.v1, gx, wU[q], wV[q]);
         }
     }
     R vn = (xin && j0 - 1 >= 0) ? vs[i + (size_t)P * (j0 - 1)] : R(0);  // own y-face of the row
     R fsU = quick_flux<R>(vn, wU[0], wU[1], wU[2], wU[3]), fsV = quick_flux<R>(vn, wV[0], wV[1], wV[2], wV[3]);
     R tVprev = 0, pprev = 0;
     double nx_ = 0.0, ny_ = 0.0;
+    // Row jc's stores are issued at the top of iteration jc + 1, right after
+    // that iteration's prefetch.  With loads and stores both in flight the
+    // compiler waits for all vector memory operations at once, so the loop is
+    // laid out to need one such wait per row (before the row's arithmetic
+    // reads the prefetched values), and the loads and stores of a row overlap
+    // a full row of arithmetic.
+    unsigned sk = 0;  // n <= 16384: every index fits 32 bits
+    bool sxf = false, syf = false;
+    double sx = 0.0, sy = 0.0;                  // MODE 0-2: the face values
+    float fx0 = 0.f, fx1 = 0.f, fy0 = 0.f, fy1 = 0.f;  // MODE 3: FP32 base and residual
+    auto flush = [&]() {
+        if (sxf) {
+            if (MODE == 3) { a.uk[sk] = fx0; a.ru[sk] = fx1; }
+            else a.uo[sk] = sx;
+        }
+        if (syf) {
+            if (MODE == 3) { a.vk[sk] = fy0; a.rv[sk] = fy1; }
+            else a.vo[sk] = sy;
+        }
+    };
     Raw cur, nxt;
     load(j0 - 1, cur);
     for (int jc = j0 - 1; jc < jend; ++jc) {
         if (jc + 1 < jend) load(jc + 1, nxt);  // prefetch
-        finish_cells<R, false>(a, lane, i, jc + 2, edge, cur.u0, cur.u1, cur.v0, cur.v1, wU[4], wV[4]);
+        flush();
+        sxf = syf = false;
+        finish_cells<R, false>(a, lane, i, jc + 2, edge, cur.u0, lane_next(cur.u0), cur.v0, cur.v1, gx, wU[4], wV[4]);
         const R vnf = cur.vn;
         const R fnU = quick_flux<R>(vnf, wU[1], wU[2], wU[3], wU[4]);
         const R fnV = quick_flux<R>(vnf, wV[1], wV[2], wV[3], wV[4]);
@@ -303,33 +351,33 @@ __global__ __launch_bounds__(256) void k_stage64(StageArgs a)
         const R TUw = lane_prev(TU);
         const R pc = cur.pc, pw = lane_prev(pc);
         if (jc >= j0 && outlane) {
-            const size_t k = (size_t)(i < 0 ? 0 : i) + (size_t)P * jc;
+            sk = (unsigned)(i < 0 ? 0 : i) + (unsigned)P * jc;
             if (xface) {  // x-face (i, jc); wall faces get R = 0 exactly
                 const bool wall = i == 0 || i == n;
                 const double Rx = wall ? 0.0 : -(pc - pw) * a.inv_dx + R(0.5) * (TUw + TU);
-                if (a.mode == 2) a.uo[k] = Rx;
-                else if (a.mode == 3) {
-                    const double r = Rx - a.c15 * uw + cur.cu;
-                    a.uk[k] = (float)uw;
-                    a.ru[k] = (float)r;
+                sxf = true;
+                if (MODE == 2) sx = Rx;
+                else if (MODE == 3) {
+                    fx0 = (float)uw;
+                    fx1 = (float)(Rx - a.c15 * uw + cur.cu);
                 } else {
                     const double unew = cur.ub + a.rk * (Rx - a.c15 * uw + cur.cu);
-                    a.uo[k] = unew;
-                    if (a.mode == 1) { const double d = cur.ub - unew; nx_ += fabs(d * d * (wall ? 0.5 : 1.0)); }
+                    sx = unew;
+                    if (MODE == 1) { const double d = cur.ub - unew; nx_ += fabs(d * d * (wall ? 0.5 : 1.0)); }
                 }
             }
             if (xin) {  // y-face (i, jc)
                 const bool wall = jc == 0;
                 const double Ry = wall ? 0.0 : -(pc - pprev) * a.inv_dy + R(0.5) * (tVprev + TV);
-                if (a.mode == 2) a.vo[k] = Ry;
-                else if (a.mode == 3) {
-                    const double r = Ry - a.c15 * vn + cur.cv;
-                    a.vk[k] = (float)vn;
-                    a.rv[k] = (float)r;
+                syf = true;
+                if (MODE == 2) sy = Ry;
+                else if (MODE == 3) {
+                    fy0 = (float)vn;
+                    fy1 = (float)(Ry - a.c15 * vn + cur.cv);
                 } else {
                     const double vnew = cur.vb + a.rk * (Ry - a.c15 * vn + cur.cv);
-                    a.vo[k] = vnew;
-                    if (a.mode == 1) { const double d = cur.vb - vnew; ny_ += fabs(d * d * (wall ? 0.5 : 1.0)); }
+                    sy = vnew;
+                    if (MODE == 1) { const double d = cur.vb - vnew; ny_ += fabs(d * d * (wall ? 0.5 : 1.0)); }
                 }
             }
         }
@@ -338,6 +386,7 @@ __global__ __launch_bounds__(256) void k_stage64(StageArgs a)
         for (int q = 0; q < 4; ++q) { wU[q] = wU[q + 1]; wV[q] = wV[q + 1]; }
         cur = nxt;
     }
+    flush();
     if (jend == n && outlane && xin) {  // north wall row of y-faces: R = 0
         const size_t k = i + (size_t)P * n;
         if (a.mode == 2) a.vo[k] = 0.0;
@@ -389,8 +438,11 @@ __global__ __launch_bounds__(256) void k_tangent(StageArgs a)
     const bool xin = i >= 0 && i < n, xface = i >= 0 && i <= n;
     const bool edge = i0 < 3 || i0 + 61 > n;
     const float *uk = a.uk, *vk = a.vk, *ru = a.ru, *rv = a.rv;
+    const R gx = (R)((xin && a.lid) ? a.lid[i] : a.inflow0);
+    const bool outlane = lane >= 3 && lane < 3 + WB;
     struct Raw {
         R u0, u1, v0, v1, d0, d1, e0, e1, vn, dvn, uw, duw;
+        double ub, vb;  // FP64 base of the row's own faces, updated below
     };
     auto load = [&](int jc, Raw &r) {
         const int jr = jc + 2;
@@ -416,8 +468,8 @@ __global__ __launch_bounds__(256) void k_tangent(StageArgs a)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             load(j0 - 5 + q, r);
-            finish_cells<R, false>(a, lane, i, j0 - 3 + q, edge, r.u0, r.u1, r.v0, r.v1, wU[q], wV[q]);
-            finish_cells<R, true>(a, lane, i, j0 - 3 + q, edge, r.d0, r.d1, r.e0, r.e1, dU[q], dV[q]);
+            finish_cells<R, false>(a, lane, i, j0 - 3 + q, edge, r.u0, r.u1, r.v0, r.v1, gx, wU[q], wV[q]);
+            finish_cells<R, true>(a, lane, i, j0 - 3 + q, edge, r.d0, r.d1, r.e0, r.e1, R(0), dU[q], dV[q]);
         }
     }
     const bool v0ok = xin && j0 - 1 >= 0;
@@ -426,15 +478,14 @@ __global__ __launch_bounds__(256) void k_tangent(StageArgs a)
     R fsV = quick_tl(vn, dvn, wV[0], wV[1], wV[2], wV[3], dV[0], dV[1], dV[2], dV[3]);
     R tVprev = 0;
     double nx_ = 0.0, ny_ = 0.0;
-    const bool outlane = lane >= 3 && lane < 3 + WB;
     const int jend = min(j0 + a.rows, n);
     const double cr = a.cr, cj = a.cj;
     Raw cur, nxt;
     load(j0 - 1, cur);
     for (int jc = j0 - 1; jc < jend; ++jc) {
         if (jc + 1 < jend) load(jc + 1, nxt);
-        finish_cells<R, false>(a, lane, i, jc + 2, edge, cur.u0, cur.u1, cur.v0, cur.v1, wU[4], wV[4]);
-        finish_cells<R, true>(a, lane, i, jc + 2, edge, cur.d0, cur.d1, cur.e0, cur.e1, dU[4], dV[4]);
+        finish_cells<R, false>(a, lane, i, jc + 2, edge, cur.u0, cur.u1, cur.v0, cur.v1, gx, wU[4], wV[4]);
+        finish_cells<R, true>(a, lane, i, jc + 2, edge, cur.d0, cur.d1, cur.e0, cur.e1, R(0), dU[4], dV[4]);
         const R fnU = quick_tl(cur.vn, cur.dvn, wU[1], wU[2], wU[3], wU[4], dU[1], dU[2], dU[3], dU[4]);
         const R fnV = quick_tl(cur.vn, cur.dvn, wV[1], wV[2], wV[3], wV[4], dV[1], dV[2], dV[3], dV[4]);
         const R Uc = wU[2], Vc = wV[2], dUc = dU[2], dVc = dV[2];

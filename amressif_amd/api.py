@@ -152,6 +152,9 @@ _sig = {
     "amressif_project": (_c.c_int, [_c.c_void_p, _A4, _A4, _A4, _P(_c.c_double)]),
     "amressif_time_kernel": (_c.c_int, [_c.c_void_p, _c.c_int, _c.c_int, _c.c_int64, _P(_c.c_double),
                                         _P(_c.c_double)]),
+    "amressif_kernel_timing": (_c.c_int, [_c.c_void_p, _c.c_int]),
+    "amressif_kernel_time": (_c.c_int, [_c.c_void_p, _c.c_int, _P(_c.c_double), _P(_c.c_int64),
+                                        _P(_c.c_double)]),
     "amressif_tape_create": (_c.c_int, [_c.c_void_p, _c.c_int, _P(_c.c_void_p)]),
     "amressif_tape_destroy": (_c.c_int, [_c.c_void_p]),
     "amressif_tape_reset": (_c.c_int, [_c.c_void_p]),
@@ -333,6 +336,22 @@ class Context:
         ms, b = ctypes.c_double(), ctypes.c_double()
         self._check(_lib.amressif_time_kernel(self._h, which, reps, copy_bytes, ctypes.byref(ms), ctypes.byref(b)))
         return ms.value, b.value
+
+    # in-step kernel timing (HIP events around the step's main launches)
+    KERNELS = ("stage", "stage_last", "resid", "tangent", "mg_down", "mg_up", "project", "poisson_start")
+
+    def kernel_timing(self, enable: bool) -> None:
+        """start (clearing the totals) or stop per-launch timing inside advance()"""
+        self._check(_lib.amressif_kernel_timing(self._h, 1 if enable else 0))
+
+    def kernel_times(self) -> dict:
+        """{kernel: (average ms per counted launch, counted launches, algorithmic bytes per launch)}"""
+        out = {}
+        for k, name in enumerate(self.KERNELS):
+            ms, cnt, nb = ctypes.c_double(), ctypes.c_int64(), ctypes.c_double()
+            self._check(_lib.amressif_kernel_time(self._h, k, ctypes.byref(ms), ctypes.byref(cnt), ctypes.byref(nb)))
+            out[name] = (ms.value, cnt.value, nb.value)
+        return out
 
     # ---- discrete adjoint (forward/backward pair)
     def tape(self, max_steps: int) -> "Tape":

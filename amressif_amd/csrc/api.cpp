@@ -180,6 +180,7 @@ int poll(amressif_ctx *ctx)
     Ctx &c = ctx->c;
     HIPCHK(hipMemcpyAsync(c.ctl_host, c.ctl, sizeof(amr::Ctl), hipMemcpyDeviceToHost, c.stream));
     HIPCHK(hipStreamSynchronize(c.stream));
+    HIPCHK(amr::kprof_collect(c));
     return 0;
 }
 
@@ -303,12 +304,21 @@ int mg_run(amressif_ctx *ctx, const WarmStart &w, bool center, const double *us,
     const int mixed = c.in.precision == AMRESSIF_MIXED;
     int nb = 0;
     HIPCHK(amr::launch_mg_start(c, require_mom ? 1 : 0));
+    {
+        // per cell: u*, v* in (or b), the warm-start iterates in, b (when formed or
+        // centred), phi0 and the residual (V-cycle precision) out
+        const double t = mixed ? 4.0 : 8.0, warm = w.mode == 2 ? 16.0 : w.mode == 1 ? 8.0 : 0.0;
+        const double per = (us ? 16.0 : 8.0) + warm + (us || center ? 8.0 : 0.0) + 8.0 + t;
+        HIPCHK(amr::kprof_begin(c, amr::KP_PSTART, amr::KL_PHASE, 0, per * (double)c.g.n * c.g.n));
+    }
     HIPCHK(amr::launch_poisson_start(c, w.mode, center ? 1 : 0, w.phi1, w.phi2, c.phiv[0], mixed, us, vs));
+    HIPCHK(amr::kprof_end(c));
     int launched = 0, target = std::max(1, c.last_V);
     const int cap = std::max(1, c.in.mg_max_iter);
     for (;;) {
         while (launched < target && launched < cap) {
             const int par = launched & 1;  // executed cycles always follow this order
+            if (c.kp) c.kp->mg_idx = launched;
             HIPCHK(amr::launch_vcycle_step(c, mixed, par, c.phiv[par], c.phiv[par ^ 1], &nb));
             HIPCHK(amr::launch_fin_resid(c, nb));
             ++launched;
@@ -316,11 +326,15 @@ int mg_run(amressif_ctx *ctx, const WarmStart &w, bool center, const double *us,
         if (int rc = poll(ctx)) return rc;
         if (require_mom && c.ctl_host->mg_skip) {
             *skipped = true;
+            amr::kprof_close(c, amr::KL_MG, 0);  // every kernel of the phase exited at once
+            amr::kprof_close(c, amr::KL_PHASE, 0);
             return 0;
         }
         if (c.ctl_host->mg_done || launched >= cap) break;
         target = launched + 2;
     }
+    amr::kprof_close(c, amr::KL_MG, c.ctl_host->mg_iters);
+    amr::kprof_close(c, amr::KL_PHASE, 1);
     c.last_V = std::max(1, c.ctl_host->mg_iters);
     c.phi = c.phiv[c.ctl_host->mg_iters & 1];
     return 0;
@@ -382,6 +396,7 @@ int amr_momentum_finish(amressif_ctx *ctx, int launched)
         target = launched + 2;
     }
     c.last_P = std::max(1, c.ctl_host->mom_count);
+    amr::kprof_close(c, amr::KL_MOM, c.ctl_host->mom_count);
     return 0;
 }
 
@@ -406,7 +421,6 @@ namespace {
 int launch_pseudo_iteration(amressif_ctx *ctx, int it, int *nb)
 {
     Ctx &c = ctx->c;
-    (void)it;
     if (c.in.precision == AMRESSIF_MIXED) {
         amr::MixedStageLaunch a{};
         a.u64 = c.U[0]; a.v64 = c.U[1]; a.cu = c.C[0]; a.cv = c.C[1]; a.p = c.p;
@@ -422,9 +436,16 @@ int launch_pseudo_iteration(amressif_ctx *ctx, int it, int *nb)
             a.cr = c.rk[3] - a.cj * c15;
         }
         a.stage = 0;
+        // algorithmic bytes per cell: residual 56 (u^k, v^k, c_u, c_v, p in; FP32 base and
+        // residual out), tangent 48 (FP32 base and residual in; u^k, v^k in and out)
+        const double cells = (double)c.g.n * c.g.n;
+        HIPCHK(amr::kprof_begin(c, amr::KP_RESID, amr::KL_MOM, it, 56.0 * cells));
         HIPCHK(amr::launch_stage_mixed(c, a, nb));
+        HIPCHK(amr::kprof_end(c));
         a.stage = 1;
+        HIPCHK(amr::kprof_begin(c, amr::KP_TANGENT, amr::KL_MOM, it, 48.0 * cells));
         HIPCHK(amr::launch_stage_mixed(c, a, nb));
+        HIPCHK(amr::kprof_end(c));
         return 0;
     }
     amr::StageLaunch a{};
@@ -437,12 +458,82 @@ int launch_pseudo_iteration(amressif_ctx *ctx, int it, int *nb)
         a.us = in_u[sub]; a.vs = in_v[sub]; a.uo = out_u[sub]; a.vo = out_v[sub];
         a.rk = c.rk[sub];
         a.mode = sub == 3 ? 1 : 0;
+        // 72 B per cell: u*, v*, u^k, v^k, c_u, c_v, p in; u*, v* out
+        HIPCHK(amr::kprof_begin(c, sub == 3 ? amr::KP_STAGE_LAST : amr::KP_STAGE, amr::KL_MOM, it,
+                                72.0 * (double)c.g.n * c.g.n));
         HIPCHK(amr::launch_stage(c, a, nb));
+        HIPCHK(amr::kprof_end(c));
     }
     return 0;
 }
 
 }  // namespace
+
+namespace amr {
+
+hipError_t kprof_begin(const Ctx &c, int id, int loop, int idx, double bytes)
+{
+    KProf *k = c.kp;
+    if (!k || !k->on) return hipSuccess;
+    hipEvent_t e[2];
+    for (int q = 0; q < 2; ++q) {
+        if (k->pool.empty()) {
+            // timing-only events: no system-scope fence (no cache writeback
+            // between the bracketed kernels); they are read after a stream sync
+            hipError_t err = hipEventCreateWithFlags(&e[q], hipEventDisableSystemFence);
+            if (err != hipSuccess) return err;
+        } else {
+            e[q] = k->pool.back();
+            k->pool.pop_back();
+        }
+    }
+    k->recs.push_back({e[0], e[1], id, loop, idx, loop == KL_ONCE ? 1 : 0, bytes});
+    return hipEventRecord(e[0], c.stream);
+}
+
+hipError_t kprof_end(const Ctx &c)
+{
+    KProf *k = c.kp;
+    if (!k || !k->on) return hipSuccess;
+    return hipEventRecord(k->recs.back().e1, c.stream);
+}
+
+void kprof_close(const Ctx &c, int loop, int count)
+{
+    KProf *k = c.kp;
+    if (!k) return;
+    for (KProf::Rec &r : k->recs)
+        if (r.state == 0 && r.loop == loop) r.state = r.idx < count ? 1 : 2;
+}
+
+// records whose loop has closed, once their end event has completed
+hipError_t kprof_collect(const Ctx &c)
+{
+    KProf *k = c.kp;
+    if (!k || k->recs.empty()) return hipSuccess;
+    std::vector<KProf::Rec> keep;
+    for (const KProf::Rec &r : k->recs) {
+        if (r.state == 0 || hipEventQuery(r.e1) != hipSuccess) {
+            (void)hipGetLastError();  // hipErrorNotReady is not an error here
+            keep.push_back(r);
+            continue;
+        }
+        if (r.state == 1) {
+            float ms = 0.f;
+            hipError_t err = hipEventElapsedTime(&ms, r.e0, r.e1);
+            if (err != hipSuccess) return err;
+            k->sum_ms[r.id] += ms;
+            k->sum_bytes[r.id] += r.bytes;
+            k->cnt[r.id] += 1;
+        }
+        k->pool.push_back(r.e0);
+        k->pool.push_back(r.e1);
+    }
+    k->recs.swap(keep);
+    return hipSuccess;
+}
+
+}  // namespace amr
 
 extern "C" {
 
@@ -784,6 +875,11 @@ int amressif_destroy(amressif_ctx *ctx)
     for (void *p : c.allocs) hipFree(p);
     if (c.ctl_host) hipHostFree(c.ctl_host);
     if (c.log) hipFree(c.log);
+    if (c.kp) {
+        for (const amr::KProf::Rec &r : c.kp->recs) { hipEventDestroy(r.e0); hipEventDestroy(r.e1); }
+        for (hipEvent_t e : c.kp->pool) hipEventDestroy(e);
+        delete c.kp;
+    }
     delete c.adj;
     delete ctx;
     return AMRESSIF_OK;
@@ -909,6 +1005,7 @@ int advance_impl(amressif_ctx *ctx, int nsteps, amressif_step_stats *stats, amre
                 if ((rc = mg_run(ctx, w, false, c.U[0], c.U[1], false, nullptr))) return rc;
             } else {
                 c.last_P = std::max(1, c.ctl_host->mom_count);
+                amr::kprof_close(c, amr::KL_MOM, c.ctl_host->mom_count);
             }
         } else {
             if (int rc = amr_momentum_predictor(ctx, bdf_ready)) return rc;
@@ -929,7 +1026,11 @@ int advance_impl(amressif_ctx *ctx, int nsteps, amressif_step_stats *stats, amre
         // step's BDF2 constant (main.cpp:879-884); the last step leaves
         // C and velContPrev as the reference does
         const bool fold = s + 1 < nsteps && !(flags & AMRESSIF_FLAG_NONFINITE);
+        // per cell: u*, v*, phi, p in, u, v, p out (56 B); with the next step's
+        // BDF2 constant also u^n in, c and the copy of u^{n+1} out (104 B)
+        HIPCHK(amr::kprof_begin(c, amr::KP_PROJECT, amr::KL_ONCE, 0, (fold ? 104.0 : 56.0) * (double)c.g.n * c.g.n));
         HIPCHK(amr::launch_project(c, c.U[0], c.U[1], c.S1[0], c.S1[1], &nb, nullptr, nullptr, fold ? 1 : 0));
+        HIPCHK(amr::kprof_end(c));
         HIPCHK(amr::launch_fin_project(c, nb, s));
         bdf_ready = fold;
         std::swap(c.U[0], c.S1[0]);
@@ -943,6 +1044,7 @@ int advance_impl(amressif_ctx *ctx, int nsteps, amressif_step_stats *stats, amre
     if (stats) HIPCHK(hipMemcpyAsync(stats, c.log, sizeof(amressif_step_stats) * (size_t)nsteps,
                                      hipMemcpyDeviceToHost, c.stream));
     HIPCHK(hipStreamSynchronize(c.stream));
+    HIPCHK(amr::kprof_collect(c));
     return status;
 }
 }  // namespace
@@ -1109,6 +1211,32 @@ int amressif_project(amressif_ctx *ctx, const amr_array4 velStar[2], amr_array4 
     HIPCHK(amr::launch_unpack_cell(c, c.Mc[0], *userCtx, 0, 1));
     if (int rc = poll(ctx)) return rc;
     if (div_linf) *div_linf = c.ctl_host->div_new;
+    return AMRESSIF_OK;
+}
+
+int amressif_kernel_timing(amressif_ctx *ctx, int enable)
+{
+    if (!ctx) return AMRESSIF_ERR_ARG;
+    Ctx &c = ctx->c;
+    if (!c.kp) c.kp = new amr::KProf();
+    if (enable) {
+        HIPCHK(hipStreamSynchronize(c.stream));
+        HIPCHK(amr::kprof_collect(c));
+        for (int k = 0; k < amr::KP_COUNT; ++k) { c.kp->sum_ms[k] = c.kp->sum_bytes[k] = 0.0; c.kp->cnt[k] = 0; }
+    }
+    c.kp->on = enable != 0;
+    return AMRESSIF_OK;
+}
+
+int amressif_kernel_time(const amressif_ctx *ctx, int which, double *avg_ms, int64_t *launches,
+                         double *bytes_per_launch)
+{
+    if (!ctx || !avg_ms || !launches || !bytes_per_launch) return AMRESSIF_ERR_ARG;
+    if (which < 0 || which >= amr::KP_COUNT) return AMRESSIF_ERR_ARG;
+    const amr::KProf *k = ctx->c.kp;
+    *launches = k ? k->cnt[which] : 0;
+    *avg_ms = *launches ? k->sum_ms[which] / (double)*launches : 0.0;
+    *bytes_per_launch = *launches ? k->sum_bytes[which] / (double)*launches : 0.0;
     return AMRESSIF_OK;
 }
 

@@ -81,6 +81,42 @@ struct AdjWork {
     double *L[2][2]{};                                  // Richardson iterates (ping-pong, 2 face arrays)
 };
 
+// Per-launch HIP-event timing of the step's main kernels inside the step loop
+// (amressif_kernel_timing).  A record is counted once its loop's final count
+// shows the launch did work: launches past the converged pseudo-iteration or
+// V-cycle, and a speculative Poisson phase that was skipped, exit at once and
+// are dropped.
+enum KProfId {
+    KP_STAGE = 0,    // k_stage64<0>: FP64 RK stage
+    KP_STAGE_LAST,   // k_stage64<1>: FP64 last RK stage + norm partials
+    KP_RESID,        // k_stage64<3>: mixed FP64 residual
+    KP_TANGENT,      // k_tangent
+    KP_MG_DOWN,      // finest down-leg smoother + restriction
+    KP_MG_UP,        // finest up-leg smoother + FP64 outer step
+    KP_PROJECT,      // k_project
+    KP_PSTART,       // k_poisson_start
+    KP_COUNT
+};
+enum KProfLoop { KL_MOM = 0, KL_MG, KL_PHASE, KL_ONCE };
+struct KProf {
+    bool on = false;
+    struct Rec {
+        hipEvent_t e0, e1;
+        int id, loop, idx, state;  // state: 0 open, 1 counted, 2 dropped
+        double bytes;              // algorithmic HBM bytes of the launch
+    };
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> pool;
+    double sum_ms[KP_COUNT]{}, sum_bytes[KP_COUNT]{};
+    int64_t cnt[KP_COUNT]{};
+    int mg_idx = 0;  // V-cycle being enqueued
+};
+struct Ctx;
+hipError_t kprof_begin(const Ctx &c, int id, int loop, int idx, double bytes);
+hipError_t kprof_end(const Ctx &c);
+void kprof_close(const Ctx &c, int loop, int count);  // open records of the loop: counted if idx < count
+hipError_t kprof_collect(const Ctx &c);               // accumulate the finished records
+
 struct Ctx {
     amressif_inputs in{};
     hipStream_t stream = nullptr;
@@ -147,6 +183,7 @@ struct Ctx {
     double *dscalar = nullptr;
     std::vector<void *> allocs;
     std::string err;
+    KProf *kp = nullptr;  // in-step kernel timing (null: off)
 };
 
 // ---- kernel launchers (kernels_*.hip)

@@ -1418,9 +1418,20 @@ static hipError_t vcycle_impl(const Ctx &c, const T *pinv, const T *r0, const Ou
     const int ext = sizeof(T) == 4 && c.tail_ext && c.dct_tab32;  // level tail_level - 1 joins the tail kernel
     const int lt = c.tail_level - ext;
     const OuterArgs none{};
-    for (int l = 0; l < lt; ++l)  // down: smooth from zero, fused residual + restriction
+    // (the finest-level smoothers of an outer-step cycle are timed when the
+    // context's in-step kernel timing is on)
+    const int vi = c.kp ? c.kp->mg_idx : 0;
+    // algorithmic bytes per finest cell (t = sizeof(T)): down-leg r in, e out, coarse r
+    // out (2.25 t); up-leg e, r, coarse e, phi, b in, phi and the next r out (3.25 t + 24)
+    const double cells = (double)c.lev[0].nx * c.lev[0].ny, t = (double)sizeof(T);
+    for (int l = 0; l < lt; ++l) {  // down: smooth from zero, fused residual + restriction
+        if (l == 0 && fuse)
+            if (hipError_t e = kprof_begin(c, KP_MG_DOWN, KL_MG, vi, 2.25 * t * cells)) return e;
         launch_smoother<T, 1>(c, l, (const T *)nullptr, (const T *)nullptr, 0, lev_e<T>(c.lev[l], 0),
                               l == 0 ? r0 : (const T *)lev_r<T>(c.lev[l]), none);
+        if (l == 0 && fuse)
+            if (hipError_t e = kprof_end(c)) return e;
+    }
     if (ext) {
         const Level &L1 = c.lev[lt], &Lt = c.lev[lt + 1];
         const float *r1 = (const float *)(const void *)lev_r<T>(L1);
@@ -1449,10 +1460,12 @@ static hipError_t vcycle_impl(const Ctx &c, const T *pinv, const T *r0, const Ou
     for (int l = lt - 1; l >= 0; --l) {  // up: add the coarse correction, smooth
         const Level &L = c.lev[l], &Cl = c.lev[l + 1];
         const T *rl = l == 0 ? r0 : (const T *)lev_r<T>(L);
-        if (l == 0 && fuse)
+        if (l == 0 && fuse) {
+            if (hipError_t e = kprof_begin(c, KP_MG_UP, KL_MG, vi, (3.25 * t + 24.0) * cells)) return e;
             launch_smoother<T, 2>(c, l, (const T *)lev_e<T>(L, 0), (const T *)lev_e<T>(Cl, 1), Cl.pitch, r0_next, rl,
                                   *fuse);
-        else
+            if (hipError_t e = kprof_end(c)) return e;
+        } else
             launch_smoother<T, 0>(c, l, (const T *)lev_e<T>(L, 0), (const T *)lev_e<T>(Cl, 1), Cl.pitch,
                                   lev_e<T>(L, 1), rl, none);
     }

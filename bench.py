@@ -20,9 +20,16 @@ of a 512^2, Re = 1000 rollout (backward sweep with checkpoint recomputation
 over the forward time) and the two-level composite step (512^2 coarse level +
 a ratio-2 patch over the top quarter).  One JSON line is printed.
 
-roofline: algorithmic bytes per launch of the headline run's dominant kernel
-over its HIP-event launch time; traffic: rocprofv3 FETCH_SIZE + WRITE_SIZE per
-launch from profiles/pmc_summary.txt, scaled by a 64 MiB copy.
+roofline: after the timed steps of each precision, the same run advances K
+more steps with the library's in-step kernel timing on (HIP events, created
+without a system-scope fence, around each launch of the step's main kernels on
+the library's stream; launches that exited at once past convergence are not
+counted).  achieved = the kernel's algorithmic bytes per launch / its average
+launch time there; `roofline` is the kernel of the headline run with the most
+time per step, `kernel_rooflines` lists every timed kernel.  The events cost a
+few per cent of the step time, so `value` comes from the first timed pass,
+which runs without them.  traffic: rocprofv3 FETCH_SIZE + WRITE_SIZE per launch
+from profiles/pmc_summary.txt, scaled by a 64 MiB copy.
 fp64_step_roofline: algorithmic bytes of a whole FP64 step (model in
 step_bytes_fp64) over the measured step time.
 """
@@ -39,11 +46,21 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
-# timing kinds of amressif_time_kernel -> kernel symbol
-KERNELS = {0: "amr::k_stage64 (FP64 stage)", 1: "amr::k_gsrb2<T, 1, R> (finest down-leg smoother + restriction)",
-           3: "amr::k_stage64 (mixed residual)", 4: "amr::k_tangent",
+# timing kinds of amressif_time_kernel (--kernel-probe) -> kernel symbol
+KERNELS = {0: "amr::k_stage64<0> (FP64 stage)", 1: "amr::k_gsrb2<T, 1, R> (finest down-leg smoother + restriction)",
+           3: "amr::k_stage64<3> (mixed residual)", 4: "amr::k_tangent",
            5: "amr::k_gsrb2<T, 2, R> (finest up-leg smoother + FP64 outer step)"}
-PMC_KEYS = {0: "stage_fp64", 3: "residual_mixed", 4: "tangent_mixed"}
+# in-step timed kernels (amressif_kernel_time) -> kernel symbol, key in profiles/pmc_summary.txt
+STEP_KERNELS = {
+    "stage": ("amr::k_stage64<0> (FP64 RK stage)", "stage_fp64"),
+    "stage_last": ("amr::k_stage64<1> (FP64 last RK stage + norm partials)", None),
+    "resid": ("amr::k_stage64<3> (mixed FP64 residual)", "residual_mixed"),
+    "tangent": ("amr::k_tangent (mixed FP32 tangent-linear update)", "tangent_mixed"),
+    "mg_down": ("amr::k_gsrb2<T, 1, 8> (finest down-leg smoother + restriction)", "smooth1_{prec}"),
+    "mg_up": ("amr::k_gsrb2<T, 2, 4> (finest up-leg smoother + FP64 outer step)", "smooth5_{prec}"),
+    "project": ("amr::k_project (update + next step's BDF2 constant)", None),
+    "poisson_start": ("amr::k_poisson_start (Poisson RHS, warm start, first residual)", None),
+}
 
 
 def parse():
@@ -110,6 +127,28 @@ def run_precision(api, W, n, ren, precision, state, warmup, steps, torch):
     return run, stats, t1 - t0
 
 
+def in_step_kernels(run, steps, prec_name, n, torch):
+    """Second timed pass of `steps` steps with per-launch HIP events: one
+    roofline entry per kernel that ran, with its launches per step."""
+    run.ctx.kernel_timing(True)
+    run.advance(steps)
+    torch.cuda.synchronize()
+    run.ctx.kernel_timing(False)
+    rows = []
+    for key, (ms, cnt, nbytes) in run.ctx.kernel_times().items():
+        if cnt == 0:
+            continue
+        sym, pmc = STEP_KERNELS[key]
+        achieved = nbytes / (ms * 1e-3)
+        rows.append({"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK,
+                     "traffic": read_traffic(f"{pmc.format(prec=prec_name)}_{n}") if pmc else None,
+                     "kernel": sym, "avg_launch_ms": ms, "algorithmic_bytes_per_launch": nbytes,
+                     "launches_per_step": cnt / steps, "ms_per_step": ms * cnt / steps,
+                     "timed": f"in-step, HIP events around each launch, {steps} steps"})
+    return rows
+
+
 def adjoint_leg(api, W, n, steps, every, torch):
     """Lid-and-initial-velocity gradient of the lower-left kinetic energy after
     `steps` FP64 steps (512^2, Re = 1000 by default): forward with checkpoints
@@ -169,6 +208,8 @@ def amr_leg(api, W, n0, steps, torch):
 
 
 def kernel_probe(api, W, n, ren, which_prec):
+    """Isolated launches for the counter runs (rocprofv3 --pmc): one step,
+    then a few timed launches of each kernel and a 64 MiB copy."""
     prec = api.AMRESSIF_MIXED if which_prec == "mixed" else api.AMRESSIF_FP64
     run = api.Run.from_numpy(api.cavity_inputs(n, ren, precision=prec), W.initial_state(n, "seeded"))
     run.advance(1)
@@ -208,11 +249,14 @@ def main():
         out[name] = dict(mcell_steps_per_s=n * n * K / sec / 1e6, ms_per_step=1e3 * sec / K, pseudo_iters=P,
                          vcycles=V, flags=flags, max_div_ratio=worst_div)
         runs[name] = run
+    # per-field error of mixed vs FP64: both runs have advanced W + K steps here
+    fetched = {name: runs[name].fetch() for name in runs} if len(runs) == 2 else {}
+    kernels = {name: in_step_kernels(runs[name], K, name, n, torch) for name in runs}
 
     # per-field error of mixed vs FP64 after the same W+K steps
     err = None
-    if "mixed" in runs and "fp64" in runs:
-        a, b = runs["mixed"].fetch(), runs["fp64"].fetch()
+    if fetched:
+        a, b = fetched["mixed"], fetched["fp64"]
 
         def rel(x, y):
             return float(np.abs(x - y).max() / max(np.abs(y).max(), 1e-300))
@@ -222,43 +266,16 @@ def main():
         err = {"U": rel(a["u"], b["u"]), "V": rel(a["v"], b["v"]),
                "p": rel(pa - pa.mean(), pb - pb.mean()), "phi": rel(fa - fa.mean(), fb - fb.mean())}
 
-    # roofline of the dominant kernel of the headline run: the FP64 run's fused
-    # RK stage, or the larger of the mixed run's two pseudo-iteration kernels
-    def kernel_roofline(run, which, name):
-        ms, nbytes = run.ctx.time_kernel(which, reps=50)
-        achieved = nbytes / (ms * 1e-3)
-        return {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK, "traffic": read_traffic(f"{name}_{n}"), "kernel": KERNELS[which],
-                "avg_launch_ms": ms, "algorithmic_bytes_per_launch": nbytes}
-
-    def dominant(name, kinds):
-        """the kernel with the largest time per step among the per-iteration kernels
-        (launches per step: the pseudo-iteration kernels once per iteration, the
-        finest-level smoothers once per V-cycle)"""
-        o = out[name]
-        cand = []
-        for w in kinds:
-            r = kernel_roofline(runs[name], w, f"{PMC_KEYS.get(w, 'smooth%d' % w)}_{name}" if w in (1, 5)
-                                else PMC_KEYS[w])
-            per_step = o["pseudo_iters"] * (4 if w == 0 else 1) if w in (0, 3, 4) else o["vcycles"]
-            r["launches_per_step"] = per_step
-            r["ms_per_step"] = per_step * r["avg_launch_ms"]
-            cand.append(r)
-        return max(cand, key=lambda r: r["ms_per_step"]), cand
-
+    # roofline of the headline run's dominant kernel: the one with the most
+    # time per step in the in-step timed pass
     head = "mixed" if "mixed" in runs else "fp64"
-    extra = {}
+    roofline = dict(max(kernels[head], key=lambda r: r["ms_per_step"]))
+    extra = {f"{name}_kernels": kernels[name] for name in runs}
     if "fp64" in out:
         o = out["fp64"]
         nbytes = step_bytes_fp64(n, o["pseudo_iters"], o["vcycles"])
         extra["fp64_step_roofline"] = {"bytes_per_step": nbytes, "achieved": nbytes / (o["ms_per_step"] * 1e-3) / 1e9,
                                        "unit": "GB/s", "frac": nbytes / (o["ms_per_step"] * 1e-3) / HBM_PEAK}
-    if head == "mixed":
-        roofline, extra["mixed_kernels"] = dominant("mixed", (3, 4, 1, 5))
-    else:
-        roofline, extra["fp64_kernels"] = dominant("fp64", (0, 1, 5))
-    if "fp64" in runs and head != "fp64":
-        _, extra["fp64_kernels"] = dominant("fp64", (0, 1, 5))
 
     adj = None
     if args.adjoint_n > 0 and not args.profile_run:

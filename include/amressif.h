@@ -292,6 +292,27 @@ int amressif_amr_advance(amressif_amr *amr, int nsteps, amressif_amr_stats *stat
 int amressif_time_kernel(amressif_ctx *ctx, int which, int reps, int64_t copy_bytes, double *avg_ms,
                          double *bytes_per_launch);
 
+/* No reference counterpart (a measurement hook).  In-step kernel timing:
+ * with enable != 0, amressif_advance brackets each launch of the step's main
+ * kernels with HIP events on the context's stream and keeps, per kernel, the
+ * number of launches that did work and their total time (launches past the
+ * converged pseudo-iteration or V-cycle, which exit at once, are not counted).
+ * Enabling clears the totals; enable = 0 stops recording and keeps them.
+ *   which = 0  k_stage64<0>: FP64 RK stage
+ *   which = 1  k_stage64<1>: FP64 last RK stage (+ norm partials)
+ *   which = 2  k_stage64<3>: mixed FP64 residual
+ *   which = 3  k_tangent: mixed FP32 tangent-linear update
+ *   which = 4  finest-level down-leg smoother + restriction (outer-step cycles)
+ *   which = 5  finest-level up-leg smoother + FP64 outer step
+ *   which = 6  k_project
+ *   which = 7  k_poisson_start
+ * amressif_kernel_time returns the average time per counted launch (0 if none),
+ * the number of counted launches and their average algorithmic HBM bytes (the
+ * bytes each launch must read and write, counted once per cell). */
+int amressif_kernel_timing(amressif_ctx *ctx, int enable);
+int amressif_kernel_time(const amressif_ctx *ctx, int which, double *avg_ms, int64_t *launches,
+                         double *bytes_per_launch);
+
 #ifdef __cplusplus
 }
 #endif

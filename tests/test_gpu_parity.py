@@ -352,3 +352,43 @@ def test_kernel_timing_diagnostic():
                 run.ctx.time_kernel(3, reps=1)
     ms, nbytes = run.ctx.time_kernel(2, reps=5, copy_bytes=1 << 24)
     assert ms > 0 and nbytes == 2 * (1 << 24)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_in_step_kernel_timing_counts_and_leaves_results_unchanged(prec):
+    """The in-step timing hook counts exactly the launches that did work (from
+    the step statistics) and does not change the step's results."""
+    api = _api()
+    n, steps = 256, 4
+    state = W.initial_state(n, "seeded")
+    plain = api.Run.from_numpy(api.cavity_inputs(n, 1000.0, precision=prec), state)
+    plain.advance(steps)
+    ref = plain.fetch()
+    run = api.Run.from_numpy(api.cavity_inputs(n, 1000.0, precision=prec), state)
+    run.ctx.kernel_timing(True)
+    stats = run.advance(steps)
+    t = run.ctx.kernel_times()
+    run.ctx.kernel_timing(False)
+    got = run.fetch()
+    for k in ("u", "v", "ctx"):
+        assert np.array_equal(got[k], ref[k]), k
+    pi = sum(s["pseudo_iters"] for s in stats)
+    vc = sum(s["vcycles"] for s in stats)
+    if prec == 0:
+        assert t["stage"][1] == 3 * pi and t["stage_last"][1] == pi
+        assert t["resid"][1] == 0 and t["tangent"][1] == 0
+    else:
+        assert t["resid"][1] == pi and t["tangent"][1] == pi
+        assert t["stage"][1] == 0
+    assert t["mg_down"][1] == vc and t["mg_up"][1] == vc
+    assert t["project"][1] == steps and t["poisson_start"][1] == steps
+    cells = float(n * n)
+    per_cell = {"stage": 72, "stage_last": 72, "resid": 56, "tangent": 48}
+    for name, (ms, cnt, nbytes) in t.items():
+        assert (ms > 0) == (cnt > 0) and (nbytes > 0) == (cnt > 0), name
+        assert ms < 50.0, name  # a 256^2 launch takes microseconds
+        if cnt and name in per_cell:
+            assert nbytes == per_cell[name] * cells, name
+    # disabled: the totals stay, nothing more is counted
+    run.advance(1)
+    assert run.ctx.kernel_times() == t

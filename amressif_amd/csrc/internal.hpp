@@ -159,6 +159,7 @@ struct Ctx {
     // exact solve of that level by the DCT (k_tail_dct): cosine and inverse
     // eigenvalue tables (null: the tail runs the V-cycle down to the bottom)
     int tail_dct = 1;
+    int dct_mfma64 = 1;  // FP64 DCT products on the matrix cores (0: on the VALU)
     double *dct_tab = nullptr;
     float *dct_tab32 = nullptr;
     // FP32 V-cycle: the level above it smoothed inside the same kernel

@@ -771,9 +771,9 @@ __global__ __launch_bounds__(1024) void k_tail(const T *rtop, int ptop, T *etop,
 // the orthonormal DCT-II basis, L = Cy D Cx^T with D(m, k) = -ax (2 - 2 cos(pi
 // k / nx)) - ay (2 - 2 cos(pi m / ny)), so e = Cy (D^+ (Cy^T r Cx)) Cx^T, the
 // (0, 0) mode (the constant) set to zero as the pseudo-inverse does.  Four
-// 64 x 64 x 64 matrix products in one workgroup (zero padding to 64): on the
-// matrix cores in FP32, on the VALU in FP64; the cosine tables are staged in
-// LDS with a padded row stride against bank conflicts.
+// 64 x 64 x 64 matrix products in one workgroup (zero padding to 64), on the
+// matrix cores (FP64: or on the VALU); the cosine tables are staged in LDS
+// with a padded row stride against bank conflicts.
 constexpr int kDct = 64, kDctLd = 65;
 
 // table layout (global): Cx [64 x 64], Cy [64 x 64], D^+ [64 (m) x 64 (k)]
@@ -841,31 +841,66 @@ __device__ __forceinline__ void dct_gemm_mfma(const float *A, const float *B, fl
         }
 }
 
-template <class T, bool TA, bool TB>
+// FP64 on the matrix cores: v_mfma_f64_16x16x4f64.  A and B as for the FP32
+// instruction (one double per lane); the 16 x 16 result differs: lane l holds
+// D[l / 16 + 4 v][l % 16], v = 0..3.
+typedef double dct_v4d __attribute__((ext_vector_type(4)));
+template <bool TA, bool TB>
+__device__ __forceinline__ void dct_gemm_mfma64(const double *A, const double *B, double *O, const double *scale)
+{
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6, li = l & 15, lk = l >> 4;
+    dct_v4d acc[4];
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) acc[cb] = dct_v4d{0.0, 0.0, 0.0, 0.0};
+    const int ar = 16 * w + li;
+#pragma unroll 4
+    for (int k0 = 0; k0 < kDct; k0 += 4) {
+        const int kk = k0 + lk;
+        const double a = TA ? A[kk * kDctLd + ar] : A[ar * kDctLd + kk];
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) {
+            const int bc = 16 * cb + li;
+            const double b = TB ? B[bc * kDctLd + kk] : B[kk * kDctLd + bc];
+            acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[cb], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int row = 16 * w + lk + 4 * v, col = 16 * cb + li;
+            O[row * kDctLd + col] = scale ? acc[cb][v] * scale[row * kDct + col] : acc[cb][v];
+        }
+}
+
+// MF: FP64 products on the matrix cores (FP32 ones always are)
+template <class T, bool TA, bool TB, bool MF>
 __device__ __forceinline__ void dct_product(const T *A, const T *B, T *O, const T *scale)
 {
     if constexpr (sizeof(T) == 4)
         dct_gemm_mfma<TA, TB>((const float *)A, (const float *)B, (float *)O, (const float *)scale);
+    else if constexpr (MF)
+        dct_gemm_mfma64<TA, TB>((const double *)A, (const double *)B, (double *)O, (const double *)scale);
     else
         dct_gemm<T, TA, TB>(A, B, O, scale);
 }
 
 // the DCT solve of the workgroup: R in B0 (zero outside ny x nx), the tables
 // in Cx, Cy (LDS) and dinv (global); E in B0 on return (B1 overwritten)
-template <class T>
+template <class T, bool MF = true>
 __device__ __forceinline__ void dct_solve(const T *Cx, const T *Cy, T *B0, T *B1, const T *dinv)
 {
-    dct_product<T, false, false>(B0, Cx, B1, (const T *)nullptr);  // T1 = R Cx
+    dct_product<T, false, false, MF>(B0, Cx, B1, (const T *)nullptr);  // T1 = R Cx
     __syncthreads();
-    dct_product<T, true, false>(Cy, B1, B0, dinv);  // S = (Cy^T T1) D^+
+    dct_product<T, true, false, MF>(Cy, B1, B0, dinv);  // S = (Cy^T T1) D^+
     __syncthreads();
-    dct_product<T, false, false>(Cy, B0, B1, (const T *)nullptr);  // T2 = Cy S
+    dct_product<T, false, false, MF>(Cy, B0, B1, (const T *)nullptr);  // T2 = Cy S
     __syncthreads();
-    dct_product<T, false, true>(B1, Cx, B0, (const T *)nullptr);  // E = T2 Cx^T
+    dct_product<T, false, true, MF>(B1, Cx, B0, (const T *)nullptr);  // E = T2 Cx^T
     __syncthreads();
 }
 
-template <class T>
+template <class T, bool MF>
 __global__ __launch_bounds__(256) void k_tail_dct(const T *rtop, int ptop, T *etop, int nx, int ny, const T *tab,
                                                   const Ctl *ctl)
 {
@@ -889,7 +924,7 @@ __global__ __launch_bounds__(256) void k_tail_dct(const T *rtop, int ptop, T *et
         B0[row * kDctLd + col] = vr[q];
     }
     __syncthreads();
-    dct_solve<T>(Cx, Cy, B0, B1, tab + 2 * kDct * kDct);
+    dct_solve<T, MF>(Cx, Cy, B0, B1, tab + 2 * kDct * kDct);
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
         const int e = t + 256 * q, row = e >> 6, col = e & 63;
@@ -1448,8 +1483,10 @@ static hipError_t vcycle_impl(const Ctx &c, const T *pinv, const T *r0, const Ou
         const T *rt = lt == 0 ? r0 : (const T *)lev_r<T>(Lt);
         const T *tab = sizeof(T) == 4 ? (const T *)(const void *)c.dct_tab32 : (const T *)(const void *)c.dct_tab;
         if (tab) {  // the exact solve of the top tail level
-            hipLaunchKernelGGL(k_tail_dct<T>, dim3(1), dim3(256), 0, c.stream, rt, Lt.pitch, lev_e<T>(Lt, 1), Lt.nx,
-                               Lt.ny, tab, ctl);
+            void (*kern)(const T *, int, T *, int, int, const T *, const Ctl *) =
+                c.dct_mfma64 ? k_tail_dct<T, true> : k_tail_dct<T, false>;
+            hipLaunchKernelGGL(kern, dim3(1), dim3(256), 0, c.stream, rt, Lt.pitch, lev_e<T>(Lt, 1), Lt.nx, Lt.ny,
+                               tab, ctl);
         } else {
             size_t lds = 0;
             const TailLevels tl = tail_levels(c, lt, &lds, sizeof(T));

@@ -19,7 +19,7 @@ from test_gpu_parity import _api, cuda, rel
 pytestmark = pytest.mark.gpu
 
 ENV_KEYS = ("AMRESSIF_MG_SMOOTHER", "AMRESSIF_MG_ROWS32", "AMRESSIF_MG_ROWS64", "AMRESSIF_MG_ROWS_COARSE",
-            "AMRESSIF_TAIL_DCT", "AMRESSIF_TAIL_EXT")
+            "AMRESSIF_TAIL_DCT", "AMRESSIF_TAIL_EXT", "AMRESSIF_DCT_MFMA64")
 
 
 def _solve(n, precision, b, cycles, tol=0.0, **env):
@@ -115,3 +115,15 @@ def test_tail_kernel_with_the_level_above_matches_separate_launches(n):
     assert res_ref < 1e-2
     assert abs(res - res_ref) <= 1e-5, (res, res_ref)
     assert rel(got, ref) < 1e-5
+
+
+@pytest.mark.parametrize("n", [64, 100, 256, 1024])
+def test_fp64_dct_on_matrix_cores_matches_valu(n):
+    """FP64 exact solve of the small levels: the products on the matrix cores
+    (v_mfma_f64_16x16x4f64) and on the VALU give the same cycles up to FP64
+    rounding (a wrong operand or result layout would not)."""
+    b = _rhs(n, 41 + n)
+    ref, res_ref = _solve(n, 0, b, 3, AMRESSIF_DCT_MFMA64="0")
+    got, res = _solve(n, 0, b, 3, AMRESSIF_DCT_MFMA64="1")
+    assert abs(res - res_ref) <= 1e-9 * max(res_ref, 1e-300) + 1e-14, (res, res_ref)
+    assert rel(got, ref) < 1e-12

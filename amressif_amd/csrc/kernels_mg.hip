@@ -1209,6 +1209,150 @@ __global__ __launch_bounds__(256) void k_poisson_start(double *b, const double *
 
 // (1.5/dt) times the mean cell divergence of u, v from the boundary faces alone:
 // the cell divergences telescope to the net flux through the walls
+// The same start in the row-march layout of the stage kernel: a wave owns a
+// band of 62 columns (64 lanes: the band and one halo lane on each side) and
+// a chunk of rows, and marches up them.  The guess of the rows j - 1, j, j + 1
+// and the south y-face stay in registers, x-neighbours come by DPP, so a cell
+// costs one load of each input array (u, v, phi1, phi2 or b) and the three
+// stores.  The next row's loads are issued one iteration ahead and a row's
+// stores one iteration behind (one wait for memory per row).  The per-cell
+// arithmetic is that of k_poisson_start; per-workgroup partials as there.
+constexpr int kStartWB = 62;
+template <class T, bool FROM_U>
+__global__ __launch_bounds__(256) void k_poisson_start_rows(double *b, const double *u, const double *v, double dx,
+                                                            double dy, double dt, const double *phi1,
+                                                            const double *phi2, double *phi0, T *r, int n, int P,
+                                                            double ax, double ay, int mode, int center,
+                                                            const Ctl *ctl, double *partials, int nbands, int nwaves,
+                                                            int rows)
+{
+    __shared__ double sh[4 * kStartParts];
+    if (ctl->mg_done) return;  // a skipped phase (k_mg_start)
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int B = gridDim.x;  // a multiple of 8
+    const int blk = (blockIdx.x & 7) * (B >> 3) + (blockIdx.x >> 3);
+    const int w = blk * 4 + wv;
+    const bool live = w < nwaves;  // a wave past the last one only joins the reduction
+    const int band = live ? w % nbands : 0, chunk = live ? w / nbands : 0;
+    const int i = band * kStartWB + lane - 1;
+    const int j0 = chunk * rows, jend = live ? min(j0 + rows, n) : j0;
+    const bool xin = i >= 0 && i < n, xface = i >= 0 && i <= n;
+    const bool out = xin && lane >= 1 && lane <= kStartWB;
+    const bool keep_b = FROM_U || center;
+    const double bmean = keep_b ? ctl->b_mean : 0.0;
+    struct Raw {
+        double p1, p2;  // guess iterates at (i, j + 1)
+        double uu, vn;  // x-face (i, j), y-face (i, j + 1)
+        double bb;      // b at (i, j) (b given)
+    };
+    auto load = [&](int j, Raw &q) {  // what row j consumes
+        q.p1 = q.p2 = q.uu = q.vn = q.bb = 0.0;
+        const size_t k = (size_t)(i < 0 ? 0 : i) + (size_t)P * j;
+        if (xin && j + 1 < n && mode != 0) {
+            q.p1 = phi1[k + P];
+            if (mode == 2) q.p2 = phi2[k + P];
+        }
+        if (FROM_U) {
+            if (xface) q.uu = u[k];  // also the halo lane: u(i + 1) of the band's last column
+            if (out) q.vn = v[k + P];
+        } else if (out) {
+            q.bb = b[k];
+        }
+    };
+    auto guess = [&](double p1, double p2) -> double { return mode == 0 ? 0.0 : mode == 1 ? p1 : 2.0 * p1 - p2; };
+    double gS = 0.0, gC = 0.0, vS = 0.0;
+    if (jend > j0 && xin) {
+        const size_t k = i + (size_t)P * j0;
+        if (mode != 0) {
+            gC = guess(phi1[k], mode == 2 ? phi2[k] : 0.0);
+            if (j0 > 0) gS = guess(phi1[k - P], mode == 2 ? phi2[k - P] : 0.0);
+        }
+        if (FROM_U && out) vS = v[k];
+    }
+    double mr = 0.0, sp = 0.0, mb = 0.0, md = 0.0, sb = 0.0;
+    size_t sk = 0;
+    bool pend = false;
+    double s_b = 0.0, s_phi = 0.0, s_r = 0.0;
+    auto flush = [&]() {
+        if (pend) {
+            if (keep_b) b[sk] = s_b;
+            phi0[sk] = s_phi;
+            r[sk] = (T)s_r;
+        }
+    };
+    Raw cur, nxt;
+    if (jend > j0) load(j0, cur);
+    for (int j = j0; j < jend; ++j) {
+        if (j + 1 < jend) load(j + 1, nxt);  // prefetch
+        flush();
+        pend = false;
+        const double Pv = gC;
+        const double gN = guess(cur.p1, cur.p2);
+        const double Wl = lane_prev(Pv), El = lane_next(Pv);  // (every lane shifts: no divergence around DPP)
+        const double ue = lane_next(cur.uu);
+        const double Wv = i > 0 ? Wl : Pv;
+        const double Ev = i < n - 1 ? El : Pv;
+        const double Sv = j > 0 ? gS : Pv;
+        const double Nv = j < n - 1 ? gN : Pv;
+        double bc, d = 0.0, braw = 0.0;
+        if (FROM_U) {
+            d = (ue - cur.uu) / dx + (cur.vn - vS) / dy;
+            braw = (1.5 / dt) * d;
+            bc = braw - bmean;
+        } else {
+            bc = cur.bb - bmean;
+        }
+        const double rr = bc - ((Ev + Wv - 2.0 * Pv) * ax + (Nv + Sv - 2.0 * Pv) * ay);
+        if (out) {
+            sk = i + (size_t)P * j;
+            pend = true;
+            s_b = bc;
+            s_phi = Pv;
+            s_r = rr;
+            mr = (fabs(rr) > mr || rr != rr) ? fabs(rr) : mr;
+            sp += Pv;
+            mb = (fabs(bc) > mb || bc != bc) ? fabs(bc) : mb;
+            if (FROM_U) {
+                md = (fabs(d) > md || d != d) ? fabs(d) : md;
+                sb += braw;
+            }
+        }
+        gS = gC;
+        gC = gN;
+        vS = cur.vn;
+        cur = nxt;
+    }
+    flush();
+    // wave trees, then the four waves in order
+    for (int o = 32; o > 0; o >>= 1) {
+        const double a0 = __shfl_down(mr, o, 64), a2 = __shfl_down(mb, o, 64), a3 = __shfl_down(md, o, 64);
+        mr = (a0 > mr || a0 != a0) ? a0 : mr;
+        mb = (a2 > mb || a2 != a2) ? a2 : mb;
+        md = (a3 > md || a3 != a3) ? a3 : md;
+        sp += __shfl_down(sp, o, 64);
+        sb += __shfl_down(sb, o, 64);
+    }
+    if (lane == 0) {
+        double *o = sh + kStartParts * wv;
+        o[0] = mr; o[1] = sp; o[2] = mb; o[3] = md; o[4] = sb;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double q[kStartParts];
+        for (int e = 0; e < kStartParts; ++e) q[e] = sh[e];
+        for (int x = 1; x < 4; ++x) {
+            const double *o = sh + kStartParts * x;
+            q[0] = (o[0] > q[0] || o[0] != o[0]) ? o[0] : q[0];
+            q[1] += o[1];
+            q[2] = (o[2] > q[2] || o[2] != o[2]) ? o[2] : q[2];
+            q[3] = (o[3] > q[3] || o[3] != o[3]) ? o[3] : q[3];
+            q[4] += o[4];
+        }
+        double *o = partials + (size_t)kStartParts * blockIdx.x;
+        for (int e = 0; e < kStartParts; ++e) o[e] = q[e];
+    }
+}
+
 __global__ __launch_bounds__(1024) void k_flux_mean(const double *u, const double *v, int n, int P, double dx,
                                                     double dy, double dt, Ctl *ctl)
 {
@@ -1264,24 +1408,43 @@ hipError_t launch_poisson_start(const Ctx &c, int mode, int center, const double
                                 double *phi0, int mixed, const double *us, const double *vs)
 {
     const int n = c.g.n;
-    const int ntx = (n + 63) / 64, nty = (n + 15) / 16;
-    const int nb = (ntx * nty + 7) / 8 * 8;  // 64 x 16 tiles, a multiple of 8 workgroups
-    if ((int64_t)nb * kStartParts > (int64_t)kMaxPartials * 4) return hipErrorInvalidValue;
-    const dim3 grid(nb);
     const Level &L = c.lev[0];
     const double dx = c.g.dx, dy = c.g.dy, dt = c.g.dt;
     if (us) hipLaunchKernelGGL(k_flux_mean, dim3(1), dim3(1024), 0, c.stream, us, vs, n, c.g.P, dx, dy, dt, c.ctl);
+    int nb = 0;
+    if (c.start_rows > 0) {  // row-march kernel
+        const int rows = c.start_rows;
+        const int nbands = (n + kStartWB - 1) / kStartWB, nwaves = nbands * ((n + rows - 1) / rows);
+        nb = ((nwaves + 3) / 4 + 7) / 8 * 8;  // four waves per workgroup, a multiple of 8 workgroups
+        if ((int64_t)nb * kStartParts > (int64_t)kMaxPartials * 4) return hipErrorInvalidValue;
 #define AMR_START(T, FU, RR)                                                                                         \
-    hipLaunchKernelGGL((k_poisson_start<T, FU>), grid, dim3(256), 0, c.stream, c.b, us, vs, dx, dy, dt, phi1, phi2,   \
-                       phi0, RR, n, c.g.P, L.ax, L.ay, mode, center, (const Ctl *)c.ctl, c.partials, ntx, nty)
-    if (mixed) {
-        if (us) AMR_START(float, true, L.r32);
-        else AMR_START(float, false, L.r32);
-    } else {
-        if (us) AMR_START(double, true, L.r);
-        else AMR_START(double, false, L.r);
-    }
+    hipLaunchKernelGGL((k_poisson_start_rows<T, FU>), dim3(nb), dim3(256), 0, c.stream, c.b, us, vs, dx, dy, dt,      \
+                       phi1, phi2, phi0, RR, n, c.g.P, L.ax, L.ay, mode, center, (const Ctl *)c.ctl, c.partials,     \
+                       nbands, nwaves, rows)
+        if (mixed) {
+            if (us) AMR_START(float, true, L.r32);
+            else AMR_START(float, false, L.r32);
+        } else {
+            if (us) AMR_START(double, true, L.r);
+            else AMR_START(double, false, L.r);
+        }
 #undef AMR_START
+    } else {  // 64 x 16 cell tiles
+        const int ntx = (n + 63) / 64, nty = (n + 15) / 16;
+        nb = (ntx * nty + 7) / 8 * 8;  // a multiple of 8 workgroups
+        if ((int64_t)nb * kStartParts > (int64_t)kMaxPartials * 4) return hipErrorInvalidValue;
+#define AMR_START(T, FU, RR)                                                                                         \
+    hipLaunchKernelGGL((k_poisson_start<T, FU>), dim3(nb), dim3(256), 0, c.stream, c.b, us, vs, dx, dy, dt, phi1,    \
+                       phi2, phi0, RR, n, c.g.P, L.ax, L.ay, mode, center, (const Ctl *)c.ctl, c.partials, ntx, nty)
+        if (mixed) {
+            if (us) AMR_START(float, true, L.r32);
+            else AMR_START(float, false, L.r32);
+        } else {
+            if (us) AMR_START(double, true, L.r);
+            else AMR_START(double, false, L.r);
+        }
+#undef AMR_START
+    }
     hipLaunchKernelGGL(k_fin_start, dim3(1), dim3(1024), 0, c.stream, (const double *)c.partials, nb, c.ctl, n,
                        us ? 1 : 0);
     return hipGetLastError();

@@ -19,7 +19,8 @@ from test_gpu_parity import _api, cuda, rel
 pytestmark = pytest.mark.gpu
 
 ENV_KEYS = ("AMRESSIF_MG_SMOOTHER", "AMRESSIF_MG_ROWS32", "AMRESSIF_MG_ROWS64", "AMRESSIF_MG_ROWS_COARSE",
-            "AMRESSIF_TAIL_DCT", "AMRESSIF_TAIL_EXT", "AMRESSIF_DCT_MFMA64")
+            "AMRESSIF_TAIL_DCT", "AMRESSIF_TAIL_EXT", "AMRESSIF_DCT_MFMA64",
+            "AMRESSIF_START_ROWS")
 
 
 def _solve(n, precision, b, cycles, tol=0.0, **env):
@@ -126,4 +127,18 @@ def test_fp64_dct_on_matrix_cores_matches_valu(n):
     ref, res_ref = _solve(n, 0, b, 3, AMRESSIF_DCT_MFMA64="0")
     got, res = _solve(n, 0, b, 3, AMRESSIF_DCT_MFMA64="1")
     assert abs(res - res_ref) <= 1e-9 * max(res_ref, 1e-300) + 1e-14, (res, res_ref)
+    assert rel(got, ref) < 1e-12
+
+
+@pytest.mark.parametrize("n", [8, 100, 256, 1024])
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("rows", ["4", "16"])
+def test_row_march_poisson_start_matches_cell_tile_kernel(n, precision, rows):
+    """The Poisson start (centred b, first residual, partials) in the row-march
+    layout gives the cell-tile kernel's cycles: the same per-cell arithmetic;
+    only the order of the two partial sums differs."""
+    b = _rhs(n, 53 + n)
+    ref, res_ref = _solve(n, precision, b, 3, AMRESSIF_START_ROWS="0")
+    got, res = _solve(n, precision, b, 3, AMRESSIF_START_ROWS=rows)
+    assert abs(res - res_ref) <= 1e-12 * max(res_ref, 1e-300), (res, res_ref)
     assert rel(got, ref) < 1e-12

@@ -392,3 +392,31 @@ def test_in_step_kernel_timing_counts_and_leaves_results_unchanged(prec):
     # disabled: the totals stay, nothing more is counted
     run.advance(1)
     assert run.ctx.kernel_times() == t
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_row_march_poisson_start_in_the_step(prec):
+    """Whole steps (b from u*, warm start in the mixed step) with the row-march
+    Poisson start match those with the cell-tile kernel to rounding."""
+    import os
+    api = _api()
+    n, steps = 256, 4
+    state = W.initial_state(n, "seeded")
+    out = {}
+    for rows in ("0", "16"):
+        old = os.environ.get("AMRESSIF_START_ROWS")
+        os.environ["AMRESSIF_START_ROWS"] = rows
+        try:
+            run = api.Run.from_numpy(api.cavity_inputs(n, 1000.0, precision=prec), state)
+        finally:
+            if old is None:
+                os.environ.pop("AMRESSIF_START_ROWS")
+            else:
+                os.environ["AMRESSIF_START_ROWS"] = old
+        stats = run.advance(steps)
+        assert all(s["flags"] == 0 for s in stats)
+        out[rows] = run.fetch()
+    a, b = out["16"], out["0"]
+    assert rel(a["u"], b["u"]) < 1e-12 and rel(a["v"], b["v"]) < 1e-12
+    pa, pb = a["ctx"][0, 1:-1, 1:-1], b["ctx"][0, 1:-1, 1:-1]
+    assert rel(pa - pa.mean(), pb - pb.mean()) < 1e-10

@@ -700,6 +700,8 @@ int amressif_create(const amressif_inputs *in, void *hip_stream, amressif_ctx **
     if (const char *e = std::getenv("AMRESSIF_TAIL_DCT")) c.tail_dct = std::atoi(e) ? 1 : 0;
     if (const char *e = std::getenv("AMRESSIF_DCT_MFMA64")) c.dct_mfma64 = std::atoi(e) ? 1 : 0;
     if (const char *e = std::getenv("AMRESSIF_START_ROWS")) c.start_rows = std::max(0, std::min(256, std::atoi(e)));
+    if (const char *e = std::getenv("AMRESSIF_PROJECT_ROWS"))
+        c.project_rows = std::max(0, std::min(256, std::atoi(e)));
     if (const char *e = std::getenv("AMRESSIF_TAIL_EXT")) c.tail_ext = std::atoi(e) ? 1 : 0;
     auto rows_ok = [](int r) { return r == 4 || r == 8 || r == 16 || r == 32; };
     for (int t = 0; t < 2; ++t)

@@ -161,6 +161,7 @@ struct Ctx {
     int tail_dct = 1;
     int dct_mfma64 = 1;  // FP64 DCT products on the matrix cores (0: on the VALU)
     int start_rows = 16;  // Poisson start: rows per wave of the row-march kernel (0: cell-tile kernel)
+    int project_rows = 16;  // projection: rows per wave of the row-march kernel (0: cell-tile kernel)
     double *dct_tab = nullptr;
     float *dct_tab32 = nullptr;
     // FP32 V-cycle: the level above it smoothed inside the same kernel

@@ -319,12 +319,151 @@ __global__ __launch_bounds__(256) void k_project(const double *us, const double 
     if (threadIdx.x == 0) partials[blockIdx.x] = m;
 }
 
+// The same update in the row-march layout of the stage kernel: a wave owns a
+// band of 62 columns (64 lanes: one halo lane on each side) and a chunk of
+// rows.  phi of the rows j and j + 1 and the new y-face j stay in registers,
+// phi(i - 1) and the new x-face i + 1 come by DPP, so each face is formed once
+// and a cell costs one load of each input and one store of each output.  The
+// next row's loads go out one iteration ahead, a row's stores one iteration
+// behind (one wait for memory per row).  x-face n is written by the lane with
+// i = n, y-face n by the row n - 1.  Per-face arithmetic is that of k_project.
+constexpr int kProjWB = 62;
+__global__ __launch_bounds__(256) void k_project_rows(const double *us, const double *vs, const float *au,
+                                                      const float *av, double *uo, double *vo, double *p,
+                                                      const double *phi, int n, int P, double dx, double dy,
+                                                      double dt, const Ctl *ctl, int use_mean, double *partials,
+                                                      int nbands, int nwaves, int rows, NextBdf nx)
+{
+    __shared__ double sh[4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int B = gridDim.x;  // a multiple of 8
+    const int blk = (blockIdx.x & 7) * (B >> 3) + (blockIdx.x >> 3);
+    const int w = blk * 4 + wv;
+    const bool live = w < nwaves;  // a wave past the last one only joins the reduction
+    const int band = live ? w % nbands : 0, chunk = live ? w / nbands : 0;
+    const int i = band * kProjWB + lane - 1;
+    const int j0 = chunk * rows, jend = live ? min(j0 + rows, n) : j0;
+    const bool xin = i >= 0 && i < n, xface = i >= 0 && i <= n;
+    const bool own = lane >= 1 && lane <= kProjWB;
+    const bool outc = xin && own, outx = xface && own;
+    const bool fold = nx.cu != nullptr;
+    const double pm = use_mean ? ctl->phi_mean : 0.0;
+    struct Raw {
+        double phn;       // phi (i, j + 1)
+        double us, vsn;   // u* at x-face (i, j), v* at y-face (i, j + 1)
+        double pc;        // p (i, j)
+        double upu, upv;  // u^n at the faces (i, j) (fold)
+        double upvn;      // u^n at y-face (i, n) (fold, row n - 1)
+    };
+    auto load = [&](int j, Raw &q) {
+        q.phn = q.us = q.vsn = q.pc = q.upu = q.upv = q.upvn = 0.0;
+        const size_t k = (size_t)(i < 0 ? 0 : i) + (size_t)P * j;
+        if (xin && j + 1 < n) q.phn = phi[k + P];
+        if (xface) q.us = au ? us[k] + (double)au[k] : us[k];
+        if (xin) q.vsn = av ? vs[k + P] + (double)av[k + P] : vs[k + P];
+        if (outc) q.pc = p[k];
+        if (fold) {
+            if (outx) q.upu = nx.up_u[k];
+            if (outc) {
+                q.upv = nx.up_v[k];
+                if (j == n - 1) q.upvn = nx.up_v[k + P];
+            }
+        }
+    };
+    auto bdf = [&](double un, double um) { return (1.5 / dt) * un + (0.5 / dt) * (un - um); };
+    // prologue: phi (i, j0) and the new y-face (i, j0)
+    double phC = 0.0, vprev = 0.0;
+    if (jend > j0 && xin) {
+        const size_t k = i + (size_t)P * j0;
+        phC = phi[k];
+        const double vstar = av ? vs[k] + (double)av[k] : vs[k];
+        vprev = j0 == 0 ? vstar : vstar - (((phC - phi[k - P]) / dy) * dt / 1.5);
+    }
+    double m = 0.0;
+    // deferred stores of row j - 1
+    size_t sk = 0;
+    bool sx = false, sy = false, syn = false;
+    double su = 0.0, scu = 0.0, sv = 0.0, scv = 0.0, sp = 0.0, svn = 0.0, scvn = 0.0;
+    auto flush = [&]() {
+        if (sx) {
+            uo[sk] = su;
+            if (fold) { nx.cu[sk] = scu; nx.wu[sk] = su; }
+        }
+        if (sy) {
+            vo[sk] = sv;
+            p[sk] = sp;
+            if (fold) { nx.cv[sk] = scv; nx.wv[sk] = sv; }
+        }
+        if (syn) {
+            vo[sk + P] = svn;
+            if (fold) { nx.cv[sk + P] = scvn; nx.wv[sk + P] = svn; }
+        }
+    };
+    Raw cur, nxt;
+    if (jend > j0) load(j0, cur);
+    for (int j = j0; j < jend; ++j) {
+        if (j + 1 < jend) load(j + 1, nxt);  // prefetch
+        flush();
+        sx = sy = syn = false;
+        const double phW = lane_prev(phC);  // (every lane shifts: no divergence around DPP)
+        const double uw = (i == 0 || i == n) ? cur.us : cur.us - (((phC - phW) / dx) * dt / 1.5);
+        const double ue = lane_next(uw);
+        const double vn = j + 1 == n ? cur.vsn : cur.vsn - (((cur.phn - phC) / dy) * dt / 1.5);
+        sk = (size_t)(i < 0 ? 0 : i) + (size_t)P * j;
+        if (outx) {
+            sx = true;
+            su = uw;
+            if (fold) scu = bdf(uw, cur.upu);
+        }
+        if (outc) {
+            sy = true;
+            sv = vprev;
+            sp = cur.pc + (phC - pm);
+            if (fold) scv = bdf(vprev, cur.upv);
+            if (j == n - 1) {
+                syn = true;
+                svn = vn;
+                if (fold) scvn = bdf(vn, cur.upvn);
+            }
+            const double dv = fabs((ue - uw) / dx + (vn - vprev) / dy);
+            m = (dv > m || dv != dv) ? dv : m;
+        }
+        phC = cur.phn;
+        vprev = vn;
+        cur = nxt;
+    }
+    flush();
+    for (int o = 32; o > 0; o >>= 1) {
+        const double a = __shfl_down(m, o, 64);
+        m = (a > m || a != a) ? a : m;
+    }
+    if (lane == 0) sh[wv] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double q = sh[0];
+        for (int x = 1; x < 4; ++x) q = (sh[x] > q || sh[x] != sh[x]) ? sh[x] : q;
+        partials[blockIdx.x] = q;
+    }
+}
+
 hipError_t launch_project_ex(const Ctx &c, const double *us, const double *vs, double *uo, double *vo, double *p,
                              const double *phi, int use_mean, int *nblocks, const float *au, const float *av,
                              int next_bdf)
 {
     const NextBdf nx = next_bdf ? NextBdf{c.Uprev[0], c.Uprev[1], c.C[0], c.C[1], c.S2[0], c.S2[1]} : NextBdf{};
     const int n = c.g.n;
+    if (c.project_rows > 0) {  // row-march kernel
+        const int rows = c.project_rows;
+        const int nbands = (n + 1 + kProjWB - 1) / kProjWB;  // x-face n needs a lane inside a band
+        const int nwaves = nbands * ((n + rows - 1) / rows);
+        const int nb = ((nwaves + 3) / 4 + 7) / 8 * 8;  // four waves per workgroup, a multiple of 8
+        if (nb > kMaxPartials) return hipErrorInvalidValue;
+        *nblocks = nb;
+        hipLaunchKernelGGL(k_project_rows, dim3(nb), dim3(256), 0, c.stream, us, vs, au, av, uo, vo, p, phi, n,
+                           c.g.P, c.g.dx, c.g.dy, c.g.dt, (const Ctl *)c.ctl, use_mean, c.partials, nbands, nwaves,
+                           rows, nx);
+        return hipGetLastError();
+    }
     const int ntx = (n + 63) / 64, nty = (n + 15) / 16;
     const int nb = (ntx * nty + 7) / 8 * 8;  // 64 x 16 tiles, a multiple of 8 workgroups
     if (nb > kMaxPartials) return hipErrorInvalidValue;

@@ -395,24 +395,28 @@ def test_in_step_kernel_timing_counts_and_leaves_results_unchanged(prec):
 
 
 @pytest.mark.parametrize("prec", [0, 1])
-def test_row_march_poisson_start_in_the_step(prec):
-    """Whole steps (b from u*, warm start in the mixed step) with the row-march
-    Poisson start match those with the cell-tile kernel to rounding."""
+@pytest.mark.parametrize("var", ["AMRESSIF_START_ROWS", "AMRESSIF_PROJECT_ROWS"])
+@pytest.mark.parametrize("n", [62, 256])
+def test_row_march_kernels_in_the_step(prec, var, n):
+    """Whole steps with the row-march Poisson start (b from u*, warm start in
+    the mixed step) or the row-march projection (with the next step's BDF2
+    constant) match those with the cell-tile kernels to rounding; n = 62 puts
+    the east wall face on the first lane past a band."""
     import os
     api = _api()
-    n, steps = 256, 4
+    steps = 4
     state = W.initial_state(n, "seeded")
     out = {}
     for rows in ("0", "16"):
-        old = os.environ.get("AMRESSIF_START_ROWS")
-        os.environ["AMRESSIF_START_ROWS"] = rows
+        old = os.environ.get(var)
+        os.environ[var] = rows
         try:
             run = api.Run.from_numpy(api.cavity_inputs(n, 1000.0, precision=prec), state)
         finally:
             if old is None:
-                os.environ.pop("AMRESSIF_START_ROWS")
+                os.environ.pop(var)
             else:
-                os.environ["AMRESSIF_START_ROWS"] = old
+                os.environ[var] = old
         stats = run.advance(steps)
         assert all(s["flags"] == 0 for s in stats)
         out[rows] = run.fetch()

@@ -371,6 +371,9 @@ int amr_momentum_begin(amressif_ctx *ctx, bool bdf_ready, int *launched)
     } else {
         HIPCHK(amr::launch_bdf2(c));
     }
+    // FP64: p^n is the same in every stage of the step; its gradient term goes
+    // into C once and the stage kernels read no p
+    if (c.in.precision != AMRESSIF_MIXED) HIPCHK(amr::launch_fold_gradp(c));
     const int target = std::min(std::max(1, c.last_P), c.in.IterNum + 1);
     int nb = 0;
     for (*launched = 0; *launched < target; ++*launched) {
@@ -450,6 +453,7 @@ int launch_pseudo_iteration(amressif_ctx *ctx, int it, int *nb)
     }
     amr::StageLaunch a{};
     a.ub = c.U[0]; a.vb = c.U[1]; a.cu = c.C[0]; a.cv = c.C[1]; a.p = c.p;
+    a.foldp = 1;  // C holds -G p (amr_momentum_begin)
     const double *in_u[4] = {c.U[0], c.S1[0], c.S2[0], c.S1[0]};
     const double *in_v[4] = {c.U[1], c.S1[1], c.S2[1], c.S1[1]};
     double *out_u[4] = {c.S1[0], c.S2[0], c.S1[0], c.U[0]};
@@ -458,9 +462,9 @@ int launch_pseudo_iteration(amressif_ctx *ctx, int it, int *nb)
         a.us = in_u[sub]; a.vs = in_v[sub]; a.uo = out_u[sub]; a.vo = out_v[sub];
         a.rk = c.rk[sub];
         a.mode = sub == 3 ? 1 : 0;
-        // 72 B per cell: u*, v*, u^k, v^k, c_u, c_v, p in; u*, v* out
+        // 64 B per cell: u*, v*, u^k, v^k, c_u, c_v in; u*, v* out
         HIPCHK(amr::kprof_begin(c, sub == 3 ? amr::KP_STAGE_LAST : amr::KP_STAGE, amr::KL_MOM, it,
-                                72.0 * (double)c.g.n * c.g.n));
+                                64.0 * (double)c.g.n * c.g.n));
         HIPCHK(amr::launch_stage(c, a, nb));
         HIPCHK(amr::kprof_end(c));
     }
@@ -1283,6 +1287,7 @@ int amressif_time_kernel(amressif_ctx *ctx, int which, int reps, int64_t copy_by
             a.uo = c.S2[0]; a.vo = c.S2[1];
             a.rk = c.rk[0];
             a.mode = 0;
+            a.foldp = 1;  // the step's form (C holds -G p)
             a.unguarded = 1;
             int nb = 0;
             return amr::launch_stage(c, a, &nb);
@@ -1307,7 +1312,7 @@ int amressif_time_kernel(amressif_ctx *ctx, int which, int reps, int64_t copy_by
     *avg_ms = (double)ms / reps;
     if (bytes_per_launch) {
         // algorithmic bytes per cell (t = 4 or 8: the V-cycle precision):
-        // 0 FP64 stage: reads u*, v*, u^k, v^k, c_u, c_v, p, writes u*, v* (9 x 8 B);
+        // 0 FP64 stage: reads u*, v*, u^k, v^k, c_u, c_v (with -G p), writes u*, v* (8 x 8 B);
         // 1 finest down-leg smoother: reads r, writes e (2 t), the coarse r (t / 4);
         // 2 copy: read + write of copy_bytes;
         // 3 mixed residual: reads u^k, v^k, c_u, c_v, p (5 x 8 B), writes the FP32 base and
@@ -1317,7 +1322,7 @@ int amressif_time_kernel(amressif_ctx *ctx, int which, int reps, int64_t copy_by
         // 5 finest up-leg smoother + FP64 outer step: reads e, r (2 t), the coarse e (t / 4),
         //   phi, b (2 x 8 B), writes phi (8 B) and the next r (t)
         const double t = c.in.precision == AMRESSIF_MIXED ? 4.0 : 8.0;
-        const double per_cell[6] = {72.0, 2.25 * t, 0.0, 56.0, 48.0, 3.25 * t + 24.0};
+        const double per_cell[6] = {64.0, 2.25 * t, 0.0, 56.0, 48.0, 3.25 * t + 24.0};
         *bytes_per_launch = which == 2 ? 2.0 * (double)copy_bytes : per_cell[which] * cells;
     }
     return AMRESSIF_OK;

@@ -199,8 +199,11 @@ struct StageLaunch {
     double rk;
     int mode;               // 0 = stage update, 1 = stage update + norm partials, 2 = RHS only
     int unguarded;          // 1: ignore the control block (diagnostic timing)
+    int foldp;              // modes 0, 1: cu, cv already hold -G p (launch_fold_gradp); p is not read
 };
 hipError_t launch_stage(const Ctx &c, const StageLaunch &a, int *nblocks);
+// cu, cv += -G p on the interior faces, once per FP64 step (p fixed over its pseudo-iterations)
+hipError_t launch_fold_gradp(const Ctx &c);
 // mixed (defect-correction) pseudo-iteration: see kernels_stage.hip
 struct MixedStageLaunch {
     int stage;                   // 0: FP64 residual of the base; 1: FP32 tangent-linear update

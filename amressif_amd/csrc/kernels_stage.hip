@@ -110,8 +110,10 @@ __device__ __forceinline__ void finish_cells(const StageArgs &a, int lane, int i
 
 // ============================================================ FP64 stage
 // MODE: the launch's a.mode (0 stage, 1 last stage + norm, 2 RHS only, 3 mixed
-// residual) as a template argument, so each form compiles to its own kernel
-template <int MODE>
+// residual) as a template argument, so each form compiles to its own kernel.
+// FOLDP (modes 0, 1): the constant c_u, c_v already holds -G p on the interior
+// faces (k_fold_gradp, once per step), so p is not read.
+template <int MODE, bool FOLDP = false>
 __global__ __launch_bounds__(256) void k_stage64(StageArgs a)
 {
     using R = double;
@@ -146,7 +148,7 @@ __global__ __launch_bounds__(256) void k_stage64(StageArgs a)
         const bool row = jc >= 0 && jc < n;
         const size_t k = (size_t)(i < 0 ? 0 : i) + (size_t)P * (jc < 0 ? 0 : jc);
         r.uw = (xface && row) ? us[k] : R(0);
-        r.pc = (xin && row) ? a.p[k] : R(0);
+        r.pc = (!FOLDP && xin && row) ? a.p[k] : R(0);
         const bool out = row && jc >= j0 && a.mode != 2 && outlane;
         r.ub = (out && xface && a.mode != 3) ? a.ub[k] : R(0);
         r.cu = (out && xface) ? a.cu[k] : R(0);
@@ -181,7 +183,7 @@ __global__ __launch_bounds__(256) void k_stage64(StageArgs a)
             const bool row = jc >= 0 && jc < n;
             const size_t k = (size_t)(i < 0 ? 0 : i) + (size_t)P * (jc < 0 ? 0 : jc);
             r.uw = (xface && row) ? us[k] : R(0);
-            r.pc = (xin && row) ? a.p[k] : R(0);
+            r.pc = (!FOLDP && xin && row) ? a.p[k] : R(0);
             const bool out = row && jc >= j0 && MODE != 2 && outlane;
             r.ub = (out && xface && MODE != 3) ? a.ub[k] : R(0);
             r.cu = (out && xface) ? a.cu[k] : R(0);
@@ -243,7 +245,9 @@ __global__ __launch_bounds__(256) void k_stage64(StageArgs a)
             if (jc >= j0 && outlane && xface) {  // x-face (i, jc); wall faces get R = 0 exactly
                 skx = (unsigned)(i < 0 ? 0 : i) + (unsigned)P * jc;
                 const bool wall = i == 0 || i == n;
-                const double Rx = wall ? 0.0 : -(pc - pw) * a.inv_dx + R(0.5) * (TUw + TU);
+                const double Rx = wall ? 0.0
+                                 : FOLDP ? R(0.5) * (TUw + TU)
+                                         : -(pc - pw) * a.inv_dx + R(0.5) * (TUw + TU);
                 sxf = true;
                 if (MODE == 2) sx = Rx;
                 else if (MODE == 3) {
@@ -258,7 +262,9 @@ __global__ __launch_bounds__(256) void k_stage64(StageArgs a)
             if (jc + 1 < jend && jc + 1 >= j0 && outlane && xin) {  // y-face (i, jc + 1)
                 sky = (unsigned)i + (unsigned)P * (jc + 1);
                 const bool wall = jc + 1 == 0;
-                const double Ry = wall ? 0.0 : -(pup - pc) * a.inv_dy + R(0.5) * (TV + tVup);
+                const double Ry = wall ? 0.0
+                                 : FOLDP ? R(0.5) * (TV + tVup)
+                                         : -(pup - pc) * a.inv_dy + R(0.5) * (TV + tVup);
                 syf = true;
                 if (MODE == 2) sy = Ry;
                 else if (MODE == 3) {
@@ -354,7 +360,9 @@ __global__ __launch_bounds__(256) void k_stage64(StageArgs a)
             sk = (unsigned)(i < 0 ? 0 : i) + (unsigned)P * jc;
             if (xface) {  // x-face (i, jc); wall faces get R = 0 exactly
                 const bool wall = i == 0 || i == n;
-                const double Rx = wall ? 0.0 : -(pc - pw) * a.inv_dx + R(0.5) * (TUw + TU);
+                const double Rx = wall ? 0.0
+                                 : FOLDP ? R(0.5) * (TUw + TU)
+                                         : -(pc - pw) * a.inv_dx + R(0.5) * (TUw + TU);
                 sxf = true;
                 if (MODE == 2) sx = Rx;
                 else if (MODE == 3) {
@@ -368,7 +376,9 @@ __global__ __launch_bounds__(256) void k_stage64(StageArgs a)
             }
             if (xin) {  // y-face (i, jc)
                 const bool wall = jc == 0;
-                const double Ry = wall ? 0.0 : -(pc - pprev) * a.inv_dy + R(0.5) * (tVprev + TV);
+                const double Ry = wall ? 0.0
+                                 : FOLDP ? R(0.5) * (tVprev + TV)
+                                         : -(pc - pprev) * a.inv_dy + R(0.5) * (tVprev + TV);
                 syf = true;
                 if (MODE == 2) sy = Ry;
                 else if (MODE == 3) {
@@ -578,10 +588,17 @@ hipError_t launch_stage(const Ctx &c, const StageLaunch &s, int *nblocks)
     const dim3 grid = stage_grid(a);
     if ((int64_t)a.nbands * a.nchunks > kMaxPartials) return hipErrorInvalidValue;
     if (nblocks) *nblocks = a.nbands * a.nchunks;  // one norm partial per wave
+    if (s.foldp && s.mode > 1) return hipErrorInvalidValue;
     switch (s.mode) {
-    case 0: hipLaunchKernelGGL(k_stage64<0>, grid, dim3(256), 0, c.stream, a); break;
-    case 1: hipLaunchKernelGGL(k_stage64<1>, grid, dim3(256), 0, c.stream, a); break;
-    case 2: hipLaunchKernelGGL(k_stage64<2>, grid, dim3(256), 0, c.stream, a); break;
+    case 0:
+        if (s.foldp) hipLaunchKernelGGL((k_stage64<0, true>), grid, dim3(256), 0, c.stream, a);
+        else hipLaunchKernelGGL((k_stage64<0, false>), grid, dim3(256), 0, c.stream, a);
+        break;
+    case 1:
+        if (s.foldp) hipLaunchKernelGGL((k_stage64<1, true>), grid, dim3(256), 0, c.stream, a);
+        else hipLaunchKernelGGL((k_stage64<1, false>), grid, dim3(256), 0, c.stream, a);
+        break;
+    case 2: hipLaunchKernelGGL((k_stage64<2, false>), grid, dim3(256), 0, c.stream, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -597,7 +614,7 @@ hipError_t launch_stage_mixed(const Ctx &c, const MixedStageLaunch &s, int *nblo
     a.uk = s.uk; a.vk = s.vk; a.ru = s.ru; a.rv = s.rv;
     if (s.stage == 0) {  // FP64 residual of the pseudo base
         a.us = s.u64; a.vs = s.v64; a.cu = s.cu; a.cv = s.cv; a.p = s.p;
-        hipLaunchKernelGGL(k_stage64<3>, grid, dim3(256), 0, c.stream, a);
+        hipLaunchKernelGGL((k_stage64<3, false>), grid, dim3(256), 0, c.stream, a);
     } else {  // FP32 tangent-linear correction, base updated in place
         a.u64 = s.u64; a.v64 = s.v64; a.cr = s.cr; a.cj = s.cj;
         hipLaunchKernelGGL(k_tangent, grid, dim3(256), 0, c.stream, a);

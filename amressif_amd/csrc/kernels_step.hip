@@ -150,6 +150,29 @@ hipError_t launch_bdf2(const Ctx &c)
     return hipGetLastError();
 }
 
+// The pressure-gradient term of the momentum RHS (fn_flux_calc.cpp:825-911,
+// fn_rhs.cpp:8-68) folded into the BDF2 constant: p is the same in every
+// stage of a step, so c += -(p(i) - p(i - 1)) / dx on the interior x-faces and
+// likewise on the interior y-faces (the stage kernel's expression), and the
+// FP64 stage then reads no p.  Wall faces keep c (their RHS is 0).
+__global__ __launch_bounds__(256) void k_fold_gradp(double *cu, double *cv, const double *p, int n, int P,
+                                                    double inv_dx, double inv_dy)
+{
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63), j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (i > n || j > n) return;
+    const size_t k = i + (size_t)P * j;
+    if (i > 0 && i < n && j < n) cu[k] += -(p[k] - p[k - 1]) * inv_dx;
+    if (j > 0 && j < n && i < n) cv[k] += -(p[k] - p[k - P]) * inv_dy;
+}
+
+hipError_t launch_fold_gradp(const Ctx &c)
+{
+    const int n = c.g.n;
+    hipLaunchKernelGGL(k_fold_gradp, dim3((n + 1 + 63) / 64, (n + 1 + 3) / 4), dim3(256), 0, c.stream, c.C[0], c.C[1],
+                       (const double *)c.p, n, c.g.P, 1.0 / c.g.dx, 1.0 / c.g.dy);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------ Poisson RHS
 // fn_rhs.cpp:71-125: D = (u_{i+1}-u_i)/dx + (v_{j+1}-v_j)/dy, b = (1.5/dt) D,
 // block partials of ||D||inf and sum b.

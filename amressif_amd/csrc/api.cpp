@@ -303,7 +303,7 @@ int mg_run(amressif_ctx *ctx, const WarmStart &w, bool center, const double *us,
     Ctx &c = ctx->c;
     const int mixed = c.in.precision == AMRESSIF_MIXED;
     int nb = 0;
-    HIPCHK(amr::launch_mg_start(c, require_mom ? 1 : 0));
+    HIPCHK(amr::launch_mg_start(c, require_mom ? 1 : 0, us, vs));
     {
         // per cell: u*, v* in (or b), the warm-start iterates in, b (when formed or
         // centred), phi0 and the residual (V-cycle precision) out
@@ -361,10 +361,12 @@ int launch_pseudo_iteration(amressif_ctx *ctx, int it, int *nb);
 // BDF2 history and the RK4 dual-time pseudo-iterations of one step
 // (main.cpp:879-884, 898-1040): c.U holds the converged predictor u* after it,
 // the control block copy c.ctl_host its count and flags
-int amr_momentum_begin(amressif_ctx *ctx, bool bdf_ready, int *launched)
+// reset_done: the previous step's k_fin_project has already cleared the
+// pseudo-iteration control
+int amr_momentum_begin(amressif_ctx *ctx, bool bdf_ready, int *launched, bool reset_done)
 {
     Ctx &c = ctx->c;
-    HIPCHK(amr::launch_ctl_reset_step(c));
+    if (!reset_done) HIPCHK(amr::launch_ctl_reset_step(c));
     if (bdf_ready) {  // the previous projection formed C and copied u^n to S2
         std::swap(c.Uprev[0], c.S2[0]);
         std::swap(c.Uprev[1], c.S2[1]);
@@ -403,10 +405,10 @@ int amr_momentum_finish(amressif_ctx *ctx, int launched)
     return 0;
 }
 
-int amr_momentum_predictor(amressif_ctx *ctx, bool bdf_ready)
+int amr_momentum_predictor(amressif_ctx *ctx, bool bdf_ready, bool reset_done)
 {
     int launched = 0;
-    if (int rc = amr_momentum_begin(ctx, bdf_ready, &launched)) return rc;
+    if (int rc = amr_momentum_begin(ctx, bdf_ready, &launched, reset_done)) return rc;
     return amr_momentum_finish(ctx, launched);
 }
 
@@ -1002,7 +1004,7 @@ int advance_impl(amressif_ctx *ctx, int nsteps, amressif_step_stats *stats, amre
         int nb = 0;
         if (speculate) {
             int launched = 0, rc = 0;
-            if ((rc = amr_momentum_begin(ctx, bdf_ready, &launched))) return rc;
+            if ((rc = amr_momentum_begin(ctx, bdf_ready, &launched, s > 0))) return rc;
             bdf_ready = false;
             const WarmStart w = mg_prepare(ctx, warm, &rc);
             if (rc) return rc;
@@ -1016,7 +1018,7 @@ int advance_impl(amressif_ctx *ctx, int nsteps, amressif_step_stats *stats, amre
                 amr::kprof_close(c, amr::KL_MOM, c.ctl_host->mom_count);
             }
         } else {
-            if (int rc = amr_momentum_predictor(ctx, bdf_ready)) return rc;
+            if (int rc = amr_momentum_predictor(ctx, bdf_ready, s > 0)) return rc;
             bdf_ready = false;
         }
         const int flags = c.ctl_host->flags;
@@ -1039,7 +1041,8 @@ int advance_impl(amressif_ctx *ctx, int nsteps, amressif_step_stats *stats, amre
         HIPCHK(amr::kprof_begin(c, amr::KP_PROJECT, amr::KL_ONCE, 0, (fold ? 104.0 : 56.0) * (double)c.g.n * c.g.n));
         HIPCHK(amr::launch_project(c, c.U[0], c.U[1], c.S1[0], c.S1[1], &nb, nullptr, nullptr, fold ? 1 : 0));
         HIPCHK(amr::kprof_end(c));
-        HIPCHK(amr::launch_fin_project(c, nb, s));
+        // (between two steps of the call it also clears the next step's control)
+        HIPCHK(amr::launch_fin_project(c, nb, s, s + 1 < nsteps ? 1 : 0));
         bdf_ready = fold;
         std::swap(c.U[0], c.S1[0]);
         std::swap(c.U[1], c.S1[1]);

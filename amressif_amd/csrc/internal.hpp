@@ -222,7 +222,10 @@ hipError_t launch_poisson_rhs(const Ctx &c, const double *u, const double *v, do
                               const float *au = nullptr, const float *av = nullptr);
 hipError_t launch_fin_prhs(const Ctx &c, int nblocks);
 hipError_t launch_center_b(const Ctx &c, double *b);
-hipError_t launch_mg_start(const Ctx &c, int require_mom = 0);
+// control fields of a Poisson solve; with the predictor faces us, vs also the
+// mean of the b they give (to be followed by launch_poisson_start with them)
+hipError_t launch_mg_start(const Ctx &c, int require_mom = 0, const double *us = nullptr,
+                           const double *vs = nullptr);
 // first residual of a Poisson solve (see kernels_mg.hip), with its finalize
 hipError_t launch_poisson_start(const Ctx &c, int mode, int center, const double *phi1, const double *phi2,
                                 double *phi0, int mixed, const double *us = nullptr, const double *vs = nullptr);
@@ -238,7 +241,7 @@ hipError_t launch_smooth_fine(const Ctx &c, int kind, double *scratch_phi);
 // velocity in S2 (that step then takes S2 as u^n and skips launch_bdf2)
 hipError_t launch_project(const Ctx &c, const double *us, const double *vs, double *uo, double *vo, int *nblocks,
                           const float *au = nullptr, const float *av = nullptr, int next_bdf = 0);
-hipError_t launch_fin_project(const Ctx &c, int nblocks, int step_index);
+hipError_t launch_fin_project(const Ctx &c, int nblocks, int step_index, int reset_next = 0);
 hipError_t launch_cont2cart_out(const Ctx &c, const double *u, const double *v, const amr_array4 &vc);
 hipError_t launch_pack_face(const Ctx &c, const amr_array4 &a, double *dst, int dir);
 hipError_t launch_unpack_face(const Ctx &c, const double *src, const amr_array4 &a, int dir);
@@ -333,10 +336,10 @@ hipError_t launch_adj_finish(const Ctx &c, const double *su, const double *sv, d
 struct amressif_ctx {
     amr::Ctx c;
 };
-int amr_momentum_predictor(amressif_ctx *ctx, bool bdf_ready = false);
+int amr_momentum_predictor(amressif_ctx *ctx, bool bdf_ready = false, bool reset_done = false);
 // its two halves: the step's set-up and first batch of pseudo-iterations (no
 // host wait), then the polled continuation until the loop ends
-int amr_momentum_begin(amressif_ctx *ctx, bool bdf_ready, int *launched);
+int amr_momentum_begin(amressif_ctx *ctx, bool bdf_ready, int *launched, bool reset_done = false);
 int amr_momentum_finish(amressif_ctx *ctx, int launched);
 int amr_poll(amressif_ctx *ctx);
 int amr_set_ctl(amressif_ctx *ctx);

@@ -1119,27 +1119,46 @@ __global__ __launch_bounds__(256) void k_correct_residual(const double *phi_in, 
 
 // require_mom: the step's Poisson phase was enqueued before the host saw the
 // pseudo-iteration loop end; if it has not ended, the whole phase is skipped
-// (mg_done and mg_skip set) and the host runs it again later
-__global__ void k_mg_start(Ctl *ctl, int require_mom)
+// (mg_done and mg_skip set) and the host runs it again later.  With the
+// predictor faces u, v it also forms the mean of b for k_poisson_start(_rows).
+__global__ __launch_bounds__(1024) void k_mg_start(Ctl *ctl, int require_mom, const double *u, const double *v,
+                                                   int n, int P, double dx, double dy, double dt)
 {
-    const int skip = require_mom && !ctl->mom_done;
-    ctl->mg_done = skip;
-    ctl->mg_skip = skip;
-    ctl->gm_stop = 0;
-    ctl->mg_iters = 0;
-    ctl->mg_started = 0;
-    ctl->b_norm_fixed = 0;
-    ctl->res_norm = 0.0;
-    ctl->b_norm = 0.0;
-    ctl->phi_sum = 0.0;
-    ctl->phi_mean = 0.0;
+    __shared__ double sh[16];
+    __shared__ int skip_s;
+    if (threadIdx.x == 0) {
+        const int skip = require_mom && !ctl->mom_done;
+        ctl->mg_done = skip;
+        ctl->mg_skip = skip;
+        ctl->gm_stop = 0;
+        ctl->mg_iters = 0;
+        ctl->mg_started = 0;
+        ctl->b_norm_fixed = 0;
+        ctl->res_norm = 0.0;
+        ctl->b_norm = 0.0;
+        ctl->phi_sum = 0.0;
+        ctl->phi_mean = 0.0;
+        skip_s = skip;
+    }
+    __syncthreads();
+    if (!u || skip_s) return;
+    // with the predictor faces: the mean of b = (1.5/dt) D(u*) from the
+    // boundary fluxes (sum of D over the cells = net outflow), for the start
+    double sum = 0.0;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        const size_t row = (size_t)P * j;
+        sum += (u[row + n] - u[row]) / dx;
+    }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) sum += (v[(size_t)P * n + i] - v[i]) / dy;
+    sum = block_reduce<false>(sum, sh);
+    if (threadIdx.x == 0) ctl->b_mean = (1.5 / dt) * sum / ((double)n * (double)n);
 }
 
 // Start of the Poisson iteration, one pass.  The right-hand side is either b
 // (b <- b - mean(b) in place when center is set) or, with FROM_U, formed here
 // from the predictor faces u*, v* as b = (1.5/dt) D(u*) - mean (the step's
 // poisson_righthand_side_calc, fn_rhs.cpp:71-125, with the mean from
-// k_flux_mean) and stored.  Then the starting iterate phi0 = 0 (mode 0), phi1
+// k_mg_start) and stored.  Then the starting iterate phi0 = 0 (mode 0), phi1
 // (mode 1) or the extrapolation 2 phi1 - phi2 (mode 2) of the two previous
 // solutions (newest first), r0 = b - L phi0 in FP64 (stored T), and block
 // partials of ||r0||inf, sum phi0, ||b||inf and, with FROM_U, ||D(u*)||inf and
@@ -1353,21 +1372,6 @@ __global__ __launch_bounds__(256) void k_poisson_start_rows(double *b, const dou
     }
 }
 
-__global__ __launch_bounds__(1024) void k_flux_mean(const double *u, const double *v, int n, int P, double dx,
-                                                    double dy, double dt, Ctl *ctl)
-{
-    __shared__ double sh[16];
-    if (ctl->mg_done) return;  // a skipped phase (k_mg_start)
-    double s = 0.0;
-    for (int j = threadIdx.x; j < n; j += blockDim.x) {
-        const size_t row = (size_t)P * j;
-        s += (u[row + n] - u[row]) / dx;
-    }
-    for (int i = threadIdx.x; i < n; i += blockDim.x) s += (v[(size_t)P * n + i] - v[i]) / dy;
-    s = block_reduce<false>(s, sh);
-    if (threadIdx.x == 0) ctl->b_mean = (1.5 / dt) * s / ((double)n * (double)n);
-}
-
 // the first residual's norm and ||b||inf (and with from_u the divergence
 // statistics of u*), and the stopping test on it
 __global__ __launch_bounds__(1024) void k_fin_start(const double *partials, int nb, Ctl *ctl, int n, int from_u)
@@ -1410,8 +1414,7 @@ hipError_t launch_poisson_start(const Ctx &c, int mode, int center, const double
     const int n = c.g.n;
     const Level &L = c.lev[0];
     const double dx = c.g.dx, dy = c.g.dy, dt = c.g.dt;
-    if (us) hipLaunchKernelGGL(k_flux_mean, dim3(1), dim3(1024), 0, c.stream, us, vs, n, c.g.P, dx, dy, dt, c.ctl);
-    int nb = 0;
+    int nb = 0;  // (with us, launch_mg_start has formed the mean of b)
     if (c.start_rows > 0) {  // row-march kernel
         const int rows = c.start_rows;
         const int nbands = (n + kStartWB - 1) / kStartWB, nwaves = nbands * ((n + rows - 1) / rows);
@@ -1450,9 +1453,10 @@ hipError_t launch_poisson_start(const Ctx &c, int mode, int center, const double
     return hipGetLastError();
 }
 
-hipError_t launch_mg_start(const Ctx &c, int require_mom)
+hipError_t launch_mg_start(const Ctx &c, int require_mom, const double *us, const double *vs)
 {
-    hipLaunchKernelGGL(k_mg_start, dim3(1), dim3(1), 0, c.stream, c.ctl, require_mom);
+    hipLaunchKernelGGL(k_mg_start, dim3(1), dim3(1024), 0, c.stream, c.ctl, require_mom, us, vs, c.g.n, c.g.P, c.g.dx,
+                       c.g.dy, c.g.dt);
     return hipGetLastError();
 }
 

@@ -502,8 +502,10 @@ hipError_t launch_project(const Ctx &c, const double *us, const double *vs, doub
     return launch_project_ex(c, us, vs, uo, vo, c.p, c.phi, 1, nblocks, au, av, next_bdf);
 }
 
+// reset_next: also clear the pseudo-iteration control for the next step (as
+// k_ctl_reset_step does), after this step's statistics are logged
 __global__ __launch_bounds__(1024) void k_fin_project(const double *partials, int nb, Ctl *ctl,
-                                                      amressif_step_stats *log, int step_index)
+                                                      amressif_step_stats *log, int step_index, int reset_next)
 {
     __shared__ double sh[16];
     double m = 0.0;
@@ -527,13 +529,19 @@ __global__ __launch_bounds__(1024) void k_fin_project(const double *partials, in
             s.div_new = m;
             log[step_index] = s;
         }
+        if (reset_next) {
+            ctl->mom_done = 0;
+            ctl->mom_count = 0;
+            ctl->flags = 0;
+            ctl->norm_error = 1.e9;
+        }
     }
 }
 
-hipError_t launch_fin_project(const Ctx &c, int nblocks, int step_index)
+hipError_t launch_fin_project(const Ctx &c, int nblocks, int step_index, int reset_next)
 {
     hipLaunchKernelGGL(k_fin_project, dim3(1), dim3(1024), 0, c.stream, c.partials, nblocks, c.ctl, c.log,
-                       step_index);
+                       step_index, reset_next);
     return hipGetLastError();
 }
 

@@ -372,10 +372,15 @@ int amr_momentum_begin(amressif_ctx *ctx, bool bdf_ready, int *launched, bool re
         std::swap(c.Uprev[1], c.S2[1]);
     } else {
         HIPCHK(amr::launch_bdf2(c));
+        c.c_gradp = 0;
     }
-    // FP64: p^n is the same in every stage of the step; its gradient term goes
-    // into C once and the stage kernels read no p
-    if (c.in.precision != AMRESSIF_MIXED) HIPCHK(amr::launch_fold_gradp(c));
+    // p^n is the same in every stage of the step: its gradient term sits in C
+    // (put there by the previous projection, or here) and the stage kernels
+    // read no p
+    if (!c.c_gradp) {
+        HIPCHK(amr::launch_fold_gradp(c));
+        c.c_gradp = 1;
+    }
     const int target = std::min(std::max(1, c.last_P), c.in.IterNum + 1);
     int nb = 0;
     for (*launched = 0; *launched < target; ++*launched) {
@@ -429,6 +434,7 @@ int launch_pseudo_iteration(amressif_ctx *ctx, int it, int *nb)
     if (c.in.precision == AMRESSIF_MIXED) {
         amr::MixedStageLaunch a{};
         a.u64 = c.U[0]; a.v64 = c.U[1]; a.cu = c.C[0]; a.cv = c.C[1]; a.p = c.p;
+        a.foldp = 1;  // C holds -G p (amr_momentum_begin)
         a.uk = c.uk32[0]; a.vk = c.uk32[1]; a.ru = c.r32[0]; a.rv = c.r32[1];
         const double c15 = 1.5 / c.g.dt;
         if (c.in.mixed_momentum == 1) {
@@ -441,10 +447,10 @@ int launch_pseudo_iteration(amressif_ctx *ctx, int it, int *nb)
             a.cr = c.rk[3] - a.cj * c15;
         }
         a.stage = 0;
-        // algorithmic bytes per cell: residual 56 (u^k, v^k, c_u, c_v, p in; FP32 base and
+        // algorithmic bytes per cell: residual 48 (u^k, v^k, c_u, c_v in; FP32 base and
         // residual out), tangent 48 (FP32 base and residual in; u^k, v^k in and out)
         const double cells = (double)c.g.n * c.g.n;
-        HIPCHK(amr::kprof_begin(c, amr::KP_RESID, amr::KL_MOM, it, 56.0 * cells));
+        HIPCHK(amr::kprof_begin(c, amr::KP_RESID, amr::KL_MOM, it, 48.0 * cells));
         HIPCHK(amr::launch_stage_mixed(c, a, nb));
         HIPCHK(amr::kprof_end(c));
         a.stage = 1;
@@ -773,7 +779,7 @@ int amressif_create(const amressif_inputs *in, void *hip_stream, amressif_ctx **
         ALLOC(&c.U[d], fsz); ALLOC(&c.Uprev[d], fsz); ALLOC(&c.C[d], fsz);
         ALLOC(&c.S1[d], fsz); ALLOC(&c.S2[d], fsz);
     }
-    ALLOC(&c.p, fsz); ALLOC(&c.phiv[0], fsz); ALLOC(&c.phiv[1], fsz); ALLOC(&c.b, fsz);
+    ALLOC(&c.p, fsz); ALLOC(&c.p_alt, fsz); ALLOC(&c.phiv[0], fsz); ALLOC(&c.phiv[1], fsz); ALLOC(&c.b, fsz);
     c.phi = c.phiv[0];
     if (in->precision == AMRESSIF_MIXED)
         for (int d = 0; d < 2; ++d) {
@@ -869,6 +875,7 @@ int amressif_create(const amressif_inputs *in, void *hip_stream, amressif_ctx **
         hipMemset(c.Uprev[d], 0, fsz * 8);
     }
     hipMemset(c.p, 0, fsz * 8);
+    hipMemset(c.p_alt, 0, fsz * 8);
     hipMemset(c.phiv[0], 0, fsz * 8);
     hipMemset(c.phiv[1], 0, fsz * 8);
     if (hipDeviceSynchronize() != hipSuccess) return bail(AMRESSIF_ERR_HIP, "device sync failed");
@@ -1038,9 +1045,14 @@ int advance_impl(amressif_ctx *ctx, int nsteps, amressif_step_stats *stats, amre
         const bool fold = s + 1 < nsteps && !(flags & AMRESSIF_FLAG_NONFINITE);
         // per cell: u*, v*, phi, p in, u, v, p out (56 B); with the next step's
         // BDF2 constant also u^n in, c and the copy of u^{n+1} out (104 B)
+        // p^{n+1} goes to the other pressure array (p^n stays readable for the
+        // neighbour terms), and the next constant takes -G p^{n+1}
         HIPCHK(amr::kprof_begin(c, amr::KP_PROJECT, amr::KL_ONCE, 0, (fold ? 104.0 : 56.0) * (double)c.g.n * c.g.n));
-        HIPCHK(amr::launch_project(c, c.U[0], c.U[1], c.S1[0], c.S1[1], &nb, nullptr, nullptr, fold ? 1 : 0));
+        HIPCHK(amr::launch_project_ex(c, c.U[0], c.U[1], c.S1[0], c.S1[1], c.p, c.phi, 1, &nb, nullptr, nullptr,
+                                      fold ? 1 : 0, c.p_alt));
         HIPCHK(amr::kprof_end(c));
+        std::swap(c.p, c.p_alt);
+        c.c_gradp = fold && c.project_rows > 0;
         // (between two steps of the call it also clears the next step's control)
         HIPCHK(amr::launch_fin_project(c, nb, s, s + 1 < nsteps ? 1 : 0));
         bdf_ready = fold;
@@ -1275,6 +1287,7 @@ int amressif_time_kernel(amressif_ctx *ctx, int which, int reps, int64_t copy_by
             a.u64 = which == 3 ? c.U[0] : c.S2[0];  // the residual only reads it
             a.v64 = which == 3 ? c.U[1] : c.S2[1];
             a.cu = c.C[0]; a.cv = c.C[1]; a.p = c.p;
+            a.foldp = 1;  // the step's form (C holds -G p)
             a.uk = c.uk32[0]; a.vk = c.uk32[1]; a.ru = c.r32[0]; a.rv = c.r32[1];
             a.cj = 0.0;  // keeps the timed in-place update of S2 a no-op in value
             a.cr = 0.0;
@@ -1318,14 +1331,14 @@ int amressif_time_kernel(amressif_ctx *ctx, int which, int reps, int64_t copy_by
         // 0 FP64 stage: reads u*, v*, u^k, v^k, c_u, c_v (with -G p), writes u*, v* (8 x 8 B);
         // 1 finest down-leg smoother: reads r, writes e (2 t), the coarse r (t / 4);
         // 2 copy: read + write of copy_bytes;
-        // 3 mixed residual: reads u^k, v^k, c_u, c_v, p (5 x 8 B), writes the FP32 base and
-        //   residual (4 x 4 B);
+        // 3 mixed residual: reads u^k, v^k, c_u, c_v (with -G p; 4 x 8 B), writes the FP32 base
+        //   and residual (4 x 4 B);
         // 4 tangent-linear update: reads the FP32 base and residual (4 x 4 B), reads and
         //   writes u^k, v^k (4 x 8 B);
         // 5 finest up-leg smoother + FP64 outer step: reads e, r (2 t), the coarse e (t / 4),
         //   phi, b (2 x 8 B), writes phi (8 B) and the next r (t)
         const double t = c.in.precision == AMRESSIF_MIXED ? 4.0 : 8.0;
-        const double per_cell[6] = {64.0, 2.25 * t, 0.0, 56.0, 48.0, 3.25 * t + 24.0};
+        const double per_cell[6] = {64.0, 2.25 * t, 0.0, 48.0, 48.0, 3.25 * t + 24.0};
         *bytes_per_launch = which == 2 ? 2.0 * (double)copy_bytes : per_cell[which] * cells;
     }
     return AMRESSIF_OK;

@@ -128,6 +128,8 @@ struct Ctx {
     // its residual
     float *uk32[2]{}, *r32[2]{};
     double *p = nullptr, *phi = nullptr, *b = nullptr;
+    double *p_alt = nullptr;  // the step's projection writes p^{n+1} here; then the two swap
+    int c_gradp = 0;          // C holds -G p of the current p (amr_momentum_begin)
     // dual state of the tangent-linear / adjoint step (faces, faces, cells, lid)
     double *Du[2]{}, *Dup[2]{}, *Dp = nullptr, *Dlid = nullptr;
     AdjWork *adj = nullptr;  // their working arrays (allocated on first use)
@@ -213,6 +215,7 @@ struct MixedStageLaunch {
     float *ru, *rv;              // FP64 residual of the base, stored FP32 (written by stage 0)
     double cr, cj;               // stage 1: u^{k+1} = u^k + cr r + cj R'(u^k) r
     int unguarded;
+    int foldp;                   // stage 0: cu, cv already hold -G p; p is not read
 };
 hipError_t launch_stage_mixed(const Ctx &c, const MixedStageLaunch &a, int *nblocks);
 hipError_t launch_fin_pseudo(const Ctx &c, int nblocks);
@@ -253,9 +256,12 @@ hipError_t launch_norm(const Ctx &c, const double *ub, const double *vb, const d
                        double *out_dev);
 hipError_t launch_ctl_set(const Ctx &c, double tol_mom, double tol_rel, double tol_abs, int iter_num,
                           int mg_max_iter);
+// p_out: where p + phi - mean goes (null: in place).  With next_bdf and a
+// separate p_out, the row-march kernel also folds -G p^{n+1} into the next
+// constant (project_rows > 0).
 hipError_t launch_project_ex(const Ctx &c, const double *us, const double *vs, double *uo, double *vo, double *p,
                              const double *phi, int use_mean, int *nblocks, const float *au = nullptr,
-                             const float *av = nullptr, int next_bdf = 0);
+                             const float *av = nullptr, int next_bdf = 0, double *p_out = nullptr);
 hipError_t launch_unpack_cell_shift(const Ctx &c, const double *src, const amr_array4 &a, int comp,
                                     int fill_ghosts, const double *shift);
 hipError_t launch_copy_bytes(const Ctx &c, const void *src, void *dst, int64_t bytes);

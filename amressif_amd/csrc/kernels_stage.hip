@@ -111,8 +111,8 @@ __device__ __forceinline__ void finish_cells(const StageArgs &a, int lane, int i
 // ============================================================ FP64 stage
 // MODE: the launch's a.mode (0 stage, 1 last stage + norm, 2 RHS only, 3 mixed
 // residual) as a template argument, so each form compiles to its own kernel.
-// FOLDP (modes 0, 1): the constant c_u, c_v already holds -G p on the interior
-// faces (k_fold_gradp, once per step), so p is not read.
+// FOLDP (modes 0, 1, 3): the constant c_u, c_v already holds -G p on the
+// interior faces (k_fold_gradp or the projection), so p is not read.
 template <int MODE, bool FOLDP = false>
 __global__ __launch_bounds__(256) void k_stage64(StageArgs a)
 {
@@ -614,7 +614,8 @@ hipError_t launch_stage_mixed(const Ctx &c, const MixedStageLaunch &s, int *nblo
     a.uk = s.uk; a.vk = s.vk; a.ru = s.ru; a.rv = s.rv;
     if (s.stage == 0) {  // FP64 residual of the pseudo base
         a.us = s.u64; a.vs = s.v64; a.cu = s.cu; a.cv = s.cv; a.p = s.p;
-        hipLaunchKernelGGL((k_stage64<3, false>), grid, dim3(256), 0, c.stream, a);
+        if (s.foldp) hipLaunchKernelGGL((k_stage64<3, true>), grid, dim3(256), 0, c.stream, a);
+        else hipLaunchKernelGGL((k_stage64<3, false>), grid, dim3(256), 0, c.stream, a);
     } else {  // FP32 tangent-linear correction, base updated in place
         a.u64 = s.u64; a.v64 = s.v64; a.cr = s.cr; a.cj = s.cj;
         hipLaunchKernelGGL(k_tangent, grid, dim3(256), 0, c.stream, a);

@@ -289,11 +289,13 @@ hipError_t launch_center_b(const Ctx &c, double *b)
 struct NextBdf {
     const double *up_u, *up_v;
     double *cu, *cv, *wu, *wv;
+    int gradp;  // k_project_rows: also fold -G p^{n+1} into c (interior faces)
 };
 
 __global__ __launch_bounds__(256) void k_project(const double *us, const double *vs, const float *au,
-                                                 const float *av, double *uo, double *vo, double *p,
-                                                 const double *phi, int n, int P, double dx, double dy, double dt,
+                                                 const float *av, double *uo, double *vo, const double *p,
+                                                 double *p_out, const double *phi, int n, int P, double dx,
+                                                 double dy, double dt,
                                                  const Ctl *ctl, int use_mean, double *partials, int ntx, int nty,
                                                  NextBdf nx)
 {
@@ -334,7 +336,7 @@ __global__ __launch_bounds__(256) void k_project(const double *us, const double 
         put(vo, nx.up_v, nx.cv, nx.wv, k, vsth);
         if (i == n - 1) put(uo, nx.up_u, nx.cu, nx.wu, k + 1, ue);
         if (j == n - 1) put(vo, nx.up_v, nx.cv, nx.wv, k + P, vn);
-        p[k] = p[k] + (ph - pm);
+        p_out[k] = p[k] + (ph - pm);
         const double dv = fabs((ue - uw) / dx + (vn - vsth) / dy);
         m = (dv > m || dv != dv) ? dv : m;
     }
@@ -349,11 +351,15 @@ __global__ __launch_bounds__(256) void k_project(const double *us, const double 
 // and a cell costs one load of each input and one store of each output.  The
 // next row's loads go out one iteration ahead, a row's stores one iteration
 // behind (one wait for memory per row).  x-face n is written by the lane with
-// i = n, y-face n by the row n - 1.  Per-face arithmetic is that of k_project.
+// i = n, y-face n by the row n - 1.  Per-face arithmetic is that of k_project;
+// with nx.gradp (p_out separate from p) the next step's constant also takes
+// -G p^{n+1} on the interior faces, as k_fold_gradp adds it: p^{n+1} of the
+// west cell by DPP, of the south cell from the row before.
 constexpr int kProjWB = 62;
 __global__ __launch_bounds__(256) void k_project_rows(const double *us, const double *vs, const float *au,
-                                                      const float *av, double *uo, double *vo, double *p,
-                                                      const double *phi, int n, int P, double dx, double dy,
+                                                      const float *av, double *uo, double *vo, const double *p,
+                                                      double *p_out, const double *phi, int n, int P, double dx,
+                                                      double dy,
                                                       double dt, const Ctl *ctl, int use_mean, double *partials,
                                                       int nbands, int nwaves, int rows, NextBdf nx)
 {
@@ -371,6 +377,11 @@ __global__ __launch_bounds__(256) void k_project_rows(const double *us, const do
     const bool outc = xin && own, outx = xface && own;
     const bool fold = nx.cu != nullptr;
     const double pm = use_mean ? ctl->phi_mean : 0.0;
+    // with nx.gradp, p_out is a separate array: p (p^n) is read-only here, so
+    // the halo lane and the row before the chunk can form p^{n+1} of cells
+    // that other waves own
+    const bool gradp = fold && nx.gradp;
+    const double inv_dx = 1.0 / dx, inv_dy = 1.0 / dy;
     struct Raw {
         double phn;       // phi (i, j + 1)
         double us, vsn;   // u* at x-face (i, j), v* at y-face (i, j + 1)
@@ -384,7 +395,7 @@ __global__ __launch_bounds__(256) void k_project_rows(const double *us, const do
         if (xin && j + 1 < n) q.phn = phi[k + P];
         if (xface) q.us = au ? us[k] + (double)au[k] : us[k];
         if (xin) q.vsn = av ? vs[k + P] + (double)av[k + P] : vs[k + P];
-        if (outc) q.pc = p[k];
+        if (outc || (gradp && xin)) q.pc = p[k];
         if (fold) {
             if (outx) q.upu = nx.up_u[k];
             if (outc) {
@@ -394,13 +405,14 @@ __global__ __launch_bounds__(256) void k_project_rows(const double *us, const do
         }
     };
     auto bdf = [&](double un, double um) { return (1.5 / dt) * un + (0.5 / dt) * (un - um); };
-    // prologue: phi (i, j0) and the new y-face (i, j0)
-    double phC = 0.0, vprev = 0.0;
+    // prologue: phi (i, j0), the new y-face (i, j0) and, folding -G p, p^{n+1} (i, j0 - 1)
+    double phC = 0.0, vprev = 0.0, pnS = 0.0;
     if (jend > j0 && xin) {
         const size_t k = i + (size_t)P * j0;
         phC = phi[k];
         const double vstar = av ? vs[k] + (double)av[k] : vs[k];
         vprev = j0 == 0 ? vstar : vstar - (((phC - phi[k - P]) / dy) * dt / 1.5);
+        if (gradp && j0 > 0) pnS = p[k - P] + (phi[k - P] - pm);
     }
     double m = 0.0;
     // deferred stores of row j - 1
@@ -414,7 +426,7 @@ __global__ __launch_bounds__(256) void k_project_rows(const double *us, const do
         }
         if (sy) {
             vo[sk] = sv;
-            p[sk] = sp;
+            p_out[sk] = sp;
             if (fold) { nx.cv[sk] = scv; nx.wv[sk] = sv; }
         }
         if (syn) {
@@ -432,17 +444,21 @@ __global__ __launch_bounds__(256) void k_project_rows(const double *us, const do
         const double uw = (i == 0 || i == n) ? cur.us : cur.us - (((phC - phW) / dx) * dt / 1.5);
         const double ue = lane_next(uw);
         const double vn = j + 1 == n ? cur.vsn : cur.vsn - (((cur.phn - phC) / dy) * dt / 1.5);
+        const double pn = cur.pc + (phC - pm);  // p^{n+1} (i, j)
+        const double pnW = lane_prev(pn);
         sk = (size_t)(i < 0 ? 0 : i) + (size_t)P * j;
         if (outx) {
             sx = true;
             su = uw;
             if (fold) scu = bdf(uw, cur.upu);
+            if (gradp && i > 0 && i < n) scu += -(pn - pnW) * inv_dx;  // as k_fold_gradp adds it
         }
         if (outc) {
             sy = true;
             sv = vprev;
-            sp = cur.pc + (phC - pm);
+            sp = pn;
             if (fold) scv = bdf(vprev, cur.upv);
+            if (gradp && j > 0) scv += -(pn - pnS) * inv_dy;
             if (j == n - 1) {
                 syn = true;
                 svn = vn;
@@ -453,6 +469,7 @@ __global__ __launch_bounds__(256) void k_project_rows(const double *us, const do
         }
         phC = cur.phn;
         vprev = vn;
+        pnS = pn;
         cur = nxt;
     }
     flush();
@@ -471,9 +488,12 @@ __global__ __launch_bounds__(256) void k_project_rows(const double *us, const do
 
 hipError_t launch_project_ex(const Ctx &c, const double *us, const double *vs, double *uo, double *vo, double *p,
                              const double *phi, int use_mean, int *nblocks, const float *au, const float *av,
-                             int next_bdf)
+                             int next_bdf, double *p_out)
 {
-    const NextBdf nx = next_bdf ? NextBdf{c.Uprev[0], c.Uprev[1], c.C[0], c.C[1], c.S2[0], c.S2[1]} : NextBdf{};
+    if (!p_out) p_out = p;
+    const int gradp = next_bdf && p_out != p && c.project_rows > 0;
+    const NextBdf nx =
+        next_bdf ? NextBdf{c.Uprev[0], c.Uprev[1], c.C[0], c.C[1], c.S2[0], c.S2[1], gradp} : NextBdf{};
     const int n = c.g.n;
     if (c.project_rows > 0) {  // row-march kernel
         const int rows = c.project_rows;
@@ -482,16 +502,16 @@ hipError_t launch_project_ex(const Ctx &c, const double *us, const double *vs, d
         const int nb = ((nwaves + 3) / 4 + 7) / 8 * 8;  // four waves per workgroup, a multiple of 8
         if (nb > kMaxPartials) return hipErrorInvalidValue;
         *nblocks = nb;
-        hipLaunchKernelGGL(k_project_rows, dim3(nb), dim3(256), 0, c.stream, us, vs, au, av, uo, vo, p, phi, n,
-                           c.g.P, c.g.dx, c.g.dy, c.g.dt, (const Ctl *)c.ctl, use_mean, c.partials, nbands, nwaves,
-                           rows, nx);
+        hipLaunchKernelGGL(k_project_rows, dim3(nb), dim3(256), 0, c.stream, us, vs, au, av, uo, vo, (const double *)p,
+                           p_out, phi, n, c.g.P, c.g.dx, c.g.dy, c.g.dt, (const Ctl *)c.ctl, use_mean, c.partials,
+                           nbands, nwaves, rows, nx);
         return hipGetLastError();
     }
     const int ntx = (n + 63) / 64, nty = (n + 15) / 16;
     const int nb = (ntx * nty + 7) / 8 * 8;  // 64 x 16 tiles, a multiple of 8 workgroups
     if (nb > kMaxPartials) return hipErrorInvalidValue;
     *nblocks = nb;
-    hipLaunchKernelGGL(k_project, dim3(nb), dim3(256), 0, c.stream, us, vs, au, av, uo, vo, p, phi, n, c.g.P, c.g.dx,
+    hipLaunchKernelGGL(k_project, dim3(nb), dim3(256), 0, c.stream, us, vs, au, av, uo, vo, (const double *)p, p_out, phi, n, c.g.P, c.g.dx,
                        c.g.dy, c.g.dt, c.ctl, use_mean, c.partials, ntx, nty, nx);
     return hipGetLastError();
 }

@@ -48,13 +48,13 @@ sys.path.insert(0, ROOT)
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 # timing kinds of amressif_time_kernel (--kernel-probe) -> kernel symbol
 KERNELS = {0: "amr::k_stage64<0, true> (FP64 stage)", 1: "amr::k_gsrb2<T, 1, R> (finest down-leg smoother + restriction)",
-           3: "amr::k_stage64<3> (mixed residual)", 4: "amr::k_tangent",
+           3: "amr::k_stage64<3, true> (mixed residual)", 4: "amr::k_tangent",
            5: "amr::k_gsrb2<T, 2, R> (finest up-leg smoother + FP64 outer step)"}
 # in-step timed kernels (amressif_kernel_time) -> kernel symbol, key in profiles/pmc_summary.txt
 STEP_KERNELS = {
     "stage": ("amr::k_stage64<0, true> (FP64 RK stage)", "stage_fp64"),
     "stage_last": ("amr::k_stage64<1, true> (FP64 last RK stage + norm partials)", None),
-    "resid": ("amr::k_stage64<3, false> (mixed FP64 residual)", "residual_mixed"),
+    "resid": ("amr::k_stage64<3, true> (mixed FP64 residual)", "residual_mixed"),
     "tangent": ("amr::k_tangent (mixed FP32 tangent-linear update)", "tangent_mixed"),
     "mg_down": ("amr::k_gsrb2<T, 1, 8> (finest down-leg smoother + restriction)", "smooth1_{prec}"),
     "mg_up": ("amr::k_gsrb2<T, 2, 4> (finest up-leg smoother + FP64 outer step)", "smooth5_{prec}"),
@@ -101,8 +101,8 @@ def read_traffic(kernel_key: str):
 
 def step_bytes_fp64(n, pseudo_iters, vcycles):
     """Algorithmic HBM bytes of one FP64 step at n^2 cells, kernel by kernel:
-    once per step the pressure gradient folded into the BDF2 constant (p, c in;
-    c out: 40 B); 4 fused RK4 stages of 64 B/cell per pseudo-iteration; per V-cycle the
+    4 fused RK4 stages of 64 B/cell per pseudo-iteration (the pressure gradient
+    sits in the BDF2 constant, put there by the previous projection); per V-cycle the
     finest down-leg smoother with the fused restriction (r in, e out, coarse
     r out: 18 B), the finest up-leg smoother with the fused FP64 outer step
     (e, r, coarse e, phi, b in; phi, next r out: 50 B) and the coarser levels
@@ -112,7 +112,7 @@ def step_bytes_fp64(n, pseudo_iters, vcycles):
     the next step's BDF2 constant (u*, v*, phi, p, u^n in; u, v, p, c and the
     copy of u out: 104 B)."""
     cells = float(n) * n
-    return cells * (40 + pseudo_iters * 4 * 64 + vcycles * (18 + 50 + 44 / 3) + 40 + 104)
+    return cells * (pseudo_iters * 4 * 64 + vcycles * (18 + 50 + 44 / 3) + 40 + 104)
 
 
 def run_precision(api, W, n, ren, precision, state, warmup, steps, torch):

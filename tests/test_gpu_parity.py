@@ -383,7 +383,7 @@ def test_in_step_kernel_timing_counts_and_leaves_results_unchanged(prec):
     assert t["mg_down"][1] == vc and t["mg_up"][1] == vc
     assert t["project"][1] == steps and t["poisson_start"][1] == steps
     cells = float(n * n)
-    per_cell = {"stage": 64, "stage_last": 64, "resid": 56, "tangent": 48}
+    per_cell = {"stage": 64, "stage_last": 64, "resid": 48, "tangent": 48}
     for name, (ms, cnt, nbytes) in t.items():
         assert (ms > 0) == (cnt > 0) and (nbytes > 0) == (cnt > 0), name
         assert ms < 50.0, name  # a 256^2 launch takes microseconds

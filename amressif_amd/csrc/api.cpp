@@ -15,6 +15,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <functional>
 #include <map>
 #include <sstream>
 #include <string>
@@ -297,9 +298,18 @@ WarmStart mg_prepare(amressif_ctx *ctx, bool warm, int *rc)
 // before the host saw the pseudo-iteration loop end; *skipped reports that it
 // had not (every kernel of the phase then did nothing) and the caller runs it
 // again once the loop has ended.
+// Work enqueued behind each batch of V-cycles before the host waits (the
+// step's projection, gated on the device by the solve's end), and the undo of
+// its host-side effects when the wait shows the solve has not ended.
+struct SpecTail {
+    std::function<int()> enqueue;
+    std::function<void()> undo;
+};
+
 int mg_run(amressif_ctx *ctx, const WarmStart &w, bool center, const double *us, const double *vs,
-           bool require_mom, bool *skipped)
+           bool require_mom, bool *skipped, const SpecTail *tail = nullptr, bool *tail_done = nullptr)
 {
+    if (tail_done) *tail_done = false;
     Ctx &c = ctx->c;
     const int mixed = c.in.precision == AMRESSIF_MIXED;
     int nb = 0;
@@ -323,14 +333,30 @@ int mg_run(amressif_ctx *ctx, const WarmStart &w, bool center, const double *us,
             HIPCHK(amr::launch_fin_resid(c, nb));
             ++launched;
         }
-        if (int rc = poll(ctx)) return rc;
+        if (tail) {
+            // the control block is copied right behind the batch; the host waits
+            // for that copy only, so it decides while the device runs the tail
+            HIPCHK(hipMemcpyAsync(c.ctl_host, c.ctl, sizeof(amr::Ctl), hipMemcpyDeviceToHost, c.stream));
+            HIPCHK(hipEventRecord(c.poll_ev, c.stream));
+            if (int rc = tail->enqueue()) return rc;
+            HIPCHK(hipEventSynchronize(c.poll_ev));
+            HIPCHK(amr::kprof_collect(c));
+        } else if (int rc = poll(ctx)) {
+            return rc;
+        }
         if (require_mom && c.ctl_host->mg_skip) {
             *skipped = true;
+            if (tail) tail->undo();
             amr::kprof_close(c, amr::KL_MG, 0);  // every kernel of the phase exited at once
             amr::kprof_close(c, amr::KL_PHASE, 0);
             return 0;
         }
-        if (c.ctl_host->mg_done || launched >= cap) break;
+        if (c.ctl_host->mg_done) {
+            if (tail_done) *tail_done = tail != nullptr;
+            break;
+        }
+        if (tail) tail->undo();
+        if (launched >= cap) break;
         target = launched + 2;
     }
     amr::kprof_close(c, amr::KL_MG, c.ctl_host->mg_iters);
@@ -865,6 +891,8 @@ int amressif_create(const amressif_inputs *in, void *hip_stream, amressif_ctx **
     {
         hipError_t e = hipHostMalloc((void **)&c.ctl_host, sizeof(amr::Ctl), hipHostMallocDefault);
         if (e != hipSuccess) return bail(AMRESSIF_ERR_HIP, hipGetErrorString(e));
+        e = hipEventCreateWithFlags(&c.poll_ev, hipEventDisableTiming);
+        if (e != hipSuccess) return bail(AMRESSIF_ERR_HIP, hipGetErrorString(e));
         std::memset(c.ctl_host, 0, sizeof(amr::Ctl));
         e = hipMemcpy(c.ctl, c.ctl_host, sizeof(amr::Ctl), hipMemcpyHostToDevice);
         if (e != hipSuccess) return bail(AMRESSIF_ERR_HIP, hipGetErrorString(e));
@@ -891,6 +919,7 @@ int amressif_destroy(amressif_ctx *ctx)
     if (c.stream) hipStreamSynchronize(c.stream);
     for (void *p : c.allocs) hipFree(p);
     if (c.ctl_host) hipHostFree(c.ctl_host);
+    if (c.poll_ev) hipEventDestroy(c.poll_ev);
     if (c.log) hipFree(c.log);
     if (c.kp) {
         for (const amr::KProf::Rec &r : c.kp->recs) { hipEventDestroy(r.e0); hipEventDestroy(r.e1); }
@@ -1009,26 +1038,57 @@ int advance_impl(amressif_ctx *ctx, int nsteps, amressif_step_stats *stats, amre
     const bool speculate = !tape && c.in.poisson_solver != 1;
     for (int s = 0; s < nsteps; ++s) {
         int nb = 0;
+        bool projected = false;  // the projection ran as the Poisson phase's gated tail
         if (speculate) {
             int launched = 0, rc = 0;
             if ((rc = amr_momentum_begin(ctx, bdf_ready, &launched, s > 0))) return rc;
             bdf_ready = false;
             const WarmStart w = mg_prepare(ctx, warm, &rc);
             if (rc) return rc;
+            // the projection is enqueued behind every batch of V-cycles, before the
+            // host waits: it runs on the device only if the solve has ended, and
+            // the host undoes its buffer swaps if it has not
+            const bool fold_next = s + 1 < nsteps;
+            const int saved_gradp = c.c_gradp;
+            SpecTail tail;
+            tail.enqueue = [&]() -> int {
+                HIPCHK(amr::kprof_begin(c, amr::KP_PROJECT, amr::KL_PROJ, 0,
+                                        (fold_next ? 104.0 : 56.0) * (double)c.g.n * c.g.n));
+                HIPCHK(amr::launch_project_ex(c, c.U[0], c.U[1], c.S1[0], c.S1[1], c.p, c.phiv[0], 1, &nb, nullptr,
+                                              nullptr, fold_next ? 1 : 0, c.p_alt, c.phiv[1], 1));
+                HIPCHK(amr::kprof_end(c));
+                HIPCHK(amr::launch_fin_project(c, nb, s, fold_next ? 1 : 0, 1));
+                std::swap(c.U[0], c.S1[0]);
+                std::swap(c.U[1], c.S1[1]);
+                std::swap(c.p, c.p_alt);
+                c.c_gradp = fold_next && c.project_rows > 0;
+                return 0;
+            };
+            tail.undo = [&]() {
+                std::swap(c.U[0], c.S1[0]);
+                std::swap(c.U[1], c.S1[1]);
+                std::swap(c.p, c.p_alt);
+                c.c_gradp = saved_gradp;
+                amr::kprof_close(c, amr::KL_PROJ, 0);
+            };
             bool skipped = false;
-            if ((rc = mg_run(ctx, w, false, c.U[0], c.U[1], true, &skipped))) return rc;
+            if ((rc = mg_run(ctx, w, false, c.U[0], c.U[1], true, &skipped, &tail, &projected))) return rc;
             if (skipped) {
                 if ((rc = amr_momentum_finish(ctx, launched))) return rc;
-                if ((rc = mg_run(ctx, w, false, c.U[0], c.U[1], false, nullptr))) return rc;
+                if ((rc = mg_run(ctx, w, false, c.U[0], c.U[1], false, nullptr, &tail, &projected))) return rc;
             } else {
                 c.last_P = std::max(1, c.ctl_host->mom_count);
                 amr::kprof_close(c, amr::KL_MOM, c.ctl_host->mom_count);
+            }
+            if (projected) {
+                amr::kprof_close(c, amr::KL_PROJ, 1);
+                bdf_ready = fold_next;
             }
         } else {
             if (int rc = amr_momentum_predictor(ctx, bdf_ready, s > 0)) return rc;
             bdf_ready = false;
         }
-        const int flags = c.ctl_host->flags;
+        const int flags = c.ctl_host->flags;  // (copied before the projection clears them)
         if (tape) {  // record the converged predictor u* of this step
             const size_t bytes = tape->fsz * sizeof(double);
             HIPCHK(hipMemcpyAsync(tape->slot(tape->count), c.U[0], bytes, hipMemcpyDeviceToDevice, c.stream));
@@ -1042,22 +1102,25 @@ int advance_impl(amressif_ctx *ctx, int nsteps, amressif_step_stats *stats, amre
         // between two steps of this call the projection also forms the next
         // step's BDF2 constant (main.cpp:879-884); the last step leaves
         // C and velContPrev as the reference does
-        const bool fold = s + 1 < nsteps && !(flags & AMRESSIF_FLAG_NONFINITE);
-        // per cell: u*, v*, phi, p in, u, v, p out (56 B); with the next step's
-        // BDF2 constant also u^n in, c and the copy of u^{n+1} out (104 B)
-        // p^{n+1} goes to the other pressure array (p^n stays readable for the
-        // neighbour terms), and the next constant takes -G p^{n+1}
-        HIPCHK(amr::kprof_begin(c, amr::KP_PROJECT, amr::KL_ONCE, 0, (fold ? 104.0 : 56.0) * (double)c.g.n * c.g.n));
-        HIPCHK(amr::launch_project_ex(c, c.U[0], c.U[1], c.S1[0], c.S1[1], c.p, c.phi, 1, &nb, nullptr, nullptr,
-                                      fold ? 1 : 0, c.p_alt));
-        HIPCHK(amr::kprof_end(c));
-        std::swap(c.p, c.p_alt);
-        c.c_gradp = fold && c.project_rows > 0;
-        // (between two steps of the call it also clears the next step's control)
-        HIPCHK(amr::launch_fin_project(c, nb, s, s + 1 < nsteps ? 1 : 0));
-        bdf_ready = fold;
-        std::swap(c.U[0], c.S1[0]);
-        std::swap(c.U[1], c.S1[1]);
+        if (!projected) {
+            const bool fold = s + 1 < nsteps && !(flags & AMRESSIF_FLAG_NONFINITE);
+            // per cell: u*, v*, phi, p in, u, v, p out (56 B); with the next step's
+            // BDF2 constant also u^n in, c and the copy of u^{n+1} out (104 B)
+            // p^{n+1} goes to the other pressure array (p^n stays readable for the
+            // neighbour terms), and the next constant takes -G p^{n+1}
+            HIPCHK(amr::kprof_begin(c, amr::KP_PROJECT, amr::KL_ONCE, 0,
+                                    (fold ? 104.0 : 56.0) * (double)c.g.n * c.g.n));
+            HIPCHK(amr::launch_project_ex(c, c.U[0], c.U[1], c.S1[0], c.S1[1], c.p, c.phi, 1, &nb, nullptr, nullptr,
+                                          fold ? 1 : 0, c.p_alt));
+            HIPCHK(amr::kprof_end(c));
+            std::swap(c.p, c.p_alt);
+            c.c_gradp = fold && c.project_rows > 0;
+            // (between two steps of the call it also clears the next step's control)
+            HIPCHK(amr::launch_fin_project(c, nb, s, s + 1 < nsteps ? 1 : 0));
+            bdf_ready = fold;
+            std::swap(c.U[0], c.S1[0]);
+            std::swap(c.U[1], c.S1[1]);
+        }
         if (flags & AMRESSIF_FLAG_NONFINITE) {
             status = fail(ctx, AMRESSIF_ERR_NONFINITE, "non-finite pseudo-iteration norm at step " + std::to_string(s));
             nsteps = s + 1;

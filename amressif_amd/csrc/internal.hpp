@@ -97,7 +97,7 @@ enum KProfId {
     KP_PSTART,       // k_poisson_start
     KP_COUNT
 };
-enum KProfLoop { KL_MOM = 0, KL_MG, KL_PHASE, KL_ONCE };
+enum KProfLoop { KL_MOM = 0, KL_MG, KL_PHASE, KL_ONCE, KL_PROJ };
 struct KProf {
     bool on = false;
     struct Rec {
@@ -178,6 +178,7 @@ struct Ctx {
     double *partials = nullptr;  // kMaxPartials * 4 doubles
     Ctl *ctl = nullptr;
     Ctl *ctl_host = nullptr;  // pinned
+    hipEvent_t poll_ev = nullptr;  // marks the control-block copy behind a V-cycle batch
     amressif_step_stats *log = nullptr;  // device step log
     int log_cap = 0;
     int last_P = 16, last_V = 8;
@@ -244,7 +245,7 @@ hipError_t launch_smooth_fine(const Ctx &c, int kind, double *scratch_phi);
 // velocity in S2 (that step then takes S2 as u^n and skips launch_bdf2)
 hipError_t launch_project(const Ctx &c, const double *us, const double *vs, double *uo, double *vo, int *nblocks,
                           const float *au = nullptr, const float *av = nullptr, int next_bdf = 0);
-hipError_t launch_fin_project(const Ctx &c, int nblocks, int step_index, int reset_next = 0);
+hipError_t launch_fin_project(const Ctx &c, int nblocks, int step_index, int reset_next = 0, int gated = 0);
 hipError_t launch_cont2cart_out(const Ctx &c, const double *u, const double *v, const amr_array4 &vc);
 hipError_t launch_pack_face(const Ctx &c, const amr_array4 &a, double *dst, int dir);
 hipError_t launch_unpack_face(const Ctx &c, const double *src, const amr_array4 &a, int dir);
@@ -258,10 +259,13 @@ hipError_t launch_ctl_set(const Ctx &c, double tol_mom, double tol_rel, double t
                           int mg_max_iter);
 // p_out: where p + phi - mean goes (null: in place).  With next_bdf and a
 // separate p_out, the row-march kernel also folds -G p^{n+1} into the next
-// constant (project_rows > 0).
+// constant (project_rows > 0).  gated: enqueued before the host saw the solve
+// end (does nothing unless it has); phi_alt: the iterate buffer taken after an
+// odd cycle count.
 hipError_t launch_project_ex(const Ctx &c, const double *us, const double *vs, double *uo, double *vo, double *p,
                              const double *phi, int use_mean, int *nblocks, const float *au = nullptr,
-                             const float *av = nullptr, int next_bdf = 0, double *p_out = nullptr);
+                             const float *av = nullptr, int next_bdf = 0, double *p_out = nullptr,
+                             const double *phi_alt = nullptr, int gated = 0);
 hipError_t launch_unpack_cell_shift(const Ctx &c, const double *src, const amr_array4 &a, int comp,
                                     int fill_ghosts, const double *shift);
 hipError_t launch_copy_bytes(const Ctx &c, const void *src, void *dst, int64_t bytes);

@@ -291,15 +291,30 @@ struct NextBdf {
     double *cu, *cv, *wu, *wv;
     int gradp;  // k_project_rows: also fold -G p^{n+1} into c (interior faces)
 };
+// A projection enqueued behind a batch of V-cycles before the host has seen
+// the solve end: it does nothing unless the solve has ended (mg_done, not a
+// skipped phase), and takes phi from the iterate buffer the cycle count left
+// it in (phi after an even count, phi_alt after an odd one).
+struct ProjGate {
+    const double *phi_alt;
+    int gated;
+};
+__device__ __forceinline__ bool proj_gate(const ProjGate &g, const Ctl *ctl, const double *&phi)
+{
+    if (g.gated && !(ctl->mg_done && !ctl->mg_skip)) return false;
+    if (g.phi_alt && (ctl->mg_iters & 1)) phi = g.phi_alt;
+    return true;
+}
 
 __global__ __launch_bounds__(256) void k_project(const double *us, const double *vs, const float *au,
                                                  const float *av, double *uo, double *vo, const double *p,
                                                  double *p_out, const double *phi, int n, int P, double dx,
                                                  double dy, double dt,
                                                  const Ctl *ctl, int use_mean, double *partials, int ntx, int nty,
-                                                 NextBdf nx)
+                                                 NextBdf nx, ProjGate gate)
 {
     __shared__ double sh[16];
+    if (!proj_gate(gate, ctl, phi)) return;
     const CellTile t = cell_tile(ntx, nty);
     const double pm = use_mean ? ctl->phi_mean : 0.0;
     double m = 0.0;
@@ -361,9 +376,10 @@ __global__ __launch_bounds__(256) void k_project_rows(const double *us, const do
                                                       double *p_out, const double *phi, int n, int P, double dx,
                                                       double dy,
                                                       double dt, const Ctl *ctl, int use_mean, double *partials,
-                                                      int nbands, int nwaves, int rows, NextBdf nx)
+                                                      int nbands, int nwaves, int rows, NextBdf nx, ProjGate gate)
 {
     __shared__ double sh[4];
+    if (!proj_gate(gate, ctl, phi)) return;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int B = gridDim.x;  // a multiple of 8
     const int blk = (blockIdx.x & 7) * (B >> 3) + (blockIdx.x >> 3);
@@ -488,9 +504,10 @@ __global__ __launch_bounds__(256) void k_project_rows(const double *us, const do
 
 hipError_t launch_project_ex(const Ctx &c, const double *us, const double *vs, double *uo, double *vo, double *p,
                              const double *phi, int use_mean, int *nblocks, const float *au, const float *av,
-                             int next_bdf, double *p_out)
+                             int next_bdf, double *p_out, const double *phi_alt, int gated)
 {
     if (!p_out) p_out = p;
+    const ProjGate gate{phi_alt, gated};
     const int gradp = next_bdf && p_out != p && c.project_rows > 0;
     const NextBdf nx =
         next_bdf ? NextBdf{c.Uprev[0], c.Uprev[1], c.C[0], c.C[1], c.S2[0], c.S2[1], gradp} : NextBdf{};
@@ -504,7 +521,7 @@ hipError_t launch_project_ex(const Ctx &c, const double *us, const double *vs, d
         *nblocks = nb;
         hipLaunchKernelGGL(k_project_rows, dim3(nb), dim3(256), 0, c.stream, us, vs, au, av, uo, vo, (const double *)p,
                            p_out, phi, n, c.g.P, c.g.dx, c.g.dy, c.g.dt, (const Ctl *)c.ctl, use_mean, c.partials,
-                           nbands, nwaves, rows, nx);
+                           nbands, nwaves, rows, nx, gate);
         return hipGetLastError();
     }
     const int ntx = (n + 63) / 64, nty = (n + 15) / 16;
@@ -512,7 +529,7 @@ hipError_t launch_project_ex(const Ctx &c, const double *us, const double *vs, d
     if (nb > kMaxPartials) return hipErrorInvalidValue;
     *nblocks = nb;
     hipLaunchKernelGGL(k_project, dim3(nb), dim3(256), 0, c.stream, us, vs, au, av, uo, vo, (const double *)p, p_out, phi, n, c.g.P, c.g.dx,
-                       c.g.dy, c.g.dt, c.ctl, use_mean, c.partials, ntx, nty, nx);
+                       c.g.dy, c.g.dt, c.ctl, use_mean, c.partials, ntx, nty, nx, gate);
     return hipGetLastError();
 }
 
@@ -525,9 +542,11 @@ hipError_t launch_project(const Ctx &c, const double *us, const double *vs, doub
 // reset_next: also clear the pseudo-iteration control for the next step (as
 // k_ctl_reset_step does), after this step's statistics are logged
 __global__ __launch_bounds__(1024) void k_fin_project(const double *partials, int nb, Ctl *ctl,
-                                                      amressif_step_stats *log, int step_index, int reset_next)
+                                                      amressif_step_stats *log, int step_index, int reset_next,
+                                                      int gated)
 {
     __shared__ double sh[16];
+    if (gated && !(ctl->mg_done && !ctl->mg_skip)) return;  // its projection did nothing (ProjGate)
     double m = 0.0;
     for (int b = threadIdx.x; b < nb; b += blockDim.x) {
         const double pm = partials[b];
@@ -558,10 +577,10 @@ __global__ __launch_bounds__(1024) void k_fin_project(const double *partials, in
     }
 }
 
-hipError_t launch_fin_project(const Ctx &c, int nblocks, int step_index, int reset_next)
+hipError_t launch_fin_project(const Ctx &c, int nblocks, int step_index, int reset_next, int gated)
 {
     hipLaunchKernelGGL(k_fin_project, dim3(1), dim3(1024), 0, c.stream, c.partials, nblocks, c.ctl, c.log,
-                       step_index, reset_next);
+                       step_index, reset_next, gated);
     return hipGetLastError();
 }
 

@@ -241,10 +241,6 @@ hipError_t launch_correct_residual(const Ctx &c, const double *phi_in, double *p
 size_t tail_lds_bytes(const Ctx &c, int mixed);
 hipError_t configure_tail(Ctx &c);
 hipError_t launch_smooth_fine(const Ctx &c, int kind, double *scratch_phi);
-// next_bdf: also form the next step's BDF2 constant in C and a copy of the new
-// velocity in S2 (that step then takes S2 as u^n and skips launch_bdf2)
-hipError_t launch_project(const Ctx &c, const double *us, const double *vs, double *uo, double *vo, int *nblocks,
-                          const float *au = nullptr, const float *av = nullptr, int next_bdf = 0);
 hipError_t launch_fin_project(const Ctx &c, int nblocks, int step_index, int reset_next = 0, int gated = 0);
 hipError_t launch_cont2cart_out(const Ctx &c, const double *u, const double *v, const amr_array4 &vc);
 hipError_t launch_pack_face(const Ctx &c, const amr_array4 &a, double *dst, int dir);
@@ -257,6 +253,8 @@ hipError_t launch_norm(const Ctx &c, const double *ub, const double *vb, const d
                        double *out_dev);
 hipError_t launch_ctl_set(const Ctx &c, double tol_mom, double tol_rel, double tol_abs, int iter_num,
                           int mg_max_iter);
+// next_bdf: also form the next step's BDF2 constant in C and a copy of the new
+// velocity in S2 (that step then takes S2 as u^n and skips launch_bdf2).
 // p_out: where p + phi - mean goes (null: in place).  With next_bdf and a
 // separate p_out, the row-march kernel also folds -G p^{n+1} into the next
 // constant (project_rows > 0).  gated: enqueued before the host saw the solve

@@ -533,12 +533,6 @@ hipError_t launch_project_ex(const Ctx &c, const double *us, const double *vs, d
     return hipGetLastError();
 }
 
-hipError_t launch_project(const Ctx &c, const double *us, const double *vs, double *uo, double *vo, int *nblocks,
-                          const float *au, const float *av, int next_bdf)
-{
-    return launch_project_ex(c, us, vs, uo, vo, c.p, c.phi, 1, nblocks, au, av, next_bdf);
-}
-
 // reset_next: also clear the pseudo-iteration control for the next step (as
 // k_ctl_reset_step does), after this step's statistics are logged
 __global__ __launch_bounds__(1024) void k_fin_project(const double *partials, int nb, Ctl *ctl,

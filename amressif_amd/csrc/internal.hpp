@@ -87,9 +87,9 @@ struct AdjWork {
 // V-cycle, and a speculative Poisson phase that was skipped, exit at once and
 // are dropped.
 enum KProfId {
-    KP_STAGE = 0,    // k_stage64<0>: FP64 RK stage
-    KP_STAGE_LAST,   // k_stage64<1>: FP64 last RK stage + norm partials
-    KP_RESID,        // k_stage64<3>: mixed FP64 residual
+    KP_STAGE = 0,    // k_stage64<0, *>: FP64 RK stage
+    KP_STAGE_LAST,   // k_stage64<1, *>: FP64 last RK stage + norm partials
+    KP_RESID,        // k_stage64<3, *>: mixed FP64 residual
     KP_TANGENT,      // k_tangent
     KP_MG_DOWN,      // finest down-leg smoother + restriction
     KP_MG_UP,        // finest up-leg smoother + FP64 outer step

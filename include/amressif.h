@@ -298,9 +298,9 @@ int amressif_time_kernel(amressif_ctx *ctx, int which, int reps, int64_t copy_by
  * number of launches that did work and their total time (launches past the
  * converged pseudo-iteration or V-cycle, which exit at once, are not counted).
  * Enabling clears the totals; enable = 0 stops recording and keeps them.
- *   which = 0  k_stage64<0>: FP64 RK stage
- *   which = 1  k_stage64<1>: FP64 last RK stage (+ norm partials)
- *   which = 2  k_stage64<3>: mixed FP64 residual
+ *   which = 0  FP64 RK stage
+ *   which = 1  FP64 last RK stage (+ norm partials)
+ *   which = 2  mixed FP64 residual
  *   which = 3  k_tangent: mixed FP32 tangent-linear update
  *   which = 4  finest-level down-leg smoother + restriction (outer-step cycles)
  *   which = 5  finest-level up-leg smoother + FP64 outer step

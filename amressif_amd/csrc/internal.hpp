@@ -147,6 +147,7 @@ struct Ctx {
     // kernels by [FP64, FP32][up leg, down leg, finest up leg], and on the
     // levels below the finest
     int mg_smoother = 2;
+    int mg_late = 1;  // finest up-leg smoother: phi and b loaded after the sweeps
     int mg_rows[2][3] = {{8, 8, 8}, {8, 8, 4}};
     int mg_rows_coarse = 4;
     // FGMRES Poisson solver (poisson_solver = 1)

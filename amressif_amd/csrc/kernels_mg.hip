@@ -449,7 +449,10 @@ __device__ __forceinline__ void gsrb4_pair(T (&x0)[NR], T (&x1)[NR], const T (&r
     (void)in_i;
 }
 
-template <class T, int MODE, int R>
+// LATE (MODE 2): phi_in and b, needed only by the outer step, are loaded
+// after the sweeps, so their registers are not live during them (more waves
+// per SIMD, at the cost of one more wait for memory)
+template <class T, int MODE, int R, bool LATE = false>
 __global__ __launch_bounds__(256) void k_gsrb2(const T *xin, const T *ec, int pc, T *xout, const T *rhs, int nx,
                                                int ny, int pitch, T ax, T ay, T *rc, int ncx, int ncy, int pcc,
                                                OuterArgs o, const Ctl *ctl, int nbands, int nwaves)
@@ -496,7 +499,7 @@ __global__ __launch_bounds__(256) void k_gsrb2(const T *xin, const T *ec, int pc
     }
     constexpr int NP = MODE == 2 ? R + 2 : 1, NB = MODE == 2 ? R : 1;
     double ph0[NP], ph1[NP], bb0[NB], bb1[NB];
-    if (MODE == 2) {
+    auto load_outer = [&]() {
         int k = ca + pitch * (j0 - 1);
 #pragma unroll
         for (int t = 0; t < NP; ++t) {
@@ -513,7 +516,8 @@ __global__ __launch_bounds__(256) void k_gsrb2(const T *xin, const T *ec, int pc
             }
             k += pitch;
         }
-    }
+    };
+    if (MODE == 2 && !LATE) load_outer();
     if (ctl->mg_done | ctl->gm_stop) return;
     // per cell: d = cw + ce + cs + cn; the even cell has a west wall at column 0,
     // the odd one an east wall at column nx - 1
@@ -575,6 +579,10 @@ __global__ __launch_bounds__(256) void k_gsrb2(const T *xin, const T *ec, int pc
         }
     }
     if (MODE == 2) {  // phi_out = phi_in + e, r = b - L phi_out (FP64), one partial per workgroup
+        if (LATE) {
+            __builtin_amdgcn_sched_barrier(0);  // keep the loads below the sweeps
+            load_outer();
+        }
         double P0[NP], P1[NP];
 #pragma unroll
         for (int t = 0; t < NP; ++t) {
@@ -1575,7 +1583,11 @@ static void launch_gsrb(const Ctx &c, const SmoothShapeRt &s, const T *xin, cons
                         const T *rhs, const Level &L, T *rc, const Level *Cl, const OuterArgs &o)
 {
     constexpr int R2 = R < pair_rows_max<T, MODE>() ? R : pair_rows_max<T, MODE>();
-    if (s.wave == 2)
+    if (s.wave == 2 && MODE == 2 && c.mg_late)
+        hipLaunchKernelGGL((k_gsrb2<T, MODE, R2, true>), s.grid, dim3(256), 0, c.stream, xin, ec, pc, xout, rhs, L.nx,
+                           L.ny, L.pitch, (T)L.ax, (T)L.ay, rc, Cl ? Cl->nx : 0, Cl ? Cl->ny : 0,
+                           Cl ? Cl->pitch : 0, o, (const Ctl *)c.ctl, s.nbands, s.nwaves);
+    else if (s.wave == 2)
         hipLaunchKernelGGL((k_gsrb2<T, MODE, R2>), s.grid, dim3(256), 0, c.stream, xin, ec, pc, xout, rhs, L.nx, L.ny,
                            L.pitch, (T)L.ax, (T)L.ay, rc, Cl ? Cl->nx : 0, Cl ? Cl->ny : 0, Cl ? Cl->pitch : 0, o,
                            (const Ctl *)c.ctl, s.nbands, s.nwaves);

@@ -324,6 +324,7 @@ int mg_run(amressif_ctx *ctx, const WarmStart &w, bool center, const double *us,
     HIPCHK(amr::launch_poisson_start(c, w.mode, center ? 1 : 0, w.phi1, w.phi2, c.phiv[0], mixed, us, vs));
     HIPCHK(amr::kprof_end(c));
     int launched = 0, target = std::max(1, c.last_V);
+    if (c.mg_batch > 0) target = std::min(target, c.mg_batch);
     const int cap = std::max(1, c.in.mg_max_iter);
     for (;;) {
         while (launched < target && launched < cap) {
@@ -737,6 +738,7 @@ int amressif_create(const amressif_inputs *in, void *hip_stream, amressif_ctx **
     if (const char *e = std::getenv("AMRESSIF_MG_SMOOTHER")) c.mg_smoother = std::max(0, std::min(2, std::atoi(e)));
     if (const char *e = std::getenv("AMRESSIF_TAIL_DCT")) c.tail_dct = std::atoi(e) ? 1 : 0;
     if (const char *e = std::getenv("AMRESSIF_MG_LATE")) c.mg_late = std::atoi(e) ? 1 : 0;
+    if (const char *e = std::getenv("AMRESSIF_MG_BATCH")) c.mg_batch = std::max(0, std::atoi(e));
     if (const char *e = std::getenv("AMRESSIF_DCT_MFMA64")) c.dct_mfma64 = std::atoi(e) ? 1 : 0;
     if (const char *e = std::getenv("AMRESSIF_START_ROWS")) c.start_rows = std::max(0, std::min(256, std::atoi(e)));
     if (const char *e = std::getenv("AMRESSIF_PROJECT_ROWS"))

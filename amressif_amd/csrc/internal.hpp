@@ -183,6 +183,7 @@ struct Ctx {
     amressif_step_stats *log = nullptr;  // device step log
     int log_cap = 0;
     int last_P = 16, last_V = 8;
+    int mg_batch = 0;  // > 0: cap on the first batch of V-cycles of a solve (tests of the later batches)
     int64_t bytes = 0;
     // scratch of the module-level entry points (allocated on first use)
     double *M[4][2]{};

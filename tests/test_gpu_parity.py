@@ -424,3 +424,33 @@ def test_row_march_kernels_in_the_step(prec, var, n):
     assert rel(a["u"], b["u"]) < 1e-12 and rel(a["v"], b["v"]) < 1e-12
     pa, pb = a["ctx"][0, 1:-1, 1:-1], b["ctx"][0, 1:-1, 1:-1]
     assert rel(pa - pa.mean(), pb - pb.mean()) < 1e-10
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_short_first_vcycle_batch_gives_the_same_steps(prec):
+    """A first batch of one V-cycle per solve (AMRESSIF_MG_BATCH=1): every step's
+    projection, enqueued behind the batch, finds the solve unfinished and does
+    nothing; the host undoes its buffer swaps and runs more cycles. The steps
+    are identical to those of the default batches."""
+    import os
+    api = _api()
+    n, steps = 128, 6
+    state = W.initial_state(n, "seeded")
+    out = {}
+    for batch in ("0", "1"):
+        old = os.environ.get("AMRESSIF_MG_BATCH")
+        os.environ["AMRESSIF_MG_BATCH"] = batch
+        try:
+            run = api.Run.from_numpy(api.cavity_inputs(n, 1000.0, precision=prec), state)
+        finally:
+            if old is None:
+                os.environ.pop("AMRESSIF_MG_BATCH")
+            else:
+                os.environ["AMRESSIF_MG_BATCH"] = old
+        stats = run.advance(steps)
+        assert all(s["flags"] == 0 for s in stats) and all(s["vcycles"] > 1 for s in stats)
+        out[batch] = (run.fetch(), [(s["pseudo_iters"], s["vcycles"]) for s in stats])
+    (a, sa), (b, sb) = out["1"], out["0"]
+    assert sa == sb
+    for k in ("u", "v", "ctx"):
+        assert np.array_equal(a[k], b[k]), k

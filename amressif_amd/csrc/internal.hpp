@@ -148,6 +148,7 @@ struct Ctx {
     // levels below the finest
     int mg_smoother = 2;
     int mg_late = 1;  // finest up-leg smoother: phi and b loaded after the sweeps
+    int mg_up64_single = 1;  // FP64 finest up leg on the one-column kernel (with mg_smoother 2)
     int mg_rows[2][3] = {{8, 8, 8}, {8, 8, 4}};
     int mg_rows_coarse = 4;
     // FGMRES Poisson solver (poisson_solver = 1)

@@ -1557,9 +1557,12 @@ static SmoothShapeRt smooth_shape(const Ctx &c, int l, int mode)
         s.nwaves = s.parts = (int)(s.grid.x * s.grid.y);
         return s;
     }
-    const int OC = c.mg_smoother == 2 ? (mode == 0 ? 120 : 116) : (mode == 0 ? 56 : 54);
+    // FP64 finest up leg: the one-column kernel (106 VGPRs, 4 waves per SIMD)
+    // beats the paired one (2 waves) there
+    if (s.wave == 2 && sizeof(T) == 8 && mode == 2 && l == 0 && c.mg_up64_single) s.wave = 1;
+    const int OC = s.wave == 2 ? (mode == 0 ? 120 : 116) : (mode == 0 ? 56 : 54);
     s.rows = l == 0 ? c.mg_rows[sizeof(T) == 4][mode] : c.mg_rows_coarse;
-    if (c.mg_smoother == 2) {  // the paired-column kernel's register limit (pair_rows_max)
+    if (s.wave == 2) {  // the paired-column kernel's register limit (pair_rows_max)
         const int cap = sizeof(T) == 4 ? (mode == 2 ? 8 : 16) : (mode == 2 ? 4 : 8);
         s.rows = std::min(s.rows, cap);
     }

@@ -20,7 +20,7 @@ pytestmark = pytest.mark.gpu
 
 ENV_KEYS = ("AMRESSIF_MG_SMOOTHER", "AMRESSIF_MG_ROWS32", "AMRESSIF_MG_ROWS64", "AMRESSIF_MG_ROWS_COARSE",
             "AMRESSIF_TAIL_DCT", "AMRESSIF_TAIL_EXT", "AMRESSIF_DCT_MFMA64",
-            "AMRESSIF_START_ROWS")
+            "AMRESSIF_START_ROWS", "AMRESSIF_MG_UP64")
 
 
 def _solve(n, precision, b, cycles, tol=0.0, **env):
@@ -141,4 +141,15 @@ def test_row_march_poisson_start_matches_cell_tile_kernel(n, precision, rows):
     ref, res_ref = _solve(n, precision, b, 3, AMRESSIF_START_ROWS="0")
     got, res = _solve(n, precision, b, 3, AMRESSIF_START_ROWS=rows)
     assert abs(res - res_ref) <= 1e-12 * max(res_ref, 1e-300), (res, res_ref)
+    assert rel(got, ref) < 1e-12
+
+
+@pytest.mark.parametrize("n", [100, 256, 1024])
+def test_fp64_up_leg_kernels_agree(n):
+    """FP64 finest up leg on the one-column kernel (the default) or the paired
+    one: the same cycles up to FP64 rounding."""
+    b = _rhs(n, 61 + n)
+    ref, res_ref = _solve(n, 0, b, 3, AMRESSIF_MG_UP64="0")
+    got, res = _solve(n, 0, b, 3, AMRESSIF_MG_UP64="1")
+    assert abs(res - res_ref) <= 1e-9 * max(res_ref, 1e-300) + 1e-14, (res, res_ref)
     assert rel(got, ref) < 1e-12

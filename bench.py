@@ -58,7 +58,7 @@ STEP_KERNELS = {
     "tangent": ("amr::k_tangent (mixed FP32 tangent-linear update)", "tangent_mixed"),
     "mg_down": ("finest down-leg smoother + restriction (amr::k_gsrb2<T, 1, 8, false>)", "smooth1_{prec}"),
     "mg_up": ("finest up-leg smoother + FP64 outer step (amr::k_gsrb2<float, 2, 4, true> mixed, "
-              "amr::k_gsrb<double, 2, 4> FP64)", "smooth5_{prec}"),
+              "amr::k_gsrb<double, 2, 8> FP64)", "smooth5_{prec}"),
     "project": ("amr::k_project (update + next step's BDF2 constant)", None),
     "poisson_start": ("amr::k_poisson_start (Poisson RHS, warm start, first residual)", None),
 }

@@ -745,6 +745,7 @@ int amressif_create(const amressif_inputs *in, void *hip_stream, amressif_ctx **
     if (const char *e = std::getenv("AMRESSIF_PROJECT_ROWS"))
         c.project_rows = std::max(0, std::min(256, std::atoi(e)));
     if (const char *e = std::getenv("AMRESSIF_TAIL_EXT")) c.tail_ext = std::atoi(e) ? 1 : 0;
+    if (const char *e = std::getenv("AMRESSIF_TAIL_WAVES")) c.tail_waves = std::atoi(e) == 4 ? 4 : 16;
     auto rows_ok = [](int r) { return r == 4 || r == 8 || r == 16 || r == 32; };
     for (int t = 0; t < 2; ++t)
         if (const char *e = std::getenv(t ? "AMRESSIF_MG_ROWS32" : "AMRESSIF_MG_ROWS64")) {

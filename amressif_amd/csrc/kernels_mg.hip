@@ -881,11 +881,36 @@ __device__ __forceinline__ void dct_gemm_mfma64(const double *A, const double *B
         }
 }
 
-// MF: FP64 products on the matrix cores (FP32 ones always are)
-template <class T, bool TA, bool TB, bool MF>
+// FP32 with 16 waves: wave w computes the one 16 x 16 block (w / 4, w % 4) of O
+template <bool TA, bool TB>
+__device__ __forceinline__ void dct_gemm_mfma16(const float *A, const float *B, float *O, const float *scale)
+{
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6, li = l & 15, lk = l >> 4;
+    const int rb = w >> 2, cb = w & 3;
+    dct_v4f acc = dct_v4f{0.f, 0.f, 0.f, 0.f};
+    const int ar = 16 * rb + li, bc = 16 * cb + li;
+#pragma unroll 4
+    for (int k0 = 0; k0 < kDct; k0 += 4) {
+        const int kk = k0 + lk;
+        const float a = TA ? A[kk * kDctLd + ar] : A[ar * kDctLd + kk];
+        const float b = TB ? B[bc * kDctLd + kk] : B[kk * kDctLd + bc];
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int row = 16 * rb + 4 * lk + v, col = 16 * cb + li;
+        O[row * kDctLd + col] = scale ? acc[v] * scale[row * kDct + col] : acc[v];
+    }
+}
+
+// MF: FP64 products on the matrix cores (FP32 ones always are); NW: waves in
+// the workgroup (4, or 16 for the FP32 products)
+template <class T, bool TA, bool TB, bool MF, int NW = 4>
 __device__ __forceinline__ void dct_product(const T *A, const T *B, T *O, const T *scale)
 {
-    if constexpr (sizeof(T) == 4)
+    if constexpr (sizeof(T) == 4 && NW == 16)
+        dct_gemm_mfma16<TA, TB>((const float *)A, (const float *)B, (float *)O, (const float *)scale);
+    else if constexpr (sizeof(T) == 4)
         dct_gemm_mfma<TA, TB>((const float *)A, (const float *)B, (float *)O, (const float *)scale);
     else if constexpr (MF)
         dct_gemm_mfma64<TA, TB>((const double *)A, (const double *)B, (double *)O, (const double *)scale);
@@ -895,16 +920,16 @@ __device__ __forceinline__ void dct_product(const T *A, const T *B, T *O, const 
 
 // the DCT solve of the workgroup: R in B0 (zero outside ny x nx), the tables
 // in Cx, Cy (LDS) and dinv (global); E in B0 on return (B1 overwritten)
-template <class T, bool MF = true>
+template <class T, bool MF = true, int NW = 4>
 __device__ __forceinline__ void dct_solve(const T *Cx, const T *Cy, T *B0, T *B1, const T *dinv)
 {
-    dct_product<T, false, false, MF>(B0, Cx, B1, (const T *)nullptr);  // T1 = R Cx
+    dct_product<T, false, false, MF, NW>(B0, Cx, B1, (const T *)nullptr);  // T1 = R Cx
     __syncthreads();
-    dct_product<T, true, false, MF>(Cy, B1, B0, dinv);  // S = (Cy^T T1) D^+
+    dct_product<T, true, false, MF, NW>(Cy, B1, B0, dinv);  // S = (Cy^T T1) D^+
     __syncthreads();
-    dct_product<T, false, false, MF>(Cy, B0, B1, (const T *)nullptr);  // T2 = Cy S
+    dct_product<T, false, false, MF, NW>(Cy, B0, B1, (const T *)nullptr);  // T2 = Cy S
     __syncthreads();
-    dct_product<T, false, true, MF>(B1, Cx, B0, (const T *)nullptr);  // E = T2 Cx^T
+    dct_product<T, false, true, MF, NW>(B1, Cx, B0, (const T *)nullptr);  // E = T2 Cx^T
     __syncthreads();
 }
 
@@ -948,10 +973,10 @@ __global__ __launch_bounds__(256) void k_tail_dct(const T *rtop, int ptop, T *et
 // the residual and its 2x2 average as the DCT input; then the DCT solve; up
 // leg: the coarse correction added, two sweeps, the level's correction stored.
 // Replaces that level's two smoother launches.
-template <int R1>
+template <int R1, int NW>
 __device__ __forceinline__ void ext_sweeps(float (&x0)[R1 + 2], float (&x1)[R1 + 2], const float (&rs0)[R1],
                                            const float (&rs1)[R1], const float (&A)[2][2], const float (&B)[2][2],
-                                           int j0, int ny1, float (*xch)[4][2][128], int &par, int w, int lane,
+                                           int j0, int ny1, float (*xch)[NW][2][128], int &par, int w, int lane,
                                            bool last_exchange)
 {
 #pragma unroll
@@ -983,7 +1008,7 @@ __device__ __forceinline__ void ext_sweeps(float (&x0)[R1 + 2], float (&x1)[R1 +
                 x0[0] = xch[par][w - 1][1][2 * lane];
                 x1[0] = xch[par][w - 1][1][2 * lane + 1];
             }
-            if (w < 3) {
+            if (w < NW - 1) {
                 x0[R1 + 1] = xch[par][w + 1][0][2 * lane];
                 x1[R1 + 1] = xch[par][w + 1][0][2 * lane + 1];
             }
@@ -992,22 +1017,26 @@ __device__ __forceinline__ void ext_sweeps(float (&x0)[R1 + 2], float (&x1)[R1 +
     }
 }
 
-template <int R1>
-__global__ __launch_bounds__(256) void k_tail_dct_ext(const float *r1g, int p1, float *e1g, int nx1, float ax1,
-                                                      float ay1, int nx, int ny, const float *tab, const Ctl *ctl)
+// NW: waves in the workgroup (4 or 16; ny1 = NW R1 rows).  With 16 waves
+// each SIMD runs four, which hides the latency of the row exchanges and of the
+// products better than one.
+template <int R1, int NW>
+__global__ __launch_bounds__(64 * NW) void k_tail_dct_ext(const float *r1g, int p1, float *e1g, int nx1, float ax1,
+                                                          float ay1, int nx, int ny, const float *tab,
+                                                          const Ctl *ctl)
 {
     __shared__ float Cx[kDct * kDctLd], Cy[kDct * kDctLd], B0[kDct * kDctLd], B1[kDct * kDctLd];
-    __shared__ float xch[2][4][2][128];  // [buffer][wave][first, last own row][column]
-    constexpr int ny1 = 4 * R1;
+    __shared__ float xch[2][NW][2][128];  // [buffer][wave][first, last own row][column]
+    constexpr int ny1 = NW * R1, NT = 64 * NW, NQ = kDct * kDct / NT;
     const int t = threadIdx.x, w = t >> 6, lane = t & 63;
     const int j0 = w * R1, ca = 2 * lane;
     const bool in_i = ca < nx1;
     // loads first: the tables and the own rows of r; their latency overlaps the flag's
-    float vx[16], vy[16], r0[R1], r1[R1];
+    float vx[NQ], vy[NQ], r0[R1], r1[R1];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        vx[q] = tab[t + 256 * q];
-        vy[q] = tab[kDct * kDct + t + 256 * q];
+    for (int q = 0; q < NQ; ++q) {
+        vx[q] = tab[t + NT * q];
+        vy[q] = tab[kDct * kDct + t + NT * q];
     }
 #pragma unroll
     for (int q = 0; q < R1; ++q) {
@@ -1018,8 +1047,8 @@ __global__ __launch_bounds__(256) void k_tail_dct_ext(const float *r1g, int p1, 
     }
     if (ctl->mg_done | ctl->gm_stop) return;
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int e = t + 256 * q, row = e >> 6, col = e & 63;
+    for (int q = 0; q < NQ; ++q) {
+        const int e = t + NT * q, row = e >> 6, col = e & 63;
         Cx[row * kDctLd + col] = vx[q];
         Cy[row * kDctLd + col] = vy[q];
         B0[row * kDctLd + col] = 0.f;  // the DCT input, zero outside ny x nx
@@ -1049,7 +1078,7 @@ __global__ __launch_bounds__(256) void k_tail_dct_ext(const float *r1g, int p1, 
     for (int q = 0; q < R1 + 2; ++q) x0[q] = x1[q] = 0.f;
     int par = 0;
     __syncthreads();  // B0 cleared before the restriction writes it
-    ext_sweeps<R1>(x0, x1, r0, r1, A, B, j0, ny1, xch, par, w, lane, true);
+    ext_sweeps<R1, NW>(x0, x1, r0, r1, A, B, j0, ny1, xch, par, w, lane, true);
     // residual of the own rows, 2x2 averages (lane-local) into the DCT input
 #pragma unroll
     for (int m = 0; m < R1 / 2; ++m) {
@@ -1071,7 +1100,7 @@ __global__ __launch_bounds__(256) void k_tail_dct_ext(const float *r1g, int p1, 
         if (in_i && I < nx && J < ny) B0[J * kDctLd + I] = 0.25f * (((t4[0] + t4[1]) + t4[2]) + t4[3]);
     }
     __syncthreads();
-    dct_solve<float>(Cx, Cy, B0, B1, tab + 2 * kDct * kDct);  // the coarse correction in B0
+    dct_solve<float, true, NW>(Cx, Cy, B0, B1, tab + 2 * kDct * kDct);  // the coarse correction in B0
     // up leg: add the coarse correction (the halo rows too: they hold the
     // neighbours' down-leg values), two sweeps, store the own rows
 #pragma unroll
@@ -1083,7 +1112,7 @@ __global__ __launch_bounds__(256) void k_tail_dct_ext(const float *r1g, int p1, 
             x1[q] += e;
         }
     }
-    ext_sweeps<R1>(x0, x1, r0, r1, A, B, j0, ny1, xch, par, w, lane, false);
+    ext_sweeps<R1, NW>(x0, x1, r0, r1, A, B, j0, ny1, xch, par, w, lane, false);
 #pragma unroll
     for (int q = 1; q <= R1; ++q)
         if (in_i) *reinterpret_cast<float2 *>(e1g + ca + (size_t)p1 * (j0 + q - 1)) = float2{x0[q], x1[q]};
@@ -1653,12 +1682,18 @@ static hipError_t vcycle_impl(const Ctx &c, const T *pinv, const T *r0, const Ou
         const Level &L1 = c.lev[lt], &Lt = c.lev[lt + 1];
         const float *r1 = (const float *)(const void *)lev_r<T>(L1);
         float *e1 = (float *)(void *)lev_e<T>(L1, 1);
-#define AMR_EXT(R1)                                                                                                  \
-    hipLaunchKernelGGL(k_tail_dct_ext<R1>, dim3(1), dim3(256), 0, c.stream, r1, L1.pitch, e1, L1.nx, (float)L1.ax,    \
-                       (float)L1.ay, Lt.nx, Lt.ny, (const float *)c.dct_tab32, ctl)
-        if (L1.ny == 32) AMR_EXT(8);
-        else if (L1.ny == 64) AMR_EXT(16);
-        else AMR_EXT(32);
+#define AMR_EXT(R1, NW)                                                                                              \
+    hipLaunchKernelGGL((k_tail_dct_ext<R1, NW>), dim3(1), dim3(64 * NW), 0, c.stream, r1, L1.pitch, e1, L1.nx,        \
+                       (float)L1.ax, (float)L1.ay, Lt.nx, Lt.ny, (const float *)c.dct_tab32, ctl)
+        if (c.tail_waves == 16) {
+            if (L1.ny == 32) AMR_EXT(2, 16);
+            else if (L1.ny == 64) AMR_EXT(4, 16);
+            else AMR_EXT(8, 16);
+        } else {
+            if (L1.ny == 32) AMR_EXT(8, 4);
+            else if (L1.ny == 64) AMR_EXT(16, 4);
+            else AMR_EXT(32, 4);
+        }
 #undef AMR_EXT
     } else {
         const Level &Lt = c.lev[lt];

@@ -20,7 +20,7 @@ pytestmark = pytest.mark.gpu
 
 ENV_KEYS = ("AMRESSIF_MG_SMOOTHER", "AMRESSIF_MG_ROWS32", "AMRESSIF_MG_ROWS64", "AMRESSIF_MG_ROWS_COARSE",
             "AMRESSIF_TAIL_DCT", "AMRESSIF_TAIL_EXT", "AMRESSIF_DCT_MFMA64",
-            "AMRESSIF_START_ROWS", "AMRESSIF_MG_UP64")
+            "AMRESSIF_START_ROWS", "AMRESSIF_MG_UP64", "AMRESSIF_TAIL_WAVES")
 
 
 def _solve(n, precision, b, cycles, tol=0.0, **env):
@@ -106,13 +106,14 @@ def test_exact_tail_solve_matches_tail_v_cycle(n, precision):
 
 
 @pytest.mark.parametrize("n", [256, 512, 1024, 2048])
-def test_tail_kernel_with_the_level_above_matches_separate_launches(n):
+@pytest.mark.parametrize("waves", ["4", "16"])
+def test_tail_kernel_with_the_level_above_matches_separate_launches(n, waves):
     """FP32 V-cycle: the 128^2 level smoothed inside the tail kernel (rows
-    traded between its waves through LDS) gives the same cycles as its two
-    smoother launches, up to FP32 rounding."""
+    traded between its 4 or 16 waves through LDS) gives the same cycles as its
+    two smoother launches, up to FP32 rounding."""
     b = _rhs(n, 31 + n)
     ref, res_ref = _solve(n, 1, b, 3, AMRESSIF_TAIL_EXT="0")
-    got, res = _solve(n, 1, b, 3, AMRESSIF_TAIL_EXT="1")
+    got, res = _solve(n, 1, b, 3, AMRESSIF_TAIL_EXT="1", AMRESSIF_TAIL_WAVES=waves)
     assert res_ref < 1e-2
     assert abs(res - res_ref) <= 1e-5, (res, res_ref)
     assert rel(got, ref) < 1e-5

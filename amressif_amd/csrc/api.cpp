@@ -1,6 +1,6 @@
-// C-ABI entry points (include/amressif.h) and the host-side step driver.
+// C-ABI entry points (include/amressif.h) and the host-side step loop.
 //
-// The driver mirrors the reference's time loop (Source/src1_rever/main.cpp:
+// The step loop mirrors the reference's time loop (Source/src1_rever/main.cpp:
 // 875-1202) with the per-iteration host syncs replaced by device-side loop
 // control: pseudo-iterations and V-cycles are enqueued in batches of guarded
 // kernels (each returns at once when the control block says the loop is
@@ -85,7 +85,7 @@ int check_bc(const amressif_inputs &in, std::string &msg)
             return AMRESSIF_ERR_BC;
         }
         if (lo == 111 || hi == 111) {
-            msg = "periodic boundaries (111) are not supported by this build";
+            msg = "periodic boundaries (111) are not supported by this library";
             return AMRESSIF_ERR_UNSUPPORTED;
         }
     }

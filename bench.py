@@ -224,7 +224,7 @@ def kernel_probe(api, W, n, ren, which_prec):
 def main():
     args = parse()
     if args.gpus != 1:
-        sys.stderr.write("bench.py: this build runs on one GPU only (the step is measured on one MI355X); "
+        sys.stderr.write("bench.py: the library runs on one GPU only (the step is measured on one MI355X); "
                          f"--gpus {args.gpus} is not supported\n")
         sys.exit(2)
     import numpy as np

@@ -47,8 +47,8 @@ extern "C" {
 #define AMRESSIF_FLAG_LIN_CAP 16  /* tangent/adjoint momentum solve reached its sweep cap */
 
 /* The reference's ParmParse keys (Source/src1_rever/main.cpp:89-157) plus the
- * solver settings it hard-codes (poisson.cpp:159-182) and this build's
- * precision switch.  Defaults are the reference's. */
+ * solver settings it hard-codes (poisson.cpp:159-182) and this library's
+ * precision and solver switches.  Defaults are the reference's. */
 typedef struct {
     int n_cell;                 /* required */
     int max_grid_size[3];       /* required (x, y, z) */
@@ -75,7 +75,7 @@ typedef struct {
     double poisson_tol_abs;     /* 0 (poisson.cpp:182) */
     int mg_max_iter;            /* 2000 (poisson.cpp:159) */
     int mg_bottom_max;          /* coarsen while the level is larger than this and even; 4 */
-    /* this build */
+    /* this library's own settings */
     int precision;              /* AMRESSIF_FP64 or AMRESSIF_MIXED */
     int poisson_solver;         /* 0: multigrid V-cycle iteration (the MLMG solve, default);
                                    1: FGMRES(gmres_restart) right-preconditioned by one V-cycle

@@ -171,7 +171,7 @@ struct Ctx {
     // FP32 V-cycle: the level above it smoothed inside the same kernel
     // (k_tail_dct_ext; 32, 64 or 128 rows, at most 128 columns)
     int tail_ext = 1;
-    int tail_waves = 16;  // waves of that kernel (4 or 16)
+    int tail_waves = 16;  // waves of that kernel and of the DCT tail kernel (4 or 16)
     const double *lid = nullptr;  // caller-owned device lid profile or null
     Level lev[kMaxLevels]{};
     int nlev = 0;

@@ -881,6 +881,28 @@ __device__ __forceinline__ void dct_gemm_mfma64(const double *A, const double *B
         }
 }
 
+// FP64 with 16 waves: wave w computes the one 16 x 16 block (w / 4, w % 4) of O
+template <bool TA, bool TB>
+__device__ __forceinline__ void dct_gemm_mfma64_16(const double *A, const double *B, double *O, const double *scale)
+{
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6, li = l & 15, lk = l >> 4;
+    const int rb = w >> 2, cb = w & 3;
+    dct_v4d acc = dct_v4d{0.0, 0.0, 0.0, 0.0};
+    const int ar = 16 * rb + li, bc = 16 * cb + li;
+#pragma unroll 4
+    for (int k0 = 0; k0 < kDct; k0 += 4) {
+        const int kk = k0 + lk;
+        const double a = TA ? A[kk * kDctLd + ar] : A[ar * kDctLd + kk];
+        const double b = TB ? B[bc * kDctLd + kk] : B[kk * kDctLd + bc];
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int row = 16 * rb + lk + 4 * v, col = 16 * cb + li;
+        O[row * kDctLd + col] = scale ? acc[v] * scale[row * kDct + col] : acc[v];
+    }
+}
+
 // FP32 with 16 waves: wave w computes the one 16 x 16 block (w / 4, w % 4) of O
 template <bool TA, bool TB>
 __device__ __forceinline__ void dct_gemm_mfma16(const float *A, const float *B, float *O, const float *scale)
@@ -904,11 +926,14 @@ __device__ __forceinline__ void dct_gemm_mfma16(const float *A, const float *B, 
 }
 
 // MF: FP64 products on the matrix cores (FP32 ones always are); NW: waves in
-// the workgroup (4, or 16 for the FP32 products)
+// the workgroup (4, or 16 for the products on the matrix cores)
 template <class T, bool TA, bool TB, bool MF, int NW = 4>
 __device__ __forceinline__ void dct_product(const T *A, const T *B, T *O, const T *scale)
 {
-    if constexpr (sizeof(T) == 4 && NW == 16)
+    static_assert(NW == 4 || (NW == 16 && (MF || sizeof(T) == 4)), "16 waves: matrix-core products only");
+    if constexpr (sizeof(T) == 8 && MF && NW == 16)
+        dct_gemm_mfma64_16<TA, TB>((const double *)A, (const double *)B, (double *)O, (const double *)scale);
+    else if constexpr (sizeof(T) == 4 && NW == 16)
         dct_gemm_mfma16<TA, TB>((const float *)A, (const float *)B, (float *)O, (const float *)scale);
     else if constexpr (sizeof(T) == 4)
         dct_gemm_mfma<TA, TB>((const float *)A, (const float *)B, (float *)O, (const float *)scale);
@@ -933,34 +958,36 @@ __device__ __forceinline__ void dct_solve(const T *Cx, const T *Cy, T *B0, T *B1
     __syncthreads();
 }
 
-template <class T, bool MF>
-__global__ __launch_bounds__(256) void k_tail_dct(const T *rtop, int ptop, T *etop, int nx, int ny, const T *tab,
-                                                  const Ctl *ctl)
+// NW: waves in the workgroup (4, or 16 with the products on the matrix cores)
+template <class T, bool MF, int NW = 4>
+__global__ __launch_bounds__(64 * NW) void k_tail_dct(const T *rtop, int ptop, T *etop, int nx, int ny,
+                                                      const T *tab, const Ctl *ctl)
 {
     __shared__ T Cx[kDct * kDctLd], Cy[kDct * kDctLd], B0[kDct * kDctLd], B1[kDct * kDctLd];
+    constexpr int NT = 64 * NW, NQ = kDct * kDct / NT;
     const int t = threadIdx.x;
     // loads first (the tables and r, zero-padded to 64 x 64); their latency overlaps the flag's
-    T vx[16], vy[16], vr[16];
+    T vx[NQ], vy[NQ], vr[NQ];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int e = t + 256 * q, row = e >> 6, col = e & 63;
+    for (int q = 0; q < NQ; ++q) {
+        const int e = t + NT * q, row = e >> 6, col = e & 63;
         vx[q] = tab[e];
         vy[q] = tab[kDct * kDct + e];
         vr[q] = row < ny && col < nx ? rtop[col + (size_t)ptop * row] : T(0);
     }
     if (ctl->mg_done | ctl->gm_stop) return;
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int e = t + 256 * q, row = e >> 6, col = e & 63;
+    for (int q = 0; q < NQ; ++q) {
+        const int e = t + NT * q, row = e >> 6, col = e & 63;
         Cx[row * kDctLd + col] = vx[q];
         Cy[row * kDctLd + col] = vy[q];
         B0[row * kDctLd + col] = vr[q];
     }
     __syncthreads();
-    dct_solve<T, MF>(Cx, Cy, B0, B1, tab + 2 * kDct * kDct);
+    dct_solve<T, MF, NW>(Cx, Cy, B0, B1, tab + 2 * kDct * kDct);
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int e = t + 256 * q, row = e >> 6, col = e & 63;
+    for (int q = 0; q < NQ; ++q) {
+        const int e = t + NT * q, row = e >> 6, col = e & 63;
         if (row < ny && col < nx) etop[col + (size_t)ptop * row] = B0[row * kDctLd + col];
     }
 }
@@ -1414,9 +1441,9 @@ __global__ __launch_bounds__(256) void k_poisson_start_rows(double *b, const dou
 __global__ __launch_bounds__(1024) void k_fin_start(const double *partials, int nb, Ctl *ctl, int n, int from_u)
 {
     __shared__ double sh[80];
-    if (ctl->mg_done) return;  // a skipped phase (k_mg_start)
+    const int done = __hip_atomic_load(&ctl->mg_done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // a skipped phase (k_mg_start)
     double mr = 0.0, sp = 0.0, mb = 0.0, md = 0.0, sb = 0.0;
-    for (int q = threadIdx.x; q < nb; q += blockDim.x) {
+    for (int q = threadIdx.x; q < nb; q += blockDim.x) {  // (loads issued before the flag is waited for)
         const double *o = partials + (size_t)kStartParts * q;
         mr = (o[0] > mr || o[0] != o[0]) ? o[0] : mr;
         sp += o[1];
@@ -1424,6 +1451,7 @@ __global__ __launch_bounds__(1024) void k_fin_start(const double *partials, int 
         md = (o[3] > md || o[3] != o[3]) ? o[3] : md;
         sb += o[4];
     }
+    if (done) return;
     mr = block_reduce<true>(mr, sh);
     sp = block_reduce<false>(sp, sh + 16);
     mb = block_reduce<true>(mb, sh + 32);
@@ -1500,13 +1528,14 @@ hipError_t launch_mg_start(const Ctx &c, int require_mom, const double *us, cons
 __global__ __launch_bounds__(1024) void k_fin_resid(const double *partials, int nb, Ctl *ctl, int n)
 {
     __shared__ double sh[32];
-    if (ctl->mg_done) return;
+    const int done = __hip_atomic_load(&ctl->mg_done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     double m = 0.0, s = 0.0;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {  // (loads issued before the flag is waited for)
         const double pm = partials[2 * b];
         m = (pm > m || pm != pm) ? pm : m;
         s += partials[2 * b + 1];
     }
+    if (done) return;
     m = block_reduce<true>(m, sh);
     s = block_reduce<false>(s, sh + 16);
     if (threadIdx.x == 0) {
@@ -1700,10 +1729,11 @@ static hipError_t vcycle_impl(const Ctx &c, const T *pinv, const T *r0, const Ou
         const T *rt = lt == 0 ? r0 : (const T *)lev_r<T>(Lt);
         const T *tab = sizeof(T) == 4 ? (const T *)(const void *)c.dct_tab32 : (const T *)(const void *)c.dct_tab;
         if (tab) {  // the exact solve of the top tail level
+            const bool mf = sizeof(T) == 4 || c.dct_mfma64, w16 = mf && c.tail_waves == 16;
             void (*kern)(const T *, int, T *, int, int, const T *, const Ctl *) =
-                c.dct_mfma64 ? k_tail_dct<T, true> : k_tail_dct<T, false>;
-            hipLaunchKernelGGL(kern, dim3(1), dim3(256), 0, c.stream, rt, Lt.pitch, lev_e<T>(Lt, 1), Lt.nx, Lt.ny,
-                               tab, ctl);
+                w16 ? k_tail_dct<T, true, 16> : mf ? k_tail_dct<T, true, 4> : k_tail_dct<T, false, 4>;
+            hipLaunchKernelGGL(kern, dim3(1), dim3(w16 ? 1024 : 256), 0, c.stream, rt, Lt.pitch, lev_e<T>(Lt, 1),
+                               Lt.nx, Lt.ny, tab, ctl);
         } else {
             size_t lds = 0;
             const TailLevels tl = tail_levels(c, lt, &lds, sizeof(T));
@@ -1792,9 +1822,9 @@ hipError_t configure_tail(Ctx &c)
 {
     const int mixed = c.in.precision == AMRESSIF_MIXED;
     int dev = 0;
-    hipGetDevice(&dev);
+    if (hipError_t e = hipGetDevice(&dev)) return e;
     int max_lds = 65536;
-    hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    if (hipError_t e = hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev)) return e;
     int lt = 0;
     while (lt < c.nlev - 1 && c.lev[lt].nx > 64) ++lt;
     for (; lt < c.nlev; ++lt) {

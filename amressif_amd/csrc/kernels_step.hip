@@ -21,9 +21,12 @@ namespace amr {
 __global__ __launch_bounds__(1024) void k_fin_pseudo(const double *partials, int nb, Ctl *ctl, int n)
 {
     __shared__ double sh[32];
-    if (ctl->mom_done) return;
+    // the partials' loads go out before the flag's value is needed (one memory
+    // round trip instead of two); they stay within the partials buffer
+    const int done = __hip_atomic_load(&ctl->mom_done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     double sx = 0.0, sy = 0.0;
     for (int b = threadIdx.x; b < nb; b += blockDim.x) { sx += partials[2 * b]; sy += partials[2 * b + 1]; }
+    if (done) return;
     sx = block_reduce<false>(sx, sh);
     sy = block_reduce<false>(sy, sh + 16);
     if (threadIdx.x == 0) {
@@ -540,12 +543,13 @@ __global__ __launch_bounds__(1024) void k_fin_project(const double *partials, in
                                                       int gated)
 {
     __shared__ double sh[16];
-    if (gated && !(ctl->mg_done && !ctl->mg_skip)) return;  // its projection did nothing (ProjGate)
+    const bool skip = gated && !(ctl->mg_done && !ctl->mg_skip);  // its projection did nothing (ProjGate)
     double m = 0.0;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {  // (loads issued before the flag is waited for)
         const double pm = partials[b];
         m = (pm > m || pm != pm) ? pm : m;
     }
+    if (skip) return;
     m = block_reduce<true>(m, sh);
     if (threadIdx.x == 0) {
         ctl->div_new = m;

@@ -120,13 +120,15 @@ def test_tail_kernel_with_the_level_above_matches_separate_launches(n, waves):
 
 
 @pytest.mark.parametrize("n", [64, 100, 256, 1024])
-def test_fp64_dct_on_matrix_cores_matches_valu(n):
+@pytest.mark.parametrize("waves", ["4", "16"])
+def test_fp64_dct_on_matrix_cores_matches_valu(n, waves):
     """FP64 exact solve of the small levels: the products on the matrix cores
-    (v_mfma_f64_16x16x4f64) and on the VALU give the same cycles up to FP64
-    rounding (a wrong operand or result layout would not)."""
+    (v_mfma_f64_16x16x4f64; one 16 x 16 output block per wave with 16 waves,
+    four with 4) and on the VALU give the same cycles up to FP64 rounding (a
+    wrong operand or result layout would not)."""
     b = _rhs(n, 41 + n)
     ref, res_ref = _solve(n, 0, b, 3, AMRESSIF_DCT_MFMA64="0")
-    got, res = _solve(n, 0, b, 3, AMRESSIF_DCT_MFMA64="1")
+    got, res = _solve(n, 0, b, 3, AMRESSIF_DCT_MFMA64="1", AMRESSIF_TAIL_WAVES=waves)
     assert abs(res - res_ref) <= 1e-9 * max(res_ref, 1e-300) + 1e-14, (res, res_ref)
     assert rel(got, ref) < 1e-12
 

@@ -31,6 +31,29 @@ __device__ inline double block_reduce(double v, double *sh)
     return r;
 }
 
+// Strided walk of a block over nb records of K doubles (record q at
+// p + S q): thread t visits q = t, t + T, t + 2T, ... (T = blockDim.x) in that
+// order, as the plain loop does, so a reduction over the records gives the
+// same bits; the loads of four records are issued before any is used (one
+// memory round trip per four records instead of one per record).
+template <int K, int S, class F>
+__device__ __forceinline__ void for_records(const double *p, int nb, F &&f)
+{
+    const int T = blockDim.x;
+    for (int q0 = threadIdx.x; q0 < nb; q0 += 4 * T) {
+        double a[4][K];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int q = q0 + u * T;
+#pragma unroll
+            for (int k = 0; k < K; ++k) a[u][k] = q < nb ? p[(size_t)S * q + k] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (q0 + u * T < nb) f(a[u]);
+    }
+}
+
 // Cell tiles of 64 x 16 for a workgroup of 256 threads (four passes of 64 x 4)
 // on a one-dimensional grid whose size is a multiple of 8.  Workgroup b runs
 // on XCD b mod 8; the map hands each XCD a contiguous range of tiles in

@@ -1190,8 +1190,21 @@ __global__ __launch_bounds__(1024) void k_mg_start(Ctl *ctl, int require_mom, co
 {
     __shared__ double sh[16];
     __shared__ int skip_s;
+    // the flag's load and the boundary faces' loads go out together (the
+    // faces are read even when the phase is skipped, and then unused)
+    const int mom_done = __hip_atomic_load(&ctl->mom_done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // with the predictor faces: the mean of b = (1.5/dt) D(u*) from the
+    // boundary fluxes (sum of D over the cells = net outflow), for the start
+    double sum = 0.0;
+    if (u) {
+        for (int j = threadIdx.x; j < n; j += blockDim.x) {
+            const size_t row = (size_t)P * j;
+            sum += (u[row + n] - u[row]) / dx;
+        }
+        for (int i = threadIdx.x; i < n; i += blockDim.x) sum += (v[(size_t)P * n + i] - v[i]) / dy;
+    }
     if (threadIdx.x == 0) {
-        const int skip = require_mom && !ctl->mom_done;
+        const int skip = require_mom && !mom_done;
         ctl->mg_done = skip;
         ctl->mg_skip = skip;
         ctl->gm_stop = 0;
@@ -1206,14 +1219,6 @@ __global__ __launch_bounds__(1024) void k_mg_start(Ctl *ctl, int require_mom, co
     }
     __syncthreads();
     if (!u || skip_s) return;
-    // with the predictor faces: the mean of b = (1.5/dt) D(u*) from the
-    // boundary fluxes (sum of D over the cells = net outflow), for the start
-    double sum = 0.0;
-    for (int j = threadIdx.x; j < n; j += blockDim.x) {
-        const size_t row = (size_t)P * j;
-        sum += (u[row + n] - u[row]) / dx;
-    }
-    for (int i = threadIdx.x; i < n; i += blockDim.x) sum += (v[(size_t)P * n + i] - v[i]) / dy;
     sum = block_reduce<false>(sum, sh);
     if (threadIdx.x == 0) ctl->b_mean = (1.5 / dt) * sum / ((double)n * (double)n);
 }
@@ -1443,14 +1448,13 @@ __global__ __launch_bounds__(1024) void k_fin_start(const double *partials, int 
     __shared__ double sh[80];
     const int done = __hip_atomic_load(&ctl->mg_done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // a skipped phase (k_mg_start)
     double mr = 0.0, sp = 0.0, mb = 0.0, md = 0.0, sb = 0.0;
-    for (int q = threadIdx.x; q < nb; q += blockDim.x) {  // (loads issued before the flag is waited for)
-        const double *o = partials + (size_t)kStartParts * q;
+    for_records<5, kStartParts>(partials, nb, [&](const double *o) {  // (loads issued before the flag is waited for)
         mr = (o[0] > mr || o[0] != o[0]) ? o[0] : mr;
         sp += o[1];
         mb = (o[2] > mb || o[2] != o[2]) ? o[2] : mb;
         md = (o[3] > md || o[3] != o[3]) ? o[3] : md;
         sb += o[4];
-    }
+    });
     if (done) return;
     mr = block_reduce<true>(mr, sh);
     sp = block_reduce<false>(sp, sh + 16);
@@ -1530,11 +1534,10 @@ __global__ __launch_bounds__(1024) void k_fin_resid(const double *partials, int 
     __shared__ double sh[32];
     const int done = __hip_atomic_load(&ctl->mg_done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     double m = 0.0, s = 0.0;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) {  // (loads issued before the flag is waited for)
-        const double pm = partials[2 * b];
-        m = (pm > m || pm != pm) ? pm : m;
-        s += partials[2 * b + 1];
-    }
+    for_records<2, 2>(partials, nb, [&](const double *o) {  // (loads issued before the flag is waited for)
+        m = (o[0] > m || o[0] != o[0]) ? o[0] : m;
+        s += o[1];
+    });
     if (done) return;
     m = block_reduce<true>(m, sh);
     s = block_reduce<false>(s, sh + 16);

@@ -25,7 +25,10 @@ __global__ __launch_bounds__(1024) void k_fin_pseudo(const double *partials, int
     // round trip instead of two); they stay within the partials buffer
     const int done = __hip_atomic_load(&ctl->mom_done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     double sx = 0.0, sy = 0.0;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) { sx += partials[2 * b]; sy += partials[2 * b + 1]; }
+    for_records<2, 2>(partials, nb, [&](const double *o) {
+        sx += o[0];
+        sy += o[1];
+    });
     if (done) return;
     sx = block_reduce<false>(sx, sh);
     sy = block_reduce<false>(sy, sh + 16);
@@ -545,10 +548,9 @@ __global__ __launch_bounds__(1024) void k_fin_project(const double *partials, in
     __shared__ double sh[16];
     const bool skip = gated && !(ctl->mg_done && !ctl->mg_skip);  // its projection did nothing (ProjGate)
     double m = 0.0;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) {  // (loads issued before the flag is waited for)
-        const double pm = partials[b];
-        m = (pm > m || pm != pm) ? pm : m;
-    }
+    for_records<1, 1>(partials, nb, [&](const double *o) {  // (loads issued before the flag is waited for)
+        m = (o[0] > m || o[0] != o[0]) ? o[0] : m;
+    });
     if (skip) return;
     m = block_reduce<true>(m, sh);
     if (threadIdx.x == 0) {

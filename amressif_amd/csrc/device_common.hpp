@@ -31,6 +31,39 @@ __device__ inline double block_reduce(double v, double *sh)
     return r;
 }
 
+// N block reductions at once: value k is a max (NaN wins) when bit k of
+// MAXMASK is set, else a sum.  Each value is reduced in block_reduce's order
+// (the same bits), with two barriers in all instead of two per value.  sh
+// holds 16 N doubles; the results are valid in thread 0.
+template <unsigned MAXMASK, int N>
+__device__ inline void block_reduce_n(double (&v)[N], double *sh)
+{
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            const double w = __shfl_down(v[k], o, 64);
+            if ((MAXMASK >> k) & 1u) v[k] = (w > v[k] || w != w) ? w : v[k];
+            else v[k] += w;
+        }
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < N; ++k) sh[16 * k + wid] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            double r = sh[16 * k];
+            for (int w = 1; w < nw; ++w) {
+                const double x = sh[16 * k + w];
+                if ((MAXMASK >> k) & 1u) r = (x > r || x != x) ? x : r;
+                else r += x;
+            }
+            v[k] = r;
+        }
+}
+
 // Strided walk of a block over nb records of K doubles (record q at
 // p + S q): thread t visits q = t, t + T, t + 2T, ... (T = blockDim.x) in that
 // order, as the plain loop does, so a reduction over the records gives the

@@ -1196,12 +1196,32 @@ __global__ __launch_bounds__(1024) void k_mg_start(Ctl *ctl, int require_mom, co
     // with the predictor faces: the mean of b = (1.5/dt) D(u*) from the
     // boundary fluxes (sum of D over the cells = net outflow), for the start
     double sum = 0.0;
-    if (u) {
-        for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    const int T = blockDim.x, t = threadIdx.x;
+    if (u && n <= 2 * T) {
+        // two rows of the x-walls' faces and two columns of the y-walls' faces per
+        // thread: all eight loads first, then the terms in the loops' order below
+        double ue[2], uw[2], vn[2], vs[2];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int q = t + k * T;
+            const bool in = q < n;
+            ue[k] = in ? u[(size_t)P * q + n] : 0.0;
+            uw[k] = in ? u[(size_t)P * q] : 0.0;
+            vn[k] = in ? v[(size_t)P * n + q] : 0.0;
+            vs[k] = in ? v[q] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+            if (t + k * T < n) sum += (ue[k] - uw[k]) / dx;
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+            if (t + k * T < n) sum += (vn[k] - vs[k]) / dy;
+    } else if (u) {
+        for (int j = t; j < n; j += T) {
             const size_t row = (size_t)P * j;
             sum += (u[row + n] - u[row]) / dx;
         }
-        for (int i = threadIdx.x; i < n; i += blockDim.x) sum += (v[(size_t)P * n + i] - v[i]) / dy;
+        for (int i = t; i < n; i += T) sum += (v[(size_t)P * n + i] - v[i]) / dy;
     }
     if (threadIdx.x == 0) {
         const int skip = require_mom && !mom_done;
@@ -1456,11 +1476,13 @@ __global__ __launch_bounds__(1024) void k_fin_start(const double *partials, int 
         sb += o[4];
     });
     if (done) return;
-    mr = block_reduce<true>(mr, sh);
-    sp = block_reduce<false>(sp, sh + 16);
-    mb = block_reduce<true>(mb, sh + 32);
-    md = block_reduce<true>(md, sh + 48);
-    sb = block_reduce<false>(sb, sh + 64);
+    double red[5] = {mr, sp, mb, md, sb};
+    block_reduce_n<0b01101u>(red, sh);  // max, sum, max, max, sum
+    mr = red[0];
+    sp = red[1];
+    mb = red[2];
+    md = red[3];
+    sb = red[4];
     if (threadIdx.x == 0) {
         if (from_u) {
             ctl->div_star = md;
@@ -1539,8 +1561,10 @@ __global__ __launch_bounds__(1024) void k_fin_resid(const double *partials, int 
         s += o[1];
     });
     if (done) return;
-    m = block_reduce<true>(m, sh);
-    s = block_reduce<false>(s, sh + 16);
+    double red[2] = {m, s};
+    block_reduce_n<1u>(red, sh);  // max, sum
+    m = red[0];
+    s = red[1];
     if (threadIdx.x == 0) {
         ctl->res_norm = m;
         ctl->phi_sum = s;

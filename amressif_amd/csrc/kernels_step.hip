@@ -30,8 +30,10 @@ __global__ __launch_bounds__(1024) void k_fin_pseudo(const double *partials, int
         sy += o[1];
     });
     if (done) return;
-    sx = block_reduce<false>(sx, sh);
-    sy = block_reduce<false>(sy, sh + 16);
+    double red[2] = {sx, sy};
+    block_reduce_n<0u>(red, sh);
+    sx = red[0];
+    sy = red[1];
     if (threadIdx.x == 0) {
         const double npts = (double)n * (double)n;
         const double ex = sqrt(sx) / sqrt(npts), ey = sqrt(sy) / sqrt(npts);

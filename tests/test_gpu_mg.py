@@ -110,9 +110,10 @@ def test_exact_tail_solve_matches_tail_v_cycle(n, precision):
 def test_tail_kernel_with_the_level_above_matches_separate_launches(n, waves):
     """FP32 V-cycle: the 128^2 level smoothed inside the tail kernel (rows
     traded between its 4 or 16 waves through LDS) gives the same cycles as its
-    two smoother launches, up to FP32 rounding."""
+    two smoother launches and the DCT-only tail kernel of the same wave count,
+    up to FP32 rounding."""
     b = _rhs(n, 31 + n)
-    ref, res_ref = _solve(n, 1, b, 3, AMRESSIF_TAIL_EXT="0")
+    ref, res_ref = _solve(n, 1, b, 3, AMRESSIF_TAIL_EXT="0", AMRESSIF_TAIL_WAVES=waves)
     got, res = _solve(n, 1, b, 3, AMRESSIF_TAIL_EXT="1", AMRESSIF_TAIL_WAVES=waves)
     assert res_ref < 1e-2
     assert abs(res - res_ref) <= 1e-5, (res, res_ref)

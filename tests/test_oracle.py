@@ -192,6 +192,94 @@ def test_momentum_rhs_vanishes_on_wall_faces():
     assert np.abs(rx[:, 1:-1]).min() > 0
 
 
+def test_momentum_rhs_pressure_term_closed_form():
+    """fn_rhs.cpp:45,55 with fluxTotal = 0: R = -G p.  At rest with no-slip
+    walls and a resting lid every flux is 0; p = a x + b y gives R_x = -a,
+    R_y = -b on the interior faces and 0 on the wall faces (mirror ghosts).
+    phi (userCtx comp 1) is set to a different field: G phi must not enter R.
+    A flipped sign, G phi in place of G p, or a swapped component fails."""
+    n = 12
+    h = 1.0 / n
+    a, b = 0.625, -2.375
+    o = Oracle(n, 50.0, 3e-4, bc_hi=(135, 135), inflow=(0.0, 0.0))
+    xc = (np.arange(n) + 0.5) * h
+    ctx = o.ctx()
+    ctx[0, 1:-1, 1:-1] = a * xc[None, :] + b * xc[:, None]
+    ctx[1, 1:-1, 1:-1] = 7.0 * xc[None, :] ** 2 - 3.0 * xc[:, None]
+    o.userctx_ghosts(ctx)
+    rx, ry = o.stage_rhs(o.xface(), o.yface(), ctx)
+    assert np.allclose(rx[:, 1:-1], -a, rtol=1e-13, atol=0)
+    assert np.allclose(ry[1:-1, :], -b, rtol=1e-13, atol=0)
+    assert np.all(rx[:, [0, -1]] == 0.0) and np.all(ry[[0, -1], :] == 0.0)
+
+
+def test_momentum_rhs_face_average_of_total_flux():
+    """fn_rhs.cpp:45,55 with G p = 0: R_x(i,j) = (T_U(i-1,j) + T_U(i,j))/2 and
+    R_y(i,j) = (T_V(i,j-1) + T_V(i,j))/2.  A cell flux linear in the index,
+    T_U(i,j) = alpha i + beta j + gamma, puts R_x at the face's position
+    alpha (i - 1/2) + beta j + gamma; a row-constant T gives R = T.  A dropped
+    1/2, an index shift or the wrong component fails."""
+    n = 9
+    o = Oracle(n, 50.0, 3e-4)
+    al, be, ga = 1.5, -0.25, 3.0
+    de, ep, ze = -0.75, 2.0, -1.0
+    I = np.arange(-1, n + 1)[None, :].astype(float)
+    J = np.arange(-1, n + 1)[:, None].astype(float)
+    ft = o.ctx()
+    ft[0] = al * I + be * J + ga
+    ft[1] = de * I + ep * J + ze
+    gpx, gpy = o.xface(), o.yface()
+    rx, ry = o.momentum_rhs(ft, gpx, gpy)
+    i = np.arange(n + 1)[None, :]
+    j = np.arange(n)[:, None]
+    assert np.array_equal(rx, al * (i - 0.5) + be * j + ga + 0.0 * rx)
+    i = np.arange(n)[None, :]
+    j = np.arange(n + 1)[:, None]
+    assert np.array_equal(ry, de * i + ep * (j - 0.5) + ze + 0.0 * ry)
+    # row-constant T: R equals T
+    ft[0] = 4.0 * J + 0.0 * I
+    ft[1] = -1.5 + 0.0 * I * J
+    rx, ry = o.momentum_rhs(ft, gpx, gpy)
+    assert np.array_equal(rx, np.broadcast_to(4.0 * np.arange(n)[:, None], rx.shape))
+    assert np.all(ry == -1.5)
+
+
+def test_projection_scale_closed_form():
+    """poisson.cpp:251,256,309 with fn_rhs.cpp:116: take u discretely
+    divergence-free and u* = u + G psi (Neumann face gradient, 0 on the walls).
+    Then b = (1.5/dt) D u* = (1.5/dt) L psi, the mean-zero solution is
+    phi = (1.5/dt)(psi - mean psi), the update u* - (dt/1.5) G phi returns u,
+    and p gains exactly phi.  dt != 1.5, so swapping 1.5/dt for dt/1.5 in
+    either place, or dropping the factor, fails."""
+    n = 32
+    dt = 2.5e-3
+    rng = np.random.default_rng(21)
+    o = Oracle(n, 100.0, dt, poisson="dct")
+    u, v = W.seeded_velocity(n)
+    psi = rng.uniform(-1, 1, (n, n))
+    tmp = o.ctx()
+    tmp[0, 1:-1, 1:-1] = psi
+    o.userctx_ghosts(tmp)
+    gx, gy, _, _ = o.gradient(tmp)
+    us, vs = u + gx, v + gy
+    rhs, _, _ = o.poisson_rhs(us, vs)
+    sol, _, res = o.poisson_solve(rhs)
+    phi_exact = (1.5 / dt) * (psi - psi.mean())
+    assert res < 1e-12
+    assert np.allclose(sol[1:-1, 1:-1], phi_exact, rtol=0, atol=1e-10 * np.abs(phi_exact).max())
+    ctx = o.ctx()
+    p0 = rng.uniform(-1, 1, (n, n))
+    ctx[0, 1:-1, 1:-1] = p0
+    ctx[1, 1:-1, 1:-1] = sol[1:-1, 1:-1]
+    o.userctx_ghosts(ctx)
+    un, vn, _ = o.update_solution(ctx, us, vs)
+    scale = np.abs(u).max()
+    assert np.abs(un - u).max() <= 1e-10 * scale and np.abs(vn - v).max() <= 1e-10 * scale
+    assert np.allclose(ctx[0, 1:-1, 1:-1], p0 + phi_exact, rtol=0, atol=1e-10 * np.abs(phi_exact).max())
+    # the Neumann mirror of the new p (main.cpp:1078-1158, poisson.cpp:312-391)
+    assert np.array_equal(ctx[0, 0, 1:-1], ctx[0, 1, 1:-1]) and np.array_equal(ctx[0, 1:-1, -1], ctx[0, 1:-1, -2])
+
+
 # ------------------------------------------------------------------- RK4
 def test_rk4_dual_time_amplification_polynomial():
     n = 8
@@ -283,6 +371,29 @@ def test_step_invariants_and_solver_agreement(init):
     assert np.all(a["u"][:, [0, -1]] == 0.0) and np.all(a["v"][[0, -1], :] == 0.0)
     p = a["ctx"][0, 1:-1, 1:-1]
     assert abs(p.mean()) <= 1e-12 * max(1.0, np.abs(p).max())
+
+
+def test_step_reads_p_and_not_phi():
+    """main.cpp:933 computes G p and G phi from userCtx; the momentum RHS takes
+    G p only (fn_rhs.cpp:45,55), and the step's Poisson solve starts from 0
+    (poisson.cpp:63).  So the phi a step starts with cannot change the step,
+    while p does.  Using G phi in the step's RHS fails the first check."""
+    n = 16
+    rng = np.random.default_rng(22)
+    base = W.initial_state(n, "seeded")
+    base["ctx"][0, 1:-1, 1:-1] = rng.uniform(-1, 1, (n, n))
+    runs = []
+    for phi_scale, p_shift in ((0.0, 0.0), (5.0, 0.0), (0.0, 1.0)):
+        st = {k: v.copy() for k, v in base.items()}
+        st["ctx"][1, 1:-1, 1:-1] = phi_scale * rng.uniform(-1, 1, (n, n))
+        st["ctx"][0, 1:-1, 1:-1] += p_shift * np.linspace(0, 1, n)[None, :]
+        o = Oracle(n, 400.0, W.dt_from_cfl(n, 0.5), poisson="dct")
+        o.advance(st, 2)
+        runs.append(st)
+    for k in ("u", "v"):
+        assert np.array_equal(runs[0][k], runs[1][k])
+        assert np.abs(runs[0][k] - runs[2][k]).max() > 1e-6
+    assert np.array_equal(runs[0]["ctx"][0], runs[1]["ctx"][0])
 
 
 def test_first_step_from_rest_is_lid_driven():

@@ -82,6 +82,14 @@ class Array4(ctypes.Structure):
     ]
 
 
+class MultiFabC(ctypes.Structure):
+    _fields_ = [
+        ("nbox", ctypes.c_int),
+        ("fab", ctypes.POINTER(Array4)),
+        ("ngrow", ctypes.c_int),
+    ]
+
+
 class StepStats(ctypes.Structure):
     _fields_ = [
         ("pseudo_iters", ctypes.c_int),
@@ -141,7 +149,13 @@ _sig = {
     "amressif_set_lid_profile": (_c.c_int, [_c.c_void_p, _c.c_void_p]),
     "amressif_load_state": (_c.c_int, [_c.c_void_p, _A4, _A4, _A4]),
     "amressif_store_state": (_c.c_int, [_c.c_void_p, _A4, _A4, _A4, _A4]),
+    "amressif_load_state_mf": (_c.c_int, [_c.c_void_p, _P(MultiFabC), _P(MultiFabC), _P(MultiFabC)]),
+    "amressif_store_state_mf": (_c.c_int, [_c.c_void_p, _P(MultiFabC), _P(MultiFabC), _P(MultiFabC), _P(MultiFabC)]),
     "amressif_advance": (_c.c_int, [_c.c_void_p, _c.c_int, _P(StepStats)]),
+    "amressif_init_state": (_c.c_int, [_c.c_void_p, _P(_c.c_double), _c.c_int]),
+    "amressif_write_checkpoint": (_c.c_int, [_c.c_void_p, _c.c_char_p, _c.c_int, _c.c_double]),
+    "amressif_read_checkpoint": (_c.c_int, [_c.c_void_p, _c.c_char_p, _P(_c.c_double)]),
+    "amressif_write_plotfile": (_c.c_int, [_c.c_void_p, _c.c_char_p, _c.c_char_p, _c.c_int, _c.c_double]),
     "amressif_cont2cart": (_c.c_int, [_c.c_void_p, _A4, _A4]),
     "amressif_momentum_rhs": (_c.c_int, [_c.c_void_p, _A4, _A4, _A4]),
     "amressif_rk_stage": (_c.c_int, [_c.c_void_p, _c.c_int, _A4, _A4, _A4, _A4, _A4]),
@@ -258,6 +272,73 @@ def cell_view(t, n, ghosts):
     return _a4(t, (-ghosts, -ghosts), (n - 1 + ghosts, n - 1 + ghosts), ncomp)
 
 
+# ------------------------------------------------------ multi-box MultiFabs
+def box_array(n: int, max_grid_size) -> list:
+    """Cell boxes (lo0, lo1, hi0, hi1) of the n x n domain chopped so that no
+    side exceeds max_grid_size (x, y): BoxArray::maxSize (main.cpp:317-320).
+    A side of length L > m is cut into k = ceil(L / m) pieces as even as
+    possible, the first L mod k of them one cell longer; boxes are listed with
+    x varying fastest."""
+    def cuts(length, m):
+        k = -(-length // m)
+        size, extra = divmod(length, k)
+        out, lo = [], 0
+        for b in range(k):
+            w = size + (1 if b < extra else 0)
+            out.append((lo, lo + w - 1))
+            lo += w
+        return out
+    mx, my = (max_grid_size, max_grid_size) if isinstance(max_grid_size, int) else tuple(max_grid_size)[:2]
+    return [(x0, y0, x1, y1) for (y0, y1) in cuts(n, my) for (x0, x1) in cuts(n, mx)]
+
+
+class MultiFab:
+    """Per-box torch tensors of one field over a box_array, with the ctypes
+    amr_multifab that views them.  kind: "x" / "y" faces (nodal in that
+    direction, both bounding faces per box) or "cell" (ncomp components, ngrow
+    ghosts).  Tensor shapes: x faces (ny_b, nx_b + 1), y faces (ny_b + 1, nx_b),
+    cells (ncomp, ny_b + 2 g, nx_b + 2 g)."""
+
+    def __init__(self, boxes, kind: str, ncomp: int = 1, ngrow: int = 0, device="cuda", tensors=None):
+        self.boxes, self.kind, self.ncomp, self.ngrow = list(boxes), kind, ncomp, ngrow
+        g = ngrow
+        if tensors is None:
+            tensors = [torch.zeros(self.shape(b), dtype=torch.float64, device=device) for b in self.boxes]
+        self.fabs = list(tensors)
+        arr = (Array4 * len(self.boxes))()
+        for k, ((x0, y0, x1, y1), t) in enumerate(zip(self.boxes, self.fabs)):
+            assert tuple(t.shape) == self.shape(self.boxes[k]), (t.shape, self.shape(self.boxes[k]))
+            ex, ey = (1, 0) if kind == "x" else ((0, 1) if kind == "y" else (0, 0))
+            arr[k] = _a4(t, (x0 - g, y0 - g), (x1 + ex + g, y1 + ey + g), ncomp)
+        self._arr = arr
+        self.c = MultiFabC(len(self.boxes), ctypes.cast(arr, ctypes.POINTER(Array4)), ngrow)
+
+    def shape(self, box):
+        x0, y0, x1, y1 = box
+        nx, ny, g = x1 - x0 + 1, y1 - y0 + 1, self.ngrow
+        if self.kind == "x":
+            return (ny + 2 * g, nx + 1 + 2 * g)
+        if self.kind == "y":
+            return (ny + 1 + 2 * g, nx + 2 * g)
+        return (self.ncomp, ny + 2 * g, nx + 2 * g)
+
+    @classmethod
+    def from_global(cls, boxes, kind, a, ncomp=1, ngrow=0, src_ghosts=0, device="cuda"):
+        """split a global numpy array (the single-box layout, src_ghosts ghost
+        layers for cells) into per-box device tensors"""
+        g, sg = ngrow, src_ghosts
+        parts = []
+        for (x0, y0, x1, y1) in boxes:
+            if kind == "x":
+                blk = a[y0:y1 + 1, x0:x1 + 2]
+            elif kind == "y":
+                blk = a[y0:y1 + 2, x0:x1 + 1]
+            else:
+                blk = a[:, y0 - g + sg:y1 + 1 + g + sg, x0 - g + sg:x1 + 1 + g + sg]
+            parts.append(torch.as_tensor(np.ascontiguousarray(blk), dtype=torch.float64, device=device))
+        return cls(boxes, kind, ncomp, ngrow, device, parts)
+
+
 # ----------------------------------------------------------------- context
 class Context:
     """One amressif_ctx on the current torch HIP stream."""
@@ -325,6 +406,45 @@ class Context:
         cc = cell_view(velCart, n, (velCart.shape[-1] - n) // 2) if velCart is not None else None
         self._check(_lib.amressif_store_state(self._h, vc, vp, ctypes.byref(uc) if uc is not None else None,
                                               ctypes.byref(cc) if cc is not None else None))
+
+    def load_state_mf(self, u: MultiFab, v: MultiFab, uprev: MultiFab, vprev: MultiFab, ctx: MultiFab | None = None):
+        """amressif_load_state_mf: the state from multi-box MultiFabs"""
+        vc = (MultiFabC * 2)(u.c, v.c)
+        vp = (MultiFabC * 2)(uprev.c, vprev.c)
+        self._check(_lib.amressif_load_state_mf(self._h, vc, vp, ctypes.byref(ctx.c) if ctx is not None else None))
+
+    def store_state_mf(self, u=None, v=None, uprev=None, vprev=None, ctx=None, velCart=None):
+        """amressif_store_state_mf: the state into multi-box MultiFabs (any may be None)"""
+        vc = (MultiFabC * 2)(u.c, v.c) if u is not None else None
+        vp = (MultiFabC * 2)(uprev.c, vprev.c) if uprev is not None else None
+        self._check(_lib.amressif_store_state_mf(self._h, vc, vp, ctypes.byref(ctx.c) if ctx is not None else None,
+                                                 ctypes.byref(velCart.c) if velCart is not None else None))
+
+    def init_state(self, psi_coeffs=None):
+        """amressif_init_state: the cavity at rest (hybrid_grid_init), or the
+        seeded streamfunction field of the kmax x kmax coefficients a[kx-1, ky-1]"""
+        if psi_coeffs is None:
+            self._check(_lib.amressif_init_state(self._h, None, 0))
+        else:
+            a = np.ascontiguousarray(psi_coeffs, dtype=np.float64)
+            assert a.ndim == 2 and a.shape[0] == a.shape[1]
+            self._check(_lib.amressif_init_state(self._h, a.ctypes.data_as(_P(_c.c_double)), a.shape[0]))
+
+    def write_checkpoint(self, directory: str, step: int, time: float):
+        """SaveCheckpoint: <directory>/checkpoint%05d (AMReX Header + VisMF Level_0 MultiFabs)"""
+        self._check(_lib.amressif_write_checkpoint(self._h, directory.encode(), step, time))
+        return os.path.join(directory, f"checkpoint{step:05d}")
+
+    def read_checkpoint(self, path: str) -> float:
+        """LoadCheckpoint: the state from a checkpoint directory; returns its time"""
+        t = ctypes.c_double()
+        self._check(_lib.amressif_read_checkpoint(self._h, path.encode(), ctypes.byref(t)))
+        return t.value
+
+    def write_plotfile(self, directory: str, name: str, step: int, time: float):
+        """Export_Flow_Field: <directory>/<name>%05d with pressure, U, V"""
+        self._check(_lib.amressif_write_plotfile(self._h, directory.encode(), name.encode(), step, time))
+        return os.path.join(directory, f"{name}{step:05d}")
 
     def advance(self, nsteps: int, with_stats: bool = True):
         st = (StepStats * max(1, nsteps))() if with_stats else None

@@ -1,4 +1,5 @@
-"""Builds libamressif_hip.so (the HIP kernels + C-ABI) in-tree for gfx950.
+"""Builds libamressif_hip.so (the HIP kernels + C-ABI) in-tree for gfx950, and
+the driver program amressif_run on it.
 
 Run as ``python -m amressif_amd.build``.  hipcc cross-compiles without a GPU.
 """
@@ -12,7 +13,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libamressif_hip.so")
-SOURCES = ["kernels_stage.hip", "kernels_step.hip", "kernels_mg.hip", "kernels_adj.hip", "kernels_gmres.hip", "kernels_amr.hip", "api.cpp", "amr_two_level.cpp"]
+RUN = os.path.join(HERE, "amressif_run")
+SOURCES = ["kernels_stage.hip", "kernels_step.hip", "kernels_mg.hip", "kernels_adj.hip", "kernels_gmres.hip", "kernels_amr.hip", "api.cpp", "amr_two_level.cpp", "io_amrex.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [
@@ -48,6 +50,15 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     for o in objs:
         os.remove(o)
+    # the driver program (host C++ on the C-ABI only)
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(ROOT, "include"),
+           os.path.join(CSRC, "amressif_run.cpp"), "-o", RUN + ".tmp", "-L" + HERE, "-lamressif_hip",
+           "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("driver build failed")
+    os.replace(RUN + ".tmp", RUN)
     return LIB
 
 

@@ -903,14 +903,13 @@ int amressif_create(const amressif_inputs *in, void *hip_stream, amressif_ctx **
         if (e != hipSuccess) return bail(AMRESSIF_ERR_HIP, hipGetErrorString(e));
     }
     // zero the state (a cavity at rest) so advance works before load_state
-    for (int d = 0; d < 2; ++d) {
-        hipMemset(c.U[d], 0, fsz * 8);
-        hipMemset(c.Uprev[d], 0, fsz * 8);
+    {
+        double *zero[] = {c.U[0], c.U[1], c.Uprev[0], c.Uprev[1], c.p, c.p_alt, c.phiv[0], c.phiv[1]};
+        for (double *z : zero) {
+            const hipError_t e = hipMemset(z, 0, fsz * 8);
+            if (e != hipSuccess) return bail(AMRESSIF_ERR_HIP, hipGetErrorString(e));
+        }
     }
-    hipMemset(c.p, 0, fsz * 8);
-    hipMemset(c.p_alt, 0, fsz * 8);
-    hipMemset(c.phiv[0], 0, fsz * 8);
-    hipMemset(c.phiv[1], 0, fsz * 8);
     if (hipDeviceSynchronize() != hipSuccess) return bail(AMRESSIF_ERR_HIP, "device sync failed");
 #undef ALLOC
     *out = ctx;
@@ -921,14 +920,14 @@ int amressif_destroy(amressif_ctx *ctx)
 {
     if (!ctx) return AMRESSIF_OK;
     Ctx &c = ctx->c;
-    if (c.stream) hipStreamSynchronize(c.stream);
-    for (void *p : c.allocs) hipFree(p);
-    if (c.ctl_host) hipHostFree(c.ctl_host);
-    if (c.poll_ev) hipEventDestroy(c.poll_ev);
-    if (c.log) hipFree(c.log);
+    if (c.stream) (void)hipStreamSynchronize(c.stream);
+    for (void *p : c.allocs) (void)hipFree(p);
+    if (c.ctl_host) (void)hipHostFree(c.ctl_host);
+    if (c.poll_ev) (void)hipEventDestroy(c.poll_ev);
+    if (c.log) (void)hipFree(c.log);
     if (c.kp) {
-        for (const amr::KProf::Rec &r : c.kp->recs) { hipEventDestroy(r.e0); hipEventDestroy(r.e1); }
-        for (hipEvent_t e : c.kp->pool) hipEventDestroy(e);
+        for (const amr::KProf::Rec &r : c.kp->recs) { (void)hipEventDestroy(r.e0); (void)hipEventDestroy(r.e1); }
+        for (hipEvent_t e : c.kp->pool) (void)hipEventDestroy(e);
         delete c.kp;
     }
     delete c.adj;
@@ -1002,6 +1001,181 @@ int amressif_store_state(const amressif_ctx *cctx, amr_array4 velCont[2], amr_ar
     HIPCHK(hipStreamSynchronize(c.stream));
     return AMRESSIF_OK;
 }
+
+}  // extern "C"
+
+namespace {
+
+// Valid cell box of one fab of a multi-box MultiFab: kind 0 / 1 = x / y faces
+// (nodal in that direction), 2 = cells.  The valid boxes must tile the domain.
+struct CellBox {
+    int lo0, lo1, hi0, hi1;
+};
+
+int check_mf(amressif_ctx *ctx, const amr_multifab *mf, int kind, int ncomp, int max_grow, const char *what,
+             std::vector<CellBox> &boxes)
+{
+    const int n = ctx->c.g.n;
+    const std::string w(what);
+    if (!mf || mf->nbox < 1 || !mf->fab) return fail(ctx, AMRESSIF_ERR_ARG, w + ": empty MultiFab");
+    if (mf->ngrow < 0 || mf->ngrow > max_grow)
+        return fail(ctx, AMRESSIF_ERR_ARG, w + ": ngrow must be in [0, " + std::to_string(max_grow) + "]");
+    std::vector<unsigned char> cover((size_t)n * n, 0);
+    boxes.clear();
+    for (int b = 0; b < mf->nbox; ++b) {
+        const amr_array4 &a = mf->fab[b];
+        if (!a.ptr) return fail(ctx, AMRESSIF_ERR_ARG, w + ": null fab");
+        if (a.dtype != 0) return fail(ctx, AMRESSIF_ERR_ARG, w + ": double views only");
+        if (a.hi[2] < a.lo[2] || a.ncomp < ncomp) return fail(ctx, AMRESSIF_ERR_ARG, w + ": bad component/z extent");
+        CellBox cb{a.lo[0] + mf->ngrow, a.lo[1] + mf->ngrow, a.hi[0] - mf->ngrow, a.hi[1] - mf->ngrow};
+        if (kind == 0) cb.hi0 -= 1;  // the high nodal face column
+        if (kind == 1) cb.hi1 -= 1;
+        if (cb.lo0 < 0 || cb.lo1 < 0 || cb.hi0 >= n || cb.hi1 >= n || cb.hi0 < cb.lo0 || cb.hi1 < cb.lo1)
+            return fail(ctx, AMRESSIF_ERR_ARG, w + ": a valid box lies outside the domain");
+        for (int j = cb.lo1; j <= cb.hi1; ++j)
+            for (int i = cb.lo0; i <= cb.hi0; ++i)
+                if (cover[(size_t)i + (size_t)n * j]++)
+                    return fail(ctx, AMRESSIF_ERR_ARG, w + ": valid boxes overlap");
+        boxes.push_back(cb);
+    }
+    for (unsigned char x : cover)
+        if (!x) return fail(ctx, AMRESSIF_ERR_ARG, w + ": valid boxes do not cover the domain");
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int amressif_load_state_mf(amressif_ctx *ctx, const amr_multifab velCont[2], const amr_multifab velContPrev[2],
+                           const amr_multifab *userCtx)
+{
+    if (!ctx || !velCont || !velContPrev) return fail(ctx, AMRESSIF_ERR_ARG, "null argument");
+    Ctx &c = ctx->c;
+    // validate everything before the first copy: a failed load leaves the state as it was
+    std::vector<CellBox> bx[5];
+    for (int d = 0; d < 2; ++d) {
+        if (int rc = check_mf(ctx, &velCont[d], d, 1, 0, "velCont", bx[2 * d])) return rc;
+        if (int rc = check_mf(ctx, &velContPrev[d], d, 1, 0, "velContPrev", bx[2 * d + 1])) return rc;
+    }
+    if (userCtx)
+        if (int rc = check_mf(ctx, userCtx, 2, 2, 1 << 20, "userCtx", bx[4])) return rc;
+    for (int d = 0; d < 2; ++d)
+        for (int k = 0; k < 2; ++k) {
+            const amr_multifab &mf = k == 0 ? velCont[d] : velContPrev[d];
+            double *dst = k == 0 ? c.U[d] : c.Uprev[d];
+            for (int b = 0; b < mf.nbox; ++b) {
+                const CellBox &q = bx[2 * d + k][b];
+                HIPCHK(amr::launch_pack_window(c, mf.fab[b], 0, dst, q.lo0, q.lo1, q.hi0 + (d == 0), q.hi1 + (d == 1)));
+            }
+        }
+    if (userCtx) {
+        for (int b = 0; b < userCtx->nbox; ++b) {
+            const CellBox &q = bx[4][b];
+            HIPCHK(amr::launch_pack_window(c, userCtx->fab[b], 0, c.p, q.lo0, q.lo1, q.hi0, q.hi1));
+            HIPCHK(amr::launch_pack_window(c, userCtx->fab[b], 1, c.phi, q.lo0, q.lo1, q.hi0, q.hi1));
+        }
+    } else {
+        HIPCHK(amr::launch_fill(c, c.p, (int64_t)(c.g.n + 1) * c.g.P, 0.0));
+        HIPCHK(amr::launch_fill(c, c.phi, (int64_t)(c.g.n + 1) * c.g.P, 0.0));
+    }
+    HIPCHK(amr::launch_mg_start(c));  // phi_mean = 0 for the loaded phi
+    c.phih_valid = 0;                  // no solution history for the warm start
+    HIPCHK(hipStreamSynchronize(c.stream));
+    return AMRESSIF_OK;
+}
+
+int amressif_store_state_mf(const amressif_ctx *cctx, amr_multifab velCont[2], amr_multifab velContPrev[2],
+                            amr_multifab *userCtx, amr_multifab *velCart)
+{
+    amressif_ctx *ctx = const_cast<amressif_ctx *>(cctx);
+    if (!ctx) return AMRESSIF_ERR_ARG;
+    Ctx &c = ctx->c;
+    std::vector<CellBox> bx[4], cb;
+    for (int d = 0; d < 2; ++d) {
+        if (velCont)
+            if (int rc = check_mf(ctx, &velCont[d], d, 1, 0, "velCont", bx[2 * d])) return rc;
+        if (velContPrev)
+            if (int rc = check_mf(ctx, &velContPrev[d], d, 1, 0, "velContPrev", bx[2 * d + 1])) return rc;
+    }
+    if (userCtx)
+        if (int rc = check_mf(ctx, userCtx, 2, 2, 1, "userCtx", cb)) return rc;
+    if (velCart)
+        if (int rc = check_mf(ctx, velCart, 2, 2, 2, "velCart", cb)) return rc;
+    for (int d = 0; d < 2; ++d)
+        for (int k = 0; k < 2; ++k) {
+            amr_multifab *mf = k == 0 ? (velCont ? &velCont[d] : nullptr) : (velContPrev ? &velContPrev[d] : nullptr);
+            if (!mf) continue;
+            const double *src = k == 0 ? c.U[d] : c.Uprev[d];
+            for (int b = 0; b < mf->nbox; ++b) {
+                const CellBox &q = bx[2 * d + k][b];
+                HIPCHK(amr::launch_unpack_window(c, src, mf->fab[b], 0, q.lo0, q.lo1, q.hi0 + (d == 0),
+                                                 q.hi1 + (d == 1)));
+            }
+        }
+    // cell views are written whole, ghosts by global index: the neighbouring
+    // box's values inside the domain, the wall rules outside it
+    if (userCtx)
+        for (int b = 0; b < userCtx->nbox; ++b) {
+            HIPCHK(amr::launch_unpack_cell_shift(c, c.p, userCtx->fab[b], 0, 1, nullptr));
+            HIPCHK(amr::launch_unpack_cell_shift(c, c.phi, userCtx->fab[b], 1, 1, &c.ctl->phi_mean));
+        }
+    if (velCart)
+        for (int b = 0; b < velCart->nbox; ++b) HIPCHK(amr::launch_cont2cart_out(c, c.U[0], c.U[1], velCart->fab[b]));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    return AMRESSIF_OK;
+}
+
+}  // extern "C"
+
+int amr_fail(amressif_ctx *ctx, int code, const std::string &msg) { return fail(ctx, code, msg); }
+
+int amr_state_to_host(amressif_ctx *ctx, double *u, double *v, double *up, double *vp, double *p, double *phi)
+{
+    Ctx &c = ctx->c;
+    const size_t n = (size_t)c.g.n, P = (size_t)c.g.P;
+    struct Xfer {
+        double *h;
+        const double *d;
+        size_t w, rows;
+    } xs[] = {{u, c.U[0], n + 1, n}, {v, c.U[1], n, n + 1}, {up, c.Uprev[0], n + 1, n},
+              {vp, c.Uprev[1], n, n + 1}, {p, c.p, n, n}, {phi, c.phi, n, n}};
+    for (const Xfer &x : xs)
+        if (x.h)
+            HIPCHK(hipMemcpy2DAsync(x.h, x.w * 8, x.d, P * 8, x.w * 8, x.rows, hipMemcpyDeviceToHost, c.stream));
+    if (phi) HIPCHK(hipMemcpyAsync(c.ctl_host, c.ctl, sizeof(amr::Ctl), hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    if (phi) {
+        const double m = c.ctl_host->phi_mean;
+        for (size_t k = 0; k < n * n; ++k) phi[k] -= m;
+    }
+    return AMRESSIF_OK;
+}
+
+int amr_state_from_host(amressif_ctx *ctx, const double *u, const double *v, const double *up, const double *vp,
+                        const double *p, const double *phi)
+{
+    Ctx &c = ctx->c;
+    const size_t n = (size_t)c.g.n, P = (size_t)c.g.P;
+    struct Xfer {
+        const double *h;
+        double *d;
+        size_t w, rows;
+    } xs[] = {{u, c.U[0], n + 1, n}, {v, c.U[1], n, n + 1}, {up, c.Uprev[0], n + 1, n},
+              {vp, c.Uprev[1], n, n + 1}, {p, c.p, n, n}, {phi, c.phi, n, n}};
+    for (const Xfer &x : xs) {
+        if (x.h)
+            HIPCHK(hipMemcpy2DAsync(x.d, P * 8, x.h, x.w * 8, x.w * 8, x.rows, hipMemcpyHostToDevice, c.stream));
+        else
+            HIPCHK(amr::launch_fill(c, x.d, (int64_t)(n + 1) * (int64_t)P, 0.0));
+    }
+    HIPCHK(amr::launch_mg_start(c));  // phi_mean = 0 for the loaded phi
+    c.phih_valid = 0;                  // no solution history for the warm start
+    HIPCHK(hipStreamSynchronize(c.stream));
+    return AMRESSIF_OK;
+}
+
+extern "C" {
 
 int amressif_advance(amressif_ctx *ctx, int nsteps, amressif_step_stats *stats)
 {
@@ -1389,10 +1563,10 @@ int amressif_time_kernel(amressif_ctx *ctx, int which, int reps, int64_t copy_by
     HIPCHK(hipEventSynchronize(e1));
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-    if (src) hipFree(src);
-    if (dst) hipFree(dst);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (src) (void)hipFree(src);
+    if (dst) (void)hipFree(dst);
     *avg_ms = (double)ms / reps;
     if (bytes_per_launch) {
         // algorithmic bytes per cell (t = 4 or 8: the V-cycle precision):
@@ -1510,8 +1684,8 @@ int amressif_tape_create(amressif_ctx *ctx, int max_steps, amressif_tape **out)
 int amressif_tape_destroy(amressif_tape *tape)
 {
     if (!tape) return AMRESSIF_OK;
-    hipStreamSynchronize(tape->stream);
-    if (tape->store) hipFree(tape->store);
+    (void)hipStreamSynchronize(tape->stream);
+    if (tape->store) (void)hipFree(tape->store);
     delete tape;
     return AMRESSIF_OK;
 }

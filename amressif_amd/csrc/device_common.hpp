@@ -87,19 +87,26 @@ __device__ __forceinline__ void for_records(const double *p, int nb, F &&f)
     }
 }
 
+// Workgroup b of a one-dimensional grid of B runs on XCD b mod 8.  The
+// logical block number hands each XCD a contiguous range: the first B mod 8
+// XCDs take B / 8 + 1 blocks, the others B / 8, in order.  Tiles a stencil
+// reads again (the neighbours of a tile) then mostly sit on the same XCD and
+// its L2.
+__device__ __forceinline__ int xcd_block(int b, int B)
+{
+    const int q = B >> 3, r = B & 7, x = b & 7;
+    return x * q + (x < r ? x : r) + (b >> 3);
+}
+
 // Cell tiles of 64 x 16 for a workgroup of 256 threads (four passes of 64 x 4)
-// on a one-dimensional grid whose size is a multiple of 8.  Workgroup b runs
-// on XCD b mod 8; the map hands each XCD a contiguous range of tiles in
-// row-major order, so the tiles above and below a tile, whose rows a stencil
-// reads again, are mostly on the same XCD and its L2.
+// in row-major order over the XCD-contiguous block numbers.
 struct CellTile {
     int i0, j0;
     bool live;  // false: a padding workgroup past the last tile
 };
 __device__ inline CellTile cell_tile(int ntx, int nty)
 {
-    const int B = gridDim.x;
-    const int blk = (blockIdx.x & 7) * (B >> 3) + (blockIdx.x >> 3);
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
     CellTile t;
     t.i0 = (blk % ntx) * 64;
     t.j0 = (blk / ntx) * 16;

@@ -251,6 +251,11 @@ hipError_t launch_pack_face(const Ctx &c, const amr_array4 &a, double *dst, int 
 hipError_t launch_unpack_face(const Ctx &c, const double *src, const amr_array4 &a, int dir);
 hipError_t launch_pack_cell(const Ctx &c, const amr_array4 &a, int comp, double *dst);
 hipError_t launch_unpack_cell(const Ctx &c, const double *src, const amr_array4 &a, int comp, int fill_ghosts);
+// window [i0, i1] x [j0, j1] (global indices, inclusive) of one box view <-> pitched array
+hipError_t launch_pack_window(const Ctx &c, const amr_array4 &a, int comp, double *dst, int i0, int j0, int i1,
+                              int j1);
+hipError_t launch_unpack_window(const Ctx &c, const double *src, const amr_array4 &a, int comp, int i0, int j0,
+                                int i1, int j1);
 hipError_t launch_copy_faces(const Ctx &c, const double *src, double *dst, int dir);
 hipError_t launch_fill(const Ctx &c, double *dst, int64_t count, double value);
 hipError_t launch_norm(const Ctx &c, const double *ub, const double *vb, const double *uo, const double *vo,
@@ -355,3 +360,13 @@ int amr_momentum_begin(amressif_ctx *ctx, bool bdf_ready, int *launched, bool re
 int amr_momentum_finish(amressif_ctx *ctx, int launched);
 int amr_poll(amressif_ctx *ctx);
 int amr_set_ctl(amressif_ctx *ctx);
+
+// host copies of the state in the global one-box layout (row-major, i
+// fastest): x faces n rows of n + 1, y faces n + 1 rows of n, cells n x n.
+// state_to_host: any pointer may be null; phi is returned mean-removed as
+// amressif_store_state gives it.  state_from_host: phi may be null (0), then
+// finishes as amressif_load_state does.  Both synchronise the stream.
+int amr_state_to_host(amressif_ctx *ctx, double *u, double *v, double *up, double *vp, double *p, double *phi);
+int amr_state_from_host(amressif_ctx *ctx, const double *u, const double *v, const double *up, const double *vp,
+                        const double *p, const double *phi);
+int amr_fail(amressif_ctx *ctx, int code, const std::string &msg);

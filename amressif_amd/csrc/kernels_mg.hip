@@ -259,7 +259,7 @@ __global__ __launch_bounds__(256) void k_gsrb(const T *xin, const T *ec, int pc,
     constexpr int OC = 64 - 2 * H, NR = R + 2 * H;
     const int lane = threadIdx.x & 63;
     const int nblk = gridDim.x;
-    const int blk = (nblk & 7) == 0 ? (blockIdx.x & 7) * (nblk >> 3) + (blockIdx.x >> 3) : blockIdx.x;
+    const int blk = xcd_block(blockIdx.x, nblk);
     const int w = blk * 4 + (threadIdx.x >> 6);
     // a wave past the last one exits at once; in MODE 2 it stays for the
     // workgroup's reduction (the only barrier) and touches no memory
@@ -462,7 +462,7 @@ __global__ __launch_bounds__(256) void k_gsrb2(const T *xin, const T *ec, int pc
     constexpr int OC = 128 - 2 * HX, NR = R + 2 * H;
     const int lane = threadIdx.x & 63;
     const int nblk = gridDim.x;
-    const int blk = (nblk & 7) == 0 ? (blockIdx.x & 7) * (nblk >> 3) + (blockIdx.x >> 3) : blockIdx.x;
+    const int blk = xcd_block(blockIdx.x, nblk);
     const int w = blk * 4 + (threadIdx.x >> 6);
     // a wave past the last one exits at once; in MODE 2 it stays for the
     // workgroup's reduction (the only barrier) and touches no memory
@@ -1337,8 +1337,7 @@ __global__ __launch_bounds__(256) void k_poisson_start_rows(double *b, const dou
     __shared__ double sh[4 * kStartParts];
     if (ctl->mg_done) return;  // a skipped phase (k_mg_start)
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int B = gridDim.x;  // a multiple of 8
-    const int blk = (blockIdx.x & 7) * (B >> 3) + (blockIdx.x >> 3);
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int w = blk * 4 + wv;
     const bool live = w < nwaves;  // a wave past the last one only joins the reduction
     const int band = live ? w % nbands : 0, chunk = live ? w / nbands : 0;
@@ -1653,7 +1652,7 @@ static SmoothShapeRt smooth_shape(const Ctx &c, int l, int mode)
     }
     s.nbands = (L.nx + OC - 1) / OC;
     s.nwaves = s.nbands * ((L.ny + s.rows - 1) / s.rows);
-    s.grid = dim3(((s.nwaves + 3) / 4 + 7) / 8 * 8);  // a multiple of 8: the XCD remap applies
+    s.grid = dim3((s.nwaves + 3) / 4);
     s.parts = (int)s.grid.x;
     return s;
 }

@@ -55,7 +55,7 @@ struct StageArgs {
     int bc_lo0, bc_hi0, bc_lo1, bc_hi1;
     double inflow0, inflow1;
     int mode;  // k_stage64: 0 stage, 1 last stage (+ norm), 2 RHS only, 3 mixed residual
-    int nbands, nchunks, remap;
+    int nbands, nchunks;
     int rows;   // output rows per wave (chunk height)
     int order;  // wave -> tile: 0 band-major (chunks of one band consecutive), 1 chunk-major
     int alt;    // k_stage64: odd chunks march down (shared halo rows read at the same time)
@@ -63,14 +63,9 @@ struct StageArgs {
 
 // wave id -> (band, chunk); blocks of 4 waves, handed to the 8 XCDs in
 // contiguous ranges so vertically adjacent chunks share an L2
-__device__ __forceinline__ int wave_id(const StageArgs &a)
+__device__ __forceinline__ int wave_id(const StageArgs &)
 {
-    int blk = blockIdx.x;
-    if (a.remap) {
-        const int B = gridDim.x;
-        blk = (blk & 7) * (B >> 3) + (blk >> 3);
-    }
-    return blk * 4 + (threadIdx.x >> 6);
+    return xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
 }
 
 // cell velocities of row jr for this lane's column from raw face loads of the
@@ -573,9 +568,7 @@ static StageArgs base_args(const Ctx &c, double rk, int mode, int rows)
 
 static dim3 stage_grid(StageArgs &a)
 {
-    const int blocks = (a.nbands * a.nchunks + 3) / 4;
-    a.remap = blocks % 8 == 0 ? 1 : 0;
-    return dim3(blocks);
+    return dim3((a.nbands * a.nchunks + 3) / 4);
 }
 
 

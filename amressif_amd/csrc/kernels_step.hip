@@ -389,8 +389,7 @@ __global__ __launch_bounds__(256) void k_project_rows(const double *us, const do
     __shared__ double sh[4];
     if (!proj_gate(gate, ctl, phi)) return;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int B = gridDim.x;  // a multiple of 8
-    const int blk = (blockIdx.x & 7) * (B >> 3) + (blockIdx.x >> 3);
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int w = blk * 4 + wv;
     const bool live = w < nwaves;  // a wave past the last one only joins the reduction
     const int band = live ? w % nbands : 0, chunk = live ? w / nbands : 0;
@@ -632,6 +631,46 @@ __global__ __launch_bounds__(256) void k_unpack_cell_ghosts(A4 a, int comp, cons
     if (mi < 0 || mi >= n || mj < 0 || mj >= n) return;
     const double s = shift ? *shift : 0.0;
     a.ptr[(i - a.lo0) + a.js * (j - a.lo1) + a.ns * comp] = src[mi + (size_t)P * mj] - s;
+}
+
+// one box of a multi-box MultiFab: the window [i0, i0 + wx) x [j0, j0 + wy)
+// (global indices, inside the view) between the view and the pitched array
+__global__ __launch_bounds__(256) void k_pack_win(A4 a, int comp, double *dst, int i0, int j0, int wx, int wy,
+                                                  int P)
+{
+    const int di = blockIdx.x * 64 + (threadIdx.x & 63), dj = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (di >= wx || dj >= wy) return;
+    const int i = i0 + di, j = j0 + dj;
+    dst[i + (size_t)P * j] = a.ptr[(i - a.lo0) + a.js * (j - a.lo1) + a.ns * comp];
+}
+
+__global__ __launch_bounds__(256) void k_unpack_win(A4 a, int comp, const double *src, int i0, int j0, int wx,
+                                                    int wy, int P)
+{
+    const int di = blockIdx.x * 64 + (threadIdx.x & 63), dj = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (di >= wx || dj >= wy) return;
+    const int i = i0 + di, j = j0 + dj;
+    a.ptr[(i - a.lo0) + a.js * (j - a.lo1) + a.ns * comp] = src[i + (size_t)P * j];
+}
+
+hipError_t launch_pack_window(const Ctx &c, const amr_array4 &a, int comp, double *dst, int i0, int j0, int i1,
+                              int j1)
+{
+    const int wx = i1 - i0 + 1, wy = j1 - j0 + 1;
+    if (wx <= 0 || wy <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pack_win, dim3((wx + 63) / 64, (wy + 3) / 4), dim3(256), 0, c.stream, make_a4(a), comp,
+                       dst, i0, j0, wx, wy, c.g.P);
+    return hipGetLastError();
+}
+
+hipError_t launch_unpack_window(const Ctx &c, const double *src, const amr_array4 &a, int comp, int i0, int j0,
+                                int i1, int j1)
+{
+    const int wx = i1 - i0 + 1, wy = j1 - j0 + 1;
+    if (wx <= 0 || wy <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_unpack_win, dim3((wx + 63) / 64, (wy + 3) / 4), dim3(256), 0, c.stream, make_a4(a), comp,
+                       src, i0, j0, wx, wy, c.g.P);
+    return hipGetLastError();
 }
 
 hipError_t launch_pack_face(const Ctx &c, const amr_array4 &a, double *dst, int dir)

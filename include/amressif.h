@@ -151,9 +151,60 @@ int amressif_load_state(amressif_ctx *ctx, const amr_array4 velCont[2], const am
  * Neumann mirror (main.cpp:1078-1158). */
 int amressif_store_state(const amressif_ctx *ctx, amr_array4 velCont[2], amr_array4 velContPrev[2],
                          amr_array4 *userCtx, amr_array4 *velCart);
+/* A MultiFab of several boxes: one Array4 per box (what MultiFab::array(mfi)
+ * gives for each MFIter box), each grown by ngrow cells around its valid box.
+ * The valid cell boxes must tile the domain (BoxArray::maxSize chopping,
+ * main.cpp:317-324).  Face MultiFabs are nodal in their direction
+ * (surroundingNodes, main.cpp:333-341): a box's valid faces include both
+ * bounding faces, so neighbouring boxes share one face column or row. */
+typedef struct {
+    int nbox;
+    const amr_array4 *fab; /* nbox views (for a store the memory is written through them) */
+    int ngrow;
+} amr_multifab;
+
+/* amressif_load_state over several boxes: each box's valid region is
+ * gathered into the context's one-box working set (a shared face is read
+ * from the later box; a consistent MultiFab holds the same value in both).
+ * userCtx may be NULL (p = phi = 0). */
+int amressif_load_state_mf(amressif_ctx *ctx, const amr_multifab velCont[2], const amr_multifab velContPrev[2],
+                           const amr_multifab *userCtx);
+/* amressif_store_state over several boxes: every element of every view is
+ * written.  Face MultiFabs have ngrow = 0 (main.cpp:337-338); userCtx
+ * ngrow <= 1 and velCart (cont2cart with the wall rules) ngrow <= 2: their
+ * ghosts inside the domain get the neighbouring box's values
+ * (FillBoundary), ghosts outside it the reference's wall rules, as
+ * amressif_store_state writes them.  Any pointer may be NULL. */
+int amressif_store_state_mf(const amressif_ctx *ctx, amr_multifab velCont[2], amr_multifab velContPrev[2],
+                            amr_multifab *userCtx, amr_multifab *velCart);
 /* Advance nsteps time steps (main.cpp:875-1202 per step).  stats: host array
  * of nsteps entries or NULL.  Synchronises the stream. */
 int amressif_advance(amressif_ctx *ctx, int nsteps, amressif_step_stats *stats);
+
+/* ------------------------------------------------ initial state and files */
+/* hybrid_grid_init (fn_init.cpp:30-631, kn_init.H:8-37) for the cavity: all
+ * faces 0, velContPrev = velCont, p = phi = 0.  With psi_coeffs (host array
+ * a[kx][ky], kmax x kmax, row-major) the faces instead come from the nodal
+ * streamfunction psi = sum a[kx][ky] sin(kx pi x) sin(ky pi y) sin^2(pi x)
+ * sin^2(pi y) (x = i/n, y = j/n, 0 on the boundary nodes):
+ * u = (psi(i,j+1) - psi(i,j))/dy, v = -(psi(i+1,j) - psi(i,j))/dx, which is
+ * discretely divergence-free with zero wall-normal faces (no reference
+ * counterpart: the reference only starts from rest).  Synchronises. */
+int amressif_init_state(amressif_ctx *ctx, const double *psi_coeffs, int kmax);
+/* SaveCheckpoint (utilities.cpp:792-900): writes <dir>/checkpointNNNNN/Header
+ * (title, time, BoxArray of the inputs' max_grid_size, main.cpp:317-320) and
+ * Level_0/{pressure, vel_xCont, vel_yCont, vel_xContPrev, vel_yContPrev} as
+ * AMReX VisMF MultiFabs (_H header + _D_00000 data).  NNNNN = step. */
+int amressif_write_checkpoint(amressif_ctx *ctx, const char *dir, int step, double time);
+/* LoadCheckpoint (utilities.cpp:904-1022) and the restart re-derivation
+ * (main.cpp:386-800): reads the checkpoint directory `path` into the state
+ * (velCont, velContPrev, p; phi, which is not saved, = 0) and returns its time.
+ * Any BoxArray that tiles the domain is accepted. */
+int amressif_read_checkpoint(amressif_ctx *ctx, const char *path, double *time);
+/* Export_Flow_Field (utilities.cpp:534-570): WriteSingleLevelPlotfile of
+ * (pressure, U, V) into <dir>/<name>NNNNN (HyperCLaw-V1.1 Header and
+ * Level_0/Cell VisMF data).  U, V are the cell averages of the faces. */
+int amressif_write_plotfile(amressif_ctx *ctx, const char *dir, const char *name, int step, double time);
 
 /* ------------------------------------------- module-level entry points
  * Each runs the same kernels as amressif_advance on one call's inputs (Array4

@@ -5,6 +5,7 @@ import ctypes
 import os
 import re
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -79,3 +80,18 @@ def test_create_rejects_codes_the_reference_aborts_on(api):
     assert rc == -3 and "periodic" in msg
     assert _create_status(api, PSEUDO_TIMESTEPPING=0)[0] == -3
     assert _create_status(api, precision=7)[0] == -1
+
+
+def test_box_array_mirrors_max_size_chopping(api):
+    # the reference's Exec/inputs: n_cell 32, max_grid_size_x 32, max_grid_size_y 16
+    assert api.box_array(32, (32, 16)) == [(0, 0, 31, 15), (0, 16, 31, 31)]
+    # uneven cuts: the first pieces are one cell longer
+    assert api.box_array(10, (4, 10)) == [(0, 0, 3, 9), (4, 0, 6, 9), (7, 0, 9, 9)]
+    for n, m in ((48, (20, 32)), (100, 32), (64, 64)):
+        boxes = api.box_array(n, m)
+        cover = np.zeros((n, n), dtype=int)
+        for x0, y0, x1, y1 in boxes:
+            cover[y0:y1 + 1, x0:x1 + 1] += 1
+            mx, my = (m, m) if isinstance(m, int) else m
+            assert x1 - x0 + 1 <= mx and y1 - y0 + 1 <= my
+        assert np.all(cover == 1)

@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -31,13 +32,18 @@ struct amressif_amr {
     hipStream_t stream = nullptr;
     double dt = 0.0;
     int mixed = 0, m = 30, last_P = 4;
+    int host_gmres = 0;  // AMRESSIF_AMR_HOST_GMRES=1: the Arnoldi process on the host (for A/B runs)
     // fine level (faces pitch P1, cells pitch P1, ghosted cells pitch Pg1)
     double *U1[2]{}, *Up1[2]{}, *C1[2]{}, *F1[2]{}, *F2[2]{}, *p1 = nullptr, *XU = nullptr, *XV = nullptr;
     amr::Ctl *ctlF = nullptr, *ctlF_host = nullptr;
     double *part = nullptr, *red = nullptr, *red_host = nullptr, *coef = nullptr;
     // composite vectors [coarse segment | fine segment]
     size_t off1 = 0, vs = 0;
-    double *V = nullptr, *Z = nullptr, *x = nullptr, *b = nullptr, *r = nullptr;
+    double *V = nullptr, *Z = nullptr, *x = nullptr, *b = nullptr, *r = nullptr, *vcur = nullptr, *zcur = nullptr;
+    amr::GmState *gst = nullptr;  // the composite FGMRES's Arnoldi state (device)
+    int last_gm = 8;              // Arnoldi steps of the previous solve (first batch size)
+    double *dstat = nullptr;      // per step: composite ||D(u*)||inf, ||D(u^{n+1})||inf (device)
+    int dstat_cap = 0;
     std::vector<void *> allocs;
     std::string err;
 };
@@ -138,8 +144,53 @@ int precondition(amressif_amr *A, double *v, double *z)
     return 0;
 }
 
-// FGMRES(m) on L_c x = b (b centred); returns iterations and ||r||inf/||b||inf
+// FGMRES(m) on L_c x = b (b centred) with the Arnoldi process on the device
+// (kernels_amr.hip): batches of Arnoldi steps, each followed by a cycle edge,
+// and one host wait per batch.  The coarse context's control block is the
+// solve's: its V-cycle kernels honour the same mg_done / gm_stop flags.
+// Returns the steps taken and ||r||inf / ||b||inf.
 int composite_solve(amressif_amr *A, int *iters, double *res_rel)
+{
+    const amr::AmrGeom &a = A->a;
+    Ctx &cc = A->coarse->c;
+    hipStream_t s = A->stream;
+    const amr::CgmBufs g{cc.ctl, A->gst, A->V, A->Z, A->x, A->b, A->vcur, A->zcur, A->part, A->vs, A->off1, A->m};
+    amr::Level &L0 = cc.lev[0];
+    void *rc = A->mixed ? (void *)L0.r32 : (void *)L0.r;
+    const void *e = A->mixed ? (const void *)L0.e32b : (const void *)L0.e2;
+    ACHK(amr::launch_fill(cc, A->x, (int64_t)A->vs, 0.0));
+    ACHK(amr::amr_cgm_begin(s, cc.ctl, A->gst));
+    ACHK(amr::amr_cgm_edge(a, s, g));  // r = b, ||b||inf, the first cycle's start
+    int target = std::max(1, A->last_gm);
+    for (;;) {
+        for (int k = 0; k < target; ++k) {
+            // z = M V_j: coarse V-cycle on the restricted V_j, prolonged, two
+            // red-black Gauss-Seidel sweeps on the patch
+            ACHK(amr::amr_restrict(a, s, A->vcur, A->vcur + A->off1, rc, A->mixed, cc.ctl));
+            ACHK(amr::launch_vcycle(cc, A->mixed));
+            ACHK(amr::amr_prolong(a, s, e, A->zcur, A->zcur + A->off1, A->mixed, cc.ctl));
+            for (int sweep = 0; sweep < 2; ++sweep)
+                for (int color = 0; color < 2; ++color)
+                    ACHK(amr::amr_fine_gs(a, s, A->zcur, A->zcur + A->off1, A->vcur + A->off1, color, cc.ctl));
+            ACHK(amr::amr_cgm_apply(a, s, g));
+            ACHK(amr::amr_cgm_orthonormalise(a, s, g));
+        }
+        ACHK(amr::amr_cgm_edge(a, s, g));
+        if (int rc2 = amr_poll(A->coarse)) return afail(A, rc2, amressif_last_error(A->coarse));
+        if (cc.ctl_host->mg_done) break;
+        target = 2;
+    }
+    A->last_gm = std::max(1, cc.ctl_host->mg_iters);
+    if (cc.ctl_host->flags & AMRESSIF_FLAG_NONFINITE)
+        return afail(A, AMRESSIF_ERR_NONFINITE, "composite Poisson solve diverged");
+    *iters = cc.ctl_host->mg_iters;
+    *res_rel = cc.ctl_host->b_norm > 0.0 ? cc.ctl_host->res_norm / cc.ctl_host->b_norm : 0.0;
+    return 0;
+}
+
+// The same solve with the Arnoldi process on the host (one wait per
+// dot-product pass); kept for the A/B comparison (AMRESSIF_AMR_HOST_GMRES=1).
+int composite_solve_host(amressif_amr *A, int *iters, double *res_rel)
 {
     const amr::AmrGeom &a = A->a;
     Ctx &cc = A->coarse->c;
@@ -246,6 +297,8 @@ int amressif_amr_create(const amressif_inputs *in, int patch_jlo, void *hip_stre
     A->stream = (hipStream_t)hip_stream;
     const int n0 = in->n_cell;
     if (patch_jlo < 2 || patch_jlo >= n0 - 1) return afail(A, AMRESSIF_ERR_ARG, "patch_jlo must be in [2, n_cell - 1)");
+    if (in->PSEUDO_TIMESTEPPING != 1)
+        return afail(A, AMRESSIF_ERR_UNSUPPORTED, "the two-level step uses the dual-time predictor (PSEUDO_TIMESTEPPING = 1)");
     // one dt for both levels, from the fine spacing (main.cpp:284-294 on the finest level)
     amressif_inputs ci = *in;
     const double h0 = (in->hi_phy_dim[0] - in->lo_phy_dim[0]) / n0, h1 = 0.5 * h0;
@@ -256,6 +309,7 @@ int amressif_amr_create(const amressif_inputs *in, int patch_jlo, void *hip_stre
     if (int rc = amressif_create(&ci, hip_stream, &A->coarse)) return afail(A, rc, amressif_last_error(A->coarse));
     Ctx &cc = A->coarse->c;
     A->mixed = in->precision == AMRESSIF_MIXED;
+    if (const char *ev = std::getenv("AMRESSIF_AMR_HOST_GMRES")) A->host_gmres = std::atoi(ev) ? 1 : 0;
     amr::AmrGeom &a = A->a;
     a.n0 = n0;
     a.P0 = cc.g.P;
@@ -297,6 +351,9 @@ int amressif_amr_create(const amressif_inputs *in, int patch_jlo, void *hip_stre
     AALLOC(&A->x, A->vs);
     AALLOC(&A->b, A->vs);
     AALLOC(&A->r, A->vs);
+    AALLOC(&A->vcur, A->vs);
+    AALLOC(&A->zcur, A->vs);
+    AALLOC(&A->gst, 1);
 #undef AALLOC
     ACHK(hipStreamSynchronize(A->stream));
     return AMRESSIF_OK;
@@ -385,6 +442,10 @@ int amressif_amr_advance(amressif_amr *A, int nsteps, amressif_amr_stats *stats)
     const amr::AmrGeom &a = A->a;
     hipStream_t s = A->stream;
     if (int rc = amr_set_ctl(A->coarse)) return afail(A, rc, amressif_last_error(A->coarse));
+    if (A->dstat_cap < nsteps) {
+        ACHK(aalloc(A, &A->dstat, (size_t)2 * nsteps));
+        A->dstat_cap = nsteps;
+    }
     for (int step = 0; step < nsteps; ++step) {
         amressif_amr_stats st{};
         // 1. coarse predictor over the whole coarse domain
@@ -398,28 +459,35 @@ int amressif_amr_advance(amressif_amr *A, int nsteps, amressif_amr_stats *stats)
         st.flags |= ff;
         // 3. coarse faces under the patch from the fine predictor
         ACHK(amr::amr_average_down(a, s, A->U1[0], A->U1[1], A->p1, cc.U[0], cc.U[1], cc.p));
-        // 4. composite projection
-        double dstar;
+        // 4. composite projection (the divergence norms stay on the device
+        // until the call's end)
+        const int nblk = amr::amr_flat_blocks(a);
         ACHK(amr::amr_rhs(a, s, A->dt, cc.U[0], cc.U[1], A->U1[0], A->U1[1], A->b, A->b + A->off1, A->part));
-        if (int rc = reduce(A, 1, 1, &dstar)) return rc;
-        st.div_star = dstar;
+        ACHK(amr::launch_reduce_cols(s, A->part, nblk, 1, 1, A->dstat + 2 * step));
         if (int rc = center(A, A->b)) return rc;
         int it = 0;
         double res = 0.0;
-        if (int rc = composite_solve(A, &it, &res)) return rc;
+        if (int rc = A->host_gmres ? composite_solve_host(A, &it, &res) : composite_solve(A, &it, &res)) return rc;
         st.poisson_iters = it;
         st.poisson_res = res;
         if (int rc = center(A, A->x)) return rc;
         ACHK(amr::amr_project(a, s, A->dt, A->x, A->x + A->off1, cc.U[0], cc.U[1], A->U1[0], A->U1[1], cc.p, A->p1));
         // 5. average down and the composite divergence of the result
         ACHK(amr::amr_average_down(a, s, A->U1[0], A->U1[1], A->p1, cc.U[0], cc.U[1], cc.p));
-        double dnew;
         ACHK(amr::amr_rhs(a, s, A->dt, cc.U[0], cc.U[1], A->U1[0], A->U1[1], A->r, A->r + A->off1, A->part));
-        if (int rc = reduce(A, 1, 1, &dnew)) return rc;
-        st.div_new = dnew;
+        ACHK(amr::launch_reduce_cols(s, A->part, nblk, 1, 1, A->dstat + 2 * step + 1));
         if (stats) stats[step] = st;
         if (st.flags & AMRESSIF_FLAG_NONFINITE)
             return afail(A, AMRESSIF_ERR_NONFINITE, "non-finite pseudo-iteration norm at step " + std::to_string(step));
+    }
+    if (stats && nsteps > 0) {
+        std::vector<double> d((size_t)2 * nsteps);
+        ACHK(hipMemcpyAsync(d.data(), A->dstat, sizeof(double) * d.size(), hipMemcpyDeviceToHost, s));
+        ACHK(hipStreamSynchronize(s));
+        for (int step = 0; step < nsteps; ++step) {
+            stats[step].div_star = d[2 * step];
+            stats[step].div_new = d[2 * step + 1];
+        }
     }
     ACHK(hipStreamSynchronize(s));
     return AMRESSIF_OK;

@@ -408,8 +408,28 @@ int amr_momentum_begin(amressif_ctx *ctx, bool bdf_ready, int *launched, bool re
         HIPCHK(amr::launch_fold_gradp(c));
         c.c_gradp = 1;
     }
-    const int target = std::min(std::max(1, c.last_P), c.in.IterNum + 1);
     int nb = 0;
+    if (c.in.PSEUDO_TIMESTEPPING == 0) {
+        // main.cpp:900-926, momentum.cpp:96-191: one explicit update from u^n,
+        // u* = u^n + (dt/1.5)(R(u^n) - G p + (u^n - u^{n-1})/(2 dt)); in the form of
+        // a stage with rk = dt/1.5 whose stencil input and base are both u^n
+        amr::StageLaunch a{};
+        a.us = a.ub = c.U[0];
+        a.vs = a.vb = c.U[1];
+        a.cu = c.C[0]; a.cv = c.C[1]; a.p = c.p;
+        a.uo = c.S1[0]; a.vo = c.S1[1];
+        a.rk = c.g.dt / 1.5;
+        a.mode = 1;
+        a.foldp = 1;
+        a.expl = 1;
+        HIPCHK(amr::launch_stage(c, a, &nb));
+        HIPCHK(amr::launch_fin_explicit(c, nb));
+        std::swap(c.U[0], c.S1[0]);
+        std::swap(c.U[1], c.S1[1]);
+        *launched = 1;
+        return 0;
+    }
+    const int target = std::min(std::max(1, c.last_P), c.in.IterNum + 1);
     for (*launched = 0; *launched < target; ++*launched) {
         if (int rc = launch_pseudo_iteration(ctx, *launched, &nb)) return rc;
         HIPCHK(amr::launch_fin_pseudo(c, nb));
@@ -765,8 +785,8 @@ int amressif_create(const amressif_inputs *in, void *hip_stream, amressif_ctx **
     if (n < 4 || n > 16384) return bail(AMRESSIF_ERR_ARG, "n_cell must be in [4, 16384]");
     if (in->ren <= 0) return bail(AMRESSIF_ERR_ARG, "ren must be > 0");
     if (in->IterNum < 0) return bail(AMRESSIF_ERR_ARG, "IterNum must be >= 0");
-    if (in->PSEUDO_TIMESTEPPING != 1)
-        return bail(AMRESSIF_ERR_UNSUPPORTED, "only PSEUDO_TIMESTEPPING = 1 (the dual-time RK4 path) is built");
+    if (in->PSEUDO_TIMESTEPPING != 0 && in->PSEUDO_TIMESTEPPING != 1)
+        return bail(AMRESSIF_ERR_ARG, "PSEUDO_TIMESTEPPING must be 0 (explicit) or 1 (dual-time RK4)");
     if (in->precision != AMRESSIF_FP64 && in->precision != AMRESSIF_MIXED)
         return bail(AMRESSIF_ERR_ARG, "precision must be 0 (FP64) or 1 (mixed)");
     if (in->mg_bottom_max < 2 || in->mg_bottom_max > 32) return bail(AMRESSIF_ERR_ARG, "mg_bottom_max must be in [2, 32]");
@@ -842,7 +862,37 @@ int amressif_create(const amressif_inputs *in, void *hip_stream, amressif_ctx **
         c.nlev = l;
         const amr::Level &B = c.lev[l - 1];
         c.mb = B.nx * B.ny;
-        if (c.mb > 1024) return bail(AMRESSIF_ERR_UNSUPPORTED, "coarsest multigrid level exceeds 1024 cells");
+        if (c.mb > 1024) {
+            // an odd coarsest level too large for the dense pseudo-inverse: its
+            // exact DCT solve as four dense products (kernels_mg.hip)
+            if (B.nx > 2048 || B.ny > 2048)
+                return bail(AMRESSIF_ERR_UNSUPPORTED, "coarsest multigrid level wider than 2048 cells (n_cell = m 2^k "
+                                                      "with an odd m > 2048)");
+            const int bx = B.nx, by = B.ny;
+            const double pi = 3.14159265358979323846;
+            std::vector<double> tab((size_t)bx * bx + (size_t)by * by + (size_t)by * bx);
+            auto fill = [&](double *Cm, int m) {
+                for (int i = 0; i < m; ++i)
+                    for (int k = 0; k < m; ++k)
+                        Cm[(size_t)i * m + k] = std::sqrt((k == 0 ? 1.0 : 2.0) / m) * std::cos(pi * k * (2 * i + 1) / (2.0 * m));
+            };
+            fill(tab.data(), bx);
+            fill(tab.data() + (size_t)bx * bx, by);
+            double *Dinv = tab.data() + (size_t)bx * bx + (size_t)by * by;
+            for (int mm = 0; mm < by; ++mm)
+                for (int k = 0; k < bx; ++k) {
+                    const double lam = -B.ax * (2.0 - 2.0 * std::cos(pi * k / bx)) - B.ay * (2.0 - 2.0 * std::cos(pi * mm / by));
+                    Dinv[(size_t)mm * bx + k] = (k == 0 && mm == 0) ? 0.0 : 1.0 / lam;
+                }
+            std::vector<float> tab32(tab.begin(), tab.end());
+            ALLOC(&c.big_tab, tab.size()); ALLOC(&c.big_tab32, tab.size());
+            ALLOC(&c.big_work, (size_t)2 * bx * by); ALLOC(&c.big_work32, (size_t)2 * bx * by);
+            hipError_t e1 = hipMemcpy(c.big_tab, tab.data(), tab.size() * 8, hipMemcpyHostToDevice);
+            hipError_t e2 = hipMemcpy(c.big_tab32, tab32.data(), tab32.size() * 4, hipMemcpyHostToDevice);
+            if (e1 != hipSuccess || e2 != hipSuccess) return bail(AMRESSIF_ERR_HIP, "bottom DCT table upload failed");
+            c.tail_level = l - 1;
+            c.tail_ext = 0;
+        } else {
         std::vector<double> pinv;
         if (!neumann_pinv(B.nx, B.ny, B.ax, B.ay, pinv)) return bail(AMRESSIF_ERR_ARG, "bottom factorisation failed");
         std::vector<float> pinv32(pinv.begin(), pinv.end());
@@ -881,6 +931,7 @@ int amressif_create(const amressif_inputs *in, void *hip_stream, amressif_ctx **
                          L1->ny == 2 * T.ny && L1->nx <= 128 && (L1->ny == 32 || L1->ny == 64 || L1->ny == 128);
         } else {
             c.tail_ext = 0;
+        }
         }
     }
     if (in->poisson_solver == 1) {  // FGMRES: m+1 Arnoldi vectors, m preconditioned vectors
@@ -1605,11 +1656,45 @@ int ensure_adjoint(amressif_ctx *ctx)
         {&w->XU, gsz}, {&w->XV, gsz}, {&w->DU, gsz}, {&w->DV, gsz}, {&w->F[0], fsz}, {&w->F[1], fsz},
         {&w->F[2], fsz}, {&w->F[3], fsz}, {&w->Q[0], fsz}, {&w->Q[1], fsz}, {&w->XbU, fsz}, {&w->XbV, fsz},
         {&w->lidg, (size_t)2 * n}, {&w->L[0][0], fsz}, {&w->L[0][1], fsz}, {&w->L[1][0], fsz}, {&w->L[1][1], fsz},
-        {&c.Du[0], fsz}, {&c.Du[1], fsz}, {&c.Dup[0], fsz}, {&c.Dup[1], fsz}, {&c.Dp, fsz}, {&c.Dlid, (size_t)n}};
+        {&c.Du[0], fsz}, {&c.Du[1], fsz}, {&c.Dup[0], fsz}, {&c.Dup[1], fsz}, {&c.Dp, fsz}, {&c.Dlid, (size_t)n},
+        {&w->phi_lin[0], fsz}, {&w->phi_lin[1], fsz}, {&w->phi_mean_save, (size_t)1}};
     for (auto &b : bufs) {
         HIPCHK(amr::dev_alloc_public(c, b.first, b.second));
         HIPCHK(amr::launch_fill(c, *b.first, (int64_t)b.second, 0.0));
     }
+    return 0;
+}
+
+// The tangent and adjoint projections run the step's Poisson solve on their
+// own pair of iterates: the forward state's phi (userCtx comp 1), its mean and
+// the warm start's history stay as the forward step left them.
+struct LinPhi {
+    double *phiv[2] = {}, *phi = nullptr;
+    int last_V = 0;
+};
+int lin_phi_enter(amressif_ctx *ctx, LinPhi &keep)
+{
+    Ctx &c = ctx->c;
+    keep.phiv[0] = c.phiv[0];
+    keep.phiv[1] = c.phiv[1];
+    keep.phi = c.phi;
+    keep.last_V = c.last_V;
+    HIPCHK(hipMemcpyAsync(c.adj->phi_mean_save, &c.ctl->phi_mean, sizeof(double), hipMemcpyDeviceToDevice,
+                          c.stream));
+    c.phiv[0] = c.adj->phi_lin[0];
+    c.phiv[1] = c.adj->phi_lin[1];
+    c.phi = c.phiv[0];
+    return 0;
+}
+int lin_phi_leave(amressif_ctx *ctx, const LinPhi &keep)
+{
+    Ctx &c = ctx->c;
+    c.phiv[0] = keep.phiv[0];
+    c.phiv[1] = keep.phiv[1];
+    c.phi = keep.phi;
+    c.last_V = keep.last_V;
+    HIPCHK(hipMemcpyAsync(&c.ctl->phi_mean, c.adj->phi_mean_save, sizeof(double), hipMemcpyDeviceToDevice,
+                          c.stream));
     return 0;
 }
 
@@ -1661,6 +1746,8 @@ extern "C" {
 
 int amressif_tape_create(amressif_ctx *ctx, int max_steps, amressif_tape **out)
 {
+    if (ctx && ctx->c.in.PSEUDO_TIMESTEPPING == 0)
+        return fail(ctx, AMRESSIF_ERR_UNSUPPORTED, "the adjoint is that of the dual-time step (PSEUDO_TIMESTEPPING = 1)");
     if (!ctx || !out || max_steps < 1) return fail(ctx, AMRESSIF_ERR_ARG, "bad argument");
     *out = nullptr;
     if (int rc = ensure_adjoint(ctx)) return rc;
@@ -1777,6 +1864,8 @@ int amressif_step_tlm(amressif_ctx *ctx, const amressif_tape *tape, int first, i
     HIPCHK(amr::launch_ctl_set(c, c.in.momentum_tolerance, c.in.poisson_tol_rel, c.in.poisson_tol_abs, c.in.IterNum,
                                c.in.mg_max_iter));
     HIPCHK(amr::launch_ctl_reset_step(c));
+    LinPhi keep;
+    if (int rc = lin_phi_enter(ctx, keep)) return rc;
     for (int s = 0; s < nsteps; ++s) {
         amressif_lin_stats st{};
         const double *ustar = tape->slot(first + s);
@@ -1800,6 +1889,7 @@ int amressif_step_tlm(amressif_ctx *ctx, const amressif_tape *tape, int first, i
         st.flags = c.ctl_host->flags;
         if (stats) stats[s] = st;
     }
+    if (int rc = lin_phi_leave(ctx, keep)) return rc;
     HIPCHK(hipStreamSynchronize(c.stream));
     return AMRESSIF_OK;
 }
@@ -1814,6 +1904,8 @@ int amressif_step_bwd(amressif_ctx *ctx, amressif_tape *tape, int nsteps, amress
     HIPCHK(amr::launch_ctl_set(c, c.in.momentum_tolerance, c.in.poisson_tol_rel, c.in.poisson_tol_abs, c.in.IterNum,
                                c.in.mg_max_iter));
     HIPCHK(amr::launch_ctl_reset_step(c));
+    LinPhi keep;
+    if (int rc = lin_phi_enter(ctx, keep)) return rc;
     for (int s = 0; s < nsteps; ++s) {
         amressif_lin_stats st{};
         const double *ustar = tape->slot(tape->count - 1);
@@ -1836,6 +1928,7 @@ int amressif_step_bwd(amressif_ctx *ctx, amressif_tape *tape, int nsteps, amress
         tape->count -= 1;
         if (stats) stats[s] = st;
     }
+    if (int rc = lin_phi_leave(ctx, keep)) return rc;
     HIPCHK(hipStreamSynchronize(c.stream));
     return AMRESSIF_OK;
 }

@@ -79,6 +79,8 @@ struct AdjWork {
     double *XbU = nullptr, *XbV = nullptr;              // cell adjoints (interior)
     double *lidg = nullptr;                             // north-ghost adjoints of U (2 n)
     double *L[2][2]{};                                  // Richardson iterates (ping-pong, 2 face arrays)
+    double *phi_lin[2]{};                               // the projections' Poisson iterates (cells)
+    double *phi_mean_save = nullptr;                    // the forward phi's mean, kept across them
 };
 
 // Per-launch HIP-event timing of the step's main kernels inside the step loop
@@ -168,6 +170,11 @@ struct Ctx {
     int project_rows = 16;  // projection: rows per wave of the row-march kernel (0: cell-tile kernel)
     double *dct_tab = nullptr;
     float *dct_tab32 = nullptr;
+    // a coarsest level above 1024 cells (odd side > 32): its exact DCT solve as
+    // four dense products; tables C_x (nx^2), C_y (ny^2), 1/lambda (ny nx) and two
+    // work arrays (null: the bottom is the dense pseudo-inverse / k_tail_dct)
+    double *big_tab = nullptr, *big_work = nullptr;
+    float *big_tab32 = nullptr, *big_work32 = nullptr;
     // FP32 V-cycle: the level above it smoothed inside the same kernel
     // (k_tail_dct_ext; 32, 64 or 128 rows, at most 128 columns)
     int tail_ext = 1;
@@ -207,6 +214,9 @@ struct StageLaunch {
     int mode;               // 0 = stage update, 1 = stage update + norm partials, 2 = RHS only
     int unguarded;          // 1: ignore the control block (diagnostic timing)
     int foldp;              // modes 0, 1: cu, cv already hold -G p (launch_fold_gradp); p is not read
+    int expl;               // mode 1 with foldp: PSEUDO_TIMESTEPPING = 0's explicit update (rk = dt/1.5,
+                            // base = stencil input = u^n; fluxTotal's domain ghosts 0, so wall faces
+                            // take half the adjacent cell's flux)
 };
 hipError_t launch_stage(const Ctx &c, const StageLaunch &a, int *nblocks);
 // cu, cv += -G p on the interior faces, once per FP64 step (p fixed over its pseudo-iterations)
@@ -224,6 +234,8 @@ struct MixedStageLaunch {
 };
 hipError_t launch_stage_mixed(const Ctx &c, const MixedStageLaunch &a, int *nblocks);
 hipError_t launch_fin_pseudo(const Ctx &c, int nblocks);
+// the norm of PSEUDO_TIMESTEPPING = 0's single update: ends the loop, countIter stays 0
+hipError_t launch_fin_explicit(const Ctx &c, int nblocks);
 hipError_t launch_ctl_reset_step(const Ctx &c);
 hipError_t launch_bdf2(const Ctx &c);
 hipError_t launch_poisson_rhs(const Ctx &c, const double *u, const double *v, double *b, int *nblocks,
@@ -315,9 +327,33 @@ hipError_t amr_combine(const AmrGeom &a, hipStream_t s, double *w, double alpha,
                        const double *coef, size_t off1);
 hipError_t amr_wsum(const AmrGeom &a, hipStream_t s, const double *x0, const double *x1, double *partials);
 hipError_t amr_shift(const AmrGeom &a, hipStream_t s, double *x0, double *x1, const double *mean);
-hipError_t amr_restrict(const AmrGeom &a, hipStream_t s, const double *r0, const double *r1, void *rc, int mixed);
-hipError_t amr_prolong(const AmrGeom &a, hipStream_t s, const void *e, double *z0, double *z1, int mixed);
-hipError_t amr_fine_gs(const AmrGeom &a, hipStream_t s, const double *z0, double *z1, const double *r1, int color);
+// (guard: null, or the device composite FGMRES's control block: the kernel
+// does nothing once its Arnoldi cycle or solve has ended)
+hipError_t amr_restrict(const AmrGeom &a, hipStream_t s, const double *r0, const double *r1, void *rc, int mixed,
+                        const Ctl *guard = nullptr);
+hipError_t amr_prolong(const AmrGeom &a, hipStream_t s, const void *e, double *z0, double *z1, int mixed,
+                       const Ctl *guard = nullptr);
+hipError_t amr_fine_gs(const AmrGeom &a, hipStream_t s, const double *z0, double *z1, const double *r1, int color,
+                       const Ctl *guard = nullptr);
+// device composite FGMRES (kernels_amr.hip): vectors [coarse | fine] of stride vs
+struct CgmBufs {
+    Ctl *ctl;        // control: mg_done ends the solve, gm_stop the cycle
+    GmState *st;
+    double *V, *Z;   // m + 1 Arnoldi vectors, m preconditioned vectors
+    double *x, *b;   // iterate and right-hand side
+    double *vcur;    // the preconditioner's input (V_j normalised)
+    double *zcur;    // and its output
+    double *part;    // flat blocks x (m + 2) partials
+    size_t vs, off1;
+    int m;
+};
+hipError_t amr_cgm_begin(hipStream_t s, Ctl *ctl, GmState *st);
+hipError_t amr_cgm_edge(const AmrGeom &a, hipStream_t s, const CgmBufs &g);
+hipError_t amr_cgm_apply(const AmrGeom &a, hipStream_t s, const CgmBufs &g);
+hipError_t amr_cgm_orthonormalise(const AmrGeom &a, hipStream_t s, const CgmBufs &g);
+// the Arnoldi finalizers of kernels_gmres.hip on an nblk x (m + 2) partials layout
+hipError_t launch_gm_fin_dots_raw(hipStream_t s, double *part, int nblk, int m, GmState *st, Ctl *ctl, int pass);
+hipError_t launch_gm_fin_arnoldi_raw(hipStream_t s, double *part, int nblk, int m, GmState *st, Ctl *ctl);
 hipError_t amr_rhs(const AmrGeom &a, hipStream_t s, double dt, const double *u0, const double *v0, const double *u1,
                    const double *v1, double *b0, double *b1, double *partials);
 hipError_t amr_project(const AmrGeom &a, hipStream_t s, double dt, const double *phi0, const double *phi1, double *u0,

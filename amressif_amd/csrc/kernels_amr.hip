@@ -355,9 +355,15 @@ __global__ __launch_bounds__(256) void k_amr_shift(AmrGeom a, double *x0, double
 // ------------------------------------------------------ preconditioner
 // coarse right-hand side: r on uncovered cells, the mean of the four fine
 // residuals on covered cells
+// (guard: the device composite FGMRES's control block; its Arnoldi-step
+// kernels do nothing once the cycle or the solve has ended)
+__device__ __forceinline__ bool cgm_iter_skip(const Ctl *g) { return g && (g->mg_done | g->gm_stop); }
+
 template <class T>
-__global__ __launch_bounds__(256) void k_amr_restrict(AmrGeom a, const double *r0, const double *r1, T *rc)
+__global__ __launch_bounds__(256) void k_amr_restrict(AmrGeom a, const double *r0, const double *r1, T *rc,
+                                                      const Ctl *guard)
 {
+    if (cgm_iter_skip(guard)) return;
     const int i = blockIdx.x * 64 + (threadIdx.x & 63), j = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (i >= a.n0 || j >= a.n0) return;
     double v;
@@ -373,8 +379,9 @@ __global__ __launch_bounds__(256) void k_amr_restrict(AmrGeom a, const double *r
 // coarse correction onto the composite vector: z0 = e, z1 = e of the covering
 // coarse cell
 template <class T>
-__global__ __launch_bounds__(256) void k_amr_prolong(AmrGeom a, const T *e, double *z0, double *z1)
+__global__ __launch_bounds__(256) void k_amr_prolong(AmrGeom a, const T *e, double *z0, double *z1, const Ctl *guard)
 {
+    if (cgm_iter_skip(guard)) return;
     const long long k = blockIdx.x * 256LL + threadIdx.x;
     int lev, i, j;
     if (!flat_cell(a, k, lev, i, j)) return;
@@ -385,8 +392,9 @@ __global__ __launch_bounds__(256) void k_amr_prolong(AmrGeom a, const T *e, doub
 // one red or black Gauss-Seidel half-sweep of L_1 z1 = r1 on the fine patch
 // (coarse values z0 fixed in the C/F ghost)
 __global__ __launch_bounds__(256) void k_amr_fine_gs(AmrGeom a, const double *z0, double *z1, const double *r1,
-                                                     int color)
+                                                     int color, const Ctl *guard)
 {
+    if (cgm_iter_skip(guard)) return;
     const int i = blockIdx.x * 64 + (threadIdx.x & 63), j = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (i >= a.nx || j >= a.ny || ((i + j) & 1) != color) return;
     const int P = a.P1;
@@ -585,21 +593,261 @@ hipError_t amr_shift(const AmrGeom &a, hipStream_t s, double *x0, double *x1, co
     hipLaunchKernelGGL(k_amr_shift, dim3(flat_blocks(a)), dim3(256), 0, s, a, x0, x1, mean);
     return hipGetLastError();
 }
-hipError_t amr_restrict(const AmrGeom &a, hipStream_t s, const double *r0, const double *r1, void *rc, int mixed)
+hipError_t amr_restrict(const AmrGeom &a, hipStream_t s, const double *r0, const double *r1, void *rc, int mixed,
+                        const Ctl *guard)
 {
-    if (mixed) hipLaunchKernelGGL(k_amr_restrict<float>, g2(a.n0, a.n0), dim3(256), 0, s, a, r0, r1, (float *)rc);
-    else hipLaunchKernelGGL(k_amr_restrict<double>, g2(a.n0, a.n0), dim3(256), 0, s, a, r0, r1, (double *)rc);
+    if (mixed)
+        hipLaunchKernelGGL(k_amr_restrict<float>, g2(a.n0, a.n0), dim3(256), 0, s, a, r0, r1, (float *)rc, guard);
+    else
+        hipLaunchKernelGGL(k_amr_restrict<double>, g2(a.n0, a.n0), dim3(256), 0, s, a, r0, r1, (double *)rc, guard);
     return hipGetLastError();
 }
-hipError_t amr_prolong(const AmrGeom &a, hipStream_t s, const void *e, double *z0, double *z1, int mixed)
+hipError_t amr_prolong(const AmrGeom &a, hipStream_t s, const void *e, double *z0, double *z1, int mixed,
+                       const Ctl *guard)
 {
-    if (mixed) hipLaunchKernelGGL(k_amr_prolong<float>, dim3(flat_blocks(a)), dim3(256), 0, s, a, (const float *)e, z0, z1);
-    else hipLaunchKernelGGL(k_amr_prolong<double>, dim3(flat_blocks(a)), dim3(256), 0, s, a, (const double *)e, z0, z1);
+    if (mixed)
+        hipLaunchKernelGGL(k_amr_prolong<float>, dim3(flat_blocks(a)), dim3(256), 0, s, a, (const float *)e, z0, z1,
+                           guard);
+    else
+        hipLaunchKernelGGL(k_amr_prolong<double>, dim3(flat_blocks(a)), dim3(256), 0, s, a, (const double *)e, z0, z1,
+                           guard);
     return hipGetLastError();
 }
-hipError_t amr_fine_gs(const AmrGeom &a, hipStream_t s, const double *z0, double *z1, const double *r1, int color)
+hipError_t amr_fine_gs(const AmrGeom &a, hipStream_t s, const double *z0, double *z1, const double *r1, int color,
+                       const Ctl *guard)
 {
-    hipLaunchKernelGGL(k_amr_fine_gs, g2(a.nx, a.ny), dim3(256), 0, s, a, z0, z1, r1, color);
+    hipLaunchKernelGGL(k_amr_fine_gs, g2(a.nx, a.ny), dim3(256), 0, s, a, z0, z1, r1, color, guard);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------- device composite FGMRES
+// FGMRES(m) on the composite operator with the whole Arnoldi process on the
+// device, as the single-level solver (kernels_gmres.hip) runs it: the
+// control block's gm_stop ends a cycle, mg_done the solve.  The Arnoldi-step
+// kernels run while neither is set; the cycle-edge kernels (y = R^-1 g,
+// x += Z y, the true residual and the next cycle's start) run only once a
+// cycle has ended, so the host can enqueue batches of steps followed by a
+// cycle edge and wait once per batch.
+__device__ __forceinline__ bool cgm_edge_skip(const Ctl *g) { return g->mg_done || !g->gm_stop; }
+
+__device__ __forceinline__ size_t comp_index(const AmrGeom &a, int lev, int i, int j, size_t off1)
+{
+    return lev == 0 ? i + (size_t)a.P0 * j : off1 + i + (size_t)a.P1 * j;
+}
+
+__global__ void k_cgm_begin(Ctl *ctl, GmState *st)
+{
+    ctl->mg_done = 0;
+    ctl->gm_stop = 1;  // "between cycles": the first edge starts the first cycle from x = 0
+    ctl->mg_iters = 0;
+    ctl->mg_started = 0;
+    st->j = 0;
+    st->total = 0;
+}
+
+// y = R^-1 g over the ended cycle's columns (single thread), then x += Z y
+__global__ void k_cgm_solve(Ctl *ctl, GmState *st)
+{
+    if (cgm_edge_skip(ctl)) return;
+    const int jc = st->j;
+    for (int r = jc - 1; r >= 0; --r) {
+        double acc = st->gv[r];
+        for (int q = r + 1; q < jc; ++q) acc -= st->H[r * kGmMax + q] * st->y[q];
+        st->y[r] = acc / st->H[r * kGmMax + r];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cgm_update(AmrGeom a, double *x, const double *Z, size_t vs, const GmState *st,
+                                                    const Ctl *ctl, size_t off1)
+{
+    if (cgm_edge_skip(ctl)) return;
+    const long long k = blockIdx.x * 256LL + threadIdx.x;
+    int lev, i, j;
+    if (!flat_cell(a, k, lev, i, j)) return;
+    const size_t q = comp_index(a, lev, i, j, off1);
+    const int jc = st->j;
+    double acc = x[q];
+    for (int t = 0; t < jc; ++t) acc += st->y[t] * Z[t * vs + q];
+    x[q] = acc;
+}
+
+// V_0 = b - L_c x; partials ||V_0||inf, ||V_0||_2^2 (two per block)
+__global__ __launch_bounds__(256) void k_cgm_residual(AmrGeom a, const double *x, const double *b, double *V,
+                                                      const Ctl *ctl, size_t off1, double *partials)
+{
+    __shared__ double sh[16];
+    if (cgm_edge_skip(ctl)) return;
+    const long long k = blockIdx.x * 256LL + threadIdx.x;
+    int lev, i, j;
+    double m = 0.0, s2 = 0.0;
+    if (flat_cell(a, k, lev, i, j)) {
+        const size_t q = comp_index(a, lev, i, j, off1);
+        const double r = b[q] - (lev == 0 ? coarse_L(a, x, x + off1, i, j) : fine_L(a, x, x + off1, i, j));
+        V[q] = r;
+        m = (fabs(r) > m || r != r) ? fabs(r) : m;
+        s2 = r * r;
+    }
+    m = block_reduce<true>(m, sh);
+    s2 = block_reduce<false>(s2, sh + 8);
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = m;
+        partials[2 * blockIdx.x + 1] = s2;
+    }
+}
+
+// stopping test on the true residual (||r||inf <= max(tol_abs, tol_rel ||b||inf),
+// ||b||inf from the first residual, x = 0); otherwise start a cycle
+__global__ __launch_bounds__(1024) void k_cgm_fin_residual(const double *partials, int nblk, Ctl *ctl, GmState *st,
+                                                          int m)
+{
+    __shared__ double sh[32];
+    if (cgm_edge_skip(ctl)) return;
+    double mx = 0.0, s2 = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+        const double pm = partials[2 * b];
+        mx = (pm > mx || pm != pm) ? pm : mx;
+        s2 += partials[2 * b + 1];
+    }
+    mx = block_reduce<true>(mx, sh);
+    s2 = block_reduce<false>(s2, sh);
+    if (threadIdx.x != 0) return;
+    ctl->res_norm = mx;
+    if (!ctl->mg_started) {
+        ctl->b_norm = mx;
+        ctl->mg_started = 1;
+    }
+    const double thr = fmax(ctl->tol_abs, ctl->tol_rel * ctl->b_norm);
+    const double beta = sqrt(s2);
+    st->j = 0;
+    if (mx <= thr || !(mx == mx) || beta == 0.0) {
+        ctl->mg_done = 1;
+        if (!(mx == mx)) ctl->flags |= AMRESSIF_FLAG_NONFINITE;
+        return;
+    }
+    if (ctl->mg_iters >= ctl->mg_max_iter) {
+        ctl->mg_done = 1;
+        ctl->flags |= AMRESSIF_FLAG_MG_CAP;
+        return;
+    }
+    st->beta = beta;
+    st->target = thr * beta / mx;  // the 2-norm target of the Givens estimate
+    st->scale = 1.0 / beta;
+    for (int q = 0; q <= m; ++q) st->gv[q] = 0.0;
+    st->gv[0] = beta;
+    ctl->gm_stop = 0;
+}
+
+// V_j <- V_j scale (normalised); the preconditioner's input vcur = V_j
+__global__ __launch_bounds__(256) void k_cgm_load(AmrGeom a, double *V, size_t vs, double *vcur, const GmState *st,
+                                                  const Ctl *ctl, size_t off1)
+{
+    if (cgm_iter_skip(ctl)) return;
+    const long long k = blockIdx.x * 256LL + threadIdx.x;
+    int lev, i, j;
+    if (!flat_cell(a, k, lev, i, j)) return;
+    const size_t q = comp_index(a, lev, i, j, off1);
+    double *v = V + (size_t)st->j * vs;
+    const double x = v[q] * st->scale;
+    v[q] = x;
+    vcur[q] = x;
+}
+
+// Z_j = zcur; V_{j+1} = L_c zcur
+__global__ __launch_bounds__(256) void k_cgm_apply(AmrGeom a, const double *zcur, double *Z, double *V, size_t vs,
+                                                   const GmState *st, const Ctl *ctl, size_t off1)
+{
+    if (cgm_iter_skip(ctl)) return;
+    const long long k = blockIdx.x * 256LL + threadIdx.x;
+    int lev, i, j;
+    if (!flat_cell(a, k, lev, i, j)) return;
+    const size_t q = comp_index(a, lev, i, j, off1);
+    const int jj = st->j;
+    Z[(size_t)jj * vs + q] = zcur[q];
+    V[(size_t)(jj + 1) * vs + q] = lev == 0 ? coarse_L(a, zcur, zcur + off1, i, j) : fine_L(a, zcur, zcur + off1, i, j);
+}
+
+// partials of <V_{j+1}, V_t>, t <= j, into slot t of each block's m + 2
+__global__ __launch_bounds__(256) void k_cgm_dots(AmrGeom a, const double *V, size_t vs, const GmState *st,
+                                                  const Ctl *ctl, size_t off1, double *partials, int m)
+{
+    __shared__ double sh[16];
+    if (cgm_iter_skip(ctl)) return;
+    const long long k = blockIdx.x * 256LL + threadIdx.x;
+    int lev, i, j;
+    const bool ok = flat_cell(a, k, lev, i, j);
+    const size_t q = ok ? comp_index(a, lev, i, j, off1) : 0;
+    const int jj = st->j;
+    const double wq = ok ? V[(size_t)(jj + 1) * vs + q] : 0.0;
+    for (int t = 0; t <= jj; ++t) {
+        double acc = ok ? wq * V[(size_t)t * vs + q] : 0.0;
+        acc = block_reduce<false>(acc, sh);
+        if (threadIdx.x == 0) partials[(size_t)blockIdx.x * (m + 2) + t] = acc;
+    }
+}
+
+// V_{j+1} -= sum_t c_t V_t; partial ||V_{j+1}||_2^2 into slot m + 1
+__global__ __launch_bounds__(256) void k_cgm_orth(AmrGeom a, double *V, size_t vs, const GmState *st, const Ctl *ctl,
+                                                  size_t off1, double *partials, int m)
+{
+    __shared__ double sh[16];
+    if (cgm_iter_skip(ctl)) return;
+    const long long k = blockIdx.x * 256LL + threadIdx.x;
+    int lev, i, j;
+    double s2 = 0.0;
+    if (flat_cell(a, k, lev, i, j)) {
+        const size_t q = comp_index(a, lev, i, j, off1);
+        const int jj = st->j;
+        double w = V[(size_t)(jj + 1) * vs + q];
+        for (int t = 0; t <= jj; ++t) w -= st->c[t] * V[(size_t)t * vs + q];
+        V[(size_t)(jj + 1) * vs + q] = w;
+        s2 = w * w;
+    }
+    s2 = block_reduce<false>(s2, sh);
+    if (threadIdx.x == 0) partials[(size_t)blockIdx.x * (m + 2) + m + 1] = s2;
+}
+
+hipError_t amr_cgm_begin(hipStream_t s, Ctl *ctl, GmState *st)
+{
+    hipLaunchKernelGGL(k_cgm_begin, dim3(1), dim3(1), 0, s, ctl, st);
+    return hipGetLastError();
+}
+
+hipError_t amr_cgm_edge(const AmrGeom &a, hipStream_t s, const CgmBufs &g)
+{
+    hipLaunchKernelGGL(k_cgm_solve, dim3(1), dim3(1), 0, s, g.ctl, g.st);
+    hipLaunchKernelGGL(k_cgm_update, dim3(flat_blocks(a)), dim3(256), 0, s, a, g.x, (const double *)g.Z, g.vs,
+                       (const GmState *)g.st, (const Ctl *)g.ctl, g.off1);
+    hipLaunchKernelGGL(k_cgm_residual, dim3(flat_blocks(a)), dim3(256), 0, s, a, (const double *)g.x,
+                       (const double *)g.b, g.V, (const Ctl *)g.ctl, g.off1, g.part);
+    hipLaunchKernelGGL(k_cgm_fin_residual, dim3(1), dim3(1024), 0, s, (const double *)g.part, flat_blocks(a), g.ctl,
+                       g.st, g.m);
+    hipLaunchKernelGGL(k_cgm_load, dim3(flat_blocks(a)), dim3(256), 0, s, a, g.V, g.vs, g.vcur, (const GmState *)g.st,
+                       (const Ctl *)g.ctl, g.off1);
+    return hipGetLastError();
+}
+
+hipError_t amr_cgm_apply(const AmrGeom &a, hipStream_t s, const CgmBufs &g)
+{
+    hipLaunchKernelGGL(k_cgm_apply, dim3(flat_blocks(a)), dim3(256), 0, s, a, (const double *)g.zcur, g.Z, g.V, g.vs,
+                       (const GmState *)g.st, (const Ctl *)g.ctl, g.off1);
+    return hipGetLastError();
+}
+
+// classical Gram-Schmidt twice, then ||w||, the Givens rotations and the
+// stopping estimate; then the next V_j is normalised into vcur
+hipError_t amr_cgm_orthonormalise(const AmrGeom &a, hipStream_t s, const CgmBufs &g)
+{
+    const int nb = flat_blocks(a);
+    for (int pass = 0; pass < 2; ++pass) {
+        hipLaunchKernelGGL(k_cgm_dots, dim3(nb), dim3(256), 0, s, a, (const double *)g.V, g.vs, (const GmState *)g.st,
+                           (const Ctl *)g.ctl, g.off1, g.part, g.m);
+        if (hipError_t e = launch_gm_fin_dots_raw(s, g.part, nb, g.m, g.st, g.ctl, pass)) return e;
+        hipLaunchKernelGGL(k_cgm_orth, dim3(nb), dim3(256), 0, s, a, g.V, g.vs, (const GmState *)g.st,
+                           (const Ctl *)g.ctl, g.off1, g.part, g.m);
+    }
+    if (hipError_t e = launch_gm_fin_arnoldi_raw(s, g.part, nb, g.m, g.st, g.ctl)) return e;
+    hipLaunchKernelGGL(k_cgm_load, dim3(nb), dim3(256), 0, s, a, g.V, g.vs, g.vcur, (const GmState *)g.st,
+                       (const Ctl *)g.ctl, g.off1);
     return hipGetLastError();
 }
 hipError_t amr_rhs(const AmrGeom &a, hipStream_t s, double dt, const double *u0, const double *v0, const double *u1,

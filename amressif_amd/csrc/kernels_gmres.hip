@@ -374,4 +374,30 @@ hipError_t launch_gm_cycle_end(const Ctx &c, double *x)
     return hipGetLastError();
 }
 
+// The Arnoldi finalizers on another vector layout (the two-level composite
+// solve): they read only the block partials (nblk x (m + 2)), the state and
+// the control block.
+static GmArgs gm_raw(double *part, int nblk, int m, GmState *st, Ctl *ctl)
+{
+    GmArgs g{};
+    g.m = m;
+    g.nblk = nblk;
+    g.part = part;
+    g.st = st;
+    g.ctl = ctl;
+    return g;
+}
+
+hipError_t launch_gm_fin_dots_raw(hipStream_t s, double *part, int nblk, int m, GmState *st, Ctl *ctl, int pass)
+{
+    hipLaunchKernelGGL(k_gm_fin_dots, dim3(1), dim3(1024), 0, s, gm_raw(part, nblk, m, st, ctl), pass);
+    return hipGetLastError();
+}
+
+hipError_t launch_gm_fin_arnoldi_raw(hipStream_t s, double *part, int nblk, int m, GmState *st, Ctl *ctl)
+{
+    hipLaunchKernelGGL(k_gm_fin_arnoldi, dim3(1), dim3(1024), 0, s, gm_raw(part, nblk, m, st, ctl));
+    return hipGetLastError();
+}
+
 }  // namespace amr

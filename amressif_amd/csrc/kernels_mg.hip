@@ -1708,6 +1708,54 @@ static void launch_smoother(const Ctx &c, int l, const T *xin, const T *ec, int 
     }
 }
 
+// ------------------------------------- exact DCT solve of a large bottom
+// A coarsest level too large for the dense pseudo-inverse (an odd side above
+// 32 cells, e.g. 65 x 65 when n = 130) is solved exactly by the DCT-II, as the
+// small levels are: e = C_y (D+ .* (C_y^T r C_x)) C_x^T, here as four dense
+// products over the whole grid.  C = op(A) op(B) (.* S): one 16 x 16 output
+// tile per workgroup, K in steps of 16 through LDS.
+template <class T>
+__global__ __launch_bounds__(256) void k_dense_mm(const T *A, int lda, int ta, const T *B, int ldb, int tb, T *C,
+                                                  int ldc, int M, int N, int K, const T *S, int lds, const Ctl *ctl)
+{
+    if (ctl->mg_done | ctl->gm_stop) return;
+    __shared__ T sa[16][17], sb[16][17];
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int row = blockIdx.y * 16 + ty, col = blockIdx.x * 16 + tx;
+    T acc = 0;
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        const int ak = k0 + tx, bk = k0 + ty;
+        sa[ty][tx] = (row < M && ak < K) ? (ta ? A[(size_t)ak * lda + row] : A[(size_t)row * lda + ak]) : T(0);
+        sb[ty][tx] = (bk < K && col < N) ? (tb ? B[(size_t)col * ldb + bk] : B[(size_t)bk * ldb + col]) : T(0);
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk) acc += sa[ty][kk] * sb[kk][tx];
+        __syncthreads();
+    }
+    if (row < M && col < N) C[(size_t)row * ldc + col] = S ? acc * S[(size_t)row * lds + col] : acc;
+}
+
+template <class T>
+static hipError_t dense_bottom(const Ctx &c, const T *r, int pr, T *e, int pe, const Level &L)
+{
+    const int nx = L.nx, ny = L.ny;
+    const T *Cx = sizeof(T) == 4 ? (const T *)(const void *)c.big_tab32 : (const T *)(const void *)c.big_tab;
+    const T *Cy = Cx + (size_t)nx * nx, *Dinv = Cy + (size_t)ny * ny;
+    T *W1 = sizeof(T) == 4 ? (T *)(void *)c.big_work32 : (T *)(void *)c.big_work;
+    T *W2 = W1 + (size_t)nx * ny;
+    const dim3 grid((nx + 15) / 16, (ny + 15) / 16);
+    const Ctl *ctl = c.ctl;
+    hipLaunchKernelGGL(k_dense_mm<T>, grid, dim3(256), 0, c.stream, r, pr, 0, Cx, nx, 0, W1, nx, ny, nx, nx,
+                       (const T *)nullptr, 0, ctl);                                 // r C_x
+    hipLaunchKernelGGL(k_dense_mm<T>, grid, dim3(256), 0, c.stream, Cy, ny, 1, (const T *)W1, nx, 0, W2, nx, ny, nx,
+                       ny, Dinv, nx, ctl);                                          // D+ .* (C_y^T .)
+    hipLaunchKernelGGL(k_dense_mm<T>, grid, dim3(256), 0, c.stream, (const T *)W2, nx, 0, Cx, nx, 1, W1, nx, ny, nx,
+                       nx, (const T *)nullptr, 0, ctl);                             // . C_x^T
+    hipLaunchKernelGGL(k_dense_mm<T>, grid, dim3(256), 0, c.stream, Cy, ny, 0, (const T *)W1, nx, 0, e, pe, ny, nx, ny,
+                       (const T *)nullptr, 0, ctl);                                 // C_y .
+    return hipGetLastError();
+}
+
 // One V-cycle on the correction from zero.  r0 is the level-0 right-hand side;
 // with fuse set (and level 0 smoothed by its own kernels) the last up-leg
 // smoother also applies the correction to the FP64 iterate and writes the next
@@ -1733,7 +1781,11 @@ static hipError_t vcycle_impl(const Ctx &c, const T *pinv, const T *r0, const Ou
         if (l == 0 && fuse)
             if (hipError_t e = kprof_end(c)) return e;
     }
-    if (ext) {
+    if (c.big_tab) {
+        const Level &Lt = c.lev[lt];
+        const T *rt = lt == 0 ? r0 : (const T *)lev_r<T>(Lt);
+        if (hipError_t e = dense_bottom<T>(c, rt, Lt.pitch, lev_e<T>(Lt, 1), Lt.pitch, Lt)) return e;
+    } else if (ext) {
         const Level &L1 = c.lev[lt], &Lt = c.lev[lt + 1];
         const float *r1 = (const float *)(const void *)lev_r<T>(L1);
         float *e1 = (float *)(void *)lev_e<T>(L1, 1);

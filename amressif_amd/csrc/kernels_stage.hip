@@ -108,7 +108,7 @@ __device__ __forceinline__ void finish_cells(const StageArgs &a, int lane, int i
 // residual) as a template argument, so each form compiles to its own kernel.
 // FOLDP (modes 0, 1, 3): the constant c_u, c_v already holds -G p on the
 // interior faces (k_fold_gradp or the projection), so p is not read.
-template <int MODE, bool FOLDP = false>
+template <int MODE, bool FOLDP = false, bool EXPL = false>
 __global__ __launch_bounds__(256) void k_stage64(StageArgs a)
 {
     using R = double;
@@ -197,6 +197,7 @@ __global__ __launch_bounds__(256) void k_stage64(StageArgs a)
         const R vtop = (xin && jend <= n) ? vs[i + (size_t)P * jend] : R(0);  // y-face (i, jend)
         R fnU = quick_flux<R>(vtop, wU[1], wU[2], wU[3], wU[4]), fnV = quick_flux<R>(vtop, wV[1], wV[2], wV[3], wV[4]);
         R tVup = 0, pup = 0, vup = vtop, vbup = 0, cvup = 0;  // row jc + 1's TV, p, south-face v, base, constant
+        R tVtop = 0;  // TV of the chunk's top row (EXPL: the north wall faces take half of it)
         double nx_ = 0.0, ny_ = 0.0;
         // deferred stores, as in the upward march below: x-face (i, jc) and
         // y-face (i, jc + 1) of row jc go out at the top of iteration jc - 1
@@ -234,13 +235,16 @@ __global__ __launch_bounds__(256) void k_stage64(StageArgs a)
             const R CV = (feV - fwV) * idx + (fnV - fsV) * idy;
             const R VU = ((Up1 - R(2) * Uc + Um1) * idx2 + (wU[3] - R(2) * Uc + wU[1]) * idy2) * iren;
             const R VV = ((Vp1 - R(2) * Vc + Vm1) * idx2 + (wV[3] - R(2) * Vc + wV[1]) * idy2) * iren;
-            const R TU = (R(0) - CU) + VU, TV = (R(0) - CV) + VV;
+            // EXPL: the domain ghosts of fluxTotal keep setVal's 0 (no wall-ghost pass)
+            const bool tcell = xin && jc >= 0 && jc < n;
+            const R TU = EXPL && !tcell ? R(0) : (R(0) - CU) + VU, TV = EXPL && !tcell ? R(0) : (R(0) - CV) + VV;
             const R TUw = lane_prev(TU);
             const R pc = cur.pc, pw = lane_prev(pc);
-            if (jc >= j0 && outlane && xface) {  // x-face (i, jc); wall faces get R = 0 exactly
+            if (jc >= j0 && outlane && xface) {  // x-face (i, jc); wall faces get R = 0 exactly (EXPL: T/2)
                 skx = (unsigned)(i < 0 ? 0 : i) + (unsigned)P * jc;
                 const bool wall = i == 0 || i == n;
-                const double Rx = wall ? 0.0
+                const double Rx = EXPL   ? R(0.5) * (TUw + TU)
+                                 : wall  ? 0.0
                                  : FOLDP ? R(0.5) * (TUw + TU)
                                          : -(pc - pw) * a.inv_dx + R(0.5) * (TUw + TU);
                 sxf = true;
@@ -257,7 +261,8 @@ __global__ __launch_bounds__(256) void k_stage64(StageArgs a)
             if (jc + 1 < jend && jc + 1 >= j0 && outlane && xin) {  // y-face (i, jc + 1)
                 sky = (unsigned)i + (unsigned)P * (jc + 1);
                 const bool wall = jc + 1 == 0;
-                const double Ry = wall ? 0.0
+                const double Ry = EXPL   ? R(0.5) * (TV + tVup)
+                                 : wall  ? 0.0
                                  : FOLDP ? R(0.5) * (TV + tVup)
                                          : -(pup - pc) * a.inv_dy + R(0.5) * (TV + tVup);
                 syf = true;
@@ -271,13 +276,14 @@ __global__ __launch_bounds__(256) void k_stage64(StageArgs a)
                     if (MODE == 1) { const double d = vbup - vnew; ny_ += fabs(d * d * (wall ? 0.5 : 1.0)); }
                 }
             }
+            if (jc == jend - 1) tVtop = TV;
             fnU = fsU; fnV = fsV; tVup = TV; pup = pc; vup = vsf; vbup = cur.vb; cvup = cur.cv;
 #pragma unroll
             for (int q = 4; q > 0; --q) { wU[q] = wU[q - 1]; wV[q] = wV[q - 1]; }
             cur = nxt;
         }
         flush();
-        if (jend == n && outlane && xin) {  // north wall row of y-faces: R = 0
+        if (jend == n && outlane && xin) {  // north wall row of y-faces: R = 0 (EXPL: T(n - 1)/2)
             const size_t k = i + (size_t)P * n;
             if (MODE == 2) a.vo[k] = 0.0;
             else if (MODE == 3) {
@@ -285,7 +291,7 @@ __global__ __launch_bounds__(256) void k_stage64(StageArgs a)
                 a.rv[k] = (float)(-a.c15 * vtop + a.cv[k]);
             } else {
                 const double base = a.vb[k];
-                const double vnew = base + a.rk * (0.0 - a.c15 * vtop + a.cv[k]);
+                const double vnew = base + a.rk * ((EXPL ? R(0.5) * tVtop : 0.0) - a.c15 * vtop + a.cv[k]);
                 a.vo[k] = vnew;
                 if (MODE == 1) { const double d = base - vnew; ny_ += fabs(d * d * 0.5); }
             }
@@ -348,14 +354,17 @@ __global__ __launch_bounds__(256) void k_stage64(StageArgs a)
         const R CV = (feV - fwV) * idx + (fnV - fsV) * idy;
         const R VU = ((Up1 - R(2) * Uc + Um1) * idx2 + (wU[3] - R(2) * Uc + wU[1]) * idy2) * iren;
         const R VV = ((Vp1 - R(2) * Vc + Vm1) * idx2 + (wV[3] - R(2) * Vc + wV[1]) * idy2) * iren;
-        const R TU = (R(0) - CU) + VU, TV = (R(0) - CV) + VV;
+        // EXPL: the domain ghosts of fluxTotal keep setVal's 0 (no wall-ghost pass)
+        const bool tcell = xin && jc >= 0 && jc < n;
+        const R TU = EXPL && !tcell ? R(0) : (R(0) - CU) + VU, TV = EXPL && !tcell ? R(0) : (R(0) - CV) + VV;
         const R TUw = lane_prev(TU);
         const R pc = cur.pc, pw = lane_prev(pc);
         if (jc >= j0 && outlane) {
             sk = (unsigned)(i < 0 ? 0 : i) + (unsigned)P * jc;
-            if (xface) {  // x-face (i, jc); wall faces get R = 0 exactly
+            if (xface) {  // x-face (i, jc); wall faces get R = 0 exactly (EXPL: T/2)
                 const bool wall = i == 0 || i == n;
-                const double Rx = wall ? 0.0
+                const double Rx = EXPL   ? R(0.5) * (TUw + TU)
+                                 : wall  ? 0.0
                                  : FOLDP ? R(0.5) * (TUw + TU)
                                          : -(pc - pw) * a.inv_dx + R(0.5) * (TUw + TU);
                 sxf = true;
@@ -371,7 +380,8 @@ __global__ __launch_bounds__(256) void k_stage64(StageArgs a)
             }
             if (xin) {  // y-face (i, jc)
                 const bool wall = jc == 0;
-                const double Ry = wall ? 0.0
+                const double Ry = EXPL   ? R(0.5) * (tVprev + TV)
+                                 : wall  ? 0.0
                                  : FOLDP ? R(0.5) * (tVprev + TV)
                                          : -(pc - pprev) * a.inv_dy + R(0.5) * (tVprev + TV);
                 syf = true;
@@ -392,7 +402,7 @@ __global__ __launch_bounds__(256) void k_stage64(StageArgs a)
         cur = nxt;
     }
     flush();
-    if (jend == n && outlane && xin) {  // north wall row of y-faces: R = 0
+    if (jend == n && outlane && xin) {  // north wall row of y-faces: R = 0 (EXPL: T(n - 1)/2)
         const size_t k = i + (size_t)P * n;
         if (a.mode == 2) a.vo[k] = 0.0;
         else if (a.mode == 3) {
@@ -400,7 +410,7 @@ __global__ __launch_bounds__(256) void k_stage64(StageArgs a)
             a.rv[k] = (float)(-a.c15 * vn + a.cv[k]);
         } else {
             const double base = a.vb[k];
-            const double vnew = base + a.rk * (0.0 - a.c15 * vn + a.cv[k]);
+            const double vnew = base + a.rk * ((EXPL ? R(0.5) * tVprev : 0.0) - a.c15 * vn + a.cv[k]);
             a.vo[k] = vnew;
             if (a.mode == 1) { const double d = base - vnew; ny_ += fabs(d * d * 0.5); }
         }
@@ -588,7 +598,10 @@ hipError_t launch_stage(const Ctx &c, const StageLaunch &s, int *nblocks)
         else hipLaunchKernelGGL((k_stage64<0, false>), grid, dim3(256), 0, c.stream, a);
         break;
     case 1:
-        if (s.foldp) hipLaunchKernelGGL((k_stage64<1, true>), grid, dim3(256), 0, c.stream, a);
+        if (s.expl) {
+            if (!s.foldp) return hipErrorInvalidValue;
+            hipLaunchKernelGGL((k_stage64<1, true, true>), grid, dim3(256), 0, c.stream, a);
+        } else if (s.foldp) hipLaunchKernelGGL((k_stage64<1, true>), grid, dim3(256), 0, c.stream, a);
         else hipLaunchKernelGGL((k_stage64<1, false>), grid, dim3(256), 0, c.stream, a);
         break;
     case 2: hipLaunchKernelGGL((k_stage64<2, false>), grid, dim3(256), 0, c.stream, a); break;

@@ -53,6 +53,35 @@ hipError_t launch_fin_pseudo(const Ctx &c, int nblocks)
     return hipGetLastError();
 }
 
+// PSEUDO_TIMESTEPPING = 0 (main.cpp:900-926): Error_Computation of the one
+// explicit update, then the branch leaves the loop before countIter is
+// incremented (no cap or divergence test); a non-finite norm is flagged.
+__global__ __launch_bounds__(1024) void k_fin_explicit(const double *partials, int nb, Ctl *ctl, int n)
+{
+    __shared__ double sh[32];
+    double sx = 0.0, sy = 0.0;
+    for_records<2, 2>(partials, nb, [&](const double *o) {
+        sx += o[0];
+        sy += o[1];
+    });
+    double red[2] = {sx, sy};
+    block_reduce_n<0u>(red, sh);
+    if (threadIdx.x == 0) {
+        const double npts = (double)n * (double)n;
+        const double ex = sqrt(red[0]) / sqrt(npts), ey = sqrt(red[1]) / sqrt(npts);
+        const double ne = (ex < ey) ? ey : ex;
+        ctl->norm_error = ne;
+        if (!(ne == ne) || isinf(ne)) ctl->flags |= AMRESSIF_FLAG_NONFINITE;
+        ctl->mom_done = 1;
+    }
+}
+
+hipError_t launch_fin_explicit(const Ctx &c, int nblocks)
+{
+    hipLaunchKernelGGL(k_fin_explicit, dim3(1), dim3(1024), 0, c.stream, c.partials, nblocks, c.ctl, c.g.n);
+    return hipGetLastError();
+}
+
 // plain weighted norm of (ub - uo) for the module entry point
 __global__ __launch_bounds__(256) void k_norm_partials(const double *ub, const double *vb, const double *uo,
                                                        const double *vo, int n, int P, double *partials)

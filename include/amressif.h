@@ -6,8 +6,9 @@
  * The reference has no advance function: its step is inline in main_main
  * (Source/src1_rever/main.cpp:875-1202) and its module functions take AMReX
  * MultiFabs.  This header replaces those module functions and the step loop
- * with entry points on AMReX Array4 views of one box (device pointers, ghosts
- * included, i fastest, then j, then component), one box per GPU.
+ * with entry points on AMReX Array4 views (device pointers, ghosts included,
+ * i fastest, then j, then component): one box per view, and several boxes
+ * per MultiFab at the state load/store (amr_multifab).
  *
  * Conventions
  *  - The caller owns every field array (device memory).  A context owns its
@@ -30,7 +31,7 @@ extern "C" {
 #define AMRESSIF_OK 0
 #define AMRESSIF_ERR_ARG (-1)         /* bad argument / shape mismatch */
 #define AMRESSIF_ERR_BC (-2)          /* BC code the reference aborts on (main.cpp:822,832; utilities.cpp:128) */
-#define AMRESSIF_ERR_UNSUPPORTED (-3) /* valid in the reference but not built here (periodic 111, 3D, multi-box) */
+#define AMRESSIF_ERR_UNSUPPORTED (-3) /* valid in the reference but not built here (periodic 111, 3D; PSEUDO_TIMESTEPPING 0 on the tape and the two-level step) */
 #define AMRESSIF_ERR_HIP (-4)         /* HIP runtime failure */
 #define AMRESSIF_ERR_NONFINITE (-5)   /* non-finite pseudo-iteration norm (reference exits silently) */
 #define AMRESSIF_ERR_PARSE (-6)       /* inputs file error */

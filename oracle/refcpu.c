@@ -538,6 +538,12 @@ static double vlinf(const double *x, size_t m)
     return r;
 }
 
+/* bottom levels of more cells than this are solved by the DCT (the same
+ * mean-zero solution as the pseudo-inverse, without its O(m^3) set-up) */
+#define MG_PINV_MAX 1024
+
+static void dct_solve_rect(int nx, int ny, double hx, double hy, const double *b, double *x);
+
 typedef struct {
     int nlev;
     mg_level lev[32];
@@ -565,6 +571,8 @@ static void mg_build(const rc_params *p, mg_hier *H)
     H->nlev = l;
     mg_level *B = &H->lev[l - 1];
     H->mb = B->nx * B->ny;
+    H->pinv = NULL;
+    if (H->mb > MG_PINV_MAX) return; /* the bottom is solved by the DCT instead */
     H->pinv = (double *)malloc(sizeof(double) * (size_t)H->mb * H->mb);
     rc_pinv_neumann(B->nx, B->ny, 1.0 / sqrt(B->ax), 1.0 / sqrt(B->ay), H->pinv);
 }
@@ -585,6 +593,10 @@ static void mg_vcycle(mg_hier *H, int l)
     const size_t m = (size_t)L->nx * L->ny;
     if (l == H->nlev - 1) {
         const int mb = H->mb;
+        if (!H->pinv) { /* an odd bottom too large for the dense pseudo-inverse: its exact DCT solve */
+            dct_solve_rect(L->nx, L->ny, 1.0 / sqrt(L->ax), 1.0 / sqrt(L->ay), L->r, L->e);
+            return;
+        }
         for (int i = 0; i < mb; ++i) {
             double s = 0.0;
             for (int k = 0; k < mb; ++k) s += H->pinv[(size_t)i * mb + k] * L->r[k];
@@ -611,53 +623,62 @@ static void mg_vcycle(mg_hier *H, int l)
     for (int s = 0; s < 2; ++s) { mg_gsrb(L, L->e, L->r, 0); mg_gsrb(L, L->e, L->r, 1); }
 }
 
-/* exact mean-zero solve by the DCT-II diagonalisation of the Neumann
- * Laplacian: eigenvalues -(4/h^2) sin^2(pi k / 2n) per direction */
+/* exact mean-zero solve of the nx x ny Neumann 5-point problem (cell sizes
+ * hx, hy) by the DCT-II diagonalisation: eigenvalues -(4/h^2) sin^2(pi k / 2n)
+ * per direction, the constant mode dropped (the pseudo-inverse's solution) */
+static void dct_solve_rect(int nx, int ny, double hx, double hy, const double *b, double *x)
+{
+    const double pi = 3.14159265358979323846;
+    double *Qx = (double *)malloc(sizeof(double) * (size_t)nx * nx);
+    double *Qy = (double *)malloc(sizeof(double) * (size_t)ny * ny);
+    double *T = (double *)malloc(sizeof(double) * (size_t)nx * ny);
+    double *H = (double *)malloc(sizeof(double) * (size_t)nx * ny);
+    for (int i = 0; i < nx; ++i)
+        for (int k = 0; k < nx; ++k)
+            Qx[(size_t)i * nx + k] = (k == 0 ? sqrt(1.0 / nx) : sqrt(2.0 / nx)) * cos(pi * k * (i + 0.5) / nx);
+    for (int j = 0; j < ny; ++j)
+        for (int l = 0; l < ny; ++l)
+            Qy[(size_t)j * ny + l] = (l == 0 ? sqrt(1.0 / ny) : sqrt(2.0 / ny)) * cos(pi * l * (j + 0.5) / ny);
+    /* T[j][k] = sum_i b[j][i] Qx[i][k] */
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < ny; ++j)
+        for (int k = 0; k < nx; ++k) {
+            double s = 0.0;
+            for (int i = 0; i < nx; ++i) s += b[(size_t)j * nx + i] * Qx[(size_t)i * nx + k];
+            T[(size_t)j * nx + k] = s;
+        }
+    /* H[l][k] = sum_j Qy[j][l] T[j][k] / (lx[k] + ly[l]) */
+#pragma omp parallel for schedule(static)
+    for (int l = 0; l < ny; ++l)
+        for (int k = 0; k < nx; ++k) {
+            double s = 0.0;
+            for (int j = 0; j < ny; ++j) s += Qy[(size_t)j * ny + l] * T[(size_t)j * nx + k];
+            const double sx = sin(pi * k / (2.0 * nx)), sy = sin(pi * l / (2.0 * ny));
+            const double lam = -4.0 * sx * sx / (hx * hx) - 4.0 * sy * sy / (hy * hy);
+            H[(size_t)l * nx + k] = (k == 0 && l == 0) ? 0.0 : s / lam;
+        }
+    /* T[l][i] = sum_k H[l][k] Qx[i][k] */
+#pragma omp parallel for schedule(static)
+    for (int l = 0; l < ny; ++l)
+        for (int i = 0; i < nx; ++i) {
+            double s = 0.0;
+            for (int k = 0; k < nx; ++k) s += H[(size_t)l * nx + k] * Qx[(size_t)i * nx + k];
+            T[(size_t)l * nx + i] = s;
+        }
+    /* x[j][i] = sum_l Qy[j][l] T[l][i] */
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            double s = 0.0;
+            for (int l = 0; l < ny; ++l) s += Qy[(size_t)j * ny + l] * T[(size_t)l * nx + i];
+            x[(size_t)j * nx + i] = s;
+        }
+    free(Qx); free(Qy); free(T); free(H);
+}
+
 static void dct_solve(const rc_params *p, const double *b, double *x)
 {
-    const int n = p->n;
-    double *Q = (double *)malloc(sizeof(double) * (size_t)n * n);
-    double *T = (double *)malloc(sizeof(double) * (size_t)n * n);
-    double *H = (double *)malloc(sizeof(double) * (size_t)n * n);
-    const double pi = 3.14159265358979323846;
-    for (int i = 0; i < n; ++i)
-        for (int k = 0; k < n; ++k)
-            Q[(size_t)i * n + k] = (k == 0 ? sqrt(1.0 / n) : sqrt(2.0 / n)) * cos(pi * k * (i + 0.5) / n);
-    /* T[j][k] = sum_i b[j][i] Q[i][k] */
-#pragma omp parallel for schedule(static)
-    for (int j = 0; j < n; ++j)
-        for (int k = 0; k < n; ++k) {
-            double s = 0.0;
-            for (int i = 0; i < n; ++i) s += b[(size_t)j * n + i] * Q[(size_t)i * n + k];
-            T[(size_t)j * n + k] = s;
-        }
-    /* H[l][k] = sum_j Q[j][l] T[j][k] / (lx[k] + ly[l]) */
-#pragma omp parallel for schedule(static)
-    for (int l = 0; l < n; ++l)
-        for (int k = 0; k < n; ++k) {
-            double s = 0.0;
-            for (int j = 0; j < n; ++j) s += Q[(size_t)j * n + l] * T[(size_t)j * n + k];
-            const double sx = sin(pi * k / (2.0 * n)), sy = sin(pi * l / (2.0 * n));
-            const double lam = -4.0 * sx * sx / (p->dx * p->dx) - 4.0 * sy * sy / (p->dy * p->dy);
-            H[(size_t)l * n + k] = (k == 0 && l == 0) ? 0.0 : s / lam;
-        }
-    /* T[l][i] = sum_k H[l][k] Q[i][k] */
-#pragma omp parallel for schedule(static)
-    for (int l = 0; l < n; ++l)
-        for (int i = 0; i < n; ++i) {
-            double s = 0.0;
-            for (int k = 0; k < n; ++k) s += H[(size_t)l * n + k] * Q[(size_t)i * n + k];
-            T[(size_t)l * n + i] = s;
-        }
-    /* x[j][i] = sum_l Q[j][l] T[l][i] */
-#pragma omp parallel for schedule(static)
-    for (int j = 0; j < n; ++j)
-        for (int i = 0; i < n; ++i) {
-            double s = 0.0;
-            for (int l = 0; l < n; ++l) s += Q[(size_t)j * n + l] * T[(size_t)l * n + i];
-            x[(size_t)j * n + i] = s;
-        }
-    free(Q); free(T); free(H);
+    dct_solve_rect(p->n, p->n, p->dx, p->dy, b, x);
 }
 
 /* Source/src1_rever/poisson.cpp:41-186 (poisson_advance): sol and rhs carry
@@ -786,7 +807,29 @@ int rc_advance(const rc_params *p, double *ucont, double *vcont, double *uprev, 
         }
         int count = 0;
         double normError = 1.e9;
-        while (normError > p->mom_tol) {
+        if (!p->pseudo) {
+            /* PSEUDO_TIMESTEPPING = 0 (main.cpp:900-926): one explicit update
+             * from velCart = cont2cart(u^n); fluxTotal keeps the 0 of setVal in
+             * its domain ghosts (no wall-ghost pass on this branch), then
+             * explicit_time_marching (momentum.cpp:96-191): rhs += diff/(2 dt),
+             * u* += (dt/1.5) rhs; countIter stays 0 (the branch breaks out of
+             * the loop before it is counted). */
+            rc_gradient(p, ctx, gpx, gpy, gphix, gphiy);
+            memset(ftot, 0, ng1 * 2 * 8);
+            rc_convective_flux(p, vc, ustar, vstar, fxu, fxv, fyu, fyv, fconv, ftot);
+            rc_viscous_flux(p, vc, fvisc, ftot);
+            rc_momentum_rhs(p, ftot, gpx, gpy, rhsx, rhsy);
+            for (size_t k = 0; k < nf; ++k) {
+                rhsx[k] = rhsx[k] + (1.0 / (2.0 * p->dt)) * udiff[k];
+                rhsy[k] = rhsy[k] + (1.0 / (2.0 * p->dt)) * vdiff[k];
+                ustar[k] = ustar[k] + ((p->dt / 1.5) * rhsx[k]);
+                vstar[k] = vstar[k] + ((p->dt / 1.5) * rhsy[k]);
+            }
+            normError = rc_error_computation(p, ucont, vcont, ustar, vstar);
+            memcpy(ucont, ustar, nf * 8);
+            memcpy(vcont, vstar, nf * 8);
+        }
+        while (p->pseudo && normError > p->mom_tol) {
             for (int sub = 0; sub < 4; ++sub) {
                 rc_gradient(p, ctx, gpx, gpy, gphix, gphiy);
                 memset(ftot, 0, ng1 * 2 * 8);

@@ -39,6 +39,7 @@ typedef struct {
     int mg_max_iter;        /* MG iteration cap */
     int mg_bottom_max;      /* coarsen while n_level > mg_bottom_max and even */
     int nthreads;           /* OpenMP threads (<=0: runtime default) */
+    int pseudo;             /* PSEUDO_TIMESTEPPING: 1 = RK4 dual time (default), 0 = one explicit update */
 } rc_params;
 
 typedef struct {

@@ -43,6 +43,7 @@ class Params(ctypes.Structure):
         ("mg_max_iter", ctypes.c_int),
         ("mg_bottom_max", ctypes.c_int),
         ("nthreads", ctypes.c_int),
+        ("pseudo", ctypes.c_int),
     ]
 
 
@@ -104,7 +105,7 @@ class Oracle:
 
     def __init__(self, n, ren, dt, dx=None, dy=None, bc_lo=(135, 135), bc_hi=(135, 165),
                  inflow=(1.0, 0.0), mom_tol=1e-10, iter_num=100, poisson="mg",
-                 poisson_tol_rel=1e-10, mg_max_iter=200, mg_bottom_max=16, lid=None, nthreads=0):
+                 poisson_tol_rel=1e-10, mg_max_iter=200, mg_bottom_max=16, lid=None, nthreads=0, pseudo=1):
         self.n = n
         self._lid = None if lid is None else np.ascontiguousarray(lid, dtype=np.float64)
         prm = Params()
@@ -124,6 +125,7 @@ class Oracle:
         prm.mg_max_iter = mg_max_iter
         prm.mg_bottom_max = mg_bottom_max
         prm.nthreads = nthreads
+        prm.pseudo = pseudo
         self.prm = prm
         self.L = lib()
 

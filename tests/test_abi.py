@@ -78,7 +78,7 @@ def test_create_rejects_codes_the_reference_aborts_on(api):
     assert _create_status(api, phy_bc_hi=(135, -165, 135))[0] == -2
     rc, msg = _create_status(api, phy_bc_lo=(111, 135, 135), phy_bc_hi=(111, 165, 135))
     assert rc == -3 and "periodic" in msg
-    assert _create_status(api, PSEUDO_TIMESTEPPING=0)[0] == -3
+    assert _create_status(api, PSEUDO_TIMESTEPPING=2)[0] == -1     # only 0 (explicit) and 1 (dual time)
     assert _create_status(api, precision=7)[0] == -1
 
 

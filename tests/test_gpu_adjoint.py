@@ -218,3 +218,35 @@ def test_tape_bounds_are_checked():
         run.ctx.step_bwd(tape, 3)
     with pytest.raises(api.AmressifError):
         run.ctx.step_tlm(tape, 1, 2)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_linear_solves_leave_the_forward_phi_and_history_untouched(prec):
+    """amressif_step_tlm / amressif_step_bwd run their projections' Poisson
+    solves on their own iterates: store_state afterwards returns the forward
+    phi, and later steps (whose warm start, in the mixed step, extrapolates
+    from the phi history) are bit-identical to those of a context that ran no
+    tangent or adjoint."""
+    api = _api()
+    n = 32
+    rng = np.random.default_rng(31)
+    runs = []
+    for with_linear in (True, False):
+        run = api.Run.from_numpy(api.cavity_inputs(n, 1000.0, precision=prec), W.initial_state(n, "seeded"))
+        run.advance(3)
+        tape = run.ctx.tape(2)
+        run.ctx.step_fwd(2, tape)
+        before = run.fetch()
+        if with_linear:
+            du, dv = _faces(rng, n)
+            run.ctx.load_dual(du, dv, None, None, None, None)
+            run.ctx.step_tlm(tape, 0, 2)
+            run.ctx.load_dual(du, dv, None, None, None, None)
+            run.ctx.step_bwd(tape, 2)
+            after = run.fetch()
+            for k in ("u", "v", "uprev", "vprev", "ctx"):
+                assert np.array_equal(after[k], before[k]), k
+        run.advance(3)
+        runs.append(run.fetch())
+    for k in ("u", "v", "ctx"):
+        assert np.array_equal(runs[0][k], runs[1][k]), k

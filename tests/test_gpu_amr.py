@@ -118,3 +118,33 @@ def test_patch_bounds_are_checked():
         _two_level(32, 31, 100.0)
     with pytest.raises(api.AmressifError):
         _two_level(32, 1, 100.0)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_device_composite_fgmres_matches_the_host_driven_one(prec):
+    """The composite FGMRES with its Arnoldi process on the device (one host
+    wait per batch of steps) takes the same steps as the host-driven form
+    (AMRESSIF_AMR_HOST_GMRES=1, one wait per dot-product pass) and gives the
+    same fields to rounding."""
+    import os
+    api = _api()
+    n0, jc0, steps = 64, 48, 4
+    out = {}
+    for host in ("1", "0"):
+        old = os.environ.get("AMRESSIF_AMR_HOST_GMRES")
+        os.environ["AMRESSIF_AMR_HOST_GMRES"] = host
+        try:
+            tl = _two_level(n0, jc0, 2000.0, precision=prec)
+        finally:
+            if old is None:
+                os.environ.pop("AMRESSIF_AMR_HOST_GMRES")
+            else:
+                os.environ["AMRESSIF_AMR_HOST_GMRES"] = old
+        st = tl.advance(steps)
+        out[host] = (tl.fetch(), st)
+    (fh, sh), (fd, sd) = out["1"], out["0"]
+    assert [s["poisson_iters"] for s in sd] == [s["poisson_iters"] for s in sh]
+    for s in sd:
+        assert s["flags"] == 0 and s["poisson_res"] <= 1e-10 and s["div_new"] <= 1e-9 * s["div_star"] + 1e-10
+    for k in ("u0", "v0", "u1", "v1"):
+        assert rel(fd[k], fh[k]) < 1e-9, k

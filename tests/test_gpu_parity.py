@@ -265,10 +265,32 @@ def test_advance_odd_sizes(n):
     check_step_parity(ref, rs, got, st)
 
 
+@pytest.mark.parametrize("n", [130, 98])
+def test_odd_bottom_poisson_solve_matches_the_exact_solve(n):
+    """n = 130 coarsens to an odd 65 x 65 bottom (98 to 49 x 49), above the
+    dense pseudo-inverse's 1024 cells: that level is solved exactly by the DCT
+    as four dense products.  The solve reaches the exact mean-zero solution."""
+    api = _api()
+    rng = np.random.default_rng(n)
+    rhs = np.zeros((n + 2, n + 2))
+    rhs[1:-1, 1:-1] = rng.standard_normal((n, n))
+    xe, _, _ = Oracle(n, 100.0, 1e-4, poisson="dct").poisson_solve(rhs)
+    for prec, tol, bar in ((0, 1e-12, 1e-9), (1, 1e-10, 1e-7)):
+        ctx = api.Context(api.cavity_inputs(n, 100.0, precision=prec))
+        sol, it, res = ctx.poisson_advance(cuda(rhs), tol_rel=tol)
+        assert res <= tol and it <= 40, (prec, it, res)
+        assert rel(sol.cpu().numpy()[1:-1, 1:-1], xe[1:-1, 1:-1]) < bar, prec
+
+
+def test_odd_bottom_steps_match_oracle():
+    ref, rs, got, st = run_both(130, 400.0, 4, init="seeded")
+    check_step_parity(ref, rs, got, st)
+
+
 def test_unsupported_multigrid_size_reported():
     api = _api()
     with pytest.raises(api.AmressifError) as e:
-        api.Context(api.cavity_inputs(130, 100.0))  # coarsens to an odd 65 x 65 bottom level
+        api.Context(api.cavity_inputs(4098, 100.0))  # coarsens to a 2049 x 2049 bottom level
     assert e.value.code == -3
 
 
@@ -466,3 +488,43 @@ def test_short_first_vcycle_batch_gives_the_same_steps(prec):
     assert sa == sb
     for k in ("u", "v", "ctx"):
         assert np.array_equal(a[k], b[k]), k
+
+
+# ------------------------------------------- PSEUDO_TIMESTEPPING = 0 (explicit)
+@pytest.mark.parametrize("n,init,steps", [(64, "zero", 10), (128, "seeded", 6), (100, "seeded", 4)])
+def test_explicit_time_marching_matches_oracle(n, init, steps):
+    """main.cpp:900-926 + momentum.cpp:96-191: one explicit update per step,
+    countIter 0, and the reference's wall faces: with no wall-ghost pass on
+    fluxTotal the lid row's wall-normal faces move (the oracle's closed form
+    pins that), so the projected field keeps a uniform divergence equal to the
+    net wall flux; both sides agree on all of it."""
+    api = _api()
+    ren = 400.0
+    inp = api.cavity_inputs(n, ren, PSEUDO_TIMESTEPPING=0)
+    o = Oracle(n, ren, api.dt_of(inp), poisson="mg", poisson_tol_rel=1e-13, pseudo=0)
+    st = W.initial_state(n, init)
+    ref = {k: v.copy() for k, v in st.items()}
+    rs = o.advance(ref, steps)
+    run = api.Run.from_numpy(inp, st)
+    gs = run.advance(steps)
+    got = run.fetch()
+    assert rel(got["u"], ref["u"]) < 1e-10 and rel(got["v"], ref["v"]) < 1e-10
+    assert rel(got["uprev"], ref["uprev"]) < 1e-10
+    p, pr = got["ctx"][0, 1:-1, 1:-1], ref["ctx"][0, 1:-1, 1:-1]
+    assert rel(p - p.mean(), pr - pr.mean()) < 1e-10
+    for s, r in zip(gs, rs):
+        assert s["pseudo_iters"] == 0 == r["pseudo_iters"] and s["flags"] == 0
+        assert s["norm_error"] == pytest.approx(r["norm_error"], rel=1e-9)
+        assert s["div_new"] == pytest.approx(r["div_new"], rel=1e-6, abs=1e-9 * r["div_star"])
+    assert np.abs(got["u"][-1, [0, -1]]).min() > 0  # the lid row's wall faces moved, as in the reference
+
+
+def test_explicit_time_marching_refuses_tape_and_two_level():
+    api = _api()
+    run = api.Run.from_numpy(api.cavity_inputs(32, 100.0, PSEUDO_TIMESTEPPING=0), W.initial_state(32, "zero"))
+    with pytest.raises(api.AmressifError) as e:
+        run.ctx.tape(2)
+    assert e.value.code == -3
+    with pytest.raises(api.AmressifError) as e:
+        api.TwoLevel(api.cavity_inputs(32, 100.0, PSEUDO_TIMESTEPPING=0), 24, W.seeded_two_level(32, 24))
+    assert e.value.code == -3

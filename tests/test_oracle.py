@@ -449,3 +449,60 @@ def test_lid_profile_is_smooth_and_seeded():
     assert g.shape == (512,) and abs(g.mean() - 1.0) < 0.02
     assert np.abs(np.diff(g)).max() < 0.05
     assert np.array_equal(g, W.lid_profile(512))
+
+
+# ------------------------------------- PSEUDO_TIMESTEPPING = 0 (explicit)
+def test_explicit_step_from_rest_leaks_the_lid_row_wall_faces():
+    """main.cpp:900-926 + momentum.cpp:96-191.  At rest under a moving lid only
+    the lid row's viscous flux is non-zero: T_U = 2/(h^2 Re) there (the north
+    ghost is 2 - 0).  This branch runs no wall-ghost pass on fluxTotal, so the
+    two wall faces of that row get R = (0 + T_U)/2 instead of 0, and the
+    explicit update u* = u^n + (dt/1.5) R leaves them at dt/(1.5 h^2 Re), which
+    the projection (G phi = 0 on walls) keeps.  Every other wall face stays 0;
+    countIter stays 0."""
+    n, ren, dt = 16, 50.0, 1e-3
+    h = 1.0 / n
+    o = Oracle(n, ren, dt, poisson="dct", pseudo=0)
+    st = W.initial_state(n, "zero")
+    stats = o.advance(st, 1)
+    assert stats[0]["pseudo_iters"] == 0 and stats[0]["flags"] == 0
+    u = st["u"]
+    leak = dt / (1.5 * h * h * ren)
+    assert np.allclose(u[-1, [0, -1]], leak, rtol=1e-13, atol=0)
+    assert np.all(u[:-1, [0, -1]] == 0.0) and np.all(st["v"][[0, -1], :] == 0.0)
+    # the RK4 dual-time branch keeps every wall face at 0
+    st = W.initial_state(n, "zero")
+    Oracle(n, ren, dt, poisson="dct").advance(st, 1)
+    assert np.all(st["u"][:, [0, -1]] == 0.0)
+
+
+def test_explicit_step_history_term_closed_form():
+    """With u^n = 0, a resting lid and p = 0 every flux is 0, so
+    u* = (dt/1.5) (u^n - u^{n-1})/(2 dt) = -u^{n-1}/3; for a divergence-free
+    u^{n-1} the projection leaves it, and the normError is the RMS of u*."""
+    n = 24
+    o = Oracle(n, 100.0, 2e-3, poisson="dct", pseudo=0, bc_hi=(135, 135), inflow=(0.0, 0.0))
+    st = W.initial_state(n, "zero")
+    up, vp = W.seeded_velocity(n)
+    st["uprev"], st["vprev"] = up.copy(), vp.copy()
+    stats = o.advance(st, 1)
+    scale = np.abs(up).max()
+    assert np.abs(st["u"] + up / 3).max() <= 1e-12 * scale and np.abs(st["v"] + vp / 3).max() <= 1e-12 * scale
+    ref = Oracle(n, 100.0, 2e-3).error_computation(np.zeros_like(up), np.zeros_like(vp), -up / 3, -vp / 3)
+    assert stats[0]["norm_error"] == pytest.approx(ref, rel=1e-12)
+    assert np.array_equal(st["uprev"], np.zeros_like(up))
+
+
+@pytest.mark.parametrize("n", [130, 98])
+def test_multigrid_with_an_odd_bottom_matches_exact_solve(n):
+    """n = 130 coarsens once to an odd 65 x 65 bottom (49 x 49 for n = 98),
+    too large for the dense pseudo-inverse; that bottom is solved by the DCT.
+    The V-cycle iteration converges to the same mean-zero solution as the
+    exact solve of the whole grid."""
+    rng = np.random.default_rng(n)
+    rhs = np.zeros((n + 2, n + 2))
+    rhs[1:-1, 1:-1] = rng.standard_normal((n, n))
+    xe, _, _ = Oracle(n, 100.0, 1e-4, poisson="dct").poisson_solve(rhs)
+    xm, it, res = Oracle(n, 100.0, 1e-4, poisson="mg", poisson_tol_rel=1e-12).poisson_solve(rhs)
+    assert res <= 1e-12 and 2 <= it <= 30
+    assert np.allclose(xm, xe, rtol=0, atol=1e-9 * np.abs(xe).max())

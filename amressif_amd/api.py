@@ -164,6 +164,7 @@ _sig = {
     "amressif_poisson_solve": (_c.c_int, [_c.c_void_p, _A4, _A4, _c.c_double, _c.c_double, _P(_c.c_int),
                                           _P(_c.c_double)]),
     "amressif_project": (_c.c_int, [_c.c_void_p, _A4, _A4, _A4, _P(_c.c_double)]),
+    "amressif_poisson_rhs_noalloc": (_c.c_int, [_c.c_void_p, _A4, _A4, _c.c_void_p]),
     "amressif_time_kernel": (_c.c_int, [_c.c_void_p, _c.c_int, _c.c_int, _c.c_int64, _P(_c.c_double),
                                         _P(_c.c_double)]),
     "amressif_kernel_timing": (_c.c_int, [_c.c_void_p, _c.c_int]),

@@ -235,6 +235,16 @@ int amressif_poisson_rhs(amressif_ctx *ctx, const amr_array4 vel[2], amr_array4 
  * ||r||inf <= max(tol_abs, tol_rel ||rhs - mean||inf); sol returned mean-zero. */
 int amressif_poisson_solve(amressif_ctx *ctx, amr_array4 *sol, const amr_array4 *rhs, double tol_rel,
                            double tol_abs, int *iters, double *res_rel);
+/* No counterpart in the reference: the allocation-free variant of
+ * amressif_poisson_rhs, for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops).  It computes rhs = (1.5/dt)
+ * D(vel) on the caller's views and writes dev_out[0] = ||D||inf and
+ * dev_out[1] = sum rhs to the caller's device buffer (2 doubles), without
+ * synchronising.  Its block partials live in a static device array of 128 KiB
+ * per device (n_cell <= 16384; larger is refused with AMRESSIF_ERR_ARG).
+ * The context's device must be current.  Calls that overlap on one device
+ * share that array and race. */
+int amressif_poisson_rhs_noalloc(amressif_ctx *ctx, const amr_array4 vel[2], amr_array4 *rhs, double *dev_out);
 /* update_solution (poisson.cpp:192-392) with phi from userCtx comp 1:
  * velCont = velStar - G phi dt/1.5, p += phi.  Returns ||D(velCont)||inf. */
 int amressif_project(amressif_ctx *ctx, const amr_array4 velStar[2], amr_array4 *userCtx,

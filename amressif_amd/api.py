@@ -459,7 +459,8 @@ class Context:
         return ms.value, b.value
 
     # in-step kernel timing (HIP events around the step's main launches)
-    KERNELS = ("stage", "stage_last", "resid", "tangent", "mg_down", "mg_up", "project", "poisson_start")
+    KERNELS = ("stage", "stage_last", "resid", "tangent", "mg_down", "mg_up", "project", "poisson_start", "pair",
+               "pair_last")
 
     def kernel_timing(self, enable: bool) -> None:
         """start (clearing the totals) or stop per-launch timing inside advance()"""

@@ -509,6 +509,25 @@ int launch_pseudo_iteration(amressif_ctx *ctx, int it, int *nb)
     amr::StageLaunch a{};
     a.ub = c.U[0]; a.vb = c.U[1]; a.cu = c.C[0]; a.cv = c.C[1]; a.p = c.p;
     a.foldp = 1;  // C holds -G p (amr_momentum_begin)
+    if (c.stage_pair && amr::stage_pair_fits(c)) {
+        // stages 1-2 from u^k into S2, stages 3-4 from S2 into S1, then S1 becomes
+        // u^k (a launch skipped after convergence copies u^k to S1)
+        const double cells = (double)c.g.n * c.g.n;
+        a.us = c.U[0]; a.vs = c.U[1]; a.uo = c.S2[0]; a.vo = c.S2[1];
+        a.rk = c.rk[0]; a.rk2 = c.rk[1]; a.pair = 1;
+        // 64 B per cell: u*, v*, u^k, v^k, c_u, c_v in; u*, v* out
+        HIPCHK(amr::kprof_begin(c, amr::KP_PAIR, amr::KL_MOM, it, 64.0 * cells));
+        HIPCHK(amr::launch_stage_pair(c, a, nb));
+        HIPCHK(amr::kprof_end(c));
+        a.us = c.S2[0]; a.vs = c.S2[1]; a.uo = c.S1[0]; a.vo = c.S1[1];
+        a.rk = c.rk[2]; a.rk2 = c.rk[3]; a.pair = 2;
+        HIPCHK(amr::kprof_begin(c, amr::KP_PAIR_LAST, amr::KL_MOM, it, 64.0 * cells));
+        HIPCHK(amr::launch_stage_pair(c, a, nb));
+        HIPCHK(amr::kprof_end(c));
+        std::swap(c.U[0], c.S1[0]);
+        std::swap(c.U[1], c.S1[1]);
+        return 0;
+    }
     const double *in_u[4] = {c.U[0], c.S1[0], c.S2[0], c.S1[0]};
     const double *in_v[4] = {c.U[1], c.S1[1], c.S2[1], c.S1[1]};
     double *out_u[4] = {c.S1[0], c.S2[0], c.S1[0], c.U[0]};
@@ -754,6 +773,9 @@ int amressif_create(const amressif_inputs *in, void *hip_stream, amressif_ctx **
         c.tangent_rows = std::max(4, std::min(256, std::atoi(e)));
     if (const char *e = std::getenv("AMRESSIF_STAGE_ORDER")) c.stage_order = std::atoi(e) ? 1 : 0;
     if (const char *e = std::getenv("AMRESSIF_STAGE_ALT")) c.stage_alt = std::atoi(e) ? 1 : 0;
+    if (const char *e = std::getenv("AMRESSIF_STAGE_FAST")) c.stage_fast = std::atoi(e) ? 1 : 0;
+    if (const char *e = std::getenv("AMRESSIF_STAGE_PAIR")) c.stage_pair = std::atoi(e) ? 1 : 0;
+    if (const char *e = std::getenv("AMRESSIF_PAIR_ROWS")) c.pair_rows = std::max(4, std::min(256, std::atoi(e)));
     // and of the multigrid smoothers (kernel choice and rows per wave: 4, 8, 16 or 32)
     if (const char *e = std::getenv("AMRESSIF_MG_SMOOTHER")) c.mg_smoother = std::max(0, std::min(2, std::atoi(e)));
     if (const char *e = std::getenv("AMRESSIF_TAIL_DCT")) c.tail_dct = std::atoi(e) ? 1 : 0;
@@ -1568,9 +1590,10 @@ int amressif_time_kernel(amressif_ctx *ctx, int which, int reps, int64_t copy_by
         HIPCHK(hipMalloc(&src, (size_t)copy_bytes));
         HIPCHK(hipMalloc(&dst, (size_t)copy_bytes));
         HIPCHK(hipMemsetAsync(src, 0, (size_t)copy_bytes, c.stream));
-    } else if (which < 0 || which > 5) {
-        return fail(ctx, AMRESSIF_ERR_ARG, "which must be 0..5");
+    } else if (which < 0 || which > 6) {
+        return fail(ctx, AMRESSIF_ERR_ARG, "which must be 0..6");
     }
+    if (which == 6 && !amr::stage_pair_fits(c)) return fail(ctx, AMRESSIF_ERR_ARG, "no two-stage kernel at this size");
     if ((which == 3 || which == 4) && c.in.precision != AMRESSIF_MIXED)
         return fail(ctx, AMRESSIF_ERR_ARG, "which = 3, 4 need a mixed-precision context");
     auto launch = [&]() -> hipError_t {
@@ -1601,6 +1624,17 @@ int amressif_time_kernel(amressif_ctx *ctx, int which, int reps, int64_t copy_by
             int nb = 0;
             return amr::launch_stage(c, a, &nb);
         }
+        if (which == 6) {  // stages 1-2 in one pass, as inside a step
+            amr::StageLaunch a{};
+            a.us = c.S1[0]; a.vs = c.S1[1]; a.ub = c.U[0]; a.vb = c.U[1];
+            a.cu = c.C[0]; a.cv = c.C[1]; a.p = c.p;
+            a.uo = c.S2[0]; a.vo = c.S2[1];
+            a.rk = c.rk[0]; a.rk2 = c.rk[1]; a.pair = 1;
+            a.foldp = 1;
+            a.unguarded = 1;
+            int nb = 0;
+            return amr::launch_stage_pair(c, a, &nb);
+        }
         if (which == 1 || which == 5) return amr::launch_smooth_fine(c, which, c.S2[0]);
         return amr::launch_copy_bytes(c, src, dst, copy_bytes);
     };
@@ -1629,9 +1663,10 @@ int amressif_time_kernel(amressif_ctx *ctx, int which, int reps, int64_t copy_by
         // 4 tangent-linear update: reads the FP32 base and residual (4 x 4 B), reads and
         //   writes u^k, v^k (4 x 8 B);
         // 5 finest up-leg smoother + FP64 outer step: reads e, r (2 t), the coarse e (t / 4),
-        //   phi, b (2 x 8 B), writes phi (8 B) and the next r (t)
+        //   phi, b (2 x 8 B), writes phi (8 B) and the next r (t);
+        // 6 two FP64 stages in one pass: the stage's 64 B (the first stage stays in registers)
         const double t = c.in.precision == AMRESSIF_MIXED ? 4.0 : 8.0;
-        const double per_cell[6] = {64.0, 2.25 * t, 0.0, 48.0, 48.0, 3.25 * t + 24.0};
+        const double per_cell[7] = {64.0, 2.25 * t, 0.0, 48.0, 48.0, 3.25 * t + 24.0, 64.0};
         *bytes_per_launch = which == 2 ? 2.0 * (double)copy_bytes : per_cell[which] * cells;
     }
     return AMRESSIF_OK;

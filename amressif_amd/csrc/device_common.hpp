@@ -146,21 +146,23 @@ __device__ __forceinline__ void ghost_rule(int side, int code, R um, R vm, R gx,
     }
 }
 
-// values held by lane - 1 / lane + 1 (DPP wave shifts; the edge lanes get 0)
+// values held by lane - 1 / lane + 1 (DPP wave shifts; bound_ctrl zero-fills
+// the edge lanes, so each 32-bit half is one v_mov_b32_dpp with no prior
+// initialisation of the destination)
 __device__ __forceinline__ double lane_prev(double x)
 {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), 0x138, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), 0x138, 0xF, 0xF, false);
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), 0x138, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), 0x138, 0xF, 0xF, true);
     return __hiloint2double(hi, lo);
 }
 __device__ __forceinline__ double lane_next(double x)
 {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), 0x130, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), 0x130, 0xF, 0xF, false);
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), 0x130, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), 0x130, 0xF, 0xF, true);
     return __hiloint2double(hi, lo);
 }
-// (float: bound_ctrl zero-fills the edge lanes, which lets the compiler fold
-// the shift into the consuming VALU instruction)
+// (float: the compiler can also fold the shift into the consuming VALU
+// instruction)
 __device__ __forceinline__ float lane_prev(float x)
 {
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x138, 0xF, 0xF, true));

@@ -97,6 +97,8 @@ enum KProfId {
     KP_MG_UP,        // finest up-leg smoother + FP64 outer step
     KP_PROJECT,      // k_project
     KP_PSTART,       // k_poisson_start
+    KP_PAIR,         // k_stage_pair<false>: FP64 RK stages 1-2
+    KP_PAIR_LAST,    // k_stage_pair<true>: FP64 RK stages 3-4 + norm partials
     KP_COUNT
 };
 enum KProfLoop { KL_MOM = 0, KL_MG, KL_PHASE, KL_ONCE, KL_PROJ };
@@ -143,6 +145,9 @@ struct Ctx {
     int tangent_rows = 16;   // k_tangent: output rows per wave
     int stage_order = 1;     // stage kernels: 1 = consecutive waves take horizontally adjacent bands
     int stage_alt = 1;       // k_stage64: odd row chunks march down
+    int stage_fast = 1;      // k_stage64: unrolled interior path (0: every wave on the general march)
+    int stage_pair = 1;      // FP64 pseudo-iteration as two k_stage_pair launches (0: four k_stage64)
+    int pair_rows = 16;      // k_stage_pair: output rows per wave
     // multigrid smoothers of the levels above the tail: 2 = wave-column kernel
     // with two columns per lane, 1 = one column per lane (both registers +
     // DPP), 0 = LDS tile kernel; output rows per wave of the wave-column
@@ -214,11 +219,18 @@ struct StageLaunch {
     int mode;               // 0 = stage update, 1 = stage update + norm partials, 2 = RHS only
     int unguarded;          // 1: ignore the control block (diagnostic timing)
     int foldp;              // modes 0, 1: cu, cv already hold -G p (launch_fold_gradp); p is not read
+    double rk2;             // launch_stage_pair: the second stage's coefficient
+    int pair;               // launch_stage_pair: 1 = stages 1-2, 2 = stages 3-4
     int expl;               // mode 1 with foldp: PSEUDO_TIMESTEPPING = 0's explicit update (rk = dt/1.5,
                             // base = stencil input = u^n; fluxTotal's domain ghosts 0, so wall faces
                             // take half the adjacent cell's flux)
 };
 hipError_t launch_stage(const Ctx &c, const StageLaunch &a, int *nblocks);
+// two stages in one launch (k_stage_pair): stage rk from us into registers,
+// then stage rk2 from those into uo; pair 1 = stages 1-2, pair 2 = stages 3-4
+// (+ norm partials).  uo must differ from us and ub.
+hipError_t launch_stage_pair(const Ctx &c, const StageLaunch &a, int *nblocks);
+bool stage_pair_fits(const Ctx &c);
 // cu, cv += -G p on the interior faces, once per FP64 step (p fixed over its pseudo-iterations)
 hipError_t launch_fold_gradp(const Ctx &c);
 // mixed (defect-correction) pseudo-iteration: see kernels_stage.hip

@@ -47,6 +47,7 @@ struct StageArgs {
     float *uk, *vk, *ru, *rv;  // FP32 copy of the base and its FP64 residual (stored FP32)
     double *u64, *v64;         // FP64 pseudo base, updated in place by k_tangent
     double cr, cj;             // u^{k+1} = u^k + cr r + cj R'(u^k) r
+    double rk2;                // k_stage_pair: the second stage's coefficient
     const double *lid;
     double *partials;
     const Ctl *ctl;
@@ -59,13 +60,16 @@ struct StageArgs {
     int rows;   // output rows per wave (chunk height)
     int order;  // wave -> tile: 0 band-major (chunks of one band consecutive), 1 chunk-major
     int alt;    // k_stage64: odd chunks march down (shared halo rows read at the same time)
+    int fast;   // k_stage64: interior waves take the unrolled path (rows 8 or 16)
 };
 
 // wave id -> (band, chunk); blocks of 4 waves, handed to the 8 XCDs in
 // contiguous ranges so vertically adjacent chunks share an L2
 __device__ __forceinline__ int wave_id(const StageArgs &)
 {
-    return xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+    // wave-uniform by construction; readfirstlane lets the compiler keep the
+    // tile coordinates, row offsets and wave-level branches in scalar registers
+    return __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * 4 + (int)(threadIdx.x >> 6));
 }
 
 // cell velocities of row jr for this lane's column from raw face loads of the
@@ -103,6 +107,113 @@ __device__ __forceinline__ void finish_cells(const StageArgs &a, int lane, int i
     }
 }
 
+// ============================================ fully unrolled interior march
+// The row march of k_stage64 for a wave whose band and rows touch no wall,
+// ghost cell or array edge: every lane's column and every row of the chunk's
+// stencil (cells j0 - 3 .. j0 + ROWS + 1, y-faces up to j0 + ROWS + 2) lie
+// inside the domain.  With ROWS a compile-time constant the loop unrolls
+// fully, so the rolling window, the carried fluxes and the face rows are
+// register renames rather than copies, every load is unconditional, and each
+// face row is loaded once (u(jc + 3) and v(jc + 4) one iteration ahead of
+// their use; the x-face of a row is reused as the cell velocity of that row).
+// Per face the arithmetic is k_stage64's (FOLDP, no explicit form), so the two
+// paths give identical results.  MODE 0 stage, 1 last stage + norm partials,
+// 3 mixed residual.
+template <int ROWS, int MODE>
+__device__ __forceinline__ void stage_interior(const StageArgs &a, int lane, int i, int j0, int wid)
+{
+    using R = double;
+    const size_t P = a.P;
+    const bool outlane = lane >= 3 && lane < 3 + WB;
+    // i >= 0 here: every access is a uniform row base plus this lane's 32-bit byte offset
+    const unsigned ob = (unsigned)i * 8u;
+    auto at = [&](const double *base, int row) -> const double & {
+        return *(const double *)((const char *)(base + P * row) + ob);
+    };
+    auto put = [&](auto *base, int row, auto val) {
+        using T = decltype(val);
+        *(T *)((char *)(base + P * row) + ob * sizeof(T) / 8u) = val;
+    };
+    const R idx = a.inv_dx, idy = a.inv_dy, idx2 = a.inv_dx2, idy2 = a.inv_dy2, iren = a.inv_ren;
+    const R c15 = a.c15, rk = a.rk;
+    auto U_of = [&](R u) { return R(0.5) * (u + lane_next(u)); };
+    // rows j0 - 3 .. j0 + 1 of u, j0 - 3 .. j0 + 2 of v
+    const double *us = a.us, *vs = a.vs;
+    R u0 = at(us, j0 - 3), u1 = at(us, j0 - 2), u2 = at(us, j0 - 1), u3 = at(us, j0), u4 = at(us, j0 + 1);
+    R v0 = at(vs, j0 - 3), v1 = at(vs, j0 - 2), v2 = at(vs, j0 - 1), v3 = at(vs, j0), v4 = at(vs, j0 + 1),
+      v5 = at(vs, j0 + 2);
+    R wU[5], wV[5];  // cell (i, jc - 2 + q)
+    wU[0] = U_of(u0); wV[0] = R(0.5) * (v0 + v1);
+    wU[1] = U_of(u1); wV[1] = R(0.5) * (v1 + v2);
+    wU[2] = U_of(u2); wV[2] = R(0.5) * (v2 + v3);
+    wU[3] = U_of(u3); wV[3] = R(0.5) * (v3 + v4);
+    // carried rows at the top of iteration jc: uA = u(jc), uB = u(jc + 1), uC = u(jc + 2);
+    // vA = v(jc), vB = v(jc + 1), vC = v(jc + 2), vD = v(jc + 3)
+    R uA = u2, uB = u3, uC = u4, vA = v2, vB = v3, vC = v4, vD = v5;
+    R fsU = quick_flux<R>(vA, wU[0], wU[1], wU[2], wU[3]), fsV = quick_flux<R>(vA, wV[0], wV[1], wV[2], wV[3]);
+    R tVprev = 0;
+    double nx_ = 0.0, ny_ = 0.0;
+    R bu = 0, bcu = 0, bv = 0, bcv = 0;  // base and constant of the row
+#pragma unroll
+    for (int t = 0; t <= ROWS; ++t) {
+        const int jc = j0 - 1 + t;
+        // one iteration ahead: the next iteration's new rows and the next row's base and constant
+        R un = 0, vn = 0, nbu = 0, ncu = 0, nbv = 0, ncv = 0;
+        if (t < ROWS) {
+            un = at(us, jc + 3);
+            vn = at(vs, jc + 4);
+            if (MODE != 3) { nbu = at(a.ub, jc + 1); nbv = at(a.vb, jc + 1); }
+            ncu = at(a.cu, jc + 1); ncv = at(a.cv, jc + 1);
+        }
+        wU[4] = U_of(uC);
+        wV[4] = R(0.5) * (vC + vD);
+        const R fnU = quick_flux<R>(vB, wU[1], wU[2], wU[3], wU[4]);
+        const R fnV = quick_flux<R>(vB, wV[1], wV[2], wV[3], wV[4]);
+        const R Uc = wU[2], Vc = wV[2];
+        const R Um1 = lane_prev(Uc), Um2 = lane_prev(Um1), Up1 = lane_next(Uc);
+        const R Vm1 = lane_prev(Vc), Vm2 = lane_prev(Vm1), Vp1 = lane_next(Vc);
+        const R uw = uA;
+        const R fwU = quick_flux<R>(uw, Um2, Um1, Uc, Up1), fwV = quick_flux<R>(uw, Vm2, Vm1, Vc, Vp1);
+        const R feU = lane_next(fwU), feV = lane_next(fwV);
+        const R CU = (feU - fwU) * idx + (fnU - fsU) * idy;
+        const R CV = (feV - fwV) * idx + (fnV - fsV) * idy;
+        const R VU = ((Up1 - R(2) * Uc + Um1) * idx2 + (wU[3] - R(2) * Uc + wU[1]) * idy2) * iren;
+        const R VV = ((Vp1 - R(2) * Vc + Vm1) * idx2 + (wV[3] - R(2) * Vc + wV[1]) * idy2) * iren;
+        const R TU = (R(0) - CU) + VU, TV = (R(0) - CV) + VV;
+        const R TUw = lane_prev(TU);
+        if (t > 0 && outlane) {  // row jc >= j0: no wall face in this wave
+            const double Rx = R(0.5) * (TUw + TU);
+            const double Ry = R(0.5) * (tVprev + TV);
+            if (MODE == 3) {
+                put(a.uk, jc, (float)uw);
+                put(a.ru, jc, (float)(Rx - c15 * uw + bcu));
+                put(a.vk, jc, (float)vA);
+                put(a.rv, jc, (float)(Ry - c15 * vA + bcv));
+            } else {
+                const double unew = bu + rk * (Rx - c15 * uw + bcu);
+                const double vnew = bv + rk * (Ry - c15 * vA + bcv);
+                put(a.uo, jc, unew);
+                put(a.vo, jc, vnew);
+                if (MODE == 1) {
+                    const double du = bu - unew, dv = bv - vnew;
+                    nx_ += fabs(du * du * 1.0);
+                    ny_ += fabs(dv * dv * 1.0);
+                }
+            }
+        }
+        fsU = fnU; fsV = fnV; tVprev = TV;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { wU[q] = wU[q + 1]; wV[q] = wV[q + 1]; }
+        uA = uB; uB = uC; uC = un;
+        vA = vB; vB = vC; vC = vD; vD = vn;
+        bu = nbu; bcu = ncu; bv = nbv; bcv = ncv;
+    }
+    if (MODE == 1) {
+        for (int o = 32; o > 0; o >>= 1) { nx_ += __shfl_down(nx_, o, 64); ny_ += __shfl_down(ny_, o, 64); }
+        if (lane == 0) { a.partials[2 * wid] = nx_; a.partials[2 * wid + 1] = ny_; }
+    }
+}
+
 // ============================================================ FP64 stage
 // MODE: the launch's a.mode (0 stage, 1 last stage + norm, 2 RHS only, 3 mixed
 // residual) as a template argument, so each form compiles to its own kernel.
@@ -124,6 +235,11 @@ __global__ __launch_bounds__(256) void k_stage64(StageArgs a)
     const int i = i0 + lane - 3;
     const bool xin = i >= 0 && i < n, xface = i >= 0 && i <= n;
     const bool edge = i0 < 3 || i0 + 61 > n;
+    if (FOLDP && !EXPL && MODE != 2 && !edge && j0 >= 3 && a.fast) {
+        // the whole stencil inside the domain: the unrolled path
+        if (a.rows == 8 && j0 + 8 + 2 <= n) { stage_interior<8, MODE>(a, lane, i, j0, wid); return; }
+        if (a.rows == 16 && j0 + 16 + 2 <= n) { stage_interior<16, MODE>(a, lane, i, j0, wid); return; }
+    }
     const bool outlane = lane >= 3 && lane < 3 + WB;
     const double *us = a.us, *vs = a.vs;
     const R gx = (xin && a.lid) ? a.lid[i] : a.inflow0;
@@ -421,6 +537,408 @@ __global__ __launch_bounds__(256) void k_stage64(StageArgs a)
     }
 }
 
+// ================================================= two RK4 stages in one pass
+// Stages s and s + 1 of an FP64 pseudo-iteration (the constant holds -G p) in
+// one launch.  A wave owns 52 output columns (a 6-lane halo on each side) and a
+// chunk of rows, and runs two upward marches in one loop.  March A is
+// k_stage64's stage s (stencil input, base and constant from memory) three
+// rows ahead of march B, stage s + 1, which takes A's faces from registers (a
+// delay line of four x-face rows and three y-face rows) instead of memory.  A
+// is exact on lanes 3..60, so B is on lanes 6..57.  The cells B needs outside
+// the domain are the wall ghosts of its own window cells (the ghost of cell row
+// -1 - j or 2n - 1 - j is the ghost rule applied to cell row j).  Per face the
+// arithmetic is k_stage64's; the memory traffic is one stage's: the stencil
+// input, base and constant in and stage s + 1 out, 64 B per cell (B's reads
+// of the base and constant repeat A's three rows later and hit in cache).
+// The output must not alias the stencil input or the base: A reads both in the
+// halo rows and columns that neighbouring waves write.
+//
+// LAST (the iteration's second pair): norm partials of the change, and when
+// the loop has already converged (the launch is skipped) the base is copied to
+// the output, so that the caller's exchange of the two buffers keeps the
+// converged state.
+constexpr int WB2 = 52;  // output columns per wave of k_stage_pair
+
+// The pair for a wave whose whole stencil lies inside the domain (no lane or
+// row of A's march, cells j0 - 6 .. j0 + ROWS + 4, touches a wall, a ghost
+// cell or an array edge), fully unrolled as stage_interior: A runs its rows
+// j0 - 3 .. j0 + ROWS + 2 (t = 1 .. ROWS + 6 after a first iteration without
+// output), B its rows j0 - 1 .. j0 + ROWS - 1 three rows behind (t >= 6).
+template <int ROWS, bool LAST>
+__device__ __forceinline__ void pair_interior(const StageArgs &a, int lane, int i, int j0, int wid)
+{
+    using R = double;
+    const size_t P = a.P;
+    const bool outB = lane >= 6 && lane <= 57;
+    const unsigned ob = (unsigned)i * 8u;
+    auto at = [&](const double *base, int row) -> const double & {
+        return *(const double *)((const char *)(base + P * row) + ob);
+    };
+    auto put = [&](double *base, int row, double val) { *(double *)((char *)(base + P * row) + ob) = val; };
+    const R idx = a.inv_dx, idy = a.inv_dy, idx2 = a.inv_dx2, idy2 = a.inv_dy2, iren = a.inv_ren;
+    const R c15 = a.c15, rkA = a.rk, rkB = a.rk2;
+    auto U_of = [&](R u) { return R(0.5) * (u + lane_next(u)); };
+    const double *us = a.us, *vs = a.vs;
+    // the stencil's T = -C + V at cell (i, row) from the row's window (w[1..3] the rows
+    // below, at, above; w[0], w[4] two away), its two y-face fluxes and its west x-face velocity
+    auto T_of = [&](const R *wU, const R *wV, R fsU, R fsV, R fnU, R fnV, R uw, R &TU, R &TV) {
+        const R Uc = wU[2], Vc = wV[2];
+        const R Um1 = lane_prev(Uc), Um2 = lane_prev(Um1), Up1 = lane_next(Uc);
+        const R Vm1 = lane_prev(Vc), Vm2 = lane_prev(Vm1), Vp1 = lane_next(Vc);
+        const R fwU = quick_flux<R>(uw, Um2, Um1, Uc, Up1), fwV = quick_flux<R>(uw, Vm2, Vm1, Vc, Vp1);
+        const R feU = lane_next(fwU), feV = lane_next(fwV);
+        const R CU = (feU - fwU) * idx + (fnU - fsU) * idy;
+        const R CV = (feV - fwV) * idx + (fnV - fsV) * idy;
+        const R VU = ((Up1 - R(2) * Uc + Um1) * idx2 + (wU[3] - R(2) * Uc + wU[1]) * idy2) * iren;
+        const R VV = ((Vp1 - R(2) * Vc + Vm1) * idx2 + (wV[3] - R(2) * Vc + wV[1]) * idy2) * iren;
+        TU = (R(0) - CU) + VU;
+        TV = (R(0) - CV) + VV;
+    };
+    // ---- A: rows j0 - 6 .. j0 - 2 of u, j0 - 6 .. j0 - 1 of v
+    const int jA = j0 - 3;  // A's first output row
+    R aU[5], aV[5];
+    {
+        const R u0 = at(us, jA - 3), u1 = at(us, jA - 2), u2 = at(us, jA - 1), u3 = at(us, jA);
+        const R v0 = at(vs, jA - 3), v1 = at(vs, jA - 2), v2 = at(vs, jA - 1), v3 = at(vs, jA), v4 = at(vs, jA + 1);
+        aU[0] = U_of(u0); aV[0] = R(0.5) * (v0 + v1);
+        aU[1] = U_of(u1); aV[1] = R(0.5) * (v1 + v2);
+        aU[2] = U_of(u2); aV[2] = R(0.5) * (v2 + v3);
+        aU[3] = U_of(u3); aV[3] = R(0.5) * (v3 + v4);
+    }
+    R uA = at(us, jA - 1), uB = at(us, jA), uC = at(us, jA + 1);
+    R vA = at(vs, jA - 1), vB = at(vs, jA), vC = at(vs, jA + 1), vD = at(vs, jA + 2);
+    R afsU = quick_flux<R>(vA, aU[0], aU[1], aU[2], aU[3]), afsV = quick_flux<R>(vA, aV[0], aV[1], aV[2], aV[3]);
+    R atV = 0;
+    R pu = 0, pcu = 0, pv = 0, pcv = 0;  // A's base and constant of its row
+    // ---- B
+    R bU[5], bV[5];
+    R x0 = 0, x1 = 0, x2 = 0, x3 = 0, y0 = 0, y1 = 0, y2 = 0, y3 = 0;  // A's rows j0 - 3 .. j0
+    R uq0 = 0, uq1 = 0, uq2 = 0, vq1 = 0, vq2 = 0, bvn = 0;           // as in k_stage_pair
+    R bfsU = 0, bfsV = 0, btV = 0;
+    double nx_ = 0.0, ny_ = 0.0;
+#pragma unroll
+    for (int t = 0; t <= ROWS + 6; ++t) {
+        // keep each iteration's instructions in it: the prefetch below is the
+        // only look-ahead, so the live registers stay those of one iteration
+        __builtin_amdgcn_sched_barrier(0);
+        const int jc = jA - 1 + t;  // A's row
+        const int jb = jc - 3;      // B's row (t >= 6)
+        R bub = 0, bcu = 0, bvb = 0, bcv = 0;
+        if (t >= 7) {  // B's base and constant (read by A three rows ago)
+            bub = at(a.ub, jb); bcu = at(a.cu, jb); bvb = at(a.vb, jb); bcv = at(a.cv, jb);
+        }
+        R un = 0, vn = 0, nbu = 0, ncu = 0, nbv = 0, ncv = 0;
+        if (t < ROWS + 6) {  // A's next iteration
+            un = at(us, jc + 3);
+            vn = at(vs, jc + 4);
+            nbu = at(a.ub, jc + 1); ncu = at(a.cu, jc + 1); nbv = at(a.vb, jc + 1); ncv = at(a.cv, jc + 1);
+        }
+        // A, row jc
+        aU[4] = U_of(uC);
+        aV[4] = R(0.5) * (vC + vD);
+        const R fnU = quick_flux<R>(vB, aU[1], aU[2], aU[3], aU[4]);
+        const R fnV = quick_flux<R>(vB, aV[1], aV[2], aV[3], aV[4]);
+        R TU, TV;
+        T_of(aU, aV, afsU, afsV, fnU, fnV, uA, TU, TV);
+        const R TUw = lane_prev(TU);
+        const R eu = pu + rkA * (R(0.5) * (TUw + TU) - c15 * uA + pcu);   // A's x-face (i, jc)
+        const R ev = pv + rkA * (R(0.5) * (atV + TV) - c15 * vA + pcv);   // A's y-face (i, jc)
+        afsU = fnU; afsV = fnV; atV = TV;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { aU[q] = aU[q + 1]; aV[q] = aV[q + 1]; }
+        uA = uB; uB = uC; uC = un;
+        vA = vB; vB = vC; vC = vD; vD = vn;
+        pu = nbu; pcu = ncu; pv = nbv; pcv = ncv;
+        __builtin_amdgcn_sched_barrier(0);  // A's temporaries end here
+        // B
+        if (t == 1) { x0 = eu; y0 = ev; }
+        if (t == 2) { x1 = eu; y1 = ev; bU[0] = U_of(x0); bV[0] = R(0.5) * (y0 + y1); }
+        if (t == 3) { x2 = eu; y2 = ev; bU[1] = U_of(x1); bV[1] = R(0.5) * (y1 + y2); }
+        if (t == 4) { x3 = eu; y3 = ev; bU[2] = U_of(x2); bV[2] = R(0.5) * (y2 + y3); }
+        if (t == 5) {  // row j0 + 1: B's prologue is complete
+            bU[3] = U_of(x3); bV[3] = R(0.5) * (y3 + ev);
+            bvn = y2;
+            uq0 = x2; uq1 = x3; uq2 = eu; vq1 = y3; vq2 = ev;
+            bfsU = quick_flux<R>(bvn, bU[0], bU[1], bU[2], bU[3]);
+            bfsV = quick_flux<R>(bvn, bV[0], bV[1], bV[2], bV[3]);
+        }
+        if (t >= 6) {  // row jb + 3 from A; B's row jb
+            const R uq3 = eu, vq3 = ev;
+            bU[4] = U_of(uq2);
+            bV[4] = R(0.5) * (vq2 + vq3);
+            const R gnU = quick_flux<R>(vq1, bU[1], bU[2], bU[3], bU[4]);
+            const R gnV = quick_flux<R>(vq1, bV[1], bV[2], bV[3], bV[4]);
+            R SU, SV;
+            T_of(bU, bV, bfsU, bfsV, gnU, gnV, uq0, SU, SV);
+            const R SUw = lane_prev(SU);
+            if (t >= 7 && outB) {  // B's row jb >= j0: no wall face in this wave
+                const double unew = bub + rkB * (R(0.5) * (SUw + SU) - c15 * uq0 + bcu);
+                const double vnew = bvb + rkB * (R(0.5) * (btV + SV) - c15 * bvn + bcv);
+                put(a.uo, jb, unew);
+                put(a.vo, jb, vnew);
+                if (LAST) {
+                    const double du = bub - unew, dv = bvb - vnew;
+                    nx_ += fabs(du * du * 1.0);
+                    ny_ += fabs(dv * dv * 1.0);
+                }
+            }
+            bfsU = gnU; bfsV = gnV; btV = SV; bvn = vq1;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) { bU[q] = bU[q + 1]; bV[q] = bV[q + 1]; }
+            uq0 = uq1; uq1 = uq2; uq2 = uq3;
+            vq1 = vq2; vq2 = vq3;
+        }
+    }
+    if (LAST) {
+        for (int o = 32; o > 0; o >>= 1) { nx_ += __shfl_down(nx_, o, 64); ny_ += __shfl_down(ny_, o, 64); }
+        if (lane == 0) { a.partials[2 * wid] = nx_; a.partials[2 * wid + 1] = ny_; }
+    }
+}
+
+template <bool LAST>
+__global__ __launch_bounds__(256) void k_stage_pair(StageArgs a)
+{
+    using R = double;
+    const int lane = threadIdx.x & 63;
+    const int wid = wave_id(a);
+    if (wid >= a.nbands * a.nchunks) return;  // whole wave
+    const int band = a.order ? wid % a.nbands : wid / a.nchunks;
+    const int chunk = a.order ? wid / a.nbands : wid % a.nchunks;
+    const int n = a.n, P = a.P;
+    const int i0 = band * WB2, j0 = chunk * a.rows;
+    const int i = i0 + lane - 6;
+    const bool xin = i >= 0 && i < n, xface = i >= 0 && i <= n;
+    const bool outA = lane >= 3 && lane <= 60, outB = lane >= 6 && lane <= 57;
+    const int jend = min(j0 + a.rows, n);
+    const size_t ic = (size_t)(i < 0 ? 0 : i);
+    if (a.ctl && a.ctl->mom_done) {  // the same for every wave of the launch
+        if (LAST && outB) {
+            for (int j = j0; j < jend; ++j) {
+                const size_t k = ic + (size_t)P * j;
+                if (xface) a.uo[k] = a.ub[k];
+                if (xin) a.vo[k] = a.vb[k];
+            }
+            if (jend == n && xin) a.vo[ic + (size_t)P * n] = a.vb[ic + (size_t)P * n];
+        }
+        return;
+    }
+    const bool edge = i0 < 6 || i0 + 58 > n;
+    if (!edge && j0 >= 6 && a.fast) {  // the whole stencil inside the domain: the unrolled path
+        if (a.rows == 16 && j0 + 16 + 5 <= n) { pair_interior<16, LAST>(a, lane, i, j0, wid); return; }
+        if (a.rows == 12 && j0 + 12 + 5 <= n) { pair_interior<12, LAST>(a, lane, i, j0, wid); return; }
+    }
+    const double *us = a.us, *vs = a.vs;
+    const R gx = (xin && a.lid) ? a.lid[i] : a.inflow0;
+    const R idx = a.inv_dx, idy = a.inv_dy, idx2 = a.inv_dx2, idy2 = a.inv_dy2, iren = a.inv_ren;
+    const int ja0 = max(j0 - 3, 0), jaEnd = min(jend + 3, n);
+
+    // ---------------------------------------------------------------- march A
+    struct Raw {
+        R u0, v0, v1, vn, uw, ub, cu, vb, cv;  // as k_stage64's (u(i + 1) = u0 of lane + 1)
+    };
+    auto loadA = [&](int jc, Raw &r) {  // everything A's iteration jc consumes
+        const int jr = jc + 2;
+        const int jm = jr < 0 ? -jr - 1 : (jr >= n ? 2 * n - 1 - jr : jr);
+        r.u0 = r.v0 = r.v1 = 0;
+        if (jm >= 0 && jm < n && xface) {
+            const size_t k = i + (size_t)P * jm;
+            r.u0 = us[k];
+            if (xin) { r.v0 = vs[k]; r.v1 = vs[k + P]; }
+        }
+        r.vn = (xin && jc + 1 >= 0 && jc + 1 <= n) ? vs[i + (size_t)P * (jc + 1)] : R(0);
+        const bool row = jc >= 0 && jc < n;
+        const size_t k = ic + (size_t)P * (jc < 0 ? 0 : jc);
+        r.uw = (xface && row) ? us[k] : R(0);
+        const bool out = row && jc >= ja0 && outA;
+        r.ub = (out && xface) ? a.ub[k] : R(0);
+        r.cu = (out && xface) ? a.cu[k] : R(0);
+        r.vb = (out && xin) ? a.vb[k] : R(0);
+        r.cv = (out && xin) ? a.cv[k] : R(0);
+    };
+    R aU[5], aV[5];  // A's rolling window: cell (i, ja - 2 + q)
+    {
+        Raw r;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            loadA(ja0 - 5 + q, r);
+            finish_cells<R, false>(a, lane, i, ja0 - 3 + q, edge, r.u0, lane_next(r.u0), r.v0, r.v1, gx, aU[q], aV[q]);
+        }
+    }
+    R avn = (xin && ja0 - 1 >= 0) ? vs[i + (size_t)P * (ja0 - 1)] : R(0);
+    R afsU = quick_flux<R>(avn, aU[0], aU[1], aU[2], aU[3]), afsV = quick_flux<R>(avn, aV[0], aV[1], aV[2], aV[3]);
+    R atV = 0;
+    Raw acur, anxt{};
+    loadA(ja0 - 1, acur);
+    int ja = ja0 - 1;  // A's next iteration
+    // the north wall row of y-faces (base and constant, shared by A and B)
+    R vbN = 0, cvN = 0;
+    if (jend + 2 >= n && xin) {
+        vbN = a.vb[ic + (size_t)P * n];
+        cvN = a.cv[ic + (size_t)P * n];
+    }
+    // A's iteration ja (ja < n): its x-face and y-face of row ja (0 off the domain)
+    auto stepA = [&](R &uo, R &vo) {
+        if (ja + 1 < jaEnd) loadA(ja + 1, anxt);  // prefetch
+        finish_cells<R, false>(a, lane, i, ja + 2, edge, acur.u0, lane_next(acur.u0), acur.v0, acur.v1, gx, aU[4],
+                               aV[4]);
+        const R vnf = acur.vn;
+        const R fnU = quick_flux<R>(vnf, aU[1], aU[2], aU[3], aU[4]);
+        const R fnV = quick_flux<R>(vnf, aV[1], aV[2], aV[3], aV[4]);
+        const R Uc = aU[2], Vc = aV[2];
+        const R Um1 = lane_prev(Uc), Um2 = lane_prev(Um1), Up1 = lane_next(Uc);
+        const R Vm1 = lane_prev(Vc), Vm2 = lane_prev(Vm1), Vp1 = lane_next(Vc);
+        const R uw = acur.uw;
+        const R fwU = quick_flux<R>(uw, Um2, Um1, Uc, Up1), fwV = quick_flux<R>(uw, Vm2, Vm1, Vc, Vp1);
+        const R feU = lane_next(fwU), feV = lane_next(fwV);
+        const R CU = (feU - fwU) * idx + (fnU - afsU) * idy;
+        const R CV = (feV - fwV) * idx + (fnV - afsV) * idy;
+        const R VU = ((Up1 - R(2) * Uc + Um1) * idx2 + (aU[3] - R(2) * Uc + aU[1]) * idy2) * iren;
+        const R VV = ((Vp1 - R(2) * Vc + Vm1) * idx2 + (aV[3] - R(2) * Vc + aV[1]) * idy2) * iren;
+        const R TU = (R(0) - CU) + VU, TV = (R(0) - CV) + VV;
+        const R TUw = lane_prev(TU);
+        const double Rx = (i == 0 || i == n) ? 0.0 : R(0.5) * (TUw + TU);
+        uo = xface ? acur.ub + a.rk * (Rx - a.c15 * uw + acur.cu) : R(0);
+        const double Ry = ja == 0 ? 0.0 : R(0.5) * (atV + TV);
+        vo = xin ? acur.vb + a.rk * (Ry - a.c15 * avn + acur.cv) : R(0);
+        afsU = fnU; afsV = fnV; atV = TV; avn = vnf;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { aU[q] = aU[q + 1]; aV[q] = aV[q + 1]; }
+        acur = anxt;
+        ++ja;
+    };
+    // A's next row: a stage row below n, the north wall row at n (R = 0 there)
+    auto emitA = [&](R &uo, R &vo) {
+        if (ja < n) {
+            stepA(uo, vo);
+        } else {
+            uo = 0;
+            vo = (ja == n && xin) ? vbN + a.rk * (0.0 - a.c15 * avn + cvN) : R(0);
+            ++ja;
+        }
+    };
+
+    // ---------------------------------------------------------------- march B
+    // cell row jr in [0, n) from A's faces
+    auto cellB = [&](int jr, R u, R v0, R v1, R &U, R &V) {
+        finish_cells<R, false>(a, lane, i, jr, edge, u, lane_next(u), v0, v1, gx, U, V);
+    };
+    // the wall ghost of cell row jr (outside [0, n)) from its mirror cell
+    auto ghostB = [&](int jr, R Um, R Vm, R &U, R &V) {
+        U = Um;
+        V = Vm;
+        if (xin) ghost_rule<R, false>(jr < 0 ? 2 : 3, jr < 0 ? a.bc_lo1 : a.bc_hi1, Um, Vm, gx, (R)a.inflow1, U, V);
+    };
+    R bU[5], bV[5];             // B's rolling window: cell (i, jb - 2 + q)
+    R uq0, uq1, uq2, uq3 = 0;   // A's x-faces of rows jb .. jb + 3
+    R vq1, vq2, vq3 = 0;        // A's y-faces of rows jb + 1 .. jb + 3
+    R bvn, bfsU, bfsV, btV = 0; // A's y-face of row jb; B's flux through it; TV of row jb - 1
+    int jb;
+    {
+        R t0, t1;
+        stepA(t0, t1);  // A's iteration ja0 - 1 (no row)
+    }
+    if (j0 == 0) {
+        // rows 0..2; B starts at row 0 (its south y-faces are the wall)
+        R x0, y0, x1, y1, x2, y2;
+        emitA(x0, y0);
+        emitA(x1, y1);
+        emitA(x2, y2);
+        cellB(0, x0, y0, y1, bU[2], bV[2]);
+        cellB(1, x1, y1, y2, bU[3], bV[3]);
+        ghostB(-2, bU[3], bV[3], bU[0], bV[0]);
+        ghostB(-1, bU[2], bV[2], bU[1], bV[1]);
+        bvn = y0;
+        uq0 = x0; uq1 = x1; uq2 = x2; vq1 = y1; vq2 = y2;
+        jb = 0;
+    } else {
+        // rows j0 - 3 .. j0 + 1 (j0 >= 3); B starts at row j0 - 1
+        R x0, y0, x1, y1, x2, y2, x3, y3, x4, y4;
+        emitA(x0, y0);
+        emitA(x1, y1);
+        cellB(j0 - 3, x0, y0, y1, bU[0], bV[0]);
+        emitA(x2, y2);
+        cellB(j0 - 2, x1, y1, y2, bU[1], bV[1]);
+        emitA(x3, y3);
+        cellB(j0 - 1, x2, y2, y3, bU[2], bV[2]);
+        emitA(x4, y4);
+        cellB(j0, x3, y3, y4, bU[3], bV[3]);
+        bvn = y2;
+        uq0 = x2; uq1 = x3; uq2 = x4; vq1 = y3; vq2 = y4;
+        jb = j0 - 1;
+    }
+    bfsU = quick_flux<R>(bvn, bU[0], bU[1], bU[2], bU[3]);
+    bfsV = quick_flux<R>(bvn, bV[0], bV[1], bV[2], bV[3]);
+    struct RawB {
+        R ub, cu, vb, cv;
+    };
+    auto loadB = [&](int jc, RawB &r) {  // B's base and constant of row jc
+        const bool out = jc >= j0 && jc < jend && outB;
+        const size_t k = ic + (size_t)P * (jc < 0 ? 0 : jc);
+        r.ub = (out && xface) ? a.ub[k] : R(0);
+        r.cu = (out && xface) ? a.cu[k] : R(0);
+        r.vb = (out && xin) ? a.vb[k] : R(0);
+        r.cv = (out && xin) ? a.cv[k] : R(0);
+    };
+    RawB bcur, bnxt{};
+    loadB(jb, bcur);
+    double nx_ = 0.0, ny_ = 0.0;
+    for (; jb < jend; ++jb) {
+        if (jb + 1 < jend) loadB(jb + 1, bnxt);  // prefetch
+        emitA(uq3, vq3);                          // row jb + 3
+        const int jr = jb + 2;                    // B's new window cell
+        if (jr < n) cellB(jr, uq2, vq2, vq3, bU[4], bV[4]);
+        else if (jr == n) ghostB(jr, bU[3], bV[3], bU[4], bV[4]);  // mirror of row n - 1
+        else ghostB(jr, bU[1], bV[1], bU[4], bV[4]);               // row n + 1: mirror of row n - 2
+        const R vnf = vq1;
+        const R fnU = quick_flux<R>(vnf, bU[1], bU[2], bU[3], bU[4]);
+        const R fnV = quick_flux<R>(vnf, bV[1], bV[2], bV[3], bV[4]);
+        const R Uc = bU[2], Vc = bV[2];
+        const R Um1 = lane_prev(Uc), Um2 = lane_prev(Um1), Up1 = lane_next(Uc);
+        const R Vm1 = lane_prev(Vc), Vm2 = lane_prev(Vm1), Vp1 = lane_next(Vc);
+        const R uw = uq0;
+        const R fwU = quick_flux<R>(uw, Um2, Um1, Uc, Up1), fwV = quick_flux<R>(uw, Vm2, Vm1, Vc, Vp1);
+        const R feU = lane_next(fwU), feV = lane_next(fwV);
+        const R CU = (feU - fwU) * idx + (fnU - bfsU) * idy;
+        const R CV = (feV - fwV) * idx + (fnV - bfsV) * idy;
+        const R VU = ((Up1 - R(2) * Uc + Um1) * idx2 + (bU[3] - R(2) * Uc + bU[1]) * idy2) * iren;
+        const R VV = ((Vp1 - R(2) * Vc + Vm1) * idx2 + (bV[3] - R(2) * Vc + bV[1]) * idy2) * iren;
+        const R TU = (R(0) - CU) + VU, TV = (R(0) - CV) + VV;
+        const R TUw = lane_prev(TU);
+        if (jb >= j0 && outB) {
+            const size_t k = ic + (size_t)P * jb;
+            if (xface) {  // x-face (i, jb); wall faces get R = 0 exactly
+                const bool wall = i == 0 || i == n;
+                const double Rx = wall ? 0.0 : R(0.5) * (TUw + TU);
+                const double unew = bcur.ub + a.rk2 * (Rx - a.c15 * uw + bcur.cu);
+                a.uo[k] = unew;
+                if (LAST) { const double d = bcur.ub - unew; nx_ += fabs(d * d * (wall ? 0.5 : 1.0)); }
+            }
+            if (xin) {  // y-face (i, jb)
+                const bool wall = jb == 0;
+                const double Ry = wall ? 0.0 : R(0.5) * (btV + TV);
+                const double vnew = bcur.vb + a.rk2 * (Ry - a.c15 * bvn + bcur.cv);
+                a.vo[k] = vnew;
+                if (LAST) { const double d = bcur.vb - vnew; ny_ += fabs(d * d * (wall ? 0.5 : 1.0)); }
+            }
+        }
+        bfsU = fnU; bfsV = fnV; btV = TV; bvn = vnf;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { bU[q] = bU[q + 1]; bV[q] = bV[q + 1]; }
+        uq0 = uq1; uq1 = uq2; uq2 = uq3;
+        vq1 = vq2; vq2 = vq3;
+        bcur = bnxt;
+    }
+    if (jend == n && outB && xin) {  // north wall row of y-faces: R = 0
+        const double vnew = vbN + a.rk2 * (0.0 - a.c15 * bvn + cvN);
+        a.vo[ic + (size_t)P * n] = vnew;
+        if (LAST) { const double d = vbN - vnew; ny_ += fabs(d * d * 0.5); }
+    }
+    if (LAST) {
+        for (int o = 32; o > 0; o >>= 1) { nx_ += __shfl_down(nx_, o, 64); ny_ += __shfl_down(ny_, o, 64); }
+        if (lane == 0) { a.partials[2 * wid] = nx_; a.partials[2 * wid + 1] = ny_; }
+    }
+}
+
 // ================================================= FP32 tangent-linear step
 // Tangent-linear flux: derivative of QUICK(vel, X) in direction (dvel, dX)
 // with the upwind choice frozen at vel: vel*phi(dX) + dvel*phi(X).
@@ -571,6 +1089,7 @@ static StageArgs base_args(const Ctx &c, double rk, int mode, int rows)
     a.rows = rows;
     a.order = c.stage_order;
     a.alt = c.stage_alt;
+    a.fast = c.stage_fast;
     a.nbands = (g.n + 1 + WB - 1) / WB;
     a.nchunks = (g.n + a.rows - 1) / a.rows;
     return a;
@@ -607,6 +1126,31 @@ hipError_t launch_stage(const Ctx &c, const StageLaunch &s, int *nblocks)
     case 2: hipLaunchKernelGGL((k_stage64<2, false>), grid, dim3(256), 0, c.stream, a); break;
     default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+bool stage_pair_fits(const Ctx &c)
+{
+    const int nb = (c.g.n + 1 + WB2 - 1) / WB2, nc = (c.g.n + c.pair_rows - 1) / c.pair_rows;
+    return c.g.n >= 8 && c.pair_rows >= 4 && (int64_t)nb * nc <= kMaxPartials;
+}
+
+hipError_t launch_stage_pair(const Ctx &c, const StageLaunch &s, int *nblocks)
+{
+    if (s.pair != 1 && s.pair != 2) return hipErrorInvalidValue;
+    if (!s.foldp || !stage_pair_fits(c)) return hipErrorInvalidValue;
+    if (s.uo == s.us || s.uo == s.ub || s.vo == s.vs || s.vo == s.vb) return hipErrorInvalidValue;
+    StageArgs a = base_args(c, s.rk, s.pair == 2 ? 1 : 0, c.pair_rows);
+    a.rk2 = s.rk2;
+    a.us = s.us; a.vs = s.vs; a.ub = s.ub; a.vb = s.vb; a.cu = s.cu; a.cv = s.cv; a.p = s.p;
+    a.uo = s.uo; a.vo = s.vo;
+    if (s.unguarded) a.ctl = nullptr;
+    a.nbands = (c.g.n + 1 + WB2 - 1) / WB2;
+    a.nchunks = (c.g.n + a.rows - 1) / a.rows;
+    const dim3 grid = stage_grid(a);
+    if (nblocks) *nblocks = a.nbands * a.nchunks;  // one norm partial per wave
+    if (s.pair == 2) hipLaunchKernelGGL(k_stage_pair<true>, grid, dim3(256), 0, c.stream, a);
+    else hipLaunchKernelGGL(k_stage_pair<false>, grid, dim3(256), 0, c.stream, a);
     return hipGetLastError();
 }
 

@@ -349,6 +349,7 @@ int amressif_amr_advance(amressif_amr *amr, int nsteps, amressif_amr_stats *stat
  *   which = 3  mixed precision: FP64 residual of the pseudo base
  *   which = 4  mixed precision: FP32 tangent-linear update of the pseudo base
  *   which = 5  finest-level up-leg smoother with the fused FP64 outer step (V-cycle precision)
+ *   which = 6  two FP64 RK stages in one pass (stages 1-2 of a pseudo-iteration)
  * which = 3 and 4 need a context created with precision = 1.
  * bytes_per_launch receives the kernel's algorithmic HBM bytes per launch. */
 int amressif_time_kernel(amressif_ctx *ctx, int which, int reps, int64_t copy_bytes, double *avg_ms,
@@ -368,6 +369,8 @@ int amressif_time_kernel(amressif_ctx *ctx, int which, int reps, int64_t copy_by
  *   which = 5  finest-level up-leg smoother + FP64 outer step
  *   which = 6  k_project
  *   which = 7  k_poisson_start
+ *   which = 8  FP64 RK stages 1-2 in one pass
+ *   which = 9  FP64 RK stages 3-4 in one pass (+ norm partials)
  * amressif_kernel_time returns the average time per counted launch (0 if none),
  * the number of counted launches and their average algorithmic HBM bytes (the
  * bytes each launch must read and write, counted once per cell). */

@@ -610,6 +610,9 @@ __device__ __forceinline__ void pair_interior(const StageArgs &a, int lane, int 
     R afsU = quick_flux<R>(vA, aU[0], aU[1], aU[2], aU[3]), afsV = quick_flux<R>(vA, aV[0], aV[1], aV[2], aV[3]);
     R atV = 0;
     R pu = 0, pcu = 0, pv = 0, pcv = 0;  // A's base and constant of its row
+    // the same of A's three previous rows (the last is B's row): B takes them
+    // from here rather than loading them again
+    R d0u = 0, d0c = 0, d0v = 0, d0w = 0, d1u = 0, d1c = 0, d1v = 0, d1w = 0, d2u = 0, d2c = 0, d2v = 0, d2w = 0;
     // ---- B
     R bU[5], bV[5];
     R x0 = 0, x1 = 0, x2 = 0, x3 = 0, y0 = 0, y1 = 0, y2 = 0, y3 = 0;  // A's rows j0 - 3 .. j0
@@ -623,10 +626,7 @@ __device__ __forceinline__ void pair_interior(const StageArgs &a, int lane, int 
         __builtin_amdgcn_sched_barrier(0);
         const int jc = jA - 1 + t;  // A's row
         const int jb = jc - 3;      // B's row (t >= 6)
-        R bub = 0, bcu = 0, bvb = 0, bcv = 0;
-        if (t >= 7) {  // B's base and constant (read by A three rows ago)
-            bub = at(a.ub, jb); bcu = at(a.cu, jb); bvb = at(a.vb, jb); bcv = at(a.cv, jb);
-        }
+        const R bub = d2u, bcu = d2c, bvb = d2v, bcv = d2w;  // B's base and constant (A's row jc - 3)
         R un = 0, vn = 0, nbu = 0, ncu = 0, nbv = 0, ncv = 0;
         if (t < ROWS + 6) {  // A's next iteration
             un = at(us, jc + 3);
@@ -648,6 +648,9 @@ __device__ __forceinline__ void pair_interior(const StageArgs &a, int lane, int 
         for (int q = 0; q < 4; ++q) { aU[q] = aU[q + 1]; aV[q] = aV[q + 1]; }
         uA = uB; uB = uC; uC = un;
         vA = vB; vB = vC; vC = vD; vD = vn;
+        d2u = d1u; d2c = d1c; d2v = d1v; d2w = d1w;
+        d1u = d0u; d1c = d0c; d1v = d0v; d1w = d0w;
+        d0u = pu; d0c = pcu; d0v = pv; d0w = pcv;
         pu = nbu; pcu = ncu; pv = nbv; pcv = ncv;
         __builtin_amdgcn_sched_barrier(0);  // A's temporaries end here
         // B

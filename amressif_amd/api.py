@@ -156,7 +156,9 @@ _sig = {
     "amressif_write_checkpoint": (_c.c_int, [_c.c_void_p, _c.c_char_p, _c.c_int, _c.c_double]),
     "amressif_read_checkpoint": (_c.c_int, [_c.c_void_p, _c.c_char_p, _P(_c.c_double)]),
     "amressif_write_plotfile": (_c.c_int, [_c.c_void_p, _c.c_char_p, _c.c_char_p, _c.c_int, _c.c_double]),
+    "amressif_write_flux_plotfile": (_c.c_int, [_c.c_void_p, _c.c_char_p, _c.c_int, _c.c_double]),
     "amressif_cont2cart": (_c.c_int, [_c.c_void_p, _A4, _A4]),
+    "amressif_fluxes": (_c.c_int, [_c.c_void_p, _A4, _A4, _A4, _A4, _A4]),
     "amressif_momentum_rhs": (_c.c_int, [_c.c_void_p, _A4, _A4, _A4]),
     "amressif_rk_stage": (_c.c_int, [_c.c_void_p, _c.c_int, _A4, _A4, _A4, _A4, _A4]),
     "amressif_pseudo_norm": (_c.c_int, [_c.c_void_p, _A4, _A4, _P(_c.c_double)]),
@@ -447,6 +449,11 @@ class Context:
         self._check(_lib.amressif_write_plotfile(self._h, directory.encode(), name.encode(), step, time))
         return os.path.join(directory, f"{name}{step:05d}")
 
+    def write_flux_plotfile(self, directory: str, step: int, time: float):
+        """Export_Fluxes (utilities.cpp:496-532): pltFluxNNNNN of the last step's predictor."""
+        self._check(_lib.amressif_write_flux_plotfile(self._h, directory.encode(), step, time))
+        return os.path.join(directory, f"pltFlux{step:05d}")
+
     def advance(self, nsteps: int, with_stats: bool = True):
         st = (StepStats * max(1, nsteps))() if with_stats else None
         self._check(_lib.amressif_advance(self._h, nsteps, st))
@@ -536,6 +543,23 @@ class Context:
         self._check(_lib.amressif_momentum_rhs(self._h, faces_view(ustar, vstar, n), ctypes.byref(uc),
                                                faces_view(rx, ry, n)))
         return rx, ry
+
+    def fluxes(self, ustar, vstar):
+        """convective_flux_calc_new_quick / viscous_flux_calc outputs (fn_flux_calc.cpp:11-202, 695-766):
+        returns fluxConvect (2, n, n), fluxViscous (2, n, n) and the half-node fluxes
+        (N1 x-face, N1 y-face, N2 x-face, N2 y-face)."""
+        n = self.n
+        dev = ustar.device
+        fc = torch.zeros((2, n, n), dtype=torch.float64, device=dev)
+        fv = torch.zeros((2, n, n), dtype=torch.float64, device=dev)
+        h1 = (torch.zeros((n, n + 1), dtype=torch.float64, device=dev),
+              torch.zeros((n + 1, n), dtype=torch.float64, device=dev))
+        h2 = (torch.zeros((n, n + 1), dtype=torch.float64, device=dev),
+              torch.zeros((n + 1, n), dtype=torch.float64, device=dev))
+        cv, vv = cell_view(fc, n, 0), cell_view(fv, n, 0)
+        self._check(_lib.amressif_fluxes(self._h, faces_view(ustar, vstar, n), ctypes.byref(cv), ctypes.byref(vv),
+                                         faces_view(*h1, n), faces_view(*h2, n)))
+        return fc, fv, (h1[0], h1[1], h2[0], h2[1])
 
     def runge_kutta4_pseudo_time_stepping(self, sub, ustar, vstar, ucont, vcont, uprev, vprev, uprevprev,
                                           vprevprev, userCtx):

@@ -3,7 +3,8 @@
 // start from rest (hybrid_grid_init, main.cpp:793) or from the checkpoint
 // chk_out (main.cpp:326-328, 386-800), advance steps starting_step..nsteps
 // (main.cpp:875), write a checkpoint every chk_int steps (main.cpp:1167-1170)
-// and a plotfile every plot_int steps (main.cpp:1185-1191), print the
+// and the flux and flow-field plotfiles every plot_int steps (main.cpp:1045-1048,
+// 1185-1191), print the
 // per-step report and the run time (main.cpp:1173-1175, 1207-1212).
 //
 //   amressif_run <inputs> [key=value ...]   (files go to the working directory)
@@ -77,9 +78,13 @@ int main(int argc, char **argv)
         if (in.chk_int > 0 && last % in.chk_int == 0 &&
             (rc = amressif_write_checkpoint(ctx, ".", last, time)) != AMRESSIF_OK)
             return die(ctx, "write_checkpoint", rc);
-        if (in.plot_int > 0 && last % in.plot_int == 0 &&
-            (rc = amressif_write_plotfile(ctx, ".", "pltResults", last, time)) != AMRESSIF_OK)
-            return die(ctx, "write_plotfile", rc);
+        if (in.plot_int > 0 && last % in.plot_int == 0) {
+            // Export_Fluxes at the end of the momentum solve (main.cpp:1045-1048)
+            if ((rc = amressif_write_flux_plotfile(ctx, ".", last, time)) != AMRESSIF_OK)
+                return die(ctx, "write_flux_plotfile", rc);
+            if ((rc = amressif_write_plotfile(ctx, ".", "pltResults", last, time)) != AMRESSIF_OK)
+                return die(ctx, "write_plotfile", rc);
+        }
         n = last + 1;
     }
     amressif_destroy(ctx);

@@ -197,6 +197,12 @@ int ensure_module_scratch(amressif_ctx *ctx)
     return 0;
 }
 
+}  // namespace
+
+int amr_module_scratch(amressif_ctx *ctx) { return ensure_module_scratch(ctx); }
+
+namespace {
+
 // ---- FGMRES(m) with a V-cycle preconditioner on c.b (already centred):
 // cycles of guarded Arnoldi steps, polled in batches; the true residual is
 // tested at every cycle start
@@ -1042,6 +1048,7 @@ int amressif_load_state(amressif_ctx *ctx, const amr_array4 velCont[2], const am
     }
     HIPCHK(amr::launch_mg_start(c));  // phi_mean = 0 for the loaded phi
     c.phih_valid = 0;                  // no solution history for the warm start
+    c.pred_valid = 0;
     HIPCHK(hipStreamSynchronize(c.stream));
     return AMRESSIF_OK;
 }
@@ -1154,6 +1161,7 @@ int amressif_load_state_mf(amressif_ctx *ctx, const amr_multifab velCont[2], con
     }
     HIPCHK(amr::launch_mg_start(c));  // phi_mean = 0 for the loaded phi
     c.phih_valid = 0;                  // no solution history for the warm start
+    c.pred_valid = 0;
     HIPCHK(hipStreamSynchronize(c.stream));
     return AMRESSIF_OK;
 }
@@ -1244,6 +1252,7 @@ int amr_state_from_host(amressif_ctx *ctx, const double *u, const double *v, con
     }
     HIPCHK(amr::launch_mg_start(c));  // phi_mean = 0 for the loaded phi
     c.phih_valid = 0;                  // no solution history for the warm start
+    c.pred_valid = 0;
     HIPCHK(hipStreamSynchronize(c.stream));
     return AMRESSIF_OK;
 }
@@ -1378,6 +1387,7 @@ int advance_impl(amressif_ctx *ctx, int nsteps, amressif_step_stats *stats, amre
             nsteps = s + 1;
             break;
         }
+        c.pred_valid = 1;  // S1 now holds this step's predictor u* (the projection's input)
     }
     if (stats) HIPCHK(hipMemcpyAsync(stats, c.log, sizeof(amressif_step_stats) * (size_t)nsteps,
                                      hipMemcpyDeviceToHost, c.stream));
@@ -1462,6 +1472,45 @@ int amressif_rk_stage(amressif_ctx *ctx, int sub, amr_array4 velStar[2], const a
     int nb = 0;
     HIPCHK(amr::launch_stage(c, a, &nb));
     for (int d = 0; d < 2; ++d) HIPCHK(amr::launch_unpack_face(c, c.M[3][d], velStar[d], d));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    return AMRESSIF_OK;
+}
+
+int amressif_fluxes(amressif_ctx *ctx, const amr_array4 velStar[2], amr_array4 *fluxConvect, amr_array4 *fluxViscous,
+                    amr_array4 fluxHalfN1[2], amr_array4 fluxHalfN2[2])
+{
+    if (!ctx || !velStar) return fail(ctx, AMRESSIF_ERR_ARG, "null argument");
+    if ((fluxHalfN1 == nullptr) != (fluxHalfN2 == nullptr))
+        return fail(ctx, AMRESSIF_ERR_ARG, "fluxHalfN1 and fluxHalfN2 go together");
+    Ctx &c = ctx->c;
+    if (int rc = ensure_module_scratch(ctx)) return rc;
+    for (int d = 0; d < 2; ++d) {
+        if (int rc = check_face(ctx, &velStar[d], d, "velStar")) return rc;
+        if (fluxHalfN1) {
+            if (int rc = check_face(ctx, &fluxHalfN1[d], d, "fluxHalfN1")) return rc;
+            if (int rc = check_face(ctx, &fluxHalfN2[d], d, "fluxHalfN2")) return rc;
+        }
+        HIPCHK(amr::launch_pack_face(c, velStar[d], c.M[0][d], d));
+    }
+    if (fluxConvect)
+        if (int rc = check_cell(ctx, fluxConvect, 2, "fluxConvect")) return rc;
+    if (fluxViscous)
+        if (int rc = check_cell(ctx, fluxViscous, 2, "fluxViscous")) return rc;
+    // M[1]: fluxHalfN1 (x-, y-faces), M[2]: fluxHalfN2, Mc: fluxConvect, M[3]: fluxViscous
+    const bool half = fluxHalfN1 != nullptr;
+    HIPCHK(amr::launch_flux_fields(c, c.M[0][0], c.M[0][1], half ? c.M[1][0] : nullptr, half ? c.M[2][0] : nullptr,
+                                   half ? c.M[1][1] : nullptr, half ? c.M[2][1] : nullptr,
+                                   fluxConvect ? c.Mc[0] : nullptr, fluxConvect ? c.Mc[1] : nullptr,
+                                   fluxViscous ? c.M[3][0] : nullptr, fluxViscous ? c.M[3][1] : nullptr));
+    if (half)
+        for (int d = 0; d < 2; ++d) {
+            HIPCHK(amr::launch_unpack_face(c, c.M[1][d], fluxHalfN1[d], d));
+            HIPCHK(amr::launch_unpack_face(c, c.M[2][d], fluxHalfN2[d], d));
+        }
+    for (int comp = 0; comp < 2; ++comp) {
+        if (fluxConvect) HIPCHK(amr::launch_unpack_cell(c, c.Mc[comp], *fluxConvect, comp, 0));
+        if (fluxViscous) HIPCHK(amr::launch_unpack_cell(c, c.M[3][comp], *fluxViscous, comp, 0));
+    }
     HIPCHK(hipStreamSynchronize(c.stream));
     return AMRESSIF_OK;
 }
@@ -1583,6 +1632,7 @@ int amressif_time_kernel(amressif_ctx *ctx, int which, int reps, int64_t copy_by
 {
     if (!ctx || !avg_ms || reps < 1) return fail(ctx, AMRESSIF_ERR_ARG, "bad argument");
     Ctx &c = ctx->c;
+    c.pred_valid = 0;  // the timed kernels overwrite the step buffers
     const double cells = (double)c.g.n * (double)c.g.n;
     void *src = nullptr, *dst = nullptr;
     if (which == 2) {

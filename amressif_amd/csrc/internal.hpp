@@ -166,6 +166,7 @@ struct Ctx {
     double *phiv[2]{};   // Poisson iterates (ping-pong); phi points at the last solution
     double *phih = nullptr;  // warm start: the solution of the step before (allocated on first use)
     int phih_valid = 0;
+    int pred_valid = 0;  // S1 holds the last step's converged predictor u* (set by a step, cleared by a load)
     int tail_level = 0;  // first multigrid level handled by the single-workgroup tail kernel
     // exact solve of that level by the DCT (k_tail_dct): cosine and inverse
     // eigenvalue tables (null: the tail runs the V-cycle down to the bottom)
@@ -302,6 +303,11 @@ hipError_t launch_unpack_cell_shift(const Ctx &c, const double *src, const amr_a
 hipError_t launch_copy_bytes(const Ctx &c, const void *src, void *dst, int64_t bytes);
 hipError_t launch_mean_b(const Ctx &c, const double *b);
 hipError_t launch_bdf2_const(const Ctx &c, const double *un, const double *um, double *cc, int dir);
+// flux fields of the faces u, v (kernels_export.hip): half-node QUICK fluxes
+// on x-faces (fxu, fxv) and y-faces (fyu, fyv), fluxConvect (cu, cv) and
+// fluxViscous (vu, vv); any output pair may be null
+hipError_t launch_flux_fields(const Ctx &c, const double *u, const double *v, double *fxu, double *fxv, double *fyu,
+                              double *fyv, double *cu, double *cv, double *vu, double *vv);
 
 // FGMRES Poisson solver (kernels_gmres.hip)
 size_t gm_partials_doubles(const Ctx &c, int m);
@@ -418,3 +424,5 @@ int amr_state_to_host(amressif_ctx *ctx, double *u, double *v, double *up, doubl
 int amr_state_from_host(amressif_ctx *ctx, const double *u, const double *v, const double *up, const double *vp,
                         const double *p, const double *phi);
 int amr_fail(amressif_ctx *ctx, int code, const std::string &msg);
+// the module-level entry points' scratch arrays M[4][2], Mc[2] (allocated on first use)
+int amr_module_scratch(amressif_ctx *ctx);

@@ -399,6 +399,43 @@ std::string step_name(const char *prefix, int step)
     return std::string(prefix) + buf;
 }
 
+// WriteSingleLevelPlotfile of cell fields (n x n each, i fastest) named
+// `names` into <dir>/<name>NNNNN: HyperCLaw-V1.1 Header and Level_0/Cell
+int write_plot(amressif_ctx *ctx, const char *dir, const char *name, int step, double time,
+               const std::vector<std::string> &names, const std::vector<const double *> &data)
+{
+    const amr::Ctx &c = ctx->c;
+    const int n = c.g.n;
+    const std::string root = std::string(dir) + "/" + step_name(name, step);
+    if (!make_dir(root) || !make_dir(root + "/Level_0"))
+        return amr_fail(ctx, AMRESSIF_ERR_ARG, "cannot create " + root + "/Level_0");
+    const std::vector<Box2> boxes = max_size_boxes(n, c.in.max_grid_size[0], c.in.max_grid_size[1]);
+    const double lo[2] = {c.in.lo_phy_dim[0], c.in.lo_phy_dim[1]}, hi[2] = {c.in.hi_phy_dim[0], c.in.hi_phy_dim[1]};
+    const double dx[2] = {c.g.dx, c.g.dy};
+    {
+        std::ofstream h(root + "/Header", std::ios::trunc);
+        if (!h) return amr_fail(ctx, AMRESSIF_ERR_ARG, "cannot open " + root + "/Header");
+        h.precision(17);
+        h << "HyperCLaw-V1.1\n" << names.size() << '\n';
+        for (const std::string &nm : names) h << nm << '\n';
+        h << 2 << '\n' << time << '\n' << 0 << '\n';
+        h << lo[0] << ' ' << lo[1] << ' ' << '\n' << hi[0] << ' ' << hi[1] << ' ' << '\n' << '\n';
+        h << box_str(Box2{{0, 0}, {n - 1, n - 1}, {0, 0}}) << ' ' << '\n' << step << ' ' << '\n';
+        h << dx[0] << ' ' << dx[1] << ' ' << '\n' << 0 << '\n' << 0 << '\n';
+        h << 0 << ' ' << boxes.size() << ' ' << time << '\n' << step << '\n';
+        for (const Box2 &b : boxes)
+            for (int d = 0; d < 2; ++d)
+                h << lo[d] + b.lo[d] * dx[d] << ' ' << lo[d] + (b.hi[d] + 1) * dx[d] << '\n';
+        h << "Level_0/Cell\n";
+        if (!h) return amr_fail(ctx, AMRESSIF_ERR_ARG, "write failed: " + root + "/Header");
+    }
+    std::vector<Field> f;
+    for (const double *d : data) f.push_back(Field{d, nullptr, n, n, -1});
+    std::string err;
+    if (!write_vismf(root + "/Level_0/Cell", boxes, f, err)) return amr_fail(ctx, AMRESSIF_ERR_ARG, err);
+    return AMRESSIF_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -493,32 +530,34 @@ int amressif_write_plotfile(amressif_ctx *ctx, const char *dir, const char *name
             U[(size_t)i + (size_t)n * j] = 0.5 * (u[(size_t)i + (size_t)(n + 1) * j] + u[(size_t)i + 1 + (size_t)(n + 1) * j]);
             V[(size_t)i + (size_t)n * j] = 0.5 * (v[(size_t)i + (size_t)n * j] + v[(size_t)i + (size_t)n * (j + 1)]);
         }
-    const std::string root = std::string(dir) + "/" + step_name(name, step);
-    if (!make_dir(root) || !make_dir(root + "/Level_0"))
-        return amr_fail(ctx, AMRESSIF_ERR_ARG, "cannot create " + root + "/Level_0");
-    const std::vector<Box2> boxes = max_size_boxes(n, c.in.max_grid_size[0], c.in.max_grid_size[1]);
-    const double lo[2] = {c.in.lo_phy_dim[0], c.in.lo_phy_dim[1]}, hi[2] = {c.in.hi_phy_dim[0], c.in.hi_phy_dim[1]};
-    const double dx[2] = {c.g.dx, c.g.dy};
-    {
-        std::ofstream h(root + "/Header", std::ios::trunc);
-        if (!h) return amr_fail(ctx, AMRESSIF_ERR_ARG, "cannot open " + root + "/Header");
-        h.precision(17);
-        h << "HyperCLaw-V1.1\n" << 3 << '\n' << "pressure\nU\nV\n" << 2 << '\n' << time << '\n' << 0 << '\n';
-        h << lo[0] << ' ' << lo[1] << ' ' << '\n' << hi[0] << ' ' << hi[1] << ' ' << '\n' << '\n';
-        h << box_str(Box2{{0, 0}, {n - 1, n - 1}, {0, 0}}) << ' ' << '\n' << step << ' ' << '\n';
-        h << dx[0] << ' ' << dx[1] << ' ' << '\n' << 0 << '\n' << 0 << '\n';
-        h << 0 << ' ' << boxes.size() << ' ' << time << '\n' << step << '\n';
-        for (const Box2 &b : boxes)
-            for (int d = 0; d < 2; ++d)
-                h << lo[d] + b.lo[d] * dx[d] << ' ' << lo[d] + (b.hi[d] + 1) * dx[d] << '\n';
-        h << "Level_0/Cell\n";
-        if (!h) return amr_fail(ctx, AMRESSIF_ERR_ARG, "write failed: " + root + "/Header");
-    }
-    std::vector<Field> f{Field{p.data(), nullptr, n, n, -1}, Field{U.data(), nullptr, n, n, -1},
-                         Field{V.data(), nullptr, n, n, -1}};
-    std::string err;
-    if (!write_vismf(root + "/Level_0/Cell", boxes, f, err)) return amr_fail(ctx, AMRESSIF_ERR_ARG, err);
-    return AMRESSIF_OK;
+    return write_plot(ctx, dir, name, step, time, {"pressure", "U", "V"}, {p.data(), U.data(), V.data()});
+}
+
+int amressif_write_flux_plotfile(amressif_ctx *ctx, const char *dir, int step, double time)
+{
+    if (!ctx || !dir || step < 0) return amr_fail(ctx, AMRESSIF_ERR_ARG, "write_flux_plotfile: bad argument");
+    amr::Ctx &c = ctx->c;
+    if (!c.pred_valid)
+        return amr_fail(ctx, AMRESSIF_ERR_ARG, "write_flux_plotfile: no step has run since the state was set");
+    const int n = c.g.n, P = c.g.P;
+    const size_t cells = (size_t)n * n;
+    // fluxConvect and fluxViscous of the last step's predictor u* (held in S1)
+    // into the first four of the module scratch arrays
+    if (int rc = amr_module_scratch(ctx)) return rc;
+    double *out[4] = {c.Mc[0], c.Mc[1], c.M[3][0], c.M[3][1]};
+    hipError_t e = amr::launch_flux_fields(c, c.S1[0], c.S1[1], nullptr, nullptr, nullptr, nullptr, out[0], out[1],
+                                           out[2], out[3]);
+    std::vector<double> host(6 * cells, 0.0);  // press_grad_x, press_grad_y stay 0 (fluxPrsGrad, main.cpp:846)
+    for (int q = 0; q < 4 && e == hipSuccess; ++q)
+        e = hipMemcpy2DAsync(host.data() + q * cells, sizeof(double) * n, out[q], sizeof(double) * P,
+                             sizeof(double) * n, n, hipMemcpyDeviceToHost, c.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+    if (e != hipSuccess) return amr_fail(ctx, AMRESSIF_ERR_HIP, std::string("write_flux_plotfile: ") + hipGetErrorString(e));
+    std::vector<const double *> data;
+    for (int q = 0; q < 6; ++q) data.push_back(host.data() + q * cells);
+    return write_plot(ctx, dir, "pltFlux", step, time,
+                      {"conv_flux_x", "conv_flux_y", "visc_flux_x", "visc_flux_y", "press_grad_x", "press_grad_y"},
+                      data);
 }
 
 int amressif_init_state(amressif_ctx *ctx, const double *psi_coeffs, int kmax)

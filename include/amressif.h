@@ -206,6 +206,15 @@ int amressif_read_checkpoint(amressif_ctx *ctx, const char *path, double *time);
  * (pressure, U, V) into <dir>/<name>NNNNN (HyperCLaw-V1.1 Header and
  * Level_0/Cell VisMF data).  U, V are the cell averages of the faces. */
 int amressif_write_plotfile(amressif_ctx *ctx, const char *dir, const char *name, int step, double time);
+/* Export_Fluxes (utilities.cpp:496-532, called every plot_int steps at
+ * main.cpp:1045-1048): <dir>/pltFluxNNNNN with conv_flux_x, conv_flux_y
+ * (fluxConvect), visc_flux_x, visc_flux_y (fluxViscous) and press_grad_x,
+ * press_grad_y (fluxPrsGrad, which the reference never fills: 0).  The fluxes
+ * are those of the last step's converged predictor u*; the reference exports
+ * those of its last RK stage's input, which differs from u* by that stage's
+ * increment (below momentum_tolerance).  AMRESSIF_ERR_ARG if no step has run
+ * since the state was set. */
+int amressif_write_flux_plotfile(amressif_ctx *ctx, const char *dir, int step, double time);
 
 /* ------------------------------------------- module-level entry points
  * Each runs the same kernels as amressif_advance on one call's inputs (Array4
@@ -225,6 +234,14 @@ int amressif_momentum_rhs(amressif_ctx *ctx, const amr_array4 velStar[2], const 
 int amressif_rk_stage(amressif_ctx *ctx, int sub, amr_array4 velStar[2], const amr_array4 velCont[2],
                       const amr_array4 velContPrev[2], const amr_array4 velContPrevPrev[2],
                       const amr_array4 *userCtx);
+/* The flux fields of convective_flux_calc_new_quick (fn_flux_calc.cpp:11-202)
+ * and viscous_flux_calc (:695-766) for velStar: fluxConvect and fluxViscous
+ * (cells, 2 comps), and the half-node fluxes fluxHalfN1 (of U) and
+ * fluxHalfN2 (of V) on the x- and y-faces, from cont2cart's cell velocities
+ * with the wall rules.  Any output may be NULL (the half-node pair together).
+ * Evaluated in the reference's operation order (bit-exact with it). */
+int amressif_fluxes(amressif_ctx *ctx, const amr_array4 velStar[2], amr_array4 *fluxConvect, amr_array4 *fluxViscous,
+                    amr_array4 fluxHalfN1[2], amr_array4 fluxHalfN2[2]);
 /* Error_Computation (utilities.cpp:421-493): weighted RMS of velCont - velStar */
 int amressif_pseudo_norm(amressif_ctx *ctx, const amr_array4 velCont[2], const amr_array4 velStar[2],
                          double *norm);

@@ -184,7 +184,8 @@ def test_driver_program_checkpoint_cadence_and_restart(tmp_path):
     r = subprocess.run([exe, "inputs"], cwd=a, capture_output=True, text=True, timeout=120)
     sys.stdout.write(r.stdout[-2000:])
     assert r.returncode == 0, r.stderr
-    for d in ("checkpoint00003", "checkpoint00006", "pltInit00000", "pltResults00003", "pltResults00006"):
+    for d in ("checkpoint00003", "checkpoint00006", "pltInit00000", "pltResults00003", "pltResults00006",
+              "pltFlux00003", "pltFlux00006"):
         assert (a / d / "Header").exists(), d
     assert r.stdout.count("STEP| ") == 6
     shutil.copytree(a / "checkpoint00003", b / "checkpoint00003")

@@ -69,6 +69,7 @@ class Inputs(ctypes.Structure):
         ("gmres_restart", ctypes.c_int),
         ("mixed_momentum", ctypes.c_int),
         ("poisson_warm_start", ctypes.c_int),
+        ("spacedim", ctypes.c_int),
     ]
 
 
@@ -269,6 +270,42 @@ def faces_view(u, v, n):
     return arr
 
 
+def face_shape(d: int, n: int, dim: int) -> tuple:
+    """torch shape (slowest axis first) of the face array of direction d on n^dim cells"""
+    ext = [n + (d == 0), n + (d == 1), n + (d == 2)]
+    return (ext[2], ext[1], ext[0]) if dim == 3 else (ext[1], ext[0])
+
+
+def _a4_nd(t: torch.Tensor, lo, hi, ncomp) -> Array4:
+    if t.dtype != torch.float64 or not t.is_contiguous() or t.device.type != "cuda":
+        raise ValueError("expected a contiguous float64 cuda tensor")
+    a = Array4()
+    a.ptr = t.data_ptr()
+    for k in range(3):
+        a.lo[k], a.hi[k] = lo[k], hi[k]
+    a.ncomp = ncomp
+    a.dtype = 0
+    return a
+
+
+def faces_view_nd(faces, n: int, dim: int):
+    """Array4 views of dim face arrays (velCont[0..dim-1]) on n^dim cells"""
+    arr = (Array4 * dim)()
+    for d in range(dim):
+        assert tuple(faces[d].shape) == face_shape(d, n, dim), (d, faces[d].shape)
+        hi = [n - 1 + (d == k) if k < dim else 0 for k in range(3)]
+        arr[d] = _a4_nd(faces[d], (0, 0, 0), hi, 1)
+    return arr
+
+
+def cell_view_nd(t, n: int, dim: int, ghosts: int):
+    """Array4 view of a cell array (comps, [n+2g,] n+2g, n+2g) with g ghost layers"""
+    ncomp = t.shape[0] if t.dim() == dim + 1 else 1
+    lo = [-ghosts if k < dim else 0 for k in range(3)]
+    hi = [n - 1 + ghosts if k < dim else 0 for k in range(3)]
+    return _a4_nd(t, lo, hi, ncomp)
+
+
 def cell_view(t, n, ghosts):
     ncomp = 1 if t.dim() == 2 else t.shape[0]
     assert tuple(t.shape[-2:]) == (n + 2 * ghosts, n + 2 * ghosts), t.shape
@@ -393,7 +430,33 @@ class Context:
             self._lid = None
             self._check(_lib.amressif_set_lid_profile(self._h, None))
 
-    # ---- state
+    # ---- state on n^dim cells (2D or 3D, any boundary codes)
+    @property
+    def dim(self) -> int:
+        return 3 if self.inp.spacedim == 3 else 2
+
+    def load_faces(self, faces, faces_prev, ctx=None):
+        """amressif_load_state with dim face arrays per state (face_shape) and userCtx
+        (2, [n+2g,] n+2g, n+2g) with g >= 0 ghost layers, or None (p = phi = 0)"""
+        n, dim = self.n, self.dim
+        uc = None
+        if ctx is not None:
+            g = (ctx.shape[-1] - n) // 2
+            uc = cell_view_nd(ctx, n, dim, g)
+        self._check(_lib.amressif_load_state(self._h, faces_view_nd(faces, n, dim), faces_view_nd(faces_prev, n, dim),
+                                             ctypes.byref(uc) if uc is not None else None))
+
+    def store_faces(self, faces=None, faces_prev=None, ctx=None, velCart=None):
+        """amressif_store_state into dim face arrays, userCtx (<= 1 ghost) and velCart (<= 2 ghosts)"""
+        n, dim = self.n, self.dim
+        vc = faces_view_nd(faces, n, dim) if faces is not None else None
+        vp = faces_view_nd(faces_prev, n, dim) if faces_prev is not None else None
+        uc = cell_view_nd(ctx, n, dim, (ctx.shape[-1] - n) // 2) if ctx is not None else None
+        cc = cell_view_nd(velCart, n, dim, (velCart.shape[-1] - n) // 2) if velCart is not None else None
+        self._check(_lib.amressif_store_state(self._h, vc, vp, ctypes.byref(uc) if uc is not None else None,
+                                              ctypes.byref(cc) if cc is not None else None))
+
+    # ---- state (2D)
     def load_state(self, u, v, uprev, vprev, ctx=None):
         n = self.n
         vc = faces_view(u, v, n)

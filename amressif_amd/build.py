@@ -14,7 +14,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libamressif_hip.so")
 RUN = os.path.join(HERE, "amressif_run")
-SOURCES = ["kernels_stage.hip", "kernels_step.hip", "kernels_mg.hip", "kernels_adj.hip", "kernels_gmres.hip", "kernels_amr.hip", "api.cpp", "amr_two_level.cpp", "io_amrex.cpp", "noalloc.hip", "kernels_export.hip"]
+SOURCES = ["kernels_stage.hip", "kernels_step.hip", "kernels_mg.hip", "kernels_adj.hip", "kernels_gmres.hip", "kernels_amr.hip", "api.cpp", "amr_two_level.cpp", "io_amrex.cpp", "noalloc.hip", "kernels_export.hip", "kernels_gen.hip", "gen_path.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

@@ -299,6 +299,7 @@ int amressif_amr_create(const amressif_inputs *in, int patch_jlo, void *hip_stre
     if (patch_jlo < 2 || patch_jlo >= n0 - 1) return afail(A, AMRESSIF_ERR_ARG, "patch_jlo must be in [2, n_cell - 1)");
     if (in->PSEUDO_TIMESTEPPING != 1)
         return afail(A, AMRESSIF_ERR_UNSUPPORTED, "the two-level step uses the dual-time predictor (PSEUDO_TIMESTEPPING = 1)");
+    if (amr_gen_wanted(in)) return afail(A, AMRESSIF_ERR_UNSUPPORTED, "the two-level step is 2D with wall boundaries");
     // one dt for both levels, from the fine spacing (main.cpp:284-294 on the finest level)
     amressif_inputs ci = *in;
     const double h0 = (in->hi_phy_dim[0] - in->lo_phy_dim[0]) / n0, h1 = 0.5 * h0;

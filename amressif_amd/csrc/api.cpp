@@ -21,6 +21,7 @@
 #include <string>
 #include <vector>
 
+#include "gen.hpp"
 #include "internal.hpp"
 
 
@@ -650,13 +651,16 @@ void amressif_inputs_default(amressif_inputs *in)
     in->gmres_restart = 30;
     in->mixed_momentum = 1;
     in->poisson_warm_start = -1;
+    in->spacedim = 2;
 }
 
 double amressif_dt(const amressif_inputs *in)
 {
-    const double dx = (in->hi_phy_dim[0] - in->lo_phy_dim[0]) / in->n_cell;
-    const double dy = (in->hi_phy_dim[1] - in->lo_phy_dim[1]) / in->n_cell;
-    const double coeff = 1. / (dx * dx) + 1. / (dy * dy);
+    double coeff = 0.0;
+    for (int d = 0; d < (in->spacedim == 3 ? 3 : 2); ++d) {
+        const double h = (in->hi_phy_dim[d] - in->lo_phy_dim[d]) / in->n_cell;
+        coeff += 1. / (h * h);
+    }
     double dt = in->cfl / (2.0 * coeff);
     if (in->fixed_dt != -1.0) dt = in->fixed_dt;
     return dt;
@@ -737,16 +741,19 @@ int amressif_parse_inputs(const char *path, int argc, const char *const *argv, a
             if (!num(key, it->second[k], dst[k])) { bad = std::string("bad number for ") + key; return false; }
         return true;
     };
-    // 2D: the array keys take AMREX_SPACEDIM = 2 values (extra values ignored)
+    // the array keys take AMREX_SPACEDIM values (extra values ignored)
+    if (!get_int("amressif.spacedim", &out->spacedim, 1, false)) return seterr(bad);
+    if (out->spacedim != 2 && out->spacedim != 3) return seterr("amressif.spacedim must be 2 or 3");
+    const int nd = out->spacedim;
     bool ok = get_int("n_cell", &out->n_cell, 1, true) && get_int("max_grid_size_x", &out->max_grid_size[0], 1, true) &&
               get_int("max_grid_size_y", &out->max_grid_size[1], 1, true) &&
               get_int("max_grid_size_z", &out->max_grid_size[2], 1, true) &&
-              get_real("lo_phy_dim", out->lo_phy_dim, 2, false) && get_real("hi_phy_dim", out->hi_phy_dim, 2, false) &&
+              get_real("lo_phy_dim", out->lo_phy_dim, nd, false) && get_real("hi_phy_dim", out->hi_phy_dim, nd, false) &&
               get_int("IterNum", &out->IterNum, 1, true) && get_int("nsteps", &out->nsteps, 1, false) &&
               get_real("cfl", &out->cfl, 1, false) && get_real("fixed_dt", &out->fixed_dt, 1, false) &&
               get_real("ren", &out->ren, 1, true) && get_real("vis", &out->vis, 1, true) &&
-              get_int("phy_bc_lo", out->phy_bc_lo, 2, false) && get_int("phy_bc_hi", out->phy_bc_hi, 2, false) &&
-              get_real("inflow_waveform", out->inflow_waveform, 2, false) &&
+              get_int("phy_bc_lo", out->phy_bc_lo, nd, false) && get_int("phy_bc_hi", out->phy_bc_hi, nd, false) &&
+              get_real("inflow_waveform", out->inflow_waveform, nd, false) &&
               get_int("target_resolution", &out->target_resolution, 1, false) &&
               get_real("momentum_tolerance", &out->momentum_tolerance, 1, false) &&
               get_int("PSEUDO_TIMESTEPPING", &out->PSEUDO_TIMESTEPPING, 1, false) &&
@@ -826,6 +833,13 @@ int amressif_create(const amressif_inputs *in, void *hip_stream, amressif_ctx **
         return bail(AMRESSIF_ERR_ARG, "poisson_solver must be 0 (multigrid) or 1 (FGMRES)");
     if (in->poisson_solver == 1 && (in->gmres_restart < 1 || in->gmres_restart > amr::kGmMax))
         return bail(AMRESSIF_ERR_ARG, "gmres_restart must be in [1, 64]");
+    if (in->spacedim != 2 && in->spacedim != 3) return bail(AMRESSIF_ERR_ARG, "spacedim must be 2 or 3");
+    if (amr_gen_wanted(in)) {
+        // 3D, or a periodic direction: the general path owns the state
+        const int rc = amr_gen_create(ctx);
+        *out = ctx;
+        return rc;
+    }
     {
         std::string msg;
         if (int rc = check_bc(*in, msg)) return bail(rc, msg);
@@ -1010,6 +1024,7 @@ int amressif_destroy(amressif_ctx *ctx)
         delete c.kp;
     }
     delete c.adj;
+    delete c.gen;
     delete ctx;
     return AMRESSIF_OK;
 }
@@ -1020,6 +1035,8 @@ int64_t amressif_workspace_bytes(const amressif_ctx *ctx) { return ctx ? ctx->c.
 
 int amressif_set_lid_profile(amressif_ctx *ctx, const double *dev_lid)
 {
+    if (ctx && ctx->c.gen)
+        return amr_fail(const_cast<amressif_ctx *>(ctx), AMRESSIF_ERR_UNSUPPORTED, "amressif_set_lid_profile: not available for 3D or periodic inputs");
     if (!ctx) return AMRESSIF_ERR_ARG;
     if (dev_lid && ctx->c.g.bc_hi1 != 165)
         return fail(ctx, AMRESSIF_ERR_ARG, "a lid profile needs the north inlet code 165");
@@ -1030,6 +1047,7 @@ int amressif_set_lid_profile(amressif_ctx *ctx, const double *dev_lid)
 int amressif_load_state(amressif_ctx *ctx, const amr_array4 velCont[2], const amr_array4 velContPrev[2],
                         const amr_array4 *userCtx)
 {
+    if (ctx && ctx->c.gen) return amr_gen_load_state(ctx, velCont, velContPrev, userCtx);
     if (!ctx || !velCont || !velContPrev) return fail(ctx, AMRESSIF_ERR_ARG, "null argument");
     Ctx &c = ctx->c;
     for (int d = 0; d < 2; ++d) {
@@ -1056,6 +1074,8 @@ int amressif_load_state(amressif_ctx *ctx, const amr_array4 velCont[2], const am
 int amressif_store_state(const amressif_ctx *cctx, amr_array4 velCont[2], amr_array4 velContPrev[2],
                          amr_array4 *userCtx, amr_array4 *velCart)
 {
+    if (cctx && cctx->c.gen)
+        return amr_gen_store_state(const_cast<amressif_ctx *>(cctx), velCont, velContPrev, userCtx, velCart);
     amressif_ctx *ctx = const_cast<amressif_ctx *>(cctx);
     if (!ctx) return AMRESSIF_ERR_ARG;
     Ctx &c = ctx->c;
@@ -1130,6 +1150,8 @@ extern "C" {
 int amressif_load_state_mf(amressif_ctx *ctx, const amr_multifab velCont[2], const amr_multifab velContPrev[2],
                            const amr_multifab *userCtx)
 {
+    if (ctx && ctx->c.gen)
+        return amr_fail(const_cast<amressif_ctx *>(ctx), AMRESSIF_ERR_UNSUPPORTED, "amressif_load_state_mf: not available for 3D or periodic inputs");
     if (!ctx || !velCont || !velContPrev) return fail(ctx, AMRESSIF_ERR_ARG, "null argument");
     Ctx &c = ctx->c;
     // validate everything before the first copy: a failed load leaves the state as it was
@@ -1169,6 +1191,8 @@ int amressif_load_state_mf(amressif_ctx *ctx, const amr_multifab velCont[2], con
 int amressif_store_state_mf(const amressif_ctx *cctx, amr_multifab velCont[2], amr_multifab velContPrev[2],
                             amr_multifab *userCtx, amr_multifab *velCart)
 {
+    if (cctx && cctx->c.gen)
+        return amr_fail(const_cast<amressif_ctx *>(cctx), AMRESSIF_ERR_UNSUPPORTED, "amressif_store_state_mf: not available for 3D or periodic inputs");
     amressif_ctx *ctx = const_cast<amressif_ctx *>(cctx);
     if (!ctx) return AMRESSIF_ERR_ARG;
     Ctx &c = ctx->c;
@@ -1261,11 +1285,14 @@ extern "C" {
 
 int amressif_advance(amressif_ctx *ctx, int nsteps, amressif_step_stats *stats)
 {
+    if (ctx && ctx->c.gen) return amr_gen_advance(ctx, nsteps, stats);
     return advance_impl(ctx, nsteps, stats, nullptr);
 }
 
 int amressif_step_fwd(amressif_ctx *ctx, int nsteps, amressif_tape *tape, amressif_step_stats *stats)
 {
+    if (ctx && ctx->c.gen)
+        return amr_fail(const_cast<amressif_ctx *>(ctx), AMRESSIF_ERR_UNSUPPORTED, "amressif_step_fwd: not available for 3D or periodic inputs");
     if (!ctx || !tape || tape->ctx != ctx || nsteps < 0) return fail(ctx, AMRESSIF_ERR_ARG, "bad argument");
     if (tape->count + nsteps > tape->cap)
         return fail(ctx, AMRESSIF_ERR_ARG, "tape holds " + std::to_string(tape->cap) + " steps");
@@ -1402,6 +1429,8 @@ extern "C" {
 // ------------------------------------------------------ module entry points
 int amressif_cont2cart(amressif_ctx *ctx, const amr_array4 velCont[2], amr_array4 *velCart)
 {
+    if (ctx && ctx->c.gen)
+        return amr_fail(const_cast<amressif_ctx *>(ctx), AMRESSIF_ERR_UNSUPPORTED, "amressif_cont2cart: not available for 3D or periodic inputs");
     if (!ctx || !velCont) return fail(ctx, AMRESSIF_ERR_ARG, "null argument");
     Ctx &c = ctx->c;
     if (int rc = ensure_module_scratch(ctx)) return rc;
@@ -1418,6 +1447,8 @@ int amressif_cont2cart(amressif_ctx *ctx, const amr_array4 velCont[2], amr_array
 int amressif_momentum_rhs(amressif_ctx *ctx, const amr_array4 velStar[2], const amr_array4 *userCtx,
                           amr_array4 rhs[2])
 {
+    if (ctx && ctx->c.gen)
+        return amr_fail(const_cast<amressif_ctx *>(ctx), AMRESSIF_ERR_UNSUPPORTED, "amressif_momentum_rhs: not available for 3D or periodic inputs");
     if (!ctx || !velStar || !userCtx || !rhs) return fail(ctx, AMRESSIF_ERR_ARG, "null argument");
     Ctx &c = ctx->c;
     if (int rc = ensure_module_scratch(ctx)) return rc;
@@ -1442,6 +1473,8 @@ int amressif_momentum_rhs(amressif_ctx *ctx, const amr_array4 velStar[2], const 
 int amressif_rk_stage(amressif_ctx *ctx, int sub, amr_array4 velStar[2], const amr_array4 velCont[2],
                       const amr_array4 velContPrev[2], const amr_array4 velContPrevPrev[2], const amr_array4 *userCtx)
 {
+    if (ctx && ctx->c.gen)
+        return amr_fail(const_cast<amressif_ctx *>(ctx), AMRESSIF_ERR_UNSUPPORTED, "amressif_rk_stage: not available for 3D or periodic inputs");
     if (!ctx || !velStar || !velCont || !velContPrev || !velContPrevPrev || !userCtx)
         return fail(ctx, AMRESSIF_ERR_ARG, "null argument");
     if (sub < 0 || sub > 3) return fail(ctx, AMRESSIF_ERR_ARG, "sub must be 0..3");
@@ -1479,6 +1512,8 @@ int amressif_rk_stage(amressif_ctx *ctx, int sub, amr_array4 velStar[2], const a
 int amressif_fluxes(amressif_ctx *ctx, const amr_array4 velStar[2], amr_array4 *fluxConvect, amr_array4 *fluxViscous,
                     amr_array4 fluxHalfN1[2], amr_array4 fluxHalfN2[2])
 {
+    if (ctx && ctx->c.gen)
+        return amr_fail(const_cast<amressif_ctx *>(ctx), AMRESSIF_ERR_UNSUPPORTED, "amressif_fluxes: not available for 3D or periodic inputs");
     if (!ctx || !velStar) return fail(ctx, AMRESSIF_ERR_ARG, "null argument");
     if ((fluxHalfN1 == nullptr) != (fluxHalfN2 == nullptr))
         return fail(ctx, AMRESSIF_ERR_ARG, "fluxHalfN1 and fluxHalfN2 go together");
@@ -1517,6 +1552,8 @@ int amressif_fluxes(amressif_ctx *ctx, const amr_array4 velStar[2], amr_array4 *
 
 int amressif_pseudo_norm(amressif_ctx *ctx, const amr_array4 velCont[2], const amr_array4 velStar[2], double *norm)
 {
+    if (ctx && ctx->c.gen)
+        return amr_fail(const_cast<amressif_ctx *>(ctx), AMRESSIF_ERR_UNSUPPORTED, "amressif_pseudo_norm: not available for 3D or periodic inputs");
     if (!ctx || !velCont || !velStar || !norm) return fail(ctx, AMRESSIF_ERR_ARG, "null argument");
     Ctx &c = ctx->c;
     if (int rc = ensure_module_scratch(ctx)) return rc;
@@ -1535,6 +1572,8 @@ int amressif_pseudo_norm(amressif_ctx *ctx, const amr_array4 velCont[2], const a
 int amressif_poisson_rhs(amressif_ctx *ctx, const amr_array4 vel[2], amr_array4 *rhs, double *div_linf,
                          double *rhs_sum)
 {
+    if (ctx && ctx->c.gen)
+        return amr_fail(const_cast<amressif_ctx *>(ctx), AMRESSIF_ERR_UNSUPPORTED, "amressif_poisson_rhs: not available for 3D or periodic inputs");
     if (!ctx || !vel || !rhs) return fail(ctx, AMRESSIF_ERR_ARG, "null argument");
     Ctx &c = ctx->c;
     if (int rc = ensure_module_scratch(ctx)) return rc;
@@ -1556,6 +1595,8 @@ int amressif_poisson_rhs(amressif_ctx *ctx, const amr_array4 vel[2], amr_array4 
 int amressif_poisson_solve(amressif_ctx *ctx, amr_array4 *sol, const amr_array4 *rhs, double tol_rel, double tol_abs,
                            int *iters, double *res_rel)
 {
+    if (ctx && ctx->c.gen)
+        return amr_fail(const_cast<amressif_ctx *>(ctx), AMRESSIF_ERR_UNSUPPORTED, "amressif_poisson_solve: not available for 3D or periodic inputs");
     if (!ctx || !sol || !rhs) return fail(ctx, AMRESSIF_ERR_ARG, "null argument");
     Ctx &c = ctx->c;
     if (int rc = check_cell(ctx, rhs, 1, "rhs")) return rc;
@@ -1580,6 +1621,8 @@ int amressif_poisson_solve(amressif_ctx *ctx, amr_array4 *sol, const amr_array4 
 int amressif_project(amressif_ctx *ctx, const amr_array4 velStar[2], amr_array4 *userCtx, amr_array4 velCont[2],
                      double *div_linf)
 {
+    if (ctx && ctx->c.gen)
+        return amr_fail(const_cast<amressif_ctx *>(ctx), AMRESSIF_ERR_UNSUPPORTED, "amressif_project: not available for 3D or periodic inputs");
     if (!ctx || !velStar || !userCtx || !velCont) return fail(ctx, AMRESSIF_ERR_ARG, "null argument");
     Ctx &c = ctx->c;
     if (int rc = ensure_module_scratch(ctx)) return rc;
@@ -1603,6 +1646,8 @@ int amressif_project(amressif_ctx *ctx, const amr_array4 velStar[2], amr_array4 
 
 int amressif_kernel_timing(amressif_ctx *ctx, int enable)
 {
+    if (ctx && ctx->c.gen)
+        return amr_fail(const_cast<amressif_ctx *>(ctx), AMRESSIF_ERR_UNSUPPORTED, "amressif_kernel_timing: not available for 3D or periodic inputs");
     if (!ctx) return AMRESSIF_ERR_ARG;
     Ctx &c = ctx->c;
     if (!c.kp) c.kp = new amr::KProf();
@@ -1618,6 +1663,8 @@ int amressif_kernel_timing(amressif_ctx *ctx, int enable)
 int amressif_kernel_time(const amressif_ctx *ctx, int which, double *avg_ms, int64_t *launches,
                          double *bytes_per_launch)
 {
+    if (ctx && ctx->c.gen)
+        return amr_fail(const_cast<amressif_ctx *>(ctx), AMRESSIF_ERR_UNSUPPORTED, "amressif_kernel_time: not available for 3D or periodic inputs");
     if (!ctx || !avg_ms || !launches || !bytes_per_launch) return AMRESSIF_ERR_ARG;
     if (which < 0 || which >= amr::KP_COUNT) return AMRESSIF_ERR_ARG;
     const amr::KProf *k = ctx->c.kp;
@@ -1630,6 +1677,11 @@ int amressif_kernel_time(const amressif_ctx *ctx, int which, double *avg_ms, int
 int amressif_time_kernel(amressif_ctx *ctx, int which, int reps, int64_t copy_bytes, double *avg_ms,
                          double *bytes_per_launch)
 {
+    if (ctx && ctx->c.gen) {
+        if (which < 7 || which > 9 || reps < 1 || !avg_ms)
+            return fail(ctx, AMRESSIF_ERR_UNSUPPORTED, "time_kernel: 3D or periodic contexts time kernels 7..9");
+        return amr_gen_time_kernel(ctx, which - 7, reps, avg_ms, bytes_per_launch);
+    }
     if (!ctx || !avg_ms || reps < 1) return fail(ctx, AMRESSIF_ERR_ARG, "bad argument");
     Ctx &c = ctx->c;
     c.pred_valid = 0;  // the timed kernels overwrite the step buffers
@@ -1831,6 +1883,8 @@ extern "C" {
 
 int amressif_tape_create(amressif_ctx *ctx, int max_steps, amressif_tape **out)
 {
+    if (ctx && ctx->c.gen)
+        return amr_fail(const_cast<amressif_ctx *>(ctx), AMRESSIF_ERR_UNSUPPORTED, "amressif_tape_create: not available for 3D or periodic inputs");
     if (ctx && ctx->c.in.PSEUDO_TIMESTEPPING == 0)
         return fail(ctx, AMRESSIF_ERR_UNSUPPORTED, "the adjoint is that of the dual-time step (PSEUDO_TIMESTEPPING = 1)");
     if (!ctx || !out || max_steps < 1) return fail(ctx, AMRESSIF_ERR_ARG, "bad argument");
@@ -1873,6 +1927,8 @@ int amressif_tape_steps(const amressif_tape *tape) { return tape ? tape->count :
 
 int amressif_set_adjoint_solver(amressif_ctx *ctx, double tol, int max_sweeps)
 {
+    if (ctx && ctx->c.gen)
+        return amr_fail(const_cast<amressif_ctx *>(ctx), AMRESSIF_ERR_UNSUPPORTED, "amressif_set_adjoint_solver: not available for 3D or periodic inputs");
     if (!ctx || !(tol >= 0.0) || max_sweeps < 1) return fail(ctx, AMRESSIF_ERR_ARG, "bad argument");
     ctx->c.adj_tol = tol;
     ctx->c.adj_cap = max_sweeps;
@@ -1882,6 +1938,8 @@ int amressif_set_adjoint_solver(amressif_ctx *ctx, double tol, int max_sweeps)
 int amressif_load_dual(amressif_ctx *ctx, const amr_array4 du[2], const amr_array4 du_prev[2], const amr_array4 *dp,
                        const double *dlid)
 {
+    if (ctx && ctx->c.gen)
+        return amr_fail(const_cast<amressif_ctx *>(ctx), AMRESSIF_ERR_UNSUPPORTED, "amressif_load_dual: not available for 3D or periodic inputs");
     if (!ctx) return AMRESSIF_ERR_ARG;
     if (int rc = ensure_adjoint(ctx)) return rc;
     Ctx &c = ctx->c;
@@ -1916,6 +1974,8 @@ int amressif_load_dual(amressif_ctx *ctx, const amr_array4 du[2], const amr_arra
 
 int amressif_store_dual(amressif_ctx *ctx, amr_array4 du[2], amr_array4 du_prev[2], amr_array4 *dp, double *dlid)
 {
+    if (ctx && ctx->c.gen)
+        return amr_fail(const_cast<amressif_ctx *>(ctx), AMRESSIF_ERR_UNSUPPORTED, "amressif_store_dual: not available for 3D or periodic inputs");
     if (!ctx) return AMRESSIF_ERR_ARG;
     Ctx &c = ctx->c;
     if (!c.adj) return fail(ctx, AMRESSIF_ERR_ARG, "no dual state");
@@ -1940,6 +2000,8 @@ int amressif_store_dual(amressif_ctx *ctx, amr_array4 du[2], amr_array4 du_prev[
 
 int amressif_step_tlm(amressif_ctx *ctx, const amressif_tape *tape, int first, int nsteps, amressif_lin_stats *stats)
 {
+    if (ctx && ctx->c.gen)
+        return amr_fail(const_cast<amressif_ctx *>(ctx), AMRESSIF_ERR_UNSUPPORTED, "amressif_step_tlm: not available for 3D or periodic inputs");
     if (!ctx) return AMRESSIF_ERR_ARG;
     if (int rc = check_tape(ctx, tape)) return rc;
     if (first < 0 || nsteps < 0 || first + nsteps > tape->count)
@@ -1981,6 +2043,8 @@ int amressif_step_tlm(amressif_ctx *ctx, const amressif_tape *tape, int first, i
 
 int amressif_step_bwd(amressif_ctx *ctx, amressif_tape *tape, int nsteps, amressif_lin_stats *stats)
 {
+    if (ctx && ctx->c.gen)
+        return amr_fail(const_cast<amressif_ctx *>(ctx), AMRESSIF_ERR_UNSUPPORTED, "amressif_step_bwd: not available for 3D or periodic inputs");
     if (!ctx) return AMRESSIF_ERR_ARG;
     if (int rc = check_tape(ctx, tape)) return rc;
     if (nsteps < 0 || nsteps > tape->count) return fail(ctx, AMRESSIF_ERR_ARG, "more steps than the tape holds");

@@ -12,6 +12,10 @@
 
 namespace amr {
 
+namespace gen {
+struct Ctx;  // the general path's working set (gen.hpp)
+}
+
 // Device-side control block.  Written only by single-block finalize kernels;
 // every guarded kernel reads its `done` flag first and returns if it is set,
 // so the host can enqueue work ahead and poll it once per batch.
@@ -207,6 +211,7 @@ struct Ctx {
     std::vector<void *> allocs;
     std::string err;
     KProf *kp = nullptr;  // in-step kernel timing (null: off)
+    gen::Ctx *gen = nullptr;  // 3D or periodic inputs: the general path (gen_path.cpp) owns the state
 };
 
 // ---- kernel launchers (kernels_*.hip)
@@ -426,3 +431,13 @@ int amr_state_from_host(amressif_ctx *ctx, const double *u, const double *v, con
 int amr_fail(amressif_ctx *ctx, int code, const std::string &msg);
 // the module-level entry points' scratch arrays M[4][2], Mc[2] (allocated on first use)
 int amr_module_scratch(amressif_ctx *ctx);
+
+// the general path (gen_path.cpp): 3D inputs or a periodic direction
+bool amr_gen_wanted(const amressif_inputs *in);
+int amr_gen_create(amressif_ctx *ctx);
+int amr_gen_load_state(amressif_ctx *ctx, const amr_array4 *vel, const amr_array4 *velPrev, const amr_array4 *userCtx);
+int amr_gen_store_state(amressif_ctx *ctx, amr_array4 *vel, amr_array4 *velPrev, amr_array4 *userCtx,
+                        amr_array4 *velCart);
+int amr_gen_init_state(amressif_ctx *ctx);
+int amr_gen_advance(amressif_ctx *ctx, int nsteps, amressif_step_stats *stats);
+int amr_gen_time_kernel(amressif_ctx *ctx, int which, int reps, double *avg_ms, double *bytes_per_launch);

@@ -442,6 +442,8 @@ extern "C" {
 
 int amressif_write_checkpoint(amressif_ctx *ctx, const char *dir, int step, double time)
 {
+    if (ctx && ctx->c.gen)
+        return amr_fail(const_cast<amressif_ctx *>(ctx), AMRESSIF_ERR_UNSUPPORTED, "amressif_write_checkpoint: not available for 3D or periodic inputs");
     if (!ctx || !dir || step < 0) return amr_fail(ctx, AMRESSIF_ERR_ARG, "write_checkpoint: bad argument");
     const amr::Ctx &c = ctx->c;
     const int n = c.g.n;
@@ -478,6 +480,8 @@ int amressif_write_checkpoint(amressif_ctx *ctx, const char *dir, int step, doub
 
 int amressif_read_checkpoint(amressif_ctx *ctx, const char *path, double *time)
 {
+    if (ctx && ctx->c.gen)
+        return amr_fail(const_cast<amressif_ctx *>(ctx), AMRESSIF_ERR_UNSUPPORTED, "amressif_read_checkpoint: not available for 3D or periodic inputs");
     if (!ctx || !path) return amr_fail(ctx, AMRESSIF_ERR_ARG, "read_checkpoint: bad argument");
     const int n = ctx->c.g.n;
     const std::string root(path);
@@ -519,6 +523,8 @@ int amressif_read_checkpoint(amressif_ctx *ctx, const char *path, double *time)
 
 int amressif_write_plotfile(amressif_ctx *ctx, const char *dir, const char *name, int step, double time)
 {
+    if (ctx && ctx->c.gen)
+        return amr_fail(const_cast<amressif_ctx *>(ctx), AMRESSIF_ERR_UNSUPPORTED, "amressif_write_plotfile: not available for 3D or periodic inputs");
     if (!ctx || !dir || !name || step < 0) return amr_fail(ctx, AMRESSIF_ERR_ARG, "write_plotfile: bad argument");
     const amr::Ctx &c = ctx->c;
     const int n = c.g.n;
@@ -535,6 +541,8 @@ int amressif_write_plotfile(amressif_ctx *ctx, const char *dir, const char *name
 
 int amressif_write_flux_plotfile(amressif_ctx *ctx, const char *dir, int step, double time)
 {
+    if (ctx && ctx->c.gen)
+        return amr_fail(const_cast<amressif_ctx *>(ctx), AMRESSIF_ERR_UNSUPPORTED, "amressif_write_flux_plotfile: not available for 3D or periodic inputs");
     if (!ctx || !dir || step < 0) return amr_fail(ctx, AMRESSIF_ERR_ARG, "write_flux_plotfile: bad argument");
     amr::Ctx &c = ctx->c;
     if (!c.pred_valid)
@@ -562,6 +570,11 @@ int amressif_write_flux_plotfile(amressif_ctx *ctx, const char *dir, int step, d
 
 int amressif_init_state(amressif_ctx *ctx, const double *psi_coeffs, int kmax)
 {
+    if (ctx && ctx->c.gen) {
+        if (psi_coeffs)
+            return amr_fail(ctx, AMRESSIF_ERR_UNSUPPORTED, "init_state: the seeded field is 2D walls only");
+        return amr_gen_init_state(ctx);
+    }
     if (!ctx || (psi_coeffs && (kmax < 1 || kmax > 64)))
         return amr_fail(ctx, AMRESSIF_ERR_ARG, "init_state: bad argument");
     const int n = ctx->c.g.n;

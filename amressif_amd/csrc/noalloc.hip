@@ -116,6 +116,8 @@ bool covers(const amr_array4 &a, int lo0, int lo1, int hi0, int hi1)
 extern "C" int amressif_poisson_rhs_noalloc(amressif_ctx *ctx, const amr_array4 vel[2], amr_array4 *rhs,
                                             double *dev_out)
 {
+    if (ctx && ctx->c.gen)
+        return amr_fail(const_cast<amressif_ctx *>(ctx), AMRESSIF_ERR_UNSUPPORTED, "amressif_poisson_rhs_noalloc: not available for 3D or periodic inputs");
     if (!ctx || !vel || !rhs || !dev_out) return amr_fail(ctx, AMRESSIF_ERR_ARG, "poisson_rhs_noalloc: null argument");
     const amr::Ctx &c = ctx->c;
     const int n = c.g.n;

@@ -31,7 +31,7 @@ extern "C" {
 #define AMRESSIF_OK 0
 #define AMRESSIF_ERR_ARG (-1)         /* bad argument / shape mismatch */
 #define AMRESSIF_ERR_BC (-2)          /* BC code the reference aborts on (main.cpp:822,832; utilities.cpp:128) */
-#define AMRESSIF_ERR_UNSUPPORTED (-3) /* valid in the reference but not built here (periodic 111, 3D; PSEUDO_TIMESTEPPING 0 on the tape and the two-level step) */
+#define AMRESSIF_ERR_UNSUPPORTED (-3) /* valid in the reference but not built here (see the entry point) */
 #define AMRESSIF_ERR_HIP (-4)         /* HIP runtime failure */
 #define AMRESSIF_ERR_NONFINITE (-5)   /* non-finite pseudo-iteration norm (reference exits silently) */
 #define AMRESSIF_ERR_PARSE (-6)       /* inputs file error */
@@ -92,6 +92,8 @@ typedef struct {
                                    alone right after a state load) instead of 0 (poisson.cpp:63);
                                    the stopping test stays ||r||inf <= tol_rel ||b||inf.
                                    -1 (default): on for precision = 1, off for FP64; 0 off; 1 on */
+    int spacedim;               /* AMREX_SPACEDIM (Exec/GNUmakefile:10, a compile-time DIM in the reference):
+                                   2 (default) or 3; the domain is n_cell^spacedim cells */
 } amressif_inputs;
 
 /* AMReX Array4 view of one box: ptr addresses element (lo[0], lo[1], lo[2], comp 0);
@@ -367,6 +369,11 @@ int amressif_amr_advance(amressif_amr *amr, int nsteps, amressif_amr_stats *stat
  *   which = 4  mixed precision: FP32 tangent-linear update of the pseudo base
  *   which = 5  finest-level up-leg smoother with the fused FP64 outer step (V-cycle precision)
  *   which = 6  two FP64 RK stages in one pass (stages 1-2 of a pseudo-iteration)
+ * On a 3D or periodic context (the general path) the kernels are those of one
+ * RK4 stage, each over the whole grid:
+ *   which = 7  cell velocities with their ghost layers (cont2cart)
+ *   which = 8  fluxTotal: QUICK convective and viscous fluxes per cell
+ *   which = 9  stage update of every face
  * which = 3 and 4 need a context created with precision = 1.
  * bytes_per_launch receives the kernel's algorithmic HBM bytes per launch. */
 int amressif_time_kernel(amressif_ctx *ctx, int which, int reps, int64_t copy_bytes, double *avg_ms,

@@ -76,8 +76,14 @@ def test_create_rejects_codes_the_reference_aborts_on(api):
     assert _create_status(api, phy_bc_lo=(0, 135, 135))[0] == -2      # main.cpp:822 Invalid bc_lo
     assert _create_status(api, phy_bc_lo=(195, 135, 135))[0] == -2    # utilities.cpp:128 Not yet implemented
     assert _create_status(api, phy_bc_hi=(135, -165, 135))[0] == -2
-    rc, msg = _create_status(api, phy_bc_lo=(111, 135, 135), phy_bc_hi=(111, 165, 135))
+    # periodic needs 111 on both sides of a direction (is_periodic, main.cpp:159-166)
+    rc, msg = _create_status(api, phy_bc_lo=(111, 135, 135), phy_bc_hi=(135, 165, 135))
     assert rc == -3 and "periodic" in msg
+    assert _create_status(api, spacedim=4)[0] == -1
+    # the general path (3D, periodic) is FP64 with the multigrid solve
+    assert _create_status(api, spacedim=3, precision=1)[0] == -3
+    assert _create_status(api, phy_bc_lo=(111, 135, 135), phy_bc_hi=(111, 165, 135), poisson_solver=1)[0] == -3
+    assert _create_status(api, spacedim=3, phy_bc_lo=(135, 135, -165))[0] == -2  # utilities.cpp:296 Not yet implemented
     assert _create_status(api, PSEUDO_TIMESTEPPING=2)[0] == -1     # only 0 (explicit) and 1 (dual time)
     assert _create_status(api, precision=7)[0] == -1
 

@@ -530,7 +530,7 @@ class Context:
 
     # in-step kernel timing (HIP events around the step's main launches)
     KERNELS = ("stage", "stage_last", "resid", "tangent", "mg_down", "mg_up", "project", "poisson_start", "pair",
-               "pair_last")
+               "pair_last", "mpair")
 
     def kernel_timing(self, enable: bool) -> None:
         """start (clearing the totals) or stop per-launch timing inside advance()"""

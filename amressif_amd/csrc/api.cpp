@@ -401,9 +401,15 @@ int amr_momentum_begin(amressif_ctx *ctx, bool bdf_ready, int *launched, bool re
 {
     Ctx &c = ctx->c;
     if (!reset_done) HIPCHK(amr::launch_ctl_reset_step(c));
-    if (bdf_ready) {  // the previous projection formed C and copied u^n to S2
-        std::swap(c.Uprev[0], c.S2[0]);
-        std::swap(c.Uprev[1], c.S2[1]);
+    c.mp_count = 0;
+    if (bdf_ready) {
+        // the previous projection formed C and copied u^n to S2 (with the
+        // mixed iteration in one pass it copied nothing: U and Uprev already
+        // hold u^n and u^{n-1})
+        if (!c.mp_on) {
+            std::swap(c.Uprev[0], c.S2[0]);
+            std::swap(c.Uprev[1], c.S2[1]);
+        }
     } else {
         HIPCHK(amr::launch_bdf2(c));
         c.c_gradp = 0;
@@ -482,6 +488,9 @@ int amr_set_ctl(amressif_ctx *ctx)
 
 namespace {
 
+// the output of the step's k-th k_mixed_pair launch: S1 for even k, S2 for odd
+double *const *mp_out(const Ctx &c, int k) { return k % 2 == 0 ? c.S1 : c.S2; }
+
 int launch_pseudo_iteration(amressif_ctx *ctx, int it, int *nb)
 {
     Ctx &c = ctx->c;
@@ -500,10 +509,26 @@ int launch_pseudo_iteration(amressif_ctx *ctx, int it, int *nb)
             a.cj = c.rk[2] * c.rk[3];
             a.cr = c.rk[3] - a.cj * c15;
         }
+        const double cells = (double)c.g.n * c.g.n;
+        if (c.mp_on) {
+            // one pass: u^k from U (the first launch, which leaves u^n in U) or
+            // the previous launch's output, u^{k+1} into S1 or S2
+            amr::MixedPairLaunch m{};
+            double *const *in = it == 0 ? c.U : mp_out(c, it - 1);
+            double *const *out = mp_out(c, it);
+            m.u64 = in[0]; m.v64 = in[1]; m.cu = c.C[0]; m.cv = c.C[1];
+            m.uo = out[0]; m.vo = out[1];
+            m.cr = a.cr; m.cj = a.cj;
+            // 48 B per cell: u^k, v^k, c_u, c_v in; u^{k+1}, v^{k+1} out
+            HIPCHK(amr::kprof_begin(c, amr::KP_MPAIR, amr::KL_MOM, it, 48.0 * cells));
+            HIPCHK(amr::launch_mixed_pair(c, m, nb));
+            HIPCHK(amr::kprof_end(c));
+            c.mp_count = it + 1;
+            return 0;
+        }
         a.stage = 0;
         // algorithmic bytes per cell: residual 48 (u^k, v^k, c_u, c_v in; FP32 base and
         // residual out), tangent 48 (FP32 base and residual in; u^k, v^k in and out)
-        const double cells = (double)c.g.n * c.g.n;
         HIPCHK(amr::kprof_begin(c, amr::KP_RESID, amr::KL_MOM, it, 48.0 * cells));
         HIPCHK(amr::launch_stage_mixed(c, a, nb));
         HIPCHK(amr::kprof_end(c));
@@ -789,6 +814,8 @@ int amressif_create(const amressif_inputs *in, void *hip_stream, amressif_ctx **
     if (const char *e = std::getenv("AMRESSIF_STAGE_FAST")) c.stage_fast = std::atoi(e) ? 1 : 0;
     if (const char *e = std::getenv("AMRESSIF_STAGE_PAIR")) c.stage_pair = std::atoi(e) ? 1 : 0;
     if (const char *e = std::getenv("AMRESSIF_PAIR_ROWS")) c.pair_rows = std::max(4, std::min(256, std::atoi(e)));
+    if (const char *e = std::getenv("AMRESSIF_MIXED_PAIR")) c.mixed_pair = std::atoi(e) ? 1 : 0;
+    if (const char *e = std::getenv("AMRESSIF_MPAIR_ROWS")) c.mpair_rows = std::max(4, std::min(256, std::atoi(e)));
     // and of the multigrid smoothers (kernel choice and rows per wave: 4, 8, 16 or 32)
     if (const char *e = std::getenv("AMRESSIF_MG_SMOOTHER")) c.mg_smoother = std::max(0, std::min(2, std::atoi(e)));
     if (const char *e = std::getenv("AMRESSIF_TAIL_DCT")) c.tail_dct = std::atoi(e) ? 1 : 0;
@@ -1324,6 +1351,17 @@ int advance_impl(amressif_ctx *ctx, int nsteps, amressif_step_stats *stats, amre
     // host wait between them; its kernels do nothing if that batch did not
     // end the loop, and the phase then runs again after the loop has ended.
     const bool speculate = !tape && c.in.poisson_solver != 1;
+    // the mixed iteration in one pass (k_mixed_pair) chains its output through
+    // S1 and S2 and leaves U = u^n untouched, so the projection needs no copy of
+    // u^{n+1}: it writes u^{n+1} to the free one of S1, S2 and the buffers rotate
+    struct MpScope {
+        Ctx &c;
+        ~MpScope() { c.mp_on = 0; }
+    } mp_scope{c};
+    c.mp_on = speculate && mixed && c.in.mixed_momentum == 1 && c.in.PSEUDO_TIMESTEPPING == 1 && c.mixed_pair &&
+              amr::mixed_pair_fits(c);
+    // where this step's predictor u* is
+    auto ustar = [&]() -> double *const * { return c.mp_on ? mp_out(c, c.mp_count - 1) : c.U; };
     for (int s = 0; s < nsteps; ++s) {
         int nb = 0;
         bool projected = false;  // the projection ran as the Poisson phase's gated tail
@@ -1338,32 +1376,61 @@ int advance_impl(amressif_ctx *ctx, int nsteps, amressif_step_stats *stats, amre
             // the host undoes its buffer swaps if it has not
             const bool fold_next = s + 1 < nsteps;
             const int saved_gradp = c.c_gradp;
+            double *saved[4][2] = {};  // U, Uprev, S1, S2 before a rotation (undo)
             SpecTail tail;
             tail.enqueue = [&]() -> int {
-                HIPCHK(amr::kprof_begin(c, amr::KP_PROJECT, amr::KL_PROJ, 0,
-                                        (fold_next ? 104.0 : 56.0) * (double)c.g.n * c.g.n));
-                HIPCHK(amr::launch_project_ex(c, c.U[0], c.U[1], c.S1[0], c.S1[1], c.p, c.phiv[0], 1, &nb, nullptr,
-                                              nullptr, fold_next ? 1 : 0, c.p_alt, c.phiv[1], 1));
-                HIPCHK(amr::kprof_end(c));
-                HIPCHK(amr::launch_fin_project(c, nb, s, fold_next ? 1 : 0, 1));
-                std::swap(c.U[0], c.S1[0]);
-                std::swap(c.U[1], c.S1[1]);
+                // per cell: u*, v*, phi, p in, u, v, p out (56 B); with the next step's
+                // BDF2 constant also u^n in, c out and (without the one-pass mixed
+                // iteration) the copy of u^{n+1} out: 88 / 104 B
+                const double fb = fold_next ? (c.mp_on ? 88.0 : 104.0) : 56.0;
+                HIPCHK(amr::kprof_begin(c, amr::KP_PROJECT, amr::KL_PROJ, 0, fb * (double)c.g.n * c.g.n));
+                if (c.mp_on) {
+                    double *X[2] = {ustar()[0], ustar()[1]};
+                    double *const *Yp = X[0] == c.S1[0] ? c.S2 : c.S1;
+                    double *Y[2] = {Yp[0], Yp[1]};
+                    HIPCHK(amr::launch_project_ex(c, X[0], X[1], Y[0], Y[1], c.p, c.phiv[0], 1, &nb, nullptr, nullptr,
+                                                  fold_next ? 1 : 0, c.p_alt, c.phiv[1], 1, c.U, 1));
+                    HIPCHK(amr::kprof_end(c));
+                    HIPCHK(amr::launch_fin_project(c, nb, s, fold_next ? 1 : 0, 1));
+                    for (int d = 0; d < 2; ++d) {
+                        saved[0][d] = c.U[d]; saved[1][d] = c.Uprev[d]; saved[2][d] = c.S1[d]; saved[3][d] = c.S2[d];
+                        // U <- u^{n+1}, Uprev <- u^n, S1 <- u*, S2 <- the free array
+                        c.Uprev[d] = saved[0][d];
+                        c.S2[d] = saved[1][d];
+                        c.U[d] = Y[d];
+                        c.S1[d] = X[d];
+                    }
+                } else {
+                    HIPCHK(amr::launch_project_ex(c, c.U[0], c.U[1], c.S1[0], c.S1[1], c.p, c.phiv[0], 1, &nb, nullptr,
+                                                  nullptr, fold_next ? 1 : 0, c.p_alt, c.phiv[1], 1));
+                    HIPCHK(amr::kprof_end(c));
+                    HIPCHK(amr::launch_fin_project(c, nb, s, fold_next ? 1 : 0, 1));
+                    std::swap(c.U[0], c.S1[0]);
+                    std::swap(c.U[1], c.S1[1]);
+                }
                 std::swap(c.p, c.p_alt);
                 c.c_gradp = fold_next && c.project_rows > 0;
                 return 0;
             };
             tail.undo = [&]() {
-                std::swap(c.U[0], c.S1[0]);
-                std::swap(c.U[1], c.S1[1]);
+                if (c.mp_on) {
+                    for (int d = 0; d < 2; ++d) {
+                        c.U[d] = saved[0][d]; c.Uprev[d] = saved[1][d]; c.S1[d] = saved[2][d]; c.S2[d] = saved[3][d];
+                    }
+                } else {
+                    std::swap(c.U[0], c.S1[0]);
+                    std::swap(c.U[1], c.S1[1]);
+                }
                 std::swap(c.p, c.p_alt);
                 c.c_gradp = saved_gradp;
                 amr::kprof_close(c, amr::KL_PROJ, 0);
             };
             bool skipped = false;
-            if ((rc = mg_run(ctx, w, false, c.U[0], c.U[1], true, &skipped, &tail, &projected))) return rc;
+            if ((rc = mg_run(ctx, w, false, ustar()[0], ustar()[1], true, &skipped, &tail, &projected))) return rc;
             if (skipped) {
                 if ((rc = amr_momentum_finish(ctx, launched))) return rc;
-                if ((rc = mg_run(ctx, w, false, c.U[0], c.U[1], false, nullptr, &tail, &projected))) return rc;
+                if ((rc = mg_run(ctx, w, false, ustar()[0], ustar()[1], false, nullptr, &tail, &projected)))
+                    return rc;
             } else {
                 c.last_P = std::max(1, c.ctl_host->mom_count);
                 amr::kprof_close(c, amr::KL_MOM, c.ctl_host->mom_count);
@@ -1397,17 +1464,32 @@ int advance_impl(amressif_ctx *ctx, int nsteps, amressif_step_stats *stats, amre
             // p^{n+1} goes to the other pressure array (p^n stays readable for the
             // neighbour terms), and the next constant takes -G p^{n+1}
             HIPCHK(amr::kprof_begin(c, amr::KP_PROJECT, amr::KL_ONCE, 0,
-                                    (fold ? 104.0 : 56.0) * (double)c.g.n * c.g.n));
-            HIPCHK(amr::launch_project_ex(c, c.U[0], c.U[1], c.S1[0], c.S1[1], c.p, c.phi, 1, &nb, nullptr, nullptr,
-                                          fold ? 1 : 0, c.p_alt));
+                                    (fold ? (c.mp_on ? 88.0 : 104.0) : 56.0) * (double)c.g.n * c.g.n));
+            if (c.mp_on) {  // u* in S1 or S2, u^{n+1} into the other, then the buffers rotate
+                double *X[2] = {ustar()[0], ustar()[1]};
+                double *const *Yp = X[0] == c.S1[0] ? c.S2 : c.S1;
+                double *Y[2] = {Yp[0], Yp[1]};
+                HIPCHK(amr::launch_project_ex(c, X[0], X[1], Y[0], Y[1], c.p, c.phi, 1, &nb, nullptr, nullptr,
+                                              fold ? 1 : 0, c.p_alt, nullptr, 0, c.U, 1));
+                for (int d = 0; d < 2; ++d) {
+                    double *un = c.U[d], *um = c.Uprev[d];
+                    c.Uprev[d] = un;
+                    c.S2[d] = um;
+                    c.U[d] = Y[d];
+                    c.S1[d] = X[d];
+                }
+            } else {
+                HIPCHK(amr::launch_project_ex(c, c.U[0], c.U[1], c.S1[0], c.S1[1], c.p, c.phi, 1, &nb, nullptr,
+                                              nullptr, fold ? 1 : 0, c.p_alt));
+                std::swap(c.U[0], c.S1[0]);
+                std::swap(c.U[1], c.S1[1]);
+            }
             HIPCHK(amr::kprof_end(c));
             std::swap(c.p, c.p_alt);
             c.c_gradp = fold && c.project_rows > 0;
             // (between two steps of the call it also clears the next step's control)
             HIPCHK(amr::launch_fin_project(c, nb, s, s + 1 < nsteps ? 1 : 0));
             bdf_ready = fold;
-            std::swap(c.U[0], c.S1[0]);
-            std::swap(c.U[1], c.S1[1]);
         }
         if (flags & AMRESSIF_FLAG_NONFINITE) {
             status = fail(ctx, AMRESSIF_ERR_NONFINITE, "non-finite pseudo-iteration norm at step " + std::to_string(s));

@@ -103,6 +103,7 @@ enum KProfId {
     KP_PSTART,       // k_poisson_start
     KP_PAIR,         // k_stage_pair<false>: FP64 RK stages 1-2
     KP_PAIR_LAST,    // k_stage_pair<true>: FP64 RK stages 3-4 + norm partials
+    KP_MPAIR,        // k_mixed_pair: mixed residual + tangent-linear update in one pass
     KP_COUNT
 };
 enum KProfLoop { KL_MOM = 0, KL_MG, KL_PHASE, KL_ONCE, KL_PROJ };
@@ -152,6 +153,12 @@ struct Ctx {
     int stage_fast = 1;      // k_stage64: unrolled interior path (0: every wave on the general march)
     int stage_pair = 1;      // FP64 pseudo-iteration as two k_stage_pair launches (0: four k_stage64)
     int pair_rows = 16;      // k_stage_pair: output rows per wave
+    // mixed iteration as one k_mixed_pair launch (0: k_stage64 mode 3 + k_tangent);
+    // used by the step without a tape (advance) with the multigrid solve
+    int mixed_pair = 1;
+    int mpair_rows = 16;     // k_mixed_pair: output rows per wave
+    int mp_on = 0;           // this step's mixed iterations run as k_mixed_pair (advance_impl)
+    int mp_count = 0;        // k_mixed_pair launches of the step so far
     // multigrid smoothers of the levels above the tail: 2 = wave-column kernel
     // with two columns per lane, 1 = one column per lane (both registers +
     // DPP), 0 = LDS tile kernel; output rows per wave of the wave-column
@@ -251,6 +258,17 @@ struct MixedStageLaunch {
     int foldp;                   // stage 0: cu, cv already hold -G p; p is not read
 };
 hipError_t launch_stage_mixed(const Ctx &c, const MixedStageLaunch &a, int *nblocks);
+// one mixed iteration in one launch (k_mixed_pair): u^{k+1} of u^k into uo, vo
+// (which must not alias u^k); after convergence it copies u^k
+struct MixedPairLaunch {
+    const double *u64, *v64;  // pseudo base u^k
+    const double *cu, *cv;    // BDF2 constant holding -G p
+    double *uo, *vo;          // u^{k+1}
+    double cr, cj;            // u^{k+1} = u^k + cr r + cj R'(u^k) r
+    int unguarded;
+};
+hipError_t launch_mixed_pair(const Ctx &c, const MixedPairLaunch &a, int *nblocks);
+bool mixed_pair_fits(const Ctx &c);
 hipError_t launch_fin_pseudo(const Ctx &c, int nblocks);
 // the norm of PSEUDO_TIMESTEPPING = 0's single update: ends the loop, countIter stays 0
 hipError_t launch_fin_explicit(const Ctx &c, int nblocks);
@@ -298,11 +316,13 @@ hipError_t launch_ctl_set(const Ctx &c, double tol_mom, double tol_rel, double t
 // separate p_out, the row-march kernel also folds -G p^{n+1} into the next
 // constant (project_rows > 0).  gated: enqueued before the host saw the solve
 // end (does nothing unless it has); phi_alt: the iterate buffer taken after an
-// odd cycle count.
+// odd cycle count.  un: where next_bdf takes u^n (null: Uprev); no_copy: no
+// copy of the new velocity into S2.
 hipError_t launch_project_ex(const Ctx &c, const double *us, const double *vs, double *uo, double *vo, double *p,
                              const double *phi, int use_mean, int *nblocks, const float *au = nullptr,
                              const float *av = nullptr, int next_bdf = 0, double *p_out = nullptr,
-                             const double *phi_alt = nullptr, int gated = 0);
+                             const double *phi_alt = nullptr, int gated = 0, double *const *un = nullptr,
+                             int no_copy = 0);
 hipError_t launch_unpack_cell_shift(const Ctx &c, const double *src, const amr_array4 &a, int comp,
                                     int fill_ghosts, const double *shift);
 hipError_t launch_copy_bytes(const Ctx &c, const void *src, void *dst, int64_t bytes);

@@ -1070,6 +1070,259 @@ __global__ __launch_bounds__(256) void k_tangent(StageArgs a)
     if (lane == 0) { a.partials[2 * wid] = nx_; a.partials[2 * wid + 1] = ny_; }
 }
 
+// ======================================= one mixed iteration in one pass
+// The mixed momentum iteration (k_stage64 mode 3, then k_tangent) in one
+// launch.  The layout is k_stage_pair's: a wave owns 52 output columns with a
+// 6-lane halo on each side and a chunk of rows.  March A is k_stage64's mode 3,
+// the FP64 residual r = R(u^k) - (1.5/dt) u^k + c of the pseudo base, three
+// rows ahead of march B, k_tangent's u^{k+1} = u^k + cr r + cj R'(u^k) r.  B
+// takes the FP32 base and residual faces (rounded as k_stage64 stores them)
+// and the FP64 base faces from A's registers: a delay line of four x-face and
+// four y-face rows.  A is exact on lanes 3..60, so B is exact on lanes 6..57;
+// B's cells outside the domain are the wall ghosts of its own window cells.
+// Per face the arithmetic is the two kernels', in the same precisions, so the
+// fields are those of the two launches; the norm partials are summed per wave
+// of this layout.  Memory traffic: u^k and c in, u^{k+1} out, 48 B per cell
+// (the two launches move 96).  The output must not alias u^k: A reads u^k in
+// the halo rows and columns that neighbouring waves update.  After convergence
+// (the launch skipped) u^k is copied to the output, so that the caller's
+// exchange of the buffers keeps the converged state.
+__global__ __launch_bounds__(256) void k_mixed_pair(StageArgs a)
+{
+    using R = double;
+    using F = float;
+    const int lane = threadIdx.x & 63;
+    const int wid = wave_id(a);
+    if (wid >= a.nbands * a.nchunks) return;  // whole wave
+    const int band = a.order ? wid % a.nbands : wid / a.nchunks;
+    const int chunk = a.order ? wid / a.nbands : wid % a.nchunks;
+    const int n = a.n, P = a.P;
+    const int i0 = band * WB2, j0 = chunk * a.rows;
+    const int i = i0 + lane - 6;
+    const bool xin = i >= 0 && i < n, xface = i >= 0 && i <= n;
+    const bool outA = lane >= 3 && lane <= 60, outB = lane >= 6 && lane <= 57;
+    const int jend = min(j0 + a.rows, n);
+    const size_t ic = (size_t)(i < 0 ? 0 : i);
+    const double *us = a.us, *vs = a.vs;  // u^k: the stencil input and the base
+    if (a.ctl && a.ctl->mom_done) {       // the same for every wave of the launch
+        if (outB) {
+            for (int j = j0; j < jend; ++j) {
+                const size_t k = ic + (size_t)P * j;
+                if (xface) a.uo[k] = us[k];
+                if (xin) a.vo[k] = vs[k];
+            }
+            if (jend == n && xin) a.vo[ic + (size_t)P * n] = vs[ic + (size_t)P * n];
+        }
+        return;
+    }
+    const bool edge = i0 < 6 || i0 + 58 > n;
+    const R gx = (xin && a.lid) ? a.lid[i] : a.inflow0;
+    const F gxf = (F)((xin && a.lid) ? a.lid[i] : a.inflow0);
+    const R idx = a.inv_dx, idy = a.inv_dy, idx2 = a.inv_dx2, idy2 = a.inv_dy2, iren = a.inv_ren;
+    const int ja0 = max(j0 - 3, 0), jaEnd = min(jend + 3, n);
+
+    // ---------------------------------------------------------------- march A
+    struct Raw {
+        R u0, v0, v1, vn, uw, cu, cv;  // as k_stage64's (u(i + 1) = u0 of lane + 1)
+    };
+    auto loadA = [&](int jc, Raw &r) {  // everything A's iteration jc consumes
+        const int jr = jc + 2;
+        const int jm = jr < 0 ? -jr - 1 : (jr >= n ? 2 * n - 1 - jr : jr);
+        r.u0 = r.v0 = r.v1 = 0;
+        if (jm >= 0 && jm < n && xface) {
+            const size_t k = i + (size_t)P * jm;
+            r.u0 = us[k];
+            if (xin) { r.v0 = vs[k]; r.v1 = vs[k + P]; }
+        }
+        r.vn = (xin && jc + 1 >= 0 && jc + 1 <= n) ? vs[i + (size_t)P * (jc + 1)] : R(0);
+        const bool row = jc >= 0 && jc < n;
+        const size_t k = ic + (size_t)P * (jc < 0 ? 0 : jc);
+        r.uw = (xface && row) ? us[k] : R(0);
+        const bool out = row && jc >= ja0 && outA;
+        r.cu = (out && xface) ? a.cu[k] : R(0);
+        r.cv = (out && xin) ? a.cv[k] : R(0);
+    };
+    R aU[5], aV[5];  // A's rolling window: cell (i, ja - 2 + q)
+    {
+        Raw r;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            loadA(ja0 - 5 + q, r);
+            finish_cells<R, false>(a, lane, i, ja0 - 3 + q, edge, r.u0, lane_next(r.u0), r.v0, r.v1, gx, aU[q], aV[q]);
+        }
+    }
+    R avn = (xin && ja0 - 1 >= 0) ? vs[i + (size_t)P * (ja0 - 1)] : R(0);
+    R afsU = quick_flux<R>(avn, aU[0], aU[1], aU[2], aU[3]), afsV = quick_flux<R>(avn, aV[0], aV[1], aV[2], aV[3]);
+    R atV = 0;
+    Raw acur, anxt{};
+    loadA(ja0 - 1, acur);
+    int ja = ja0 - 1;  // A's next iteration
+    // the constant of the north wall row of y-faces
+    const R cvN = (jaEnd == n && xin) ? a.cv[ic + (size_t)P * n] : R(0);
+    // one face of A's output: the FP32 base and residual (k_stage64's stores) and the FP64 base
+    struct Face {
+        F b, r;
+        R b64;
+    };
+    // A's iteration ja (ja < n): the faces of row ja (0 off the domain)
+    auto stepA = [&](Face &xo, Face &yo) {
+        if (ja + 1 < jaEnd) loadA(ja + 1, anxt);  // prefetch
+        finish_cells<R, false>(a, lane, i, ja + 2, edge, acur.u0, lane_next(acur.u0), acur.v0, acur.v1, gx, aU[4],
+                               aV[4]);
+        const R vnf = acur.vn;
+        const R fnU = quick_flux<R>(vnf, aU[1], aU[2], aU[3], aU[4]);
+        const R fnV = quick_flux<R>(vnf, aV[1], aV[2], aV[3], aV[4]);
+        const R Uc = aU[2], Vc = aV[2];
+        const R Um1 = lane_prev(Uc), Um2 = lane_prev(Um1), Up1 = lane_next(Uc);
+        const R Vm1 = lane_prev(Vc), Vm2 = lane_prev(Vm1), Vp1 = lane_next(Vc);
+        const R uw = acur.uw;
+        const R fwU = quick_flux<R>(uw, Um2, Um1, Uc, Up1), fwV = quick_flux<R>(uw, Vm2, Vm1, Vc, Vp1);
+        const R feU = lane_next(fwU), feV = lane_next(fwV);
+        const R CU = (feU - fwU) * idx + (fnU - afsU) * idy;
+        const R CV = (feV - fwV) * idx + (fnV - afsV) * idy;
+        const R VU = ((Up1 - R(2) * Uc + Um1) * idx2 + (aU[3] - R(2) * Uc + aU[1]) * idy2) * iren;
+        const R VV = ((Vp1 - R(2) * Vc + Vm1) * idx2 + (aV[3] - R(2) * Vc + aV[1]) * idy2) * iren;
+        const R TU = (R(0) - CU) + VU, TV = (R(0) - CV) + VV;
+        const R TUw = lane_prev(TU);
+        const double Rx = (i == 0 || i == n) ? 0.0 : R(0.5) * (TUw + TU);
+        const double Ry = ja == 0 ? 0.0 : R(0.5) * (atV + TV);
+        xo.b64 = xface ? uw : R(0);
+        xo.b = xface ? (float)uw : 0.f;
+        xo.r = xface ? (float)(Rx - a.c15 * uw + acur.cu) : 0.f;
+        yo.b64 = xin ? avn : R(0);
+        yo.b = xin ? (float)avn : 0.f;
+        yo.r = xin ? (float)(Ry - a.c15 * avn + acur.cv) : 0.f;
+        afsU = fnU; afsV = fnV; atV = TV; avn = vnf;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { aU[q] = aU[q + 1]; aV[q] = aV[q + 1]; }
+        acur = anxt;
+        ++ja;
+    };
+    // A's next row: a residual row below n, the north wall row at n (R = 0 there)
+    auto emitA = [&](Face &xo, Face &yo) {
+        if (ja < n) {
+            stepA(xo, yo);
+        } else {
+            xo = Face{0.f, 0.f, R(0)};
+            if (ja == n && xin) yo = Face{(float)avn, (float)(-a.c15 * avn + cvN), avn};
+            else yo = Face{0.f, 0.f, R(0)};
+            ++ja;
+        }
+    };
+
+    // ---------------------------------------------------------------- march B
+    // cell row jr in [0, n) from A's faces: the base (wall rules) and the
+    // residual (the rules without their constants)
+    auto cellB = [&](int jr, const Face &u, const Face &v0, const Face &v1, F &U, F &V, F &dU, F &dV) {
+        finish_cells<F, false>(a, lane, i, jr, edge, u.b, lane_next(u.b), v0.b, v1.b, gxf, U, V);
+        finish_cells<F, true>(a, lane, i, jr, edge, u.r, lane_next(u.r), v0.r, v1.r, F(0), dU, dV);
+    };
+    // the wall ghost of cell row jr (outside [0, n)) from its mirror cell
+    auto ghostB = [&](int jr, F Um, F Vm, F dUm, F dVm, F &U, F &V, F &dU, F &dV) {
+        U = Um; V = Vm; dU = dUm; dV = dVm;
+        if (xin) {
+            ghost_rule<F, false>(jr < 0 ? 2 : 3, jr < 0 ? a.bc_lo1 : a.bc_hi1, Um, Vm, gxf, (F)a.inflow1, U, V);
+            ghost_rule<F, true>(jr < 0 ? 2 : 3, jr < 0 ? a.bc_lo1 : a.bc_hi1, dUm, dVm, F(0), (F)a.inflow1, dU, dV);
+        }
+    };
+    F wU[5], wV[5], dU[5], dV[5];  // B's rolling windows: base and residual cells (i, jb - 2 + q)
+    Face uq0, uq1, uq2, uq3{};      // A's x-faces of rows jb .. jb + 3
+    Face vq0, vq1, vq2, vq3{};      // A's y-faces of rows jb .. jb + 3
+    F fsU, fsV, tVprev = 0;         // B's flux through y-face jb; TV of row jb - 1
+    int jb;
+    {
+        Face t0, t1;
+        stepA(t0, t1);  // A's iteration ja0 - 1 (no row)
+    }
+    if (j0 == 0) {
+        // rows 0..2; B starts at row 0 (its south y-faces are the wall)
+        Face x0, y0, x1, y1, x2, y2;
+        emitA(x0, y0);
+        emitA(x1, y1);
+        emitA(x2, y2);
+        cellB(0, x0, y0, y1, wU[2], wV[2], dU[2], dV[2]);
+        cellB(1, x1, y1, y2, wU[3], wV[3], dU[3], dV[3]);
+        ghostB(-2, wU[3], wV[3], dU[3], dV[3], wU[0], wV[0], dU[0], dV[0]);
+        ghostB(-1, wU[2], wV[2], dU[2], dV[2], wU[1], wV[1], dU[1], dV[1]);
+        vq0 = y0;
+        uq0 = x0; uq1 = x1; uq2 = x2; vq1 = y1; vq2 = y2;
+        jb = 0;
+    } else {
+        // rows j0 - 3 .. j0 + 1 (j0 >= 3); B starts at row j0 - 1
+        Face x0, y0, x1, y1, x2, y2, x3, y3, x4, y4;
+        emitA(x0, y0);
+        emitA(x1, y1);
+        cellB(j0 - 3, x0, y0, y1, wU[0], wV[0], dU[0], dV[0]);
+        emitA(x2, y2);
+        cellB(j0 - 2, x1, y1, y2, wU[1], wV[1], dU[1], dV[1]);
+        emitA(x3, y3);
+        cellB(j0 - 1, x2, y2, y3, wU[2], wV[2], dU[2], dV[2]);
+        emitA(x4, y4);
+        cellB(j0, x3, y3, y4, wU[3], wV[3], dU[3], dV[3]);
+        vq0 = y2;
+        uq0 = x2; uq1 = x3; uq2 = x4; vq1 = y3; vq2 = y4;
+        jb = j0 - 1;
+    }
+    fsU = quick_tl(vq0.b, vq0.r, wU[0], wU[1], wU[2], wU[3], dU[0], dU[1], dU[2], dU[3]);
+    fsV = quick_tl(vq0.b, vq0.r, wV[0], wV[1], wV[2], wV[3], dV[0], dV[1], dV[2], dV[3]);
+    const F fidx = (F)a.inv_dx, fidy = (F)a.inv_dy, fidx2 = (F)a.inv_dx2, fidy2 = (F)a.inv_dy2, firen = (F)a.inv_ren;
+    const double cr = a.cr, cj = a.cj;
+    double nx_ = 0.0, ny_ = 0.0;
+    for (; jb < jend; ++jb) {
+        emitA(uq3, vq3);        // row jb + 3
+        const int jr = jb + 2;  // B's new window cell
+        if (jr < n) cellB(jr, uq2, vq2, vq3, wU[4], wV[4], dU[4], dV[4]);
+        else if (jr == n) ghostB(jr, wU[3], wV[3], dU[3], dV[3], wU[4], wV[4], dU[4], dV[4]);  // mirror of n - 1
+        else ghostB(jr, wU[1], wV[1], dU[1], dV[1], wU[4], wV[4], dU[4], dV[4]);               // mirror of n - 2
+        // k_tangent's row: the flux through y-face jb + 1, then the x-faces
+        const F fnU = quick_tl(vq1.b, vq1.r, wU[1], wU[2], wU[3], wU[4], dU[1], dU[2], dU[3], dU[4]);
+        const F fnV = quick_tl(vq1.b, vq1.r, wV[1], wV[2], wV[3], wV[4], dV[1], dV[2], dV[3], dV[4]);
+        const F Uc = wU[2], Vc = wV[2], dUc = dU[2], dVc = dV[2];
+        const F Um1 = lane_prev(Uc), Um2 = lane_prev(Um1), Up1 = lane_next(Uc);
+        const F Vm1 = lane_prev(Vc), Vm2 = lane_prev(Vm1), Vp1 = lane_next(Vc);
+        const F dUm1 = lane_prev(dUc), dUm2 = lane_prev(dUm1), dUp1 = lane_next(dUc);
+        const F dVm1 = lane_prev(dVc), dVm2 = lane_prev(dVm1), dVp1 = lane_next(dVc);
+        const F fwU = quick_tl(uq0.b, uq0.r, Um2, Um1, Uc, Up1, dUm2, dUm1, dUc, dUp1);
+        const F fwV = quick_tl(uq0.b, uq0.r, Vm2, Vm1, Vc, Vp1, dVm2, dVm1, dVc, dVp1);
+        const F feU = lane_next(fwU), feV = lane_next(fwV);
+        const F CU = (feU - fwU) * fidx + (fnU - fsU) * fidy;
+        const F CV = (feV - fwV) * fidx + (fnV - fsV) * fidy;
+        const F VU = ((dUp1 - F(2) * dUc + dUm1) * fidx2 + (dU[3] - F(2) * dUc + dU[1]) * fidy2) * firen;
+        const F VV = ((dVp1 - F(2) * dVc + dVm1) * fidx2 + (dV[3] - F(2) * dVc + dV[1]) * fidy2) * firen;
+        const F TU = (F(0) - CU) + VU, TV = (F(0) - CV) + VV;
+        const F TUw = lane_prev(TU);
+        if (jb >= j0 && outB) {
+            const size_t k = ic + (size_t)P * jb;
+            if (xface) {
+                const bool wall = i == 0 || i == n;
+                const float Jr = wall ? 0.f : 0.5f * (TUw + TU);
+                const double du = cr * (double)uq0.r + cj * (double)Jr;
+                a.uo[k] = uq0.b64 + du;
+                nx_ += du * du * (wall ? 0.5 : 1.0);
+            }
+            if (xin) {
+                const bool wall = jb == 0;
+                const float Jr = wall ? 0.f : 0.5f * (tVprev + TV);
+                const double dv = cr * (double)vq0.r + cj * (double)Jr;
+                a.vo[k] = vq0.b64 + dv;
+                ny_ += dv * dv * (wall ? 0.5 : 1.0);
+            }
+        }
+        fsU = fnU; fsV = fnV; tVprev = TV;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { wU[q] = wU[q + 1]; wV[q] = wV[q + 1]; dU[q] = dU[q + 1]; dV[q] = dV[q + 1]; }
+        uq0 = uq1; uq1 = uq2; uq2 = uq3;
+        vq0 = vq1; vq1 = vq2; vq2 = vq3;
+    }
+    if (jend == n && outB && xin) {  // north wall row: R' r = 0 there
+        const double dv = cr * (double)vq0.r;
+        a.vo[ic + (size_t)P * n] = vq0.b64 + dv;
+        ny_ += dv * dv * 0.5;
+    }
+    for (int o = 32; o > 0; o >>= 1) { nx_ += __shfl_down(nx_, o, 64); ny_ += __shfl_down(ny_, o, 64); }
+    if (lane == 0) { a.partials[2 * wid] = nx_; a.partials[2 * wid + 1] = ny_; }
+}
+
 static StageArgs base_args(const Ctx &c, double rk, int mode, int rows)
 {
     const Geometry &g = c.g;
@@ -1154,6 +1407,29 @@ hipError_t launch_stage_pair(const Ctx &c, const StageLaunch &s, int *nblocks)
     if (nblocks) *nblocks = a.nbands * a.nchunks;  // one norm partial per wave
     if (s.pair == 2) hipLaunchKernelGGL(k_stage_pair<true>, grid, dim3(256), 0, c.stream, a);
     else hipLaunchKernelGGL(k_stage_pair<false>, grid, dim3(256), 0, c.stream, a);
+    return hipGetLastError();
+}
+
+bool mixed_pair_fits(const Ctx &c)
+{
+    const int nb = (c.g.n + 1 + WB2 - 1) / WB2, nc = (c.g.n + c.mpair_rows - 1) / c.mpair_rows;
+    return c.g.n >= 8 && c.mpair_rows >= 4 && (int64_t)nb * nc <= kMaxPartials;
+}
+
+hipError_t launch_mixed_pair(const Ctx &c, const MixedPairLaunch &s, int *nblocks)
+{
+    if (!mixed_pair_fits(c)) return hipErrorInvalidValue;
+    if (s.uo == s.u64 || s.vo == s.v64) return hipErrorInvalidValue;
+    StageArgs a = base_args(c, 0.0, 3, c.mpair_rows);
+    a.us = s.u64; a.vs = s.v64; a.cu = s.cu; a.cv = s.cv;
+    a.uo = s.uo; a.vo = s.vo;
+    a.cr = s.cr; a.cj = s.cj;
+    if (s.unguarded) a.ctl = nullptr;
+    a.nbands = (c.g.n + 1 + WB2 - 1) / WB2;
+    a.nchunks = (c.g.n + a.rows - 1) / a.rows;
+    const dim3 grid = stage_grid(a);
+    if (nblocks) *nblocks = a.nbands * a.nchunks;  // one norm partial per wave
+    hipLaunchKernelGGL(k_mixed_pair, grid, dim3(256), 0, c.stream, a);
     return hipGetLastError();
 }
 

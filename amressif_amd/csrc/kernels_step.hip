@@ -323,6 +323,7 @@ hipError_t launch_center_b(const Ctx &c, double *b)
 // constant of the next step, c = (1.5/dt) u^{n+1} + (0.5/dt)(u^{n+1} - u^n)
 // with u^n from nx.up (as k_bdf2 forms it), and a copy of u^{n+1} to nx.w,
 // which becomes that step's u^n; nx.up itself still holds u^n afterwards.
+// nx.w null: no copy (the caller keeps u^n and u^{n+1} in separate arrays).
 struct NextBdf {
     const double *up_u, *up_v;
     double *cu, *cv, *wu, *wv;
@@ -381,7 +382,7 @@ __global__ __launch_bounds__(256) void k_project(const double *us, const double 
             if (cc) {
                 const double um = up[f];
                 cc[f] = (1.5 / dt) * un + (0.5 / dt) * (un - um);
-                wc[f] = un;
+                if (wc) wc[f] = un;
             }
         };
         put(uo, nx.up_u, nx.cu, nx.wu, k, uw);
@@ -474,16 +475,16 @@ __global__ __launch_bounds__(256) void k_project_rows(const double *us, const do
     auto flush = [&]() {
         if (sx) {
             uo[sk] = su;
-            if (fold) { nx.cu[sk] = scu; nx.wu[sk] = su; }
+            if (fold) { nx.cu[sk] = scu; if (nx.wu) nx.wu[sk] = su; }
         }
         if (sy) {
             vo[sk] = sv;
             p_out[sk] = sp;
-            if (fold) { nx.cv[sk] = scv; nx.wv[sk] = sv; }
+            if (fold) { nx.cv[sk] = scv; if (nx.wv) nx.wv[sk] = sv; }
         }
         if (syn) {
             vo[sk + P] = svn;
-            if (fold) { nx.cv[sk + P] = scvn; nx.wv[sk + P] = svn; }
+            if (fold) { nx.cv[sk + P] = scvn; if (nx.wv) nx.wv[sk + P] = svn; }
         }
     };
     Raw cur, nxt;
@@ -540,13 +541,16 @@ __global__ __launch_bounds__(256) void k_project_rows(const double *us, const do
 
 hipError_t launch_project_ex(const Ctx &c, const double *us, const double *vs, double *uo, double *vo, double *p,
                              const double *phi, int use_mean, int *nblocks, const float *au, const float *av,
-                             int next_bdf, double *p_out, const double *phi_alt, int gated)
+                             int next_bdf, double *p_out, const double *phi_alt, int gated, double *const *un,
+                             int no_copy)
 {
     if (!p_out) p_out = p;
     const ProjGate gate{phi_alt, gated};
     const int gradp = next_bdf && p_out != p && c.project_rows > 0;
-    const NextBdf nx =
-        next_bdf ? NextBdf{c.Uprev[0], c.Uprev[1], c.C[0], c.C[1], c.S2[0], c.S2[1], gradp} : NextBdf{};
+    double *const *up = un ? un : c.Uprev;
+    const NextBdf nx = next_bdf ? NextBdf{up[0], up[1], c.C[0], c.C[1], no_copy ? nullptr : c.S2[0],
+                                          no_copy ? nullptr : c.S2[1], gradp}
+                                : NextBdf{};
     const int n = c.g.n;
     if (c.project_rows > 0) {  // row-march kernel
         const int rows = c.project_rows;

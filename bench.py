@@ -17,8 +17,9 @@ Both precisions are run from the same initial state:
 and the mixed result is compared with the FP64 one per field after the same
 number of steps.  Beside them, two more legs are timed: the discrete adjoint
 of a 512^2, Re = 1000 rollout (backward sweep with checkpoint recomputation
-over the forward time) and the two-level composite step (512^2 coarse level +
-a ratio-2 patch over the top quarter).  One JSON line is printed.
+over the forward time), the two-level composite step (512^2 coarse level +
+a ratio-2 patch over the top quarter) and the 3D step of the general path
+(256^3 cavity, FP64).  One JSON line is printed.
 
 roofline: after the timed steps of each precision, the same run advances K
 more steps with the library's in-step kernel timing on (HIP events, created
@@ -64,6 +65,7 @@ STEP_KERNELS = {
     "poisson_start": ("amr::k_poisson_start (Poisson RHS, warm start, first residual)", None),
     "pair": ("amr::k_stage_pair<false> (FP64 RK stages 1-2 in one pass)", "pair_fp64"),
     "pair_last": ("amr::k_stage_pair<true> (FP64 RK stages 3-4 in one pass + norm partials)", None),
+    "mpair": ("amr::k_mixed_pair (mixed residual + FP32 tangent-linear update in one pass)", None),
 }
 
 
@@ -82,6 +84,8 @@ def parse():
     ap.add_argument("--adjoint-every", type=int, default=32, help="checkpoint interval of the adjoint leg")
     ap.add_argument("--amr-n", type=int, default=512, help="two-level leg: coarse grid (0: skip the leg)")
     ap.add_argument("--amr-steps", type=int, default=500)
+    ap.add_argument("--gen-n", type=int, default=256, help="3D leg: cells per side of the 3D cavity (0: skip)")
+    ap.add_argument("--gen-steps", type=int, default=5)
     ap.add_argument("--kernel-probe", choices=["fp64", "mixed"], default=None,
                     help="for rocprofv3 --pmc: one step, then a few timed launches of each kernel of that "
                          "precision and a 64 MiB copy (calibration); prints their HIP-event times")
@@ -213,6 +217,37 @@ def amr_leg(api, W, n0, steps, torch):
             "flags": int(np.bitwise_or.reduce([x["flags"] for x in st]))}
 
 
+def general_leg(api, n, steps, torch):
+    """The general path on a 3D lid-driven cavity: n^3 cells, walls 135 with the
+    lid (165, u = 1) on the y-hi side, Re = 1000, FP64, from rest after two
+    warm-up steps; plus the three per-stage kernels' rates."""
+    import numpy as np
+    inp = api.default_inputs(n_cell=n, max_grid_size=(n, n, n), IterNum=100, cfl=0.5, ren=1000.0, vis=1e-3,
+                             phy_bc_lo=(135, 135, 135), phy_bc_hi=(135, 165, 135), inflow_waveform=(1.0, 0.0, 0.0),
+                             momentum_tolerance=1e-10, spacedim=3)
+    ctx = api.Context(inp)
+    ctx.init_state()
+    ctx.advance(2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = ctx.advance(steps)
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    names = {7: "amr::gen::k_gen_cells (cell velocities with ghosts)",
+             8: "amr::gen::k_gen_flux (QUICK + viscous fluxTotal)",
+             9: "amr::gen::k_gen_update (stage update of every face)"}
+    kern = []
+    for which, name in names.items():
+        ms, nbytes = ctx.time_kernel(which, reps=10)
+        kern.append({"kernel": name, "avg_launch_ms": ms, "algorithmic_bytes_per_launch": nbytes,
+                     "achieved": nbytes / (ms * 1e-3) / 1e9, "unit": "GB/s", "frac": nbytes / (ms * 1e-3) / HBM_PEAK})
+    return {"workload": f"3D lid-driven cavity {n}^3, Re=1000, from rest, FP64, tol 1e-10, {steps} steps",
+            "mcell_steps_per_s": n ** 3 * steps / sec / 1e6, "ms_per_step": 1e3 * sec / steps,
+            "mean_pseudo_iters": float(np.mean([s["pseudo_iters"] for s in st])),
+            "mean_vcycles": float(np.mean([s["vcycles"] for s in st])),
+            "flags": int(np.bitwise_or.reduce([s["flags"] for s in st])), "stage_kernels": kern}
+
+
 def kernel_probe(api, W, n, ren, which_prec):
     """Isolated launches for the counter runs (rocprofv3 --pmc): one step,
     then a few timed launches of each kernel and a 64 MiB copy."""
@@ -299,6 +334,10 @@ def main():
     if args.amr_n > 0 and not args.profile_run:
         amr = amr_leg(api, W, args.amr_n, args.amr_steps, torch)
 
+    gen = None
+    if args.gen_n > 0 and not args.profile_run:
+        gen = general_leg(api, args.gen_n, args.gen_steps, torch)
+
     cpu = None
     if args.cpu_steps > 0 and not args.profile_run:
         from oracle.refcpu import Oracle
@@ -342,6 +381,7 @@ def main():
         "adjoint_forward_ratio": adj["ratio"] if adj else None,
         "adjoint": adj,
         "two_level": amr,
+        "general_3d": gen,
         "roofline": roofline,
         "kernel_rooflines": extra,
         "cpu_baseline": cpu,

@@ -395,6 +395,7 @@ int amressif_time_kernel(amressif_ctx *ctx, int which, int reps, int64_t copy_by
  *   which = 7  k_poisson_start
  *   which = 8  FP64 RK stages 1-2 in one pass
  *   which = 9  FP64 RK stages 3-4 in one pass (+ norm partials)
+ *   which = 10 mixed iteration in one pass (residual + tangent-linear update + norm partials)
  * amressif_kernel_time returns the average time per counted launch (0 if none),
  * the number of counted launches and their average algorithmic HBM bytes (the
  * bytes each launch must read and write, counted once per cell). */

@@ -318,6 +318,46 @@ def test_two_stage_pass_matches_four_stage_launches(n, rows, init, bc, lid):
         assert rel(p2 - p2.mean(), p4 - p4.mean()) < 1e-12, name
 
 
+@pytest.mark.parametrize("n,rows,init,bc,lid", [
+    (8, 4, "seeded", 0, False), (64, 4, "seeded", 0, False), (64, 5, "seeded", 1, False),
+    (100, 16, "seeded", 2, True), (130, 7, "seeded", 0, False), (256, 16, "seeded", 3, False),
+    (256, 64, "zero", 0, True), (300, 12, "seeded", 0, False)])
+def test_one_pass_mixed_iteration_matches_two_launches(n, rows, init, bc, lid):
+    """The mixed iteration as one k_mixed_pair launch (the residual kept in
+    registers, u^{k+1} chained through two arrays, the projection copying
+    nothing) against k_stage64 mode 3 + k_tangent: the same pseudo-iteration
+    and V-cycle counts and the same fields to rounding, after two advance calls
+    (the state at a call boundary, velContPrev included), for row chunks that do
+    and do not divide n, every wall set, a lid profile, and batches that
+    overshoot convergence (the skipped launches copy u^k)."""
+    api = _api()
+    prof = W.lid_profile(n) if lid else None
+
+    def run():
+        inp = api.cavity_inputs(n, 400.0, precision=1, phy_bc_lo=(BCS[bc][0][0], BCS[bc][0][1], 135),
+                                phy_bc_hi=(BCS[bc][1][0], BCS[bc][1][1], 135), inflow_waveform=(0.8, -0.1, 0.0))
+        r = api.Run.from_numpy(inp, W.initial_state(n, init))
+        if prof is not None:
+            r.ctx.set_lid_profile(cuda(prof))
+        st = r.advance(5)
+        f1 = r.fetch()
+        st += r.advance(3)
+        return f1, r.fetch(), st
+
+    a1, a2, sa = _with_env({"AMRESSIF_MIXED_PAIR": "0"}, run)
+    b1, b2, sb = _with_env({"AMRESSIF_MIXED_PAIR": "1", "AMRESSIF_MPAIR_ROWS": str(rows)}, run)
+    print(f"n={n} rows={rows}: pseudo-iterations {[s['pseudo_iters'] for s in sb]}, V-cycles "
+          f"{[s['vcycles'] for s in sb]}, u {rel(b2['u'], a2['u']):.1e} v {rel(b2['v'], a2['v']):.1e}")
+    assert [s["pseudo_iters"] for s in sb] == [s["pseudo_iters"] for s in sa]
+    assert [s["vcycles"] for s in sb] == [s["vcycles"] for s in sa]
+    assert all(s["flags"] == 0 for s in sb)
+    for fa, fb in ((a1, b1), (a2, b2)):
+        for k in ("u", "v", "uprev", "vprev"):
+            assert rel(fb[k], fa[k]) < 1e-12, k
+        pa, pb = fa["ctx"][0, 1:-1, 1:-1], fb["ctx"][0, 1:-1, 1:-1]
+        assert rel(pb - pb.mean(), pa - pa.mean()) < 1e-12
+
+
 @pytest.mark.parametrize("n", [130, 98])
 def test_odd_bottom_poisson_solve_matches_the_exact_solve(n):
     """n = 130 coarsens to an odd 65 x 65 bottom (98 to 49 x 49), above the
@@ -465,13 +505,15 @@ def test_in_step_kernel_timing_counts_and_leaves_results_unchanged(prec, batch):
         assert t["pair"][1] == pi and t["pair_last"][1] == pi
         assert t["stage"][1] == 0 and t["stage_last"][1] == 0
         assert t["resid"][1] == 0 and t["tangent"][1] == 0
-    else:
-        assert t["resid"][1] == pi and t["tangent"][1] == pi
+    else:  # the mixed iteration in one pass
+        assert t["mpair"][1] == pi
+        assert t["resid"][1] == 0 and t["tangent"][1] == 0
         assert t["stage"][1] == 0
     assert t["mg_down"][1] == vc and t["mg_up"][1] == vc
     assert t["project"][1] == steps and t["poisson_start"][1] == steps
     cells = float(n * n)
-    per_cell = {"stage": 64, "stage_last": 64, "resid": 48, "tangent": 48, "pair": 64, "pair_last": 64}
+    per_cell = {"stage": 64, "stage_last": 64, "resid": 48, "tangent": 48, "pair": 64, "pair_last": 64,
+                "mpair": 48}
     for name, (ms, cnt, nbytes) in t.items():
         assert (ms > 0) == (cnt > 0) and (nbytes > 0) == (cnt > 0), name
         assert ms < 50.0, name  # a 256^2 launch takes microseconds

@@ -35,6 +35,7 @@ struct amressif_amr {
     int host_gmres = 0;  // AMRESSIF_AMR_HOST_GMRES=1: the Arnoldi process on the host (for A/B runs)
     // fine level (faces pitch P1, cells pitch P1, ghosted cells pitch Pg1)
     double *U1[2]{}, *Up1[2]{}, *C1[2]{}, *F1[2]{}, *F2[2]{}, *p1 = nullptr, *XU = nullptr, *XV = nullptr;
+    double *DXU = nullptr, *DXV = nullptr;  // cells of the residual (the mixed step's refinement form)
     amr::Ctl *ctlF = nullptr, *ctlF_host = nullptr;
     double *part = nullptr, *red = nullptr, *red_host = nullptr, *coef = nullptr;
     // composite vectors [coarse segment | fine segment]
@@ -88,6 +89,49 @@ int reduce(amressif_amr *A, int ncol, int is_max, double *out)
 }
 
 // ------------------------------------------------ fine momentum predictor
+// The mixed step's form on the patch (as the coarse level's mixed iteration,
+// in FP64): the residual of the implicit BDF2 system and the correction by the
+// two-term Neumann series of its inverse Jacobian, u += r/c + R'(u) r / c^2
+// (c = 1.5/dt), to the same stopping test.  Iteration k reads u^k from U1 (k
+// even) or F1 and writes the other; F2 holds r.  Launches past convergence
+// copy u^k, so after an odd count the converged state is in F1 and the two
+// pointers are exchanged.
+int fine_predictor_refine(amressif_amr *A, int *iters, int *flags)
+{
+    const amr::AmrGeom &a = A->a;
+    Ctx &cc = A->coarse->c;
+    hipStream_t s = A->stream;
+    const double cr = 1.0 / a.c15, cj = 1.0 / (a.c15 * a.c15);
+    double *bu[2] = {A->U1[0], A->F1[0]}, *bv[2] = {A->U1[1], A->F1[1]};
+    int launched = 0, target = std::max(1, A->last_P), nb = 0;
+    const int cap = cc.in.IterNum + 1;
+    for (;;) {
+        while (launched < target && launched < cap) {
+            const int q = launched & 1;
+            ACHK(amr::amr_fine_cells(a, s, bu[q], bv[q], cc.U[0], cc.U[1], A->XU, A->XV));
+            ACHK(amr::amr_fine_resid(a, s, bu[q], bv[q], A->C1[0], A->C1[1], A->p1, A->XU, A->XV, A->F2[0], A->F2[1],
+                                     A->ctlF));
+            ACHK(amr::amr_fine_dcells(a, s, A->F2[0], A->F2[1], A->DXU, A->DXV, A->ctlF));
+            ACHK(amr::amr_fine_tangent(a, s, bu[q], bv[q], A->XU, A->XV, A->F2[0], A->F2[1], A->DXU, A->DXV,
+                                       bu[q ^ 1], bv[q ^ 1], cr, cj, A->part, A->ctlF, &nb));
+            ACHK(amr::amr_fine_fin(a, s, A->part, nb, A->ctlF));
+            ++launched;
+        }
+        ACHK(hipMemcpyAsync(A->ctlF_host, A->ctlF, sizeof(amr::Ctl), hipMemcpyDeviceToHost, s));
+        ACHK(hipStreamSynchronize(s));
+        if (A->ctlF_host->mom_done || launched >= cap) break;
+        target = launched + 2;
+    }
+    if (launched & 1) {
+        std::swap(A->U1[0], A->F1[0]);
+        std::swap(A->U1[1], A->F1[1]);
+    }
+    A->last_P = std::max(1, A->ctlF_host->mom_count);
+    *iters = A->ctlF_host->mom_count;
+    *flags = A->ctlF_host->flags;
+    return 0;
+}
+
 int fine_predictor(amressif_amr *A, int *iters, int *flags)
 {
     const amr::AmrGeom &a = A->a;
@@ -96,6 +140,7 @@ int fine_predictor(amressif_amr *A, int *iters, int *flags)
     ACHK(amr::amr_fine_ctl_reset(s, A->ctlF, cc.in.momentum_tolerance, cc.in.IterNum));
     for (int d = 0; d < 2; ++d) ACHK(amr::amr_fine_bdf2(a, s, A->U1[d], A->Up1[d], A->C1[d], d, A->dt));
     ACHK(amr::amr_fine_interface(a, s, A->U1[1], cc.U[1]));  // fixed interface faces from the coarse u*
+    if (A->mixed) return fine_predictor_refine(A, iters, flags);
     const double *in_u[4] = {A->U1[0], A->F1[0], A->F2[0], A->F1[0]};
     const double *in_v[4] = {A->U1[1], A->F1[1], A->F2[1], A->F1[1]};
     double *out_u[4] = {A->F1[0], A->F2[0], A->F1[0], A->U1[0]};
@@ -337,6 +382,10 @@ int amressif_amr_create(const amressif_inputs *in, int patch_jlo, void *hip_stre
     AALLOC(&A->p1, fsz);
     AALLOC(&A->XU, gsz);
     AALLOC(&A->XV, gsz);
+    if (A->mixed) {
+        AALLOC(&A->DXU, gsz);
+        AALLOC(&A->DXV, gsz);
+    }
     AALLOC(&A->ctlF, 1);
     ACHK(hipHostMalloc((void **)&A->ctlF_host, sizeof(amr::Ctl), hipHostMallocDefault));
     const int nblk = amr::amr_flat_blocks(a);

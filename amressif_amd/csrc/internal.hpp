@@ -360,6 +360,18 @@ hipError_t amr_fine_stage(const AmrGeom &a, hipStream_t s, const double *us, con
                           const double *XV, double *uo, double *vo, double rk, int mode, double *partials,
                           const Ctl *ctl, int *nblocks);
 hipError_t amr_fine_fin(const AmrGeom &a, hipStream_t s, const double *partials, int nb, Ctl *ctl);
+// the patch's momentum iteration in the refinement form (mixed step): the
+// residual r, the cells of r, then u^{k+1} = u^k + cr r + cj R'(u^k) r into
+// (uo, vo) with the norm partials (a copy of u^k after convergence)
+hipError_t amr_fine_resid(const AmrGeom &a, hipStream_t s, const double *u, const double *v, const double *cu,
+                          const double *cv, const double *p, const double *XU, const double *XV, double *ru,
+                          double *rv, const Ctl *ctl);
+hipError_t amr_fine_dcells(const AmrGeom &a, hipStream_t s, const double *ru, const double *rv, double *DU, double *DV,
+                           const Ctl *ctl);
+hipError_t amr_fine_tangent(const AmrGeom &a, hipStream_t s, const double *u, const double *v, const double *XU,
+                            const double *XV, const double *ru, const double *rv, const double *DU, const double *DV,
+                            double *uo, double *vo, double cr, double cj, double *partials, const Ctl *ctl,
+                            int *nblocks);
 hipError_t amr_fine_ctl_reset(hipStream_t s, Ctl *ctl, double tol, int iter_num);
 hipError_t amr_fine_bdf2(const AmrGeom &a, hipStream_t s, double *u, double *up, double *c, int dir, double dt);
 hipError_t amr_apply(const AmrGeom &a, hipStream_t s, const double *x0, const double *x1, const double *b0,

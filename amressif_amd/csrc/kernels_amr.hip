@@ -217,6 +217,155 @@ __global__ __launch_bounds__(256) void k_fine_bdf2(double *u, double *up, double
     up[k] = un;
 }
 
+// ------------------------------- fine momentum predictor, refinement form
+// The mixed step's momentum iteration on the patch (FP64): the residual
+// r = R(u^k) - (1.5/dt) u^k + c of the implicit BDF2 system, then
+// u^{k+1} = u^k + cr r + cj R'(u^k) r with R' the tangent-linear flux
+// operator (QUICK with the upwind choice frozen at u^k, viscous flux of r).
+// Wall and interface faces keep their value (r = 0 there), and the C/F south
+// ghosts depend on the coarse predictor only, so their increment is 0.
+
+// r on every face (0 on the wall and interface faces)
+__global__ __launch_bounds__(256) void k_fine_resid(AmrGeom a, const double *u, const double *v, const double *cu,
+                                                    const double *cv, const double *p, const double *XU,
+                                                    const double *XV, double *ru, double *rv, const Ctl *ctl)
+{
+    if (ctl->mom_done) return;
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63), j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int nx = a.nx, ny = a.ny, P = a.P1;
+    if (i > nx || j > ny) return;
+    const double ih = 1.0 / a.h1;
+    const size_t k = i + (size_t)P * j;
+    if (j < ny) {
+        double r = 0.0;
+        if (i > 0 && i < nx) {
+            const double R = -(p[k] - p[k - 1]) * ih + 0.5 * (fine_T(a, XU, u, v, i - 1, j) + fine_T(a, XU, u, v, i, j));
+            r = R - a.c15 * u[k] + cu[k];
+        }
+        ru[k] = r;
+    }
+    if (i < nx) {
+        double r = 0.0;
+        if (j > 0 && j < ny) {
+            const double R = -(p[k] - p[k - P]) * ih + 0.5 * (fine_T(a, XV, u, v, i, j - 1) + fine_T(a, XV, u, v, i, j));
+            r = R - a.c15 * v[k] + cv[k];
+        }
+        rv[k] = r;
+    }
+}
+
+// cell values of r with two ghost layers: the wall rules without their
+// constants, 0 below the interface
+__global__ __launch_bounds__(256) void k_fine_dcells(AmrGeom a, const double *ru, const double *rv, double *DU,
+                                                     double *DV, const Ctl *ctl)
+{
+    if (ctl->mom_done) return;
+    const int gi = blockIdx.x * 64 + (threadIdx.x & 63), gj = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int nx = a.nx, ny = a.ny;
+    if (gi >= nx + 4 || gj >= ny + 4) return;
+    const int i = gi - 2, j = gj - 2;
+    const bool xin = i >= 0 && i < nx, yin = j >= 0 && j < ny;
+    double U = 0.0, V = 0.0;
+    if (xin && yin) {
+        U = cell_u(ru, a.P1, i, j);
+        V = cell_v(rv, a.P1, i, j);
+    } else if (yin) {  // west / east wall ghosts
+        const int mi = i < 0 ? -i - 1 : 2 * nx - 1 - i;
+        ghost_rule<double, true>(i < 0 ? 0 : 1, i < 0 ? a.bc_w : a.bc_e, cell_u(ru, a.P1, mi, j),
+                                 cell_v(rv, a.P1, mi, j), 0.0, 0.0, U, V);
+    } else if (xin && j >= ny) {  // north wall ghosts
+        const int mj = 2 * ny - 1 - j;
+        ghost_rule<double, true>(3, a.bc_n, cell_u(ru, a.P1, i, mj), cell_v(rv, a.P1, i, mj), 0.0, a.inflow1, U, V);
+    }
+    const size_t o = gi + (size_t)a.Pg1 * gj;
+    DU[o] = U;
+    DV[o] = V;
+}
+
+// derivative of QUICK(vel, X) in direction (dvel, dX), the upwind choice frozen at vel
+__device__ __forceinline__ double quick_tl64(double vel, double dvel, double a2, double a1, double b0, double b1,
+                                             double da2, double da1, double db0, double db1)
+{
+    const bool pos = vel >= 0.0;
+    // phi = 3/8 downwind + 6/8 upwind - 1/8 far-upwind
+    const double xd = pos ? b0 : a1, xu = pos ? a1 : b0, xf = pos ? a2 : b1;
+    const double dd = pos ? db0 : da1, du = pos ? da1 : db0, df = pos ? da2 : db1;
+    const double phid = 0.375 * dd + 0.75 * du - 0.125 * df;
+    const double phix = 0.375 * xd + 0.75 * xu - 0.125 * xf;
+    return vel * phid + dvel * phix;
+}
+
+// R'(u) r of one component at interior cell (i, j): T' = -div(QUICK') + viscous flux of r
+__device__ double fine_TL(const AmrGeom &a, const double *X, const double *D, const double *u, const double *v,
+                          const double *ru, const double *rv, int i, int j)
+{
+    const int Pg = a.Pg1, P = a.P1;
+    auto fx = [&](int f) {  // x-face (f, j)
+        const size_t k = f + (size_t)P * j;
+        return quick_tl64(u[k], ru[k], xg(X, Pg, f - 2, j), xg(X, Pg, f - 1, j), xg(X, Pg, f, j), xg(X, Pg, f + 1, j),
+                          xg(D, Pg, f - 2, j), xg(D, Pg, f - 1, j), xg(D, Pg, f, j), xg(D, Pg, f + 1, j));
+    };
+    auto fy = [&](int f) {  // y-face (i, f)
+        const size_t k = i + (size_t)P * f;
+        return quick_tl64(v[k], rv[k], xg(X, Pg, i, f - 2), xg(X, Pg, i, f - 1), xg(X, Pg, i, f), xg(X, Pg, i, f + 1),
+                          xg(D, Pg, i, f - 2), xg(D, Pg, i, f - 1), xg(D, Pg, i, f), xg(D, Pg, i, f + 1));
+    };
+    const double ih = 1.0 / a.h1, ih2 = ih * ih;
+    const double C = (fx(i + 1) - fx(i)) * ih + (fy(j + 1) - fy(j)) * ih;
+    const double c = xg(D, Pg, i, j);
+    const double Vs = ((xg(D, Pg, i + 1, j) - 2.0 * c + xg(D, Pg, i - 1, j)) * ih2 +
+                       (xg(D, Pg, i, j + 1) - 2.0 * c + xg(D, Pg, i, j - 1)) * ih2) *
+                      a.iren;
+    return (0.0 - C) + Vs;
+}
+
+// u^{k+1} = u^k + cr r + cj R'(u^k) r into (uo, vo) on every face, norm
+// partials of the change; after convergence a copy of u^k (the caller's
+// buffer exchange then keeps the converged state)
+__global__ __launch_bounds__(256) void k_fine_tangent(AmrGeom a, const double *u, const double *v, const double *XU,
+                                                      const double *XV, const double *ru, const double *rv,
+                                                      const double *DU, const double *DV, double *uo, double *vo,
+                                                      double cr, double cj, double *partials, const Ctl *ctl)
+{
+    __shared__ double sh[16];
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63), j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int nx = a.nx, ny = a.ny, P = a.P1;
+    const size_t k = i + (size_t)P * j;
+    if (ctl->mom_done) {  // the same for every thread of the launch
+        if (i <= nx && j < ny) uo[k] = u[k];
+        if (i < nx && j <= ny) vo[k] = v[k];
+        return;
+    }
+    double sx = 0.0, sy = 0.0;
+    if (i <= nx && j <= ny) {
+        if (j < ny) {
+            double du = 0.0;
+            if (i > 0 && i < nx) {
+                const double J = 0.5 * (fine_TL(a, XU, DU, u, v, ru, rv, i - 1, j) + fine_TL(a, XU, DU, u, v, ru, rv, i, j));
+                du = cr * ru[k] + cj * J;
+            }
+            uo[k] = u[k] + du;
+            sx = fabs(du * du * ((i == 0 || i == nx) ? 0.5 : 1.0));
+        }
+        if (i < nx) {
+            double dv = 0.0;
+            if (j > 0 && j < ny) {
+                const double J = 0.5 * (fine_TL(a, XV, DV, u, v, ru, rv, i, j - 1) + fine_TL(a, XV, DV, u, v, ru, rv, i, j));
+                dv = cr * rv[k] + cj * J;
+            }
+            vo[k] = v[k] + dv;
+            sy = fabs(dv * dv * ((j == 0 || j == ny) ? 0.5 : 1.0));
+        }
+    }
+    sx = block_reduce<false>(sx, sh);
+    sy = block_reduce<false>(sy, sh + 8);
+    if (threadIdx.x == 0) {
+        const size_t b = blockIdx.x + (size_t)gridDim.x * blockIdx.y;
+        partials[2 * b] = sx;
+        partials[2 * b + 1] = sy;
+    }
+}
+
 // -------------------------------------------------- composite operator
 // x0: coarse cells (pitch P0, active rows < jc0), x1: fine cells (pitch P1)
 __device__ __forceinline__ double cf_ghost(const AmrGeom &a, const double *x0, const double *x1, int i)
@@ -547,6 +696,30 @@ hipError_t amr_fine_stage(const AmrGeom &a, hipStream_t s, const double *us, con
     *nblocks = (int)(g.x * g.y);
     hipLaunchKernelGGL(k_fine_stage, g, dim3(256), 0, s, a, us, vs, ub, vb, cu, cv, p, XU, XV, uo, vo, rk, mode,
                        partials, ctl);
+    return hipGetLastError();
+}
+hipError_t amr_fine_resid(const AmrGeom &a, hipStream_t s, const double *u, const double *v, const double *cu,
+                          const double *cv, const double *p, const double *XU, const double *XV, double *ru,
+                          double *rv, const Ctl *ctl)
+{
+    hipLaunchKernelGGL(k_fine_resid, g2(a.nx + 1, a.ny + 1), dim3(256), 0, s, a, u, v, cu, cv, p, XU, XV, ru, rv, ctl);
+    return hipGetLastError();
+}
+hipError_t amr_fine_dcells(const AmrGeom &a, hipStream_t s, const double *ru, const double *rv, double *DU, double *DV,
+                           const Ctl *ctl)
+{
+    hipLaunchKernelGGL(k_fine_dcells, g2(a.nx + 4, a.ny + 4), dim3(256), 0, s, a, ru, rv, DU, DV, ctl);
+    return hipGetLastError();
+}
+hipError_t amr_fine_tangent(const AmrGeom &a, hipStream_t s, const double *u, const double *v, const double *XU,
+                            const double *XV, const double *ru, const double *rv, const double *DU, const double *DV,
+                            double *uo, double *vo, double cr, double cj, double *partials, const Ctl *ctl,
+                            int *nblocks)
+{
+    const dim3 g = g2(a.nx + 1, a.ny + 1);
+    *nblocks = (int)(g.x * g.y);
+    hipLaunchKernelGGL(k_fine_tangent, g, dim3(256), 0, s, a, u, v, XU, XV, ru, rv, DU, DV, uo, vo, cr, cj, partials,
+                       ctl);
     return hipGetLastError();
 }
 hipError_t amr_fine_fin(const AmrGeom &a, hipStream_t s, const double *partials, int nb, Ctl *ctl)

@@ -320,7 +320,7 @@ int mg_run(amressif_ctx *ctx, const WarmStart &w, bool center, const double *us,
     Ctx &c = ctx->c;
     const int mixed = c.in.precision == AMRESSIF_MIXED;
     int nb = 0;
-    HIPCHK(amr::launch_mg_start(c, require_mom ? 1 : 0, us, vs));
+    if (!(require_mom && c.mp_spec)) HIPCHK(amr::launch_mg_start(c, require_mom ? 1 : 0, us, vs));
     {
         // per cell: u*, v* in (or b), the warm-start iterates in, b (when formed or
         // centred), phi0 and the residual (V-cycle precision) out
@@ -445,7 +445,9 @@ int amr_momentum_begin(amressif_ctx *ctx, bool bdf_ready, int *launched, bool re
     const int target = std::min(std::max(1, c.last_P), c.in.IterNum + 1);
     for (*launched = 0; *launched < target; ++*launched) {
         if (int rc = launch_pseudo_iteration(ctx, *launched, &nb)) return rc;
-        HIPCHK(amr::launch_fin_pseudo(c, nb));
+        // with the one-pass mixed iteration, the batch's last finalizer also
+        // starts the Poisson phase enqueued behind it (mg_run then skips k_mg_start)
+        HIPCHK(amr::launch_fin_pseudo(c, nb, c.mp_spec && *launched == target - 1 ? 1 : 0));
     }
     return 0;
 }
@@ -1356,7 +1358,7 @@ int advance_impl(amressif_ctx *ctx, int nsteps, amressif_step_stats *stats, amre
     // u^{n+1}: it writes u^{n+1} to the free one of S1, S2 and the buffers rotate
     struct MpScope {
         Ctx &c;
-        ~MpScope() { c.mp_on = 0; }
+        ~MpScope() { c.mp_on = 0; c.mp_spec = 0; }
     } mp_scope{c};
     c.mp_on = speculate && mixed && c.in.mixed_momentum == 1 && c.in.PSEUDO_TIMESTEPPING == 1 && c.mixed_pair &&
               amr::mixed_pair_fits(c);
@@ -1367,6 +1369,7 @@ int advance_impl(amressif_ctx *ctx, int nsteps, amressif_step_stats *stats, amre
         bool projected = false;  // the projection ran as the Poisson phase's gated tail
         if (speculate) {
             int launched = 0, rc = 0;
+            c.mp_spec = c.mp_on;
             if ((rc = amr_momentum_begin(ctx, bdf_ready, &launched, s > 0))) return rc;
             bdf_ready = false;
             const WarmStart w = mg_prepare(ctx, warm, &rc);
@@ -1426,7 +1429,9 @@ int advance_impl(amressif_ctx *ctx, int nsteps, amressif_step_stats *stats, amre
                 amr::kprof_close(c, amr::KL_PROJ, 0);
             };
             bool skipped = false;
-            if ((rc = mg_run(ctx, w, false, ustar()[0], ustar()[1], true, &skipped, &tail, &projected))) return rc;
+            rc = mg_run(ctx, w, false, ustar()[0], ustar()[1], true, &skipped, &tail, &projected);
+            c.mp_spec = 0;
+            if (rc) return rc;
             if (skipped) {
                 if ((rc = amr_momentum_finish(ctx, launched))) return rc;
                 if ((rc = mg_run(ctx, w, false, ustar()[0], ustar()[1], false, nullptr, &tail, &projected)))
@@ -1774,13 +1779,23 @@ int amressif_time_kernel(amressif_ctx *ctx, int which, int reps, int64_t copy_by
         HIPCHK(hipMalloc(&src, (size_t)copy_bytes));
         HIPCHK(hipMalloc(&dst, (size_t)copy_bytes));
         HIPCHK(hipMemsetAsync(src, 0, (size_t)copy_bytes, c.stream));
-    } else if (which < 0 || which > 6) {
-        return fail(ctx, AMRESSIF_ERR_ARG, "which must be 0..6");
+    } else if (which < 0 || (which > 6 && which != 10)) {
+        return fail(ctx, AMRESSIF_ERR_ARG, "which must be 0..6 or 10");
     }
     if (which == 6 && !amr::stage_pair_fits(c)) return fail(ctx, AMRESSIF_ERR_ARG, "no two-stage kernel at this size");
-    if ((which == 3 || which == 4) && c.in.precision != AMRESSIF_MIXED)
-        return fail(ctx, AMRESSIF_ERR_ARG, "which = 3, 4 need a mixed-precision context");
+    if ((which == 3 || which == 4 || which == 10) && c.in.precision != AMRESSIF_MIXED)
+        return fail(ctx, AMRESSIF_ERR_ARG, "which = 3, 4, 10 need a mixed-precision context");
+    if (which == 10 && !amr::mixed_pair_fits(c)) return fail(ctx, AMRESSIF_ERR_ARG, "no one-pass mixed kernel at this size");
     auto launch = [&]() -> hipError_t {
+        if (which == 10) {  // the mixed iteration in one pass, u^k from U into S2
+            amr::MixedPairLaunch m{};
+            m.u64 = c.U[0]; m.v64 = c.U[1]; m.cu = c.C[0]; m.cv = c.C[1];
+            m.uo = c.S2[0]; m.vo = c.S2[1];
+            m.cr = 1.0e-12; m.cj = 1.0e-24;
+            m.unguarded = 1;
+            int nb = 0;
+            return amr::launch_mixed_pair(c, m, &nb);
+        }
         if (which == 3 || which == 4) {  // the two kernels of a mixed pseudo-iteration
             amr::MixedStageLaunch a{};
             a.stage = which - 3;
@@ -1848,10 +1863,11 @@ int amressif_time_kernel(amressif_ctx *ctx, int which, int reps, int64_t copy_by
         //   writes u^k, v^k (4 x 8 B);
         // 5 finest up-leg smoother + FP64 outer step: reads e, r (2 t), the coarse e (t / 4),
         //   phi, b (2 x 8 B), writes phi (8 B) and the next r (t);
-        // 6 two FP64 stages in one pass: the stage's 64 B (the first stage stays in registers)
+        // 6 two FP64 stages in one pass: the stage's 64 B (the first stage stays in registers);
+        // 10 mixed iteration in one pass: reads u^k, v^k, c_u, c_v, writes u^{k+1}, v^{k+1} (6 x 8 B)
         const double t = c.in.precision == AMRESSIF_MIXED ? 4.0 : 8.0;
         const double per_cell[7] = {64.0, 2.25 * t, 0.0, 48.0, 48.0, 3.25 * t + 24.0, 64.0};
-        *bytes_per_launch = which == 2 ? 2.0 * (double)copy_bytes : per_cell[which] * cells;
+        *bytes_per_launch = which == 2 ? 2.0 * (double)copy_bytes : which == 10 ? 48.0 * cells : per_cell[which] * cells;
     }
     return AMRESSIF_OK;
 }

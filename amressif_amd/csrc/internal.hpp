@@ -159,6 +159,7 @@ struct Ctx {
     int mpair_rows = 16;     // k_mixed_pair: output rows per wave
     int mp_on = 0;           // this step's mixed iterations run as k_mixed_pair (advance_impl)
     int mp_count = 0;        // k_mixed_pair launches of the step so far
+    int mp_spec = 0;         // the Poisson phase follows the first batch unseen: its last finalizer starts it
     // multigrid smoothers of the levels above the tail: 2 = wave-column kernel
     // with two columns per lane, 1 = one column per lane (both registers +
     // DPP), 0 = LDS tile kernel; output rows per wave of the wave-column
@@ -269,7 +270,9 @@ struct MixedPairLaunch {
 };
 hipError_t launch_mixed_pair(const Ctx &c, const MixedPairLaunch &a, int *nblocks);
 bool mixed_pair_fits(const Ctx &c);
-hipError_t launch_fin_pseudo(const Ctx &c, int nblocks);
+// the pseudo-iteration norm and loop control (k_mixed_pair's records when the
+// step runs it); start_mg: also the Poisson phase's start (k_mg_start)
+hipError_t launch_fin_pseudo(const Ctx &c, int nblocks, int start_mg = 0);
 // the norm of PSEUDO_TIMESTEPPING = 0's single update: ends the loop, countIter stays 0
 hipError_t launch_fin_explicit(const Ctx &c, int nblocks);
 hipError_t launch_ctl_reset_step(const Ctx &c);

@@ -1086,7 +1086,10 @@ __global__ __launch_bounds__(256) void k_tangent(StageArgs a)
 // (the two launches move 96).  The output must not alias u^k: A reads u^k in
 // the halo rows and columns that neighbouring waves update.  After convergence
 // (the launch skipped) u^k is copied to the output, so that the caller's
-// exchange of the buffers keeps the converged state.
+// exchange of the buffers keeps the converged state.  The partials are records
+// of three per wave: the two norm sums and the wave's share of the net outflow
+// sum (u(n) - u(0))/dx + (v(n) - v(0))/dy of u^{k+1}, whose total fixes the
+// mean of the Poisson right-hand side (the cell divergences telescope).
 __global__ __launch_bounds__(256) void k_mixed_pair(StageArgs a)
 {
     using R = double;
@@ -1268,6 +1271,7 @@ __global__ __launch_bounds__(256) void k_mixed_pair(StageArgs a)
     const F fidx = (F)a.inv_dx, fidy = (F)a.inv_dy, fidx2 = (F)a.inv_dx2, fidy2 = (F)a.inv_dy2, firen = (F)a.inv_ren;
     const double cr = a.cr, cj = a.cj;
     double nx_ = 0.0, ny_ = 0.0;
+    double sb = 0.0;  // the boundary flux of u^{k+1} through this wave's wall faces (net outflow)
     for (; jb < jend; ++jb) {
         emitA(uq3, vq3);        // row jb + 3
         const int jr = jb + 2;  // B's new window cell
@@ -1297,15 +1301,19 @@ __global__ __launch_bounds__(256) void k_mixed_pair(StageArgs a)
                 const bool wall = i == 0 || i == n;
                 const float Jr = wall ? 0.f : 0.5f * (TUw + TU);
                 const double du = cr * (double)uq0.r + cj * (double)Jr;
-                a.uo[k] = uq0.b64 + du;
+                const double unew = uq0.b64 + du;
+                a.uo[k] = unew;
                 nx_ += du * du * (wall ? 0.5 : 1.0);
+                if (wall) sb += i == n ? unew * a.inv_dx : -(unew * a.inv_dx);
             }
             if (xin) {
                 const bool wall = jb == 0;
                 const float Jr = wall ? 0.f : 0.5f * (tVprev + TV);
                 const double dv = cr * (double)vq0.r + cj * (double)Jr;
-                a.vo[k] = vq0.b64 + dv;
+                const double vnew = vq0.b64 + dv;
+                a.vo[k] = vnew;
                 ny_ += dv * dv * (wall ? 0.5 : 1.0);
+                if (wall) sb -= vnew * a.inv_dy;
             }
         }
         fsU = fnU; fsV = fnV; tVprev = TV;
@@ -1316,11 +1324,21 @@ __global__ __launch_bounds__(256) void k_mixed_pair(StageArgs a)
     }
     if (jend == n && outB && xin) {  // north wall row: R' r = 0 there
         const double dv = cr * (double)vq0.r;
-        a.vo[ic + (size_t)P * n] = vq0.b64 + dv;
+        const double vnew = vq0.b64 + dv;
+        a.vo[ic + (size_t)P * n] = vnew;
         ny_ += dv * dv * 0.5;
+        sb += vnew * a.inv_dy;
     }
-    for (int o = 32; o > 0; o >>= 1) { nx_ += __shfl_down(nx_, o, 64); ny_ += __shfl_down(ny_, o, 64); }
-    if (lane == 0) { a.partials[2 * wid] = nx_; a.partials[2 * wid + 1] = ny_; }
+    for (int o = 32; o > 0; o >>= 1) {
+        nx_ += __shfl_down(nx_, o, 64);
+        ny_ += __shfl_down(ny_, o, 64);
+        sb += __shfl_down(sb, o, 64);
+    }
+    if (lane == 0) {  // records of three: the norm partials and the wave's boundary flux of u^{k+1}
+        a.partials[3 * wid] = nx_;
+        a.partials[3 * wid + 1] = ny_;
+        a.partials[3 * wid + 2] = sb;
+    }
 }
 
 static StageArgs base_args(const Ctx &c, double rk, int mode, int rows)

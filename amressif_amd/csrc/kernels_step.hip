@@ -47,9 +47,62 @@ __global__ __launch_bounds__(1024) void k_fin_pseudo(const double *partials, int
     }
 }
 
-hipError_t launch_fin_pseudo(const Ctx &c, int nblocks)
+// The same for k_mixed_pair's records of three.  start_mg (the last
+// pseudo-iteration of a batch that the step's Poisson phase is enqueued
+// behind): also k_mg_start's work, from the final flag of this launch: skip
+// the phase unless the loop has ended, reset the solve's control, and the mean
+// of b = (1.5/dt) D(u*) from the net outflow in the partials of the last
+// launch that did work (a launch past convergence writes none).
+__global__ __launch_bounds__(1024) void k_fin_pseudo3(const double *partials, int nb, Ctl *ctl, int n, int start_mg,
+                                                      double dt)
 {
-    hipLaunchKernelGGL(k_fin_pseudo, dim3(1), dim3(1024), 0, c.stream, c.partials, nblocks, c.ctl, c.g.n);
+    __shared__ double sh[48];
+    const int done = __hip_atomic_load(&ctl->mom_done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    double sx = 0.0, sy = 0.0, sb = 0.0;
+    for_records<3, 3>(partials, nb, [&](const double *o) {
+        sx += o[0];
+        sy += o[1];
+        sb += o[2];
+    });
+    if (done && !start_mg) return;
+    double red[3] = {sx, sy, sb};
+    block_reduce_n<0u>(red, sh);
+    if (threadIdx.x == 0) {
+        if (!done) {
+            const double npts = (double)n * (double)n;
+            const double ex = sqrt(red[0]) / sqrt(npts), ey = sqrt(red[1]) / sqrt(npts);
+            const double ne = (ex < ey) ? ey : ex;
+            ctl->norm_error = ne;
+            ctl->mom_count += 1;
+            if (!(ne == ne) || isinf(ne)) ctl->flags |= AMRESSIF_FLAG_NONFINITE;
+            if (ctl->mom_count > ctl->iter_num) { ctl->flags |= AMRESSIF_FLAG_CAP; ctl->mom_done = 1; }
+            else if (ne > 1.e2) { ctl->flags |= AMRESSIF_FLAG_DIVERGED; ctl->mom_done = 1; }
+            else if (!(ne > ctl->tol_mom)) ctl->mom_done = 1;
+        }
+        if (start_mg) {  // k_mg_start with require_mom
+            const int skip = !ctl->mom_done;
+            ctl->mg_done = skip;
+            ctl->mg_skip = skip;
+            ctl->gm_stop = 0;
+            ctl->mg_iters = 0;
+            ctl->mg_started = 0;
+            ctl->b_norm_fixed = 0;
+            ctl->res_norm = 0.0;
+            ctl->b_norm = 0.0;
+            ctl->phi_sum = 0.0;
+            ctl->phi_mean = 0.0;
+            if (!skip) ctl->b_mean = (1.5 / dt) * red[2] / ((double)n * (double)n);
+        }
+    }
+}
+
+hipError_t launch_fin_pseudo(const Ctx &c, int nblocks, int start_mg)
+{
+    if (c.mp_on)
+        hipLaunchKernelGGL(k_fin_pseudo3, dim3(1), dim3(1024), 0, c.stream, c.partials, nblocks, c.ctl, c.g.n,
+                           start_mg, c.g.dt);
+    else
+        hipLaunchKernelGGL(k_fin_pseudo, dim3(1), dim3(1024), 0, c.stream, c.partials, nblocks, c.ctl, c.g.n);
     return hipGetLastError();
 }
 

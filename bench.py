@@ -51,7 +51,8 @@ HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 KERNELS = {0: "amr::k_stage64<0, true> (FP64 stage)", 1: "amr::k_gsrb2<T, 1, R> (finest down-leg smoother + restriction)",
            3: "amr::k_stage64<3, true> (mixed residual)", 4: "amr::k_tangent",
            5: "amr::k_gsrb2<T, 2, R> (finest up-leg smoother + FP64 outer step)",
-           6: "amr::k_stage_pair<false> (two FP64 stages, one pass)"}
+           6: "amr::k_stage_pair<false> (two FP64 stages, one pass)",
+           10: "amr::k_mixed_pair (mixed iteration, one pass)"}
 # in-step timed kernels (amressif_kernel_time) -> kernel symbol, key in profiles/pmc_summary.txt
 STEP_KERNELS = {
     "stage": ("amr::k_stage64<0, true> (FP64 RK stage)", "stage_fp64"),
@@ -65,7 +66,7 @@ STEP_KERNELS = {
     "poisson_start": ("amr::k_poisson_start (Poisson RHS, warm start, first residual)", None),
     "pair": ("amr::k_stage_pair<false> (FP64 RK stages 1-2 in one pass)", "pair_fp64"),
     "pair_last": ("amr::k_stage_pair<true> (FP64 RK stages 3-4 in one pass + norm partials)", None),
-    "mpair": ("amr::k_mixed_pair (mixed residual + FP32 tangent-linear update in one pass)", None),
+    "mpair": ("amr::k_mixed_pair (mixed residual + FP32 tangent-linear update in one pass)", "mpair_mixed"),
 }
 
 
@@ -254,7 +255,7 @@ def kernel_probe(api, W, n, ren, which_prec):
     prec = api.AMRESSIF_MIXED if which_prec == "mixed" else api.AMRESSIF_FP64
     run = api.Run.from_numpy(api.cavity_inputs(n, ren, precision=prec), W.initial_state(n, "seeded"))
     run.advance(1)
-    kinds = (3, 4, 1, 5) if which_prec == "mixed" else (0, 6, 1, 5)
+    kinds = (10, 3, 4, 1, 5) if which_prec == "mixed" else (0, 6, 1, 5)
     res = {KERNELS[k]: run.ctx.time_kernel(k, reps=3)[0] for k in kinds}
     res["copy 64 MiB"] = run.ctx.time_kernel(2, reps=3, copy_bytes=64 << 20)[0]
     print(json.dumps({"kernel_probe": which_prec, "n": n, "avg_ms": res}), flush=True)

@@ -369,12 +369,13 @@ int amressif_amr_advance(amressif_amr *amr, int nsteps, amressif_amr_stats *stat
  *   which = 4  mixed precision: FP32 tangent-linear update of the pseudo base
  *   which = 5  finest-level up-leg smoother with the fused FP64 outer step (V-cycle precision)
  *   which = 6  two FP64 RK stages in one pass (stages 1-2 of a pseudo-iteration)
+ *   which = 10 mixed precision: one mixed iteration in one pass (residual + tangent-linear update)
  * On a 3D or periodic context (the general path) the kernels are those of one
  * RK4 stage, each over the whole grid:
  *   which = 7  cell velocities with their ghost layers (cont2cart)
  *   which = 8  fluxTotal: QUICK convective and viscous fluxes per cell
  *   which = 9  stage update of every face
- * which = 3 and 4 need a context created with precision = 1.
+ * which = 3, 4 and 10 need a context created with precision = 1.
  * bytes_per_launch receives the kernel's algorithmic HBM bytes per launch. */
 int amressif_time_kernel(amressif_ctx *ctx, int which, int reps, int64_t copy_bytes, double *avg_ms,
                          double *bytes_per_launch);

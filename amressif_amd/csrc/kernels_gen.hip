@@ -476,6 +476,9 @@ __global__ __launch_bounds__(B) void k_gen_restrict(Geom g, Level F, Level Cl, c
     Cl.r[q] = (g.dim == 3 ? 0.125 : 0.25) * s;
 }
 
+// cell-centred (bi/tri)linear prolongation: per direction 3/4 of the coarse
+// cell and 1/4 of its neighbour on the fine cell's side (the periodic image,
+// or the cell itself past a wall: the Neumann mirror)
 __global__ __launch_bounds__(B) void k_gen_prolong(Geom g, Level F, Level Cl, const Ctl *ctl)
 {
     if (ctl->mg_done) return;
@@ -483,7 +486,24 @@ __global__ __launch_bounds__(B) void k_gen_prolong(Geom g, Level F, Level Cl, co
     if (q >= F.ncell) return;
     int f[3];
     lcoords(g, F.n, q, f);
-    F.e[q] += Cl.e[f[0] / 2 + (int64_t)Cl.n * (f[1] / 2 + (int64_t)Cl.n * (f[2] / 2))];
+    int c[3] = {0, 0, 0}, nb[3] = {0, 0, 0};
+    const int m = Cl.n;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (d >= g.dim) break;
+        c[d] = f[d] >> 1;
+        int t = (f[d] & 1) ? c[d] + 1 : c[d] - 1;
+        if (t < 0 || t >= m) t = g.per[d] ? (t + m) % m : c[d];
+        nb[d] = t;
+    }
+    double s = 0.0;
+    const int corners = 1 << g.dim;
+    for (int k = 0; k < corners; ++k) {
+        const int x = (k & 1) ? nb[0] : c[0], y = (k & 2) ? nb[1] : c[1], z = (k & 4) ? nb[2] : c[2];
+        const double w = ((k & 1) ? 0.25 : 0.75) * ((k & 2) ? 0.25 : 0.75) * (g.dim == 3 ? ((k & 4) ? 0.25 : 0.75) : 1.0);
+        s += w * Cl.e[x + (int64_t)m * (y + (int64_t)m * z)];
+    }
+    F.e[q] += s;
 }
 
 // e = pinv r on the coarsest level (pinv symmetric, read by columns)

@@ -64,6 +64,34 @@ struct OuterArgs {  // MODE 2
     double ax, ay;  // the FP64 operator's coefficients (not the V-cycle's T-rounded ones)
 };
 
+// Prolongation of the coarse correction to fine cell (i, j) in coarse cell
+// (I, J) = (i / 2, j / 2); ncx x ncy: the coarse level.
+//  * FP64 V-cycle: cell-centred bilinear, 9/16 of e(I, J), 3/16 of its x and
+//    y neighbours on the fine cell's side and 1/16 of their diagonal
+//    neighbour; past a wall the neighbour is the cell itself (the mirror ghost
+//    of the Neumann operator).  With the 2 x 2 average restriction its orders
+//    sum to 3 (> 2): the FP64 step's cold-started solve takes 7 cycles instead
+//    of 8 to 1e-10 at 2048^2.
+//  * FP32 V-cycle (the mixed step's preconditioner): e(I, J).  There the
+//    contraction of the FP64 refinement is bounded by the FP32 correction's
+//    rounding, not by the multigrid rate, and the bilinear form took more
+//    cycles (5.0 against 4.35 per step over the first 25 steps at 2048^2).
+template <class T>
+__device__ __forceinline__ T prolong_at(const T *ec, int pc, int ncx, int ncy, int i, int j)
+{
+    const int I = i >> 1, J = j >> 1;
+    if constexpr (sizeof(T) == 4) {
+        (void)ncx;
+        (void)ncy;
+        return ec[I + (size_t)pc * J];
+    }
+    const int In = (i & 1) ? min(I + 1, ncx - 1) : max(I - 1, 0);
+    const int Jn = (j & 1) ? min(J + 1, ncy - 1) : max(J - 1, 0);
+    const size_t r0 = (size_t)pc * J, r1 = (size_t)pc * Jn;
+    const T c = ec[I + r0], x = ec[In + r0], y = ec[I + r1], d = ec[In + r1];
+    return (T(9) * c + T(3) * (x + y) + d) * T(0.0625);
+}
+
 template <class T, int MODE>
 __global__ __launch_bounds__(256) void k_smooth(const T *xin, const T *ec, int pc, T *xout, const T *rhs, int nx,
                                                 int ny, int pitch, T ax, T ay, T *rc, int ncx, int ncy, int pcc,
@@ -95,7 +123,7 @@ __global__ __launch_bounds__(256) void k_smooth(const T *xin, const T *ec, int p
             if (lj >= 0 && i >= 0 && i < nx && j >= 0 && j < ny) {
                 const size_t k = i + (size_t)pitch * j;
                 if (xin) x = xin[k];
-                if (ec) x += ec[(i >> 1) + (size_t)pc * (j >> 1)];
+                if (ec) x += prolong_at(ec, pc, ncx, ncy, i, j);
                 r = rhs[k];
             }
             xv[q][c2] = x;
@@ -274,19 +302,18 @@ __global__ __launch_bounds__(256) void k_gsrb(const T *xin, const T *ec, int pc,
     const bool yin = jb >= 0 && jb + NR <= ny;  // every window row is a domain row
     T x[NR], r[NR];
     {
-        int k = i + pitch * jb, kc = (i >> 1) + pc * (jb >> 1);
+        int k = i + pitch * jb;
 #pragma unroll
         for (int q = 0; q < NR; ++q) {
             const int j = jb + q;
             T xv = 0, rv = 0;
             if (in_i && (yin || (j >= 0 && j < ny))) {
                 rv = rhs[k];
-                if (MODE != 1) xv = xin[k] + ec[kc];
+                if (MODE != 1) xv = xin[k] + prolong_at(ec, pc, ncx, ncy, i, j);
             }
             x[q] = xv;
             r[q] = rv;
             k += pitch;
-            if (((q + 1 + H) & 1) == 0) kc += pc;  // row jb + q + 1 is even: next coarse row
         }
     }
     // MODE 2: the FP64 iterate on the output rows and one row either side, b on the output rows
@@ -475,7 +502,7 @@ __global__ __launch_bounds__(256) void k_gsrb2(const T *xin, const T *ec, int pc
     const bool yin = jb >= 0 && jb + NR <= ny;
     T x0[NR], x1[NR], r0[NR], r1[NR];
     {
-        int k = ca + pitch * jb, kc = (ca >> 1) + pc * (jb >> 1);
+        int k = ca + pitch * jb;
 #pragma unroll
         for (int q = 0; q < NR; ++q) {
             const int j = jb + q;
@@ -484,9 +511,8 @@ __global__ __launch_bounds__(256) void k_gsrb2(const T *xin, const T *ec, int pc
                 rv = *reinterpret_cast<const V *>(rhs + k);
                 if (MODE != 1) {
                     xv = *reinterpret_cast<const V *>(xin + k);
-                    const T e = ec[kc];
-                    xv.x += e;
-                    xv.y += e;
+                    xv.x += prolong_at(ec, pc, ncx, ncy, ca, j);
+                    xv.y += prolong_at(ec, pc, ncx, ncy, ca + 1, j);
                 }
             }
             x0[q] = xv.x;
@@ -494,7 +520,6 @@ __global__ __launch_bounds__(256) void k_gsrb2(const T *xin, const T *ec, int pc
             r0[q] = rv.x;
             r1[q] = rv.y;
             k += pitch;
-            if (((q + 1 + H) & 1) == 0) kc += pc;  // row jb + q + 1 is even: next coarse row
         }
     }
     constexpr int NP = MODE == 2 ? R + 2 : 1, NB = MODE == 2 ? R : 1;
@@ -760,9 +785,10 @@ __global__ __launch_bounds__(1024) void k_tail(const T *rtop, int ptop, T *etop,
         const T *Rl = Rr + tl.off[l];
         const T ax = (T)tl.ax[l], ay = (T)tl.ay[l];
         const float inx = 1.0f / (float)nx;
+        const int cy = tl.ny[l + 1];
         for (int k = tid; k < nx * ny; k += nt) {
             const int j = qdiv(k, inx);
-            El[k] += Ec[((k - j * nx) >> 1) + cx * (j >> 1)];
+            El[k] += prolong_at(Ec, cx, cx, cy, k - j * nx, j);
         }
         __syncthreads();
         for (int s = 0; s < 2; ++s) { tail_gsrb(El, Rl, nx, ny, ax, ay, 0); tail_gsrb(El, Rl, nx, ny, ax, ay, 1); }
@@ -1134,9 +1160,8 @@ __global__ __launch_bounds__(64 * NW) void k_tail_dct_ext(const float *r1g, int 
     for (int q = 0; q < R1 + 2; ++q) {
         const int j = j0 + q - 1;
         if (j >= 0 && j < ny1 && in_i) {
-            const float e = B0[(j >> 1) * kDctLd + lane];
-            x0[q] += e;
-            x1[q] += e;
+            x0[q] += prolong_at<float>(B0, kDctLd, nx, ny, ca, j);
+            x1[q] += prolong_at<float>(B0, kDctLd, nx, ny, ca + 1, j);
         }
     }
     ext_sweeps<R1, NW>(x0, x1, r0, r1, A, B, j0, ny1, xch, par, w, lane, false);
@@ -1685,13 +1710,14 @@ static void launch_gsrb(const Ctx &c, const SmoothShapeRt &s, const T *xin, cons
 }
 
 // one smoother on level l (MODE 1: from zero with the fused restriction into
-// level l + 1; MODE 0: from xin + P(ec); MODE 2: MODE 0 plus the FP64 outer step)
+// level l + 1; MODE 0: from xin + P(ec), ec on level l + 1; MODE 2: MODE 0
+// plus the FP64 outer step)
 template <class T, int MODE>
 static void launch_smoother(const Ctx &c, int l, const T *xin, const T *ec, int pc, T *xout, const T *rhs,
                             const OuterArgs &o)
 {
     const Level &L = c.lev[l];
-    const Level *Cl = MODE == 1 ? &c.lev[l + 1] : nullptr;
+    const Level *Cl = &c.lev[l + 1];
     T *rc = MODE == 1 ? lev_r<T>(*Cl) : (T *)nullptr;
     const SmoothShapeRt s = smooth_shape<T>(c, l, MODE);
     if (!s.wave) {

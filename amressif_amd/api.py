@@ -332,6 +332,73 @@ def box_array(n: int, max_grid_size) -> list:
     return [(x0, y0, x1, y1) for (y0, y1) in cuts(n, my) for (x0, x1) in cuts(n, mx)]
 
 
+def box_array_nd(n: int, dim: int, max_grid_size) -> list:
+    """Cell boxes ((lo x, lo y[, lo z]), (hi x, hi y[, hi z])) of the n^dim domain
+    chopped as box_array does per direction (x fastest in the list)."""
+    def cuts(length, m):
+        k = -(-length // m)
+        size, extra = divmod(length, k)
+        out, lo = [], 0
+        for b in range(k):
+            w = size + (1 if b < extra else 0)
+            out.append((lo, lo + w - 1))
+            lo += w
+        return out
+    m = (max_grid_size,) * 3 if isinstance(max_grid_size, int) else tuple(max_grid_size)
+    cx, cy = cuts(n, m[0]), cuts(n, m[1])
+    cz = cuts(n, m[2]) if dim == 3 else [(0, 0)]
+    out = []
+    for (z0, z1) in cz:
+        for (y0, y1) in cy:
+            for (x0, x1) in cx:
+                lo, hi = (x0, y0, z0), (x1, y1, z1)
+                out.append((lo[:dim], hi[:dim]))
+    return out
+
+
+class MultiFabND:
+    """MultiFab over box_array_nd boxes in 2D or 3D: kind = face direction d
+    (nodal in d: both bounding faces per box) or "cell" (ncomp components,
+    ngrow ghosts).  Tensors are C-ordered with the slowest axis first:
+    (comps, [z,] y, x) for cells, ([z,] y, x) for faces."""
+
+    def __init__(self, boxes, dim: int, kind, ncomp: int = 1, ngrow: int = 0, device="cuda"):
+        self.boxes, self.dim, self.kind, self.ncomp, self.ngrow = list(boxes), dim, kind, ncomp, ngrow
+        self.fabs = []
+        arr = (Array4 * len(self.boxes))()
+        for k, (lo, hi) in enumerate(self.boxes):
+            vlo, vhi = self.window(lo, hi)
+            ext = [vhi[d] - vlo[d] + 1 for d in range(dim)][::-1]
+            shape = tuple(([ncomp] if kind == "cell" else []) + ext)
+            t = torch.zeros(shape, dtype=torch.float64, device=device)
+            self.fabs.append(t)
+            a = Array4()
+            a.ptr = t.data_ptr()
+            for d in range(3):
+                a.lo[d], a.hi[d] = (vlo[d], vhi[d]) if d < dim else (0, 0)
+            a.ncomp = ncomp
+            a.dtype = 0
+            arr[k] = a
+        self._arr = arr
+        self.c = MultiFabC(len(self.boxes), ctypes.cast(arr, ctypes.POINTER(Array4)), ngrow)
+
+    def window(self, lo, hi):
+        """the view's bounds (ghosts included) of a valid cell box"""
+        g = self.ngrow
+        vlo = [lo[d] - g for d in range(self.dim)]
+        vhi = [hi[d] + g + (1 if self.kind == d else 0) for d in range(self.dim)]
+        return vlo, vhi
+
+    def fill_from(self, glob, src_ghosts: int = 0):
+        """each box's view from a global array (faces: face_shape; cells: (comps, [n + 2 g,] ...))"""
+        for (lo, hi), t in zip(self.boxes, self.fabs):
+            vlo, vhi = self.window(lo, hi)
+            sl = tuple(slice(vlo[d] + src_ghosts, vhi[d] + 1 + src_ghosts) for d in range(self.dim))[::-1]
+            part = glob[(slice(None),) + sl] if self.kind == "cell" else glob[sl]
+            t.copy_(torch.as_tensor(np.ascontiguousarray(part), dtype=torch.float64))
+        return self
+
+
 class MultiFab:
     """Per-box torch tensors of one field over a box_array, with the ctypes
     amr_multifab that views them.  kind: "x" / "y" faces (nodal in that
@@ -483,6 +550,21 @@ class Context:
         """amressif_store_state_mf: the state into multi-box MultiFabs (any may be None)"""
         vc = (MultiFabC * 2)(u.c, v.c) if u is not None else None
         vp = (MultiFabC * 2)(uprev.c, vprev.c) if uprev is not None else None
+        self._check(_lib.amressif_store_state_mf(self._h, vc, vp, ctypes.byref(ctx.c) if ctx is not None else None,
+                                                 ctypes.byref(velCart.c) if velCart is not None else None))
+
+    def load_faces_mf(self, faces, faces_prev, ctx=None):
+        """amressif_load_state_mf with dim MultiFabND per face state (3D and periodic contexts too)"""
+        d = self.dim
+        vc = (MultiFabC * d)(*[m.c for m in faces])
+        vp = (MultiFabC * d)(*[m.c for m in faces_prev])
+        self._check(_lib.amressif_load_state_mf(self._h, vc, vp, ctypes.byref(ctx.c) if ctx is not None else None))
+
+    def store_faces_mf(self, faces=None, faces_prev=None, ctx=None, velCart=None):
+        """amressif_store_state_mf into dim MultiFabND per face state, userCtx and velCart (any may be None)"""
+        d = self.dim
+        vc = (MultiFabC * d)(*[m.c for m in faces]) if faces is not None else None
+        vp = (MultiFabC * d)(*[m.c for m in faces_prev]) if faces_prev is not None else None
         self._check(_lib.amressif_store_state_mf(self._h, vc, vp, ctypes.byref(ctx.c) if ctx is not None else None,
                                                  ctypes.byref(velCart.c) if velCart is not None else None))
 

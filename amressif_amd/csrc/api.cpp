@@ -1179,8 +1179,7 @@ extern "C" {
 int amressif_load_state_mf(amressif_ctx *ctx, const amr_multifab velCont[2], const amr_multifab velContPrev[2],
                            const amr_multifab *userCtx)
 {
-    if (ctx && ctx->c.gen)
-        return amr_fail(const_cast<amressif_ctx *>(ctx), AMRESSIF_ERR_UNSUPPORTED, "amressif_load_state_mf: not available for 3D or periodic inputs");
+    if (ctx && ctx->c.gen) return amr_gen_load_state_mf(ctx, velCont, velContPrev, userCtx);
     if (!ctx || !velCont || !velContPrev) return fail(ctx, AMRESSIF_ERR_ARG, "null argument");
     Ctx &c = ctx->c;
     // validate everything before the first copy: a failed load leaves the state as it was
@@ -1221,7 +1220,7 @@ int amressif_store_state_mf(const amressif_ctx *cctx, amr_multifab velCont[2], a
                             amr_multifab *userCtx, amr_multifab *velCart)
 {
     if (cctx && cctx->c.gen)
-        return amr_fail(const_cast<amressif_ctx *>(cctx), AMRESSIF_ERR_UNSUPPORTED, "amressif_store_state_mf: not available for 3D or periodic inputs");
+        return amr_gen_store_state_mf(const_cast<amressif_ctx *>(cctx), velCont, velContPrev, userCtx, velCart);
     amressif_ctx *ctx = const_cast<amressif_ctx *>(cctx);
     if (!ctx) return AMRESSIF_ERR_ARG;
     Ctx &c = ctx->c;

@@ -89,6 +89,10 @@ hipError_t fill(hipStream_t s, double *dst, int64_t count, double value);
 // Array4 views <-> internal arrays: a box region [0, ext) with the view's lo/hi
 hipError_t pack(hipStream_t s, const amr_array4 &a, int comp, double *dst, int ex, int ey, int ez);
 hipError_t unpack(hipStream_t s, const double *src, const amr_array4 &a, int comp, int ex, int ey, int ez);
+// the window [lo, hi] of a view <-> the same cells of an internal array of
+// extents ex x ey x .. (unpack 0: view -> array)
+hipError_t pack_window(hipStream_t s, const amr_array4 &a, int comp, double *dst, int ex, int ey, const int *lo,
+                       const int *hi, int unpack);
 // userCtx / velCart views with ghosts: the cells the view holds, ghosts by the
 // rules (p, phi: Neumann mirror / periodic image; phi shifted by -mean)
 hipError_t unpack_cells_ghosted(const Ctx &c, hipStream_t s, const double *src, const amr_array4 &a, int comp,

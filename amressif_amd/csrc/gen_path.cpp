@@ -366,6 +366,148 @@ int amr_gen_store_state(amressif_ctx *ctx, amr_array4 *vel, amr_array4 *velPrev,
     return AMRESSIF_OK;
 }
 
+// ---- MultiFabs of several boxes (amressif_load_state_mf / _store_state_mf)
+namespace {
+
+struct Box3 {
+    int lo[3], hi[3];
+};
+
+// The valid boxes of a MultiFab (nodal: the face direction, or -1 for cells):
+// inside the domain, pairwise disjoint and covering it; faces also returns
+// the boxes' face windows (the cell box with hi + 1 along the nodal direction)
+int check_mf3(amressif_ctx *ctx, const amr_multifab *mf, int nodal, int ncomp, int max_grow, const char *what,
+              std::vector<Box3> &win)
+{
+    const G::Geom &g = ctx->c.gen->g;
+    const std::string w(what);
+    if (!mf || mf->nbox < 1 || !mf->fab) return gfail(ctx, AMRESSIF_ERR_ARG, w + ": empty MultiFab");
+    if (mf->ngrow < 0 || mf->ngrow > max_grow)
+        return gfail(ctx, AMRESSIF_ERR_ARG, w + ": ngrow must be in [0, " + std::to_string(max_grow) + "]");
+    const int ext[3] = {g.n, g.n, g.nk};
+    std::vector<Box3> cells;
+    win.clear();
+    int64_t vol = 0;
+    for (int b = 0; b < mf->nbox; ++b) {
+        const amr_array4 &a = mf->fab[b];
+        if (!a.ptr) return gfail(ctx, AMRESSIF_ERR_ARG, w + ": null fab");
+        if (a.dtype != 0) return gfail(ctx, AMRESSIF_ERR_ARG, w + ": double views only");
+        if (a.ncomp < ncomp) return gfail(ctx, AMRESSIF_ERR_ARG, w + ": too few components");
+        Box3 c{};
+        for (int d = 0; d < 3; ++d) {
+            const int gr = d < g.dim ? mf->ngrow : 0;
+            c.lo[d] = a.lo[d] + gr;
+            c.hi[d] = a.hi[d] - gr - (d == nodal ? 1 : 0);
+            if (c.lo[d] < 0 || c.hi[d] >= ext[d] || c.hi[d] < c.lo[d])
+                return gfail(ctx, AMRESSIF_ERR_ARG, w + ": a valid box lies outside the domain");
+        }
+        for (const Box3 &o : cells) {
+            bool disjoint = false;
+            for (int d = 0; d < 3; ++d) disjoint = disjoint || c.hi[d] < o.lo[d] || o.hi[d] < c.lo[d];
+            if (!disjoint) return gfail(ctx, AMRESSIF_ERR_ARG, w + ": valid boxes overlap");
+        }
+        vol += (int64_t)(c.hi[0] - c.lo[0] + 1) * (c.hi[1] - c.lo[1] + 1) * (c.hi[2] - c.lo[2] + 1);
+        cells.push_back(c);
+        Box3 f = c;
+        if (nodal >= 0) f.hi[nodal] += 1;
+        win.push_back(f);
+    }
+    if (vol != g.ncell) return gfail(ctx, AMRESSIF_ERR_ARG, w + ": valid boxes do not cover the domain");
+    return 0;
+}
+
+}  // namespace
+
+int amr_gen_load_state_mf(amressif_ctx *ctx, const amr_multifab *vel, const amr_multifab *velPrev,
+                          const amr_multifab *userCtx)
+{
+    Ctx &c = ctx->c;
+    G::Ctx &gc = *c.gen;
+    const G::Geom &g = gc.g;
+    if (!vel || !velPrev) return gfail(ctx, AMRESSIF_ERR_ARG, "null argument");
+    // validate everything before the first copy: a failed load leaves the state as it was
+    std::vector<Box3> bx[7];
+    for (int d = 0; d < g.dim; ++d) {
+        if (int rc = check_mf3(ctx, &vel[d], d, 1, 0, "velCont", bx[2 * d])) return rc;
+        if (int rc = check_mf3(ctx, &velPrev[d], d, 1, 0, "velContPrev", bx[2 * d + 1])) return rc;
+    }
+    if (userCtx)
+        if (int rc = check_mf3(ctx, userCtx, -1, 2, 1 << 20, "userCtx", bx[6])) return rc;
+    for (int d = 0; d < g.dim; ++d) {
+        int ext[3];
+        face_ext(g, d, ext);
+        for (int k = 0; k < 2; ++k) {
+            const amr_multifab &mf = k ? velPrev[d] : vel[d];
+            double *dst = k ? gc.Up[d] : gc.U[d];
+            for (int b = 0; b < mf.nbox; ++b) {
+                const Box3 &q = bx[2 * d + k][b];
+                GCHK(G::pack_window(c.stream, mf.fab[b], 0, dst, ext[0], ext[1], q.lo, q.hi, 0));
+            }
+        }
+    }
+    if (userCtx) {
+        for (int b = 0; b < userCtx->nbox; ++b) {
+            const Box3 &q = bx[6][b];
+            GCHK(G::pack_window(c.stream, userCtx->fab[b], 0, gc.p, g.n, g.n, q.lo, q.hi, 0));
+            GCHK(G::pack_window(c.stream, userCtx->fab[b], 1, gc.phi, g.n, g.n, q.lo, q.hi, 0));
+        }
+    } else {
+        GCHK(G::fill(c.stream, gc.p, g.ncell, 0.0));
+        GCHK(G::fill(c.stream, gc.phi, g.ncell, 0.0));
+    }
+    c.pred_valid = 0;
+    GCHK(hipStreamSynchronize(c.stream));
+    return AMRESSIF_OK;
+}
+
+int amr_gen_store_state_mf(amressif_ctx *ctx, amr_multifab *vel, amr_multifab *velPrev, amr_multifab *userCtx,
+                           amr_multifab *velCart)
+{
+    Ctx &c = ctx->c;
+    G::Ctx &gc = *c.gen;
+    const G::Geom &g = gc.g;
+    std::vector<Box3> bx[6], cb;
+    for (int d = 0; d < g.dim; ++d) {
+        if (vel)
+            if (int rc = check_mf3(ctx, &vel[d], d, 1, 0, "velCont", bx[2 * d])) return rc;
+        if (velPrev)
+            if (int rc = check_mf3(ctx, &velPrev[d], d, 1, 0, "velContPrev", bx[2 * d + 1])) return rc;
+    }
+    if (userCtx)
+        if (int rc = check_mf3(ctx, userCtx, -1, 2, 1, "userCtx", cb)) return rc;
+    if (velCart)
+        if (int rc = check_mf3(ctx, velCart, -1, g.dim, 2, "velCart", cb)) return rc;
+    for (int d = 0; d < g.dim; ++d) {
+        int ext[3];
+        face_ext(g, d, ext);
+        for (int k = 0; k < 2; ++k) {
+            amr_multifab *mf = k ? velPrev : vel;
+            if (!mf) continue;
+            const double *src = k ? gc.Up[d] : gc.U[d];
+            for (int b = 0; b < mf[d].nbox; ++b) {
+                const Box3 &q = bx[2 * d + k][b];
+                GCHK(G::pack_window(c.stream, mf[d].fab[b], 0, const_cast<double *>(src), ext[0], ext[1], q.lo, q.hi,
+                                    1));
+            }
+        }
+    }
+    // cell views are written whole, ghosts by global index: the neighbouring
+    // box's values inside the domain, the mirror or periodic image outside it
+    if (userCtx) {
+        GCHK(G::phi_mean(gc, c.stream, c.ctl));
+        for (int b = 0; b < userCtx->nbox; ++b) {
+            GCHK(G::unpack_cells_ghosted(gc, c.stream, gc.p, userCtx->fab[b], 0, nullptr));
+            GCHK(G::unpack_cells_ghosted(gc, c.stream, gc.phi, userCtx->fab[b], 1, &c.ctl->phi_mean));
+        }
+    }
+    if (velCart) {
+        GCHK(G::cells(gc, c.stream, gc.U, gc.X, nullptr));
+        for (int b = 0; b < velCart->nbox; ++b) GCHK(G::unpack_velcart(gc, c.stream, gc.X, velCart->fab[b]));
+    }
+    GCHK(hipStreamSynchronize(c.stream));
+    return AMRESSIF_OK;
+}
+
 int amr_gen_init_state(amressif_ctx *ctx)
 {
     Ctx &c = ctx->c;

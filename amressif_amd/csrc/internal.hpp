@@ -474,5 +474,9 @@ int amr_gen_load_state(amressif_ctx *ctx, const amr_array4 *vel, const amr_array
 int amr_gen_store_state(amressif_ctx *ctx, amr_array4 *vel, amr_array4 *velPrev, amr_array4 *userCtx,
                         amr_array4 *velCart);
 int amr_gen_init_state(amressif_ctx *ctx);
+int amr_gen_load_state_mf(amressif_ctx *ctx, const amr_multifab *vel, const amr_multifab *velPrev,
+                          const amr_multifab *userCtx);
+int amr_gen_store_state_mf(amressif_ctx *ctx, amr_multifab *vel, amr_multifab *velPrev, amr_multifab *userCtx,
+                           amr_multifab *velCart);
 int amr_gen_advance(amressif_ctx *ctx, int nsteps, amressif_step_stats *stats);
 int amr_gen_time_kernel(amressif_ctx *ctx, int which, int reps, double *avg_ms, double *bytes_per_launch);

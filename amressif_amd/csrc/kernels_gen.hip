@@ -830,6 +830,28 @@ hipError_t unpack(hipStream_t s, const double *src, const amr_array4 &a, int com
     return hipGetLastError();
 }
 
+// the window [lo, hi] (inclusive, inside the view) between a view and an
+// internal array of extents ex x ey x ..: view -> array (unpack 0) or back
+__global__ __launch_bounds__(B) void k_gen_pack_window(V4 v, int comp, double *dst, int ex, int ey, int lo0, int lo1,
+                                                       int lo2, int wx, int wy, int wz, int unpack)
+{
+    const int64_t q = (int64_t)blockIdx.x * B + threadIdx.x;
+    if (q >= (int64_t)wx * wy * wz) return;
+    const int i = lo0 + (int)(q % wx), j = lo1 + (int)((q / wx) % wy), k = lo2 + (int)(q / ((int64_t)wx * wy));
+    const int64_t e = vidx(v, i, j, k, comp), d = i + (int64_t)ex * (j + (int64_t)ey * k);
+    if (unpack) v.ptr[e] = dst[d];
+    else dst[d] = v.ptr[e];
+}
+
+hipError_t pack_window(hipStream_t s, const amr_array4 &a, int comp, double *dst, int ex, int ey, const int *lo,
+                       const int *hi, int unpack)
+{
+    const int wx = hi[0] - lo[0] + 1, wy = hi[1] - lo[1] + 1, wz = hi[2] - lo[2] + 1;
+    hipLaunchKernelGGL(k_gen_pack_window, dim3(nblk((int64_t)wx * wy * wz)), dim3(B), 0, s, view(a), comp, dst, ex, ey,
+                       lo[0], lo[1], lo[2], wx, wy, wz, unpack);
+    return hipGetLastError();
+}
+
 hipError_t unpack_cells_ghosted(const Ctx &c, hipStream_t s, const double *src, const amr_array4 &a, int comp,
                                 const double *shift)
 {

@@ -146,7 +146,9 @@ int amressif_set_lid_profile(amressif_ctx *ctx, const double *dev_lid);
 
 /* ------------------------------------------------- state and time step */
 /* Copy the caller's state into the context: velCont[2] and velContPrev[2]
- * (faces, 0 ghosts), userCtx (2 comps p, phi; >= 0 ghosts). */
+ * (faces, 0 ghosts), userCtx (2 comps p, phi; >= 0 ghosts).  On a 3D context
+ * (spacedim = 3) every velCont / velContPrev argument of this section is an
+ * array of 3 entries (x, y, z faces), here and in the _mf forms. */
 int amressif_load_state(amressif_ctx *ctx, const amr_array4 velCont[2], const amr_array4 velContPrev[2],
                         const amr_array4 *userCtx);
 /* Copy the state out.  Any pointer may be NULL.  velCart (2 comps, 2 ghosts)
@@ -169,7 +171,9 @@ typedef struct {
 /* amressif_load_state over several boxes: each box's valid region is
  * gathered into the context's one-box working set (a shared face is read
  * from the later box; a consistent MultiFab holds the same value in both).
- * userCtx may be NULL (p = phi = 0). */
+ * userCtx may be NULL (p = phi = 0).  2D and 3D, walls and periodic
+ * directions; boxes are validated before the first copy, so a refused load
+ * leaves the state as it was. */
 int amressif_load_state_mf(amressif_ctx *ctx, const amr_multifab velCont[2], const amr_multifab velContPrev[2],
                            const amr_multifab *userCtx);
 /* amressif_store_state over several boxes: every element of every view is

@@ -229,3 +229,67 @@ def test_general_path_kernel_timing():
         ms, nbytes = c.time_kernel(which, reps=5)
         print(f"kernel {which}: {ms * 1e3:.1f} us, {nbytes / ms / 1e6:.0f} GB/s")
         assert ms > 0 and nbytes > 0
+
+
+@pytest.mark.parametrize("dim,n,mgs,bc_lo,bc_hi", [
+    (3, 20, (8, 12, 20), (135, -135, 135), (135, 165, 135)),   # ragged x boxes, walls, slip, lid
+    (3, 16, (8, 8, 8), (111, 135, 111), (111, 165, 111)),      # periodic x and z
+    (2, 40, (16, 24), (111, 135, 0), (111, 165, 0))])          # 2D periodic x
+def test_multibox_general_path_matches_one_box(dim, n, mgs, bc_lo, bc_hi):
+    """amressif_load_state_mf / _store_state_mf on the general path: a run loaded
+    from and stored to the boxes of max_grid_size (main.cpp:317-324) is bit
+    for bit the one-box run, ghosts of userCtx and velCart included (inside the
+    domain the neighbouring box's values, outside it the wall rules or the
+    periodic image)."""
+    api = _api()
+    import torch
+    inp = _inputs(api, dim, n, 200.0, bc_lo, bc_hi, inflow=(1.0, 0.0, 0.5))
+    o = GenOracle(dim, n, 200.0, api.dt_of(inp), bc_lo=bc_lo, bc_hi=bc_hi, inflow=(1.0, 0.0, 0.5))
+    rng = np.random.default_rng(11)
+    f = _random_faces(o, rng, 0.2, bc_lo, bc_hi)
+    fp = [a.copy() for a in f]
+    ctx = o.ctx()
+    ctx[(slice(None),) + (slice(1, -1),) * dim] = rng.uniform(-1, 1, (2,) + (n,) * dim)
+    steps = 3
+    one = api.Context(inp)
+    one.load_faces([_cuda(a) for a in f], [_cuda(a) for a in fp], _cuda(ctx))
+    one.advance(steps)
+    gf = [torch.zeros(api.face_shape(d, n, dim), dtype=torch.float64, device="cuda") for d in range(dim)]
+    gp = [torch.zeros(api.face_shape(d, n, dim), dtype=torch.float64, device="cuda") for d in range(dim)]
+    gc = torch.zeros(o.cell_shape(1, 2), dtype=torch.float64, device="cuda")
+    gv = torch.zeros(o.cell_shape(2, dim), dtype=torch.float64, device="cuda")
+    one.store_faces(gf, gp, gc, gv)
+    ref = {"f": [a.cpu().numpy() for a in gf], "fp": [a.cpu().numpy() for a in gp], "ctx": gc.cpu().numpy(),
+           "vc": gv.cpu().numpy()}
+
+    boxes = api.box_array_nd(n, dim, mgs)
+    assert len(boxes) > 1
+    MF = api.MultiFabND
+    mf_f = [MF(boxes, dim, d).fill_from(f[d]) for d in range(dim)]
+    mf_p = [MF(boxes, dim, d).fill_from(fp[d]) for d in range(dim)]
+    mf_c = MF(boxes, dim, "cell", 2, 1).fill_from(ctx, src_ghosts=1)
+    many = api.Context(inp)
+    many.load_faces_mf(mf_f, mf_p, mf_c)
+    stats = many.advance(steps)
+    assert all(s["flags"] == 0 for s in stats)
+    out_f = [MF(boxes, dim, d) for d in range(dim)]
+    out_p = [MF(boxes, dim, d) for d in range(dim)]
+    out_c = MF(boxes, dim, "cell", 2, 1)
+    out_v = MF(boxes, dim, "cell", dim, 2)
+    many.store_faces_mf(out_f, out_p, out_c, out_v)
+
+    def window(glob, mf, b, ghosts):
+        vlo, vhi = mf.window(*mf.boxes[b])
+        sl = tuple(slice(vlo[d] + ghosts, vhi[d] + 1 + ghosts) for d in range(dim))[::-1]
+        return glob[(slice(None),) + sl] if mf.kind == "cell" else glob[sl]
+
+    for b in range(len(boxes)):
+        for d in range(dim):
+            assert np.array_equal(out_f[d].fabs[b].cpu().numpy(), window(ref["f"][d], out_f[d], b, 0)), (b, d)
+            assert np.array_equal(out_p[d].fabs[b].cpu().numpy(), window(ref["fp"][d], out_p[d], b, 0)), (b, d)
+        assert np.array_equal(out_c.fabs[b].cpu().numpy(), window(ref["ctx"], out_c, b, 1)), b
+        assert np.array_equal(out_v.fabs[b].cpu().numpy(), window(ref["vc"], out_v, b, 2)), b
+    # a box array that does not tile the domain is refused, the state unchanged
+    with pytest.raises(api.AmressifError) as e:
+        many.load_faces_mf([MF(boxes[:-1], dim, d) for d in range(dim)], mf_p, mf_c)
+    assert e.value.code == -1

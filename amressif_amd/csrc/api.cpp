@@ -818,6 +818,7 @@ int amressif_create(const amressif_inputs *in, void *hip_stream, amressif_ctx **
     if (const char *e = std::getenv("AMRESSIF_PAIR_ROWS")) c.pair_rows = std::max(4, std::min(256, std::atoi(e)));
     if (const char *e = std::getenv("AMRESSIF_MIXED_PAIR")) c.mixed_pair = std::atoi(e) ? 1 : 0;
     if (const char *e = std::getenv("AMRESSIF_MPAIR_ROWS")) c.mpair_rows = std::max(4, std::min(256, std::atoi(e)));
+    if (const char *e = std::getenv("AMRESSIF_EDGE_FIRST")) c.stage_edge_first = std::atoi(e) ? 1 : 0;
     // and of the multigrid smoothers (kernel choice and rows per wave: 4, 8, 16 or 32)
     if (const char *e = std::getenv("AMRESSIF_MG_SMOOTHER")) c.mg_smoother = std::max(0, std::min(2, std::atoi(e)));
     if (const char *e = std::getenv("AMRESSIF_TAIL_DCT")) c.tail_dct = std::atoi(e) ? 1 : 0;

@@ -157,6 +157,7 @@ struct Ctx {
     // used by the step without a tape (advance) with the multigrid solve
     int mixed_pair = 1;
     int mpair_rows = 16;     // k_mixed_pair: output rows per wave
+    int stage_edge_first = 1;  // pair kernels: waves on the general march dispatched first
     int mp_on = 0;           // this step's mixed iterations run as k_mixed_pair (advance_impl)
     int mp_count = 0;        // k_mixed_pair launches of the step so far
     int mp_spec = 0;         // the Poisson phase follows the first batch unseen: its last finalizer starts it

@@ -32,6 +32,8 @@
 //    dual-time recursion (a1..a4) to first order in J: cr = a4 - a3 a4 c,
 //    cj = a3 a4.  A converged state (r = 0) is a fixed point of both, so the
 //    answer is fixed by the FP64 residual.
+#include <type_traits>
+
 #include "device_common.hpp"
 #include "internal.hpp"
 
@@ -60,6 +62,8 @@ struct StageArgs {
     int rows;   // output rows per wave (chunk height)
     int order;  // wave -> tile: 0 band-major (chunks of one band consecutive), 1 chunk-major
     int alt;    // k_stage64: odd chunks march down (shared halo rows read at the same time)
+    int edge_first;  // pair kernels: waves on the general march dispatched first (pair_wave)
+    int eb0, eb1, ec0, ec1;  // pair kernels: leading / trailing bands and chunks on the general march
     int fast;   // k_stage64: interior waves take the unrolled path (rows 8 or 16)
 };
 
@@ -70,6 +74,54 @@ __device__ __forceinline__ int wave_id(const StageArgs &)
     // wave-uniform by construction; readfirstlane lets the compiler keep the
     // tile coordinates, row offsets and wave-level branches in scalar registers
     return __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * 4 + (int)(threadIdx.x >> 6));
+}
+
+// Wave -> (band, chunk, id) of a pair kernel with the waves on the general
+// march first.  Those waves (every band of the leading ec0 and trailing ec1
+// chunks, the leading eb0 and trailing eb1 bands of the other chunks) take
+// about twice as long as an interior wave on the unrolled march, and alone
+// about as long as the interior waves together; dispatched last they would
+// set the kernel's tail.  They fill the first ceil(E / 4) blocks, which the
+// dispatcher hands round-robin to the 8 XCDs.  The interior waves fill the
+// rest in chunk-major order, each XCD a contiguous range of them (its
+// vertically adjacent chunks share its L2).  id (the norm partial's slot) is
+// e for edge wave e and E + k for interior wave k: dense in [0, nb nc).
+// false: a padding wave of the last edge block or past the last interior one.
+__device__ __forceinline__ bool pair_wave(const StageArgs &a, int &band, int &chunk, int &id)
+{
+    const int nb = a.nbands, nc = a.nchunks;
+    const int nbi = nb - a.eb0 - a.eb1, nci = nc - a.ec0 - a.ec1;
+    const int I = nbi * nci, E = nb * nc - I;
+    const int Be = (E + 3) >> 2;  // blocks of edge waves
+    const int b = blockIdx.x, w = (int)(threadIdx.x >> 6);
+    if (b < Be) {
+        const int e = 4 * b + w;
+        if (e >= E) return false;
+        id = e;
+        const int ne = (a.ec0 + a.ec1) * nb;  // waves of the edge chunks
+        if (e < ne) {
+            const int c = e / nb;
+            band = e - c * nb;
+            chunk = c < a.ec0 ? c : nc - a.ec1 + (c - a.ec0);
+        } else {
+            const int f = e - ne, eb = a.eb0 + a.eb1;
+            const int bb = f % eb;
+            chunk = a.ec0 + f / eb;
+            band = bb < a.eb0 ? bb : nb - a.eb1 + (bb - a.eb0);
+        }
+        return true;
+    }
+    // interior block: XCD x = b mod 8, its rank among that XCD's interior blocks
+    const int B = gridDim.x, q = B >> 3, r = B & 7, x = b & 7;
+    const int qe = Be >> 3, re = Be & 7;
+    const int rank = (b >> 3) - (qe + (x < re ? 1 : 0));
+    const int base = (x * q + min(x, r)) - (x * qe + min(x, re));
+    const int k = 4 * (base + rank) + w;
+    if (k >= I) return false;
+    id = E + k;
+    band = a.eb0 + k % nbi;
+    chunk = a.ec0 + k / nbi;
+    return true;
 }
 
 // cell velocities of row jr for this lane's column from raw face loads of the
@@ -703,10 +755,15 @@ __global__ __launch_bounds__(256) void k_stage_pair(StageArgs a)
 {
     using R = double;
     const int lane = threadIdx.x & 63;
-    const int wid = wave_id(a);
-    if (wid >= a.nbands * a.nchunks) return;  // whole wave
-    const int band = a.order ? wid % a.nbands : wid / a.nchunks;
-    const int chunk = a.order ? wid / a.nbands : wid % a.nchunks;
+    int band, chunk, wid;
+    if (a.edge_first) {
+        if (!pair_wave(a, band, chunk, wid)) return;  // whole wave
+    } else {
+        wid = wave_id(a);
+        if (wid >= a.nbands * a.nchunks) return;  // whole wave
+        band = a.order ? wid % a.nbands : wid / a.nchunks;
+        chunk = a.order ? wid / a.nbands : wid % a.nchunks;
+    }
     const int n = a.n, P = a.P;
     const int i0 = band * WB2, j0 = chunk * a.rows;
     const int i = i0 + lane - 6;
@@ -1070,6 +1127,182 @@ __global__ __launch_bounds__(256) void k_tangent(StageArgs a)
     if (lane == 0) { a.partials[2 * wid] = nx_; a.partials[2 * wid + 1] = ny_; }
 }
 
+// One mixed iteration for a wave whose whole stencil lies inside the domain
+// (no lane or row of A's march, cells j0 - 6 .. j0 + rows + 4, touches a
+// wall, a ghost cell or an array edge).  A runs its rows j0 - 3 .. j0 + rows
+// + 2 (t = 1 .. rows + 6 after a first iteration without output), B its rows
+// j0 - 1 .. j0 + rows - 1 three rows behind (t >= 6).  Every rolling window,
+// delay line and prefetch queue is a ring of K = D + 4 registers indexed by
+// row: the loop body is unrolled K times, so each slot index is a
+// compile-time constant and no value is copied from one slot to the next.
+// Loads run D iterations ahead, every iteration the same (rows past the
+// needed ones are clamped into the arrays), so the memory waits count exactly.
+// The arithmetic per face is k_mixed_pair's (below) with the wall branches
+// dropped.
+// Iterations t = 0 .. tend in blocks of K = sizeof...(Q): full(t, slot
+// constants 0 .. K - 1) while a whole block fits, then part(t, ..) for the
+// rest.  The loop holds only whole blocks, so its back edge joins no early
+// exit and the ring registers need no copies there.
+template <class Full, class Part, int... Q>
+__device__ __forceinline__ void ring_loop(Full &full, Part &part, int tend, std::integer_sequence<int, Q...>)
+{
+    constexpr int K = (int)sizeof...(Q);
+    int t = 0;
+    for (; t + K - 1 <= tend; t += K) full(t, std::integral_constant<int, Q>{}...);
+    if (t <= tend) part(t, std::integral_constant<int, Q>{}...);
+}
+
+template <int D>
+__device__ __forceinline__ void mixed_interior(const StageArgs &a, int lane, int i, int j0, int wid)
+{
+    using R = double;
+    using F = float;
+    constexpr int K = D + 4;
+    const size_t P = a.P;
+    const int n = a.n, rows = a.rows;
+    const bool outB = lane >= 6 && lane <= 57;
+    const unsigned ob = (unsigned)i * 8u;
+    auto at = [&](const double *base, int row) -> const double & {
+        return *(const double *)((const char *)(base + P * row) + ob);
+    };
+    auto put = [&](double *base, int row, double val) { *(double *)((char *)(base + P * row) + ob) = val; };
+    const R idx = a.inv_dx, idy = a.inv_dy, idx2 = a.inv_dx2, idy2 = a.inv_dy2, iren = a.inv_ren, c15 = a.c15;
+    const F fidx = (F)a.inv_dx, fidy = (F)a.inv_dy, fidx2 = (F)a.inv_dx2, fidy2 = (F)a.inv_dy2, firen = (F)a.inv_ren;
+    const double cr = a.cr, cj = a.cj;
+    const double *us = a.us, *vs = a.vs;
+    auto U_of = [&](R u) { return R(0.5) * (u + lane_next(u)); };
+    // Ring slot of row r: (r - (j0 - 4)) mod K, i.e. at iteration t (A's row
+    // jc = j0 - 4 + t) row jc + k sits in slot (t + k) mod K.
+    const int jA = j0 - 3;  // A's first output row
+    R aU[K], aV[K];         // A's cells jc - 2 .. jc + 2
+    R ru[K], rv[K];         // u^k faces: x rows jc .. jc + 2 + D, y rows jc .. jc + 3 + D
+    R rcu[K], rcv[K];       // the constant, rows jc .. jc + D
+    F xb[K], xr[K], yb[K], yr[K];  // A's output faces (FP32 base, FP32 residual), rows jc - 3 .. jc
+    F wU[K], wV[K], dU[K], dV[K];  // B's cells (base, residual), rows jc - 5 .. jc - 1
+    R obu[K], obv[K];              // the FP64 base faces of B's output rows jb .. jb + D
+    {   // cells jA - 3 .. jA (slots K - 2, K - 1, 0, 1)
+        const R u0 = at(us, jA - 3), u1 = at(us, jA - 2);
+        const R v0 = at(vs, jA - 3), v1 = at(vs, jA - 2);
+#pragma unroll
+        for (int q = 0; q <= D + 1; ++q) ru[q] = at(us, jA - 1 + q);
+#pragma unroll
+        for (int q = 0; q <= D + 2; ++q) rv[q] = at(vs, jA - 1 + q);
+#pragma unroll
+        for (int q = 0; q < D; ++q) { rcu[q] = at(a.cu, jA - 1 + q); rcv[q] = at(a.cv, jA - 1 + q); }
+        aU[K - 2] = U_of(u0); aV[K - 2] = R(0.5) * (v0 + v1);
+        aU[K - 1] = U_of(u1); aV[K - 1] = R(0.5) * (v1 + rv[0]);
+        aU[0] = U_of(ru[0]); aV[0] = R(0.5) * (rv[0] + rv[1]);
+        aU[1] = U_of(ru[1]); aV[1] = R(0.5) * (rv[1] + rv[2]);
+    }
+    R afsU = quick_flux<R>(rv[0], aU[K - 2], aU[K - 1], aU[0], aU[1]);
+    R afsV = quick_flux<R>(rv[0], aV[K - 2], aV[K - 1], aV[0], aV[1]);
+    R atV = 0;
+    F fsU = 0, fsV = 0, tVprev = 0;
+    double nx_ = 0.0, ny_ = 0.0;
+    const int tend = rows + 6;  // the last iteration
+    auto iter = [&](auto M, const int t) {
+        constexpr int m = decltype(M)::value;
+        auto sl = [](int k) constexpr { return ((m + k) % K + K) % K; };
+        constexpr int a0 = sl(0), a1 = sl(1), a2 = sl(2), a3 = sl(3), am1 = sl(-1), am2 = sl(-2);
+        constexpr int b3 = sl(-3), b4 = sl(-4), b5 = sl(-5);
+        __builtin_amdgcn_sched_barrier(0);
+        const int jc = jA - 1 + t;  // A's row
+        const int jb = jc - 3;      // B's row (t >= 6)
+        {   // the loads of iteration t + D (rows clamped into the arrays past the last one needed)
+            ru[sl(2 + D)] = at(us, min(jc + 2 + D, n - 1));
+            rv[sl(3 + D)] = at(vs, min(jc + 3 + D, n));
+            rcu[sl(D)] = at(a.cu, min(jc + D, n - 1));
+            rcv[sl(D)] = at(a.cv, min(jc + D, n));
+            obu[sl(D - 3)] = at(us, max(jb + D, 0));
+            obv[sl(D - 3)] = at(vs, max(jb + D, 0));
+        }
+        // A, row jc: the FP64 residual of its two faces
+        aU[a2] = U_of(ru[a2]);
+        aV[a2] = R(0.5) * (rv[a2] + rv[a3]);
+        {
+            const R fnU = quick_flux<R>(rv[a1], aU[am1], aU[a0], aU[a1], aU[a2]);
+            const R fnV = quick_flux<R>(rv[a1], aV[am1], aV[a0], aV[a1], aV[a2]);
+            const R uw = ru[a0], vw = rv[a0];
+            const R Uc = aU[a0], Vc = aV[a0];
+            const R Um1 = lane_prev(Uc), Um2 = lane_prev(Um1), Up1 = lane_next(Uc);
+            const R Vm1 = lane_prev(Vc), Vm2 = lane_prev(Vm1), Vp1 = lane_next(Vc);
+            const R fwU = quick_flux<R>(uw, Um2, Um1, Uc, Up1), fwV = quick_flux<R>(uw, Vm2, Vm1, Vc, Vp1);
+            const R feU = lane_next(fwU), feV = lane_next(fwV);
+            const R CU = (feU - fwU) * idx + (fnU - afsU) * idy;
+            const R CV = (feV - fwV) * idx + (fnV - afsV) * idy;
+            const R VU = ((Up1 - R(2) * Uc + Um1) * idx2 + (aU[a1] - R(2) * Uc + aU[am1]) * idy2) * iren;
+            const R VV = ((Vp1 - R(2) * Vc + Vm1) * idx2 + (aV[a1] - R(2) * Vc + aV[am1]) * idy2) * iren;
+            const R TU = (R(0) - CU) + VU, TV = (R(0) - CV) + VV;
+            const R TUw = lane_prev(TU);
+            const double Rx = R(0.5) * (TUw + TU);
+            const double Ry = R(0.5) * (atV + TV);
+            xb[a0] = (float)uw;
+            xr[a0] = (float)(Rx - c15 * uw + rcu[a0]);
+            yb[a0] = (float)vw;
+            yr[a0] = (float)(Ry - c15 * vw + rcv[a0]);
+            afsU = fnU; afsV = fnV; atV = TV;
+        }
+        __builtin_amdgcn_sched_barrier(0);  // A's temporaries end here
+        // B: cell row jc - 1 from A's faces (x row jc - 1, y rows jc - 1, jc)
+        if (t >= 2) {
+            wU[am1] = F(0.5) * (xb[am1] + lane_next(xb[am1]));
+            wV[am1] = F(0.5) * (yb[am1] + yb[a0]);
+            dU[am1] = F(0.5) * (xr[am1] + lane_next(xr[am1]));
+            dV[am1] = F(0.5) * (yr[am1] + yr[a0]);
+        }
+        if (t >= 6) {  // B's row jb: faces rows jb .. jb + 3 in slots b3, am2, am1, a0; cells jb - 2 .. jb + 2 in b5 .. am1
+            if (t == 6) {  // the flux through y-face jb
+                fsU = quick_tl(yb[b3], yr[b3], wU[b5], wU[b4], wU[b3], wU[am2], dU[b5], dU[b4], dU[b3], dU[am2]);
+                fsV = quick_tl(yb[b3], yr[b3], wV[b5], wV[b4], wV[b3], wV[am2], dV[b5], dV[b4], dV[b3], dV[am2]);
+            }
+            const F fnU = quick_tl(yb[am2], yr[am2], wU[b4], wU[b3], wU[am2], wU[am1], dU[b4], dU[b3], dU[am2], dU[am1]);
+            const F fnV = quick_tl(yb[am2], yr[am2], wV[b4], wV[b3], wV[am2], wV[am1], dV[b4], dV[b3], dV[am2], dV[am1]);
+            const F Uc = wU[b3], Vc = wV[b3], dUc = dU[b3], dVc = dV[b3];
+            const F Um1 = lane_prev(Uc), Um2 = lane_prev(Um1), Up1 = lane_next(Uc);
+            const F Vm1 = lane_prev(Vc), Vm2 = lane_prev(Vm1), Vp1 = lane_next(Vc);
+            const F dUm1 = lane_prev(dUc), dUm2 = lane_prev(dUm1), dUp1 = lane_next(dUc);
+            const F dVm1 = lane_prev(dVc), dVm2 = lane_prev(dVm1), dVp1 = lane_next(dVc);
+            const F fwU = quick_tl(xb[b3], xr[b3], Um2, Um1, Uc, Up1, dUm2, dUm1, dUc, dUp1);
+            const F fwV = quick_tl(xb[b3], xr[b3], Vm2, Vm1, Vc, Vp1, dVm2, dVm1, dVc, dVp1);
+            const F feU = lane_next(fwU), feV = lane_next(fwV);
+            const F CU = (feU - fwU) * fidx + (fnU - fsU) * fidy;
+            const F CV = (feV - fwV) * fidx + (fnV - fsV) * fidy;
+            const F VU = ((dUp1 - F(2) * dUc + dUm1) * fidx2 + (dU[am2] - F(2) * dUc + dU[b4]) * fidy2) * firen;
+            const F VV = ((dVp1 - F(2) * dVc + dVm1) * fidx2 + (dV[am2] - F(2) * dVc + dV[b4]) * fidy2) * firen;
+            const F TU = (F(0) - CU) + VU, TV = (F(0) - CV) + VV;
+            const F TUw = lane_prev(TU);
+            if (t >= 7 && outB) {  // B's row jb >= j0: no wall face in this wave
+                const float Jx = 0.5f * (TUw + TU);
+                const double du = cr * (double)xr[b3] + cj * (double)Jx;
+                put(a.uo, jb, obu[b3] + du);
+                nx_ += fabs(du * du * 1.0);
+                const float Jy = 0.5f * (tVprev + TV);
+                const double dv = cr * (double)yr[b3] + cj * (double)Jy;
+                put(a.vo, jb, obv[b3] + dv);
+                ny_ += fabs(dv * dv * 1.0);
+            }
+            fsU = fnU; fsV = fnV; tVprev = TV;
+        }
+    };
+    auto full = [&](const int t, auto... Q) { (iter(Q, t + decltype(Q)::value), ...); };
+    auto part = [&](const int t, auto... Q) {  // iterations t .. tend (fewer than K)
+        bool go = true;
+        ((go = go && t + decltype(Q)::value <= tend, go ? (iter(Q, t + decltype(Q)::value), 0) : 0), ...);
+    };
+    ring_loop(full, part, tend, std::make_integer_sequence<int, K>{});
+    double sb = 0.0;  // no wall face in this wave
+    for (int o = 32; o > 0; o >>= 1) {
+        nx_ += __shfl_down(nx_, o, 64);
+        ny_ += __shfl_down(ny_, o, 64);
+        sb += __shfl_down(sb, o, 64);
+    }
+    if (lane == 0) {
+        a.partials[3 * wid] = nx_;
+        a.partials[3 * wid + 1] = ny_;
+        a.partials[3 * wid + 2] = sb;
+    }
+}
+
 // ======================================= one mixed iteration in one pass
 // The mixed momentum iteration (k_stage64 mode 3, then k_tangent) in one
 // launch.  The layout is k_stage_pair's: a wave owns 52 output columns with a
@@ -1095,10 +1328,15 @@ __global__ __launch_bounds__(256) void k_mixed_pair(StageArgs a)
     using R = double;
     using F = float;
     const int lane = threadIdx.x & 63;
-    const int wid = wave_id(a);
-    if (wid >= a.nbands * a.nchunks) return;  // whole wave
-    const int band = a.order ? wid % a.nbands : wid / a.nchunks;
-    const int chunk = a.order ? wid / a.nbands : wid % a.nchunks;
+    int band, chunk, wid;
+    if (a.edge_first) {
+        if (!pair_wave(a, band, chunk, wid)) return;  // whole wave
+    } else {
+        wid = wave_id(a);
+        if (wid >= a.nbands * a.nchunks) return;  // whole wave
+        band = a.order ? wid % a.nbands : wid / a.nchunks;
+        chunk = a.order ? wid / a.nbands : wid % a.nchunks;
+    }
     const int n = a.n, P = a.P;
     const int i0 = band * WB2, j0 = chunk * a.rows;
     const int i = i0 + lane - 6;
@@ -1119,6 +1357,10 @@ __global__ __launch_bounds__(256) void k_mixed_pair(StageArgs a)
         return;
     }
     const bool edge = i0 < 6 || i0 + 58 > n;
+    if (!edge && j0 >= 6 && j0 + a.rows + 5 <= n && a.fast) {  // the whole stencil inside the domain
+        mixed_interior<2>(a, lane, i, j0, wid);
+        return;
+    }
     const R gx = (xin && a.lid) ? a.lid[i] : a.inflow0;
     const F gxf = (F)((xin && a.lid) ? a.lid[i] : a.inflow0);
     const R idx = a.inv_dx, idy = a.inv_dy, idx2 = a.inv_dx2, idy2 = a.inv_dy2, iren = a.inv_ren;
@@ -1363,6 +1605,7 @@ static StageArgs base_args(const Ctx &c, double rk, int mode, int rows)
     a.rows = rows;
     a.order = c.stage_order;
     a.alt = c.stage_alt;
+    a.edge_first = c.stage_edge_first;
     a.fast = c.stage_fast;
     a.nbands = (g.n + 1 + WB - 1) / WB;
     a.nchunks = (g.n + a.rows - 1) / a.rows;
@@ -1372,6 +1615,27 @@ static StageArgs base_args(const Ctx &c, double rk, int mode, int rows)
 static dim3 stage_grid(StageArgs &a)
 {
     return dim3((a.nbands * a.nchunks + 3) / 4);
+}
+
+// The pair kernels' edge-first wave order (pair_wave): the bands and chunks
+// whose waves leave the interior path's stencil box (cells j0 - 6 .. j0 + rows
+// + 4 and columns i0 - 6 .. i0 + 57 inside the domain).  Without an interior
+// path (interior false) every wave counts as an edge wave.  Returns the grid.
+static dim3 pair_grid(const Ctx &c, StageArgs &a, bool interior)
+{
+    const int n = c.g.n, nb = a.nbands, nc = a.nchunks;
+    auto band_in = [&](int b) { return b * WB2 >= 6 && b * WB2 + 58 <= n; };
+    auto chunk_in = [&](int ch) { return ch * a.rows >= 6 && ch * a.rows + a.rows + 5 <= n; };
+    int b0 = 0, b1 = nb, c0 = 0, c1 = nc;
+    while (b0 < nb && !band_in(b0)) ++b0;
+    while (b1 > b0 && !band_in(b1 - 1)) --b1;
+    while (c0 < nc && !chunk_in(c0)) ++c0;
+    while (c1 > c0 && !chunk_in(c1 - 1)) --c1;
+    if (!interior || b0 >= b1 || c0 >= c1) { b0 = nb; b1 = nb; c0 = nc; c1 = nc; }
+    a.eb0 = b0; a.eb1 = nb - b1; a.ec0 = c0; a.ec1 = nc - c1;
+    if (!a.edge_first) return stage_grid(a);
+    const int I = (b1 - b0) * (c1 - c0), E = nb * nc - I;
+    return dim3((E + 3) / 4 + (I + 3) / 4);
 }
 
 
@@ -1421,7 +1685,7 @@ hipError_t launch_stage_pair(const Ctx &c, const StageLaunch &s, int *nblocks)
     if (s.unguarded) a.ctl = nullptr;
     a.nbands = (c.g.n + 1 + WB2 - 1) / WB2;
     a.nchunks = (c.g.n + a.rows - 1) / a.rows;
-    const dim3 grid = stage_grid(a);
+    const dim3 grid = pair_grid(c, a, a.fast && (a.rows == 16 || a.rows == 12));
     if (nblocks) *nblocks = a.nbands * a.nchunks;  // one norm partial per wave
     if (s.pair == 2) hipLaunchKernelGGL(k_stage_pair<true>, grid, dim3(256), 0, c.stream, a);
     else hipLaunchKernelGGL(k_stage_pair<false>, grid, dim3(256), 0, c.stream, a);
@@ -1445,7 +1709,7 @@ hipError_t launch_mixed_pair(const Ctx &c, const MixedPairLaunch &s, int *nblock
     if (s.unguarded) a.ctl = nullptr;
     a.nbands = (c.g.n + 1 + WB2 - 1) / WB2;
     a.nchunks = (c.g.n + a.rows - 1) / a.rows;
-    const dim3 grid = stage_grid(a);
+    const dim3 grid = pair_grid(c, a, a.fast);
     if (nblocks) *nblocks = a.nbands * a.nchunks;  // one norm partial per wave
     hipLaunchKernelGGL(k_mixed_pair, grid, dim3(256), 0, c.stream, a);
     return hipGetLastError();

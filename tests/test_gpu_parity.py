@@ -358,6 +358,36 @@ def test_one_pass_mixed_iteration_matches_two_launches(n, rows, init, bc, lid):
         assert rel(pb - pb.mean(), pa - pa.mean()) < 1e-12
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("n,rows", [(300, 16), (257, 12), (130, 17)])
+def test_edge_first_wave_order_gives_the_same_steps(precision, n, rows):
+    """The pair kernels dispatch the waves on the general march (at a wall or an
+    array edge) first and the interior waves after them, each XCD a contiguous
+    range; the other order is the plain chunk-major one.  Only the order of the
+    norm partials' sum differs: the same pseudo-iteration and V-cycle counts,
+    the same fields to rounding, for sizes and chunk heights that leave
+    partial bands and chunks (and, for 17-row chunks in the FP64 step, no
+    unrolled interior path: every wave an edge wave)."""
+    api = _api()
+
+    def run():
+        inp = api.cavity_inputs(n, 400.0, precision=precision)
+        r = api.Run.from_numpy(inp, W.initial_state(n, "seeded"))
+        st = r.advance(4)
+        return r.fetch(), st
+
+    rk = "AMRESSIF_MPAIR_ROWS" if precision == 1 else "AMRESSIF_PAIR_ROWS"
+    fa, sa = _with_env({"AMRESSIF_EDGE_FIRST": "0", rk: str(rows)}, run)
+    fb, sb = _with_env({"AMRESSIF_EDGE_FIRST": "1", rk: str(rows)}, run)
+    assert [s["pseudo_iters"] for s in sb] == [s["pseudo_iters"] for s in sa]
+    assert [s["vcycles"] for s in sb] == [s["vcycles"] for s in sa]
+    assert all(s["flags"] == 0 for s in sb)
+    for k in ("u", "v", "uprev", "vprev"):
+        assert rel(fb[k], fa[k]) < 1e-12, k
+    pa, pb = fa["ctx"][0, 1:-1, 1:-1], fb["ctx"][0, 1:-1, 1:-1]
+    assert rel(pb - pb.mean(), pa - pa.mean()) < 1e-12
+
+
 @pytest.mark.parametrize("n", [130, 98])
 def test_odd_bottom_poisson_solve_matches_the_exact_solve(n):
     """n = 130 coarsens to an odd 65 x 65 bottom (98 to 49 x 49), above the

@@ -1127,18 +1127,22 @@ __global__ __launch_bounds__(256) void k_tangent(StageArgs a)
     if (lane == 0) { a.partials[2 * wid] = nx_; a.partials[2 * wid + 1] = ny_; }
 }
 
-// One mixed iteration for a wave whose whole stencil lies inside the domain
-// (no lane or row of A's march, cells j0 - 6 .. j0 + rows + 4, touches a
-// wall, a ghost cell or an array edge).  A runs its rows j0 - 3 .. j0 + rows
-// + 2 (t = 1 .. rows + 6 after a first iteration without output), B its rows
-// j0 - 1 .. j0 + rows - 1 three rows behind (t >= 6).  Every rolling window,
-// delay line and prefetch queue is a ring of K = D + 4 registers indexed by
-// row: the loop body is unrolled K times, so each slot index is a
-// compile-time constant and no value is copied from one slot to the next.
-// Loads run D iterations ahead, every iteration the same (rows past the
-// needed ones are clamped into the arrays), so the memory waits count exactly.
-// The arithmetic per face is k_mixed_pair's (below) with the wall branches
-// dropped.
+// One mixed iteration for a wave whose rows lie inside the domain (cells
+// j0 - 6 .. j0 + rows + 4: no row of A's march touches the south or north
+// wall or its ghost rows).  A runs its rows j0 - 3 .. j0 + rows + 2 (t = 1 ..
+// rows + 6 after a first iteration without output), B its rows j0 - 1 .. j0 +
+// rows - 1 three rows behind (t >= 6).  Every rolling window, delay line and
+// prefetch queue is a ring of K = D + 4 registers indexed by row: the loop
+// body is unrolled K times, so each slot index is a compile-time constant and
+// no value is copied from one slot to the next.  Loads run D iterations
+// ahead, every iteration the same (rows past the needed ones are clamped into
+// the arrays), so the memory waits count exactly.  The arithmetic per face is
+// k_mixed_pair's (below).  XE false: no lane's stencil reaches a side wall
+// (columns i0 - 6 .. i0 + 57 inside the domain), so the wall branches drop
+// out.  XE true: a band at the west or east wall; its cells take the side
+// walls' ghost rules from their mirror lanes (finish_cells), the wall x-faces
+// have R = 0 and half weight in the norm, the loads' columns are clamped into
+// the arrays, and only faces inside the domain are stored.
 // Iterations t = 0 .. tend in blocks of K = sizeof...(Q): full(t, slot
 // constants 0 .. K - 1) while a whole block fits, then part(t, ..) for the
 // rest.  The loop holds only whole blocks, so its back edge joins no early
@@ -1152,7 +1156,7 @@ __device__ __forceinline__ void ring_loop(Full &full, Part &part, int tend, std:
     if (t <= tend) part(t, std::integral_constant<int, Q>{}...);
 }
 
-template <int D>
+template <int D, bool XE>
 __device__ __forceinline__ void mixed_interior(const StageArgs &a, int lane, int i, int j0, int wid)
 {
     using R = double;
@@ -1161,16 +1165,34 @@ __device__ __forceinline__ void mixed_interior(const StageArgs &a, int lane, int
     const size_t P = a.P;
     const int n = a.n, rows = a.rows;
     const bool outB = lane >= 6 && lane <= 57;
-    const unsigned ob = (unsigned)i * 8u;
-    auto at = [&](const double *base, int row) -> const double & {
-        return *(const double *)((const char *)(base + P * row) + ob);
+    const bool xin = !XE || (i >= 0 && i < n), xface = !XE || (i >= 0 && i <= n);
+    const bool xwall = XE && (i == 0 || i == n);
+    // the lane's column in the x-face and y-face arrays (clamped on the ghost lanes of a wall band)
+    const unsigned obx = (unsigned)(XE ? min(max(i, 0), n) : i) * 8u;
+    const unsigned oby = (unsigned)(XE ? min(max(i, 0), n - 1) : i) * 8u;
+    auto atx = [&](const double *base, int row) -> const double & {
+        return *(const double *)((const char *)(base + P * row) + obx);
     };
-    auto put = [&](double *base, int row, double val) { *(double *)((char *)(base + P * row) + ob) = val; };
+    auto aty = [&](const double *base, int row) -> const double & {
+        return *(const double *)((const char *)(base + P * row) + oby);
+    };
+    auto put = [&](double *base, int row, double val) {
+        *(double *)((char *)(base + P * row) + (unsigned)i * 8u) = val;
+    };
     const R idx = a.inv_dx, idy = a.inv_dy, idx2 = a.inv_dx2, idy2 = a.inv_dy2, iren = a.inv_ren, c15 = a.c15;
     const F fidx = (F)a.inv_dx, fidy = (F)a.inv_dy, fidx2 = (F)a.inv_dx2, fidy2 = (F)a.inv_dy2, firen = (F)a.inv_ren;
     const double cr = a.cr, cj = a.cj;
     const double *us = a.us, *vs = a.vs;
-    auto U_of = [&](R u) { return R(0.5) * (u + lane_next(u)); };
+    // cells of row jr from its x-face u and its y-faces v0 (row jr), v1 (row jr + 1)
+    auto cellA = [&](int jr, R u, R v0, R v1, R &U, R &V) {
+        if constexpr (XE) {
+            finish_cells<R, false>(a, lane, i, jr, true, u, lane_next(u), v0, v1, R(0), U, V);
+        } else {
+            (void)jr;
+            U = R(0.5) * (u + lane_next(u));
+            V = R(0.5) * (v0 + v1);
+        }
+    };
     // Ring slot of row r: (r - (j0 - 4)) mod K, i.e. at iteration t (A's row
     // jc = j0 - 4 + t) row jc + k sits in slot (t + k) mod K.
     const int jA = j0 - 3;  // A's first output row
@@ -1181,24 +1203,24 @@ __device__ __forceinline__ void mixed_interior(const StageArgs &a, int lane, int
     F wU[K], wV[K], dU[K], dV[K];  // B's cells (base, residual), rows jc - 5 .. jc - 1
     R obu[K], obv[K];              // the FP64 base faces of B's output rows jb .. jb + D
     {   // cells jA - 3 .. jA (slots K - 2, K - 1, 0, 1)
-        const R u0 = at(us, jA - 3), u1 = at(us, jA - 2);
-        const R v0 = at(vs, jA - 3), v1 = at(vs, jA - 2);
+        const R u0 = atx(us, jA - 3), u1 = atx(us, jA - 2);
+        const R v0 = aty(vs, jA - 3), v1 = aty(vs, jA - 2);
 #pragma unroll
-        for (int q = 0; q <= D + 1; ++q) ru[q] = at(us, jA - 1 + q);
+        for (int q = 0; q <= D + 1; ++q) ru[q] = atx(us, jA - 1 + q);
 #pragma unroll
-        for (int q = 0; q <= D + 2; ++q) rv[q] = at(vs, jA - 1 + q);
+        for (int q = 0; q <= D + 2; ++q) rv[q] = aty(vs, jA - 1 + q);
 #pragma unroll
-        for (int q = 0; q < D; ++q) { rcu[q] = at(a.cu, jA - 1 + q); rcv[q] = at(a.cv, jA - 1 + q); }
-        aU[K - 2] = U_of(u0); aV[K - 2] = R(0.5) * (v0 + v1);
-        aU[K - 1] = U_of(u1); aV[K - 1] = R(0.5) * (v1 + rv[0]);
-        aU[0] = U_of(ru[0]); aV[0] = R(0.5) * (rv[0] + rv[1]);
-        aU[1] = U_of(ru[1]); aV[1] = R(0.5) * (rv[1] + rv[2]);
+        for (int q = 0; q < D; ++q) { rcu[q] = atx(a.cu, jA - 1 + q); rcv[q] = aty(a.cv, jA - 1 + q); }
+        cellA(jA - 3, u0, v0, v1, aU[K - 2], aV[K - 2]);
+        cellA(jA - 2, u1, v1, rv[0], aU[K - 1], aV[K - 1]);
+        cellA(jA - 1, ru[0], rv[0], rv[1], aU[0], aV[0]);
+        cellA(jA, ru[1], rv[1], rv[2], aU[1], aV[1]);
     }
     R afsU = quick_flux<R>(rv[0], aU[K - 2], aU[K - 1], aU[0], aU[1]);
     R afsV = quick_flux<R>(rv[0], aV[K - 2], aV[K - 1], aV[0], aV[1]);
     R atV = 0;
     F fsU = 0, fsV = 0, tVprev = 0;
-    double nx_ = 0.0, ny_ = 0.0;
+    double nx_ = 0.0, ny_ = 0.0, sb = 0.0;
     const int tend = rows + 6;  // the last iteration
     auto iter = [&](auto M, const int t) {
         constexpr int m = decltype(M)::value;
@@ -1209,16 +1231,15 @@ __device__ __forceinline__ void mixed_interior(const StageArgs &a, int lane, int
         const int jc = jA - 1 + t;  // A's row
         const int jb = jc - 3;      // B's row (t >= 6)
         {   // the loads of iteration t + D (rows clamped into the arrays past the last one needed)
-            ru[sl(2 + D)] = at(us, min(jc + 2 + D, n - 1));
-            rv[sl(3 + D)] = at(vs, min(jc + 3 + D, n));
-            rcu[sl(D)] = at(a.cu, min(jc + D, n - 1));
-            rcv[sl(D)] = at(a.cv, min(jc + D, n));
-            obu[sl(D - 3)] = at(us, max(jb + D, 0));
-            obv[sl(D - 3)] = at(vs, max(jb + D, 0));
+            ru[sl(2 + D)] = atx(us, min(jc + 2 + D, n - 1));
+            rv[sl(3 + D)] = aty(vs, min(jc + 3 + D, n));
+            rcu[sl(D)] = atx(a.cu, min(jc + D, n - 1));
+            rcv[sl(D)] = aty(a.cv, min(jc + D, n));
+            obu[sl(D - 3)] = atx(us, max(jb + D, 0));
+            obv[sl(D - 3)] = aty(vs, max(jb + D, 0));
         }
         // A, row jc: the FP64 residual of its two faces
-        aU[a2] = U_of(ru[a2]);
-        aV[a2] = R(0.5) * (rv[a2] + rv[a3]);
+        cellA(jc + 2, ru[a2], rv[a2], rv[a3], aU[a2], aV[a2]);
         {
             const R fnU = quick_flux<R>(rv[a1], aU[am1], aU[a0], aU[a1], aU[a2]);
             const R fnV = quick_flux<R>(rv[a1], aV[am1], aV[a0], aV[a1], aV[a2]);
@@ -1234,21 +1255,28 @@ __device__ __forceinline__ void mixed_interior(const StageArgs &a, int lane, int
             const R VV = ((Vp1 - R(2) * Vc + Vm1) * idx2 + (aV[a1] - R(2) * Vc + aV[am1]) * idy2) * iren;
             const R TU = (R(0) - CU) + VU, TV = (R(0) - CV) + VV;
             const R TUw = lane_prev(TU);
-            const double Rx = R(0.5) * (TUw + TU);
+            const double Rx = xwall ? 0.0 : R(0.5) * (TUw + TU);
             const double Ry = R(0.5) * (atV + TV);
-            xb[a0] = (float)uw;
-            xr[a0] = (float)(Rx - c15 * uw + rcu[a0]);
-            yb[a0] = (float)vw;
-            yr[a0] = (float)(Ry - c15 * vw + rcv[a0]);
+            xb[a0] = xface ? (float)uw : 0.f;
+            xr[a0] = xface ? (float)(Rx - c15 * uw + rcu[a0]) : 0.f;
+            yb[a0] = xin ? (float)vw : 0.f;
+            yr[a0] = xin ? (float)(Ry - c15 * vw + rcv[a0]) : 0.f;
             afsU = fnU; afsV = fnV; atV = TV;
         }
         __builtin_amdgcn_sched_barrier(0);  // A's temporaries end here
         // B: cell row jc - 1 from A's faces (x row jc - 1, y rows jc - 1, jc)
         if (t >= 2) {
-            wU[am1] = F(0.5) * (xb[am1] + lane_next(xb[am1]));
-            wV[am1] = F(0.5) * (yb[am1] + yb[a0]);
-            dU[am1] = F(0.5) * (xr[am1] + lane_next(xr[am1]));
-            dV[am1] = F(0.5) * (yr[am1] + yr[a0]);
+            if constexpr (XE) {
+                finish_cells<F, false>(a, lane, i, jc - 1, true, xb[am1], lane_next(xb[am1]), yb[am1], yb[a0], F(0),
+                                       wU[am1], wV[am1]);
+                finish_cells<F, true>(a, lane, i, jc - 1, true, xr[am1], lane_next(xr[am1]), yr[am1], yr[a0], F(0),
+                                      dU[am1], dV[am1]);
+            } else {
+                wU[am1] = F(0.5) * (xb[am1] + lane_next(xb[am1]));
+                wV[am1] = F(0.5) * (yb[am1] + yb[a0]);
+                dU[am1] = F(0.5) * (xr[am1] + lane_next(xr[am1]));
+                dV[am1] = F(0.5) * (yr[am1] + yr[a0]);
+            }
         }
         if (t >= 6) {  // B's row jb: faces rows jb .. jb + 3 in slots b3, am2, am1, a0; cells jb - 2 .. jb + 2 in b5 .. am1
             if (t == 6) {  // the flux through y-face jb
@@ -1271,15 +1299,25 @@ __device__ __forceinline__ void mixed_interior(const StageArgs &a, int lane, int
             const F VV = ((dVp1 - F(2) * dVc + dVm1) * fidx2 + (dV[am2] - F(2) * dVc + dV[b4]) * fidy2) * firen;
             const F TU = (F(0) - CU) + VU, TV = (F(0) - CV) + VV;
             const F TUw = lane_prev(TU);
-            if (t >= 7 && outB) {  // B's row jb >= j0: no wall face in this wave
-                const float Jx = 0.5f * (TUw + TU);
-                const double du = cr * (double)xr[b3] + cj * (double)Jx;
-                put(a.uo, jb, obu[b3] + du);
-                nx_ += fabs(du * du * 1.0);
-                const float Jy = 0.5f * (tVprev + TV);
-                const double dv = cr * (double)yr[b3] + cj * (double)Jy;
-                put(a.vo, jb, obv[b3] + dv);
-                ny_ += fabs(dv * dv * 1.0);
+            if (t >= 7 && outB) {  // B's row jb >= j0: no wall y-face in this wave
+                if (xface) {
+                    const float Jx = xwall ? 0.f : 0.5f * (TUw + TU);
+                    const double du = cr * (double)xr[b3] + cj * (double)Jx;
+                    const double unew = obu[b3] + du;
+                    put(a.uo, jb, unew);
+                    if constexpr (XE) {
+                        nx_ += du * du * (xwall ? 0.5 : 1.0);
+                        if (xwall) sb += i == n ? unew * a.inv_dx : -(unew * a.inv_dx);
+                    } else {
+                        nx_ += fabs(du * du * 1.0);
+                    }
+                }
+                if (xin) {
+                    const float Jy = 0.5f * (tVprev + TV);
+                    const double dv = cr * (double)yr[b3] + cj * (double)Jy;
+                    put(a.vo, jb, obv[b3] + dv);
+                    ny_ += fabs(dv * dv * 1.0);
+                }
             }
             fsU = fnU; fsV = fnV; tVprev = TV;
         }
@@ -1290,13 +1328,12 @@ __device__ __forceinline__ void mixed_interior(const StageArgs &a, int lane, int
         ((go = go && t + decltype(Q)::value <= tend, go ? (iter(Q, t + decltype(Q)::value), 0) : 0), ...);
     };
     ring_loop(full, part, tend, std::make_integer_sequence<int, K>{});
-    double sb = 0.0;  // no wall face in this wave
     for (int o = 32; o > 0; o >>= 1) {
         nx_ += __shfl_down(nx_, o, 64);
         ny_ += __shfl_down(ny_, o, 64);
         sb += __shfl_down(sb, o, 64);
     }
-    if (lane == 0) {
+    if (lane == 0) {  // records of three: the norm partials and the wave's boundary flux of u^{k+1}
         a.partials[3 * wid] = nx_;
         a.partials[3 * wid + 1] = ny_;
         a.partials[3 * wid + 2] = sb;
@@ -1357,8 +1394,9 @@ __global__ __launch_bounds__(256) void k_mixed_pair(StageArgs a)
         return;
     }
     const bool edge = i0 < 6 || i0 + 58 > n;
-    if (!edge && j0 >= 6 && j0 + a.rows + 5 <= n && a.fast) {  // the whole stencil inside the domain
-        mixed_interior<2>(a, lane, i, j0, wid);
+    if (j0 >= 6 && j0 + a.rows + 5 <= n && a.fast) {  // no row of the stencil at a wall: the ring loop
+        if (edge) mixed_interior<2, true>(a, lane, i, j0, wid);
+        else mixed_interior<2, false>(a, lane, i, j0, wid);
         return;
     }
     const R gx = (xin && a.lid) ? a.lid[i] : a.inflow0;
@@ -1619,12 +1657,14 @@ static dim3 stage_grid(StageArgs &a)
 
 // The pair kernels' edge-first wave order (pair_wave): the bands and chunks
 // whose waves leave the interior path's stencil box (cells j0 - 6 .. j0 + rows
-// + 4 and columns i0 - 6 .. i0 + 57 inside the domain).  Without an interior
-// path (interior false) every wave counts as an edge wave.  Returns the grid.
-static dim3 pair_grid(const Ctx &c, StageArgs &a, bool interior)
+// + 4 and columns i0 - 6 .. i0 + 57 inside the domain; with sides true the
+// interior path also takes the bands at the side walls, so only rows count).
+// Without an interior path (interior false) every wave counts as an edge
+// wave.  Returns the grid.
+static dim3 pair_grid(const Ctx &c, StageArgs &a, bool interior, bool sides)
 {
     const int n = c.g.n, nb = a.nbands, nc = a.nchunks;
-    auto band_in = [&](int b) { return b * WB2 >= 6 && b * WB2 + 58 <= n; };
+    auto band_in = [&](int b) { return sides || (b * WB2 >= 6 && b * WB2 + 58 <= n); };
     auto chunk_in = [&](int ch) { return ch * a.rows >= 6 && ch * a.rows + a.rows + 5 <= n; };
     int b0 = 0, b1 = nb, c0 = 0, c1 = nc;
     while (b0 < nb && !band_in(b0)) ++b0;
@@ -1685,7 +1725,7 @@ hipError_t launch_stage_pair(const Ctx &c, const StageLaunch &s, int *nblocks)
     if (s.unguarded) a.ctl = nullptr;
     a.nbands = (c.g.n + 1 + WB2 - 1) / WB2;
     a.nchunks = (c.g.n + a.rows - 1) / a.rows;
-    const dim3 grid = pair_grid(c, a, a.fast && (a.rows == 16 || a.rows == 12));
+    const dim3 grid = pair_grid(c, a, a.fast && (a.rows == 16 || a.rows == 12), false);
     if (nblocks) *nblocks = a.nbands * a.nchunks;  // one norm partial per wave
     if (s.pair == 2) hipLaunchKernelGGL(k_stage_pair<true>, grid, dim3(256), 0, c.stream, a);
     else hipLaunchKernelGGL(k_stage_pair<false>, grid, dim3(256), 0, c.stream, a);
@@ -1709,7 +1749,7 @@ hipError_t launch_mixed_pair(const Ctx &c, const MixedPairLaunch &s, int *nblock
     if (s.unguarded) a.ctl = nullptr;
     a.nbands = (c.g.n + 1 + WB2 - 1) / WB2;
     a.nchunks = (c.g.n + a.rows - 1) / a.rows;
-    const dim3 grid = pair_grid(c, a, a.fast);
+    const dim3 grid = pair_grid(c, a, a.fast, false);
     if (nblocks) *nblocks = a.nbands * a.nchunks;  // one norm partial per wave
     hipLaunchKernelGGL(k_mixed_pair, grid, dim3(256), 0, c.stream, a);
     return hipGetLastError();

@@ -63,6 +63,8 @@ struct Ctx {
     int64_t npartials = 0;
     int nb_face[3]{};        // blocks per face direction in k_gen_update
     int last_P = 8, last_V = 8;
+    int flux_z = 1;          // 3D: fluxTotal by the z-march kernel (0: one thread per cell)
+    int flux_zc = 16;        // z-march kernel: planes per workgroup
 };
 
 // ---- launchers (kernels_gen.hip); every guarded kernel returns at once once

@@ -15,6 +15,8 @@
 #include <string>
 #include <vector>
 
+#include <cstdlib>
+
 #include "gen.hpp"
 
 namespace {
@@ -175,6 +177,8 @@ int amr_gen_create(amressif_ctx *ctx)
     if (in.poisson_solver != 0)
         return gfail(ctx, AMRESSIF_ERR_UNSUPPORTED, "3D and periodic runs use the multigrid Poisson solve");
     auto *gp = new G::Ctx();
+    if (const char *e = std::getenv("AMRESSIF_GEN_FLUXZ")) gp->flux_z = std::atoi(e) ? 1 : 0;
+    if (const char *e = std::getenv("AMRESSIF_GEN_FZC")) gp->flux_zc = std::max(1, std::min(1024, std::atoi(e)));
     c.gen = gp;
     G::Ctx &gc = *gp;
     G::Geom &g = gc.g;

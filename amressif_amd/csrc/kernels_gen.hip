@@ -32,6 +32,18 @@ constexpr int B = kBlock;
 
 __device__ __forceinline__ int64_t gid() { return (int64_t)xcd_block(blockIdx.x, gridDim.x) * B + threadIdx.x; }
 
+// (i, j, k) of the linear index q over an nx x ny x .. box (x fastest) in
+// 32-bit unsigned arithmetic: every box here has fewer than 2^31 points
+// (n <= 1024 in 3D, n <= 16384 in 2D), and two 32-bit divisions cost a
+// fraction of the 64-bit div/mod sequence.
+__device__ __forceinline__ void split3(int64_t q, int nx, int ny, int *x)
+{
+    const unsigned u = (unsigned)q, a = u / (unsigned)nx, b = a / (unsigned)ny;
+    x[0] = (int)(u - a * (unsigned)nx);
+    x[1] = (int)(a - b * (unsigned)ny);
+    x[2] = (int)b;
+}
+
 __device__ __forceinline__ int64_t fidx(const Geom &g, int d, int i, int j, int k)
 {
     const int64_t nx = g.n + (d == 0), ny = g.n + (d == 1);
@@ -71,7 +83,11 @@ __global__ __launch_bounds__(B) void k_gen_cells(Geom g, const double *f0, const
     const int64_t q = gid();
     if (q >= s * s * sk) return;
     const int n = g.n;
-    int x[3] = {(int)(q % s) - 2, (int)((q / s) % s) - 2, g.dim == 3 ? (int)(q / (s * s)) - 2 : 0};
+    int x[3];
+    split3(q, (int)s, (int)s, x);
+    x[0] -= 2;
+    x[1] -= 2;
+    x[2] = g.dim == 3 ? x[2] - 2 : 0;
     int b[3] = {x[0], x[1], x[2]};
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -113,7 +129,8 @@ __global__ __launch_bounds__(B) void k_gen_flux(Geom g, const double *f0, const 
     const int64_t q = gid();
     if (q >= g.ncell) return;
     const int n = g.n;
-    const int x[3] = {(int)(q % n), (int)((q / n) % n), g.dim == 3 ? (int)(q / ((int64_t)n * n)) : 0};
+    int x[3];
+    split3(q, n, n, x);
     const double *f[3] = {f0, f1, f2};
     const int64_t xc = xidx(g, x[0], x[1], x[2]);
     int64_t st[3];
@@ -153,6 +170,77 @@ __global__ __launch_bounds__(B) void k_gen_flux(Geom g, const double *f0, const 
     }
 }
 
+// The same fluxTotal for a 3D box, marched along z: a workgroup owns a 64 x 4
+// column tile and zc planes, and each thread keeps its column's X(z - 2 ..
+// z + 2) of every component in registers (the next plane loaded one step
+// ahead), so the z-stencil reads X from memory once per cell instead of five
+// times, and the x and y neighbours of the current plane come from the
+// tile's own and its halo's loads of that plane (L1 and L2).  k_gen_flux's
+// one-thread-per-cell order reads the five planes of a cell from five
+// distant parts of X, which at 256^3 (540 KB per plane and component) do not
+// stay in an XCD's L2.  Per cell the arithmetic is k_gen_flux's, term by term.
+constexpr int kFzTX = 64, kFzTY = 4;
+__global__ __launch_bounds__(256) void k_gen_flux_z(Geom g, const double *f0, const double *f1, const double *f2,
+                                                    const double *X, double *T, const Ctl *guard, int ntx, int nty,
+                                                    int zc)
+{
+    if (guard && guard->mom_done) return;
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int tx = blk % ntx, ty = (blk / ntx) % nty, tz = blk / (ntx * nty);
+    const int n = g.n;
+    const int x = tx * kFzTX + (int)(threadIdx.x & 63), y = ty * kFzTY + (int)(threadIdx.x >> 6);
+    const int z0 = tz * zc, z1 = min(z0 + zc, n);
+    if (x >= n || y >= n || z0 >= n) return;  // no barrier below: a thread past the edge just ends
+    const int64_t s = n + 4, sz = s * s, xs = g.xs;
+    int64_t xc = xidx(g, x, y, z0);
+    double w[3][5];  // X(x, y, z - 2 .. z + 2) per component
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int k = 0; k < 5; ++k) w[c][k] = X[c * xs + xc + (k - 2) * sz];
+    double fzl = f2[fidx(g, 2, x, y, z0)], fzh = f2[fidx(g, 2, x, y, z0 + 1)];
+    for (int z = z0; z < z1; ++z) {
+        double nw[3] = {0.0, 0.0, 0.0}, nfz = 0.0;
+        if (z + 1 < z1) {  // the next step's new plane z + 3 (a ghost plane at most) and z-face z + 2
+#pragma unroll
+            for (int c = 0; c < 3; ++c) nw[c] = X[c * xs + xc + 3 * sz];
+            nfz = f2[fidx(g, 2, x, y, z + 2)];
+        }
+        const double fw0 = f0[fidx(g, 0, x, y, z)], fe0 = f0[fidx(g, 0, x + 1, y, z)];
+        const double fw1 = f1[fidx(g, 1, x, y, z)], fe1 = f1[fidx(g, 1, x, y + 1, z)];
+        const int64_t q = cidx(g, x, y, z);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double *Xc = X + c * xs + xc;
+            const double C = w[c][2];
+            double m2 = Xc[-2], m1 = Xc[-1], p1 = Xc[1], p2 = Xc[2];
+            double Fw = quick(fw0, m2, m1, C, p1), Fe = quick(fe0, m1, C, p1, p2);
+            double conv = (Fe - Fw) * g.idx[0];
+            const double l0 = (p1 - 2.0 * C + m1) * g.idx2[0];
+            m2 = Xc[-2 * s]; m1 = Xc[-s]; p1 = Xc[s]; p2 = Xc[2 * s];
+            Fw = quick(fw1, m2, m1, C, p1);
+            Fe = quick(fe1, m1, C, p1, p2);
+            conv = conv + (Fe - Fw) * g.idx[1];
+            const double l1 = (p1 - 2.0 * C + m1) * g.idx2[1];
+            m2 = w[c][0]; m1 = w[c][1]; p1 = w[c][3]; p2 = w[c][4];
+            Fw = quick(fzl, m2, m1, C, p1);
+            Fe = quick(fzh, m1, C, p1, p2);
+            conv = conv + (Fe - Fw) * g.idx[2];
+            const double l2 = (p1 - 2.0 * C + m1) * g.idx2[2];
+            double visc = (l0 + l1) * g.iren;
+            visc = visc + l2 * g.iren;
+            T[q + (int64_t)c * g.ncell] = (0.0 - conv) + visc;
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            w[c][0] = w[c][1]; w[c][1] = w[c][2]; w[c][2] = w[c][3]; w[c][3] = w[c][4]; w[c][4] = nw[c];
+        }
+        fzl = fzh;
+        fzh = nfz;
+        xc += sz;
+    }
+}
+
 // block range of face direction d in a grid over every face (nb[d] blocks each)
 __device__ __forceinline__ int face_dir(const int *nb, int blk, int &lb)
 {
@@ -163,10 +251,7 @@ __device__ __forceinline__ int face_dir(const int *nb, int blk, int &lb)
 }
 __device__ __forceinline__ void face_coords(const Geom &g, int d, int64_t q, int *x)
 {
-    const int64_t nx = g.n + (d == 0), ny = g.n + (d == 1);
-    x[0] = (int)(q % nx);
-    x[1] = (int)((q / nx) % ny);
-    x[2] = (int)(q / (nx * ny));
+    split3(q, g.n + (d == 0), g.n + (d == 1), x);
 }
 
 struct FaceArrays {
@@ -297,7 +382,8 @@ __global__ __launch_bounds__(B) void k_gen_div(Geom g, const double *f0, const d
     double m = 0.0, sum = 0.0;
     if (q < g.ncell) {
         const int n = g.n;
-        const int x[3] = {(int)(q % n), (int)((q / n) % n), g.dim == 3 ? (int)(q / ((int64_t)n * n)) : 0};
+        int x[3];
+        split3(q, n, n, x);
         const double *f[3] = {f0, f1, f2};
         double D = 0.0;
 #pragma unroll
@@ -367,9 +453,8 @@ __device__ __forceinline__ double lap(const Geom &g, const Level &L, const doubl
 
 __device__ __forceinline__ void lcoords(const Geom &g, int n, int64_t q, int *x)
 {
-    x[0] = (int)(q % n);
-    x[1] = (int)((q / n) % n);
-    x[2] = g.dim == 3 ? (int)(q / ((int64_t)n * n)) : 0;
+    split3(q, n, n, x);
+    if (g.dim != 3) x[2] = 0;
 }
 
 __global__ void k_gen_mg_start(Ctl *ctl)
@@ -676,6 +761,13 @@ hipError_t cells(const Ctx &c, hipStream_t s, double *const *f, double *X, const
 
 hipError_t flux(const Ctx &c, hipStream_t s, double *const *f, const double *X, double *T, const Ctl *guard)
 {
+    if (c.g.dim == 3 && c.flux_z) {
+        const int n = c.g.n, ntx = (n + kFzTX - 1) / kFzTX, nty = (n + kFzTY - 1) / kFzTY;
+        const int zc = c.flux_zc, nz = (n + zc - 1) / zc;
+        hipLaunchKernelGGL(k_gen_flux_z, dim3(ntx * nty * nz), dim3(256), 0, s, c.g, f[0], f[1], f[2], X, T, guard,
+                           ntx, nty, zc);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(k_gen_flux, dim3(nblk(c.g.ncell)), dim3(B), 0, s, c.g, f[0], f[1],
                        c.g.dim == 3 ? f[2] : f[0], X, T, guard);
     return hipGetLastError();

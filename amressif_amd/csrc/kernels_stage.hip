@@ -1014,6 +1014,29 @@ __device__ __forceinline__ float quick_tl(float vel, float dvel, float a2, float
     return vel * phid + dvel * phix;
 }
 
+// quick_tl with the base state's upwind interpolant phix given (the FP64
+// march's, rounded to FP32): only the increment's interpolant is formed here
+__device__ __forceinline__ float quick_tl_given(float vel, float dvel, float phix, float da2, float da1, float db0,
+                                                float db1)
+{
+    const bool pos = vel >= 0.f;
+    const float dd = pos ? db0 : da1, du = pos ? da1 : db0, df = pos ? da2 : db1;
+    const float phid = 0.375f * dd + 0.75f * du - 0.125f * df;
+    return vel * phid + dvel * phix;
+}
+
+// quick_flux, also returning the interpolant its upwind side selects
+// (vel >= 0: the one from a2, a1, b0), rounded to FP32
+__device__ __forceinline__ double quick_flux_sel(double vel, double a2, double a1, double b0, double b1, float &phi)
+{
+    const double con = 0.5 * vel;
+    const double ud = con + fabs(con), up = con - fabs(con);
+    const double bn = (1.0 / 8.0) * (-b1 - 2.0 * b0 + 3.0 * a1) + b0;
+    const double bp = (1.0 / 8.0) * (-a2 - 2.0 * a1 + 3.0 * b0) + a1;
+    phi = (float)(vel >= 0.0 ? bp : bn);
+    return up * bn + ud * bp;
+}
+
 // u^{k+1} = u^k + cr r + cj R'(u^k) r in FP64 with R' r formed in FP32 at the
 // FP32 base; norm partials of the change.
 __global__ __launch_bounds__(256) void k_tangent(StageArgs a)
@@ -1200,7 +1223,11 @@ __device__ __forceinline__ void mixed_interior(const StageArgs &a, int lane, int
     R ru[K], rv[K];         // u^k faces: x rows jc .. jc + 2 + D, y rows jc .. jc + 3 + D
     R rcu[K], rcv[K];       // the constant, rows jc .. jc + D
     F xb[K], xr[K], yb[K], yr[K];  // A's output faces (FP32 base, FP32 residual), rows jc - 3 .. jc
-    F wU[K], wV[K], dU[K], dV[K];  // B's cells (base, residual), rows jc - 5 .. jc - 1
+    // A's upwind QUICK interpolants of the base state (FP32) for its four fluxes
+    // of row jc (y-face jc + 1 of U and V, x-face jc of U and V), rows jc - 4 .. jc:
+    // B's tangent-linear fluxes take them instead of forming FP32 base cells
+    F pnU[K], pnV[K], pwU[K], pwV[K];
+    F dU[K], dV[K];                // B's residual cells, rows jc - 5 .. jc - 1
     R obu[K], obv[K];              // the FP64 base faces of B's output rows jb .. jb + D
     {   // cells jA - 3 .. jA (slots K - 2, K - 1, 0, 1)
         const R u0 = atx(us, jA - 3), u1 = atx(us, jA - 2);
@@ -1241,13 +1268,14 @@ __device__ __forceinline__ void mixed_interior(const StageArgs &a, int lane, int
         // A, row jc: the FP64 residual of its two faces
         cellA(jc + 2, ru[a2], rv[a2], rv[a3], aU[a2], aV[a2]);
         {
-            const R fnU = quick_flux<R>(rv[a1], aU[am1], aU[a0], aU[a1], aU[a2]);
-            const R fnV = quick_flux<R>(rv[a1], aV[am1], aV[a0], aV[a1], aV[a2]);
+            const R fnU = quick_flux_sel(rv[a1], aU[am1], aU[a0], aU[a1], aU[a2], pnU[a0]);
+            const R fnV = quick_flux_sel(rv[a1], aV[am1], aV[a0], aV[a1], aV[a2], pnV[a0]);
             const R uw = ru[a0], vw = rv[a0];
             const R Uc = aU[a0], Vc = aV[a0];
             const R Um1 = lane_prev(Uc), Um2 = lane_prev(Um1), Up1 = lane_next(Uc);
             const R Vm1 = lane_prev(Vc), Vm2 = lane_prev(Vm1), Vp1 = lane_next(Vc);
-            const R fwU = quick_flux<R>(uw, Um2, Um1, Uc, Up1), fwV = quick_flux<R>(uw, Vm2, Vm1, Vc, Vp1);
+            const R fwU = quick_flux_sel(uw, Um2, Um1, Uc, Up1, pwU[a0]);
+            const R fwV = quick_flux_sel(uw, Vm2, Vm1, Vc, Vp1, pwV[a0]);
             const R feU = lane_next(fwU), feV = lane_next(fwV);
             const R CU = (feU - fwU) * idx + (fnU - afsU) * idy;
             const R CV = (feV - fwV) * idx + (fnV - afsV) * idy;
@@ -1264,34 +1292,29 @@ __device__ __forceinline__ void mixed_interior(const StageArgs &a, int lane, int
             afsU = fnU; afsV = fnV; atV = TV;
         }
         __builtin_amdgcn_sched_barrier(0);  // A's temporaries end here
-        // B: cell row jc - 1 from A's faces (x row jc - 1, y rows jc - 1, jc)
+        // B: residual cell row jc - 1 from A's faces (x row jc - 1, y rows jc - 1, jc)
         if (t >= 2) {
             if constexpr (XE) {
-                finish_cells<F, false>(a, lane, i, jc - 1, true, xb[am1], lane_next(xb[am1]), yb[am1], yb[a0], F(0),
-                                       wU[am1], wV[am1]);
                 finish_cells<F, true>(a, lane, i, jc - 1, true, xr[am1], lane_next(xr[am1]), yr[am1], yr[a0], F(0),
                                       dU[am1], dV[am1]);
             } else {
-                wU[am1] = F(0.5) * (xb[am1] + lane_next(xb[am1]));
-                wV[am1] = F(0.5) * (yb[am1] + yb[a0]);
                 dU[am1] = F(0.5) * (xr[am1] + lane_next(xr[am1]));
                 dV[am1] = F(0.5) * (yr[am1] + yr[a0]);
             }
         }
         if (t >= 6) {  // B's row jb: faces rows jb .. jb + 3 in slots b3, am2, am1, a0; cells jb - 2 .. jb + 2 in b5 .. am1
+            // A's interpolants: row jb - 1 (the flux through y-face jb) in b4, row jb in b3
             if (t == 6) {  // the flux through y-face jb
-                fsU = quick_tl(yb[b3], yr[b3], wU[b5], wU[b4], wU[b3], wU[am2], dU[b5], dU[b4], dU[b3], dU[am2]);
-                fsV = quick_tl(yb[b3], yr[b3], wV[b5], wV[b4], wV[b3], wV[am2], dV[b5], dV[b4], dV[b3], dV[am2]);
+                fsU = quick_tl_given(yb[b3], yr[b3], pnU[b4], dU[b5], dU[b4], dU[b3], dU[am2]);
+                fsV = quick_tl_given(yb[b3], yr[b3], pnV[b4], dV[b5], dV[b4], dV[b3], dV[am2]);
             }
-            const F fnU = quick_tl(yb[am2], yr[am2], wU[b4], wU[b3], wU[am2], wU[am1], dU[b4], dU[b3], dU[am2], dU[am1]);
-            const F fnV = quick_tl(yb[am2], yr[am2], wV[b4], wV[b3], wV[am2], wV[am1], dV[b4], dV[b3], dV[am2], dV[am1]);
-            const F Uc = wU[b3], Vc = wV[b3], dUc = dU[b3], dVc = dV[b3];
-            const F Um1 = lane_prev(Uc), Um2 = lane_prev(Um1), Up1 = lane_next(Uc);
-            const F Vm1 = lane_prev(Vc), Vm2 = lane_prev(Vm1), Vp1 = lane_next(Vc);
+            const F fnU = quick_tl_given(yb[am2], yr[am2], pnU[b3], dU[b4], dU[b3], dU[am2], dU[am1]);
+            const F fnV = quick_tl_given(yb[am2], yr[am2], pnV[b3], dV[b4], dV[b3], dV[am2], dV[am1]);
+            const F dUc = dU[b3], dVc = dV[b3];
             const F dUm1 = lane_prev(dUc), dUm2 = lane_prev(dUm1), dUp1 = lane_next(dUc);
             const F dVm1 = lane_prev(dVc), dVm2 = lane_prev(dVm1), dVp1 = lane_next(dVc);
-            const F fwU = quick_tl(xb[b3], xr[b3], Um2, Um1, Uc, Up1, dUm2, dUm1, dUc, dUp1);
-            const F fwV = quick_tl(xb[b3], xr[b3], Vm2, Vm1, Vc, Vp1, dVm2, dVm1, dVc, dVp1);
+            const F fwU = quick_tl_given(xb[b3], xr[b3], pwU[b3], dUm2, dUm1, dUc, dUp1);
+            const F fwV = quick_tl_given(xb[b3], xr[b3], pwV[b3], dVm2, dVm1, dVc, dVp1);
             const F feU = lane_next(fwU), feV = lane_next(fwV);
             const F CU = (feU - fwU) * fidx + (fnU - fsU) * fidy;
             const F CV = (feV - fwV) * fidx + (fnV - fsV) * fidy;

@@ -823,6 +823,8 @@ int amressif_create(const amressif_inputs *in, void *hip_stream, amressif_ctx **
     if (const char *e = std::getenv("AMRESSIF_MG_SMOOTHER")) c.mg_smoother = std::max(0, std::min(2, std::atoi(e)));
     if (const char *e = std::getenv("AMRESSIF_TAIL_DCT")) c.tail_dct = std::atoi(e) ? 1 : 0;
     if (const char *e = std::getenv("AMRESSIF_MG_LATE")) c.mg_late = std::atoi(e) ? 1 : 0;
+    if (const char *e = std::getenv("AMRESSIF_MG_POST3")) c.mg_post3 = std::atoi(e) ? 1 : 0;
+    if (const char *e = std::getenv("AMRESSIF_MG_PRE3")) c.mg_pre3 = std::atoi(e) ? 1 : 0;
     if (const char *e = std::getenv("AMRESSIF_MG_UP64")) c.mg_up64_single = std::atoi(e) ? 1 : 0;
     if (const char *e = std::getenv("AMRESSIF_MG_BATCH")) c.mg_batch = std::max(0, std::atoi(e));
     if (const char *e = std::getenv("AMRESSIF_DCT_MFMA64")) c.dct_mfma64 = std::atoi(e) ? 1 : 0;

@@ -168,6 +168,8 @@ struct Ctx {
     // levels below the finest
     int mg_smoother = 2;
     int mg_late = 1;  // finest up-leg smoother: phi and b loaded after the sweeps
+    int mg_post3 = 1;  // FP32 V-cycle: three post-smoothing sweeps on the finest level (0: two)
+    int mg_pre3 = 1;   // FP32 V-cycle: three pre-smoothing sweeps on the finest level (0: two)
     int mg_up64_single = 1;  // FP64 finest up leg on the one-column kernel (with mg_smoother 2)
     int mg_rows[2][3] = {{8, 8, 8}, {8, 8, 4}};
     int mg_rows_coarse = 4;

@@ -442,14 +442,15 @@ struct Pair<double> {
     using V = double2;
 };
 
-template <class T, int NR, int H, bool YINT>
+template <class T, int NR, int H, bool YINT, int HS = 4>
 __device__ __forceinline__ void gsrb4_pair(T (&x0)[NR], T (&x1)[NR], const T (&rs0)[NR], const T (&rs1)[NR], int jb,
                                            int ny, const T (&A)[2][2], const T (&B)[2][2], bool in_i)
 {
     // A[cell][edge], B[cell][edge]: ax/d and ay/d of the even (0) and odd (1)
-    // cell on an interior (0) or wall (1) row; zero outside the domain
+    // cell on an interior (0) or wall (1) row; zero outside the domain; HS
+    // half-sweeps (colours 0, 1, 0, ..)
 #pragma unroll
-    for (int h = 0; h < 4; ++h) {
+    for (int h = 0; h < HS; ++h) {
 #pragma unroll
         for (int q = 1 + h; q < NR - 1 - h; ++q) {
             const int j = jb + q;
@@ -479,13 +480,14 @@ __device__ __forceinline__ void gsrb4_pair(T (&x0)[NR], T (&x1)[NR], const T (&r
 // LATE (MODE 2): phi_in and b, needed only by the outer step, are loaded
 // after the sweeps, so their registers are not live during them (more waves
 // per SIMD, at the cost of one more wait for memory)
-template <class T, int MODE, int R, bool LATE = false>
+template <class T, int MODE, int R, bool LATE = false, int HS = 4>
 __global__ __launch_bounds__(256) void k_gsrb2(const T *xin, const T *ec, int pc, T *xout, const T *rhs, int nx,
                                                int ny, int pitch, T ax, T ay, T *rc, int ncx, int ncy, int pcc,
                                                OuterArgs o, const Ctl *ctl, int nbands, int nwaves)
 {
     using V = typename Pair<T>::V;
-    constexpr int H = MODE == 0 ? 4 : 5, HX = MODE == 0 ? 4 : 6;
+    // halo rows and columns: one per half-sweep, one more for the residual of MODE 1 and 2
+    constexpr int H = MODE == 0 ? HS : HS + 1, HX = MODE == 0 ? HS : HS + 2;
     constexpr int OC = 128 - 2 * HX, NR = R + 2 * H;
     const int lane = threadIdx.x & 63;
     const int nblk = gridDim.x;
@@ -566,9 +568,9 @@ __global__ __launch_bounds__(256) void k_gsrb2(const T *xin, const T *ec, int pc
         r1[q] *= mid ? i1m : i1e;
     }
     if (yin)
-        gsrb4_pair<T, NR, H, true>(x0, x1, r0, r1, jb, ny, A, B, in_i);
+        gsrb4_pair<T, NR, H, true, HS>(x0, x1, r0, r1, jb, ny, A, B, in_i);
     else
-        gsrb4_pair<T, NR, H, false>(x0, x1, r0, r1, jb, ny, A, B, in_i);
+        gsrb4_pair<T, NR, H, false, HS>(x0, x1, r0, r1, jb, ny, A, B, in_i);
     const bool out_lane = in_i && lane >= HX / 2 && lane < HX / 2 + OC / 2;
     if (MODE != 2) {
 #pragma unroll
@@ -1655,6 +1657,14 @@ struct SmoothShapeRt {
     int parts = 0;  // MODE 2 norm partials: one per workgroup
 };
 
+// the finest legs of the FP32 V-cycle with three smoothing sweeps: the up
+// leg with mg_post3, the down leg with mg_pre3
+static bool post3(const Ctx &c, int l, int mode, size_t tsize)
+{
+    if (l != 0 || tsize != 4) return false;
+    return mode == 2 ? c.mg_post3 && c.mg_late : mode == 1 ? c.mg_pre3 != 0 : false;
+}
+
 template <class T>
 static SmoothShapeRt smooth_shape(const Ctx &c, int l, int mode)
 {
@@ -1669,7 +1679,8 @@ static SmoothShapeRt smooth_shape(const Ctx &c, int l, int mode)
     // FP64 finest up leg: the one-column kernel (106 VGPRs, 4 waves per SIMD)
     // beats the paired one (2 waves) there
     if (s.wave == 2 && sizeof(T) == 8 && mode == 2 && l == 0 && c.mg_up64_single) s.wave = 1;
-    const int OC = s.wave == 2 ? (mode == 0 ? 120 : 116) : (mode == 0 ? 56 : 54);
+    int OC = s.wave == 2 ? (mode == 0 ? 120 : 116) : (mode == 0 ? 56 : 54);
+    if (s.wave == 2 && post3(c, l, mode, sizeof(T))) OC = 128 - 2 * (6 + 2);  // three smoothing sweeps
     s.rows = l == 0 ? c.mg_rows[sizeof(T) == 4][mode] : c.mg_rows_coarse;
     if (s.wave == 2) {  // the paired-column kernel's register limit (pair_rows_max)
         const int cap = sizeof(T) == 4 ? (mode == 2 ? 8 : 16) : (mode == 2 ? 4 : 8);
@@ -1692,9 +1703,17 @@ constexpr int pair_rows_max()
 
 template <class T, int MODE, int R>
 static void launch_gsrb(const Ctx &c, const SmoothShapeRt &s, const T *xin, const T *ec, int pc, T *xout,
-                        const T *rhs, const Level &L, T *rc, const Level *Cl, const OuterArgs &o)
+                        const T *rhs, const Level &L, T *rc, const Level *Cl, const OuterArgs &o, bool p3)
 {
     constexpr int R2 = R < pair_rows_max<T, MODE>() ? R : pair_rows_max<T, MODE>();
+    if constexpr (sizeof(T) == 4 && MODE != 0) {
+        if (s.wave == 2 && p3) {
+            hipLaunchKernelGGL((k_gsrb2<T, MODE, R2, MODE == 2, 6>), s.grid, dim3(256), 0, c.stream, xin, ec, pc, xout,
+                               rhs, L.nx, L.ny, L.pitch, (T)L.ax, (T)L.ay, rc, Cl ? Cl->nx : 0, Cl ? Cl->ny : 0,
+                               Cl ? Cl->pitch : 0, o, (const Ctl *)c.ctl, s.nbands, s.nwaves);
+            return;
+        }
+    }
     if (s.wave == 2 && MODE == 2 && c.mg_late)
         hipLaunchKernelGGL((k_gsrb2<T, MODE, R2, true>), s.grid, dim3(256), 0, c.stream, xin, ec, pc, xout, rhs, L.nx,
                            L.ny, L.pitch, (T)L.ax, (T)L.ay, rc, Cl ? Cl->nx : 0, Cl ? Cl->ny : 0,
@@ -1720,6 +1739,7 @@ static void launch_smoother(const Ctx &c, int l, const T *xin, const T *ec, int 
     const Level *Cl = &c.lev[l + 1];
     T *rc = MODE == 1 ? lev_r<T>(*Cl) : (T *)nullptr;
     const SmoothShapeRt s = smooth_shape<T>(c, l, MODE);
+    const bool p3 = post3(c, l, MODE, sizeof(T));
     if (!s.wave) {
         hipLaunchKernelGGL((k_smooth<T, MODE>), s.grid, dim3(256), 0, c.stream, xin, ec, pc, xout, rhs, L.nx, L.ny,
                            L.pitch, (T)L.ax, (T)L.ay, rc, Cl ? Cl->nx : 0, Cl ? Cl->ny : 0, Cl ? Cl->pitch : 0, o,
@@ -1727,10 +1747,10 @@ static void launch_smoother(const Ctx &c, int l, const T *xin, const T *ec, int 
         return;
     }
     switch (s.rows) {
-    case 4: launch_gsrb<T, MODE, 4>(c, s, xin, ec, pc, xout, rhs, L, rc, Cl, o); break;
-    case 8: launch_gsrb<T, MODE, 8>(c, s, xin, ec, pc, xout, rhs, L, rc, Cl, o); break;
-    case 16: launch_gsrb<T, MODE, 16>(c, s, xin, ec, pc, xout, rhs, L, rc, Cl, o); break;
-    default: launch_gsrb<T, MODE, 32>(c, s, xin, ec, pc, xout, rhs, L, rc, Cl, o); break;
+    case 4: launch_gsrb<T, MODE, 4>(c, s, xin, ec, pc, xout, rhs, L, rc, Cl, o, p3); break;
+    case 8: launch_gsrb<T, MODE, 8>(c, s, xin, ec, pc, xout, rhs, L, rc, Cl, o, p3); break;
+    case 16: launch_gsrb<T, MODE, 16>(c, s, xin, ec, pc, xout, rhs, L, rc, Cl, o, p3); break;
+    default: launch_gsrb<T, MODE, 32>(c, s, xin, ec, pc, xout, rhs, L, rc, Cl, o, p3); break;
     }
 }
 

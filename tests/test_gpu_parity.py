@@ -388,6 +388,32 @@ def test_edge_first_wave_order_gives_the_same_steps(precision, n, rows):
     assert rel(pb - pb.mean(), pa - pa.mean()) < 1e-12
 
 
+@pytest.mark.parametrize("n", [256, 300, 130])
+def test_three_sweep_finest_legs_of_the_fp32_cycle(n):
+    """The mixed step's FP32 V-cycle with three red-black sweeps on each finest
+    leg (the default) against two (AMRESSIF_MG_PRE3=AMRESSIF_MG_POST3=0): every
+    step converges to the same Poisson tolerance in no more V-cycles, and the
+    fields agree to the 1e-6 bar of the mixed-against-FP64 comparison (the two
+    solves stop at different iterates inside the tolerance)."""
+    api = _api()
+
+    def run():
+        inp = api.cavity_inputs(n, 1000.0, precision=1)
+        r = api.Run.from_numpy(inp, W.initial_state(n, "seeded"))
+        st = r.advance(6)
+        return r.fetch(), st
+
+    f2, s2 = _with_env({"AMRESSIF_MG_PRE3": "0", "AMRESSIF_MG_POST3": "0"}, run)
+    f3, s3 = _with_env({"AMRESSIF_MG_PRE3": "1", "AMRESSIF_MG_POST3": "1"}, run)
+    print(f"n={n}: V-cycles two sweeps {[s['vcycles'] for s in s2]}, three {[s['vcycles'] for s in s3]}")
+    assert all(s["flags"] == 0 and s["poisson_res"] <= 1e-10 for s in s3)
+    assert sum(s["vcycles"] for s in s3) <= sum(s["vcycles"] for s in s2)
+    for k in ("u", "v"):
+        assert rel(f3[k], f2[k]) < 1e-6, k
+    p2, p3 = f2["ctx"][0, 1:-1, 1:-1], f3["ctx"][0, 1:-1, 1:-1]
+    assert rel(p3 - p3.mean(), p2 - p2.mean()) < 1e-6
+
+
 @pytest.mark.parametrize("n", [130, 98])
 def test_odd_bottom_poisson_solve_matches_the_exact_solve(n):
     """n = 130 coarsens to an odd 65 x 65 bottom (98 to 49 x 49), above the

@@ -171,7 +171,7 @@ struct Ctx {
     int mg_post3 = 1;  // FP32 V-cycle: three post-smoothing sweeps on the finest level (0: two)
     int mg_pre3 = 1;   // FP32 V-cycle: three pre-smoothing sweeps on the finest level (0: two)
     int mg_up64_single = 1;  // FP64 finest up leg on the one-column kernel (with mg_smoother 2)
-    int mg_rows[2][3] = {{8, 8, 8}, {8, 8, 4}};
+    int mg_rows[2][3] = {{8, 8, 8}, {8, 8, 8}};
     int mg_rows_coarse = 4;
     // FGMRES Poisson solver (poisson_solver = 1)
     int gm_m = 0;            // restart length

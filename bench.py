@@ -59,8 +59,8 @@ STEP_KERNELS = {
     "stage_last": ("amr::k_stage64<1, true> (FP64 last RK stage + norm partials)", None),
     "resid": ("amr::k_stage64<3, true> (mixed FP64 residual)", "residual_mixed"),
     "tangent": ("amr::k_tangent (mixed FP32 tangent-linear update)", "tangent_mixed"),
-    "mg_down": ("finest down-leg smoother + restriction (amr::k_gsrb2<T, 1, 8, false>)", "smooth1_{prec}"),
-    "mg_up": ("finest up-leg smoother + FP64 outer step (amr::k_gsrb2<float, 2, 4, true> mixed, "
+    "mg_down": ("finest down-leg smoother + restriction (amr::k_gsrb2<T, 1, 8, false, HS>)", "smooth1_{prec}"),
+    "mg_up": ("finest up-leg smoother + FP64 outer step (amr::k_gsrb2<float, 2, 8, true, 6> mixed, "
               "amr::k_gsrb<double, 2, 8> FP64)", "smooth5_{prec}"),
     "project": ("amr::k_project (update + next step's BDF2 constant)", None),
     "poisson_start": ("amr::k_poisson_start (Poisson RHS, warm start, first residual)", None),

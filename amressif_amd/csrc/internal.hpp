@@ -187,7 +187,7 @@ struct Ctx {
     // eigenvalue tables (null: the tail runs the V-cycle down to the bottom)
     int tail_dct = 1;
     int dct_mfma64 = 1;  // FP64 DCT products on the matrix cores (0: on the VALU)
-    int start_rows = 16;  // Poisson start: rows per wave of the row-march kernel (0: cell-tile kernel)
+    int start_rows = 8;   // Poisson start: rows per wave of the row-march kernel (0: cell-tile kernel)
     int project_rows = 8;   // projection: rows per wave of the row-march kernel (0: cell-tile kernel)
     double *dct_tab = nullptr;
     float *dct_tab32 = nullptr;
